@@ -1,0 +1,122 @@
+/*
+ * real_caustics.h -- C-ABI of libRealCaustics.so, the MI355X (gfx950) drop-in
+ * for the exports of "Real Caustics.dll".
+ *
+ * The first two entry points keep the reference's signatures, struct layouts
+ * and call protocol exactly (init uploads a scene, main renders it and writes
+ * the image file, then releases the scene).  Every other entry point is a
+ * non-breaking addition (prefix rc_): per-iteration rendering, in-memory image
+ * readback, options, error reporting, cancellation, and the diagnostic calls
+ * the parity tests use.  All calls act on one process-wide scene, as the
+ * reference does (Real Caustics/src/main.cpp:59); they are not re-entrant.
+ */
+#ifndef REAL_CAUSTICS_H
+#define REAL_CAUSTICS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Value types passed across the boundary (layouts of the reference). */
+typedef struct { float x, y, z; } rc_vec3;                       /* src/vec3.h:13-19 (12 B)  */
+typedef struct { float t[16]; } rc_matrix4x4;                     /* src/matrix.h:151-171, row-major t00..t33 (64 B) */
+typedef struct {                                                  /* src/Lights.h:15-36 (64 B) */
+    char type;        /* 'P' point, 'S' sun, anything else area */
+    float color[3];
+    float position[3];
+    float width, height, spread;
+    float rotation[3]; /* degrees for area lights, radians for the sun */
+    float power, angle, radius;
+} rc_python_light;
+typedef struct {                                                  /* src/materials.h:21-32 (28 B) */
+    char type;        /* 'P' plastic, 'G' glass, 'M' metal, else white plastic */
+    float ior, roughness, specular;
+    float color[3];
+} rc_python_material;
+typedef struct {                                                  /* src/Vertex_Merging.h:13-19 (24 B) */
+    int number_of_points;
+    rc_vec3 direction;
+    float roughness, specular;
+} rc_debug_stuff;
+
+/* Replaces init (Real Caustics/src/DLL.h:15, defined at src/main.cpp:89-143).
+ * meshes_pointers[i] is the address of a Blender Mesh (mpoly@200, mloop@208,
+ * mvert@256, src/Blender_definitions.h:214-311); every poly is read as the
+ * triangle of its first three loops (src/mesh.cpp:20-26).  Geometry,
+ * materials, lights and camera are copied, so unlike the reference nothing
+ * needs to outlive this call.  Invalid input records an error (rc_last_error)
+ * that main() then reports. */
+void init(int samples, float base_radius, float radius_reduction,
+          long long *meshes_pointers, unsigned int number_of_meshes, rc_matrix4x4 *mesh_matrices,
+          unsigned int *meshes_number_of_verts, unsigned int *meshes_number_of_tris,
+          unsigned int camera_x, unsigned int camera_y, rc_vec3 camera_position,
+          rc_vec3 camera_corner0, rc_vec3 camera_corner1, rc_vec3 camera_corner2, rc_vec3 camera_corner3,
+          rc_python_light *lights, unsigned int number_of_lights,
+          rc_python_material *materials, unsigned int number_of_materials,
+          int **materials_indices, int debug_test, rc_debug_stuff debug_stuff);
+
+/* Replaces main (src/DLL.h:14, defined at src/main.cpp:148-223): builds the
+ * acceleration structure, renders `samples` iterations on the GPU, averages,
+ * writes the fp32 RGB EXR (default path "D:\caustics.exr" as in
+ * src/main.cpp:206; option "output_path") and releases the scene.
+ * Returns 0 on success, nonzero on error (the reference always returns 0). */
+int main(void);
+
+/* ---- additions (no counterpart in the reference) ---- */
+
+/* Options.  Numeric keys: "seed", "photons" (light paths per iteration,
+ * 0 = width*height as the reference), "write_exr", "compat_rows",
+ * "compat_point_light", "compat_light_pdf", "max_path_length",
+ * "photon_capacity", "cell_scale", "persistent_blocks", "block_size".
+ * String keys: "output_path". Return 0, or -1 for an unknown key. */
+int rc_set_option(const char *key, double value);
+int rc_set_option_str(const char *key, const char *value);
+double rc_get_option(const char *key);
+
+/* Builds the BVH and uploads the scene (the prelude of main, src/main.cpp:157-168). */
+int rc_prepare(void);
+/* Renders the next iteration and accumulates it (one run_iteration,
+ * src/Vertex_Merging.cpp:196-366).  Returns the iteration index or <0. */
+int rc_render_iteration(void);
+/* Copies the accumulated image (width*height*3 floats, buffer row order of
+ * src/Image_Buffer.h).  scaled != 0 divides by the iterations rendered. */
+int rc_get_image(float *rgb, int scaled);
+/* Writes the current image as an uncompressed fp32 RGB OpenEXR file. */
+int rc_write_exr(const char *path);
+/* Frees the scene and device memory (the end of main, src/main.cpp:220-221). */
+void rc_release(void);
+/* Last error message, "" when none. */
+const char *rc_last_error(void);
+/* Requests cancellation; the running render stops after the current iteration. */
+void rc_cancel(void);
+/* Statistics of the last iteration: see RC_STAT_* indices. Returns the count written. */
+int rc_stats(double *out, int n);
+enum {
+    RC_STAT_LIGHT_PATHS = 0, RC_STAT_PHOTONS, RC_STAT_CAMERA_PATHS, RC_STAT_GATHERS,
+    RC_STAT_PHOTONS_TESTED, RC_STAT_PHOTONS_ACCEPTED, RC_STAT_MS_LIGHT, RC_STAT_MS_MAP,
+    RC_STAT_MS_CAMERA, RC_STAT_MS_TOTAL, RC_STAT_RADIUS, RC_STAT_NODES, RC_STAT_TRIS,
+    RC_STAT_COUNT
+};
+
+/* ---- diagnostics used by the parity tests ---- */
+
+/* Closest hit for n rays {o.xyz, d.xyz}; tri = global triangle id (mesh order,
+ * then triangle order) or -1; t = distance. */
+int rc_first_hit(const float *rays, long long n, float tmin, int *tri, float *t);
+/* Light pass of one iteration; photons in path order as 10 floats
+ * {pos.xyz, dir.xyz, throughput.rgb, dVM} and uid = path*8 + path_length.
+ * Returns the photon count (records beyond cap are not written). */
+long long rc_trace_photons(int iteration, float *photons, unsigned int *uid, long long cap);
+/* Radius sets through the engine's photon map: for each query, the input
+ * indices i with fl(fl(dx*dx+dy*dy)+dz*dz) < r2 (src/KD_Tree.cpp:316,349).
+ * CSR output; returns the total (indices beyond cap are not written). */
+long long rc_gather_sets(const float *positions, long long count, const float *queries, long long nq,
+                         float r2, long long *offsets, int *indices, long long cap);
+/* Camera pass of one iteration against a given photon list (10 floats each,
+ * as rc_trace_photons); adds into image (width*height*3). */
+int rc_camera_pass_photons(int iteration, const float *photons, long long count, float *image);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

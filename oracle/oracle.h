@@ -1,0 +1,118 @@
+/*
+ * oracle.h -- CPU restatement of the Real Caustics vertex-merging render.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This code is the checker for the HIP engine in
+ * realcausticsengine_amd/: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product path never calls it.
+ *
+ * It is compiled twice from oracle.c: liboracle_f32.so (REAL = float, the
+ * reference-mode estimator, bit-for-bit the float arithmetic order of the
+ * reference) and liboracle_f64.so (REAL = double, the FP64 oracle of the same
+ * estimator).  All array arguments are double so both builds share one
+ * Python binding; float values convert to double exactly.
+ *
+ * Reference files followed (paths relative to "Real Caustics/src"):
+ *   main.cpp:89-143 (init), main.cpp:148-223 (main), Scene.cpp, mesh.cpp,
+ *   Triangle.cpp, aabb.cpp, BVH/BVH_mesh.cpp, BVH/BVH_world.cpp, Transform.cpp,
+ *   matrix.cpp, materials.h, BXDF.h, BXDF.cpp, Frame.h, Sampling.h, Lights.cpp,
+ *   Light_list.cpp, utils.h:142-157, Camera.h, Image_Buffer.h,
+ *   Vertex_Merging.cpp, Path_Vertices.h, KD_Tree.cpp, MLT_Sampler.cpp,
+ *   random_generators.h.
+ */
+#ifndef RC_ORACLE_H
+#define RC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Photon record exported by orc_trace_light: 11 doubles per photon
+ * {pos.xyz, dir.xyz (direction_world, towards the previous vertex),
+ *  throughput.rgb, dVM, uid}; uid = path_index * 8 + path_length. */
+#define ORC_PHOTON_STRIDE 11
+
+/* Create a scene from the arguments of the reference init()
+ * (main.cpp:89-108).  cam15 = camera position, corner0 (top right),
+ * corner1 (bottom right), corner2 (bottom left), corner3 (top left).
+ * lights points at Python_Light[64 B], materials at Python_Material[28 B],
+ * meshes_pointers at Blender Mesh structs (mpoly@200, mloop@208, mvert@256). */
+void *orc_create(int samples, float base_radius, float radius_reduction,
+                 const long long *meshes_pointers, unsigned number_of_meshes,
+                 const float *mesh_matrices, const unsigned *meshes_number_of_verts,
+                 const unsigned *meshes_number_of_tris, unsigned camera_x, unsigned camera_y,
+                 const float *cam15, const void *lights, unsigned number_of_lights,
+                 const void *materials, unsigned number_of_materials,
+                 const int *const *materials_indices);
+void orc_free(void *scene);
+
+/* Options: "rng_serial" (0 = per-path Philox plain mode, 1 = reference serial
+ * xorshift stream with the adaptive MLT light pass), "seed", "photons"
+ * (light paths per iteration; 0 = W*H as the reference), "compat_point_light",
+ * "compat_light_pdf", "compat_rows", "libm" (1 = C library sin/cos/acos instead
+ * of the engine's own), "camera_stride" (trace every k-th pixel only),
+ * "max_path_length", "gather" (0 = KD tree, 1 = brute force). */
+int orc_set_option(void *scene, const char *key, double value);
+double orc_get(void *scene, const char *key);
+
+/* Prelude of main(): BVH build, lights, radius scale (main.cpp:157-168). */
+int orc_prepare(void *scene);
+
+/* Radius and VM normalisation of one iteration (Vertex_Merging.cpp:204-212). */
+void orc_iteration_params(void *scene, int iteration, double *radius, double *radius_sqr,
+                          double *vm_normalization);
+
+/* Light pass of one iteration.  Returns the photon count (may exceed cap;
+ * only min(count, cap) records are written). */
+long long orc_trace_light(void *scene, int iteration, double *photons, long long cap);
+
+/* Camera pass of one iteration against a given photon list (11 doubles per
+ * photon); adds into image (W*H*3 doubles, buffer row order). */
+void orc_camera_pass(void *scene, int iteration, const double *photons, long long count,
+                     double *image);
+
+/* Whole render: iterations run_iteration calls then scale (Vertex_Merging.cpp:367-379). */
+void orc_render(void *scene, int iterations, double *image);
+
+/* Closest hit through the reference two-level BVH (brute = 0) or over all
+ * triangles (brute = 1).  rays = n * {o.xyz, d.xyz}.  tri_out is the global
+ * triangle id (mesh offset + triangle index), -1 for a miss. */
+void orc_first_hit(void *scene, const float *rays, long long n, float tmin, int brute,
+                   int *tri_out, double *t_out);
+
+/* Brute-force radius gather: for each query q, every photon i with
+ * fl(fl(dx*dx + dy*dy) + dz*dz) < r2 (KD_Tree.cpp:316,349; vec3.h:93-96).
+ * pos = P*3 floats, q = Q*3 floats.  Writes CSR offsets[Q+1]; indices up to cap. */
+long long orc_gather_sets(const float *pos, long long P, const float *q, long long Q, float r2,
+                          long long *offsets, int *indices, long long cap);
+
+/* Same through the restated KD tree (KD_Tree.cpp:178-280); order as visited. */
+long long orc_gather_sets_kd(const float *pos, long long P, const float *q, long long Q, float r2,
+                             long long *offsets, int *indices, long long cap);
+
+/* Philox4x32-10 block (Salmon et al. 2011). */
+void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
+
+/* The engine's own sin/cos/acos (shared definition with the HIP path). */
+double orc_math(int which, double x);
+
+/* One BSDF evaluation, for unit tests.
+ * mat = {type char code, ior, roughness, specular, r, g, b}; mode 0 radiance / 1 importance.
+ * shade_n, geo_n, dir (incoming ray direction) are 3 floats each.
+ * op 0 = sample with u3 -> out[0..2] factor, out[3..5] wi, out[6] fwd, out[7] rev,
+ *        out[8] cos_out, out[9] cont, out[10] event, out[11] valid, out[12] delta
+ * op 1 = evaluate(wi) -> out[0..2]; op 2 = pdf(wi) -> out[6], out[7]. */
+void orc_bsdf(const float *mat, int mode, const float *shade_n, const float *geo_n,
+              const float *dir, const float *u3, const float *wi, int op, double *out);
+
+/* One light emission sample: u5 = 5 primary samples; out = origin.xyz, dir.xyz, throughput.rgb. */
+void orc_light_sample(void *scene, const float *u5, double *out);
+
+/* Traversal counters of the last call: box tests, triangle tests. */
+void orc_counters(void *scene, long long *out3);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,480 @@
+// rc_device.h -- device functions of the vertex-merging path: BSDFs
+// (materials.h, BXDF.cpp), light emission (Lights.cpp, Light_list.cpp),
+// BVH closest hit with an LDS stack, and the photon-map radius query.
+#pragma once
+#include "rc_engine.h"
+#include "rc_vec.h"
+
+namespace rc {
+
+constexpr float kPI = 3.14159265359f;        // main.cpp:44
+constexpr float kPI2 = 6.28318530718f;       // main.cpp:47
+constexpr float kInvPI = 0.31830988618f;     // main.cpp:48
+constexpr float kPIOver2 = 1.57079632679f;   // main.cpp:49
+constexpr float kPIOver4 = 0.78539816339f;   // main.cpp:50
+
+__device__ __forceinline__ F3 ld3(const float* p) { return f3(p[0], p[1], p[2]); }
+
+// Sampling.h:15-46 (Shirley-Chiu concentric disk, cosine hemisphere)
+__device__ __forceinline__ F3 cos_hemisphere(float ex, float ey) {
+    float ux = ex * 2.0f - 1.0f, uy = ey * 2.0f - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (!(ux == 0.0f && uy == 0.0f)) {
+        float theta, r;
+        if (fabsf(ux) > fabsf(uy)) {
+            r = ux;
+            theta = kPIOver4 * (uy / ux);
+        } else {
+            r = uy;
+            theta = kPIOver2 - kPIOver4 * (ux / uy);
+        }
+        dx = own_cos(theta) * r;
+        dy = own_sin(theta) * r;
+    }
+    float z = sqrtf(smax(0.0f, 1.0f - dx * dx - dy * dy));
+    return f3(dx, dy, z);
+}
+
+// ------------------------------------------------------------------
+// BSDFs
+// ------------------------------------------------------------------
+struct Bsdf {
+    int kind;
+    F3 R;
+    float a2, s, ior, cont;
+    F3 wo;
+    Frame fr;
+    bool radiance;
+};
+
+__device__ __forceinline__ bool bsdf_setup(Bsdf& b, const DevMaterial& m, F3 shade_n, F3 dir, bool radiance) {
+    b.kind = m.kind;
+    b.R = f3(m.R[0], m.R[1], m.R[2]);
+    b.a2 = m.a2;
+    b.s = m.s;
+    b.ior = m.ior;
+    b.cont = m.cont;
+    b.radiance = radiance;
+    b.fr = frame_from_z(shade_n);
+    b.wo = to_local(b.fr, -dir);
+    return !(fabsf(b.wo.z) < 1e-6f);
+}
+__device__ __forceinline__ bool bsdf_delta(const Bsdf& b) { return b.kind == BK_SPEC_REFL || b.kind == BK_FRESNEL_SPEC; }
+
+__device__ __forceinline__ float fresnel_eta(float c, float eta) {
+    float g = eta * eta - 1.0f + c * c;
+    if (g > 0.0f) {
+        g = sqrtf(g);
+        float A = (g - c) / (g + c);
+        float B = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+        return 0.5f * A * A * (1.0f + B * B);
+    }
+    return 1.0f;
+}
+__device__ __forceinline__ float fresnel_refl(float c, float refl) {
+    if (refl >= 1.0f) return 1.0f;
+    float n = (1.0f + sqrtf(refl)) / (1.0f - sqrtf(refl));
+    float g = n * n - 1.0f + c * c;
+    if (g > 0.0f) {
+        g = sqrtf(g);
+        float A = (g - c) / (g + c);
+        float B = (c * (g + c) - 1.0f) / (c * (g - c) + 1.0f);
+        return 0.5f * A * A * (1.0f + B * B);
+    }
+    return 1.0f;
+}
+__device__ __forceinline__ float ggx_D(float ndoth, float a2) {
+    float den = ndoth * ndoth * (a2 - 1.0f) + 1.0f;
+    return a2 / (kPI * den * den);
+}
+__device__ __forceinline__ float ggx_G2(float nw, float nl, float a2) {
+    float A = nw * sqrtf(a2 + (1.0f - a2) * nl * nl);
+    float B = nl * sqrtf(a2 + (1.0f - a2) * nw * nw);
+    return 2.0f * nl * nw / (A + B);
+}
+__device__ __forceinline__ F3 ggx_half(float a2, float e1, float e2) {
+    float theta = own_acos(sqrtf((1.0f - e1) / (e1 * (a2 - 1.0f) + 1.0f)));
+    float phi = e2 * kPI2;
+    float st = own_sin(theta);
+    return f3(st * own_cos(phi), st * own_sin(phi), own_cos(theta));
+}
+__device__ __forceinline__ bool same_side(const Bsdf& b, F3 wi, F3 gn) {
+    return !(dot(wi, gn) * dot(to_world(b.fr, b.wo), gn) <= 0.0f);
+}
+
+struct BsdfSample {
+    F3 wi;
+    float fwd, rev, cos_out, cont;
+    int event;
+};
+
+// BxDF::sample (BXDF.cpp:25-53, 136-205, 302-337, 384-489, 568-632)
+__device__ F3 bsdf_sample(const Bsdf& b, F3 gn, float u0, float u1, float u2, BsdfSample& o) {
+    const F3 zero = f3s(0.0f);
+    F3 wo = b.wo;
+    if (b.kind == BK_SPEC_REFL) {
+        o.event = EV_REFLECTION | EV_GLOSSY | EV_SPECULAR;
+        F3 wil = f3(-wo.x, -wo.y, wo.z);
+        o.cos_out = fabsf(wil.z);
+        if (o.cos_out < 1e-6f) return zero;
+        o.wi = to_world(b.fr, wil);
+        if (!same_side(b, o.wi, gn)) return zero;
+        o.fwd = 1.0f;
+        o.rev = 1.0f;
+        o.cont = b.cont;
+        return b.R;
+    }
+    if (b.kind == BK_FRESNEL_SPEC) {
+        float eta = b.ior;
+        if (wo.z > 0.0f) eta = 1.0f / b.ior;
+        float f = fresnel_eta(fabsf(wo.z), 1.0f / eta);
+        if (u0 < f) {
+            o.event = EV_GLOSSY | EV_SPECULAR | EV_REFLECTION;
+            F3 wil = f3(-wo.x, -wo.y, wo.z);
+            o.cos_out = fabsf(wil.z);
+            if (o.cos_out < 1e-6f) return zero;
+            o.wi = to_world(b.fr, wil);
+            if (!same_side(b, o.wi, gn)) return zero;
+            o.fwd = f;
+            o.rev = f;
+            o.cont = b.cont;
+            return b.R;
+        }
+        o.event = EV_TRANSMISSION | EV_GLOSSY | EV_SPECULAR;
+        F3 wil;
+        if (!refract(wo, wo.z > 0.0f ? f3(0.0f, 0.0f, 1.0f) : f3(0.0f, 0.0f, -1.0f), eta, &wil)) return zero;
+        o.cos_out = fabsf(wil.z);
+        if (o.cos_out < 1e-6f) return zero;
+        o.wi = to_world(b.fr, wil);
+        if (dot(o.wi, gn) * dot(to_world(b.fr, wo), gn) >= 0.0f) return zero;
+        o.fwd = 1.0f - f;
+        o.rev = 1.0f - f;
+        o.cont = b.cont;
+        if (b.radiance) return (b.R * eta) * eta;
+        return b.R;
+    }
+    if (b.kind == BK_MRG) {
+        o.event = EV_GLOSSY | EV_REFLECTION;
+        F3 h = ggx_half(b.a2, u1, u2);
+        if (wo.z < 0.0f) h.z = -h.z;
+        F3 wil = reflect(-wo, h);
+        o.cos_out = fabsf(wil.z);
+        if (o.cos_out < 1e-6f) return zero;
+        o.wi = to_world(b.fr, wil);
+        if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+        float chn = fabsf(h.z);
+        float d = ggx_D(chn, b.a2);
+        o.fwd = o.rev = d * chn * 0.25f / absdot(h, wo);
+        o.cont = b.cont;
+        float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+        return div_recip((b.R * g2) * absdot(wo, h), fabsf(wo.z) * chn);
+    }
+    // plastics
+    float f = fresnel_refl(fabsf(wo.z), b.s);
+    if (b.kind == BK_FMG) {
+        if (u0 < f) {
+            o.event = EV_GLOSSY | EV_REFLECTION;
+            F3 h = ggx_half(b.a2, u1, u2);
+            if (wo.z < 0.0f) h.z = -h.z;
+            F3 wil = reflect(-wo, h);
+            o.cos_out = fabsf(wil.z);
+            if (o.cos_out < 1e-6f) return zero;
+            o.wi = to_world(b.fr, wil);
+            if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+            float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+            o.cont = smax(0.05f, f);
+            float fd = fabsf(wil.z) * kInvPI, rd = fabsf(wo.z) * kInvPI;
+            float chn = fabsf(h.z);
+            float d = ggx_D(chn, b.a2);
+            float sp = d * chn * 0.25f / absdot(wo, h);
+            if (f > 1.0f) {
+                o.fwd = o.rev = sp;
+            } else {
+                o.fwd = fd * (1.0f - f) + sp * f;
+                o.rev = rd * (1.0f - f) + sp * f;
+            }
+            return f3s(g2 * absdot(wo, h) / (fabsf(wo.z) * chn));
+        }
+        o.event = EV_DIFFUSE;
+        F3 wil = cos_hemisphere(u1, u2);
+        o.cos_out = wil.z;
+        if (o.cos_out < 1e-6f) return zero;
+        if (wo.z < 0.0f) wil.z = -wil.z;
+        o.wi = to_world(b.fr, wil);
+        if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+        o.cont = smax(0.05f, maxc(b.R * (1.0f - f)));
+        float fd = fabsf(wil.z) * kInvPI, rd = fabsf(wo.z) * kInvPI;
+        F3 h = normalize(wo + wil);
+        float chn = fabsf(h.z);
+        float d = ggx_D(chn, b.a2);
+        float sp = d * chn * 0.25f / absdot(wo, h);
+        if (f > 1.0f) {
+            o.fwd = o.rev = sp;
+        } else {
+            o.fwd = fd * (1.0f - f) + sp * f;
+            o.rev = rd * (1.0f - f) + sp * f;
+        }
+        return b.R;
+    }
+    // BK_FSMG
+    if (u0 < f) {
+        o.event = EV_SPECULAR | EV_REFLECTION | EV_GLOSSY;
+        F3 wil = f3(-wo.x, -wo.y, wo.z);
+        o.cos_out = fabsf(wil.z);
+        if (o.cos_out < 1e-6f) return zero;
+        o.wi = to_world(b.fr, wil);
+        if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+        o.fwd = f;
+        o.rev = f;
+        o.cont = smax(0.05f, f);
+        return f3s(1.0f);
+    }
+    o.event = EV_DIFFUSE;
+    F3 wil = cos_hemisphere(u1, u2);
+    o.cos_out = wil.z;
+    if (o.cos_out < 1e-6f) return zero;
+    if (wo.z < 0.0f) wil.z = -wil.z;
+    o.wi = to_world(b.fr, wil);
+    if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+    o.fwd = fabsf(wil.z) * kInvPI;
+    o.rev = fabsf(wo.z) * kInvPI;
+    o.cont = smax(0.05f, maxc(b.R * (1.0f - f)));
+    return b.R;
+}
+
+// BxDF::evaluate and BxDF::pdf together, for the merge (Path_Vertices.h:46-68)
+__device__ __forceinline__ F3 bsdf_eval_pdf(const Bsdf& b, F3 wi, F3 gn, float& fwd, float& rev) {
+    const F3 zero = f3s(0.0f);
+    F3 wo = b.wo;
+    F3 wil = to_local(b.fr, wi);
+    bool side = same_side(b, wi, gn);
+    if (b.kind == BK_MRG) {
+        F3 h = normalize(wo + wil);
+        float chn = fabsf(h.z);
+        float d = ggx_D(chn, b.a2);
+        fwd = rev = d * chn * 0.25f / absdot(h, wo);
+        if (!side || wil.z * wo.z <= 0.0f) return zero;
+        float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+        return div_recip(((b.R * d) * g2) * 0.25f, fabsf(wo.z));
+    }
+    if (b.kind == BK_FMG) {
+        float fd = fabsf(wil.z) * kInvPI, rd = fabsf(wo.z) * kInvPI;
+        F3 h = normalize(wo + wil);
+        float chn = fabsf(h.z);
+        float d = ggx_D(chn, b.a2);
+        float sp = d * chn * 0.25f / absdot(h, wo);
+        float f = fresnel_refl(fabsf(wo.z), b.s);
+        if (f > 1.0f) {
+            fwd = rev = sp;
+        } else {
+            fwd = fd * (1.0f - f) + sp * f;
+            rev = rd * (1.0f - f) + sp * f;
+        }
+        if (!side || wil.z * wo.z <= 0.0f) return zero;
+        // evaluate (BXDF.cpp:491-519) normalises wi + wo in that order
+        F3 h2 = normalize(wil + wo);
+        F3 diffuse = b.R * kInvPI;
+        float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+        float chn2 = fabsf(h2.z);
+        float d2 = ggx_D(chn2, b.a2);
+        float spec = f * d2 * g2 * 0.25f / fabsf(wo.z);
+        return f3(diffuse.x + spec, diffuse.y + spec, diffuse.z + spec);
+    }
+    if (b.kind == BK_FSMG) {
+        float f = fresnel_refl(fabsf(wo.z), b.s);
+        if (f > 1.0f) {
+            fwd = rev = 0.0f;
+        } else {
+            fwd = fabsf(wil.z) * kInvPI * (1.0f - f);
+            rev = fabsf(wo.z) * kInvPI * (1.0f - f);
+        }
+        if (!side || wil.z * wo.z <= 0.0f) return zero;
+        return b.R * kInvPI;
+    }
+    fwd = rev = 0.0f;
+    return zero;
+}
+
+// ------------------------------------------------------------------
+// Lights: Lights_list::sample_lights (Light_list.cpp:10-30) and
+// Area_Light / Sun_Light / Point_Light::sample_light (Lights.cpp, Lights.h)
+// ------------------------------------------------------------------
+__device__ __forceinline__ int find_interval(const float* cdf, int size, float u) {
+    int first = 0, len = size;
+    while (len > 0) {
+        int half = len >> 1, middle = first + half;
+        if (cdf[middle] < u) {
+            first = middle + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    int r = first - 1;
+    r = r < 0 ? 0 : r;
+    r = r > size - 2 ? size - 2 : r;
+    return r;
+}
+
+// Returns false when the light emits nothing (the reference point light).
+__device__ bool sample_light(const SceneView& sc, const PathRng& rng, F3& o, F3& d, F3& tp) {
+    float rnd = rng.u(0);
+    int idx = find_interval(sc.cdf, sc.nlights + 1, rnd);
+    const DevLight& L = sc.lights[idx];
+    float pick = sc.compat_pdf ? L.power / ((float)sc.nlights * sc.integral) : L.power / sc.integral;
+    F3 t;
+    if (L.kind == L_AREA) {
+        float ax = rng.u(1), ay = rng.u(2);
+        o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
+        F3 local = cos_hemisphere(rng.u(3), rng.u(4));
+        Frame fr{ld3(L.fx), ld3(L.fy), ld3(L.fz)};
+        d = to_world(fr, local);
+        t = ld3(L.color) * kPI;
+    } else if (L.kind == L_SUN) {
+        float ax = rng.u(1), ay = rng.u(2);
+        o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
+        d = ld3(L.normal);
+        t = ld3(L.color);
+    } else {
+        if (sc.compat_point) return false;
+        float u1 = rng.u(1), u2 = rng.u(2);
+        float z = 1.0f - 2.0f * u1;
+        float rr = sqrtf(smax(0.0f, 1.0f - z * z));
+        float phi = kPI2 * u2;
+        F3 n = f3(rr * own_cos(phi), rr * own_sin(phi), z);
+        float e1 = rng.u(3), e2 = rng.u(4);
+        if (L.radius > 0.0f) {
+            o = ld3(L.pos) + n * L.radius;
+            Frame fr = frame_from_z(n);
+            d = to_world(fr, cos_hemisphere(e1, e2));
+        } else {
+            o = ld3(L.pos);
+            d = n;
+        }
+        t = (ld3(L.color) * L.power) * kPI;
+    }
+    tp = div_each(t, pick);
+    return true;
+}
+
+// ------------------------------------------------------------------
+// Closest hit: binary BVH with both child boxes in the parent node,
+// near-child-first descent, stack in LDS (kLdsStack entries per lane,
+// stride kBlock) with a private overflow.  Triangle test is
+// Triangle::hit (Triangle.cpp:51-70) with strict tmin < t < t_best.
+// ------------------------------------------------------------------
+struct Hit {
+    float t, u, v;
+    int tri;
+};
+
+__device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int i, F3 o, F3 d, float tmin, float tbest,
+                                         float& t_out, float& u_out, float& v_out) {
+    float4 a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
+    F3 v0 = f3(a.x, a.y, a.z), e1 = f3(b.x, b.y, b.z), e2 = f3(c.x, c.y, c.z);
+    F3 pvec = cross(d, e2);
+    float det = dot(e1, pvec);
+    if (fabsf(det) < 0.0000001f) return false;
+    float inv = 1.0f / det;
+    F3 tvec = o - v0;
+    float u = dot(tvec, pvec) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    F3 qvec = cross(tvec, e1);
+    float v = dot(d, qvec) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float t = dot(e2, qvec) * inv;
+    if (t > tmin && t < tbest) {
+        t_out = t;
+        u_out = u;
+        v_out = v;
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack) {
+    int ext[kMaxDepth - kLdsStack];
+    float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    float tbest = INFINITY;
+    int best = -1;
+    float bu = 0.0f, bv = 0.0f;
+    int sp = 0;
+    int ref = sc.root;
+    const float4* __restrict__ nodes = sc.nodes;
+    // Slab far distances are widened by 1 + 2^-19 so rounding in the slab
+    // arithmetic never culls a box the ray touches (boxes are also padded
+    // outwards when built); NaN slabs (0 * inf) drop out of fminf/fmaxf.
+    const float slack = 1.0000019f;
+    while (true) {
+        if (ref >= 0) {
+            const float4* n = nodes + 4 * ref;
+            float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
+            float l0x = (n0.x - o.x) * ix, l1x = (n0.y - o.x) * ix;
+            float l0y = (n0.z - o.y) * iy, l1y = (n0.w - o.y) * iy;
+            float l0z = (n1.x - o.z) * iz, l1z = (n1.y - o.z) * iz;
+            float r0x = (n1.z - o.x) * ix, r1x = (n1.w - o.x) * ix;
+            float r0y = (n2.x - o.y) * iy, r1y = (n2.y - o.y) * iy;
+            float r0z = (n2.z - o.z) * iz, r1z = (n2.w - o.z) * iz;
+            float ln = fmaxf(fmaxf(fminf(l0x, l1x), fminf(l0y, l1y)), fmaxf(fminf(l0z, l1z), tmin));
+            float lf = fminf(fminf(fmaxf(l0x, l1x), fmaxf(l0y, l1y)), fmaxf(l0z, l1z)) * slack;
+            float rn = fmaxf(fmaxf(fminf(r0x, r1x), fminf(r0y, r1y)), fmaxf(fminf(r0z, r1z), tmin));
+            float rf = fminf(fminf(fmaxf(r0x, r1x), fmaxf(r0y, r1y)), fmaxf(r0z, r1z)) * slack;
+            lf = fminf(lf, tbest);
+            rf = fminf(rf, tbest);
+            bool hl = ln <= lf, hr = rn <= rf;
+            int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
+            if (hl && hr) {
+                int nearc = ln <= rn ? cl : cr, farc = ln <= rn ? cr : cl;
+                if (sp < kLdsStack) lds_stack[sp * kBlock] = farc;
+                else ext[sp - kLdsStack] = farc;
+                ++sp;
+                ref = nearc;
+                continue;
+            }
+            if (hl || hr) {
+                ref = hl ? cl : cr;
+                continue;
+            }
+        } else {
+            int enc = ~ref;
+            int first = enc >> 3, count = (enc & 7) + 1;
+            for (int i = first; i < first + count; ++i) {
+                float t, u, v;
+                if (tri_test(sc.tri, i, o, d, tmin, tbest, t, u, v)) {
+                    tbest = t;
+                    best = i;
+                    bu = u;
+                    bv = v;
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        ref = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+    }
+    h.t = tbest;
+    h.u = bu;
+    h.v = bv;
+    h.tri = best;
+    return best >= 0;
+}
+
+// Hit record of the reference (Triangle.cpp:71-80): geometric normal,
+// barycentric (unnormalised) shading normal, position, material.
+struct Surface {
+    F3 gn, sn, pos;
+    int mat;
+};
+__device__ __forceinline__ Surface surface_at(const SceneView& sc, const Hit& h, F3 o, F3 d) {
+    float4 a = sc.tri[3 * h.tri], b = sc.tri[3 * h.tri + 1], c = sc.tri[3 * h.tri + 2];
+    F3 e1 = f3(b.x, b.y, b.z), e2 = f3(c.x, c.y, c.z);
+    float4 s0 = sc.shade[3 * h.tri], s1 = sc.shade[3 * h.tri + 1], s2 = sc.shade[3 * h.tri + 2];
+    Surface s;
+    s.gn = normalize(cross(e1, e2));
+    s.sn = (f3(s0.x, s0.y, s0.z) * (1.0f - h.u - h.v) + f3(s1.x, s1.y, s1.z) * h.u) + f3(s2.x, s2.y, s2.z) * h.v;
+    s.pos = o + d * h.t;
+    s.mat = __float_as_int(a.w);
+    return s;
+}
+
+}  // namespace rc

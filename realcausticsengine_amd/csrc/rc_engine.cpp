@@ -1,0 +1,1371 @@
+// rc_engine.cpp -- host side of libRealCaustics.so: the exported C-ABI
+// (init / main of src/main.cpp:89-223 plus the rc_* additions), Blender
+// struct ingest, the binned-SAH BVH build, light set-up, the per-iteration
+// schedule of the HIP kernels, and the EXR writer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/real_caustics.h"
+#include "rc_engine.h"
+#include "rc_vec.h"
+
+using namespace rc;
+
+namespace {
+
+// ---------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------
+std::string g_error;
+std::atomic<int> g_cancel{0};
+
+void set_error(const std::string& e) {
+    if (g_error.empty()) g_error = e;
+}
+
+#define HIP_OK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t err_ = (expr);                                                             \
+        if (err_ != hipSuccess) {                                                             \
+            set_error(std::string("HIP error: ") + hipGetErrorString(err_) + " at " #expr); \
+            return false;                                                                     \
+        }                                                                                     \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t count) {
+        if (count <= n && p) return true;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        if (count == 0) count = 1;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc failed");
+            return false;
+        }
+        n = count;
+        return true;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+struct HostLight {
+    char type;
+    float color[3], position[3], width, height, spread, rotation[3], power, angle, radius;
+};
+
+struct Box {
+    float lo[3], hi[3];
+};
+
+// ---------------------------------------------------------------------
+// engine state (one process-wide scene, as src/main.cpp:59)
+// ---------------------------------------------------------------------
+struct Engine {
+    bool initialised = false, prepared = false;
+    std::string init_error;  // what init() rejected; reported again by main()
+    int samples = 1;
+    float base_radius = 0.01f, radius_alpha = 0.75f;
+    int debug_test = 1;
+    // geometry in world space, global triangle order (mesh, then triangle)
+    std::vector<float> wv;        // 3 floats per vertex (all meshes)
+    std::vector<float> wn;        // 3 floats per vertex
+    std::vector<uint32_t> tris;   // v0 v1 v2 material per triangle (global vertex ids)
+    std::vector<uint32_t> mesh_tri_offset;
+    std::vector<DevMaterial> mats;
+    std::vector<HostLight> py_lights;
+    int W = 0, H = 0;
+    F3 cam_o, cam_bl, cam_h, cam_v;
+    Box bbox;
+    float scene_radius = 0.0f;
+    // options
+    uint64_t seed = 0;
+    long long photons = 0;
+    int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
+    double photon_capacity = 0, cell_scale = 1.0;
+    int persistent_blocks = 0;
+    std::string output_path = "D:\\caustics.exr";
+    // device scene
+    DevBuf<float4> d_tri, d_shade, d_nodes;
+    DevBuf<int> d_tri_id;
+    DevBuf<DevMaterial> d_mats;
+    DevBuf<DevLight> d_lights;
+    DevBuf<float> d_cdf;
+    std::vector<DevLight> lights;
+    float integral = 0.0f;
+    int root = 0, n_nodes = 0;
+    // per-iteration buffers
+    DevBuf<float4> ph_pos, ph_dir, s_pos, s_dir;
+    DevBuf<float2> ph_tp, s_tp;
+    DevBuf<uint32_t> ph_uid, s_index, vals_a, vals_b, hist, hist_sums, counters;
+    DevBuf<unsigned long long> keys_a, keys_b;
+    DevBuf<uint2> cells;
+    DevBuf<float> image;
+    DevBuf<Counters> stats_dev;
+    uint32_t capacity = 0;
+    int iterations_done = 0;
+    double stats[RC_STAT_COUNT] = {0};
+    hipStream_t stream = nullptr;
+    int device_cus = 0;
+};
+
+Engine* g = nullptr;
+
+// ---------------------------------------------------------------------
+// Scene ingest (Scene::init_meshes, Scene.cpp:14-43; Mesh::Mesh, mesh.cpp:12-26)
+// ---------------------------------------------------------------------
+void inverse3x3_of4(const float* M, float inv[3][3]) {
+    // Inverse (matrix.cpp:4-63) in double; only the upper 3x3 is used by ApplyToNormal.
+    double a[4][4];
+    for (int r = 0; r < 4; ++r)
+        for (int c = 0; c < 4; ++c) a[r][c] = M[4 * r + c];
+    double A2323 = a[2][2] * a[3][3] - a[2][3] * a[3][2], A1323 = a[2][1] * a[3][3] - a[2][3] * a[3][1];
+    double A1223 = a[2][1] * a[3][2] - a[2][2] * a[3][1], A0323 = a[2][0] * a[3][3] - a[2][3] * a[3][0];
+    double A0223 = a[2][0] * a[3][2] - a[2][2] * a[3][0], A0123 = a[2][0] * a[3][1] - a[2][1] * a[3][0];
+    double A2313 = a[1][2] * a[3][3] - a[1][3] * a[3][2], A1313 = a[1][1] * a[3][3] - a[1][3] * a[3][1];
+    double A1213 = a[1][1] * a[3][2] - a[1][2] * a[3][1], A0313 = a[1][0] * a[3][3] - a[1][3] * a[3][0];
+    double A0213 = a[1][0] * a[3][2] - a[1][2] * a[3][0], A0113 = a[1][0] * a[3][1] - a[1][1] * a[3][0];
+    double det = a[0][0] * (a[1][1] * A2323 - a[1][2] * A1323 + a[1][3] * A1223) -
+                 a[0][1] * (a[1][0] * A2323 - a[1][2] * A0323 + a[1][3] * A0223) +
+                 a[0][2] * (a[1][0] * A1323 - a[1][1] * A0323 + a[1][3] * A0123) -
+                 a[0][3] * (a[1][0] * A1223 - a[1][1] * A0223 + a[1][2] * A0123);
+    det = 1.0 / det;
+    inv[0][0] = (float)(det * (a[1][1] * A2323 - a[1][2] * A1323 + a[1][3] * A1223));
+    inv[0][1] = (float)(det * -(a[0][1] * A2323 - a[0][2] * A1323 + a[0][3] * A1223));
+    inv[0][2] = (float)(det * (a[0][1] * A2313 - a[0][2] * A1313 + a[0][3] * A1213));
+    inv[1][0] = (float)(det * -(a[1][0] * A2323 - a[1][2] * A0323 + a[1][3] * A0223));
+    inv[1][1] = (float)(det * (a[0][0] * A2323 - a[0][2] * A0323 + a[0][3] * A0223));
+    inv[1][2] = (float)(det * -(a[0][0] * A2313 - a[0][2] * A0313 + a[0][3] * A0213));
+    inv[2][0] = (float)(det * (a[1][0] * A1323 - a[1][1] * A0323 + a[1][3] * A0123));
+    inv[2][1] = (float)(det * -(a[0][0] * A1323 - a[0][1] * A0323 + a[0][3] * A0123));
+    inv[2][2] = (float)(det * (a[0][0] * A1313 - a[0][1] * A0313 + a[0][3] * A0113));
+}
+
+DevMaterial convert_material(const rc_python_material& pm) {
+    // Scene::init_materials (Scene.cpp:45-68) and materials.h:59-146
+    DevMaterial m{};
+    F3 c = f3(pm.color[0], pm.color[1], pm.color[2]);
+    char t = pm.type;
+    if (t != 'P' && t != 'G' && t != 'M') {
+        t = 'P';
+        c = f3(1.0f, 1.0f, 1.0f);
+    }
+    auto cont_of = [](F3 v) { return smax(0.05f, smax(smax(v.x, v.y), v.z)); };
+    m.R[0] = c.x;
+    m.R[1] = c.y;
+    m.R[2] = c.z;
+    m.ior = 1.0f;
+    if (t == 'P') {
+        float rough = (pm.type == 'P') ? pm.roughness * pm.roughness : 1.0f;
+        float spec = (pm.type == 'P') ? pm.specular * 0.08f : 0.0f * 0.08f;
+        m.s = spec;
+        m.cont = 1.0f;
+        if (rough < 0.01f) {
+            m.kind = BK_FSMG;
+        } else {
+            m.kind = BK_FMG;
+            m.a2 = rough * rough;
+        }
+    } else if (t == 'G') {
+        m.kind = BK_FRESNEL_SPEC;
+        m.ior = pm.ior;
+        m.cont = cont_of(c);
+    } else {
+        if (pm.roughness < 0.01f) {
+            m.kind = BK_SPEC_REFL;
+        } else {
+            m.kind = BK_MRG;
+            m.a2 = pm.roughness * pm.roughness;
+        }
+        m.cont = cont_of(c);
+    }
+    return m;
+}
+
+// ---------------------------------------------------------------------
+// Lights (Scene::init_lights, Scene.cpp:70-108; Lights.cpp)
+// ---------------------------------------------------------------------
+struct M3 {
+    F3 i, j, k;
+};
+M3 rot_matrix(F3 r) {  // matrix_3x3(vec3), matrix.cpp:77-89 (double cos/sin on this platform)
+    double x = r.x, y = r.y, z = r.z;
+    M3 m;
+    m.i = f3((float)(cos(z) * cos(y)), (float)(sin(z) * cos(y)), (float)(-sin(y)));
+    m.j = f3((float)(cos(z) * sin(y) * sin(x) - sin(z) * cos(x)), (float)(sin(z) * sin(y) * sin(x) + cos(z) * cos(x)),
+             (float)(cos(y) * sin(x)));
+    m.k = f3((float)(cos(z) * sin(y) * cos(x) + sin(z) * sin(x)), (float)(sin(z) * sin(y) * cos(x) - cos(z) * sin(x)),
+             (float)(cos(y) * cos(x)));
+    return m;
+}
+F3 mul_vm(F3 v, M3 m) { return (m.i * v.x + m.j * v.y) + m.k * v.z; }
+F3 rotate_deg(F3 v, F3 p, F3 rot) {
+    const float PI = 3.14159265359f;
+    M3 m = rot_matrix(f3(rot.x * PI / 180.f, rot.y * PI / 180.f, rot.z * PI / 180.f));
+    return mul_vm(v - p, m) + p;
+}
+F3 project_onto_plane(F3 point, F3 center, F3 normal) {
+    F3 tp = point - center;
+    return (tp - normal * (dot(tp, normal) / len2(normal))) + center;
+}
+void put3(float* d, F3 v) {
+    d[0] = v.x;
+    d[1] = v.y;
+    d[2] = v.z;
+}
+
+void build_lights(Engine& e) {
+    e.lights.clear();
+    for (const HostLight& p : e.py_lights) {
+        DevLight L{};
+        F3 c = f3(p.color[0], p.color[1], p.color[2]);
+        L.power = p.power;
+        L.radius = p.radius;
+        put3(L.pos, f3(p.position[0], p.position[1], p.position[2]));
+        if (p.type == 'P') {
+            L.kind = L_POINT;
+            put3(L.color, c);
+        } else if (p.type == 'S') {
+            L.kind = L_SUN;
+            put3(L.color, c * p.power);
+            M3 m = rot_matrix(f3(p.rotation[0], p.rotation[1], p.rotation[2]));
+            F3 normal = mul_vm(f3(0.0f, 0.0f, -1.0f) - f3s(0.0f), m) + f3s(0.0f);
+            M3 inv{f3(m.i.x, m.j.x, m.k.x), f3(m.i.y, m.j.y, m.k.y), f3(m.i.z, m.j.z, m.k.z)};
+            F3 mn = f3(e.bbox.lo[0], e.bbox.lo[1], e.bbox.lo[2]), mx = f3(e.bbox.hi[0], e.bbox.hi[1], e.bbox.hi[2]);
+            F3 pos = (-normal) * sqrtf(len2(mx - mn)) + (mx + mn) * 0.5f;
+            F3 pts[8] = {mn, f3(mx.x, mn.y, mn.z), f3(mn.x, mx.y, mn.z), f3(mx.x, mx.y, mn.z),
+                         f3(mn.x, mn.y, mx.z), f3(mx.x, mn.y, mx.z), f3(mn.x, mx.y, mx.z), mx};
+            F3 hi = f3(-INFINITY, -INFINITY, 0.0f), lo = f3(INFINITY, INFINITY, 0.0f);
+            for (int k = 0; k < 8; ++k) {
+                F3 loc = mul_vm(project_onto_plane(pts[k], pos, normal) - pos, inv);
+                if (loc.x > hi.x) hi.x = loc.x;
+                if (loc.y > hi.y) hi.y = loc.y;
+                if (loc.x < lo.x) lo.x = loc.x;
+                if (loc.y < lo.y) lo.y = loc.y;
+            }
+            F3 bl = mul_vm(lo, m) + pos;
+            F3 brc = mul_vm(f3(hi.x, lo.y, 0.0f), m) + pos, tlc = mul_vm(f3(lo.x, hi.y, 0.0f), m) + pos;
+            put3(L.bl, bl);
+            put3(L.h, brc - bl);
+            put3(L.v, tlc - bl);
+            put3(L.normal, normal);
+            put3(L.pos, pos);
+        } else {
+            L.kind = L_AREA;
+            put3(L.color, c * p.power);
+            F3 pos = f3(p.position[0], p.position[1], p.position[2]);
+            F3 rr = f3(p.rotation[0], p.rotation[1], p.rotation[2]);
+            F3 bl = pos - f3(p.width / 2.f, p.height / 2.f, 0.f);
+            F3 hh = f3(p.width, 0.0f, 0.0f), vv = f3(0.0f, p.height, 0.0f), n = f3(0.0f, 0.0f, -1.0f);
+            F3 tl = bl + vv, br = bl + hh;
+            bl = rotate_deg(bl, pos, rr);
+            tl = rotate_deg(tl, pos, rr);
+            br = rotate_deg(br, pos, rr);
+            n = rotate_deg(n, f3(0.0f, 0.0f, 0.0f), rr);
+            put3(L.bl, bl);
+            put3(L.h, br - bl);
+            put3(L.v, tl - bl);
+            put3(L.normal, n);
+            Frame fr = frame_from_z(n);
+            put3(L.fx, fr.x);
+            put3(L.fy, fr.y);
+            put3(L.fz, fr.z);
+        }
+        e.lights.push_back(L);
+    }
+}
+
+// ---------------------------------------------------------------------
+// BVH build: binned SAH over all world-space triangles, binary nodes that
+// hold both child boxes, leaves of at most kLeafMax triangles stored
+// contiguously (the triangle arrays are reordered to leaf order).
+// ---------------------------------------------------------------------
+struct BuildItem {
+    float lo[3], hi[3], c[3];
+    uint32_t id;
+};
+
+struct Builder {
+    std::vector<BuildItem> items;
+    std::vector<float4> nodes;  // 4 per node
+    std::vector<uint32_t> order;
+    int max_depth = 0;
+
+    static void grow(Box& b, const float* lo, const float* hi) {
+        for (int k = 0; k < 3; ++k) {
+            b.lo[k] = std::min(b.lo[k], lo[k]);
+            b.hi[k] = std::max(b.hi[k], hi[k]);
+        }
+    }
+    static Box empty() {
+        Box b;
+        for (int k = 0; k < 3; ++k) {
+            b.lo[k] = INFINITY;
+            b.hi[k] = -INFINITY;
+        }
+        return b;
+    }
+    static float area(const Box& b) {
+        float d[3];
+        for (int k = 0; k < 3; ++k) d[k] = std::max(0.0f, b.hi[k] - b.lo[k]);
+        return 2.0f * (d[0] * d[1] + d[0] * d[2] + d[1] * d[2]);
+    }
+    static Box padded(Box b) {
+        // outward padding so the kernel's slab test never rejects a box
+        // containing a hit because of rounding
+        for (int k = 0; k < 3; ++k) {
+            float m = std::max(std::fabs(b.lo[k]), std::fabs(b.hi[k]));
+            float pad = m * 9.5367431640625e-07f + 1e-30f;  // 2^-20 relative
+            b.lo[k] -= pad;
+            b.hi[k] += pad;
+        }
+        return b;
+    }
+
+    int make_leaf(int start, int end) {
+        int first = (int)order.size();
+        for (int i = start; i < end; ++i) order.push_back(items[i].id);
+        return ~((first << 3) | (end - start - 1));
+    }
+
+    // returns child ref; writes bounds
+    int build(int start, int end, int depth, Box& out) {
+        max_depth = std::max(max_depth, depth);
+        Box b = empty(), cb = empty();
+        for (int i = start; i < end; ++i) {
+            grow(b, items[i].lo, items[i].hi);
+            grow(cb, items[i].c, items[i].c);
+        }
+        out = b;
+        int n = end - start;
+        if (n <= 2 && n <= kLeafMax) return make_leaf(start, end);
+        int mid = -1;
+        constexpr int kBins = 16;
+        if (depth < kMaxDepth - 8) {
+            float best_cost = INFINITY;
+            int best_axis = -1, best_split = -1;
+            for (int ax = 0; ax < 3; ++ax) {
+                float ext = cb.hi[ax] - cb.lo[ax];
+                if (!(ext > 0.0f)) continue;
+                float scale = kBins / ext;
+                Box bb[kBins];
+                int cnt[kBins] = {0};
+                for (int k = 0; k < kBins; ++k) bb[k] = empty();
+                for (int i = start; i < end; ++i) {
+                    int bi = std::min(kBins - 1, (int)((items[i].c[ax] - cb.lo[ax]) * scale));
+                    cnt[bi]++;
+                    grow(bb[bi], items[i].lo, items[i].hi);
+                }
+                float left_area[kBins];
+                int left_cnt[kBins];
+                Box acc = empty();
+                int c = 0;
+                for (int k = 0; k < kBins; ++k) {
+                    if (cnt[k]) grow(acc, bb[k].lo, bb[k].hi);
+                    c += cnt[k];
+                    left_area[k] = c ? area(acc) : 0.0f;
+                    left_cnt[k] = c;
+                }
+                acc = empty();
+                c = 0;
+                for (int k = kBins - 1; k >= 1; --k) {
+                    if (cnt[k]) grow(acc, bb[k].lo, bb[k].hi);
+                    c += cnt[k];
+                    if (c == 0 || left_cnt[k - 1] == 0) continue;
+                    float cost = left_area[k - 1] * left_cnt[k - 1] + area(acc) * c;
+                    if (cost < best_cost) {
+                        best_cost = cost;
+                        best_axis = ax;
+                        best_split = k;
+                    }
+                }
+            }
+            float parent = area(b);
+            float leaf_cost = (float)n;
+            float split_cost = 0.5f + (parent > 0.0f ? best_cost / parent : INFINITY);
+            if (best_axis >= 0 && (n > kLeafMax || split_cost < leaf_cost)) {
+                float scale = kBins / (cb.hi[best_axis] - cb.lo[best_axis]);
+                auto pred = [&](const BuildItem& it) {
+                    return std::min(kBins - 1, (int)((it.c[best_axis] - cb.lo[best_axis]) * scale)) < best_split;
+                };
+                mid = (int)(std::partition(items.begin() + start, items.begin() + end, pred) - items.begin());
+                if (mid == start || mid == end) mid = -1;
+            } else if (best_axis < 0 && n <= kLeafMax) {
+                return make_leaf(start, end);
+            } else if (n <= kLeafMax) {
+                return make_leaf(start, end);
+            }
+        }
+        if (mid < 0) {
+            if (n <= kLeafMax) return make_leaf(start, end);
+            // median split on the widest centroid axis
+            int ax = 0;
+            float best = -1.0f;
+            for (int k = 0; k < 3; ++k)
+                if (cb.hi[k] - cb.lo[k] > best) {
+                    best = cb.hi[k] - cb.lo[k];
+                    ax = k;
+                }
+            mid = (start + end) / 2;
+            std::nth_element(items.begin() + start, items.begin() + mid, items.begin() + end,
+                             [&](const BuildItem& a, const BuildItem& b2) { return a.c[ax] < b2.c[ax]; });
+        }
+        int id = (int)(nodes.size() / 4);
+        nodes.resize(nodes.size() + 4);
+        Box lb, rb;
+        int l = build(start, mid, depth + 1, lb);
+        int r = build(mid, end, depth + 1, rb);
+        lb = padded(lb);
+        rb = padded(rb);
+        float4* nd = &nodes[4 * (size_t)id];
+        nd[0] = make_float4(lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]);
+        nd[1] = make_float4(lb.lo[2], lb.hi[2], rb.lo[0], rb.hi[0]);
+        nd[2] = make_float4(rb.lo[1], rb.hi[1], rb.lo[2], rb.hi[2]);
+        int refs[2] = {l, r};
+        float fl, fr;
+        memcpy(&fl, &refs[0], 4);
+        memcpy(&fr, &refs[1], 4);
+        nd[3] = make_float4(fl, fr, 0.0f, 0.0f);
+        return id;
+    }
+};
+
+template <typename T>
+bool upload(DevBuf<T>& b, const std::vector<T>& v) {
+    if (!b.alloc(v.size())) return false;
+    if (!v.empty()) HIP_OK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return true;
+}
+
+SceneView scene_view(const Engine& e) {
+    SceneView sc{};
+    sc.tri = e.d_tri.p;
+    sc.shade = e.d_shade.p;
+    sc.tri_id = e.d_tri_id.p;
+    sc.nodes = e.d_nodes.p;
+    sc.root = e.root;
+    sc.mats = e.d_mats.p;
+    sc.lights = e.d_lights.p;
+    sc.cdf = e.d_cdf.p;
+    sc.nlights = (int)e.lights.size();
+    sc.integral = e.integral;
+    put3(sc.cam_o, e.cam_o);
+    put3(sc.cam_bl, e.cam_bl);
+    put3(sc.cam_h, e.cam_h);
+    put3(sc.cam_v, e.cam_v);
+    sc.W = e.W;
+    sc.H = e.H;
+    sc.max_path_length = e.max_path_length;
+    sc.compat_point = e.compat_point;
+    sc.compat_pdf = e.compat_pdf;
+    sc.compat_rows = e.compat_rows;
+    return sc;
+}
+
+bool ensure_device(Engine& e) {
+    if (e.stream) return true;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        set_error("no HIP device available: libRealCaustics needs an MI355X (gfx950) GPU");
+        return false;
+    }
+    HIP_OK(hipSetDevice(0));
+    hipDeviceProp_t prop;
+    HIP_OK(hipGetDeviceProperties(&prop, 0));
+    e.device_cus = prop.multiProcessorCount;
+    HIP_OK(hipStreamCreate(&e.stream));
+    return true;
+}
+
+bool prepare(Engine& e) {
+    if (e.prepared) return true;
+    if (!e.initialised) {
+        set_error("main/rc_prepare called before init");
+        return false;
+    }
+    if (!g_error.empty()) return false;
+    if (!ensure_device(e)) return false;
+    size_t ntri = e.tris.size() / 4;
+    if (ntri == 0) {
+        set_error("scene has no triangles");
+        return false;
+    }
+    // scene bounds (union of triangle boxes, as the root of BVH_world.cpp)
+    Builder bld;
+    bld.items.resize(ntri);
+    Box sb = Builder::empty();
+    for (size_t t = 0; t < ntri; ++t) {
+        BuildItem& it = bld.items[t];
+        it.id = (uint32_t)t;
+        for (int k = 0; k < 3; ++k) {
+            float a = e.wv[3 * e.tris[4 * t] + k], b = e.wv[3 * e.tris[4 * t + 1] + k], c = e.wv[3 * e.tris[4 * t + 2] + k];
+            it.lo[k] = smin(smin(a, b), c);
+            it.hi[k] = smax(smax(a, b), c);
+            it.c[k] = (it.lo[k] + it.hi[k]) * 0.5f;
+        }
+        for (int k = 0; k < 3; ++k) {
+            sb.lo[k] = smin(sb.lo[k], it.lo[k]);
+            sb.hi[k] = smax(sb.hi[k], it.hi[k]);
+        }
+    }
+    e.bbox = sb;
+    Box rootb;
+    bld.order.reserve(ntri);
+    int root = bld.build(0, (int)ntri, 0, rootb);
+    if (bld.max_depth >= kMaxDepth) {
+        set_error("BVH deeper than the traversal stack");
+        return false;
+    }
+    if (root < 0) {
+        // whole scene in one leaf: wrap it in a node whose second child is an empty box
+        bld.nodes.resize(4);
+        Box lb = Builder::padded(rootb);
+        bld.nodes[0] = make_float4(lb.lo[0], lb.hi[0], lb.lo[1], lb.hi[1]);
+        bld.nodes[1] = make_float4(lb.lo[2], lb.hi[2], INFINITY, -INFINITY);
+        bld.nodes[2] = make_float4(INFINITY, -INFINITY, INFINITY, -INFINITY);
+        float fl, fr;
+        int refs[2] = {root, root};
+        memcpy(&fl, &refs[0], 4);
+        memcpy(&fr, &refs[1], 4);
+        bld.nodes[3] = make_float4(fl, fr, 0.0f, 0.0f);
+        root = 0;
+    }
+    e.root = root;
+    e.n_nodes = (int)(bld.nodes.size() / 4);
+    std::vector<float4> tri(3 * ntri), shade(3 * ntri);
+    std::vector<int> tri_id(ntri);
+    for (size_t i = 0; i < ntri; ++i) {
+        uint32_t t = bld.order[i];
+        const uint32_t* T = &e.tris[4 * (size_t)t];
+        F3 p0 = f3(e.wv[3 * T[0]], e.wv[3 * T[0] + 1], e.wv[3 * T[0] + 2]);
+        F3 p1 = f3(e.wv[3 * T[1]], e.wv[3 * T[1] + 1], e.wv[3 * T[1] + 2]);
+        F3 p2 = f3(e.wv[3 * T[2]], e.wv[3 * T[2] + 1], e.wv[3 * T[2] + 2]);
+        F3 e1 = p1 - p0, e2 = p2 - p0;
+        float matf;
+        int mat = (int)T[3];
+        memcpy(&matf, &mat, 4);
+        tri[3 * i] = make_float4(p0.x, p0.y, p0.z, matf);
+        tri[3 * i + 1] = make_float4(e1.x, e1.y, e1.z, 0.0f);
+        tri[3 * i + 2] = make_float4(e2.x, e2.y, e2.z, 0.0f);
+        for (int k = 0; k < 3; ++k)
+            shade[3 * i + k] = make_float4(e.wn[3 * T[k]], e.wn[3 * T[k] + 1], e.wn[3 * T[k] + 2], 0.0f);
+        tri_id[i] = (int)t;
+    }
+    build_lights(e);
+    if (e.lights.empty()) {
+        set_error("scene has no lights");
+        return false;
+    }
+    // Lights_list::update_pdf_distribution (Light_list.cpp:32-44)
+    std::vector<float> cdf(e.lights.size() + 1);
+    cdf[0] = 0.0f;
+    for (size_t i = 1; i < cdf.size(); ++i) cdf[i] = cdf[i - 1] + e.lights[i - 1].power;
+    e.integral = cdf.back();
+    if (!(e.integral > 0.0f)) {
+        set_error("total light power must be positive");
+        return false;
+    }
+    for (size_t i = 1; i < cdf.size(); ++i) cdf[i] /= e.integral;
+    // main.cpp:166-168
+    F3 mn = f3(sb.lo[0], sb.lo[1], sb.lo[2]), mx = f3(sb.hi[0], sb.hi[1], sb.hi[2]);
+    e.scene_radius = sqrtf(len2(mn - (mn + mx) * 0.5f));
+    e.base_radius *= e.scene_radius;
+    if (!upload(e.d_tri, tri) || !upload(e.d_shade, shade) || !upload(e.d_tri_id, tri_id) ||
+        !upload(e.d_nodes, bld.nodes) || !upload(e.d_mats, e.mats) || !upload(e.d_lights, e.lights) ||
+        !upload(e.d_cdf, cdf))
+        return false;
+    if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
+    HIP_OK(hipMemset(e.image.p, 0, (size_t)e.W * e.H * 3 * sizeof(float)));
+    if (!e.counters.alloc(4) || !e.stats_dev.alloc(1)) return false;
+    e.iterations_done = 0;
+    e.prepared = true;
+    return true;
+}
+
+long long light_paths(const Engine& e) { return e.photons > 0 ? e.photons : (long long)e.W * e.H; }
+
+// Vertex_Merging.cpp:201-212, in float with the C library powf.
+void iteration_params(const Engine& e, int iteration, float& radius, float& r2, float& norm) {
+    float count = (float)light_paths(e);
+    radius = e.base_radius;
+    radius /= powf((float)(iteration + 1), 0.5f * (1.0f - e.radius_alpha));
+    radius = smax(radius, 1e-7f);
+    r2 = radius * radius;
+    const float PI = 3.14159265359f;
+    norm = 1.0f / (PI * r2 * count);
+}
+
+int log2_ceil(uint64_t v) {
+    int b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
+
+int blocks_for(const Engine& e) {
+    if (e.persistent_blocks > 0) return e.persistent_blocks;
+    return std::max(1, e.device_cus) * 8;
+}
+
+bool ensure_photon_buffers(Engine& e, uint32_t cap) {
+    if (cap <= e.capacity && e.ph_pos.p) return true;
+    e.capacity = cap;
+    return e.ph_pos.alloc(cap) && e.ph_dir.alloc(cap) && e.ph_tp.alloc(cap) && e.ph_uid.alloc(cap) &&
+           e.s_pos.alloc(cap) && e.s_dir.alloc(cap) && e.s_tp.alloc(cap) && e.s_index.alloc(cap) &&
+           e.keys_a.alloc(cap) && e.keys_b.alloc(cap) && e.vals_a.alloc(cap) && e.vals_b.alloc(cap) &&
+           e.hist.alloc(photon_map_hist_entries(cap)) && e.hist_sums.alloc(photon_map_sum_entries(cap));
+}
+
+// Light pass into the unsorted photon buffers; returns the photon count.
+bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
+    long long N = light_paths(e);
+    if (N <= 0 || N > (1ll << 28)) {
+        set_error("light path count out of range (1 .. 2^28)");
+        return false;
+    }
+    uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
+    if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        LightPassArgs a{};
+        a.sc = scene_view(e);
+        a.k0 = (uint32_t)e.seed;
+        a.k1 = (uint32_t)(e.seed >> 32);
+        a.iteration = (uint32_t)iteration;
+        a.stream = STREAM_LIGHT;
+        a.n_paths = (uint32_t)N;
+        a.path_counter = e.counters.p;
+        a.photon_counter = e.counters.p + 1;
+        a.capacity = e.capacity;
+        a.out_pos = e.ph_pos.p;
+        a.out_dir = e.ph_dir.p;
+        a.out_tp = e.ph_tp.p;
+        a.out_uid = e.ph_uid.p;
+        launch_light_pass(a, blocks_for(e), e.stream);
+        HIP_OK(hipGetLastError());
+        uint32_t c = 0;
+        HIP_OK(hipMemcpyAsync(&c, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        HIP_OK(hipStreamSynchronize(e.stream));
+        if (c <= e.capacity) {
+            count = c;
+            return true;
+        }
+        if (!ensure_photon_buffers(e, (uint32_t)std::min<double>((double)c * 1.25 + 1024, 4.0e9))) return false;
+    }
+    set_error("photon buffer overflow");
+    return false;
+}
+
+struct MapInfo {
+    PhotonMapView pm;
+};
+
+bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
+    long long N = light_paths(e);
+    int uid_bits = log2_ceil((uint64_t)N) + 3;
+    uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
+    int bucket_bits = log2_ceil(want);
+    if (bucket_bits + uid_bits > 64) {
+        set_error("photon map key does not fit 64 bits");
+        return false;
+    }
+    uint32_t buckets = 1u << bucket_bits;
+    if (!e.cells.alloc(buckets)) return false;
+    float maxc = 0.0f;
+    for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
+    float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
+    float cell = rq * (float)std::max(1.0, e.cell_scale);
+    MapBuildArgs a{};
+    a.in_pos = e.ph_pos.p;
+    a.in_dir = e.ph_dir.p;
+    a.in_tp = e.ph_tp.p;
+    a.in_uid = e.ph_uid.p;
+    a.count = count;
+    for (int k = 0; k < 3; ++k) a.origin[k] = e.bbox.lo[k] - cell;
+    a.inv_cell = 1.0f / cell;
+    a.mask = buckets - 1;
+    a.bucket_bits = bucket_bits;
+    a.uid_bits = uid_bits;
+    a.keys_a = e.keys_a.p;
+    a.keys_b = e.keys_b.p;
+    a.vals_a = e.vals_a.p;
+    a.vals_b = e.vals_b.p;
+    a.hist = e.hist.p;
+    a.hist_sums = e.hist_sums.p;
+    a.out_pos = e.s_pos.p;
+    a.out_dir = e.s_dir.p;
+    a.out_tp = e.s_tp.p;
+    a.out_index = e.s_index.p;
+    a.cells = e.cells.p;
+    build_photon_map(a, e.stream);
+    HIP_OK(hipGetLastError());
+    PhotonMapView& pm = mi.pm;
+    pm.pos = e.s_pos.p;
+    pm.dir_tr = e.s_dir.p;
+    pm.tp_gb = e.s_tp.p;
+    pm.cells = e.cells.p;
+    pm.mask = a.mask;
+    for (int k = 0; k < 3; ++k) pm.origin[k] = a.origin[k];
+    pm.inv_cell = a.inv_cell;
+    pm.r2 = r2;
+    pm.r_query = rq;
+    return true;
+}
+
+bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, float* image) {
+    launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+    HIP_OK(hipMemsetAsync(e.stats_dev.p, 0, sizeof(Counters), e.stream));
+    CameraPassArgs a{};
+    a.sc = scene_view(e);
+    a.pm = mi.pm;
+    a.k0 = (uint32_t)e.seed;
+    a.k1 = (uint32_t)(e.seed >> 32);
+    a.iteration = (uint32_t)iteration;
+    a.stream = STREAM_CAMERA;
+    a.n_paths = (uint32_t)((long long)e.W * e.H);
+    a.path_counter = e.counters.p;
+    a.vm_norm = norm;
+    a.image = image;
+    a.counters = e.stats_dev.p;
+    launch_camera_pass(a, blocks_for(e), e.stream);
+    HIP_OK(hipGetLastError());
+    return true;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+bool render_iteration(Engine& e, int iteration) {
+    float radius, r2, norm;
+    iteration_params(e, iteration, radius, r2, norm);
+    auto t0 = std::chrono::steady_clock::now();
+    uint32_t count = 0;
+    if (!run_light_pass(e, iteration, count)) return false;
+    double t_light = ms_since(t0);
+    auto t1 = std::chrono::steady_clock::now();
+    MapInfo mi;
+    if (!build_map(e, count, radius, r2, mi)) return false;
+    HIP_OK(hipStreamSynchronize(e.stream));
+    double t_map = ms_since(t1);
+    auto t2 = std::chrono::steady_clock::now();
+    if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
+    Counters c{};
+    HIP_OK(hipMemcpyAsync(&c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
+    HIP_OK(hipStreamSynchronize(e.stream));
+    double t_cam = ms_since(t2);
+    e.stats[RC_STAT_LIGHT_PATHS] = (double)light_paths(e);
+    e.stats[RC_STAT_PHOTONS] = count;
+    e.stats[RC_STAT_CAMERA_PATHS] = (double)e.W * e.H;
+    e.stats[RC_STAT_GATHERS] = (double)c.gathers;
+    e.stats[RC_STAT_PHOTONS_TESTED] = (double)c.photons_tested;
+    e.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)c.photons_accepted;
+    e.stats[RC_STAT_MS_LIGHT] = t_light;
+    e.stats[RC_STAT_MS_MAP] = t_map;
+    e.stats[RC_STAT_MS_CAMERA] = t_cam;
+    e.stats[RC_STAT_MS_TOTAL] = ms_since(t0);
+    e.stats[RC_STAT_RADIUS] = radius;
+    e.stats[RC_STAT_NODES] = e.n_nodes;
+    e.stats[RC_STAT_TRIS] = (double)(e.tris.size() / 4);
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// OpenEXR writer: single-part scanline, uncompressed, FLOAT channels B G R.
+// ---------------------------------------------------------------------
+bool write_exr(const char* path, const float* rgb, int W, int H) {
+    std::vector<unsigned char> out;
+    auto put = [&](const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        out.insert(out.end(), c, c + n);
+    };
+    auto put_i32 = [&](int32_t v) { put(&v, 4); };
+    auto put_u32 = [&](uint32_t v) { put(&v, 4); };
+    auto put_f32 = [&](float v) { put(&v, 4); };
+    auto put_str = [&](const char* s) { put(s, strlen(s) + 1); };
+    auto attr = [&](const char* name, const char* type, uint32_t size) {
+        put_str(name);
+        put_str(type);
+        put_u32(size);
+    };
+    put_u32(20000630u);  // magic
+    put_u32(2u);         // version 2, scanline
+    attr("channels", "chlist", 3 * (2 + 16) + 1);
+    for (const char* ch : {"B", "G", "R"}) {
+        put_str(ch);
+        put_i32(2);  // FLOAT
+        unsigned char z[4] = {0, 0, 0, 0};
+        put(z, 4);  // pLinear + reserved
+        put_i32(1);
+        put_i32(1);  // sampling
+    }
+    out.push_back(0);
+    attr("compression", "compression", 1);
+    out.push_back(0);  // NO_COMPRESSION
+    attr("dataWindow", "box2i", 16);
+    put_i32(0);
+    put_i32(0);
+    put_i32(W - 1);
+    put_i32(H - 1);
+    attr("displayWindow", "box2i", 16);
+    put_i32(0);
+    put_i32(0);
+    put_i32(W - 1);
+    put_i32(H - 1);
+    attr("lineOrder", "lineOrder", 1);
+    out.push_back(0);
+    attr("pixelAspectRatio", "float", 4);
+    put_f32(1.0f);
+    attr("screenWindowCenter", "v2f", 8);
+    put_f32(0.0f);
+    put_f32(0.0f);
+    attr("screenWindowWidth", "float", 4);
+    put_f32(1.0f);
+    out.push_back(0);  // end of header
+    size_t table = out.size();
+    out.resize(out.size() + 8 * (size_t)H);
+    uint32_t line_bytes = (uint32_t)(3 * 4 * W);
+    for (int y = 0; y < H; ++y) {
+        uint64_t off = out.size();
+        memcpy(&out[table + 8 * (size_t)y], &off, 8);
+        put_i32(y);
+        put_u32(line_bytes);
+        for (int ch : {2, 1, 0})
+            for (int x = 0; x < W; ++x) put_f32(rgb[3 * ((size_t)y * W + x) + ch]);
+    }
+    FILE* f = fopen(path, "wb");
+    if (!f) {
+        set_error(std::string("cannot open ") + path);
+        return false;
+    }
+    size_t w = fwrite(out.data(), 1, out.size(), f);
+    fclose(f);
+    if (w != out.size()) {
+        set_error(std::string("short write to ") + path);
+        return false;
+    }
+    return true;
+}
+
+void release(Engine*& e) {
+    if (!e) return;
+    DevBuf<float4>* f4[] = {&e->d_tri, &e->d_shade, &e->d_nodes, &e->ph_pos, &e->ph_dir, &e->s_pos, &e->s_dir};
+    for (auto* b : f4) b->release();
+    e->ph_tp.release();
+    e->s_tp.release();
+    DevBuf<uint32_t>* u[] = {&e->ph_uid, &e->s_index, &e->vals_a, &e->vals_b, &e->hist, &e->hist_sums, &e->counters};
+    for (auto* b : u) b->release();
+    e->keys_a.release();
+    e->keys_b.release();
+    e->cells.release();
+    e->image.release();
+    e->stats_dev.release();
+    e->d_tri_id.release();
+    e->d_mats.release();
+    e->d_lights.release();
+    e->d_cdf.release();
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+    e = nullptr;
+}
+
+bool get_image(Engine& e, float* rgb, bool scaled) {
+    size_t n = (size_t)e.W * e.H * 3;
+    HIP_OK(hipMemcpy(rgb, e.image.p, n * sizeof(float), hipMemcpyDeviceToHost));
+    if (scaled && e.iterations_done > 0) {
+        // Image_Buffer::scale (Image_Buffer.h:38-44)
+        float s = 1.0f / (float)e.iterations_done;
+        for (size_t i = 0; i < n; ++i) rgb[i] *= s;
+    }
+    return true;
+}
+
+}  // namespace
+
+// =====================================================================
+// exported C-ABI
+// =====================================================================
+extern "C" {
+
+static void init_impl(int samples, float base_radius, float radius_reduction,
+                                                 long long* meshes_pointers, unsigned int number_of_meshes,
+                                                 rc_matrix4x4* mesh_matrices, unsigned int* meshes_number_of_verts,
+                                                 unsigned int* meshes_number_of_tris, unsigned int camera_x,
+                                                 unsigned int camera_y, rc_vec3 camera_position, rc_vec3 camera_corner0,
+                                                 rc_vec3 camera_corner1, rc_vec3 camera_corner2, rc_vec3 camera_corner3,
+                                                 rc_python_light* lights, unsigned int number_of_lights,
+                                                 rc_python_material* materials, unsigned int number_of_materials,
+                                                 int** materials_indices, int debug_test, rc_debug_stuff debug_stuff) {
+    (void)debug_stuff;
+    // options survive a re-init; geometry does not
+    Engine* prev = g;
+    Engine* e = new Engine();
+    if (prev) {
+        e->seed = prev->seed;
+        e->photons = prev->photons;
+        e->write_exr = prev->write_exr;
+        e->compat_rows = prev->compat_rows;
+        e->compat_point = prev->compat_point;
+        e->compat_pdf = prev->compat_pdf;
+        e->max_path_length = prev->max_path_length;
+        e->photon_capacity = prev->photon_capacity;
+        e->cell_scale = prev->cell_scale;
+        e->persistent_blocks = prev->persistent_blocks;
+        e->output_path = prev->output_path;
+        release(prev);
+    }
+    g = e;
+    g_error.clear();
+    g_cancel = 0;
+    e->samples = samples;
+    e->base_radius = base_radius;
+    e->radius_alpha = radius_reduction;
+    e->debug_test = debug_test;
+    e->initialised = true;
+    if (samples < 1) set_error("samples must be >= 1");
+    if (camera_x == 0 || camera_y == 0) set_error("camera resolution must be positive");
+    if (number_of_meshes > 0 && (!meshes_pointers || !mesh_matrices || !meshes_number_of_verts ||
+                                 !meshes_number_of_tris || !materials_indices))
+        set_error("null mesh arrays");
+    if (number_of_materials > 0 && !materials) set_error("null material array");
+    if (number_of_lights > 0 && !lights) set_error("null light array");
+    if (!g_error.empty()) return;
+    for (unsigned i = 0; i < number_of_materials; ++i) e->mats.push_back(convert_material(materials[i]));
+    for (unsigned i = 0; i < number_of_lights; ++i) {
+        HostLight h;
+        memcpy(&h, &lights[i], sizeof h);
+        e->py_lights.push_back(h);
+    }
+    uint32_t vbase = 0;
+    for (unsigned i = 0; i < number_of_meshes; ++i) {
+        const unsigned char* mb = reinterpret_cast<const unsigned char*>((intptr_t)meshes_pointers[i]);
+        unsigned nv = meshes_number_of_verts[i], nt = meshes_number_of_tris[i];
+        e->mesh_tri_offset.push_back((uint32_t)(e->tris.size() / 4));
+        if (!mb) {
+            set_error("null Blender mesh pointer");
+            return;
+        }
+        const unsigned char *mpoly, *mloop, *mvert;
+        memcpy(&mpoly, mb + 200, sizeof(void*));
+        memcpy(&mloop, mb + 208, sizeof(void*));
+        memcpy(&mvert, mb + 256, sizeof(void*));
+        if (nt > 0 && (!mpoly || !mloop || !mvert)) {
+            set_error("Blender mesh without mpoly/mloop/mvert");
+            return;
+        }
+        if (!materials_indices[i]) {
+            set_error("null materials_indices entry");
+            return;
+        }
+        const float* M = mesh_matrices[i].t;
+        float inv[3][3];
+        inverse3x3_of4(M, inv);
+        for (unsigned v = 0; v < nv; ++v) {
+            float co[3];
+            int16_t no[3];
+            memcpy(co, mvert + 20 * (size_t)v, 12);
+            memcpy(no, mvert + 20 * (size_t)v + 12, 6);
+            if (!std::isfinite(co[0]) || !std::isfinite(co[1]) || !std::isfinite(co[2])) {
+                set_error("non-finite vertex coordinate");
+                return;
+            }
+            // Transform::ApplyToPoint (Transform.cpp:3-11)
+            float x = co[0], y = co[1], z = co[2];
+            e->wv.push_back(M[0] * x + M[1] * y + M[2] * z + M[3]);
+            e->wv.push_back(M[4] * x + M[5] * y + M[6] * z + M[7]);
+            e->wv.push_back(M[8] * x + M[9] * y + M[10] * z + M[11]);
+            // normal_short_to_float (Triangle.cpp:11-16), ApplyToNormal (Transform.cpp:23-31)
+            const float s = 1.0f / 32767.0f;
+            float nx = no[0] * s, ny = no[1] * s, nz = no[2] * s;
+            F3 n = f3(inv[0][0] * nx + inv[1][0] * ny + inv[2][0] * nz, inv[0][1] * nx + inv[1][1] * ny + inv[2][1] * nz,
+                      inv[0][2] * nx + inv[1][2] * ny + inv[2][2] * nz);
+            n = normalize(n);
+            e->wn.push_back(n.x);
+            e->wn.push_back(n.y);
+            e->wn.push_back(n.z);
+        }
+        for (unsigned t = 0; t < nt; ++t) {
+            int32_t loopstart, totloop;
+            int16_t mat_nr;
+            memcpy(&loopstart, mpoly + 12 * (size_t)t, 4);
+            memcpy(&totloop, mpoly + 12 * (size_t)t + 4, 4);
+            memcpy(&mat_nr, mpoly + 12 * (size_t)t + 8, 2);
+            if (totloop < 3 || loopstart < 0) {
+                set_error("polygon with fewer than 3 loops");
+                return;
+            }
+            uint32_t vid[3];
+            for (int k = 0; k < 3; ++k) {
+                memcpy(&vid[k], mloop + 8 * ((size_t)loopstart + k), 4);
+                if (vid[k] >= nv) {
+                    set_error("loop vertex index out of range");
+                    return;
+                }
+                e->tris.push_back(vbase + vid[k]);
+            }
+            int mat = materials_indices[i][mat_nr < 0 ? 0 : mat_nr];
+            if (mat < 0 || (unsigned)mat >= number_of_materials) {
+                set_error("material index out of range");
+                return;
+            }
+            e->tris.push_back((uint32_t)mat);
+        }
+        vbase += nv;
+    }
+    // Camera (Camera.h:21-27): corners top right, bottom right, bottom left, top left
+    e->W = (int)camera_x;
+    e->H = (int)camera_y;
+    e->cam_o = f3(camera_position.x, camera_position.y, camera_position.z);
+    F3 br = f3(camera_corner1.x, camera_corner1.y, camera_corner1.z);
+    F3 bl = f3(camera_corner2.x, camera_corner2.y, camera_corner2.z);
+    F3 tl = f3(camera_corner3.x, camera_corner3.y, camera_corner3.z);
+    (void)camera_corner0;
+    e->cam_bl = bl;
+    e->cam_h = br - bl;
+    e->cam_v = tl - bl;
+}
+
+__attribute__((visibility("default"))) void init(int samples, float base_radius, float radius_reduction,
+                                                 long long* meshes_pointers, unsigned int number_of_meshes,
+                                                 rc_matrix4x4* mesh_matrices, unsigned int* meshes_number_of_verts,
+                                                 unsigned int* meshes_number_of_tris, unsigned int camera_x,
+                                                 unsigned int camera_y, rc_vec3 camera_position, rc_vec3 camera_corner0,
+                                                 rc_vec3 camera_corner1, rc_vec3 camera_corner2, rc_vec3 camera_corner3,
+                                                 rc_python_light* lights, unsigned int number_of_lights,
+                                                 rc_python_material* materials, unsigned int number_of_materials,
+                                                 int** materials_indices, int debug_test, rc_debug_stuff debug_stuff) {
+    init_impl(samples, base_radius, radius_reduction, meshes_pointers, number_of_meshes, mesh_matrices,
+              meshes_number_of_verts, meshes_number_of_tris, camera_x, camera_y, camera_position, camera_corner0,
+              camera_corner1, camera_corner2, camera_corner3, lights, number_of_lights, materials, number_of_materials,
+              materials_indices, debug_test, debug_stuff);
+    // keep the first problem init found so that main() reports it
+    if (g && !g_error.empty()) g->init_error = g_error;
+}
+
+__attribute__((visibility("default"))) int main(void) {
+    g_error.clear();
+    if (!g || !g->initialised) {
+        set_error("main called before init");
+        return 1;
+    }
+    if (!g->init_error.empty()) {
+        set_error(g->init_error);
+        return 1;
+    }
+    if (!prepare(*g)) return 2;
+    Engine& e = *g;
+    if (e.debug_test == 1) {
+        for (int it = e.iterations_done; it < e.samples; ++it) {
+            if (g_cancel.load()) break;
+            if (!render_iteration(e, it)) return 3;
+            e.iterations_done = it + 1;
+        }
+    }
+    int rc = 0;
+    if (e.debug_test != 5 && e.write_exr) {
+        std::vector<float> img((size_t)e.W * e.H * 3, 0.0f);
+        if (e.debug_test == 1 && !get_image(e, img.data(), true)) rc = 4;
+        if (rc == 0 && !write_exr(e.output_path.c_str(), img.data(), e.W, e.H)) rc = 5;
+    }
+    // src/main.cpp:220-221: the scene is deleted after rendering; options survive
+    Engine* keep = new Engine();
+    keep->seed = e.seed;
+    keep->photons = e.photons;
+    keep->write_exr = e.write_exr;
+    keep->compat_rows = e.compat_rows;
+    keep->compat_point = e.compat_point;
+    keep->compat_pdf = e.compat_pdf;
+    keep->max_path_length = e.max_path_length;
+    keep->photon_capacity = e.photon_capacity;
+    keep->cell_scale = e.cell_scale;
+    keep->persistent_blocks = e.persistent_blocks;
+    keep->output_path = e.output_path;
+    release(g);
+    g = keep;
+    return rc;
+}
+
+__attribute__((visibility("default"))) int rc_set_option(const char* key, double v) {
+    if (!g) g = new Engine();
+    std::string k(key ? key : "");
+    if (k == "seed") g->seed = (uint64_t)v;
+    else if (k == "photons") g->photons = (long long)v;
+    else if (k == "write_exr") g->write_exr = (int)v;
+    else if (k == "compat_rows") g->compat_rows = (int)v;
+    else if (k == "compat_point_light") g->compat_point = (int)v;
+    else if (k == "compat_light_pdf") g->compat_pdf = (int)v;
+    else if (k == "max_path_length") g->max_path_length = std::max(0, std::min(7, (int)v));
+    else if (k == "photon_capacity") g->photon_capacity = v;
+    else if (k == "cell_scale") g->cell_scale = v;
+    else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
+    else return -1;
+    return 0;
+}
+
+__attribute__((visibility("default"))) int rc_set_option_str(const char* key, const char* value) {
+    if (!g) g = new Engine();
+    if (key && value && std::string(key) == "output_path") {
+        g->output_path = value;
+        return 0;
+    }
+    return -1;
+}
+
+__attribute__((visibility("default"))) double rc_get_option(const char* key) {
+    if (!g || !key) return NAN;
+    std::string k(key);
+    if (k == "seed") return (double)g->seed;
+    if (k == "photons") return (double)light_paths(*g);
+    if (k == "scene_radius") return g->scene_radius;
+    if (k == "base_radius") return g->base_radius;
+    if (k == "iterations") return g->iterations_done;
+    if (k == "max_path_length") return g->max_path_length;
+    if (k == "nodes") return g->n_nodes;
+    if (k == "device_cus") return g->device_cus;
+    if (k == "capacity") return g->capacity;
+    return NAN;
+}
+
+__attribute__((visibility("default"))) int rc_prepare(void) {
+    g_error.clear();
+    if (g && !g->init_error.empty()) {
+        set_error(g->init_error);
+        return -1;
+    }
+    if (!g) {
+        set_error("rc_prepare called before init");
+        return -1;
+    }
+    return prepare(*g) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_render_iteration(void) {
+    if (rc_prepare() != 0) return -1;
+    if (g_cancel.load()) {
+        set_error("cancelled");
+        return -2;
+    }
+    int it = g->iterations_done;
+    if (!render_iteration(*g, it)) return -1;
+    g->iterations_done = it + 1;
+    return it;
+}
+
+__attribute__((visibility("default"))) int rc_get_image(float* rgb, int scaled) {
+    if (!g || !g->prepared || !rgb) {
+        set_error("no rendered image");
+        return -1;
+    }
+    return get_image(*g, rgb, scaled != 0) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_write_exr(const char* path) {
+    if (!g || !g->prepared || !path) {
+        set_error("no rendered image");
+        return -1;
+    }
+    std::vector<float> img((size_t)g->W * g->H * 3);
+    if (!get_image(*g, img.data(), true)) return -1;
+    return write_exr(path, img.data(), g->W, g->H) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) void rc_release(void) {
+    release(g);
+    g_error.clear();
+}
+
+__attribute__((visibility("default"))) const char* rc_last_error(void) { return g_error.c_str(); }
+
+__attribute__((visibility("default"))) void rc_cancel(void) { g_cancel = 1; }
+
+__attribute__((visibility("default"))) int rc_stats(double* out, int n) {
+    if (!g || !out) return 0;
+    int m = std::min(n, (int)RC_STAT_COUNT);
+    for (int i = 0; i < m; ++i) out[i] = g->stats[i];
+    return m;
+}
+
+__attribute__((visibility("default"))) int rc_first_hit(const float* rays, long long n, float tmin, int* tri, float* t) {
+    if (!g || !prepare(*g)) return -1;
+    if (n <= 0) return 0;
+    DevBuf<float> dr, dt;
+    DevBuf<int> di;
+    if (!dr.alloc((size_t)n * 6) || !dt.alloc((size_t)n) || !di.alloc((size_t)n)) return -1;
+    bool ok = hipMemcpy(dr.p, rays, (size_t)n * 6 * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (ok) {
+        launch_first_hit(scene_view(*g), dr.p, (int)n, tmin, di.p, dt.p, g->stream);
+        ok = hipStreamSynchronize(g->stream) == hipSuccess &&
+             hipMemcpy(tri, di.p, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(t, dt.p, (size_t)n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    dr.release();
+    dt.release();
+    di.release();
+    if (!ok) set_error("rc_first_hit failed");
+    return ok ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) long long rc_trace_photons(int iteration, float* photons, unsigned int* uid,
+                                                                  long long cap) {
+    if (!g || !prepare(*g)) return -1;
+    uint32_t count = 0;
+    if (!run_light_pass(*g, iteration, count)) return -1;
+    std::vector<float4> pos(count), dir(count);
+    std::vector<float2> tp(count);
+    std::vector<uint32_t> u(count);
+    if (count) {
+        if (hipMemcpy(pos.data(), g->ph_pos.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(dir.data(), g->ph_dir.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(tp.data(), g->ph_tp.p, count * 8, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(u.data(), g->ph_uid.p, count * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("photon readback failed");
+            return -1;
+        }
+    }
+    // path order (the order of the reference's sequential light loop)
+    std::vector<uint32_t> ord(count);
+    for (uint32_t i = 0; i < count; ++i) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return u[a] < u[b]; });
+    long long m = std::min<long long>(cap, count);
+    for (long long k = 0; k < m; ++k) {
+        uint32_t i = ord[k];
+        float* r = photons + 10 * k;
+        r[0] = pos[i].x;
+        r[1] = pos[i].y;
+        r[2] = pos[i].z;
+        r[3] = dir[i].x;
+        r[4] = dir[i].y;
+        r[5] = dir[i].z;
+        r[6] = dir[i].w;
+        r[7] = tp[i].x;
+        r[8] = tp[i].y;
+        r[9] = pos[i].w;
+        if (uid) uid[k] = u[i];
+    }
+    return count;
+}
+
+// Loads a caller photon list (10 floats each) into the unsorted photon buffers.
+static bool load_photons(const float* photons, long long count) {
+    Engine& e = *g;
+    if (!ensure_photon_buffers(e, (uint32_t)std::max<long long>(count, std::max<uint32_t>(e.capacity, 1024)))) return false;
+    std::vector<float4> pos(count), dir(count);
+    std::vector<float2> tp(count);
+    std::vector<uint32_t> u(count);
+    for (long long i = 0; i < count; ++i) {
+        const float* r = photons + 10 * i;
+        pos[i] = make_float4(r[0], r[1], r[2], r[9]);
+        dir[i] = make_float4(r[3], r[4], r[5], r[6]);
+        tp[i] = make_float2(r[7], r[8]);
+        u[i] = (uint32_t)i;
+    }
+    if (count) {
+        HIP_OK(hipMemcpy(e.ph_pos.p, pos.data(), count * 16, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(e.ph_dir.p, dir.data(), count * 16, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(e.ph_tp.p, tp.data(), count * 8, hipMemcpyHostToDevice));
+        HIP_OK(hipMemcpy(e.ph_uid.p, u.data(), count * 4, hipMemcpyHostToDevice));
+    }
+    return true;
+}
+
+__attribute__((visibility("default"))) long long rc_gather_sets(const float* positions, long long count,
+                                                                const float* queries, long long nq, float r2,
+                                                                long long* offsets, int* indices, long long cap) {
+    if (!g || !prepare(*g)) return -1;
+    Engine& e = *g;
+    std::vector<float> ph((size_t)count * 10, 0.0f);
+    for (long long i = 0; i < count; ++i)
+        for (int k = 0; k < 3; ++k) ph[10 * i + k] = positions[3 * i + k];
+    if (!load_photons(ph.data(), count)) return -1;
+    // a photon map whose uid bits cover the input indices
+    long long saved = e.photons;
+    e.photons = std::max<long long>(1, (count + 7) / 8 + 1);
+    float radius = sqrtf(r2);
+    MapInfo mi;
+    bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
+    e.photons = saved;
+    if (!ok) return -1;
+    mi.pm.r2 = r2;
+    DevBuf<float> dq;
+    DevBuf<uint32_t> dc;
+    DevBuf<unsigned long long> doff;
+    DevBuf<int> dout;
+    if (!dq.alloc((size_t)nq * 3 + 1) || !dc.alloc((size_t)nq + 1) || !doff.alloc((size_t)nq + 1)) return -1;
+    long long total = -1;
+    do {
+        if (hipMemcpy(dq.p, queries, (size_t)nq * 12, hipMemcpyHostToDevice) != hipSuccess) break;
+        launch_gather_count(mi.pm, dq.p, (int)nq, e.s_index.p, dc.p, e.stream);
+        std::vector<uint32_t> cnt(nq);
+        if (hipStreamSynchronize(e.stream) != hipSuccess) break;
+        if (nq && hipMemcpy(cnt.data(), dc.p, (size_t)nq * 4, hipMemcpyDeviceToHost) != hipSuccess) break;
+        std::vector<unsigned long long> off(nq + 1, 0);
+        for (long long i = 0; i < nq; ++i) off[i + 1] = off[i] + cnt[i];
+        total = (long long)off[nq];
+        for (long long i = 0; i <= nq; ++i) offsets[i] = (long long)off[i];
+        if (total > cap) break;
+        if (hipMemcpy(doff.p, off.data(), (size_t)(nq + 1) * 8, hipMemcpyHostToDevice) != hipSuccess) {
+            total = -1;
+            break;
+        }
+        if (!dout.alloc((size_t)total + 1)) {
+            total = -1;
+            break;
+        }
+        launch_gather_fill(mi.pm, dq.p, (int)nq, e.s_index.p, doff.p, dout.p, e.stream);
+        if (hipStreamSynchronize(e.stream) != hipSuccess ||
+            (total && hipMemcpy(indices, dout.p, (size_t)total * 4, hipMemcpyDeviceToHost) != hipSuccess)) {
+            total = -1;
+            break;
+        }
+    } while (0);
+    dq.release();
+    dc.release();
+    doff.release();
+    dout.release();
+    if (total < 0) set_error("rc_gather_sets failed");
+    return total;
+}
+
+__attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration, const float* photons, long long count,
+                                                                 float* image) {
+    if (!g || !prepare(*g)) return -1;
+    Engine& e = *g;
+    if (!load_photons(photons, count)) return -1;
+    float radius, r2, norm;
+    iteration_params(e, iteration, radius, r2, norm);
+    // uids are the input indices; size the key for them
+    long long saved = e.photons;
+    long long n_paths = light_paths(e);
+    e.photons = std::max<long long>(n_paths, (count + 7) / 8 + 1);
+    MapInfo mi;
+    bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
+    e.photons = saved;
+    if (!ok) return -1;
+    size_t n = (size_t)e.W * e.H * 3;
+    DevBuf<float> img;
+    if (!img.alloc(n)) return -1;
+    ok = hipMemcpy(img.p, image, n * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         run_camera_pass(e, iteration, mi, norm, img.p) && hipStreamSynchronize(e.stream) == hipSuccess &&
+         hipMemcpy(image, img.p, n * 4, hipMemcpyDeviceToHost) == hipSuccess;
+    img.release();
+    if (!ok) set_error("rc_camera_pass_photons failed");
+    return ok ? 0 : -1;
+}
+
+}  // extern "C"

@@ -1,0 +1,140 @@
+// rc_engine.h -- device data layout of the vertex-merging engine and the
+// launchers the host side calls.  Layout in HBM:
+//   triangles   3 x float4 per triangle in BVH-leaf order: {v0.xyz, material},
+//               {v1-v0, 0}, {v2-v0, 0}  (48 B; the three world-space points of
+//               Triangle::hit, src/Triangle.cpp:20-88, pre-transformed once)
+//   shading     3 x float4 per triangle: vertex normals (world, normalised)
+//   nodes       4 x float4 per binary BVH node: both child boxes + child refs
+//               (64 B; a ref >= 0 is a node, < 0 is ~(first << 3 | count - 1))
+//   photons     SoA after the Morton/hash sort: float4 {pos, dVM},
+//               float4 {dir, throughput.r}, float2 {throughput.g, .b}
+//   cells       uint2 {start, end} per hash bucket
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rc {
+
+enum { BK_SPEC_REFL = 0, BK_FRESNEL_SPEC = 1, BK_FMG = 2, BK_FSMG = 3, BK_MRG = 4 };
+enum { EV_REFLECTION = 1, EV_TRANSMISSION = 2, EV_DIFFUSE = 4, EV_GLOSSY = 8, EV_SPECULAR = 16 };
+enum { L_AREA = 0, L_SUN = 1, L_POINT = 2 };
+
+constexpr int kLeafMax = 8;
+constexpr int kBlock = 128;      // threads per block of the tracing kernels
+constexpr int kLdsStack = 24;    // traversal stack entries held in LDS per lane
+constexpr int kMaxDepth = 64;    // total stack capacity (LDS + private overflow)
+
+struct DevMaterial {
+    int kind;
+    float R[3];
+    float a2, s, ior, cont;
+};
+
+struct DevLight {
+    int kind;
+    float power, radius, pad;
+    float color[3], bl[3], h[3], v[3], normal[3], pos[3];
+    float fx[3], fy[3], fz[3];
+};
+
+struct SceneView {
+    const float4* tri;
+    const float4* shade;
+    const int* tri_id;
+    const float4* nodes;
+    int root;
+    const DevMaterial* mats;
+    const DevLight* lights;
+    const float* cdf;
+    int nlights;
+    float integral;
+    float cam_o[3], cam_bl[3], cam_h[3], cam_v[3];
+    int W, H;
+    int max_path_length;
+    int compat_point, compat_pdf, compat_rows;
+};
+
+struct PhotonMapView {
+    const float4* pos;     // sorted {pos, dVM}
+    const float4* dir_tr;  // sorted {dir, tp.r}
+    const float2* tp_gb;   // sorted {tp.g, tp.b}
+    const uint2* cells;    // [start, end) per bucket
+    uint32_t mask;         // buckets - 1
+    float origin[3];
+    float inv_cell;
+    float r2;       // merge radius squared (strict d2 < r2)
+    float r_query;  // radius with margin used to pick the cells
+};
+
+struct Counters {
+    unsigned long long photons_tested, photons_accepted, gathers;
+};
+
+struct LightPassArgs {
+    SceneView sc;
+    uint32_t k0, k1, iteration, stream;
+    uint32_t n_paths;
+    uint32_t* path_counter;
+    uint32_t* photon_counter;
+    uint32_t capacity;
+    float4* out_pos;
+    float4* out_dir;
+    float2* out_tp;
+    uint32_t* out_uid;
+};
+
+struct CameraPassArgs {
+    SceneView sc;
+    PhotonMapView pm;
+    uint32_t k0, k1, iteration, stream;
+    uint32_t n_paths;
+    uint32_t* path_counter;
+    float vm_norm;
+    float* image;
+    Counters* counters;
+};
+
+// ---- launchers (rc_kernels.hip) ----
+void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s);
+void launch_camera_pass(const CameraPassArgs& a, int blocks, hipStream_t s);
+void launch_first_hit(const SceneView& sc, const float* rays, int n, float tmin, int* tri, float* t, hipStream_t s);
+
+// Photon map build: keys, sort, permute, cell ranges.
+struct MapBuildArgs {
+    const float4* in_pos;
+    const float4* in_dir;
+    const float2* in_tp;
+    const uint32_t* in_uid;
+    uint32_t count;
+    float origin[3];
+    float inv_cell;
+    uint32_t mask;
+    int bucket_bits, uid_bits;
+    // scratch
+    unsigned long long* keys_a;
+    unsigned long long* keys_b;
+    uint32_t* vals_a;
+    uint32_t* vals_b;
+    uint32_t* hist;       // 256 * max_blocks
+    uint32_t* hist_sums;  // scan partials
+    // outputs
+    float4* out_pos;
+    float4* out_dir;
+    float2* out_tp;
+    uint32_t* out_index;  // original index of each sorted photon
+    uint2* cells;         // mask + 1 entries
+};
+void build_photon_map(const MapBuildArgs& a, hipStream_t s);
+size_t photon_map_hist_entries(uint32_t max_count);
+size_t photon_map_sum_entries(uint32_t max_count);
+
+// Radius sets through the photon map (count then fill).
+void launch_gather_count(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig, uint32_t* counts,
+                         hipStream_t s);
+void launch_gather_fill(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig,
+                        const unsigned long long* offsets, int* out, hipStream_t s);
+
+void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+void launch_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
+
+}  // namespace rc

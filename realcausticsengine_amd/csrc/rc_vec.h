@@ -1,0 +1,182 @@
+// rc_vec.h -- float3 arithmetic with the evaluation order of the reference's
+// vec3 (Real Caustics/src/vec3.h), shared by host set-up code and gfx950
+// kernels.  Everything is compiled with -ffp-contract=off, so a*b+c stays two
+// rounded operations on both sides and the CPU oracle reproduces GPU bits.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define RC_HD __host__ __device__ __forceinline__
+
+namespace rc {
+
+struct F3 {
+    float x, y, z;
+};
+
+RC_HD F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+RC_HD F3 f3s(float a) { return F3{a, a, a}; }
+RC_HD F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+RC_HD F3 operator-(F3 a, F3 b) { return F3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+RC_HD F3 operator-(F3 a) { return F3{-a.x, -a.y, -a.z}; }
+RC_HD F3 operator*(F3 a, F3 b) { return F3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+RC_HD F3 operator*(F3 a, float t) { return F3{a.x * t, a.y * t, a.z * t}; }
+// vec3 / float multiplies by the reciprocal (vec3.h:154-157)
+RC_HD F3 div_recip(F3 a, float t) {
+    float s = 1.0f / t;
+    return a * s;
+}
+// vec3 /= float divides each component (vec3.h:79-85)
+RC_HD F3 div_each(F3 a, float t) { return F3{a.x / t, a.y / t, a.z / t}; }
+RC_HD float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RC_HD float absdot(F3 a, F3 b) { return fabsf(dot(a, b)); }
+RC_HD F3 cross(F3 u, F3 v) { return F3{u.y * v.z - u.z * v.y, u.z * v.x - u.x * v.z, u.x * v.y - u.y * v.x}; }
+RC_HD float len2(F3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+RC_HD F3 normalize(F3 a) { return div_recip(a, sqrtf(len2(a))); }
+RC_HD float smin(float a, float b) { return (b < a) ? b : a; }  // std::min
+RC_HD float smax(float a, float b) { return (a < b) ? b : a; }  // std::max
+RC_HD float maxc(F3 a) { return smax(smax(a.x, a.y), a.z); }
+RC_HD F3 reflect(F3 v, F3 n) { return v - n * (2.0f * dot(v, n)); }
+RC_HD bool refract(F3 wi, F3 n, float eta, F3* wo) {
+    float cos_i = dot(n, wi);
+    float sin2_i = smax(0.0f, 1.0f - cos_i * cos_i);
+    float sin2_t = eta * eta * sin2_i;
+    if (sin2_t >= 1.0f) return false;
+    float cos_t = sqrtf(1.0f - sin2_t);
+    *wo = (-wi) * eta + n * (eta * cos_i - cos_t);
+    return true;
+}
+
+// Orthonormal-ish frame from an (unnormalised) z (Frame.h:12-29).
+struct Frame {
+    F3 x, y, z;
+};
+RC_HD Frame frame_from_z(F3 z) {
+    Frame f;
+    f.z = z;
+    F3 tx = (fabsf(z.x) > 0.99f) ? f3(0.0f, 1.0f, 0.0f) : f3(1.0f, 0.0f, 0.0f);
+    f.y = normalize(cross(z, tx));
+    f.x = cross(f.y, z);
+    return f;
+}
+RC_HD F3 to_world(const Frame& f, F3 a) { return (f.x * a.x + f.y * a.y) + f.z * a.z; }
+RC_HD F3 to_local(const Frame& f, F3 a) { return F3{dot(a, f.x), dot(a, f.y), dot(a, f.z)}; }
+
+// ---------------------------------------------------------------------
+// The engine's own sin / cos / acos: Taylor series of the functions,
+// reduced to |r| <= pi/4 by a two-part pi/2 (k * 1.5703125 is exact).
+// The same float operations run on the host oracle, so both agree bitwise.
+// ---------------------------------------------------------------------
+RC_HD float own_sincos(float x, int cosine) {
+    const float two_over_pi = 0.6366197466850281f;
+    const float c1 = 1.5703125f;
+    const float c2 = 0.0004838267923332751f;
+    float fk = x * two_over_pi;
+    int k = (int)(fk + (fk >= 0.0f ? 0.5f : -0.5f));
+    float kf = (float)k;
+    float r = (x - kf * c1) - kf * c2;
+    float r2 = r * r;
+    float ps = -0.00019841270113829523f + r2 * 2.7557318844628753e-06f;
+    ps = 0.008333333767950535f + r2 * ps;
+    ps = -0.1666666716337204f + r2 * ps;
+    float s = r + (r * r2) * ps;
+    float pc = 2.4801587642286904e-05f + r2 * -2.755731998149713e-07f;
+    pc = -0.0013888889225199819f + r2 * pc;
+    pc = 0.0416666679084301f + r2 * pc;
+    pc = -0.5f + r2 * pc;
+    float c = 1.0f + r2 * pc;
+    int q = (k + cosine) & 3;
+    return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+}
+RC_HD float own_sin(float x) { return own_sincos(x, 0); }
+RC_HD float own_cos(float x) { return own_sincos(x, 1); }
+RC_HD float own_asin_poly(float y) {
+    float y2 = y * y;
+    float p = 0.008390335366129875f;
+    p = 0.009761609137058258f + y2 * p;
+    p = 0.01155180111527443f + y2 * p;
+    p = 0.013964843936264515f + y2 * p;
+    p = 0.017352763563394547f + y2 * p;
+    p = 0.022372158244252205f + y2 * p;
+    p = 0.0303819440305233f + y2 * p;
+    p = 0.0446428582072258f + y2 * p;
+    p = 0.07500000298023224f + y2 * p;
+    p = 0.1666666716337204f + y2 * p;
+    return y + (y * y2) * p;
+}
+RC_HD float own_acos(float x) {
+    const float pio2 = 1.5707963705062866f, pi = 3.1415927410125732f;
+    float ax = fabsf(x);
+    if (ax <= 0.5f) {
+        float a = own_asin_poly(ax);
+        return x >= 0.0f ? pio2 - a : pio2 + a;
+    }
+    float z = (1.0f - ax) * 0.5f;
+    float a = 2.0f * own_asin_poly(sqrtf(z));
+    return x >= 0.0f ? a : pi - a;
+}
+
+// ---------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) and the sample space.
+// Dimension d of path p, iteration i, stream s = word (d & 3) of
+// Philox(ctr = {d >> 2, p, i, s}, key = seed), mapped to [0,1) with 24 bits.
+// ---------------------------------------------------------------------
+struct U4 {
+    uint32_t a, b, c, d;
+};
+RC_HD U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0;
+        c1 = n1;
+        c2 = n2;
+        c3 = n3;
+    }
+    return U4{c0, c1, c2, c3};
+}
+RC_HD float u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
+RC_HD uint32_t pick_word(U4 w, int i) { return i == 0 ? w.a : (i == 1 ? w.b : (i == 2 ? w.c : w.d)); }
+
+enum { DIM_RR = 32, DIM_JITTER = 40 };
+enum { STREAM_LIGHT = 0, STREAM_CAMERA = 1 };
+
+struct PathRng {
+    uint32_t k0, k1, path, iter, stream;
+    RC_HD float u(int d) const {
+        U4 w = philox((uint32_t)d >> 2, path, iter, stream, k0, k1);
+        return u24(pick_word(w, d & 3));
+    }
+    // three consecutive dimensions d, d+1, d+2 with at most two Philox blocks
+    RC_HD void u3(int d, float* o) const {
+        U4 w = philox((uint32_t)d >> 2, path, iter, stream, k0, k1);
+        int j = d & 3;
+        if (j <= 1) {
+            o[0] = u24(pick_word(w, j));
+            o[1] = u24(pick_word(w, j + 1));
+            o[2] = u24(pick_word(w, j + 2));
+        } else {
+            U4 w2 = philox(((uint32_t)d >> 2) + 1u, path, iter, stream, k0, k1);
+            o[0] = u24(pick_word(w, j));
+            o[1] = j == 2 ? u24(w.d) : u24(w2.a);
+            o[2] = j == 2 ? u24(w2.a) : u24(w2.b);
+        }
+    }
+};
+
+// Spatial hash of an integer photon-map cell.
+RC_HD uint32_t cell_hash(int x, int y, int z) {
+    return ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+}
+
+}  // namespace rc

@@ -1,0 +1,194 @@
+"""ctypes binding of libRealCaustics.so, shaped like the Blender plugin's calls.
+
+The plugin loads ``Real Caustics.dll`` and calls ``init(...)`` then ``main()``
+(``src/DLL.h:14-15``).  ``Engine`` does the same against the gfx950 library and
+adds the non-breaking ``rc_*`` entry points.  There is no fallback: if the
+library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from .blender import DebugStuff, PythonLight, PythonMaterial, PythonMatrix4x4, SceneArgs, Vec3
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libRealCaustics.so")
+
+# Every symbol include/real_caustics.h declares.
+EXPORTS = [
+    "init", "main", "rc_set_option", "rc_set_option_str", "rc_get_option", "rc_prepare", "rc_render_iteration",
+    "rc_get_image", "rc_write_exr", "rc_release", "rc_last_error", "rc_cancel", "rc_stats", "rc_first_hit",
+    "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
+]
+STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
+              "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris"]
+
+_lib = None
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise EngineError(f"{path} is not built; run realcausticsengine_amd/build.py (or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    P = ctypes.POINTER
+    lib.init.restype = None
+    lib.init.argtypes = [ctypes.c_int, ctypes.c_float, ctypes.c_float, P(ctypes.c_longlong), ctypes.c_uint,
+                         P(PythonMatrix4x4), P(ctypes.c_uint), P(ctypes.c_uint), ctypes.c_uint, ctypes.c_uint,
+                         Vec3, Vec3, Vec3, Vec3, Vec3, P(PythonLight), ctypes.c_uint, P(PythonMaterial),
+                         ctypes.c_uint, P(P(ctypes.c_int)), ctypes.c_int, DebugStuff]
+    lib.main.restype = ctypes.c_int
+    lib.main.argtypes = []
+    lib.rc_set_option.argtypes = [ctypes.c_char_p, ctypes.c_double]
+    lib.rc_set_option.restype = ctypes.c_int
+    lib.rc_set_option_str.argtypes = [ctypes.c_char_p, ctypes.c_char_p]
+    lib.rc_set_option_str.restype = ctypes.c_int
+    lib.rc_get_option.argtypes = [ctypes.c_char_p]
+    lib.rc_get_option.restype = ctypes.c_double
+    lib.rc_prepare.restype = ctypes.c_int
+    lib.rc_render_iteration.restype = ctypes.c_int
+    lib.rc_get_image.argtypes = [P(ctypes.c_float), ctypes.c_int]
+    lib.rc_get_image.restype = ctypes.c_int
+    lib.rc_write_exr.argtypes = [ctypes.c_char_p]
+    lib.rc_write_exr.restype = ctypes.c_int
+    lib.rc_release.restype = None
+    lib.rc_last_error.restype = ctypes.c_char_p
+    lib.rc_cancel.restype = None
+    lib.rc_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
+    lib.rc_stats.restype = ctypes.c_int
+    lib.rc_first_hit.argtypes = [P(ctypes.c_float), ctypes.c_longlong, ctypes.c_float, P(ctypes.c_int),
+                                 P(ctypes.c_float)]
+    lib.rc_first_hit.restype = ctypes.c_int
+    lib.rc_trace_photons.argtypes = [ctypes.c_int, P(ctypes.c_float), P(ctypes.c_uint), ctypes.c_longlong]
+    lib.rc_trace_photons.restype = ctypes.c_longlong
+    lib.rc_gather_sets.argtypes = [P(ctypes.c_float), ctypes.c_longlong, P(ctypes.c_float), ctypes.c_longlong,
+                                   ctypes.c_float, P(ctypes.c_longlong), P(ctypes.c_int), ctypes.c_longlong]
+    lib.rc_gather_sets.restype = ctypes.c_longlong
+    lib.rc_camera_pass_photons.argtypes = [ctypes.c_int, P(ctypes.c_float), ctypes.c_longlong, P(ctypes.c_float)]
+    lib.rc_camera_pass_photons.restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Engine:
+    """One scene in libRealCaustics (the library holds a single global scene)."""
+
+    def __init__(self, args: SceneArgs, **options):
+        self.lib = load()
+        self.args = args
+        for k, v in options.items():
+            self.set(k, v)
+        self.lib.init(*args.init_args())
+        self._check(0)
+
+    # -- errors ---------------------------------------------------------
+    def error(self) -> str:
+        return self.lib.rc_last_error().decode()
+
+    def _check(self, rc, what="call"):
+        err = self.error()
+        if (isinstance(rc, int) and rc < 0) or err:
+            raise EngineError(f"{what}: {err or rc}")
+        return rc
+
+    # -- options --------------------------------------------------------
+    def set(self, key: str, value) -> None:
+        if isinstance(value, str):
+            r = self.lib.rc_set_option_str(key.encode(), value.encode())
+        else:
+            r = self.lib.rc_set_option(key.encode(), float(value))
+        if r != 0:
+            raise KeyError(key)
+
+    def get(self, key: str) -> float:
+        return self.lib.rc_get_option(key.encode())
+
+    # -- rendering ------------------------------------------------------
+    def prepare(self):
+        self._check(self.lib.rc_prepare(), "rc_prepare")
+        return self
+
+    def render_iteration(self) -> int:
+        return self._check(self.lib.rc_render_iteration(), "rc_render_iteration")
+
+    def image(self, scaled: bool = True) -> np.ndarray:
+        W, H = self.args.camera.width, self.args.camera.height
+        out = np.zeros(W * H * 3, dtype=np.float32)
+        self._check(self.lib.rc_get_image(_fp(out), int(scaled)), "rc_get_image")
+        return out.reshape(H, W, 3)
+
+    def main(self) -> int:
+        rc = self.lib.main()
+        if rc != 0:
+            raise EngineError(f"main returned {rc}: {self.error()}")
+        return rc
+
+    def stats(self) -> dict:
+        buf = (ctypes.c_double * len(STAT_NAMES))()
+        n = self.lib.rc_stats(buf, len(STAT_NAMES))
+        return {STAT_NAMES[i]: buf[i] for i in range(n)}
+
+    # -- diagnostics ------------------------------------------------------
+    def first_hit(self, rays: np.ndarray, tmin: float = 0.001):
+        r = np.ascontiguousarray(rays, dtype=np.float32)
+        n = len(r)
+        tri = np.zeros(n, dtype=np.int32)
+        t = np.zeros(n, dtype=np.float32)
+        self._check(self.lib.rc_first_hit(_fp(r), n, tmin, tri.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _fp(t)),
+                    "rc_first_hit")
+        return tri, t
+
+    def trace_photons(self, iteration: int, cap: int = 1 << 22):
+        buf = np.zeros((cap, 10), dtype=np.float32)
+        uid = np.zeros(cap, dtype=np.uint32)
+        n = self._check(self.lib.rc_trace_photons(iteration, _fp(buf), uid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                                                  cap), "rc_trace_photons")
+        if n > cap:
+            return self.trace_photons(iteration, int(n))
+        return buf[:n].copy(), uid[:n].copy()
+
+    def gather_sets(self, positions: np.ndarray, queries: np.ndarray, r2: float):
+        p = np.ascontiguousarray(positions, dtype=np.float32)
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        off = np.zeros(len(q) + 1, dtype=np.int64)
+        cap = max(1, 4 * len(q))
+        while True:
+            idx = np.zeros(cap, dtype=np.int32)
+            n = self._check(self.lib.rc_gather_sets(_fp(p), len(p), _fp(q), len(q), float(r2),
+                                                    off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                                    idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), cap),
+                            "rc_gather_sets")
+            if n <= cap:
+                return off, idx[:n].copy()
+            cap = int(n)
+
+    def camera_pass(self, iteration: int, photons: np.ndarray) -> np.ndarray:
+        W, H = self.args.camera.width, self.args.camera.height
+        img = np.zeros(W * H * 3, dtype=np.float32)
+        ph = np.ascontiguousarray(photons, dtype=np.float32)
+        self._check(self.lib.rc_camera_pass_photons(iteration, _fp(ph), len(ph), _fp(img)), "rc_camera_pass_photons")
+        return img.reshape(H, W, 3)
+
+    def release(self):
+        self.lib.rc_release()
+
+
+def photons_from_oracle(records: np.ndarray) -> np.ndarray:
+    """Oracle photon records (11 doubles) -> engine layout (10 floats)."""
+    out = np.zeros((len(records), 10), dtype=np.float32)
+    out[:, :9] = records[:, :9]
+    out[:, 9] = records[:, 9]
+    return out

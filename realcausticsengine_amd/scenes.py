@@ -1,0 +1,185 @@
+"""Seeded synthetic scenes for the configurations the engine is measured on.
+
+* ``c0_scene``: glass icosphere (subdivision 4, 5120 triangles, radius 1) above a
+  10 x 10 diffuse plane; light at height 5 (a small area light in the parity
+  variant, since the reference point light emits nothing: ``src/Lights.h:78-81``);
+  256 x 256 pinhole camera.
+* ``water_scene``: 512 x 512-vertex heightfield (522,242 triangles) made of 32
+  seeded sine waves, IOR 1.33, over a diffuse floor 0.5 below, 2 x 2 area light,
+  1024 x 1024 camera.
+* small scenes used by the tests (``slab_scene``, ``two_sphere_scene``).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .blender import CameraData, LightData, MaterialData, MeshData, SceneArgs, look_at_camera
+
+
+def icosphere(subdiv: int):
+    """Unit icosphere: 20 * 4**subdiv triangles, normals equal to positions."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.asarray(p, dtype=np.float64) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5),
+             (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                p = verts[a] + verts[b]
+                verts.append(p / np.linalg.norm(p))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    V = np.asarray(verts, dtype=np.float64)
+    return V.astype(np.float32), V, np.asarray(faces, dtype=np.uint32)
+
+
+def plane(size_x: float, size_y: float, z: float = 0.0, cx: float = 0.0, cy: float = 0.0):
+    hx, hy = size_x / 2.0, size_y / 2.0
+    verts = np.asarray([(cx - hx, cy - hy, z), (cx + hx, cy - hy, z), (cx + hx, cy + hy, z), (cx - hx, cy + hy, z)],
+                       dtype=np.float32)
+    normals = np.tile(np.asarray([0.0, 0.0, 1.0]), (4, 1))
+    tris = np.asarray([(0, 1, 2), (0, 2, 3)], dtype=np.uint32)
+    return verts, normals, tris
+
+
+def box(lo, hi):
+    """Closed axis-aligned box with flat per-face vertices (24 verts, 12 tris)."""
+    lo = np.asarray(lo, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    verts, normals, tris = [], [], []
+    for axis in range(3):
+        for side in (0, 1):
+            n = np.zeros(3)
+            n[axis] = 1.0 if side else -1.0
+            u, w = [a for a in range(3) if a != axis]
+            quad = []
+            for a, b in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                p = np.where(np.arange(3) == axis, hi if side else lo, 0.0).astype(np.float64)
+                p[u] = hi[u] if a else lo[u]
+                p[w] = hi[w] if b else lo[w]
+                quad.append(p)
+            base = len(verts)
+            verts += quad
+            normals += [n] * 4
+            if side:
+                tris += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
+            else:
+                tris += [(base, base + 2, base + 1), (base, base + 3, base + 2)]
+    return (np.asarray(verts, dtype=np.float32), np.asarray(normals), np.asarray(tris, dtype=np.uint32))
+
+
+def heightfield(n: int = 512, waves: int = 32, seed: int = 1, total_amp: float = 0.05, extent: float = 1.0):
+    """Sum of seeded sine waves on [-extent, extent]^2 with analytic normals."""
+    rng = np.random.default_rng(seed)
+    lam = rng.uniform(0.05, 1.0, waves)
+    amp = lam / lam.sum() * total_amp
+    theta = rng.uniform(0.0, 2 * np.pi, waves)
+    phase = rng.uniform(0.0, 2 * np.pi, waves)
+    xs = np.linspace(-extent, extent, n)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    Z = np.zeros_like(X)
+    dZx = np.zeros_like(X)
+    dZy = np.zeros_like(X)
+    for k in range(waves):
+        kx = 2 * np.pi / lam[k] * np.cos(theta[k])
+        ky = 2 * np.pi / lam[k] * np.sin(theta[k])
+        arg = kx * X + ky * Y + phase[k]
+        Z += amp[k] * np.sin(arg)
+        c = amp[k] * np.cos(arg)
+        dZx += c * kx
+        dZy += c * ky
+    verts = np.stack([X, Y, Z], axis=-1).reshape(-1, 3).astype(np.float32)
+    nrm = np.stack([-dZx, -dZy, np.ones_like(Z)], axis=-1).reshape(-1, 3)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    i = np.arange(n - 1)
+    I, J = np.meshgrid(i, i, indexing="xy")
+    a = (J * n + I).reshape(-1)
+    b = a + 1
+    c = a + n
+    d = c + 1
+    tris = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.uint32)
+    return verts, nrm, tris
+
+
+def translation(x, y, z):
+    m = np.eye(4, dtype=np.float32)
+    m[:3, 3] = (x, y, z)
+    return m
+
+
+def c0_scene(width: int = 256, height: int = 256, subdiv: int = 4, parity_light: bool = True,
+             samples: int = 16, base_radius: float = 0.01, alpha: float = 0.75) -> SceneArgs:
+    sv, sn, st = icosphere(subdiv)
+    pv, pn, pt = plane(10.0, 10.0, 0.0)
+    meshes = [MeshData(sv, sn, st, matrix=translation(0.0, 0.0, 1.25), material=0),
+              MeshData(pv, pn, pt, material=1)]
+    mats = [MaterialData("G", ior=1.5, roughness=0.0, color=(1.0, 1.0, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.8, 0.8))]
+    if parity_light:
+        lights = [LightData("A", position=(0.0, 0.0, 5.0), width=0.05, height=0.05, power=1.0)]
+    else:
+        lights = [LightData("P", position=(0.0, 0.0, 5.0), power=1.0)]
+    cam = look_at_camera(width, height, (0.0, -6.0, 4.0), (0.0, 0.0, 0.5), 40.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=base_radius, radius_reduction=alpha)
+
+
+def water_scene(n: int = 512, width: int = 1024, height: int = 1024, roughness: float = 0.0, samples: int = 64,
+                base_radius: float = 0.01, alpha: float = 0.75, seed: int = 1) -> SceneArgs:
+    hv, hn, ht = heightfield(n, 32, seed)
+    fv, fn, ft = plane(4.0, 4.0, -0.5)
+    meshes = [MeshData(hv, hn, ht, material=0), MeshData(fv, fn, ft, material=1)]
+    mats = [MaterialData("G", ior=1.33, roughness=roughness, color=(1.0, 1.0, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.8, 0.8))]
+    lights = [LightData("A", position=(0.0, 0.0, 2.0), width=2.0, height=2.0, power=1.0)]
+    cam = look_at_camera(width, height, (0.0, -1.6, 1.6), (0.0, 0.0, 0.0), 40.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=base_radius, radius_reduction=alpha)
+
+
+def slab_scene(width: int = 64, height: int = 64, albedo: float = 0.5, light_size: float = 1.0,
+               light_height: float = 5.0, samples: int = 1) -> SceneArgs:
+    """Glass slab over a diffuse plane, camera between them looking straight down.
+
+    Used for the closed-form photon-density and camera-estimate checks."""
+    gv, gn, gt = box((-3.0, -3.0, 1.0), (3.0, 3.0, 1.2))
+    pv, pn, pt = plane(10.0, 10.0, 0.0)
+    meshes = [MeshData(gv, gn, gt, material=0), MeshData(pv, pn, pt, material=1)]
+    mats = [MaterialData("G", ior=1.5, color=(1.0, 1.0, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(albedo, albedo, albedo))]
+    lights = [LightData("A", position=(0.0, 0.0, light_height), width=light_size, height=light_size, power=1.0)]
+    cam = look_at_camera(width, height, (0.0, 0.0, 0.5), (0.0, 0.0, 0.0), 30.0, up=(0.0, 1.0, 0.0))
+    return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=0.01, radius_reduction=0.75)
+
+
+def random_mesh_scene(seed: int = 0, n_spheres: int = 3, subdiv: int = 2, width: int = 32, height: int = 32):
+    """Several separated spheres of every material kind over a plane (for traversal tests)."""
+    rng = np.random.default_rng(seed)
+    meshes = []
+    mats = [MaterialData("G", ior=1.45, color=(0.9, 0.95, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.7, 0.6)),
+            MaterialData("M", roughness=0.0, color=(0.9, 0.9, 0.9)),
+            MaterialData("M", roughness=0.4, color=(0.9, 0.8, 0.7)),
+            MaterialData("P", roughness=0.05, specular=0.5, color=(0.5, 0.6, 0.7)),
+            MaterialData("P", roughness=0.5, specular=0.5, color=(0.3, 0.6, 0.3))]
+    for k in range(n_spheres):
+        sv, sn, st = icosphere(subdiv)
+        m = np.eye(4, dtype=np.float32) * 0.6
+        m[3, 3] = 1.0
+        m[:3, 3] = (3.0 * (k - (n_spheres - 1) / 2.0), rng.uniform(-0.5, 0.5), 0.8)
+        meshes.append(MeshData(sv, sn, st, matrix=m, material=int(k % len(mats))))
+    pv, pn, pt = plane(12.0, 12.0, 0.0)
+    meshes.append(MeshData(pv, pn, pt, material=1))
+    lights = [LightData("A", position=(0.0, 0.0, 6.0), width=2.0, height=1.0, power=2.0, rotation=(10.0, 0.0, 20.0))]
+    cam = look_at_camera(width, height, (0.0, -8.0, 5.0), (0.0, 0.0, 0.5), 40.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=1)
