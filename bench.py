@@ -1,0 +1,164 @@
+"""Benchmark of the vertex-merging hot path on one MI355X.
+
+Workload (water caustic): 512 x 512-vertex sine-wave heightfield (522,242
+triangles, IOR 1.33) over a diffuse floor, 2 x 2 area light, 1024 x 1024
+camera, 16,777,216 light paths per iteration, plain estimator, fp32.
+A step is one iteration: light pass, photon-map build, camera pass + merge.
+The scene is synthetic (seeded); nothing is loaded from disk.
+
+Prints one JSON line with photon paths/s (whole iteration), the roofline of
+the dominant kernel (algorithmic bytes from a separate counting launch over
+its HIP-event duration) and the CPU oracle timed on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Vertices.h:17-27)
+
+
+def algorithmic_bytes(st: dict) -> dict:
+    """Bytes per launch from the event counts (one counting iteration).
+
+    traversal: 32 B per box test, two boxes per node visit; 36 B per triangle
+    test (three points); 40 B per closest hit (three normals + material);
+    photon store 40 B; gather 8 B per cell visit, 12 B per photon tested,
+    28 B per photon accepted; film 24 B per camera path; photon map: per radix
+    pass 32 B per photon (key read twice, key+value read and written), plus
+    the 88 B permute and 28 B key generation per photon."""
+    light = (64 * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
+             + PHOTON_BYTES * st["photons"])
+    camera = (64 * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
+              + 8 * st["cell_visits"] + 12 * st["photons_tested"] + 28 * st["photons_accepted"]
+              + 24 * st["camera_paths"])
+    photon_map = st["photons"] * (32 * st["sort_passes"] + 88 + 28)
+    return {"light": light, "map": photon_map, "camera": camera}
+
+
+def cpu_baseline(args, photons: int, stride: int) -> dict:
+    """The CPU oracle (reference-mode fp32 restatement, one core) on one
+    iteration of the same scene: the full light pass, its photon map, and the
+    camera pass over every `stride`-th pixel, extrapolated to all pixels."""
+    from oracle import binding
+
+    o = binding.OracleScene(args, "f32", photons=photons, seed=0, camera_stride=stride)
+    o.prepare()
+    t0 = time.perf_counter()
+    ph = o.trace_light(0)
+    t_light = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    o.camera_pass(0, ph)
+    t_cam = time.perf_counter() - t1
+    total = t_light + stride * t_cam
+    return {
+        "value": photons / total,
+        "unit": "photon paths/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle f32, 1 iteration: {photons} light paths ({t_light:.1f} s), KD photon map + camera "
+                   f"pass over 1/{stride} of the pixels ({t_cam:.1f} s, x{stride} extrapolated)"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--photons", type=int, default=16_777_216)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--res", type=int, default=1024)
+    ap.add_argument("--cpu-stride", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one MI355X: the path has no multi-GPU mode in this build "
+              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        return 2
+
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import Engine
+
+    args = scenes.water_scene(a.grid, a.res, a.res, roughness=0.0, samples=a.warmup + a.steps + 1)
+    eng = Engine(args, photons=a.photons, seed=0, write_exr=0)
+    t = time.perf_counter()
+    eng.prepare()
+    prep_s = time.perf_counter() - t
+    print(f"[bench] scene ready: {args.total_tris} triangles, BVH {eng.get('nodes'):.0f} nodes, {prep_s:.2f} s",
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        eng.render_iteration()
+    kern = {"light": 0.0, "map": 0.0, "camera": 0.0}
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        eng.render_iteration()
+        st = eng.stats()
+        kern["light"] += st["ms_light"]
+        kern["map"] += st["ms_map"]
+        kern["camera"] += st["ms_camera"]
+    elapsed = time.perf_counter() - t0
+    ms_step = elapsed * 1000.0 / a.steps
+    kern = {k: v / a.steps for k, v in kern.items()}
+    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernels {kern}", file=sys.stderr, flush=True)
+    # counting launch (not timed): event counts give the algorithmic bytes
+    eng.set("count_events", 1)
+    eng.render_iteration()
+    cst = eng.stats()
+    eng.set("count_events", 0)
+    by = algorithmic_bytes(cst)
+    dom = max(kern, key=kern.get)
+    achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+    extra = {
+        "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
+        "algorithmic_bytes": {k: int(v) for k, v in by.items()},
+        "photons_per_iteration": int(cst["photons"]),
+        "gathers_per_s": cst["gathers"] / (ms_step * 1e-3),
+        "photons_tested_per_gather": cst["photons_tested"] / max(1.0, cst["gathers"]),
+        "counts": {k: int(cst[k]) for k in ("light_node_visits", "light_tri_tests", "light_hits",
+                                             "camera_node_visits", "camera_tri_tests", "camera_hits",
+                                             "gathers", "cell_visits", "photons_tested", "photons_accepted")},
+    }
+    eng.release()
+    cpu = None
+    if not a.no_cpu_baseline:
+        print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args, a.photons, a.cpu_stride)
+    value = a.photons / (ms_step * 1e-3)
+    line = {
+        "metric": "time-to-RMSE<=0.01 vs FP64 (guided vs plain) + photon paths/s, 1 MI355X",
+        "value": round(value, 1),
+        "unit": "photon paths/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "water caustic: 512x512-vertex sine heightfield (522,242 tris, IOR 1.33) over "
+                               "diffuse floor, 2x2 area light, 1024x1024, 16,777,216 light paths/iteration",
+                   "estimator": "plain vertex merging", "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "detail": extra,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

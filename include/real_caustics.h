@@ -67,7 +67,7 @@ int main(void);
 /* Options.  Numeric keys: "seed", "photons" (light paths per iteration,
  * 0 = width*height as the reference), "write_exr", "compat_rows",
  * "compat_point_light", "compat_light_pdf", "max_path_length",
- * "photon_capacity", "cell_scale", "persistent_blocks", "block_size".
+ * "photon_capacity", "cell_scale", "persistent_blocks", "count_events".
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
 int rc_set_option_str(const char *key, const char *value);
@@ -95,6 +95,10 @@ enum {
     RC_STAT_LIGHT_PATHS = 0, RC_STAT_PHOTONS, RC_STAT_CAMERA_PATHS, RC_STAT_GATHERS,
     RC_STAT_PHOTONS_TESTED, RC_STAT_PHOTONS_ACCEPTED, RC_STAT_MS_LIGHT, RC_STAT_MS_MAP,
     RC_STAT_MS_CAMERA, RC_STAT_MS_TOTAL, RC_STAT_RADIUS, RC_STAT_NODES, RC_STAT_TRIS,
+    /* event counts, filled only when the option "count_events" is 1 */
+    RC_STAT_LIGHT_NODE_VISITS, RC_STAT_LIGHT_TRI_TESTS, RC_STAT_LIGHT_HITS,
+    RC_STAT_CAMERA_NODE_VISITS, RC_STAT_CAMERA_TRI_TESTS, RC_STAT_CAMERA_HITS, RC_STAT_CELL_VISITS,
+    RC_STAT_SORT_PASSES,
     RC_STAT_COUNT
 };
 

@@ -392,7 +392,9 @@ __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int i, 
     return false;
 }
 
-__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack) {
+template <bool kCount>
+__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+                                              uint32_t& n_nodes, uint32_t& n_tris) {
     int ext[kMaxDepth - kLdsStack];
     float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
     float tbest = INFINITY;
@@ -407,6 +409,7 @@ __device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, f
     const float slack = 1.0000019f;
     while (true) {
         if (ref >= 0) {
+            if (kCount) ++n_nodes;
             const float4* n = nodes + 4 * ref;
             float4 n0 = n[0], n1 = n[1], n2 = n[2], n3 = n[3];
             float l0x = (n0.x - o.x) * ix, l1x = (n0.y - o.x) * ix;
@@ -438,6 +441,7 @@ __device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, f
         } else {
             int enc = ~ref;
             int first = enc >> 3, count = (enc & 7) + 1;
+            if (kCount) n_tris += count;
             for (int i = first; i < first + count; ++i) {
                 float t, u, v;
                 if (tri_test(sc.tri, i, o, d, tmin, tbest, t, u, v)) {

@@ -101,6 +101,7 @@ struct Engine {
     int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
     double photon_capacity = 0, cell_scale = 1.0;
     int persistent_blocks = 0;
+    int count_events = 0;
     std::string output_path = "D:\\caustics.exr";
     // device scene
     DevBuf<float4> d_tri, d_shade, d_nodes;
@@ -118,7 +119,9 @@ struct Engine {
     DevBuf<unsigned long long> keys_a, keys_b;
     DevBuf<uint2> cells;
     DevBuf<float> image;
-    DevBuf<Counters> stats_dev;
+    DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
+    hipEvent_t ev[6] = {};
+    int sort_passes = 0;
     uint32_t capacity = 0;
     int iterations_done = 0;
     double stats[RC_STAT_COUNT] = {0};
@@ -491,6 +494,7 @@ bool ensure_device(Engine& e) {
     HIP_OK(hipGetDeviceProperties(&prop, 0));
     e.device_cus = prop.multiProcessorCount;
     HIP_OK(hipStreamCreate(&e.stream));
+    for (auto& ev : e.ev) HIP_OK(hipEventCreate(&ev));
     return true;
 }
 
@@ -593,7 +597,7 @@ bool prepare(Engine& e) {
         return false;
     if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
     HIP_OK(hipMemset(e.image.p, 0, (size_t)e.W * e.H * 3 * sizeof(float)));
-    if (!e.counters.alloc(4) || !e.stats_dev.alloc(1)) return false;
+    if (!e.counters.alloc(4) || !e.stats_dev.alloc(2)) return false;
     e.iterations_done = 0;
     e.prepared = true;
     return true;
@@ -643,6 +647,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
     if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
         LightPassArgs a{};
         a.sc = scene_view(e);
         a.k0 = (uint32_t)e.seed;
@@ -657,7 +662,10 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.out_dir = e.ph_dir.p;
         a.out_tp = e.ph_tp.p;
         a.out_uid = e.ph_uid.p;
+        a.counters = e.count_events ? e.stats_dev.p : nullptr;
+        HIP_OK(hipEventRecord(e.ev[0], e.stream));
         launch_light_pass(a, blocks_for(e), e.stream);
+        HIP_OK(hipEventRecord(e.ev[1], e.stream));
         HIP_OK(hipGetLastError());
         uint32_t c = 0;
         HIP_OK(hipMemcpyAsync(&c, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
@@ -713,7 +721,10 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     a.out_tp = e.s_tp.p;
     a.out_index = e.s_index.p;
     a.cells = e.cells.p;
+    e.sort_passes = (bucket_bits + uid_bits + 7) / 8;
+    HIP_OK(hipEventRecord(e.ev[2], e.stream));
     build_photon_map(a, e.stream);
+    HIP_OK(hipEventRecord(e.ev[3], e.stream));
     HIP_OK(hipGetLastError());
     PhotonMapView& pm = mi.pm;
     pm.pos = e.s_pos.p;
@@ -730,7 +741,6 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
 
 bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, float* image) {
     launch_fill_u32(e.counters.p, 0u, 4, e.stream);
-    HIP_OK(hipMemsetAsync(e.stats_dev.p, 0, sizeof(Counters), e.stream));
     CameraPassArgs a{};
     a.sc = scene_view(e);
     a.pm = mi.pm;
@@ -742,8 +752,10 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.path_counter = e.counters.p;
     a.vm_norm = norm;
     a.image = image;
-    a.counters = e.stats_dev.p;
+    a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
+    HIP_OK(hipEventRecord(e.ev[4], e.stream));
     launch_camera_pass(a, blocks_for(e), e.stream);
+    HIP_OK(hipEventRecord(e.ev[5], e.stream));
     HIP_OK(hipGetLastError());
     return true;
 }
@@ -758,31 +770,37 @@ bool render_iteration(Engine& e, int iteration) {
     auto t0 = std::chrono::steady_clock::now();
     uint32_t count = 0;
     if (!run_light_pass(e, iteration, count)) return false;
-    double t_light = ms_since(t0);
-    auto t1 = std::chrono::steady_clock::now();
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
-    HIP_OK(hipStreamSynchronize(e.stream));
-    double t_map = ms_since(t1);
-    auto t2 = std::chrono::steady_clock::now();
     if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
-    Counters c{};
-    HIP_OK(hipMemcpyAsync(&c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
+    Counters c[2] = {};
+    if (e.count_events) HIP_OK(hipMemcpyAsync(c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
     HIP_OK(hipStreamSynchronize(e.stream));
-    double t_cam = ms_since(t2);
+    float ms[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
     e.stats[RC_STAT_LIGHT_PATHS] = (double)light_paths(e);
     e.stats[RC_STAT_PHOTONS] = count;
     e.stats[RC_STAT_CAMERA_PATHS] = (double)e.W * e.H;
-    e.stats[RC_STAT_GATHERS] = (double)c.gathers;
-    e.stats[RC_STAT_PHOTONS_TESTED] = (double)c.photons_tested;
-    e.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)c.photons_accepted;
-    e.stats[RC_STAT_MS_LIGHT] = t_light;
-    e.stats[RC_STAT_MS_MAP] = t_map;
-    e.stats[RC_STAT_MS_CAMERA] = t_cam;
+    e.stats[RC_STAT_MS_LIGHT] = ms[0];
+    e.stats[RC_STAT_MS_MAP] = ms[1];
+    e.stats[RC_STAT_MS_CAMERA] = ms[2];
     e.stats[RC_STAT_MS_TOTAL] = ms_since(t0);
     e.stats[RC_STAT_RADIUS] = radius;
     e.stats[RC_STAT_NODES] = e.n_nodes;
     e.stats[RC_STAT_TRIS] = (double)(e.tris.size() / 4);
+    e.stats[RC_STAT_SORT_PASSES] = e.sort_passes;
+    if (e.count_events) {
+        e.stats[RC_STAT_LIGHT_NODE_VISITS] = (double)c[0].nodes;
+        e.stats[RC_STAT_LIGHT_TRI_TESTS] = (double)c[0].tris;
+        e.stats[RC_STAT_LIGHT_HITS] = (double)c[0].hits;
+        e.stats[RC_STAT_CAMERA_NODE_VISITS] = (double)c[1].nodes;
+        e.stats[RC_STAT_CAMERA_TRI_TESTS] = (double)c[1].tris;
+        e.stats[RC_STAT_CAMERA_HITS] = (double)c[1].hits;
+        e.stats[RC_STAT_CELL_VISITS] = (double)c[1].cells;
+        e.stats[RC_STAT_GATHERS] = (double)c[1].gathers;
+        e.stats[RC_STAT_PHOTONS_TESTED] = (double)c[1].photons_tested;
+        e.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)c[1].photons_accepted;
+    }
     return true;
 }
 
@@ -880,6 +898,8 @@ void release(Engine*& e) {
     e->d_mats.release();
     e->d_lights.release();
     e->d_cdf.release();
+    for (auto& ev : e->ev)
+        if (ev) (void)hipEventDestroy(ev);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     e = nullptr;
@@ -1114,6 +1134,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "photon_capacity") g->photon_capacity = v;
     else if (k == "cell_scale") g->cell_scale = v;
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
+    else if (k == "count_events") g->count_events = (int)v;
     else return -1;
     return 0;
 }

@@ -66,9 +66,12 @@ struct PhotonMapView {
     float r_query;  // radius with margin used to pick the cells
 };
 
+// Event counts of one pass (counting launches only): they define the
+// algorithmic bytes of the roofline figures.
 struct Counters {
-    unsigned long long photons_tested, photons_accepted, gathers;
+    unsigned long long paths, nodes, tris, hits, photons_stored, gathers, cells, photons_tested, photons_accepted;
 };
+constexpr int kCounterFields = 9;
 
 struct LightPassArgs {
     SceneView sc;
@@ -81,6 +84,7 @@ struct LightPassArgs {
     float4* out_dir;
     float2* out_tp;
     uint32_t* out_uid;
+    Counters* counters;  // non-null selects the counting variant
 };
 
 struct CameraPassArgs {
@@ -91,7 +95,7 @@ struct CameraPassArgs {
     uint32_t* path_counter;
     float vm_norm;
     float* image;
-    Counters* counters;
+    Counters* counters;  // non-null selects the counting variant
 };
 
 // ---- launchers (rc_kernels.hip) ----

@@ -31,6 +31,17 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want, uin
     return base + lanes_below(m);
 }
 
+// Adds per-lane event counts into a global Counters with one atomic per wave and field.
+__device__ __forceinline__ void flush_counters(Counters* dst, unsigned long long (&c)[kCounterFields]) {
+#pragma unroll
+    for (int f = 0; f < kCounterFields; ++f) {
+        unsigned long long v = c[f];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(dst) + f, v);
+    }
+}
+enum { C_PATHS, C_NODES, C_TRIS, C_HITS, C_STORED, C_GATHERS, C_CELLS, C_TESTED, C_ACCEPTED };
+
 // ---------------------------------------------------------------------
 // Light pass
 // ---------------------------------------------------------------------
@@ -41,10 +52,13 @@ struct LightState {
     uint32_t path;
 };
 
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
     const SceneView& sc = a.sc;
+    unsigned long long cnt[kCounterFields] = {0};
+    uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
     bool active = false, exhausted = false;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
@@ -55,6 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         uint32_t slot = wave_alloc(a.path_counter, need, &m);
         if (need) {
             if (slot < a.n_paths) {
+                if (kCount) cnt[C_PATHS]++;
                 st.path = slot;
                 rng.path = slot;
                 // Generate_light_sample (Vertex_Merging.cpp:18-25)
@@ -69,6 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         }
         if (__ballot(active || !exhausted) == 0) break;
         if (__ballot(active) == 0) continue;
+        n_nodes = n_tris = 0;
         // one bounce
         bool alive = active;
         Hit h;
@@ -76,7 +92,13 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         Bsdf b;
         if (alive) {
             F3 ro = st.o + st.d * 0.001f;
-            if (!trace_closest(sc, ro, st.d, 0.001f, h, my_stack)) {
+            bool hit = trace_closest<kCount>(sc, ro, st.d, 0.001f, h, my_stack, n_nodes, n_tris);
+            if (kCount) {
+                cnt[C_NODES] += n_nodes;
+                cnt[C_TRIS] += n_tris;
+                cnt[C_HITS] += hit;
+            }
+            if (!hit) {
                 alive = false;
             } else {
                 sf = surface_at(sc, h, ro, st.d);
@@ -86,6 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         }
         bool store = alive && !bsdf_delta(b) && (st.last & EV_GLOSSY);
         uint32_t at = wave_alloc(a.photon_counter, store, &m);
+        if (kCount) cnt[C_STORED] += store;
         if (store && at < a.capacity) {
             a.out_pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
             a.out_dir[at] = make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x);
@@ -124,17 +147,20 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         }
         active = alive;
     }
+    if (kCount) flush_counters(a.counters, cnt);
 }
 
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_light_pass, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.counters) hipLaunchKernelGGL(k_light_pass<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_light_pass<false>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 // ---------------------------------------------------------------------
 // Photon-map radius query
 // ---------------------------------------------------------------------
-template <typename F>
-__device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q, F&& fn) {
+template <bool kCount, typename F>
+__device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q, F&& fn, uint32_t& n_cells,
+                                                   uint32_t& n_tested) {
     int lo[3], hi[3];
     float qq[3] = {q.x, q.y, q.z};
 #pragma unroll
@@ -153,6 +179,10 @@ __device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q
                 if (dup) continue;
                 if (nseen < 27) seen[nseen++] = bkt;
                 uint2 range = pm.cells[bkt];
+                if (kCount) {
+                    ++n_cells;
+                    n_tested += range.y - range.x;
+                }
                 for (uint32_t i = range.x; i < range.y; ++i) {
                     float4 p = pm.pos[i];
                     F3 dd = q - f3(p.x, p.y, p.z);
@@ -165,17 +195,19 @@ __device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q
 // ---------------------------------------------------------------------
 // Camera pass
 // ---------------------------------------------------------------------
+template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_camera_pass(CameraPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
     const SceneView& sc = a.sc;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
-    unsigned long long tested = 0, accepted = 0, gathers = 0;
+    unsigned long long cnt[kCounterFields] = {0};
     for (;;) {
         uint64_t m;
         uint32_t path = wave_alloc(a.path_counter, true, &m);
         if (__ballot(path < a.n_paths) == 0) break;
         if (path < a.n_paths) {
+            if (kCount) cnt[C_PATHS]++;
             rng.path = path;
             const int W = sc.W, H = sc.H;
             // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
@@ -188,23 +220,36 @@ __global__ __launch_bounds__(kBlock) void k_camera_pass(CameraPassArgs a) {
             for (int pl = 0;; ++pl) {
                 F3 ro = o + d * 0.001f;
                 Hit h;
-                if (!trace_closest(sc, ro, d, 0.001f, h, my_stack)) break;
+                uint32_t n_nodes = 0, n_tris = 0;
+                bool hit = trace_closest<kCount>(sc, ro, d, 0.001f, h, my_stack, n_nodes, n_tris);
+                if (kCount) {
+                    cnt[C_NODES] += n_nodes;
+                    cnt[C_TRIS] += n_tris;
+                    cnt[C_HITS] += hit;
+                }
+                if (!hit) break;
                 Surface sf = surface_at(sc, h, ro, d);
                 Bsdf b;
                 if (!bsdf_setup(b, sc.mats[sf.mat], sf.sn, d, true)) break;
                 dvm /= fabsf(b.wo.z);
                 if (!bsdf_delta(b)) {
                     F3 c = f3s(0.0f);
-                    ++gathers;
-                    for_each_in_radius(a.pm, sf.pos, [&](uint32_t i, float4 p) {
+                    uint32_t n_cells = 0, n_tested = 0, n_acc = 0;
+                    for_each_in_radius<kCount>(a.pm, sf.pos, [&](uint32_t i, float4 p) {
                         float4 dt = a.pm.dir_tr[i];
                         float2 gb = a.pm.tp_gb[i];
                         float fwd, rev;
                         F3 e = bsdf_eval_pdf(b, f3(dt.x, dt.y, dt.z), sf.gn, fwd, rev);
                         float w = 1.0f / (1.0f + dvm * rev + p.w * fwd);
                         c = c + (e * f3(dt.w, gb.x, gb.y)) * w;
-                        ++accepted;
-                    });
+                        if (kCount) ++n_acc;
+                    }, n_cells, n_tested);
+                    if (kCount) {
+                        cnt[C_GATHERS]++;
+                        cnt[C_CELLS] += n_cells;
+                        cnt[C_TESTED] += n_tested;
+                        cnt[C_ACCEPTED] += n_acc;
+                    }
                     color = color + (tp * a.vm_norm) * c;
                 }
                 if (pl > sc.max_path_length) break;
@@ -236,22 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_camera_pass(CameraPassArgs a) {
             }
         }
     }
-    (void)tested;
-    if (a.counters) {
-        // wave-reduce then one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) {
-            accepted += __shfl_down(accepted, off);
-            gathers += __shfl_down(gathers, off);
-        }
-        if (lane_id() == 0) {
-            atomicAdd(&a.counters->photons_accepted, accepted);
-            atomicAdd(&a.counters->gathers, gathers);
-        }
-    }
+    if (kCount) flush_counters(a.counters, cnt);
 }
 
 void launch_camera_pass(const CameraPassArgs& a, int blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_camera_pass, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.counters) hipLaunchKernelGGL(k_camera_pass<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_camera_pass<false>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 
 // ---------------------------------------------------------------------
@@ -265,7 +300,8 @@ __global__ __launch_bounds__(kBlock) void k_first_hit(SceneView sc, const float*
     F3 o = f3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
     F3 d = f3(rays[6 * i + 3], rays[6 * i + 4], rays[6 * i + 5]);
     Hit h;
-    bool hit = trace_closest(sc, o, d, tmin, h, stack_lds + threadIdx.x);
+    uint32_t n_nodes = 0, n_tris = 0;
+    bool hit = trace_closest<false>(sc, o, d, tmin, h, stack_lds + threadIdx.x, n_nodes, n_tris);
     tri[i] = hit ? sc.tri_id[h.tri] : -1;
     t[i] = hit ? h.t : INFINITY;
 }
@@ -508,8 +544,9 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
 __global__ void k_gather_count(PhotonMapView pm, const float* q, int nq, uint32_t* counts) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
-    uint32_t c = 0;
-    for_each_in_radius(pm, f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]), [&](uint32_t, float4) { ++c; });
+    uint32_t c = 0, n_cells = 0, n_tested = 0;
+    for_each_in_radius<false>(pm, f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]), [&](uint32_t, float4) { ++c; }, n_cells,
+                              n_tested);
     counts[i] = c;
 }
 __global__ void k_gather_fill(PhotonMapView pm, const float* q, int nq, const uint32_t* orig,
@@ -517,8 +554,9 @@ __global__ void k_gather_fill(PhotonMapView pm, const float* q, int nq, const ui
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     unsigned long long at = offsets[i];
-    for_each_in_radius(pm, f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]),
-                       [&](uint32_t j, float4) { out[at++] = (int)orig[j]; });
+    uint32_t n_cells = 0, n_tested = 0;
+    for_each_in_radius<false>(pm, f3(q[3 * i], q[3 * i + 1], q[3 * i + 2]),
+                              [&](uint32_t j, float4) { out[at++] = (int)orig[j]; }, n_cells, n_tested);
 }
 void launch_gather_count(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig, uint32_t* counts,
                          hipStream_t s) {

@@ -24,7 +24,9 @@ EXPORTS = [
     "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
 ]
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
-              "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris"]
+              "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris",
+              "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
+              "camera_hits", "cell_visits", "sort_passes"]
 
 _lib = None
 
