@@ -99,6 +99,7 @@ def main() -> int:
     for _ in range(a.warmup):
         eng.render_iteration()
     kern = {"light": 0.0, "map": 0.0, "camera": 0.0}
+    split = {"camera_trace": 0.0, "gather": 0.0, "resolve": 0.0}
     t0 = time.perf_counter()
     for i in range(a.steps):
         eng.render_iteration()
@@ -106,10 +107,14 @@ def main() -> int:
         kern["light"] += st["ms_light"]
         kern["map"] += st["ms_map"]
         kern["camera"] += st["ms_camera"]
+        split["camera_trace"] += st["ms_camera_trace"]
+        split["gather"] += st["ms_gather"]
+        split["resolve"] += st["ms_resolve"]
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1000.0 / a.steps
     kern = {k: v / a.steps for k, v in kern.items()}
-    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernels {kern}", file=sys.stderr, flush=True)
+    split = {k: v / a.steps for k, v in split.items()}
+    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernels {kern} camera {split}", file=sys.stderr, flush=True)
     # counting launch (not timed): event counts give the algorithmic bytes
     eng.set("count_events", 1)
     eng.render_iteration()
@@ -122,6 +127,7 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
     extra = {
         "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
+        "camera_split_ms": {k: round(v, 3) for k, v in split.items()},
         "algorithmic_bytes": {k: int(v) for k, v in by.items()},
         "photons_per_iteration": int(cst["photons"]),
         "gathers_per_s": cst["gathers"] / (ms_step * 1e-3),

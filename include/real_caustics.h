@@ -99,6 +99,8 @@ enum {
     RC_STAT_LIGHT_NODE_VISITS, RC_STAT_LIGHT_TRI_TESTS, RC_STAT_LIGHT_HITS,
     RC_STAT_CAMERA_NODE_VISITS, RC_STAT_CAMERA_TRI_TESTS, RC_STAT_CAMERA_HITS, RC_STAT_CELL_VISITS,
     RC_STAT_SORT_PASSES,
+    /* camera pass split: trace (includes the query-count readback), merge, resolve */
+    RC_STAT_MS_CAMERA_TRACE, RC_STAT_MS_GATHER, RC_STAT_MS_RESOLVE,
     RC_STAT_COUNT
 };
 

@@ -119,8 +119,12 @@ struct Engine {
     DevBuf<unsigned long long> keys_a, keys_b;
     DevBuf<uint2> cells;
     DevBuf<float> image;
+    DevBuf<float4> q_pos, q_dir, q_sn, q_gn, q_tpn;
+    DevBuf<int2> path_info;
+    DevBuf<float> contrib;
+    uint32_t q_capacity = 0;
     DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
-    hipEvent_t ev[6] = {};
+    hipEvent_t ev[8] = {};  // light, map, camera (trace..resolve), trace end, gather end
     int sort_passes = 0;
     uint32_t capacity = 0;
     int iterations_done = 0;
@@ -739,8 +743,18 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     return true;
 }
 
+bool ensure_query_buffers(Engine& e, uint32_t cap, uint32_t n_paths) {
+    if (!e.path_info.alloc(n_paths) || !e.contrib.alloc((size_t)n_paths * 8 * 3)) return false;
+    if (cap <= e.q_capacity && e.q_pos.p) return true;
+    e.q_capacity = cap;
+    return e.q_pos.alloc(cap) && e.q_dir.alloc(cap) && e.q_sn.alloc(cap) && e.q_gn.alloc(cap) && e.q_tpn.alloc(cap);
+}
+
+// Camera pass: trace (queries), merge, resolve into `image`.
 bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, float* image) {
-    launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+    uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
+    uint32_t cap = std::max(e.q_capacity, std::max<uint32_t>(2 * n_paths, 1024));
+    if (!ensure_query_buffers(e, cap, n_paths)) return false;
     CameraPassArgs a{};
     a.sc = scene_view(e);
     a.pm = mi.pm;
@@ -748,13 +762,34 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.k1 = (uint32_t)(e.seed >> 32);
     a.iteration = (uint32_t)iteration;
     a.stream = STREAM_CAMERA;
-    a.n_paths = (uint32_t)((long long)e.W * e.H);
+    a.n_paths = n_paths;
     a.path_counter = e.counters.p;
     a.vm_norm = norm;
+    a.path_info = e.path_info.p;
+    a.contrib = e.contrib.p;
     a.image = image;
     a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
-    HIP_OK(hipEventRecord(e.ev[4], e.stream));
-    launch_camera_pass(a, blocks_for(e), e.stream);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
+        launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        HIP_OK(hipEventRecord(e.ev[4], e.stream));
+        launch_camera_trace(a, blocks_for(e), e.stream);
+        HIP_OK(hipGetLastError());
+        uint32_t nq = 0;
+        HIP_OK(hipMemcpyAsync(&nq, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        HIP_OK(hipStreamSynchronize(e.stream));
+        e.stats[RC_STAT_GATHERS] = nq;
+        if (nq <= e.q_capacity) break;
+        if (attempt == 1 || !ensure_query_buffers(e, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), n_paths)) {
+            set_error("merge query buffer overflow");
+            return false;
+        }
+        if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p + 1), 0u, kCounterFields * 2, e.stream);
+    }
+    HIP_OK(hipEventRecord(e.ev[6], e.stream));
+    launch_gather(a, std::max(1, e.device_cus) * 16, e.stream);
+    HIP_OK(hipEventRecord(e.ev[7], e.stream));
+    launch_resolve(a, e.stream);
     HIP_OK(hipEventRecord(e.ev[5], e.stream));
     HIP_OK(hipGetLastError());
     return true;
@@ -778,6 +813,13 @@ bool render_iteration(Engine& e, int iteration) {
     HIP_OK(hipStreamSynchronize(e.stream));
     float ms[3] = {0, 0, 0};
     for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
+    float ms_trace = 0, ms_gather = 0, ms_resolve = 0;
+    HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[6]));
+    HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
+    HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
+    e.stats[RC_STAT_MS_CAMERA_TRACE] = ms_trace;
+    e.stats[RC_STAT_MS_GATHER] = ms_gather;
+    e.stats[RC_STAT_MS_RESOLVE] = ms_resolve;
     e.stats[RC_STAT_LIGHT_PATHS] = (double)light_paths(e);
     e.stats[RC_STAT_PHOTONS] = count;
     e.stats[RC_STAT_CAMERA_PATHS] = (double)e.W * e.H;
@@ -887,6 +929,10 @@ void release(Engine*& e) {
     for (auto* b : f4) b->release();
     e->ph_tp.release();
     e->s_tp.release();
+    DevBuf<float4>* q4[] = {&e->q_pos, &e->q_dir, &e->q_sn, &e->q_gn, &e->q_tpn};
+    for (auto* b : q4) b->release();
+    e->path_info.release();
+    e->contrib.release();
     DevBuf<uint32_t>* u[] = {&e->ph_uid, &e->s_index, &e->vals_a, &e->vals_b, &e->hist, &e->hist_sums, &e->counters};
     for (auto* b : u) b->release();
     e->keys_a.release();

@@ -87,20 +87,40 @@ struct LightPassArgs {
     Counters* counters;  // non-null selects the counting variant
 };
 
+// One merge query per non-delta camera vertex (80 B, SoA):
+//   pos  {position, camera dVM}          dir {ray direction, material}
+//   sn   {shading normal, slot}          gn  {geometric normal, 0}
+//   tpn  {throughput * VM normalisation, 0}
+// slot = path * 8 + path_length: the merge result lands in contrib[slot] and
+// the resolve adds a path's slots in vertex order, as the reference does.
+struct QueryBuffers {
+    float4* pos;
+    float4* dir;
+    float4* sn;
+    float4* gn;
+    float4* tpn;
+    uint32_t capacity;
+};
+
 struct CameraPassArgs {
     SceneView sc;
     PhotonMapView pm;
     uint32_t k0, k1, iteration, stream;
     uint32_t n_paths;
-    uint32_t* path_counter;
+    uint32_t* path_counter;   // [0] paths, [1] queries, [2] gather work counter
     float vm_norm;
+    QueryBuffers q;
+    int2* path_info;          // per path: {image buffer index or -1, mask of vertices with a query}
+    float* contrib;           // 3 floats per slot
     float* image;
     Counters* counters;  // non-null selects the counting variant
 };
 
 // ---- launchers (rc_kernels.hip) ----
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s);
-void launch_camera_pass(const CameraPassArgs& a, int blocks, hipStream_t s);
+void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s);
+void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s);
+void launch_resolve(const CameraPassArgs& a, hipStream_t s);
 void launch_first_hit(const SceneView& sc, const float* rays, int n, float tmin, int* tri, float* t, hipStream_t s);
 
 // Photon map build: keys, sort, permute, cell ranges.

@@ -8,8 +8,11 @@
 //  photon map     hash-grid keys -> LSD radix sort of (bucket, uid) ->
 //                 permute -> per-bucket [start, end) (replaces the KD tree of
 //                 KD_Tree.cpp:110-190; the radius sets are identical).
-//  k_camera_pass  persistent camera tracer with the merge at every non-delta
-//                 vertex (Vertex_Merging.cpp:309-363, Path_Vertices.h:46-68).
+//  k_camera_trace persistent camera tracer (Vertex_Merging.cpp:309-363); every
+//                 non-delta vertex becomes a merge query.
+//  k_gather       the merge of each query (Path_Vertices.h:46-68), one wave
+//                 per query with ballot-compacted BSDF evaluation.
+//  k_resolve      per-path sum of the vertex contributions into the film.
 #include "rc_device.h"
 
 namespace rc {
@@ -193,100 +196,301 @@ __device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q
 }
 
 // ---------------------------------------------------------------------
-// Camera pass
+// Camera pass, part 1: trace camera paths (Vertex_Merging.cpp:309-363) with
+// per-bounce refill like the light pass; every non-delta vertex becomes a
+// merge query instead of gathering inline.
 // ---------------------------------------------------------------------
+struct CameraState {
+    F3 o, d, tp;
+    float dvm;
+    int pl;
+    uint32_t path, mask;
+    int pixel;
+};
+
 template <bool kCount>
-__global__ __launch_bounds__(kBlock) void k_camera_pass(CameraPassArgs a) {
+__global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
     const SceneView& sc = a.sc;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
     unsigned long long cnt[kCounterFields] = {0};
+    CameraState st;
+    bool active = false, exhausted = false;
     for (;;) {
+        bool need = !active && !exhausted;
         uint64_t m;
-        uint32_t path = wave_alloc(a.path_counter, true, &m);
-        if (__ballot(path < a.n_paths) == 0) break;
-        if (path < a.n_paths) {
-            if (kCount) cnt[C_PATHS]++;
-            rng.path = path;
-            const int W = sc.W, H = sc.H;
-            // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
-            float sx = (float)(int)(path % (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER);
-            float sy = (float)(int)(path / (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER + 1);
-            F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
-            dir = dir - ld3(sc.cam_o);
-            F3 o = ld3(sc.cam_o), d = normalize(dir), tp = f3s(1.0f), color = f3s(0.0f);
-            float dvm = 0.0f;
-            for (int pl = 0;; ++pl) {
-                F3 ro = o + d * 0.001f;
-                Hit h;
-                uint32_t n_nodes = 0, n_tris = 0;
-                bool hit = trace_closest<kCount>(sc, ro, d, 0.001f, h, my_stack, n_nodes, n_tris);
-                if (kCount) {
-                    cnt[C_NODES] += n_nodes;
-                    cnt[C_TRIS] += n_tris;
-                    cnt[C_HITS] += hit;
+        uint32_t slot = wave_alloc(a.path_counter, need, &m);
+        if (need) {
+            if (slot < a.n_paths) {
+                if (kCount) cnt[C_PATHS]++;
+                st.path = slot;
+                rng.path = slot;
+                const int W = sc.W, H = sc.H;
+                // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
+                float sx = (float)(int)(slot % (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER);
+                float sy = (float)(int)(slot / (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER + 1);
+                F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
+                dir = dir - ld3(sc.cam_o);
+                st.o = ld3(sc.cam_o);
+                st.d = normalize(dir);
+                st.tp = f3s(1.0f);
+                st.dvm = 0.0f;
+                st.pl = 0;
+                st.mask = 0;
+                // Image_Buffer::add_color (Image_Buffer.h:24-37): target buffer element
+                st.pixel = -1;
+                if (!(sx < 0.0f || sx >= (float)W) && !(sy < 0.0f || sy >= (float)H)) {
+                    int X = (int)sx;
+                    int Y = sc.compat_rows ? H - (int)sy : H - 1 - (int)sy;
+                    long long at = (long long)X + (long long)Y * W;
+                    if (at >= 0 && at <= (long long)W * H - 1) st.pixel = (int)at;
                 }
-                if (!hit) break;
-                Surface sf = surface_at(sc, h, ro, d);
-                Bsdf b;
-                if (!bsdf_setup(b, sc.mats[sf.mat], sf.sn, d, true)) break;
-                dvm /= fabsf(b.wo.z);
-                if (!bsdf_delta(b)) {
-                    F3 c = f3s(0.0f);
-                    uint32_t n_cells = 0, n_tested = 0, n_acc = 0;
-                    for_each_in_radius<kCount>(a.pm, sf.pos, [&](uint32_t i, float4 p) {
-                        float4 dt = a.pm.dir_tr[i];
-                        float2 gb = a.pm.tp_gb[i];
-                        float fwd, rev;
-                        F3 e = bsdf_eval_pdf(b, f3(dt.x, dt.y, dt.z), sf.gn, fwd, rev);
-                        float w = 1.0f / (1.0f + dvm * rev + p.w * fwd);
-                        c = c + (e * f3(dt.w, gb.x, gb.y)) * w;
-                        if (kCount) ++n_acc;
-                    }, n_cells, n_tested);
-                    if (kCount) {
-                        cnt[C_GATHERS]++;
-                        cnt[C_CELLS] += n_cells;
-                        cnt[C_TESTED] += n_tested;
-                        cnt[C_ACCEPTED] += n_acc;
-                    }
-                    color = color + (tp * a.vm_norm) * c;
-                }
-                if (pl > sc.max_path_length) break;
+                active = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(active || !exhausted) == 0) break;
+        if (__ballot(active) == 0) continue;
+        bool alive = active;
+        Hit h;
+        Surface sf;
+        Bsdf b;
+        if (alive) {
+            F3 ro = st.o + st.d * 0.001f;
+            uint32_t n_nodes = 0, n_tris = 0;
+            bool hit = trace_closest<kCount>(sc, ro, st.d, 0.001f, h, my_stack, n_nodes, n_tris);
+            if (kCount) {
+                cnt[C_NODES] += n_nodes;
+                cnt[C_TRIS] += n_tris;
+                cnt[C_HITS] += hit;
+            }
+            if (!hit) {
+                alive = false;
+            } else {
+                sf = surface_at(sc, h, ro, st.d);
+                if (!bsdf_setup(b, sc.mats[sf.mat], sf.sn, st.d, true)) alive = false;
+                else st.dvm /= fabsf(b.wo.z);
+            }
+        }
+        bool query = alive && !bsdf_delta(b);
+        uint32_t at = wave_alloc(a.path_counter + 1, query, &m);
+        if (query && at < a.q.capacity) {
+            F3 tpn = st.tp * a.vm_norm;
+            a.q.pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
+            a.q.dir[at] = make_float4(st.d.x, st.d.y, st.d.z, __int_as_float(sf.mat));
+            a.q.sn[at] = make_float4(sf.sn.x, sf.sn.y, sf.sn.z, __uint_as_float(st.path * 8u + (uint32_t)st.pl));
+            a.q.gn[at] = make_float4(sf.gn.x, sf.gn.y, sf.gn.z, 0.0f);
+            a.q.tpn[at] = make_float4(tpn.x, tpn.y, tpn.z, 0.0f);
+        }
+        if (query) st.mask |= 1u << st.pl;
+        if (alive) {
+            if (st.pl > sc.max_path_length) {
+                alive = false;
+            } else {
+                // Sample_Scattering_camera (Vertex_Merging.cpp:98-135)
                 float u[3];
-                rng.u3(3 * pl, u);
+                rng.path = st.path;
+                rng.u3(3 * st.pl, u);
                 BsdfSample bs;
                 F3 f = bsdf_sample(b, sf.gn, u[0], u[1], u[2], bs);
-                if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) break;
-                if (pl > 2) {
-                    if (rng.u(DIM_RR + (pl & 7)) > bs.cont) break;
-                    f = div_each(f, bs.cont);
-                }
-                o = sf.pos;
-                d = bs.wi;
-                tp = tp * f;
-                if (bs.event & EV_SPECULAR) dvm *= (bs.cos_out / bs.fwd) * bs.rev;
-                else dvm = (bs.cos_out / bs.fwd) * (1.0f + bs.rev * dvm);
-            }
-            // Image_Buffer::add_color (Image_Buffer.h:24-37)
-            if (!(sx < 0.0f || sx >= (float)W) && !(sy < 0.0f || sy >= (float)H)) {
-                int X = (int)sx;
-                int Y = sc.compat_rows ? H - (int)sy : H - 1 - (int)sy;
-                long long at = (long long)X + (long long)Y * W;
-                if (at >= 0 && at <= (long long)W * H - 1) {
-                    unsafeAtomicAdd(&a.image[3 * at], color.x);
-                    unsafeAtomicAdd(&a.image[3 * at + 1], color.y);
-                    unsafeAtomicAdd(&a.image[3 * at + 2], color.z);
+                if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {
+                    alive = false;
+                } else {
+                    if (st.pl > 2) {
+                        if (rng.u(DIM_RR + (st.pl & 7)) > bs.cont) alive = false;
+                        else f = div_each(f, bs.cont);
+                    }
+                    if (alive) {
+                        st.o = sf.pos;
+                        st.d = bs.wi;
+                        st.tp = st.tp * f;
+                        if (bs.event & EV_SPECULAR) st.dvm *= (bs.cos_out / bs.fwd) * bs.rev;
+                        else st.dvm = (bs.cos_out / bs.fwd) * (1.0f + bs.rev * st.dvm);
+                        st.pl++;
+                    }
                 }
             }
+        }
+        if (active && !alive) a.path_info[st.path] = make_int2(st.pixel, (int)st.mask);
+        active = alive;
+    }
+    if (kCount) flush_counters(a.counters, cnt);
+}
+
+// ---------------------------------------------------------------------
+// Camera pass, part 2: the merge (Path_Vertices.h:46-68).  One wave per
+// query: lanes stride over the query's photon cells with coalesced loads,
+// accepted photons are compacted (ballot + prefix) into an LDS queue and
+// evaluated 64 at a time with every lane busy; the lane sums are reduced
+// with a fixed butterfly, so a query's result does not depend on timing.
+// ---------------------------------------------------------------------
+constexpr int kGatherBlock = 256;
+constexpr int kGatherWaves = kGatherBlock / 64;
+
+// Wave-wide inclusive prefix sum.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = (uint32_t)__shfl_up((int)v, off);
+        if (lane >= (uint32_t)off) v += y;
+    }
+    return v;
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
+    __shared__ uint32_t queue_lds[kGatherWaves][128];
+    __shared__ uint32_t rfirst_lds[kGatherWaves][32];  // first photon of each non-empty cell range
+    __shared__ uint32_t rbase_lds[kGatherWaves][32];   // offset of that range in the query's photon stream
+    const uint32_t wv = threadIdx.x >> 6;
+    uint32_t* queue = queue_lds[wv];
+    uint32_t* rfirst = rfirst_lds[wv];
+    uint32_t* rbase = rbase_lds[wv];
+    const uint32_t lane = lane_id();
+    const PhotonMapView& pm = a.pm;
+    unsigned long long cnt[kCounterFields] = {0};
+    const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
+    const uint32_t nwaves = gridDim.x * kGatherWaves;
+    // queries are dealt to waves round-robin: no shared counter
+    for (uint32_t qi = blockIdx.x * kGatherWaves + wv; qi < nq; qi += nwaves) {
+        float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
+        F3 q = f3(qp.x, qp.y, qp.z);
+        float cdvm = qp.w;
+        Bsdf b;
+        bsdf_setup(b, a.sc.mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
+        F3 gn = f3(qg.x, qg.y, qg.z);
+        F3 c = f3s(0.0f);
+        auto eval_one = [&](uint32_t i) {
+            float4 p = pm.pos[i];
+            float4 dt = pm.dir_tr[i];
+            float2 gb = pm.tp_gb[i];
+            float fwd, rev;
+            F3 e = bsdf_eval_pdf(b, f3(dt.x, dt.y, dt.z), gn, fwd, rev);
+            float w = 1.0f / (1.0f + cdvm * rev + p.w * fwd);
+            c = c + (e * f3(dt.w, gb.x, gb.y)) * w;
+        };
+        // cells overlapping the ball: at most 3 per axis; lane l < 27 takes cell l
+        int lo[3], hi[3];
+        float qq[3] = {q.x, q.y, q.z};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = (int)floorf(((qq[k] - pm.r_query) - pm.origin[k]) * pm.inv_cell);
+            hi[k] = (int)floorf(((qq[k] + pm.r_query) - pm.origin[k]) * pm.inv_cell);
+        }
+        int cx = lo[0] + (int)(lane % 3u), cy = lo[1] + (int)((lane / 3u) % 3u), cz = lo[2] + (int)(lane / 9u);
+        bool valid = lane < 27u && cx <= hi[0] && cy <= hi[1] && cz <= hi[2];
+        uint32_t bkt = valid ? (cell_hash(cx, cy, cz) & pm.mask) : 0xffffffffu;
+        // two cells hashing to one bucket must not be gathered twice
+        for (int k = 0; k < 26; ++k) {
+            uint32_t other = (uint32_t)__shfl((int)bkt, k);
+            if ((uint32_t)k < lane && other == bkt) valid = false;
+        }
+        uint2 range = valid ? pm.cells[bkt] : make_uint2(0u, 0u);
+        uint32_t len = range.y - range.x;
+        uint64_t ne = __ballot(len > 0u);
+        uint32_t nr = (uint32_t)__builtin_popcountll(ne);
+        uint32_t incl = wave_incl_scan(len);
+        uint32_t total = (uint32_t)__shfl((int)incl, 63);
+        if (len > 0u) {
+            uint32_t r = lanes_below(ne);
+            rfirst[r] = range.x;
+            rbase[r] = incl - len;
+        }
+        if (kCount && lane == 0) {
+            cnt[C_CELLS] += (uint32_t)__builtin_popcountll(__ballot(valid));
+            cnt[C_TESTED] += total;
+        }
+        __builtin_amdgcn_wave_barrier();
+        uint32_t qhead = 0, qcount = 0;  // wave-uniform queue state
+        uint32_t rcur = 0;               // per lane: range of its current stream position
+        auto locate = [&](uint32_t sidx) -> uint32_t {
+            while (rcur + 1 < nr && rbase[rcur + 1] <= sidx) ++rcur;
+            return rfirst[rcur] + (sidx - rbase[rcur]);
+        };
+        auto push = [&](bool acc, uint32_t i) {
+            uint64_t mk = __ballot(acc);
+            uint32_t n = (uint32_t)__builtin_popcountll(mk);
+            if (acc) queue[(qhead + qcount + lanes_below(mk)) & 127u] = i;
+            qcount += n;
+            if (kCount && lane == 0) cnt[C_ACCEPTED] += n;
+            __builtin_amdgcn_wave_barrier();
+            if (qcount >= 64) {
+                eval_one(queue[(qhead + lane) & 127u]);
+                qhead = (qhead + 64) & 127u;
+                qcount -= 64;
+                __builtin_amdgcn_wave_barrier();
+            }
+        };
+        for (uint32_t sb = 0; sb < total; sb += 128) {
+            // two chunks of the stream in flight per lane
+            uint32_t s0 = sb + lane, s1 = sb + 64 + lane;
+            bool v0 = s0 < total, v1 = s1 < total;
+            uint32_t i0 = v0 ? locate(s0) : 0u;
+            uint32_t i1 = v1 ? locate(s1) : 0u;
+            float4 p0 = v0 ? pm.pos[i0] : make_float4(0, 0, 0, 0);
+            float4 p1 = v1 ? pm.pos[i1] : make_float4(0, 0, 0, 0);
+            bool acc0 = v0 && len2(q - f3(p0.x, p0.y, p0.z)) < pm.r2;
+            bool acc1 = v1 && len2(q - f3(p1.x, p1.y, p1.z)) < pm.r2;
+            push(acc0, i0);
+            push(acc1, i1);
+        }
+        if (lane < qcount) eval_one(queue[(qhead + lane) & 127u]);
+        __builtin_amdgcn_wave_barrier();
+        // fixed butterfly reduction over the 64 lanes
+        for (int off = 32; off > 0; off >>= 1) {
+            c.x += __shfl_xor(c.x, off);
+            c.y += __shfl_xor(c.y, off);
+            c.z += __shfl_xor(c.z, off);
+        }
+        if (lane == 0) {
+            float4 t = a.q.tpn[qi];
+            F3 r = f3(t.x, t.y, t.z) * c;
+            uint32_t slot = __float_as_uint(qs.w);
+            a.contrib[3 * (size_t)slot] = r.x;
+            a.contrib[3 * (size_t)slot + 1] = r.y;
+            a.contrib[3 * (size_t)slot + 2] = r.z;
+            if (kCount) cnt[C_GATHERS]++;
         }
     }
     if (kCount) flush_counters(a.counters, cnt);
 }
 
-void launch_camera_pass(const CameraPassArgs& a, int blocks, hipStream_t s) {
-    if (a.counters) hipLaunchKernelGGL(k_camera_pass<true>, dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_camera_pass<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+// Camera pass, part 3: color = sum of the path's vertex contributions in
+// vertex order, added to the film (Vertex_Merging.cpp:347, :361).
+__global__ void k_resolve(CameraPassArgs a) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n_paths) return;
+    int2 info = a.path_info[p];
+    if (info.x < 0) return;
+    F3 color = f3s(0.0f);
+    uint32_t mask = (uint32_t)info.y;
+    while (mask) {
+        int d = __builtin_ctz(mask);
+        mask &= mask - 1;
+        size_t s = ((size_t)p * 8 + d) * 3;
+        color = color + f3(a.contrib[s], a.contrib[s + 1], a.contrib[s + 2]);
+    }
+    if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f || (uint32_t)info.y != 0) {
+        unsafeAtomicAdd(&a.image[3 * (size_t)info.x], color.x);
+        unsafeAtomicAdd(&a.image[3 * (size_t)info.x + 1], color.y);
+        unsafeAtomicAdd(&a.image[3 * (size_t)info.x + 2], color.z);
+    }
+}
+
+void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
+    if (a.counters) hipLaunchKernelGGL(k_camera_trace<true>, dim3(blocks), dim3(kBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_camera_trace<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+}
+void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s) {
+    if (a.counters) hipLaunchKernelGGL(k_gather<true>, dim3(blocks), dim3(kGatherBlock), 0, s, a);
+    else hipLaunchKernelGGL(k_gather<false>, dim3(blocks), dim3(kGatherBlock), 0, s, a);
+}
+void launch_resolve(const CameraPassArgs& a, hipStream_t s) {
+    if (a.n_paths == 0) return;
+    hipLaunchKernelGGL(k_resolve, dim3((a.n_paths + 255) / 256), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------

@@ -26,7 +26,7 @@ EXPORTS = [
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
               "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris",
               "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
-              "camera_hits", "cell_visits", "sort_passes"]
+              "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve"]
 
 _lib = None
 
