@@ -187,13 +187,18 @@ void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
 
 /* Per-path sample space of the plain estimator.  Dimension d of path p in
  * iteration i and stream s is word (d & 3) of Philox(ctr = {d >> 2, p, i, s},
- * key = seed), mapped to [0,1) with 24 bits.  Dimensions: 0..31 primary
- * samples (MLT_Sampler layout, MLT_Sampler.h:29-40), 32 + k Russian roulette
- * at path length k, 40/41 camera jitter x/y. */
-enum { DIM_RR = 32, DIM_JITTER = 40 };
+ * key = seed), mapped to [0,1) with 24 bits, so every Philox block holds four
+ * consecutive dimensions.  Light paths: dimensions 0..4 are the light sample
+ * (pick, two for the position, two for the direction), bounce k uses block
+ * 2 + k: its first three words are the BSDF sample and the fourth the Russian
+ * roulette draw.  Camera paths: dimensions 0 and 1 are the pixel jitter and
+ * bounce k uses block 1 + k the same way.  The primary sample vector the
+ * reference indexes (MLT_Sampler.h:29-40) is filled from these dimensions. */
 enum { STREAM_LIGHT = 0, STREAM_CAMERA = 1 };
 
 typedef struct { uint32_t key[2]; uint32_t path, iter, stream; } PathKey;
+
+static int bounce_block(int stream, int path_length) { return (stream == STREAM_LIGHT ? 2 : 1) + (path_length & 7); }
 
 static real u24(uint32_t x) { return (real)(x >> 8) * K(5.9604644775390625e-08); }
 
@@ -262,13 +267,18 @@ static void sampler_mutate(Sampler *s, Xorshift *x) {
 }
 static void sampler_reject(Sampler *s) { for (int i = 0; i < s->n; ++i) s->value[i] = s->backup[i]; }
 static void sampler_plain(Sampler *s, const PathKey *k) {
-    s->n = 32;
     s->index = 0;
-    for (int b = 0; b < 8; ++b) {
-        uint32_t ctr[4] = {(uint32_t)b, k->path, k->iter, k->stream}, out[4];
-        orc_philox(ctr, k->key, out);
-        for (int j = 0; j < 4; ++j) s->value[4 * b + j] = u24(out[j]);
+    int first = 0;
+    if (k->stream == STREAM_LIGHT) {
+        for (int d = 0; d < 5; ++d) s->value[d] = path_u(k, d);
+        first = 5;
     }
+    for (int b = 0; b < 8; ++b) {
+        uint32_t ctr[4] = {(uint32_t)bounce_block((int)k->stream, b), k->path, k->iter, k->stream}, out[4];
+        orc_philox(ctr, k->key, out);
+        for (int j = 0; j < 3; ++j) s->value[first + 3 * b + j] = u24(out[j]);
+    }
+    s->n = first + 24;
 }
 static real get1d(Sampler *s) {
     real v = s->index < s->n ? s->value[s->index] : K(0);
@@ -280,7 +290,7 @@ static real get1d(Sampler *s) {
  * Russian roulette and camera jitter. */
 typedef struct { int serial; Xorshift *xs; PathKey key; } Draws;
 static real draw_rr(Draws *d, int path_length) {
-    return d->serial ? random_float_0_1(d->xs) : path_u(&d->key, DIM_RR + (path_length & 7));
+    return d->serial ? random_float_0_1(d->xs) : path_u(&d->key, 4 * bounce_block((int)d->key.stream, path_length) + 3);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1645,8 +1655,8 @@ static void camera_pass(Scene *s, int iteration, const double *ph, long long P, 
             sampler_init_uniform(smp, &s->xs);
         } else {
             dr.key.path = (uint32_t)p;
-            jx = path_u(&dr.key, DIM_JITTER);
-            jy = path_u(&dr.key, DIM_JITTER + 1);
+            jx = path_u(&dr.key, 0);
+            jy = path_u(&dr.key, 1);
             sampler_plain(smp, &dr.key);
         }
         smp->index = 0;

@@ -242,54 +242,110 @@ __device__ F3 bsdf_sample(const Bsdf& b, F3 gn, float u0, float u1, float u2, Bs
     return b.R;
 }
 
-// BxDF::evaluate and BxDF::pdf together, for the merge (Path_Vertices.h:46-68)
-__device__ __forceinline__ F3 bsdf_eval_pdf(const Bsdf& b, F3 wi, F3 gn, float& fwd, float& rev) {
+// Per-query constants of the merge: the same operations the reference
+// repeats for every gathered photon, evaluated once (bitwise identical).
+struct MergeConst {
+    float f;        // Fresnel reflectance of wo (plastics), fresnel_dielectric_reflectance
+    float side_wo;  // dot(to_world(wo), gn) of the same-hemisphere test
+    float rd;       // |wo.z| / pi, reverse diffuse pdf
+    float awz;      // |wo.z|
+    F3 diffuse;     // R / pi
+    F3 gn;
+};
+__device__ __forceinline__ MergeConst merge_const(const Bsdf& b, F3 gn) {
+    MergeConst m;
+    m.awz = fabsf(b.wo.z);
+    m.f = (b.kind == BK_FMG || b.kind == BK_FSMG) ? fresnel_refl(m.awz, b.s) : 0.0f;
+    m.side_wo = dot(to_world(b.fr, b.wo), gn);
+    m.rd = m.awz * kInvPI;
+    m.diffuse = b.R * kInvPI;
+    m.gn = gn;
+    return m;
+}
+
+// Division and square root of the merge evaluation.  kFast uses the
+// hardware reciprocal and square root (v_rcp_f32 / v_sqrt_f32, <= 1 ulp each,
+// so a quotient is within 2 ulp); otherwise the correctly rounded forms that
+// match the reference bit for bit.
+template <bool kFast>
+__device__ __forceinline__ float mdiv(float a, float b) {
+    return kFast ? a * __builtin_amdgcn_rcpf(b) : a / b;
+}
+template <bool kFast>
+__device__ __forceinline__ float msqrt(float x) {
+    return kFast ? __builtin_amdgcn_sqrtf(x) : sqrtf(x);
+}
+template <bool kFast>
+__device__ __forceinline__ F3 mnormalize(F3 a) {
+    float s = mdiv<kFast>(1.0f, msqrt<kFast>(len2(a)));
+    return a * s;
+}
+template <bool kFast>
+__device__ __forceinline__ float mggx_D(float ndoth, float a2) {
+    float den = ndoth * ndoth * (a2 - 1.0f) + 1.0f;
+    return mdiv<kFast>(a2, kPI * den * den);
+}
+template <bool kFast>
+__device__ __forceinline__ float mggx_G2(float nw, float nl, float a2) {
+    float A = nw * msqrt<kFast>(a2 + (1.0f - a2) * nl * nl);
+    float B = nl * msqrt<kFast>(a2 + (1.0f - a2) * nw * nw);
+    return mdiv<kFast>(2.0f * nl * nw, A + B);
+}
+
+// BxDF::evaluate and BxDF::pdf together, for the merge (Path_Vertices.h:46-68):
+// MRG BXDF.cpp:339-367, FMG :491-548, FSMG :634-667; delta lobes give zero.
+template <bool kFast>
+__device__ __forceinline__ F3 bsdf_eval_pdf(const Bsdf& b, const MergeConst& m, F3 wi, float& fwd, float& rev) {
     const F3 zero = f3s(0.0f);
     F3 wo = b.wo;
     F3 wil = to_local(b.fr, wi);
-    bool side = same_side(b, wi, gn);
+    bool side = !(dot(wi, m.gn) * m.side_wo <= 0.0f);
     if (b.kind == BK_MRG) {
-        F3 h = normalize(wo + wil);
+        F3 h = mnormalize<kFast>(wo + wil);
         float chn = fabsf(h.z);
-        float d = ggx_D(chn, b.a2);
-        fwd = rev = d * chn * 0.25f / absdot(h, wo);
+        float d = mggx_D<kFast>(chn, b.a2);
+        fwd = rev = mdiv<kFast>(d * chn * 0.25f, absdot(h, wo));
         if (!side || wil.z * wo.z <= 0.0f) return zero;
-        float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
-        return div_recip(((b.R * d) * g2) * 0.25f, fabsf(wo.z));
+        float g2 = mggx_G2<kFast>(m.awz, fabsf(wil.z), b.a2);
+        return (((b.R * d) * g2) * 0.25f) * mdiv<kFast>(1.0f, m.awz);
+    }
+    if (kFast && b.kind == BK_FMG && m.f == 0.0f) {
+        // specular 0: the lobe is R / pi with pdf |cos| / pi; the GGX terms are
+        // multiplied by f = 0 (they only differ when wi = -wo, where they are NaN)
+        fwd = fabsf(wil.z) * kInvPI;
+        rev = m.rd;
+        if (!side || wil.z * wo.z <= 0.0f) return zero;
+        return m.diffuse;
     }
     if (b.kind == BK_FMG) {
-        float fd = fabsf(wil.z) * kInvPI, rd = fabsf(wo.z) * kInvPI;
-        F3 h = normalize(wo + wil);
+        float fd = fabsf(wil.z) * kInvPI;
+        // wo + wi and wi + wo round identically, so pdf and evaluate share h
+        F3 h = mnormalize<kFast>(wo + wil);
         float chn = fabsf(h.z);
-        float d = ggx_D(chn, b.a2);
-        float sp = d * chn * 0.25f / absdot(h, wo);
-        float f = fresnel_refl(fabsf(wo.z), b.s);
+        float d = mggx_D<kFast>(chn, b.a2);
+        float sp = mdiv<kFast>(d * chn * 0.25f, absdot(h, wo));
+        float f = m.f;
         if (f > 1.0f) {
             fwd = rev = sp;
         } else {
             fwd = fd * (1.0f - f) + sp * f;
-            rev = rd * (1.0f - f) + sp * f;
+            rev = m.rd * (1.0f - f) + sp * f;
         }
         if (!side || wil.z * wo.z <= 0.0f) return zero;
-        // evaluate (BXDF.cpp:491-519) normalises wi + wo in that order
-        F3 h2 = normalize(wil + wo);
-        F3 diffuse = b.R * kInvPI;
-        float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
-        float chn2 = fabsf(h2.z);
-        float d2 = ggx_D(chn2, b.a2);
-        float spec = f * d2 * g2 * 0.25f / fabsf(wo.z);
-        return f3(diffuse.x + spec, diffuse.y + spec, diffuse.z + spec);
+        float g2 = mggx_G2<kFast>(m.awz, fabsf(wil.z), b.a2);
+        float spec = mdiv<kFast>(f * d * g2 * 0.25f, m.awz);
+        return f3(m.diffuse.x + spec, m.diffuse.y + spec, m.diffuse.z + spec);
     }
     if (b.kind == BK_FSMG) {
-        float f = fresnel_refl(fabsf(wo.z), b.s);
+        float f = m.f;
         if (f > 1.0f) {
             fwd = rev = 0.0f;
         } else {
             fwd = fabsf(wil.z) * kInvPI * (1.0f - f);
-            rev = fabsf(wo.z) * kInvPI * (1.0f - f);
+            rev = m.rd * (1.0f - f);
         }
         if (!side || wil.z * wo.z <= 0.0f) return zero;
-        return b.R * kInvPI;
+        return m.diffuse;
     }
     fwd = rev = 0.0f;
     return zero;
@@ -318,31 +374,34 @@ __device__ __forceinline__ int find_interval(const float* cdf, int size, float u
 
 // Returns false when the light emits nothing (the reference point light).
 __device__ bool sample_light(const SceneView& sc, const PathRng& rng, F3& o, F3& d, F3& tp) {
-    float rnd = rng.u(0);
+    U4 w0 = rng.block(0);
+    float u[5] = {u24(w0.a), u24(w0.b), u24(w0.c), u24(w0.d), 0.0f};
+    float rnd = u[0];
     int idx = find_interval(sc.cdf, sc.nlights + 1, rnd);
     const DevLight& L = sc.lights[idx];
     float pick = sc.compat_pdf ? L.power / ((float)sc.nlights * sc.integral) : L.power / sc.integral;
     F3 t;
     if (L.kind == L_AREA) {
-        float ax = rng.u(1), ay = rng.u(2);
+        float ax = u[1], ay = u[2];
         o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
-        F3 local = cos_hemisphere(rng.u(3), rng.u(4));
+        u[4] = u24(rng.block(1).a);
+        F3 local = cos_hemisphere(u[3], u[4]);
         Frame fr{ld3(L.fx), ld3(L.fy), ld3(L.fz)};
         d = to_world(fr, local);
         t = ld3(L.color) * kPI;
     } else if (L.kind == L_SUN) {
-        float ax = rng.u(1), ay = rng.u(2);
+        float ax = u[1], ay = u[2];
         o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
         d = ld3(L.normal);
         t = ld3(L.color);
     } else {
         if (sc.compat_point) return false;
-        float u1 = rng.u(1), u2 = rng.u(2);
+        float u1 = u[1], u2 = u[2];
         float z = 1.0f - 2.0f * u1;
         float rr = sqrtf(smax(0.0f, 1.0f - z * z));
         float phi = kPI2 * u2;
         F3 n = f3(rr * own_cos(phi), rr * own_sin(phi), z);
-        float e1 = rng.u(3), e2 = rng.u(4);
+        float e1 = u[3], e2 = u24(rng.block(1).a);
         if (L.radius > 0.0f) {
             o = ld3(L.pos) + n * L.radius;
             Frame fr = frame_from_z(n);

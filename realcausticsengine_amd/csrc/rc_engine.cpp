@@ -99,9 +99,10 @@ struct Engine {
     uint64_t seed = 0;
     long long photons = 0;
     int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
-    double photon_capacity = 0, cell_scale = 1.0;
+    double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
     int count_events = 0;
+    int exact_merge = 0;
     std::string output_path = "D:\\caustics.exr";
     // device scene
     DevBuf<float4> d_tri, d_shade, d_nodes;
@@ -702,7 +703,8 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     float maxc = 0.0f;
     for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
     float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
-    float cell = rq * (float)std::max(1.0, e.cell_scale);
+    // cells of at least 2/3 of the query radius: a query touches at most 4 per axis
+    float cell = rq * (float)std::max(0.67, e.cell_scale);
     MapBuildArgs a{};
     a.in_pos = e.ph_pos.p;
     a.in_dir = e.ph_dir.p;
@@ -769,6 +771,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.contrib = e.contrib.p;
     a.image = image;
     a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
+    a.exact_merge = e.exact_merge;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
@@ -787,7 +790,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p + 1), 0u, kCounterFields * 2, e.stream);
     }
     HIP_OK(hipEventRecord(e.ev[6], e.stream));
-    launch_gather(a, std::max(1, e.device_cus) * 16, e.stream);
+    launch_gather(a, std::max(8, e.device_cus) * 16, e.stream);  // a multiple of the 8 XCDs
     HIP_OK(hipEventRecord(e.ev[7], e.stream));
     launch_resolve(a, e.stream);
     HIP_OK(hipEventRecord(e.ev[5], e.stream));
@@ -993,6 +996,7 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         e->photon_capacity = prev->photon_capacity;
         e->cell_scale = prev->cell_scale;
         e->persistent_blocks = prev->persistent_blocks;
+        e->exact_merge = prev->exact_merge;
         e->output_path = prev->output_path;
         release(prev);
     }
@@ -1161,6 +1165,7 @@ __attribute__((visibility("default"))) int main(void) {
     keep->photon_capacity = e.photon_capacity;
     keep->cell_scale = e.cell_scale;
     keep->persistent_blocks = e.persistent_blocks;
+    keep->exact_merge = e.exact_merge;
     keep->output_path = e.output_path;
     release(g);
     g = keep;
@@ -1181,6 +1186,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "cell_scale") g->cell_scale = v;
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
     else if (k == "count_events") g->count_events = (int)v;
+    else if (k == "exact_merge") g->exact_merge = (int)v;
     else return -1;
     return 0;
 }

@@ -114,6 +114,7 @@ struct CameraPassArgs {
     float* contrib;           // 3 floats per slot
     float* image;
     Counters* counters;  // non-null selects the counting variant
+    int exact_merge;     // 1: correctly rounded division/sqrt in the merge
 };
 
 // ---- launchers (rc_kernels.hip) ----

@@ -34,6 +34,35 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want, uin
     return base + lanes_below(m);
 }
 
+// Per-wave pool of path indices taken from the global counter in chunks of
+// kPathChunk, so the refill of finished lanes costs one atomic per chunk.
+// Indices are handed out in increasing order; a wave only stops when its
+// lanes draw indices past the end, so no index below the end is skipped.
+constexpr uint32_t kPathChunk = 256;
+struct WavePool {
+    uint32_t next = 0, end = 0;  // wave-uniform
+    __device__ __forceinline__ uint32_t draw(uint32_t* counter, bool want) {
+        uint64_t m = __ballot(want);
+        if (m == 0) return 0;
+        uint32_t n = (uint32_t)__builtin_popcountll(m);
+        uint32_t r = lanes_below(m);
+        uint32_t rem = end - next;
+        uint32_t slot;
+        if (rem >= n) {
+            slot = next + r;
+            next += n;
+        } else {
+            uint32_t base = 0;
+            if (lane_id() == 0) base = atomicAdd(counter, kPathChunk);
+            base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+            slot = r < rem ? next + r : base + (r - rem);
+            next = base + (n - rem);
+            end = base + kPathChunk;
+        }
+        return slot;
+    }
+};
+
 // Adds per-lane event counts into a global Counters with one atomic per wave and field.
 __device__ __forceinline__ void flush_counters(Counters* dst, unsigned long long (&c)[kCounterFields]) {
 #pragma unroll
@@ -65,11 +94,12 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     LightState st;
     bool active = false, exhausted = false;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
+    WavePool pool;
     for (;;) {
         // refill finished lanes
         bool need = !active && !exhausted;
         uint64_t m;
-        uint32_t slot = wave_alloc(a.path_counter, need, &m);
+        uint32_t slot = pool.draw(a.path_counter, need);
         if (need) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
@@ -123,16 +153,15 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
                 alive = false;
             } else {
                 // Sample_Scattering_light (Vertex_Merging.cpp:48-96)
-                float u[3];
                 rng.path = st.path;
-                rng.u3(5 + 3 * st.pl, u);
+                U4 w = rng.block((uint32_t)bounce_block(STREAM_LIGHT, st.pl));
                 BsdfSample bs;
-                F3 f = bsdf_sample(b, sf.gn, u[0], u[1], u[2], bs);
+                F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
                 if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {
                     alive = false;
                 } else {
                     if (st.pl > 2) {
-                        if (rng.u(DIM_RR + (st.pl & 7)) > bs.cont) alive = false;
+                        if (u24(w.d) > bs.cont) alive = false;
                         else f = div_each(f, bs.cont);
                     }
                     if (alive) {
@@ -171,16 +200,16 @@ __device__ __forceinline__ void for_each_in_radius(const PhotonMapView& pm, F3 q
         lo[k] = (int)floorf(((qq[k] - pm.r_query) - pm.origin[k]) * pm.inv_cell);
         hi[k] = (int)floorf(((qq[k] + pm.r_query) - pm.origin[k]) * pm.inv_cell);
     }
-    uint32_t seen[27];
+    uint32_t seen[64];
     int nseen = 0;
     for (int z = lo[2]; z <= hi[2]; ++z)
         for (int y = lo[1]; y <= hi[1]; ++y)
             for (int x = lo[0]; x <= hi[0]; ++x) {
-                uint32_t bkt = cell_hash(x, y, z) & pm.mask;
+                uint32_t bkt = cell_bucket(x, y, z, pm.mask);
                 bool dup = false;
                 for (int i = 0; i < nseen; ++i) dup |= seen[i] == bkt;
                 if (dup) continue;
-                if (nseen < 27) seen[nseen++] = bkt;
+                if (nseen < 64) seen[nseen++] = bkt;
                 uint2 range = pm.cells[bkt];
                 if (kCount) {
                     ++n_cells;
@@ -217,10 +246,11 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     unsigned long long cnt[kCounterFields] = {0};
     CameraState st;
     bool active = false, exhausted = false;
+    WavePool pool;
     for (;;) {
         bool need = !active && !exhausted;
         uint64_t m;
-        uint32_t slot = wave_alloc(a.path_counter, need, &m);
+        uint32_t slot = pool.draw(a.path_counter, need);
         if (need) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
@@ -228,8 +258,9 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
                 rng.path = slot;
                 const int W = sc.W, H = sc.H;
                 // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
-                float sx = (float)(int)(slot % (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER);
-                float sy = (float)(int)(slot / (uint32_t)W) + 0.0f + 1.0f * rng.u(DIM_JITTER + 1);
+                U4 wj = rng.block(0);
+                float sx = (float)(int)(slot % (uint32_t)W) + 0.0f + 1.0f * u24(wj.a);
+                float sy = (float)(int)(slot / (uint32_t)W) + 0.0f + 1.0f * u24(wj.b);
                 F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
                 dir = dir - ld3(sc.cam_o);
                 st.o = ld3(sc.cam_o);
@@ -290,16 +321,15 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
                 alive = false;
             } else {
                 // Sample_Scattering_camera (Vertex_Merging.cpp:98-135)
-                float u[3];
                 rng.path = st.path;
-                rng.u3(3 * st.pl, u);
+                U4 w = rng.block((uint32_t)bounce_block(STREAM_CAMERA, st.pl));
                 BsdfSample bs;
-                F3 f = bsdf_sample(b, sf.gn, u[0], u[1], u[2], bs);
+                F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
                 if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {
                     alive = false;
                 } else {
                     if (st.pl > 2) {
-                        if (rng.u(DIM_RR + (st.pl & 7)) > bs.cont) alive = false;
+                        if (u24(w.d) > bs.cont) alive = false;
                         else f = div_each(f, bs.cont);
                     }
                     if (alive) {
@@ -340,39 +370,47 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
-template <bool kCount>
+// The gather grid is a multiple of kXcds blocks; the dispatcher places block b
+// on XCD b % kXcds, so XCD x takes the x-th contiguous segment of the query
+// list (queries come out of the trace in pixel order, so a segment is a band
+// of the image) and its L2 holds that band's photons.
+constexpr int kXcds = 8;
+
+template <bool kCount, bool kFast>
 __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     __shared__ uint32_t queue_lds[kGatherWaves][128];
-    __shared__ uint32_t rfirst_lds[kGatherWaves][32];  // first photon of each non-empty cell range
-    __shared__ uint32_t rbase_lds[kGatherWaves][32];   // offset of that range in the query's photon stream
+    __shared__ uint32_t rfirst_lds[kGatherWaves][64];
+    __shared__ uint32_t rbase_lds[kGatherWaves][64];
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* queue = queue_lds[wv];
-    uint32_t* rfirst = rfirst_lds[wv];
-    uint32_t* rbase = rbase_lds[wv];
     const uint32_t lane = lane_id();
     const PhotonMapView& pm = a.pm;
     unsigned long long cnt[kCounterFields] = {0};
     const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
-    const uint32_t nwaves = gridDim.x * kGatherWaves;
-    // queries are dealt to waves round-robin: no shared counter
-    for (uint32_t qi = blockIdx.x * kGatherWaves + wv; qi < nq; qi += nwaves) {
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t seg = (nq + kXcds - 1) / kXcds;
+    const uint32_t q_begin = min(nq, xcd * seg), q_end = min(nq, q_begin + seg);
+    const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
+    for (uint32_t qi = q_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; qi < q_end; qi += local_waves) {
         float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
         F3 q = f3(qp.x, qp.y, qp.z);
         float cdvm = qp.w;
         Bsdf b;
         bsdf_setup(b, a.sc.mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
-        F3 gn = f3(qg.x, qg.y, qg.z);
+        const MergeConst mc = merge_const(b, f3(qg.x, qg.y, qg.z));
         F3 c = f3s(0.0f);
         auto eval_one = [&](uint32_t i) {
             float4 p = pm.pos[i];
             float4 dt = pm.dir_tr[i];
             float2 gb = pm.tp_gb[i];
             float fwd, rev;
-            F3 e = bsdf_eval_pdf(b, f3(dt.x, dt.y, dt.z), gn, fwd, rev);
-            float w = 1.0f / (1.0f + cdvm * rev + p.w * fwd);
+            F3 e = bsdf_eval_pdf<kFast>(b, mc, f3(dt.x, dt.y, dt.z), fwd, rev);
+            float w = mdiv<kFast>(1.0f, 1.0f + cdvm * rev + p.w * fwd);
             c = c + (e * f3(dt.w, gb.x, gb.y)) * w;
         };
-        // cells overlapping the ball: at most 3 per axis; lane l < 27 takes cell l
+        // cells overlapping the ball: at most 4 per axis (cell >= 2/3 of the
+        // query radius), so lane l takes cell (l & 3, l >> 2 & 3, l >> 4); those
+        // cells never share a bucket (cell_bucket)
         int lo[3], hi[3];
         float qq[3] = {q.x, q.y, q.z};
 #pragma unroll
@@ -380,35 +418,48 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
             lo[k] = (int)floorf(((qq[k] - pm.r_query) - pm.origin[k]) * pm.inv_cell);
             hi[k] = (int)floorf(((qq[k] + pm.r_query) - pm.origin[k]) * pm.inv_cell);
         }
-        int cx = lo[0] + (int)(lane % 3u), cy = lo[1] + (int)((lane / 3u) % 3u), cz = lo[2] + (int)(lane / 9u);
-        bool valid = lane < 27u && cx <= hi[0] && cy <= hi[1] && cz <= hi[2];
-        uint32_t bkt = valid ? (cell_hash(cx, cy, cz) & pm.mask) : 0xffffffffu;
-        // two cells hashing to one bucket must not be gathered twice
-        for (int k = 0; k < 26; ++k) {
-            uint32_t other = (uint32_t)__shfl((int)bkt, k);
-            if ((uint32_t)k < lane && other == bkt) valid = false;
-        }
-        uint2 range = valid ? pm.cells[bkt] : make_uint2(0u, 0u);
+        int cx = lo[0] + (int)(lane & 3u), cy = lo[1] + (int)((lane >> 2) & 3u), cz = lo[2] + (int)(lane >> 4);
+        bool valid = cx <= hi[0] && cy <= hi[1] && cz <= hi[2];
+        uint2 range = valid ? pm.cells[cell_bucket(cx, cy, cz, pm.mask)] : make_uint2(0u, 0u);
         uint32_t len = range.y - range.x;
         uint64_t ne = __ballot(len > 0u);
-        uint32_t nr = (uint32_t)__builtin_popcountll(ne);
+        const uint32_t nr = (uint32_t)__builtin_popcountll(ne);
         uint32_t incl = wave_incl_scan(len);
-        uint32_t total = (uint32_t)__shfl((int)incl, 63);
+        const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+        // compact the non-empty ranges so that lane r holds range r
         if (len > 0u) {
             uint32_t r = lanes_below(ne);
-            rfirst[r] = range.x;
-            rbase[r] = incl - len;
-        }
-        if (kCount && lane == 0) {
-            cnt[C_CELLS] += (uint32_t)__builtin_popcountll(__ballot(valid));
-            cnt[C_TESTED] += total;
+            rfirst_lds[wv][r] = range.x;
+            rbase_lds[wv][r] = incl - len;
         }
         __builtin_amdgcn_wave_barrier();
+        const uint32_t r_first = rfirst_lds[wv][lane];
+        const uint32_t r_base = rbase_lds[wv][lane];
+        if (kCount) {
+            uint64_t lookups = __ballot(valid);
+            if (lane == 0) {
+                cnt[C_CELLS] += (uint32_t)__builtin_popcountll(lookups);
+                cnt[C_TESTED] += total;
+            }
+        }
         uint32_t qhead = 0, qcount = 0;  // wave-uniform queue state
-        uint32_t rcur = 0;               // per lane: range of its current stream position
-        auto locate = [&](uint32_t sidx) -> uint32_t {
-            while (rcur + 1 < nr && rbase[rcur + 1] <= sidx) ++rcur;
-            return rfirst[rcur] + (sidx - rbase[rcur]);
+        uint32_t next_r = 0;             // wave-uniform: next range start to apply
+        uint32_t cur_first = 0, cur_base = 0;
+        // stream position s of this lane -> photon index.  Range starts below
+        // the chunk's end are applied in increasing order with uniform
+        // read-lanes; a start is retired (next_r) once the whole chunk is past it.
+        auto locate = [&](uint32_t chunk_base, uint32_t s) -> uint32_t {
+            for (uint32_t r = next_r; r < nr; ++r) {
+                uint32_t bnd = (uint32_t)__builtin_amdgcn_readlane((int)r_base, (int)r);
+                if (bnd >= chunk_base + 64u) break;
+                uint32_t fst = (uint32_t)__builtin_amdgcn_readlane((int)r_first, (int)r);
+                if (s >= bnd) {
+                    cur_first = fst;
+                    cur_base = bnd;
+                }
+                if (bnd <= chunk_base) next_r = r + 1;
+            }
+            return cur_first + (s - cur_base);
         };
         auto push = [&](bool acc, uint32_t i) {
             uint64_t mk = __ballot(acc);
@@ -427,9 +478,9 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
         for (uint32_t sb = 0; sb < total; sb += 128) {
             // two chunks of the stream in flight per lane
             uint32_t s0 = sb + lane, s1 = sb + 64 + lane;
+            uint32_t i0 = locate(sb, s0);
+            uint32_t i1 = locate(sb + 64, s1);
             bool v0 = s0 < total, v1 = s1 < total;
-            uint32_t i0 = v0 ? locate(s0) : 0u;
-            uint32_t i1 = v1 ? locate(s1) : 0u;
             float4 p0 = v0 ? pm.pos[i0] : make_float4(0, 0, 0, 0);
             float4 p1 = v1 ? pm.pos[i1] : make_float4(0, 0, 0, 0);
             bool acc0 = v0 && len2(q - f3(p0.x, p0.y, p0.z)) < pm.r2;
@@ -485,8 +536,13 @@ void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
     else hipLaunchKernelGGL(k_camera_trace<false>, dim3(blocks), dim3(kBlock), 0, s, a);
 }
 void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s) {
-    if (a.counters) hipLaunchKernelGGL(k_gather<true>, dim3(blocks), dim3(kGatherBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_gather<false>, dim3(blocks), dim3(kGatherBlock), 0, s, a);
+    if (a.exact_merge) {
+        if (a.counters) hipLaunchKernelGGL((k_gather<true, false>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_gather<false, false>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
+    } else {
+        if (a.counters) hipLaunchKernelGGL((k_gather<true, true>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_gather<false, true>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
+    }
 }
 void launch_resolve(const CameraPassArgs& a, hipStream_t s) {
     if (a.n_paths == 0) return;
@@ -633,7 +689,7 @@ __global__ void k_map_keys(const float4* pos, const uint32_t* uid, uint32_t n, f
     int cx = (int)floorf((p.x - ox) * inv_cell);
     int cy = (int)floorf((p.y - oy) * inv_cell);
     int cz = (int)floorf((p.z - oz) * inv_cell);
-    uint32_t bkt = cell_hash(cx, cy, cz) & mask;
+    uint32_t bkt = cell_bucket(cx, cy, cz, mask);
     keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)uid[i];
     vals[i] = i;
 }

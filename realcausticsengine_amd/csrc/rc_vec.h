@@ -148,35 +148,29 @@ RC_HD U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
 RC_HD float u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
 RC_HD uint32_t pick_word(U4 w, int i) { return i == 0 ? w.a : (i == 1 ? w.b : (i == 2 ? w.c : w.d)); }
 
-enum { DIM_RR = 32, DIM_JITTER = 40 };
+// Light paths: dimensions 0..4 are the light sample, bounce k uses Philox
+// block 2 + k ({BSDF u0, u1, u2, Russian roulette}).  Camera paths:
+// dimensions 0, 1 are the pixel jitter, bounce k uses block 1 + k.
 enum { STREAM_LIGHT = 0, STREAM_CAMERA = 1 };
+RC_HD int bounce_block(int stream, int path_length) { return (stream == STREAM_LIGHT ? 2 : 1) + (path_length & 7); }
 
 struct PathRng {
     uint32_t k0, k1, path, iter, stream;
-    RC_HD float u(int d) const {
-        U4 w = philox((uint32_t)d >> 2, path, iter, stream, k0, k1);
-        return u24(pick_word(w, d & 3));
-    }
-    // three consecutive dimensions d, d+1, d+2 with at most two Philox blocks
-    RC_HD void u3(int d, float* o) const {
-        U4 w = philox((uint32_t)d >> 2, path, iter, stream, k0, k1);
-        int j = d & 3;
-        if (j <= 1) {
-            o[0] = u24(pick_word(w, j));
-            o[1] = u24(pick_word(w, j + 1));
-            o[2] = u24(pick_word(w, j + 2));
-        } else {
-            U4 w2 = philox(((uint32_t)d >> 2) + 1u, path, iter, stream, k0, k1);
-            o[0] = u24(pick_word(w, j));
-            o[1] = j == 2 ? u24(w.d) : u24(w2.a);
-            o[2] = j == 2 ? u24(w2.a) : u24(w2.b);
-        }
-    }
+    RC_HD U4 block(uint32_t b) const { return philox(b, path, iter, stream, k0, k1); }
+    RC_HD float u(int d) const { return u24(pick_word(block((uint32_t)d >> 2), d & 3)); }
 };
 
 // Spatial hash of an integer photon-map cell.
 RC_HD uint32_t cell_hash(int x, int y, int z) {
     return ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+}
+// Bucket of a cell: the low six bits are (x mod 4, y mod 4, z mod 4) and the
+// rest a hash of the enclosing 4x4x4 block, so the cells of any run of four
+// along each axis (a whole merge neighbourhood) occupy distinct buckets.
+// Needs mask >= 63.
+RC_HD uint32_t cell_bucket(int x, int y, int z, uint32_t mask) {
+    uint32_t low = (uint32_t)(x & 3) | ((uint32_t)(y & 3) << 2) | ((uint32_t)(z & 3) << 4);
+    return ((cell_hash(x >> 2, y >> 2, z >> 2) << 6) | low) & mask;
 }
 
 }  // namespace rc

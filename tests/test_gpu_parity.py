@@ -86,9 +86,11 @@ def test_gather_sets_bit_exact(oracle):
     eng.release()
 
 
-def test_camera_pass_given_photons(oracle):
+@pytest.mark.parametrize("exact", [0, 1])
+def test_camera_pass_given_photons(oracle, exact):
+    # exact = 1: correctly rounded merge arithmetic; 0: hardware rcp/sqrt (<= 2 ulp)
     args = scenes.c0_scene(48, 48, subdiv=3)
-    eng = Engine(args, photons=20000, seed=3)
+    eng = Engine(args, photons=20000, seed=3, exact_merge=exact)
     orc = oracle.OracleScene(args, photons=20000, seed=3)
     ph = orc.trace_light(0)
     img_o = orc.camera_pass(0, ph)
