@@ -28,21 +28,37 @@ PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Ver
 
 
 def algorithmic_bytes(st: dict) -> dict:
-    """Bytes per launch from the event counts (one counting iteration).
+    """Bytes per launch from the event counts of one counting iteration.
 
     traversal: 32 B per box test, two boxes per node visit; 36 B per triangle
-    test (three points); 40 B per closest hit (three normals + material);
-    photon store 40 B; gather 8 B per cell visit, 12 B per photon tested,
-    28 B per photon accepted; film 24 B per camera path; photon map: per radix
-    pass 32 B per photon (key read twice, key+value read and written), plus
-    the 88 B permute and 28 B key generation per photon."""
+    test (three points); 40 B per closest hit (three normals + material).
+    light pass: + 40 B per stored photon.  photon map: per radix pass 32 B per
+    photon, + 88 B permute and 28 B key generation per photon.  camera trace:
+    + 80 B per merge query written + 8 B per path record.  gather: 80 B per
+    query read, 8 B per cell lookup, 12 B per photon tested, 28 B per photon
+    accepted, 12 B per result.  resolve: 8 B per path, 12 B per query, 24 B
+    per film update."""
     light = (64 * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
              + PHOTON_BYTES * st["photons"])
-    camera = (64 * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
-              + 8 * st["cell_visits"] + 12 * st["photons_tested"] + 28 * st["photons_accepted"]
-              + 24 * st["camera_paths"])
+    trace = (64 * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
+             + 80 * st["gathers"] + 8 * st["camera_paths"])
+    gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
+              + 28 * st["photons_accepted"])
+    resolve = 32 * st["camera_paths"] + 12 * st["gathers"]
     photon_map = st["photons"] * (32 * st["sort_passes"] + 88 + 28)
-    return {"light": light, "map": photon_map, "camera": camera}
+    return {"light_pass": light, "photon_map": photon_map, "camera_trace": trace, "gather": gather,
+            "resolve": resolve}
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_water.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
 
 
 def cpu_baseline(args, photons: int, stride: int) -> dict:
@@ -98,24 +114,22 @@ def main() -> int:
           file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         eng.render_iteration()
-    kern = {"light": 0.0, "map": 0.0, "camera": 0.0}
-    split = {"camera_trace": 0.0, "gather": 0.0, "resolve": 0.0}
+    keys = {"light_pass": "ms_light", "photon_map": "ms_map", "camera_trace": "ms_camera_trace",
+            "gather": "ms_gather", "resolve": "ms_resolve"}
+    kern = {k: 0.0 for k in keys}
     t0 = time.perf_counter()
     for i in range(a.steps):
         eng.render_iteration()
         st = eng.stats()
-        kern["light"] += st["ms_light"]
-        kern["map"] += st["ms_map"]
-        kern["camera"] += st["ms_camera"]
-        split["camera_trace"] += st["ms_camera_trace"]
-        split["gather"] += st["ms_gather"]
-        split["resolve"] += st["ms_resolve"]
+        for k, sk in keys.items():
+            kern[k] += st[sk]
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1000.0 / a.steps
     kern = {k: v / a.steps for k, v in kern.items()}
-    split = {k: v / a.steps for k, v in split.items()}
-    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernels {kern} camera {split}", file=sys.stderr, flush=True)
-    # counting launch (not timed): event counts give the algorithmic bytes
+    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernel ms {kern}", file=sys.stderr, flush=True)
+    # counting launch (not timed) of the middle timed iteration: its event
+    # counts give the algorithmic bytes (the merge radius shrinks per iteration)
+    eng.set("next_iteration", a.warmup + a.steps // 2)
     eng.set("count_events", 1)
     eng.render_iteration()
     cst = eng.stats()
@@ -124,10 +138,10 @@ def main() -> int:
     dom = max(kern, key=kern.get)
     achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom)}
     extra = {
         "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
-        "camera_split_ms": {k: round(v, 3) for k, v in split.items()},
+        "kernel_achieved_GBs": {k: round(by[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern if kern[k] > 0},
         "algorithmic_bytes": {k: int(v) for k, v in by.items()},
         "photons_per_iteration": int(cst["photons"]),
         "gathers_per_s": cst["gathers"] / (ms_step * 1e-3),

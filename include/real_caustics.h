@@ -68,7 +68,8 @@ int main(void);
  * 0 = width*height as the reference), "write_exr", "compat_rows",
  * "compat_point_light", "compat_light_pdf", "max_path_length",
  * "photon_capacity", "cell_scale", "persistent_blocks", "count_events",
- * "exact_merge" (1 = correctly rounded division and sqrt in the merge).
+ * "exact_merge" (1 = correctly rounded division and sqrt in the merge),
+ * "next_iteration" (index of the next iteration rc_render_iteration renders).
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
 int rc_set_option_str(const char *key, const char *value);

@@ -1187,6 +1187,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
     else if (k == "count_events") g->count_events = (int)v;
     else if (k == "exact_merge") g->exact_merge = (int)v;
+    else if (k == "next_iteration") g->iterations_done = std::max(0, (int)v);
     else return -1;
     return 0;
 }
