@@ -70,6 +70,13 @@ int main(void);
  * "photon_capacity", "cell_scale", "persistent_blocks", "count_events",
  * "exact_merge" (1 = correctly rounded division and sqrt in the merge),
  * "next_iteration" (index of the next iteration rc_render_iteration renders).
+ * Learned emission proposal: "guide" (1 = sample area and sphere lights from
+ * the trained proposal, mixed with plain emission, and train it every
+ * iteration), "guide_alpha" (plain share, default 0.2), "guide_lr" (Adam,
+ * 1e-3), "guide_batch" (training paths per iteration, 65536), "guide_cells"
+ * (position cells per light side, multiple of 8, default 16), "guide_warmup"
+ * (Adam steps before the proposal is sampled, default 2), "guide_train"
+ * (0 = keep the network fixed), "guide_seed", "guide_reset".
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
 int rc_set_option_str(const char *key, const char *value);
@@ -103,6 +110,10 @@ enum {
     RC_STAT_SORT_PASSES,
     /* camera pass split: trace (includes the query-count readback), merge, resolve */
     RC_STAT_MS_CAMERA_TRACE, RC_STAT_MS_GATHER, RC_STAT_MS_RESOLVE,
+    /* learned proposal (option "guide"): training + table time, mean visible
+     * luminance per training path, share of training paths with visible
+     * photons, weighted cross-entropy of the step */
+    RC_STAT_MS_GUIDE, RC_STAT_GUIDE_VISIBLE, RC_STAT_GUIDE_HIT_FRACTION, RC_STAT_GUIDE_LOSS,
     RC_STAT_COUNT
 };
 
@@ -123,6 +134,30 @@ long long rc_gather_sets(const float *positions, long long count, const float *q
 /* Camera pass of one iteration against a given photon list (10 floats each,
  * as rc_trace_photons); adds into image (width*height*3). */
 int rc_camera_pass_photons(int iteration, const float *photons, long long count, float *image);
+
+/* ---- learned emission proposal (no counterpart in the reference) ----
+ * Parameters are one flat float vector of RC_GUIDE_PARAMS entries: W1
+ * [192][64], b1[64], W2 [64][64], b2, W3 [64][64], b3, W4 [64][1024], b4[1024]
+ * (weights row-major [in][out]).  An input row is 6 floats: the emission point
+ * in scene-box coordinates and the emission axis.  Calls need rc_prepare. */
+enum { RC_GUIDE_PARAMS = 87232, RC_GUIDE_BINS = 1024 };
+/* Copies the FP32 master parameters out / in (setting resets Adam). */
+int rc_guide_get_params(float *params, long long n);
+int rc_guide_set_params(const float *params, long long n);
+/* Logits (n x 1024) of n input rows through the BF16 matrix-core network. */
+int rc_guide_forward(const float *x, long long n, float *logits);
+/* One training step on n rows with target bins (-1 = none) and weights:
+ * writes the loss sum_i w_i CE_i / sum_i w_i and the gradient (RC_GUIDE_PARAMS
+ * floats, either may be null); apply != 0 also takes the Adam step. */
+int rc_guide_train_step(const float *x, const int *bins, const float *w, long long n, float *grad, float *loss,
+                        int apply);
+/* The proposal table: rows = lights with a table x cells^2, each the CDF of
+ * 1025 floats over the bins; inputs = the rows' 6-float input records.
+ * Returns the row count (nothing beyond max_rows is written). */
+long long rc_guide_table(float *cdf, float *inputs, long long max_rows);
+/* Training records of the last light pass: 6 floats and the bin of each of the
+ * first n paths (bin -1: no guidable emission).  Returns the count written. */
+long long rc_guide_records(float *x, int *bins, long long n);
 
 #ifdef __cplusplus
 }

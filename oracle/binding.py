@@ -61,6 +61,8 @@ def load(precision: str = "f32") -> ctypes.CDLL:
     lib.orc_bsdf.argtypes = [_fp, ctypes.c_int, _fp, _fp, _fp, _fp, _fp, ctypes.c_int, _dp]
     lib.orc_light_sample.argtypes = [ctypes.c_void_p, _fp, _dp]
     lib.orc_counters.argtypes = [ctypes.c_void_p, _llp]
+    lib.orc_set_guide.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double]
+    lib.orc_set_guide.restype = ctypes.c_int
     _libs[precision] = lib
     return lib
 
@@ -150,6 +152,17 @@ class OracleScene:
         out = np.zeros(9, dtype=np.float64)
         self.lib.orc_light_sample(self.h, _ptr(u, _fp), _ptr(out, _dp))
         return out
+
+    def set_guide(self, cdf, slots: int, cells: int, slot_of_light, alpha: float) -> None:
+        """Emission from the engine's learned proposal table (rows of 1025-float CDFs); cdf=None: plain."""
+        if cdf is None:
+            self.lib.orc_set_guide(self.h, None, 0, 0, None, 0.0)
+            return
+        self._guide_cdf = np.ascontiguousarray(cdf, dtype=np.float32)
+        sl = np.full(8, -1, dtype=np.int32)
+        sl[:len(slot_of_light)] = slot_of_light
+        if self.lib.orc_set_guide(self.h, _ptr(self._guide_cdf, _fp), slots, cells, _ptr(sl, _ip), alpha) != 0:
+            raise MemoryError("orc_set_guide")
 
     def __del__(self):
         try:

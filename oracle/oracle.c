@@ -922,6 +922,8 @@ static V project_onto_plane(V point, V center, V normal) {
 /* ------------------------------------------------------------------ */
 /* Scene                                                               */
 /* ------------------------------------------------------------------ */
+enum { GUIDE_BINS = 1024, GUIDE_MAX_LIGHTS = 8 };
+
 typedef struct {
     int samples;
     real base_radius, radius_alpha;
@@ -950,6 +952,11 @@ typedef struct {
     Sampler serial_smp; /* the iteration's MLT_Sampler, shared by both passes */
     Counters ctr;
     long long visible_paths;
+    /* learned emission proposal: per-cell CDFs given by the caller */
+    real *guide_cdf;            /* [slot][cell][GUIDE_BINS + 1] */
+    int guide_cells, guide_slot[GUIDE_MAX_LIGHTS];
+    real guide_alpha;
+    uint32_t guide_words[4];    /* this path's {dim 3, dim 4, mixture, bin} words */
 } Scene;
 
 void *orc_create(int samples, float base_radius, float radius_reduction, const long long *meshes_pointers,
@@ -1066,7 +1073,7 @@ void orc_free(void *sp) {
         free(s->meshes[i].nodes); free(s->meshes[i].tris_indices);
     }
     free(s->meshes); free(s->wnodes); free(s->mesh_indices); free(s->mats);
-    free(s->py_lights); free(s->lights); free(s->cdf);
+    free(s->py_lights); free(s->lights); free(s->cdf); free(s->guide_cdf);
     free(s);
 }
 
@@ -1280,18 +1287,45 @@ static int find_interval(const real *cdf, int size, real u) {
     return r;
 }
 
+/* Guided direction (the engine's learned proposal): with probability alpha
+ * the plain words, otherwise a bin from the CDF of the position cell (cx, cy)
+ * and a point inside it; returns q = alpha + (1 - alpha) * 1024 * P(bin). */
+static real guide_direction(const Scene *s, int light, real ax, real ay, real *ex, real *ey) {
+    uint32_t wx = s->guide_words[0], wy = s->guide_words[1];
+    int G = s->guide_cells;
+    int cx = (int)(ax * (real)G), cy = (int)(ay * (real)G);
+    if (cx > G - 1) cx = G - 1;
+    if (cy > G - 1) cy = G - 1;
+    const real *cdf = s->guide_cdf + ((size_t)(s->guide_slot[light] * G + cy) * G + cx) * (GUIDE_BINS + 1);
+    int bin = (int)(wx >> 27) | ((int)(wy >> 27) << 5);
+    if (u24(s->guide_words[2]) >= s->guide_alpha) {
+        real uc = (real)((s->guide_words[3] >> 8) + 1u) * K(5.9604644775390625e-08);
+        int k = find_interval(cdf, GUIDE_BINS + 1, uc);
+        wx = ((uint32_t)(k & 31) << 27) | (wx >> 5);
+        wy = ((uint32_t)(k >> 5) << 27) | (wy >> 5);
+        bin = k;
+    }
+    *ex = u24(wx);
+    *ey = u24(wy);
+    real p = cdf[bin + 1] - cdf[bin];
+    return s->guide_alpha + (K(1.0f) - s->guide_alpha) * (real)GUIDE_BINS * p;
+}
+
 /* Lights_list::sample_lights (Light_list.cpp:10-30) and the per-light sample_light. */
 static void sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
     real rng = get1d(smp);
     int idx = find_interval(s->cdf, (int)s->nlights + 1, rng);
     const Light *L = &s->lights[idx];
     real pick = s->compat_pdf ? L->power / ((real)(int)s->nlights * s->integral) : L->power / s->integral;
+    int guided = s->guide_cdf && idx < GUIDE_MAX_LIGHTS && s->guide_slot[idx] >= 0;
+    real q = K(1.0f);
     V t;
     if (L->kind == L_AREA) {
         /* Area_Light::sample_light (Lights.cpp:41-55) */
         real ax = get1d(smp), ay = get1d(smp);
         *origin = vadd(vadd(L->bl, vscale(L->h, ax)), vscale(L->v, ay));
         real ex = get1d(smp), ey = get1d(smp);
+        if (guided) q = guide_direction(s, idx, ax, ay, &ex, &ey);
         V local = sample_cos_hemisphere(ex, ey);
         *dir = to_world(&L->fr, local);
         t = vscale(L->color, PI);
@@ -1318,6 +1352,7 @@ static void sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
         V n = v3(rr * rcos(phi), rr * rsin(phi), z);
         real e1 = get1d(smp), e2 = get1d(smp);
         if (L->radius > 0) {
+            if (guided) q = guide_direction(s, idx, u1, u2, &e1, &e2);
             *origin = vadd(L->pos, vscale(n, L->radius));
             Frame fr = frame_from_z(n);
             *dir = to_world(&fr, sample_cos_hemisphere(e1, e2));
@@ -1328,6 +1363,23 @@ static void sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
         t = vscale(vscale(L->color, L->power), PI);
     }
     *tp = vdiv_each(t, pick);
+    if (q != K(1.0f)) *tp = vdiv_each(*tp, q);
+}
+
+int orc_set_guide(void *sp, const float *cdf, int slots, int cells, const int *slot_of_light, double alpha) {
+    Scene *s = (Scene *)sp;
+    free(s->guide_cdf);
+    s->guide_cdf = NULL;
+    if (!cdf || slots <= 0) return 0;
+    size_t n = (size_t)slots * cells * cells * (GUIDE_BINS + 1);
+    s->guide_cdf = (real *)malloc(n * sizeof(real));
+    if (!s->guide_cdf) return -1;
+    for (size_t i = 0; i < n; ++i) s->guide_cdf[i] = (real)cdf[i];
+    s->guide_cells = cells;
+    for (int i = 0; i < GUIDE_MAX_LIGHTS; ++i) s->guide_slot[i] = slot_of_light[i];
+    /* alpha on the 2^-24 grid of the uniform draws */
+    s->guide_alpha = (real)(float)(ceil(alpha * 16777216.0) / 16777216.0);
+    return 0;
 }
 
 void orc_light_sample(void *sp, const float *u5, double *out) {
@@ -1425,6 +1477,16 @@ static void light_pass(Scene *s, int iteration, PList *out) {
             dr.key.path = (uint32_t)p;
             Sampler smp;
             sampler_plain(&smp, &dr.key);
+            if (s->guide_cdf) {
+                uint32_t c0[4] = {0u, (uint32_t)p, (uint32_t)iteration, STREAM_LIGHT}, c1[4] = {1u, (uint32_t)p,
+                    (uint32_t)iteration, STREAM_LIGHT}, w0[4], w1[4];
+                orc_philox(c0, dr.key.key, w0);
+                orc_philox(c1, dr.key.key, w1);
+                s->guide_words[0] = w0[3];
+                s->guide_words[1] = w1[0];
+                s->guide_words[2] = w1[1];
+                s->guide_words[3] = w1[2];
+            }
             s->visible_paths += trace_light_path(s, &smp, &dr, (uint32_t)p, out);
         }
         return;

@@ -108,6 +108,10 @@ void orc_bsdf(const float *mat, int mode, const float *shade_n, const float *geo
 
 /* One light emission sample: u5 = 5 primary samples; out = origin.xyz, dir.xyz, throughput.rgb. */
 void orc_light_sample(void *scene, const float *u5, double *out);
+/* Learned emission proposal of the engine: per-cell CDFs (slots * cells^2
+ * rows of 1025 floats), the slot of each of the first 8 lights (-1: plain)
+ * and the plain share alpha.  cdf = NULL switches it off. */
+int orc_set_guide(void *scene, const float *cdf, int slots, int cells, const int *slot_of_light, double alpha);
 
 /* Traversal counters of the last call: box tests, triangle tests. */
 void orc_counters(void *scene, long long *out3);

@@ -17,8 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libRealCaustics.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rc_kernels.hip", "rc_engine.cpp"]
-HEADERS = ["rc_device.h", "rc_engine.h", "rc_vec.h"]
+SOURCES = ["rc_kernels.hip", "rc_guide.hip", "rc_engine.cpp", "rc_guide_host.cpp"]
+HEADERS = ["rc_device.h", "rc_engine.h", "rc_guide.h", "rc_guide_model.h", "rc_host.h", "rc_vec.h"]
 
 COMMON = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
@@ -38,14 +38,17 @@ def build(verbose: bool = False) -> str:
     """Compile every source (always) and link libRealCaustics.so; returns its path."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     cc = hipcc()
-    objs = []
-    for src in SOURCES:
+    objs, procs = [], []
+    for src in SOURCES:  # the translation units compile in parallel
         obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
         cmd = [cc, *COMMON, "-I", os.path.join(HERE, "..", "include"), "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        procs.append((src, subprocess.Popen(cmd)))
         objs.append(obj)
+    failed = [src for src, p in procs if p.wait() != 0]
+    if failed:
+        raise RuntimeError("hipcc failed on " + ", ".join(failed))
     tmp = LIB + ".tmp"
     cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:

@@ -372,22 +372,56 @@ __device__ __forceinline__ int find_interval(const float* cdf, int size, float u
     return r;
 }
 
+// Guided direction of light `light` (rc_guide.h): with probability alpha
+// the plain words, otherwise a bin from the CDF of the position cell and a
+// uniform point inside it (the bin's index goes into the top five bits of
+// each word, the old words' top 19 bits below).  Returns q, the mixture
+// density relative to plain sampling; `bin` is the sampled bin either way.
+__device__ __forceinline__ float guide_direction(const GuideView& g, int light, float ax, float ay, const U4& w1,
+                                                 uint32_t& wx, uint32_t& wy, int& bin) {
+    bin = (int)(wx >> 27) | ((int)(wy >> 27) << 5);
+    if (!g.cdf) return 1.0f;
+    const int G = g.cells;
+    int cx = (int)(ax * (float)G), cy = (int)(ay * (float)G);
+    cx = cx > G - 1 ? G - 1 : cx;
+    cy = cy > G - 1 ? G - 1 : cy;
+    const float* cdf = g.cdf + ((size_t)(g.slot[light] * G + cy) * G + cx) * kGuideCdfStride;
+    if (u24(w1.b) >= g.alpha) {
+        float uc = (float)((w1.c >> 8) + 1u) * 5.9604644775390625e-08f;  // (0, 1]
+        int k = find_interval(cdf, kGuideBins + 1, uc);
+        wx = ((uint32_t)(k & 31) << 27) | (wx >> 5);
+        wy = ((uint32_t)(k >> 5) << 27) | (wy >> 5);
+        bin = k;
+    }
+    float p = cdf[bin + 1] - cdf[bin];
+    return g.alpha + (1.0f - g.alpha) * (float)kGuideBins * p;
+}
+
 // Returns false when the light emits nothing (the reference point light).
-__device__ bool sample_light(const SceneView& sc, const PathRng& rng, F3& o, F3& d, F3& tp) {
+// bin / axis: the direction bin and emission axis of a guidable emission
+// (area and sphere lights), bin = -1 otherwise.
+__device__ bool sample_light(const SceneView& sc, const PathRng& rng, const GuideView& gv, F3& o, F3& d, F3& tp,
+                             int& bin, F3& axis) {
     U4 w0 = rng.block(0);
     float u[5] = {u24(w0.a), u24(w0.b), u24(w0.c), u24(w0.d), 0.0f};
     float rnd = u[0];
     int idx = find_interval(sc.cdf, sc.nlights + 1, rnd);
     const DevLight& L = sc.lights[idx];
     float pick = sc.compat_pdf ? L.power / ((float)sc.nlights * sc.integral) : L.power / sc.integral;
+    const bool guided = idx < kGuideMaxLights && gv.slot[idx] >= 0;
+    float q = 1.0f;
+    bin = -1;
     F3 t;
     if (L.kind == L_AREA) {
         float ax = u[1], ay = u[2];
         o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
-        u[4] = u24(rng.block(1).a);
-        F3 local = cos_hemisphere(u[3], u[4]);
+        U4 w1 = rng.block(1);
+        uint32_t wx = w0.d, wy = w1.a;
+        if (guided) q = guide_direction(gv, idx, ax, ay, w1, wx, wy, bin);
+        F3 local = cos_hemisphere(u24(wx), u24(wy));
         Frame fr{ld3(L.fx), ld3(L.fy), ld3(L.fz)};
         d = to_world(fr, local);
+        axis = ld3(L.fz);
         t = ld3(L.color) * kPI;
     } else if (L.kind == L_SUN) {
         float ax = u[1], ay = u[2];
@@ -401,11 +435,14 @@ __device__ bool sample_light(const SceneView& sc, const PathRng& rng, F3& o, F3&
         float rr = sqrtf(smax(0.0f, 1.0f - z * z));
         float phi = kPI2 * u2;
         F3 n = f3(rr * own_cos(phi), rr * own_sin(phi), z);
-        float e1 = u[3], e2 = u24(rng.block(1).a);
+        U4 w1 = rng.block(1);
+        uint32_t wx = w0.d, wy = w1.a;
         if (L.radius > 0.0f) {
+            if (guided) q = guide_direction(gv, idx, u1, u2, w1, wx, wy, bin);
             o = ld3(L.pos) + n * L.radius;
             Frame fr = frame_from_z(n);
-            d = to_world(fr, cos_hemisphere(e1, e2));
+            d = to_world(fr, cos_hemisphere(u24(wx), u24(wy)));
+            axis = n;
         } else {
             o = ld3(L.pos);
             d = n;
@@ -413,6 +450,7 @@ __device__ bool sample_light(const SceneView& sc, const PathRng& rng, F3& o, F3&
         t = (ld3(L.color) * L.power) * kPI;
     }
     tp = div_each(t, pick);
+    if (q != 1.0f) tp = div_each(tp, q);
     return true;
 }
 

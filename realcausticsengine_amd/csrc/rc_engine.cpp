@@ -16,55 +16,22 @@
 
 #include "../../include/real_caustics.h"
 #include "rc_engine.h"
+#include "rc_guide_model.h"
+#include "rc_host.h"
 #include "rc_vec.h"
 
 using namespace rc;
 
-namespace {
-
-// ---------------------------------------------------------------------
-// errors
-// ---------------------------------------------------------------------
+namespace rc {
 std::string g_error;
-std::atomic<int> g_cancel{0};
-
 void set_error(const std::string& e) {
     if (g_error.empty()) g_error = e;
 }
+}  // namespace rc
 
-#define HIP_OK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t err_ = (expr);                                                             \
-        if (err_ != hipSuccess) {                                                             \
-            set_error(std::string("HIP error: ") + hipGetErrorString(err_) + " at " #expr); \
-            return false;                                                                     \
-        }                                                                                     \
-    } while (0)
+namespace {
 
-template <typename T>
-struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    bool alloc(size_t count) {
-        if (count <= n && p) return true;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (count == 0) count = 1;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
-            p = nullptr;
-            set_error("hipMalloc failed");
-            return false;
-        }
-        n = count;
-        return true;
-    }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
+std::atomic<int> g_cancel{0};
 
 struct HostLight {
     char type;
@@ -132,6 +99,9 @@ struct Engine {
     double stats[RC_STAT_COUNT] = {0};
     hipStream_t stream = nullptr;
     int device_cus = 0;
+    uint32_t last_queries = 0;
+    GuideModel guide;  // learned emission proposal (option "guide")
+    hipEvent_t ev_guide[2] = {};
 };
 
 Engine* g = nullptr;
@@ -500,6 +470,7 @@ bool ensure_device(Engine& e) {
     e.device_cus = prop.multiProcessorCount;
     HIP_OK(hipStreamCreate(&e.stream));
     for (auto& ev : e.ev) HIP_OK(hipEventCreate(&ev));
+    for (auto& ev : e.ev_guide) HIP_OK(hipEventCreate(&ev));
     return true;
 }
 
@@ -603,6 +574,7 @@ bool prepare(Engine& e) {
     if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
     HIP_OK(hipMemset(e.image.p, 0, (size_t)e.W * e.H * 3 * sizeof(float)));
     if (!e.counters.alloc(4) || !e.stats_dev.alloc(2)) return false;
+    if (!e.guide.setup_scene(e.lights, e.compat_point, e.bbox.lo, e.bbox.hi)) return false;
     e.iterations_done = 0;
     e.prepared = true;
     return true;
@@ -641,6 +613,33 @@ bool ensure_photon_buffers(Engine& e, uint32_t cap) {
            e.hist.alloc(photon_map_hist_entries(cap)) && e.hist_sums.alloc(photon_map_sum_entries(cap));
 }
 
+// Training paths of one iteration: the first guide_batch paths, a multiple of 64.
+int guide_rows(const Engine& e) {
+    long long n = std::min<long long>(e.guide.batch, light_paths(e));
+    return (int)(n - n % 64);
+}
+
+bool guide_active(const Engine& e) { return e.guide.enabled && e.guide.slots > 0; }
+
+GuideView guide_light_view(Engine& e) {
+    if (!guide_active(e)) return plain_guide_view();
+    GuideView v = e.guide.view(e.guide.train != 0, e.guide.step >= e.guide.warmup);
+    v.train_paths = e.guide.train ? (uint32_t)guide_rows(e) : 0u;
+    return v;
+}
+
+bool ensure_guide_buffers(Engine& e) {
+    GuideModel& gd = e.guide;
+    if (!guide_active(e)) return true;
+    if (gd.cells_built != gd.cells &&
+        !gd.setup_scene(e.lights, e.compat_point, e.bbox.lo, e.bbox.hi))
+        return false;
+    int n = std::max(64, guide_rows(e));
+    if (!gd.params_ready && !gd.init_params(e.stream)) return false;
+    if (!gd.table_ready && gd.step >= gd.warmup && !gd.build_table(e.stream)) return false;
+    return gd.train_x.alloc((size_t)n * kGuideRowFloats) && gd.train_bin.alloc(n) && gd.train_w.alloc(n);
+}
+
 // Light pass into the unsorted photon buffers; returns the photon count.
 bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
     long long N = light_paths(e);
@@ -650,6 +649,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
     }
     uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
     if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
+    if (!ensure_guide_buffers(e)) return false;
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
@@ -668,6 +668,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.out_tp = e.ph_tp.p;
         a.out_uid = e.ph_uid.p;
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
+        a.guide = guide_light_view(e);
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
         launch_light_pass(a, blocks_for(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
@@ -782,6 +783,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
         HIP_OK(hipMemcpyAsync(&nq, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
         HIP_OK(hipStreamSynchronize(e.stream));
         e.stats[RC_STAT_GATHERS] = nq;
+        e.last_queries = nq;
         if (nq <= e.q_capacity) break;
         if (attempt == 1 || !ensure_query_buffers(e, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), n_paths)) {
             set_error("merge query buffer overflow");
@@ -798,6 +800,27 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     return true;
 }
 
+// One training step of the emission proposal on this iteration's first
+// paths: camera-visible merge cells from the queries, per-path visible
+// luminance, forward, backward, Adam, then the table for the next iteration.
+bool guide_iteration(Engine& e, uint32_t photon_count, const MapInfo& mi) {
+    GuideModel& gd = e.guide;
+    const int n = guide_rows(e);
+    if (n <= 0) return true;
+    hipStream_t s = e.stream;
+    const PhotonMapView& pm = mi.pm;
+    size_t buckets = (size_t)pm.mask + 1;
+    if (!gd.vis.alloc(buckets + 3)) return false;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(gd.vis.p), 0u, (buckets + 3) / 4, s);
+    launch_mark_visible(e.q_pos.p, std::min(e.last_queries, e.q_capacity), pm.origin[0], pm.origin[1], pm.origin[2],
+                        pm.inv_cell, pm.mask, gd.vis.p, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(gd.train_w.p), 0u, (size_t)n, s);
+    launch_guide_targets(e.ph_pos.p, e.ph_dir.p, e.ph_tp.p, e.ph_uid.p, photon_count, pm.origin[0], pm.origin[1],
+                         pm.origin[2], pm.inv_cell, pm.mask, gd.vis.p, (uint32_t)n, gd.train_w.p, s);
+    return gd.forward(gd.train_x.p, n, s) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) && gd.adam(s) &&
+           gd.build_table(s);
+}
+
 double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -811,6 +834,13 @@ bool render_iteration(Engine& e, int iteration) {
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
     if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
+    const bool training = guide_active(e) && e.guide.train;
+    if (training) {
+        HIP_OK(hipEventRecord(e.ev_guide[0], e.stream));
+        if (!guide_iteration(e, count, mi)) return false;
+        HIP_OK(hipEventRecord(e.ev_guide[1], e.stream));
+        HIP_OK(hipMemcpyAsync(e.guide.last, e.guide.scal.p, sizeof e.guide.last, hipMemcpyDeviceToHost, e.stream));
+    }
     Counters c[2] = {};
     if (e.count_events) HIP_OK(hipMemcpyAsync(c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
     HIP_OK(hipStreamSynchronize(e.stream));
@@ -823,6 +853,15 @@ bool render_iteration(Engine& e, int iteration) {
     e.stats[RC_STAT_MS_CAMERA_TRACE] = ms_trace;
     e.stats[RC_STAT_MS_GATHER] = ms_gather;
     e.stats[RC_STAT_MS_RESOLVE] = ms_resolve;
+    if (training) {
+        float ms_guide = 0;
+        HIP_OK(hipEventElapsedTime(&ms_guide, e.ev_guide[0], e.ev_guide[1]));
+        int n = std::max(1, guide_rows(e));
+        e.stats[RC_STAT_MS_GUIDE] = ms_guide;
+        e.stats[RC_STAT_GUIDE_VISIBLE] = e.guide.last[0] / n;
+        e.stats[RC_STAT_GUIDE_HIT_FRACTION] = e.guide.last[1] / n;
+        e.stats[RC_STAT_GUIDE_LOSS] = e.guide.last[2];
+    }
     e.stats[RC_STAT_LIGHT_PATHS] = (double)light_paths(e);
     e.stats[RC_STAT_PHOTONS] = count;
     e.stats[RC_STAT_CAMERA_PATHS] = (double)e.W * e.H;
@@ -949,6 +988,9 @@ void release(Engine*& e) {
     e->d_cdf.release();
     for (auto& ev : e->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : e->ev_guide)
+        if (ev) (void)hipEventDestroy(ev);
+    e->guide.release();
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     e = nullptr;
@@ -998,6 +1040,16 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         e->persistent_blocks = prev->persistent_blocks;
         e->exact_merge = prev->exact_merge;
         e->output_path = prev->output_path;
+        const GuideModel& pg = prev->guide;  // guide options, not its state
+        GuideModel& ng = e->guide;
+        ng.enabled = pg.enabled;
+        ng.train = pg.train;
+        ng.alpha = pg.alpha;
+        ng.lr = pg.lr;
+        ng.batch = pg.batch;
+        ng.cells = pg.cells;
+        ng.warmup = pg.warmup;
+        ng.seed = pg.seed;
         release(prev);
     }
     g = e;
@@ -1188,7 +1240,18 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "count_events") g->count_events = (int)v;
     else if (k == "exact_merge") g->exact_merge = (int)v;
     else if (k == "next_iteration") g->iterations_done = std::max(0, (int)v);
-    else return -1;
+    else if (k == "guide") g->guide.enabled = (int)v;
+    else if (k == "guide_alpha") g->guide.alpha = (float)std::min(1.0, std::max(1e-3, v));
+    else if (k == "guide_lr") g->guide.lr = (float)v;
+    else if (k == "guide_batch") g->guide.batch = (int)std::max(0.0, std::min(v, 1048576.0));
+    else if (k == "guide_cells") g->guide.cells = (int)v;
+    else if (k == "guide_warmup") g->guide.warmup = std::max(0, (int)v);
+    else if (k == "guide_train") g->guide.train = (int)v;
+    else if (k == "guide_seed") g->guide.seed = (uint64_t)v;
+    else if (k == "guide_reset") {
+        g->guide.params_ready = g->guide.table_ready = false;
+        g->guide.step = 0;
+    } else return -1;
     return 0;
 }
 
@@ -1213,6 +1276,11 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "nodes") return g->n_nodes;
     if (k == "device_cus") return g->device_cus;
     if (k == "capacity") return g->capacity;
+    if (k == "guide") return g->guide.enabled;
+    if (k == "guide_step") return g->guide.step;
+    if (k == "guide_slots") return g->guide.slots;
+    if (k == "guide_table_rows") return g->guide.table_rows;
+    if (k == "guide_alpha") return g->guide.alpha;
     return NAN;
 }
 
@@ -1440,6 +1508,135 @@ __attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration,
     img.release();
     if (!ok) set_error("rc_camera_pass_photons failed");
     return ok ? 0 : -1;
+}
+
+// ---- learned emission proposal ----
+static GuideModel* guide_ready() {
+    g_error.clear();
+    if (!g || !prepare(*g)) return nullptr;
+    GuideModel& gd = g->guide;
+    if (!gd.params_ready && !gd.init_params(g->stream)) return nullptr;
+    return &gd;
+}
+
+// Uploads n rows of 6 floats as 8-float records, padded with zero rows to a multiple of 64.
+static bool upload_rows(GuideModel& gd, const float* x, long long n, int padded) {
+    std::vector<float> rec((size_t)padded * kGuideRowFloats, 0.0f);
+    for (long long i = 0; i < n; ++i)
+        for (int k = 0; k < kGuideIn; ++k) rec[(size_t)i * kGuideRowFloats + k] = x[i * kGuideIn + k];
+    if (!gd.train_x.alloc(rec.size())) return false;
+    HIP_OK(hipMemcpy(gd.train_x.p, rec.data(), rec.size() * sizeof(float), hipMemcpyHostToDevice));
+    return true;
+}
+
+__attribute__((visibility("default"))) int rc_guide_get_params(float* params, long long n) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    if (!params || n != kGuideParams) {
+        set_error("rc_guide_get_params: n must be RC_GUIDE_PARAMS");
+        return -1;
+    }
+    return gd->get_params(params, g->stream) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_guide_set_params(const float* params, long long n) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    if (!params || n != kGuideParams) {
+        set_error("rc_guide_set_params: n must be RC_GUIDE_PARAMS");
+        return -1;
+    }
+    return gd->set_params(params, g->stream) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_guide_forward(const float* x, long long n, float* logits) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    if (!x || !logits || n <= 0 || n > (1 << 20)) {
+        set_error("rc_guide_forward: 1 <= n <= 2^20 rows and non-null buffers required");
+        return -1;
+    }
+    int padded = (int)((n + 63) / 64 * 64);
+    bool ok = upload_rows(*gd, x, n, padded) && gd->forward(gd->train_x.p, padded, g->stream) &&
+              hipMemcpyAsync(logits, gd->Z.p, (size_t)n * kGuideBins * sizeof(float), hipMemcpyDeviceToHost,
+                             g->stream) == hipSuccess &&
+              hipStreamSynchronize(g->stream) == hipSuccess;
+    gd->table_ready = false;  // the work buffers were reused
+    if (!ok) set_error("rc_guide_forward failed");
+    return ok ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_guide_train_step(const float* x, const int* bins, const float* w,
+                                                              long long n, float* grad, float* loss, int apply) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    if (!x || !bins || !w || n <= 0 || n > (1 << 20)) {
+        set_error("rc_guide_train_step: 1 <= n <= 2^20 rows and non-null inputs required");
+        return -1;
+    }
+    for (long long i = 0; i < n; ++i)
+        if (bins[i] < -1 || bins[i] >= kGuideBins || !(w[i] >= 0.0f)) {
+            set_error("rc_guide_train_step: bins must be in [-1, 1023] and weights >= 0");
+            return -1;
+        }
+    int padded = (int)((n + 63) / 64 * 64);
+    std::vector<int> b(padded, -1);
+    std::vector<float> ww(padded, 0.0f);
+    std::copy(bins, bins + n, b.begin());
+    std::copy(w, w + n, ww.begin());
+    bool ok = upload_rows(*gd, x, n, padded) && gd->train_bin.alloc(padded) && gd->train_w.alloc(padded) &&
+              hipMemcpy(gd->train_bin.p, b.data(), padded * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(gd->train_w.p, ww.data(), padded * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+              gd->forward(gd->train_x.p, padded, g->stream) &&
+              gd->backward(gd->train_bin.p, gd->train_w.p, padded, g->stream) && (!apply || gd->adam(g->stream));
+    float sc[4] = {0, 0, 0, 0};
+    ok = ok && hipMemcpyAsync(sc, gd->scal.p, sizeof sc, hipMemcpyDeviceToHost, g->stream) == hipSuccess;
+    if (ok && grad)
+        ok = hipMemcpyAsync(grad, gd->grad.p, sizeof(float) * kGuideParams, hipMemcpyDeviceToHost, g->stream) ==
+             hipSuccess;
+    ok = ok && hipStreamSynchronize(g->stream) == hipSuccess;
+    gd->table_ready = false;
+    if (ok && loss) *loss = sc[2];
+    if (!ok) set_error("rc_guide_train_step failed");
+    return ok ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) long long rc_guide_table(float* cdf, float* inputs, long long max_rows) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    if (!gd->table_ready && !gd->build_table(g->stream)) return -1;
+    long long rows = std::min<long long>(gd->table_rows, std::max(0ll, max_rows));
+    std::vector<float> t((size_t)rows * kGuideCdfStride);
+    if (rows > 0 && cdf) {
+        if (hipMemcpy(t.data(), gd->cdf.p, t.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error("rc_guide_table: copy failed");
+            return -1;
+        }
+        for (long long r = 0; r < rows; ++r)
+            std::copy(t.begin() + r * kGuideCdfStride, t.begin() + r * kGuideCdfStride + kGuideBins + 1,
+                      cdf + r * (kGuideBins + 1));
+    }
+    if (inputs)
+        for (long long r = 0; r < rows; ++r)
+            for (int k = 0; k < kGuideIn; ++k) inputs[r * kGuideIn + k] = gd->table_inputs[r * kGuideRowFloats + k];
+    return gd->table_rows;
+}
+
+__attribute__((visibility("default"))) long long rc_guide_records(float* x, int* bins, long long n) {
+    GuideModel* gd = guide_ready();
+    if (!gd) return -1;
+    long long avail = std::min<long long>(n, std::min<long long>(gd->train_bin.n, guide_rows(*g)));
+    if (avail <= 0 || !gd->train_x.p) return 0;
+    std::vector<float> rec((size_t)avail * kGuideRowFloats);
+    if (hipMemcpy(rec.data(), gd->train_x.p, rec.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess ||
+        (bins && hipMemcpy(bins, gd->train_bin.p, avail * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)) {
+        set_error("rc_guide_records: copy failed");
+        return -1;
+    }
+    if (x)
+        for (long long i = 0; i < avail; ++i)
+            for (int k = 0; k < kGuideIn; ++k) x[i * kGuideIn + k] = rec[i * kGuideRowFloats + k];
+    return avail;
 }
 
 }  // extern "C"

@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "rc_guide.h"
+
 namespace rc {
 
 enum { BK_SPEC_REFL = 0, BK_FRESNEL_SPEC = 1, BK_FMG = 2, BK_FSMG = 3, BK_MRG = 4 };
@@ -85,6 +87,7 @@ struct LightPassArgs {
     float2* out_tp;
     uint32_t* out_uid;
     Counters* counters;  // non-null selects the counting variant
+    GuideView guide;     // learned emission proposal (all slots -1: plain)
 };
 
 // One merge query per non-delta camera vertex (80 B, SoA):

@@ -1,0 +1,361 @@
+// rc_guide.hip -- gfx950 kernels of the learned emission proposal: Fourier
+// encoding, an LDS-tiled BF16 matrix-core product with fused epilogues (the
+// forward layers, the backward data products and the split-K weight
+// gradients), softmax cross-entropy, Adam, the proposal table (softmax ->
+// CDF per position cell) and the training targets.
+#include "rc_guide.h"
+#include "rc_vec.h"
+
+namespace rc {
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+__constant__ float c_freq[kGuideFreqs] = {
+    3.1415927410125732f, 4.442883014678955f,  6.2831854820251465f, 8.88576602935791f,
+    12.566370964050293f, 17.77153205871582f,  25.132741928100586f, 35.54306411743164f,
+    50.26548385620117f,  71.08612823486328f,  100.53096771240234f, 142.17225646972656f,
+    201.0619354248047f,  284.3445129394531f,  402.1238708496094f,  568.6890258789062f};
+
+// ---------------------------------------------------------------------
+// Encoding: feature i*32 + 2k + {0: sin, 1: cos} of (pi 2^(k/2)) * x_i.
+// ---------------------------------------------------------------------
+__global__ void k_guide_encode(const float* __restrict__ x, int n, uint16_t* __restrict__ X) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float* xr = x + (size_t)r * kGuideRowFloats;
+    uint16_t* out = X + (size_t)r * kGuideFeat;
+    for (int i = 0; i < kGuideIn; ++i) {
+        float xi = xr[i];
+        uint16_t f[32];
+#pragma unroll
+        for (int k = 0; k < kGuideFreqs; ++k) {
+            float s, c;
+            own_sincos2(c_freq[k] * xi, &s, &c);
+            f[2 * k] = f2bf(s);
+            f[2 * k + 1] = f2bf(c);
+        }
+        uint4* o4 = reinterpret_cast<uint4*>(out + i * 32);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            uint4 v;
+            v.x = f[8 * q] | ((uint32_t)f[8 * q + 1] << 16);
+            v.y = f[8 * q + 2] | ((uint32_t)f[8 * q + 3] << 16);
+            v.z = f[8 * q + 4] | ((uint32_t)f[8 * q + 5] << 16);
+            v.w = f[8 * q + 6] | ((uint32_t)f[8 * q + 7] << 16);
+            o4[q] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// C = op(A) op(B): a 64 x 64 tile per workgroup of four waves (2 x 2), each
+// wave one 32 x 32 v_mfma_f32_32x32x16_bf16 accumulator; K advances 32 at a
+// time through LDS, where both operands are kept k-contiguous ([m][k] and
+// [n][k]) so every lane reads its 8-element fragment with one 16-byte load.
+// Fragment layout: lane l holds row (l % 32) and k = 8 (l / 32) .. +7 of A,
+// column (l % 32) and the same k of B; accumulator element i is row
+// 8 (i / 4) + 4 (l / 32) + (i % 4), column l % 32.
+// ---------------------------------------------------------------------
+constexpr int kGT = 64, kGK = 32, kGPad = 8;
+
+template <int EPI, bool AT, bool BT>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[kGT][kGK + kGPad];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[kGT][kGK + kGPad];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = blockIdx.y * kGT, n0 = blockIdx.x * kGT;
+    const int kb = blockIdx.z * g.k_split;
+    const int ke = min(g.K, kb + g.k_split);
+    f32x16_t acc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    for (int k0 = kb; k0 < ke; k0 += kGK) {
+        if (!AT) {  // A row-major M x K
+            int r = tid >> 2, c = (tid & 3) * 8;
+            *reinterpret_cast<uint4*>(&As[r][c]) =
+                *reinterpret_cast<const uint4*>(g.A + (size_t)(m0 + r) * g.lda + k0 + c);
+        } else {  // A stored K x M
+            int kk = tid >> 3, c = (tid & 7) * 8;
+            uint4 v = *reinterpret_cast<const uint4*>(g.A + (size_t)(k0 + kk) * g.lda + m0 + c);
+            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) As[c + j][kk] = e[j];
+        }
+        if (!BT) {  // B row-major K x N
+            int kk = tid >> 3, c = (tid & 7) * 8;
+            uint4 v = *reinterpret_cast<const uint4*>(g.B + (size_t)(k0 + kk) * g.ldb + n0 + c);
+            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) Bs[c + j][kk] = e[j];
+        } else {  // B stored N x K
+            int r = tid >> 2, c = (tid & 3) * 8;
+            *reinterpret_cast<uint4*>(&Bs[r][c]) =
+                *reinterpret_cast<const uint4*>(g.B + (size_t)(n0 + r) * g.ldb + k0 + c);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < kGK; kk += 16) {
+            bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+            bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int n = n0 + wn * 32 + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int m = m0 + wm * 32 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        const size_t at = (size_t)m * g.ldc + n;
+        float v = acc[i];
+        if (EPI == EPI_BIAS_RELU_BF16) {
+            v = v + g.bias[n];
+            reinterpret_cast<uint16_t*>(g.C)[at] = f2bf(v > 0.0f ? v : 0.0f);
+        } else if (EPI == EPI_BIAS_F32) {
+            reinterpret_cast<float*>(g.C)[at] = v + g.bias[n];
+        } else if (EPI == EPI_MASK_BF16) {
+            reinterpret_cast<uint16_t*>(g.C)[at] = f2bf(bf2f(g.mask[at]) > 0.0f ? v : 0.0f);
+        } else {
+            atomicAdd(reinterpret_cast<float*>(g.C) + at, v);
+        }
+    }
+}
+
+template <int EPI>
+static void gemm_epi(const GemmArgs& g, bool at, bool bt, dim3 grid, hipStream_t s) {
+    if (!at && !bt) hipLaunchKernelGGL((k_gemm<EPI, false, false>), grid, dim3(256), 0, s, g);
+    else if (!at && bt) hipLaunchKernelGGL((k_gemm<EPI, false, true>), grid, dim3(256), 0, s, g);
+    else if (at && !bt) hipLaunchKernelGGL((k_gemm<EPI, true, false>), grid, dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_gemm<EPI, true, true>), grid, dim3(256), 0, s, g);
+}
+
+bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStream_t s) {
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0 || g.M % kGT || g.N % kGT || g.K % kGK || g.k_split <= 0 ||
+        g.k_split % kGK || g.lda % 8 || g.ldb % 8)
+        return false;
+    if (epi == EPI_MASK_BF16 && !g.mask) return false;
+    if ((epi == EPI_BIAS_RELU_BF16 || epi == EPI_BIAS_F32) && !g.bias) return false;
+    if (epi != EPI_ATOMIC_F32 && g.k_split < g.K) return false;  // only the atomic epilogue sums partial K
+    dim3 grid(g.N / kGT, g.M / kGT, (g.K + g.k_split - 1) / g.k_split);
+    switch (epi) {
+        case EPI_BIAS_RELU_BF16: gemm_epi<EPI_BIAS_RELU_BF16>(g, a_trans, b_trans, grid, s); break;
+        case EPI_BIAS_F32: gemm_epi<EPI_BIAS_F32>(g, a_trans, b_trans, grid, s); break;
+        case EPI_MASK_BF16: gemm_epi<EPI_MASK_BF16>(g, a_trans, b_trans, grid, s); break;
+        case EPI_ATOMIC_F32: gemm_epi<EPI_ATOMIC_F32>(g, a_trans, b_trans, grid, s); break;
+        default: return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------------
+// Softmax cross-entropy, one wave per row of 1024 logits (16 per lane):
+//   loss += w_r / W * (log sum exp(z) - z_b),
+//   dZ_r  = bf16(w_r / W * (softmax(z) - onehot(b)))   (W = sum of w)
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z, const float* __restrict__ w,
+                                                    const int* __restrict__ bins, int n,
+                                                    const float* __restrict__ wsum, uint16_t* __restrict__ dZ,
+                                                    float* __restrict__ loss) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    const float4* zr = reinterpret_cast<const float4*>(Z + (size_t)r * kGuideBins) + lane * 4;
+    float z[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float4 v = zr[q];
+        z[4 * q] = v.x;
+        z[4 * q + 1] = v.y;
+        z[4 * q + 2] = v.z;
+        z[4 * q + 3] = v.w;
+    }
+    float m = z[0];
+#pragma unroll
+    for (int j = 1; j < 16; ++j) m = fmaxf(m, z[j]);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    float e[16], s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        e[j] = __expf(z[j] - m);
+        s += e[j];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    const float W = *wsum;
+    const int b = bins[r];
+    const float wr = (W > 0.0f && b >= 0) ? w[r] / W : 0.0f;
+    const float inv_s = 1.0f / s;
+    uint16_t d[16];
+    float zb = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        int col = lane * 16 + j;
+        float p = e[j] * inv_s - (col == b ? 1.0f : 0.0f);
+        if (col == b) zb = z[j];
+        d[j] = f2bf(wr * p);
+    }
+    uint4* out = reinterpret_cast<uint4*>(dZ + (size_t)r * kGuideBins + lane * 16);
+    out[0] = make_uint4(d[0] | ((uint32_t)d[1] << 16), d[2] | ((uint32_t)d[3] << 16), d[4] | ((uint32_t)d[5] << 16),
+                        d[6] | ((uint32_t)d[7] << 16));
+    out[1] = make_uint4(d[8] | ((uint32_t)d[9] << 16), d[10] | ((uint32_t)d[11] << 16),
+                        d[12] | ((uint32_t)d[13] << 16), d[14] | ((uint32_t)d[15] << 16));
+    for (int off = 32; off > 0; off >>= 1) zb += __shfl_xor(zb, off);
+    if (lane == 0 && wr != 0.0f) atomicAdd(loss, wr * (__logf(s) + m - zb));
+}
+
+// Column sums of a BF16 matrix (bias gradients): 64 columns x 256 rows per block.
+__global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* __restrict__ M, int rows, int cols,
+                                                     float* __restrict__ out) {
+    __shared__ float part[4][64];
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int g = threadIdx.x >> 6;
+    const int r0 = blockIdx.y * 256;
+    float s = 0.0f;
+    for (int r = r0 + g; r < min(rows, r0 + 256); r += 4) s += bf2f(M[(size_t)r * cols + c]);
+    part[g][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (g == 0) atomicAdd(out + c, (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]));
+}
+
+__global__ __launch_bounds__(256) void k_sum_count(const float* __restrict__ v, int n, float* __restrict__ out) {
+    float s = 0.0f, c = 0.0f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        float x = v[i];
+        s += x;
+        c += x > 0.0f ? 1.0f : 0.0f;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off);
+        c += __shfl_xor(c, off);
+    }
+    if ((threadIdx.x & 63) == 0 && c != 0.0f) {
+        atomicAdd(out, s);
+        atomicAdd(out + 1, c);
+    }
+}
+
+// Adam (Kingma & Ba 2015, Algorithm 1) on the FP32 master copy; refreshes the BF16 copy.
+__global__ __launch_bounds__(256) void k_adam(AdamArgs a) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= a.n) return;
+    if (a.wsum && !(*a.wsum > 0.0f)) return;
+    float g = a.grad[i];
+    float m = a.beta1 * a.m[i] + (1.0f - a.beta1) * g;
+    float v = a.beta2 * a.v[i] + (1.0f - a.beta2) * (g * g);
+    a.m[i] = m;
+    a.v[i] = v;
+    float bc1 = 1.0f - powf(a.beta1, (float)a.step);
+    float bc2 = 1.0f - powf(a.beta2, (float)a.step);
+    float mh = m / bc1, vh = v / bc2;
+    float w = a.master[i] - a.lr * mh / (sqrtf(vh) + a.eps);
+    a.master[i] = w;
+    a.wbf[i] = f2bf(w);
+}
+
+__global__ void k_to_bf16(const float* __restrict__ in, uint16_t* __restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = f2bf(in[i]);
+}
+
+// Proposal table: one wave per row, p = exp(z - max), CDF of the prefix sums
+// (16 consecutive bins per lane, then a wave scan of the lane totals), each
+// divided by the total; cdf[0] = 0 and cdf[1024] = 1 exactly.
+__global__ __launch_bounds__(256) void k_guide_table(const float* __restrict__ Z, int rows, float* __restrict__ cdf) {
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const float* zr = Z + (size_t)r * kGuideBins + lane * 16;
+    float z[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) z[j] = zr[j];
+    float m = z[0];
+#pragma unroll
+    for (int j = 1; j < 16; ++j) m = fmaxf(m, z[j]);
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+    float pre[16], s = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        s += __expf(z[j] - m);
+        pre[j] = s;
+    }
+    // inclusive scan of the lane totals
+    float incl = s;
+    for (int off = 1; off < 64; off <<= 1) {
+        float t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    float total = __shfl(incl, 63);
+    float base = incl - s;
+    float* out = cdf + (size_t)r * kGuideCdfStride;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) out[lane * 16 + j + 1] = (base + pre[j]) / total;
+    if (lane == 0) out[0] = 0.0f;
+    if (lane == 63) out[kGuideBins] = 1.0f;
+}
+
+__device__ __forceinline__ uint32_t bucket_of(float4 p, float ox, float oy, float oz, float inv_cell, uint32_t mask) {
+    int cx = (int)floorf((p.x - ox) * inv_cell);
+    int cy = (int)floorf((p.y - oy) * inv_cell);
+    int cz = (int)floorf((p.z - oz) * inv_cell);
+    return cell_bucket(cx, cy, cz, mask);
+}
+
+__global__ void k_mark_visible(const float4* __restrict__ qpos, uint32_t n, float ox, float oy, float oz,
+                               float inv_cell, uint32_t mask, uint8_t* __restrict__ vis) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        vis[bucket_of(qpos[i], ox, oy, oz, inv_cell, mask)] = 1;
+}
+
+__global__ void k_guide_targets(const float4* __restrict__ pos, const float4* __restrict__ dir_tr,
+                                const float2* __restrict__ tp_gb, const uint32_t* __restrict__ uid, uint32_t n,
+                                float ox, float oy, float oz, float inv_cell, uint32_t mask,
+                                const uint8_t* __restrict__ vis, uint32_t train_paths, float* __restrict__ w) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t path = uid[i] >> 3;
+        if (path >= train_paths) continue;
+        if (!vis[bucket_of(pos[i], ox, oy, oz, inv_cell, mask)]) continue;
+        float2 gb = tp_gb[i];
+        float lum = (dir_tr[i].w + gb.x + gb.y) * (1.0f / 3.0f);
+        atomicAdd(w + path, lum);
+    }
+}
+
+// ---- launchers ----
+void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_guide_encode, dim3((n + 127) / 128), dim3(128), 0, s, x, n, X);
+}
+void launch_softmax_ce(const float* Z, const float* w, const int* bins, int n, const float* wsum, uint16_t* dZ,
+                       float* loss, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, Z, w, bins, n, wsum, dZ, loss);
+}
+void launch_colsum_bf16(const uint16_t* M, int rows, int cols, float* out, hipStream_t s) {
+    if (rows > 0 && cols % 64 == 0)
+        hipLaunchKernelGGL(k_colsum_bf16, dim3(cols / 64, (rows + 255) / 256), dim3(256), 0, s, M, rows, cols, out);
+}
+void launch_sum_count(const float* v, int n, float* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_sum_count, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, s, v, n, out);
+}
+void launch_adam(const AdamArgs& a, hipStream_t s) {
+    if (a.n > 0) hipLaunchKernelGGL(k_adam, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+}
+void launch_to_bf16(const float* in, uint16_t* out, int n, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, s, in, out, n);
+}
+void launch_guide_table(const float* Z, int rows, float* cdf, hipStream_t s) {
+    if (rows > 0) hipLaunchKernelGGL(k_guide_table, dim3((rows + 3) / 4), dim3(256), 0, s, Z, rows, cdf);
+}
+void launch_mark_visible(const float4* qpos, uint32_t nq, float ox, float oy, float oz, float inv_cell, uint32_t mask,
+                         uint8_t* vis, hipStream_t s) {
+    if (nq > 0)
+        hipLaunchKernelGGL(k_mark_visible, dim3(std::min<uint32_t>(4096, (nq + 255) / 256)), dim3(256), 0, s, qpos, nq,
+                           ox, oy, oz, inv_cell, mask, vis);
+}
+void launch_guide_targets(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
+                          uint32_t count, float ox, float oy, float oz, float inv_cell, uint32_t mask,
+                          const uint8_t* vis, uint32_t train_paths, float* w, hipStream_t s) {
+    if (count > 0 && train_paths > 0)
+        hipLaunchKernelGGL(k_guide_targets, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0, s, pos,
+                           dir_tr, tp_gb, uid, count, ox, oy, oz, inv_cell, mask, vis, train_paths, w);
+}
+
+}  // namespace rc

@@ -89,6 +89,30 @@ RC_HD float own_sincos(float x, int cosine) {
     int q = (k + cosine) & 3;
     return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
 }
+// Both values of own_sincos from one reduction (identical results).
+RC_HD void own_sincos2(float x, float* sin_out, float* cos_out) {
+    const float two_over_pi = 0.6366197466850281f;
+    const float c1 = 1.5703125f;
+    const float c2 = 0.0004838267923332751f;
+    float fk = x * two_over_pi;
+    int k = (int)(fk + (fk >= 0.0f ? 0.5f : -0.5f));
+    float kf = (float)k;
+    float r = (x - kf * c1) - kf * c2;
+    float r2 = r * r;
+    float ps = -0.00019841270113829523f + r2 * 2.7557318844628753e-06f;
+    ps = 0.008333333767950535f + r2 * ps;
+    ps = -0.1666666716337204f + r2 * ps;
+    float s = r + (r * r2) * ps;
+    float pc = 2.4801587642286904e-05f + r2 * -2.755731998149713e-07f;
+    pc = -0.0013888889225199819f + r2 * pc;
+    pc = 0.0416666679084301f + r2 * pc;
+    pc = -0.5f + r2 * pc;
+    float c = 1.0f + r2 * pc;
+    int q = k & 3;
+    *sin_out = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    int qc = (k + 1) & 3;
+    *cos_out = qc == 0 ? s : (qc == 1 ? c : (qc == 2 ? -s : -c));
+}
 RC_HD float own_sin(float x) { return own_sincos(x, 0); }
 RC_HD float own_cos(float x) { return own_sincos(x, 1); }
 RC_HD float own_asin_poly(float y) {
@@ -159,6 +183,14 @@ struct PathRng {
     RC_HD U4 block(uint32_t b) const { return philox(b, path, iter, stream, k0, k1); }
     RC_HD float u(int d) const { return u24(pick_word(block((uint32_t)d >> 2), d & 3)); }
 };
+
+// BF16 <-> FP32 with round-to-nearest-even (finite values).
+RC_HD uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+RC_HD float bf2f(uint16_t h) { return __builtin_bit_cast(float, (uint32_t)h << 16); }
 
 // Spatial hash of an integer photon-map cell.
 RC_HD uint32_t cell_hash(int x, int y, int z) {

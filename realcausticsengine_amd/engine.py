@@ -22,11 +22,16 @@ EXPORTS = [
     "init", "main", "rc_set_option", "rc_set_option_str", "rc_get_option", "rc_prepare", "rc_render_iteration",
     "rc_get_image", "rc_write_exr", "rc_release", "rc_last_error", "rc_cancel", "rc_stats", "rc_first_hit",
     "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
+    "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
+    "rc_guide_records",
 ]
+GUIDE_PARAMS = 87232
+GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
               "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris",
               "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
-              "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve"]
+              "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve",
+              "ms_guide", "guide_visible", "guide_hit_fraction", "guide_loss"]
 
 _lib = None
 
@@ -77,6 +82,17 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_gather_sets.restype = ctypes.c_longlong
     lib.rc_camera_pass_photons.argtypes = [ctypes.c_int, P(ctypes.c_float), ctypes.c_longlong, P(ctypes.c_float)]
     lib.rc_camera_pass_photons.restype = ctypes.c_int
+    FP, IP = P(ctypes.c_float), P(ctypes.c_int)
+    lib.rc_guide_get_params.argtypes = [FP, ctypes.c_longlong]
+    lib.rc_guide_set_params.argtypes = [FP, ctypes.c_longlong]
+    lib.rc_guide_forward.argtypes = [FP, ctypes.c_longlong, FP]
+    lib.rc_guide_train_step.argtypes = [FP, IP, FP, ctypes.c_longlong, FP, FP, ctypes.c_int]
+    for fn in (lib.rc_guide_get_params, lib.rc_guide_set_params, lib.rc_guide_forward, lib.rc_guide_train_step):
+        fn.restype = ctypes.c_int
+    lib.rc_guide_table.argtypes = [FP, FP, ctypes.c_longlong]
+    lib.rc_guide_table.restype = ctypes.c_longlong
+    lib.rc_guide_records.argtypes = [FP, IP, ctypes.c_longlong]
+    lib.rc_guide_records.restype = ctypes.c_longlong
     _lib = lib
     return lib
 
@@ -183,6 +199,48 @@ class Engine:
         ph = np.ascontiguousarray(photons, dtype=np.float32)
         self._check(self.lib.rc_camera_pass_photons(iteration, _fp(ph), len(ph), _fp(img)), "rc_camera_pass_photons")
         return img.reshape(H, W, 3)
+
+    # -- learned emission proposal -----------------------------------------
+    def guide_params(self) -> np.ndarray:
+        out = np.zeros(GUIDE_PARAMS, dtype=np.float32)
+        self._check(self.lib.rc_guide_get_params(_fp(out), GUIDE_PARAMS), "rc_guide_get_params")
+        return out
+
+    def guide_set_params(self, params: np.ndarray) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        if p.size != GUIDE_PARAMS:
+            raise ValueError("guide parameter vector must hold 87,232 floats")
+        self._check(self.lib.rc_guide_set_params(_fp(p), GUIDE_PARAMS), "rc_guide_set_params")
+
+    def guide_forward(self, x: np.ndarray) -> np.ndarray:
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        out = np.zeros((len(xx), GUIDE_BINS), dtype=np.float32)
+        self._check(self.lib.rc_guide_forward(_fp(xx), len(xx), _fp(out)), "rc_guide_forward")
+        return out
+
+    def guide_train_step(self, x: np.ndarray, bins: np.ndarray, w: np.ndarray, apply: bool = False):
+        xx = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, 6)
+        b = np.ascontiguousarray(bins, dtype=np.int32)
+        ww = np.ascontiguousarray(w, dtype=np.float32)
+        grad = np.zeros(GUIDE_PARAMS, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        self._check(self.lib.rc_guide_train_step(_fp(xx), b.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _fp(ww),
+                                                 len(xx), _fp(grad), _fp(loss), int(apply)), "rc_guide_train_step")
+        return float(loss[0]), grad
+
+    def guide_table(self):
+        n = self._check(self.lib.rc_guide_table(None, None, 0), "rc_guide_table")
+        cdf = np.zeros((n, GUIDE_BINS + 1), dtype=np.float32)
+        inp = np.zeros((n, 6), dtype=np.float32)
+        self._check(self.lib.rc_guide_table(_fp(cdf), _fp(inp), n), "rc_guide_table")
+        return cdf, inp
+
+    def guide_records(self, n: int):
+        x = np.zeros((n, 6), dtype=np.float32)
+        b = np.zeros(n, dtype=np.int32)
+        m = self._check(self.lib.rc_guide_records(_fp(x), b.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), n),
+                        "rc_guide_records")
+        return x[:m], b[:m]
 
     def release(self):
         self.lib.rc_release()
