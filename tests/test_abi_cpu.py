@@ -94,3 +94,20 @@ def test_main_before_init_reports_error(lib):
     lib.rc_release()
     assert lib.main() != 0
     assert "before init" in lib.rc_last_error().decode()
+
+
+def test_guide_options_and_no_cpu_fallback(lib):
+    for key in (b"guide", b"guide_alpha", b"guide_lr", b"guide_batch", b"guide_cells", b"guide_warmup",
+                b"guide_train", b"guide_seed"):
+        assert lib.rc_set_option(key, 1.0) == 0, key
+    assert lib.rc_get_option(b"guide") == 1.0
+    assert _init(lib, scenes.c0_scene(8, 8, subdiv=1)) == ""
+    # without a GPU every guide call fails and says why; nothing runs on the CPU
+    x = np.zeros(6, dtype=np.float32)
+    out = np.zeros(engine.GUIDE_BINS, dtype=np.float32)
+    rc = lib.rc_guide_forward(x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 1,
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    assert rc == -1
+    assert "HIP device" in lib.rc_last_error().decode()
+    lib.rc_set_option(b"guide", 0.0)
+    lib.rc_release()
