@@ -8,9 +8,13 @@ The scene is synthetic (seeded); nothing is loaded from disk.
 
 Prints one JSON line with photon paths/s (whole iteration), the roofline of
 the dominant kernel (algorithmic bytes from a separate counting launch over
-its HIP-event duration) and the CPU oracle timed on a bounded sample.
+its HIP-event duration), the CPU oracle timed on a bounded sample, and
+time-to-RMSE <= 0.01 against the FP64 oracle's reference image
+(tests/golden/water1024_fp64_s61.npz: fixed merge radius, 1/61 of the
+pixels) for the plain and the guided estimator, each at 16,777,216 light
+paths per iteration and at the light-path count the photon-budget tuner picks.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ttr]
 """
 from __future__ import annotations
 
@@ -86,6 +90,82 @@ def cpu_baseline(args, photons: int, stride: int) -> dict:
     }
 
 
+def time_to_rmse(a, cpu_s_per_iteration):
+    """Wall time of rc_render_iteration calls until the running image reaches
+    RMSE <= target against the FP64 reference (readback and scoring untimed)."""
+    from realcausticsengine_amd import convergence, scenes
+    from realcausticsengine_amd.engine import GUIDE_STEP_FLOPS_PER_ROW, Engine
+
+    ref = convergence.SubsetReference.load(os.path.join(ROOT, "tests", "golden", "water1024_fp64_s61.npz"))
+    target = a.ttr_target
+    legs = {}
+
+    def engine(guided, photons):
+        args = scenes.water_scene(a.grid, 1024, 1024, alpha=1.0, samples=1, base_radius=0.01)
+        opts = dict(photons=photons, seed=0, write_exr=0, compat_rows=0)
+        if guided:
+            opts.update(guide=1, guide_warmup=2, guide_train_every=a.guide_train_every)
+        e = Engine(args, **opts)
+        e.prepare()
+        return e
+
+    for guided in (False, True):
+        name = "guided" if guided else "plain"
+        print(f"[bench] time-to-RMSE: {name}, 16,777,216 light paths/iteration", file=sys.stderr, flush=True)
+        eng = engine(guided, a.photons)
+        run = convergence.run_to_target(eng, ref, target, a.ttr_max_iterations, label=f"{name} 16M",
+                                        max_seconds=a.ttr_max_seconds)
+        eng.release()
+        legs[f"{name}_16M"] = run.summary(target)
+        if guided:
+            legs[f"{name}_16M"]["guide_hit_fraction_last"] = round(run.stats[-1].get("guide_hit_fraction", 0), 4)
+        print(f"[bench]   {legs[f'{name}_16M']}", file=sys.stderr, flush=True)
+        print(f"[bench] time-to-RMSE: {name}, tuned light-path count", file=sys.stderr, flush=True)
+        eng = engine(guided, a.photons)
+        tuner = convergence.PhotonBudgetTuner(4 << 20, 1 << 20, pilots=6)
+        run = convergence.run_tuned(eng, ref, target, a.ttr_max_iterations, tuner, label=f"{name} tuned",
+                                    max_seconds=a.ttr_max_seconds)
+        mfma = None
+        if guided:
+            eng.set("guide_train_every", 1)
+            eng.render_iteration()  # one training iteration to time the network step
+            st = eng.stats()
+            if st["ms_guide_step"] > 0:
+                tf = GUIDE_STEP_FLOPS_PER_ROW * st["guide_rows"] / (st["ms_guide_step"] * 1e-3) / 1e12
+                mfma = {"rows": int(st["guide_rows"]), "ms_step": round(st["ms_guide_step"], 4),
+                        "ms_guide_total": round(st["ms_guide"], 4), "tflops": round(tf, 2),
+                        "bf16_peak_tflops": 2500.0, "frac": round(tf / 2500.0, 5)}
+        eng.release()
+        f = tuner.fit
+        legs[f"{name}_tuned"] = run.summary(target)
+        if f:
+            legs[f"{name}_tuned"]["fit"] = {"C": round(f.C, 5), "P_per_path": f.P, "T0_ms": round(f.T0, 3),
+                                            "ms_per_Mpath": round(f.t_per_path * 1e6, 4), "chosen": f.best}
+        if mfma:
+            legs["guide_training_step"] = mfma
+        print(f"[bench]   {legs[f'{name}_tuned']}", file=sys.stderr, flush=True)
+    plain = legs["plain_16M"]
+    it_plain = plain["iterations"] or plain.get("extrapolated_iterations")
+    out = {"target": target, "definition": "RMSE over reference pixels and channels of (img - ref) / mean(ref)",
+           "reference": {"file": "tests/golden/water1024_fp64_s61.npz", "estimator": "FP64 oracle, fixed radius",
+                         "pixels": int(len(ref.rows)), "own_rmse": round(ref.noise, 5)},
+           "legs": legs}
+    if cpu_s_per_iteration and it_plain:
+        out["cpu_reference_mode_estimate_s"] = round(cpu_s_per_iteration * it_plain, 1)
+
+    def secs(leg):
+        return leg["seconds"] if leg["seconds"] is not None else leg.get("extrapolated_seconds")
+
+    base = secs(plain)
+    if base:
+        out["speedup_vs_plain_16M"] = {k: round(base / secs(v), 3) for k, v in legs.items()
+                                       if isinstance(v, dict) and "iterations_run" in v and secs(v)}
+        if "cpu_reference_mode_estimate_s" in out:
+            best = min(secs(v) for v in legs.values() if isinstance(v, dict) and "iterations_run" in v and secs(v))
+            out["speedup_best_vs_cpu"] = round(out["cpu_reference_mode_estimate_s"] / best, 1)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +176,11 @@ def main() -> int:
     ap.add_argument("--res", type=int, default=1024)
     ap.add_argument("--cpu-stride", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ttr", action="store_true", help="skip the time-to-RMSE legs")
+    ap.add_argument("--ttr-target", type=float, default=0.01)
+    ap.add_argument("--ttr-max-iterations", type=int, default=3000)
+    ap.add_argument("--ttr-max-seconds", type=float, default=75.0)
+    ap.add_argument("--guide-train-every", type=int, default=8)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the path has no multi-GPU mode in this build "
@@ -155,6 +240,9 @@ def main() -> int:
     if not a.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args, a.photons, a.cpu_stride)
+    ttr = None
+    if not a.no_ttr:
+        ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None)
     value = a.photons / (ms_step * 1e-3)
     line = {
         "metric": "time-to-RMSE<=0.01 vs FP64 (guided vs plain) + photon paths/s, 1 MI355X",
@@ -174,6 +262,7 @@ def main() -> int:
                    "estimator": "plain vertex merging", "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "time_to_rmse": ttr,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)

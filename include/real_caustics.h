@@ -76,7 +76,8 @@ int main(void);
  * 1e-3), "guide_batch" (training paths per iteration, 65536), "guide_cells"
  * (position cells per light side, multiple of 8, default 16), "guide_warmup"
  * (Adam steps before the proposal is sampled, default 2), "guide_train"
- * (0 = keep the network fixed), "guide_seed", "guide_reset".
+ * (0 = keep the network fixed), "guide_train_every" (after warm-up, train on
+ * every n-th iteration, default 1), "guide_seed", "guide_reset".
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
 int rc_set_option_str(const char *key, const char *value);
@@ -110,10 +111,12 @@ enum {
     RC_STAT_SORT_PASSES,
     /* camera pass split: trace (includes the query-count readback), merge, resolve */
     RC_STAT_MS_CAMERA_TRACE, RC_STAT_MS_GATHER, RC_STAT_MS_RESOLVE,
-    /* learned proposal (option "guide"): training + table time, mean visible
-     * luminance per training path, share of training paths with visible
-     * photons, weighted cross-entropy of the step */
+    /* learned proposal (option "guide"), on iterations that train: training +
+     * table time, mean visible luminance per training path, share of training
+     * paths with visible photons, weighted cross-entropy of the step, time of
+     * the network step alone (forward, backward, Adam) and its row count */
     RC_STAT_MS_GUIDE, RC_STAT_GUIDE_VISIBLE, RC_STAT_GUIDE_HIT_FRACTION, RC_STAT_GUIDE_LOSS,
+    RC_STAT_MS_GUIDE_STEP, RC_STAT_GUIDE_ROWS,
     RC_STAT_COUNT
 };
 

@@ -1,0 +1,210 @@
+"""Convergence measurement and the photon-budget tuner.
+
+RMSE follows one definition throughout: over all reference pixels and the
+three channels of (image - reference) / mean(reference).  With a fixed merge
+radius (radius_reduction = 1) every iteration of the vertex-merging estimator
+has the same expectation, whatever the number of light paths it traces, so
+
+  * iterations with different light-path counts may be averaged, and
+  * the per-iteration relative variance splits as  V(N) = C + P / N:
+    C from the camera side (pixel jitter, Fresnel choices, camera bounces),
+    P / N from the photon density at the merge vertices.
+
+With an iteration time T(N) = T0 + t1 N, the wall time to a target error is
+proportional to V(N) T(N), smallest at N* = sqrt(P T0 / (C t1)).  The tuner
+measures V at two light-path counts from the per-pixel variance across pilot
+iterations (no reference image needed) and T from the engine's own iteration
+times, then renders on at N*.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .engine import Engine
+
+
+def relative_rmse(values: np.ndarray, reference: np.ndarray) -> float:
+    ref = np.asarray(reference, dtype=np.float64)
+    v = np.asarray(values, dtype=np.float64)
+    return float(np.sqrt(np.mean((v - ref) ** 2)) / ref.mean())
+
+
+@dataclass
+class SubsetReference:
+    """Reference values at a subset of pixels (image rows/cols, 3 channels each)."""
+    rows: np.ndarray
+    cols: np.ndarray
+    mean: np.ndarray
+    noise: float = 0.0  # the reference's own RMSE estimate
+
+    @classmethod
+    def load(cls, path: str) -> "SubsetReference":
+        d = np.load(path)
+        return cls(d["rows"], d["cols"], d["mean"].astype(np.float64), float(d["rmse_noise"]))
+
+    def rmse(self, accumulated: np.ndarray, iterations: int) -> float:
+        return relative_rmse(accumulated[self.rows, self.cols] / float(iterations), self.mean)
+
+
+@dataclass
+class ConvergenceRun:
+    label: str
+    photons: list = field(default_factory=list)   # light paths of each iteration
+    seconds: list = field(default_factory=list)   # cumulative wall time after each iteration
+    rmse: list = field(default_factory=list)      # running-image RMSE after each iteration
+    reached_at: int | None = None                  # 1-based iteration that first reached the target
+    stats: list = field(default_factory=list)
+
+    def summary(self, target: float) -> dict:
+        it = self.reached_at
+        out = {"label": self.label, "target": target, "iterations": it,
+               "seconds": round(self.seconds[it - 1], 4) if it else None,
+               "final_rmse": round(self.rmse[-1], 5) if self.rmse else None,
+               "iterations_run": len(self.rmse), "light_paths_per_iteration": sorted(set(self.photons))}
+        if not it and len(self.rmse) >= 8:
+            # not reached: extrapolate rmse^2 = a / k from the last quarter of the run
+            k = np.arange(1, len(self.rmse) + 1, dtype=np.float64)
+            tail = slice(3 * len(k) // 4, None)
+            a = float(np.mean(np.asarray(self.rmse)[tail] ** 2 * k[tail]))
+            need = a / target ** 2
+            out["extrapolated_iterations"] = int(math.ceil(need))
+            out["extrapolated_seconds"] = round(self.seconds[-1] / len(k) * need, 3)
+        return out
+
+
+def run_to_target(eng: Engine, ref: SubsetReference, target: float, max_iterations: int,
+                  schedule=None, label: str = "", max_seconds: float = 1e9) -> ConvergenceRun:
+    """Renders until the running image reaches `target` RMSE against `ref`.
+
+    `schedule(i)` may return the light-path count of iteration i (None keeps
+    the current one).  Only rc_render_iteration calls are timed; the image
+    readback and the RMSE evaluation after each iteration are not."""
+    run = ConvergenceRun(label)
+    spent = 0.0
+    for i in range(max_iterations):
+        if schedule is not None:
+            n = schedule(i)
+            if n is not None:
+                eng.set("photons", int(n))
+        t0 = time.perf_counter()
+        eng.render_iteration()
+        spent += time.perf_counter() - t0
+        img = eng.image(scaled=False)
+        e = ref.rmse(img, i + 1)
+        run.photons.append(int(eng.get("photons")))
+        run.seconds.append(spent)
+        run.rmse.append(e)
+        st = eng.stats()
+        run.stats.append({k: st[k] for k in ("ms_total", "ms_guide", "guide_hit_fraction") if k in st})
+        if e <= target and run.reached_at is None:
+            run.reached_at = i + 1
+            break
+        if spent > max_seconds:
+            break
+    return run
+
+
+@dataclass
+class BudgetFit:
+    n1: int
+    n2: int
+    v1: float
+    v2: float
+    t1_ms: float
+    t2_ms: float
+    C: float = 0.0
+    P: float = 0.0
+    T0: float = 0.0
+    t_per_path: float = 0.0
+    best: int = 0
+
+    def solve(self, n_min: int, n_max: int) -> "BudgetFit":
+        # V(N) = C + P / N through the two pilot points; T(N) = T0 + t N likewise
+        self.P = max(0.0, (self.v2 - self.v1) / (1.0 / self.n2 - 1.0 / self.n1))
+        self.C = max(1e-12, self.v1 - self.P / self.n1)
+        self.t_per_path = max(1e-12, (self.t1_ms - self.t2_ms) / (self.n1 - self.n2))
+        self.T0 = max(1e-6, self.t1_ms - self.t_per_path * self.n1)
+        n = math.sqrt(self.P * self.T0 / (self.C * self.t_per_path)) if self.P > 0 else n_min
+        # keep N a multiple of 65536 (whole training batches and sort tiles)
+        self.best = int(min(n_max, max(n_min, round(n / 65536) * 65536)))
+        return self
+
+    def predicted_cost(self, n: int) -> float:
+        """Relative variance x milliseconds per iteration at n light paths."""
+        return (self.C + self.P / n) * (self.T0 + self.t_per_path * n)
+
+
+def pixel_variance(deltas: list) -> float:
+    """Mean over pixels of the per-pixel variance across iterations, relative to the squared mean."""
+    a = np.stack(deltas).astype(np.float64)
+    m = a.mean()
+    return float(a.var(axis=0, ddof=1).mean() / (m * m)) if m > 0 else 0.0
+
+
+class PhotonBudgetTuner:
+    """Pilot iterations at two light-path counts, then the cost-optimal count.
+
+    The pilot iterations are ordinary iterations of the render (they count in
+    its image and its wall time)."""
+
+    def __init__(self, n1: int, n2: int, pilots: int = 6, n_min: int = 65536, n_max: int = 1 << 26):
+        self.n1, self.n2, self.pilots = n1, n2, pilots
+        self.n_min, self.n_max = n_min, n_max
+        self.fit: BudgetFit | None = None
+        self._deltas = {n1: [], n2: []}
+        self._ms = {n1: [], n2: []}
+        self._prev = None
+
+    def schedule(self, i: int):
+        if i < self.pilots:
+            return self.n1
+        if i < 2 * self.pilots:
+            return self.n2
+        return self.fit.best if self.fit else None
+
+    def observe(self, eng: Engine, i: int, accumulated: np.ndarray) -> None:
+        """Called after iteration i with the accumulated (unscaled) image."""
+        prev = self._prev if self._prev is not None else np.zeros_like(accumulated)
+        delta = accumulated - prev
+        self._prev = accumulated.copy()
+        if i < 2 * self.pilots:
+            n = self.n1 if i < self.pilots else self.n2
+            # the first pilot of each count also carries one-time costs (buffers, warm-up)
+            if i not in (0, self.pilots):
+                self._ms[n].append(eng.stats()["ms_total"])
+            self._deltas[n].append(delta)
+        if i == 2 * self.pilots - 1:
+            self.fit = BudgetFit(self.n1, self.n2, pixel_variance(self._deltas[self.n1]),
+                                 pixel_variance(self._deltas[self.n2]), float(np.median(self._ms[self.n1])),
+                                 float(np.median(self._ms[self.n2]))).solve(self.n_min, self.n_max)
+            self._deltas = {}
+
+
+def run_tuned(eng: Engine, ref: SubsetReference, target: float, max_iterations: int, tuner: PhotonBudgetTuner,
+              label: str = "", max_seconds: float = 1e9) -> ConvergenceRun:
+    """run_to_target with the tuner choosing the light-path count."""
+    run = ConvergenceRun(label)
+    spent = 0.0
+    for i in range(max_iterations):
+        n = tuner.schedule(i)
+        if n is not None:
+            eng.set("photons", int(n))
+        t0 = time.perf_counter()
+        eng.render_iteration()
+        spent += time.perf_counter() - t0
+        img = eng.image(scaled=False)
+        tuner.observe(eng, i, img)
+        e = ref.rmse(img, i + 1)
+        run.photons.append(int(eng.get("photons")))
+        run.seconds.append(spent)
+        run.rmse.append(e)
+        if e <= target and run.reached_at is None:
+            run.reached_at = i + 1
+            break
+        if spent > max_seconds:
+            break
+    return run

@@ -101,7 +101,7 @@ struct Engine {
     int device_cus = 0;
     uint32_t last_queries = 0;
     GuideModel guide;  // learned emission proposal (option "guide")
-    hipEvent_t ev_guide[2] = {};
+    hipEvent_t ev_guide[4] = {};  // guide work begin / end, network step begin / end
 };
 
 Engine* g = nullptr;
@@ -621,10 +621,16 @@ int guide_rows(const Engine& e) {
 
 bool guide_active(const Engine& e) { return e.guide.enabled && e.guide.slots > 0; }
 
-GuideView guide_light_view(Engine& e) {
+bool guide_trains(const Engine& e, int iteration) {
+    const GuideModel& gd = e.guide;
+    return guide_active(e) && gd.train && (gd.step < gd.warmup || iteration % std::max(1, gd.train_every) == 0);
+}
+
+GuideView guide_light_view(Engine& e, int iteration) {
     if (!guide_active(e)) return plain_guide_view();
-    GuideView v = e.guide.view(e.guide.train != 0, e.guide.step >= e.guide.warmup);
-    v.train_paths = e.guide.train ? (uint32_t)guide_rows(e) : 0u;
+    bool rec = guide_trains(e, iteration);
+    GuideView v = e.guide.view(rec, e.guide.step >= e.guide.warmup);
+    v.train_paths = rec ? (uint32_t)guide_rows(e) : 0u;
     return v;
 }
 
@@ -668,7 +674,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.out_tp = e.ph_tp.p;
         a.out_uid = e.ph_uid.p;
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
-        a.guide = guide_light_view(e);
+        a.guide = guide_light_view(e, iteration);
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
         launch_light_pass(a, blocks_for(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
@@ -817,8 +823,10 @@ bool guide_iteration(Engine& e, uint32_t photon_count, const MapInfo& mi) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(gd.train_w.p), 0u, (size_t)n, s);
     launch_guide_targets(e.ph_pos.p, e.ph_dir.p, e.ph_tp.p, e.ph_uid.p, photon_count, pm.origin[0], pm.origin[1],
                          pm.origin[2], pm.inv_cell, pm.mask, gd.vis.p, (uint32_t)n, gd.train_w.p, s);
-    return gd.forward(gd.train_x.p, n, s) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) && gd.adam(s) &&
-           gd.build_table(s);
+    HIP_OK(hipEventRecord(e.ev_guide[2], s));
+    bool ok = gd.forward(gd.train_x.p, n, s) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) && gd.adam(s);
+    HIP_OK(hipEventRecord(e.ev_guide[3], s));
+    return ok && gd.build_table(s);
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0) {
@@ -834,7 +842,8 @@ bool render_iteration(Engine& e, int iteration) {
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
     if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
-    const bool training = guide_active(e) && e.guide.train;
+    const bool training = guide_trains(e, iteration);
+    e.stats[RC_STAT_MS_GUIDE] = e.stats[RC_STAT_MS_GUIDE_STEP] = e.stats[RC_STAT_GUIDE_ROWS] = 0;
     if (training) {
         HIP_OK(hipEventRecord(e.ev_guide[0], e.stream));
         if (!guide_iteration(e, count, mi)) return false;
@@ -858,6 +867,10 @@ bool render_iteration(Engine& e, int iteration) {
         HIP_OK(hipEventElapsedTime(&ms_guide, e.ev_guide[0], e.ev_guide[1]));
         int n = std::max(1, guide_rows(e));
         e.stats[RC_STAT_MS_GUIDE] = ms_guide;
+        float ms_step = 0;
+        HIP_OK(hipEventElapsedTime(&ms_step, e.ev_guide[2], e.ev_guide[3]));
+        e.stats[RC_STAT_MS_GUIDE_STEP] = ms_step;
+        e.stats[RC_STAT_GUIDE_ROWS] = guide_rows(e);
         e.stats[RC_STAT_GUIDE_VISIBLE] = e.guide.last[0] / n;
         e.stats[RC_STAT_GUIDE_HIT_FRACTION] = e.guide.last[1] / n;
         e.stats[RC_STAT_GUIDE_LOSS] = e.guide.last[2];
@@ -1049,6 +1062,7 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         ng.batch = pg.batch;
         ng.cells = pg.cells;
         ng.warmup = pg.warmup;
+        ng.train_every = pg.train_every;
         ng.seed = pg.seed;
         release(prev);
     }
@@ -1247,6 +1261,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "guide_cells") g->guide.cells = (int)v;
     else if (k == "guide_warmup") g->guide.warmup = std::max(0, (int)v);
     else if (k == "guide_train") g->guide.train = (int)v;
+    else if (k == "guide_train_every") g->guide.train_every = std::max(1, (int)v);
     else if (k == "guide_seed") g->guide.seed = (uint64_t)v;
     else if (k == "guide_reset") {
         g->guide.params_ready = g->guide.table_ready = false;

@@ -21,6 +21,7 @@ struct GuideModel {
     int batch = 65536;     // training paths per iteration (the first `batch` path indices)
     int cells = 16;        // position cells per side of a light (a multiple of 8)
     int warmup = 2;        // Adam steps before the table is sampled
+    int train_every = 1;   // after warm-up, train on every n-th iteration
     uint64_t seed = 1;     // parameter initialisation
 
     // state
