@@ -81,19 +81,20 @@ struct GemmArgs {
     const uint16_t* B;
     int ldb;
     int M, N, K;
-    int k_split;          // K range of one blockIdx.z (a multiple of 32)
+    int k_split;          // K range of one blockIdx.z (a multiple of 64)
     const float* bias;    // EPI_BIAS_*: N floats
     const uint16_t* mask; // EPI_MASK_BF16: keep C(m, n) where mask(m, n) > 0 (leading dim ldc)
     void* C;
     int ldc;
+    float* colsum;        // optional (B row-major only): += column sums of op(B) (bias gradients)
 };
-// Requires M % 64 == 0, N % 64 == 0, K % 32 == 0, k_split % 32 == 0, lda/ldb % 8 == 0.
+// Requires M % 64 == 0, N % 64 == 0, K % 64 == 0, k_split % 64 == 0, lda/ldb % 8 == 0.
 bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStream_t s);
 
 void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s);
+// dZ and the loss (added into *loss through the per-row losses in row_loss[n]).
 void launch_softmax_ce(const float* Z, const float* w, const int* bins, int n, const float* wsum, uint16_t* dZ,
-                       float* loss, hipStream_t s);
-void launch_colsum_bf16(const uint16_t* M, int rows, int cols, float* out, hipStream_t s);
+                       float* row_loss, float* loss, hipStream_t s);
 // out[0] += sum of v, out[1] += number of entries > 0.
 void launch_sum_count(const float* v, int n, float* out, hipStream_t s);
 struct AdamArgs {

@@ -50,14 +50,55 @@ __global__ void k_guide_encode(const float* __restrict__ x, int n, uint16_t* __r
 
 // ---------------------------------------------------------------------
 // C = op(A) op(B): a 64 x 64 tile per workgroup of four waves (2 x 2), each
-// wave one 32 x 32 v_mfma_f32_32x32x16_bf16 accumulator; K advances 32 at a
-// time through LDS, where both operands are kept k-contiguous ([m][k] and
-// [n][k]) so every lane reads its 8-element fragment with one 16-byte load.
+// wave one 32 x 32 v_mfma_f32_32x32x16_bf16 accumulator; K advances 64 at a
+// time through LDS (four MFMAs per wave between barriers), and the global
+// loads of the next K step are issued before the MFMAs of the current one.
+// Both operands are kept k-contiguous in LDS ([m][k] and [n][k]) so every
+// lane reads its 8-element fragment with one 16-byte load.
 // Fragment layout: lane l holds row (l % 32) and k = 8 (l / 32) .. +7 of A,
 // column (l % 32) and the same k of B; accumulator element i is row
 // 8 (i / 4) + 4 (l / 32) + (i % 4), column l % 32.
 // ---------------------------------------------------------------------
-constexpr int kGT = 64, kGK = 32, kGPad = 8;
+constexpr int kGT = 64, kGK = 64, kGPad = 8;
+
+struct TileRegs {
+    uint4 a[2], b[2];
+};
+
+// Thread t loads 16 consecutive elements along the contiguous axis of each
+// operand: row (or k) t / 4, elements 16 (t % 4) .. +15.
+template <bool AT, bool BT>
+__device__ __forceinline__ void gemm_load(const GemmArgs& g, int m0, int n0, int k0, int tid, TileRegs& r) {
+    const int row = tid >> 2, c = (tid & 3) * 16;
+    const uint16_t* pa = AT ? g.A + (size_t)(k0 + row) * g.lda + m0 + c : g.A + (size_t)(m0 + row) * g.lda + k0 + c;
+    const uint16_t* pb = BT ? g.B + (size_t)(n0 + row) * g.ldb + k0 + c : g.B + (size_t)(k0 + row) * g.ldb + n0 + c;
+    r.a[0] = reinterpret_cast<const uint4*>(pa)[0];
+    r.a[1] = reinterpret_cast<const uint4*>(pa)[1];
+    r.b[0] = reinterpret_cast<const uint4*>(pb)[0];
+    r.b[1] = reinterpret_cast<const uint4*>(pb)[1];
+}
+
+template <bool AT, bool BT>
+__device__ __forceinline__ void gemm_stage(uint16_t (*As)[kGK + kGPad], uint16_t (*Bs)[kGK + kGPad], int tid,
+                                           const TileRegs& r) {
+    const int row = tid >> 2, c = (tid & 3) * 16;
+    if (!AT) {  // row = m, c = k
+        *reinterpret_cast<uint4*>(&As[row][c]) = r.a[0];
+        *reinterpret_cast<uint4*>(&As[row][c + 8]) = r.a[1];
+    } else {  // row = k, c = m
+        const uint16_t* e = reinterpret_cast<const uint16_t*>(r.a);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) As[c + j][row] = e[j];
+    }
+    if (BT) {  // row = n, c = k
+        *reinterpret_cast<uint4*>(&Bs[row][c]) = r.b[0];
+        *reinterpret_cast<uint4*>(&Bs[row][c + 8]) = r.b[1];
+    } else {  // row = k, c = n
+        const uint16_t* e = reinterpret_cast<const uint16_t*>(r.b);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) Bs[c + j][row] = e[j];
+    }
+}
 
 template <int EPI, bool AT, bool BT>
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
@@ -71,37 +112,32 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     f32x16_t acc;
 #pragma unroll
     for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+    TileRegs r;
+    // column sums of op(B) over this block's K range: the first row of
+    // workgroups, waves with wm == 0; lane l adds k = 32 (l / 32) .. +31 of column l % 32
+    const bool sums = !BT && g.colsum && blockIdx.y == 0 && wm == 0;
+    float csum = 0.0f;
+    if (kb < ke) gemm_load<AT, BT>(g, m0, n0, kb, tid, r);
     for (int k0 = kb; k0 < ke; k0 += kGK) {
-        if (!AT) {  // A row-major M x K
-            int r = tid >> 2, c = (tid & 3) * 8;
-            *reinterpret_cast<uint4*>(&As[r][c]) =
-                *reinterpret_cast<const uint4*>(g.A + (size_t)(m0 + r) * g.lda + k0 + c);
-        } else {  // A stored K x M
-            int kk = tid >> 3, c = (tid & 7) * 8;
-            uint4 v = *reinterpret_cast<const uint4*>(g.A + (size_t)(k0 + kk) * g.lda + m0 + c);
-            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) As[c + j][kk] = e[j];
-        }
-        if (!BT) {  // B row-major K x N
-            int kk = tid >> 3, c = (tid & 7) * 8;
-            uint4 v = *reinterpret_cast<const uint4*>(g.B + (size_t)(k0 + kk) * g.ldb + n0 + c);
-            const uint16_t* e = reinterpret_cast<const uint16_t*>(&v);
-#pragma unroll
-            for (int j = 0; j < 8; ++j) Bs[c + j][kk] = e[j];
-        } else {  // B stored N x K
-            int r = tid >> 2, c = (tid & 3) * 8;
-            *reinterpret_cast<uint4*>(&Bs[r][c]) =
-                *reinterpret_cast<const uint4*>(g.B + (size_t)(n0 + r) * g.ldb + k0 + c);
-        }
+        gemm_stage<AT, BT>(As, Bs, tid, r);
         __syncthreads();
+        if (k0 + kGK < ke) gemm_load<AT, BT>(g, m0, n0, k0 + kGK, tid, r);
 #pragma unroll
         for (int kk = 0; kk < kGK; kk += 16) {
             bf16x8_t a = *reinterpret_cast<const bf16x8_t*>(&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
             bf16x8_t b = *reinterpret_cast<const bf16x8_t*>(&Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
             acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
+        if (sums) {
+            const uint16_t* row = &Bs[wn * 32 + (lane & 31)][(lane >> 5) * 32];
+#pragma unroll
+            for (int j = 0; j < 32; ++j) csum += bf2f(row[j]);
+        }
         __syncthreads();
+    }
+    if (sums) {
+        csum += __shfl_xor(csum, 32);
+        if (lane < 32) atomicAdd(g.colsum + n0 + wn * 32 + lane, csum);
     }
     const int n = n0 + wn * 32 + (lane & 31);
 #pragma unroll
@@ -135,6 +171,7 @@ bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStre
         g.k_split % kGK || g.lda % 8 || g.ldb % 8)
         return false;
     if (epi == EPI_MASK_BF16 && !g.mask) return false;
+    if (g.colsum && b_trans) return false;
     if ((epi == EPI_BIAS_RELU_BF16 || epi == EPI_BIAS_F32) && !g.bias) return false;
     if (epi != EPI_ATOMIC_F32 && g.k_split < g.K) return false;  // only the atomic epilogue sums partial K
     dim3 grid(g.N / kGT, g.M / kGT, (g.K + g.k_split - 1) / g.k_split);
@@ -156,7 +193,7 @@ bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStre
 __global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z, const float* __restrict__ w,
                                                     const int* __restrict__ bins, int n,
                                                     const float* __restrict__ wsum, uint16_t* __restrict__ dZ,
-                                                    float* __restrict__ loss) {
+                                                    float* __restrict__ row_loss) {
     const int lane = threadIdx.x & 63;
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= n) return;
@@ -200,24 +237,15 @@ __global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z,
     out[1] = make_uint4(d[8] | ((uint32_t)d[9] << 16), d[10] | ((uint32_t)d[11] << 16),
                         d[12] | ((uint32_t)d[13] << 16), d[14] | ((uint32_t)d[15] << 16));
     for (int off = 32; off > 0; off >>= 1) zb += __shfl_xor(zb, off);
-    if (lane == 0 && wr != 0.0f) atomicAdd(loss, wr * (__logf(s) + m - zb));
+    if (lane == 0) row_loss[r] = wr != 0.0f ? wr * (__logf(s) + m - zb) : 0.0f;
 }
 
-// Column sums of a BF16 matrix (bias gradients): 64 columns x 256 rows per block.
-__global__ __launch_bounds__(256) void k_colsum_bf16(const uint16_t* __restrict__ M, int rows, int cols,
-                                                     float* __restrict__ out) {
-    __shared__ float part[4][64];
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int g = threadIdx.x >> 6;
-    const int r0 = blockIdx.y * 256;
-    float s = 0.0f;
-    for (int r = r0 + g; r < min(rows, r0 + 256); r += 4) s += bf2f(M[(size_t)r * cols + c]);
-    part[g][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (g == 0) atomicAdd(out + c, (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]));
-}
-
-__global__ __launch_bounds__(256) void k_sum_count(const float* __restrict__ v, int n, float* __restrict__ out) {
+// out[0] += sum of v; with `count`, out[1] += entries > 0.  One atomic per
+// block and value (at most kReduceBlocks blocks).
+constexpr int kReduceBlocks = 64;
+__global__ __launch_bounds__(256) void k_sum_count(const float* __restrict__ v, int n, float* __restrict__ out,
+                                                   int count) {
+    __shared__ float part[2][4];
     float s = 0.0f, c = 0.0f;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         float x = v[i];
@@ -228,9 +256,17 @@ __global__ __launch_bounds__(256) void k_sum_count(const float* __restrict__ v, 
         s += __shfl_xor(s, off);
         c += __shfl_xor(c, off);
     }
-    if ((threadIdx.x & 63) == 0 && c != 0.0f) {
-        atomicAdd(out, s);
-        atomicAdd(out + 1, c);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][wave] = s;
+        part[1][wave] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float ts = (part[0][0] + part[0][1]) + (part[0][2] + part[0][3]);
+        float tc = (part[1][0] + part[1][1]) + (part[1][2] + part[1][3]);
+        if (ts != 0.0f) atomicAdd(out, ts);
+        if (count && tc != 0.0f) atomicAdd(out + 1, tc);
     }
 }
 
@@ -325,15 +361,15 @@ void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_guide_encode, dim3((n + 127) / 128), dim3(128), 0, s, x, n, X);
 }
 void launch_softmax_ce(const float* Z, const float* w, const int* bins, int n, const float* wsum, uint16_t* dZ,
-                       float* loss, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, Z, w, bins, n, wsum, dZ, loss);
-}
-void launch_colsum_bf16(const uint16_t* M, int rows, int cols, float* out, hipStream_t s) {
-    if (rows > 0 && cols % 64 == 0)
-        hipLaunchKernelGGL(k_colsum_bf16, dim3(cols / 64, (rows + 255) / 256), dim3(256), 0, s, M, rows, cols, out);
+                       float* row_loss, float* loss, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, Z, w, bins, n, wsum, dZ, row_loss);
+    hipLaunchKernelGGL(k_sum_count, dim3(std::min(kReduceBlocks, (n + 255) / 256)), dim3(256), 0, s, row_loss, n,
+                       loss, 0);
 }
 void launch_sum_count(const float* v, int n, float* out, hipStream_t s) {
-    if (n > 0) hipLaunchKernelGGL(k_sum_count, dim3(std::min(1024, (n + 255) / 256)), dim3(256), 0, s, v, n, out);
+    if (n > 0)
+        hipLaunchKernelGGL(k_sum_count, dim3(std::min(kReduceBlocks, (n + 255) / 256)), dim3(256), 0, s, v, n, out, 1);
 }
 void launch_adam(const AdamArgs& a, hipStream_t s) {
     if (a.n > 0) hipLaunchKernelGGL(k_adam, dim3((a.n + 255) / 256), dim3(256), 0, s, a);

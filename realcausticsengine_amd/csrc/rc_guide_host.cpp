@@ -28,8 +28,17 @@ bool zero_f32(float* p, size_t n, hipStream_t s) {
     return true;
 }
 
+// Split of K = rows for the weight gradients: about 256 workgroups in all.
+int wgrad_split(int M, int N, int K) {
+    int tiles = (M / 64) * (N / 64);
+    int splits = std::max(1, std::min(K / 64, 256 / std::max(1, tiles)));
+    int ks = (K + splits - 1) / splits;
+    return (ks + 63) / 64 * 64;
+}
+
 bool gemm(const uint16_t* A, int lda, bool at, const uint16_t* B, int ldb, bool bt, int M, int N, int K, int epi,
-          const float* bias, const uint16_t* mask, void* C, int ldc, hipStream_t s, int k_split = 0) {
+          const float* bias, const uint16_t* mask, void* C, int ldc, hipStream_t s, int k_split = 0,
+          float* colsum = nullptr) {
     GemmArgs g{};
     g.A = A;
     g.lda = lda;
@@ -43,6 +52,7 @@ bool gemm(const uint16_t* A, int lda, bool at, const uint16_t* B, int ldb, bool 
     g.mask = mask;
     g.C = C;
     g.ldc = ldc;
+    g.colsum = colsum;
     if (!launch_gemm(g, epi, at, bt, s)) {
         set_error("guide: matrix product shape not supported");
         return false;
@@ -103,7 +113,7 @@ bool GuideModel::ensure_rows(int n) {
     size_t r = (size_t)n;
     if (!X.alloc(r * kGuideFeat) || !H1.alloc(r * kGuideHidden) || !H2.alloc(r * kGuideHidden) ||
         !H3.alloc(r * kGuideHidden) || !dA1.alloc(r * kGuideHidden) || !dA2.alloc(r * kGuideHidden) ||
-        !dA3.alloc(r * kGuideHidden) || !dZ.alloc(r * kGuideBins) || !Z.alloc(r * kGuideBins))
+        !dA3.alloc(r * kGuideHidden) || !dZ.alloc(r * kGuideBins) || !Z.alloc(r * kGuideBins) || !row_loss.alloc(r))
         return false;
     rows_cap = n;
     return true;
@@ -132,40 +142,35 @@ bool GuideModel::forward(const float* d_x, int n, hipStream_t s) {
 bool GuideModel::backward(const int* d_bin, const float* d_w, int n, hipStream_t s) {
     const uint16_t* w = wbf.p;
     float* g = grad.p;
-    const int ks = std::min(n, 2048);  // split-K of the weight gradients (n % 64 == 0)
     zero_f32(scal.p, 4, s);
     zero_f32(g, kGuideParams, s);
     launch_sum_count(d_w, n, scal.p, s);  // [0] sum of w, [1] rows with w > 0
-    launch_softmax_ce(Z.p, d_w, d_bin, n, scal.p, dZ.p, scal.p + 2, s);
+    launch_softmax_ce(Z.p, d_w, d_bin, n, scal.p, dZ.p, row_loss.p, scal.p + 2, s);
     // output layer
     if (!gemm(H3.p, kGuideHidden, true, dZ.p, kGuideBins, false, kGuideHidden, kGuideBins, n, EPI_ATOMIC_F32, nullptr,
-              nullptr, g + kOffW4, kGuideBins, s, ks))
+              nullptr, g + kOffW4, kGuideBins, s, wgrad_split(kGuideHidden, kGuideBins, n), g + kOffB4))
         return false;
-    launch_colsum_bf16(dZ.p, n, kGuideBins, g + kOffB4, s);
     if (!gemm(dZ.p, kGuideBins, false, w + kOffW4, kGuideBins, true, n, kGuideHidden, kGuideBins, EPI_MASK_BF16,
               nullptr, H3.p, dA3.p, kGuideHidden, s))
         return false;
     // hidden layer 3
     if (!gemm(H2.p, kGuideHidden, true, dA3.p, kGuideHidden, false, kGuideHidden, kGuideHidden, n, EPI_ATOMIC_F32,
-              nullptr, nullptr, g + kOffW3, kGuideHidden, s, ks))
+              nullptr, nullptr, g + kOffW3, kGuideHidden, s, wgrad_split(kGuideHidden, kGuideHidden, n), g + kOffB3))
         return false;
-    launch_colsum_bf16(dA3.p, n, kGuideHidden, g + kOffB3, s);
     if (!gemm(dA3.p, kGuideHidden, false, w + kOffW3, kGuideHidden, true, n, kGuideHidden, kGuideHidden,
               EPI_MASK_BF16, nullptr, H2.p, dA2.p, kGuideHidden, s))
         return false;
     // hidden layer 2
     if (!gemm(H1.p, kGuideHidden, true, dA2.p, kGuideHidden, false, kGuideHidden, kGuideHidden, n, EPI_ATOMIC_F32,
-              nullptr, nullptr, g + kOffW2, kGuideHidden, s, ks))
+              nullptr, nullptr, g + kOffW2, kGuideHidden, s, wgrad_split(kGuideHidden, kGuideHidden, n), g + kOffB2))
         return false;
-    launch_colsum_bf16(dA2.p, n, kGuideHidden, g + kOffB2, s);
     if (!gemm(dA2.p, kGuideHidden, false, w + kOffW2, kGuideHidden, true, n, kGuideHidden, kGuideHidden,
               EPI_MASK_BF16, nullptr, H1.p, dA1.p, kGuideHidden, s))
         return false;
     // hidden layer 1
     if (!gemm(X.p, kGuideFeat, true, dA1.p, kGuideHidden, false, kGuideFeat, kGuideHidden, n, EPI_ATOMIC_F32, nullptr,
-              nullptr, g + kOffW1, kGuideHidden, s, ks))
+              nullptr, g + kOffW1, kGuideHidden, s, wgrad_split(kGuideFeat, kGuideHidden, n), g + kOffB1))
         return false;
-    launch_colsum_bf16(dA1.p, n, kGuideHidden, g + kOffB1, s);
     return true;
 }
 
@@ -269,7 +274,8 @@ GuideView GuideModel::view(bool records, bool sample) const {
 }
 
 void GuideModel::release() {
-    for (DevBuf<float>* b : {&master, &m, &v, &grad, &scal, &train_x, &train_w, &Z, &table_x, &cdf}) b->release();
+    for (DevBuf<float>* b : {&master, &m, &v, &grad, &scal, &train_x, &train_w, &Z, &row_loss, &table_x, &cdf})
+        b->release();
     for (DevBuf<uint16_t>* b : {&wbf, &X, &H1, &H2, &H3, &dA1, &dA2, &dA3, &dZ}) b->release();
     train_bin.release();
     vis.release();

@@ -38,7 +38,7 @@ struct GuideModel {
     DevBuf<float> train_x, train_w;
     DevBuf<int> train_bin;
     DevBuf<uint16_t> X, H1, H2, H3, dA1, dA2, dA3, dZ;
-    DevBuf<float> Z;
+    DevBuf<float> Z, row_loss;
     DevBuf<float> table_x, cdf;
     DevBuf<uint8_t> vis;
     int rows_cap = 0;
