@@ -138,6 +138,21 @@ long long rc_gather_sets(const float *positions, long long count, const float *q
  * as rc_trace_photons); adds into image (width*height*3). */
 int rc_camera_pass_photons(int iteration, const float *photons, long long count, float *image);
 
+/* ---- batches of scenes (no counterpart in the reference) ----
+ * Several scenes render together, one launch per pass for all of them:
+ * define a scene with rc_set_option(...) and init(...) as usual, then
+ * rc_batch_add() moves it into the batch (the next init starts a new scene,
+ * with default options).  Each scene keeps its own options, seed, radius
+ * schedule and film; a scene in a batch renders the same image as alone.
+ * "exact_merge" selects the merge kernel and must be equal in all scenes. */
+int rc_batch_add(void);               /* returns the scene's index in the batch */
+int rc_batch_size(void);
+int rc_batch_prepare(void);           /* BVH build and upload of every scene */
+int rc_batch_render_iteration(void);  /* one iteration of every scene; returns its index or < 0 */
+int rc_batch_get_image(int scene, float *rgb, int scaled);
+int rc_batch_stats(double *out, int n);  /* RC_STAT_* totals of the last batch iteration */
+void rc_batch_release(void);
+
 /* ---- learned emission proposal (no counterpart in the reference) ----
  * Parameters are one flat float vector of RC_GUIDE_PARAMS entries: W1
  * [192][64], b1[64], W2 [64][64], b2, W3 [64][64], b3, W4 [64][1024], b4[1024]

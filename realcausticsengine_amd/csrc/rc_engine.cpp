@@ -1020,6 +1020,269 @@ bool get_image(Engine& e, float* rgb, bool scaled) {
     return true;
 }
 
+// ---------------------------------------------------------------------
+// Batches: several scenes (each an Engine with its own BVH, lights, camera,
+// film, options and iteration count) rendered by one launch per pass.  The
+// work Engine holds the shared photon, map and query buffers.
+// ---------------------------------------------------------------------
+struct Batch {
+    std::vector<Engine*> scenes;
+    Engine work;
+    bool prepared = false;
+    DevBuf<SceneView> d_sc;
+    DevBuf<PhotonMapView> d_pm;
+    DevBuf<BatchScene> d_bs;
+    DevBuf<uint32_t> d_starts;  // light starts (n + 1), then camera starts (n + 1)
+    double stats[RC_STAT_COUNT] = {0};
+    int iterations = 0;
+};
+
+Batch* g_batch = nullptr;
+
+void release_batch(Batch*& b) {
+    if (!b) return;
+    for (Engine*& e : b->scenes) release(e);
+    Engine& w = b->work;
+    for (auto* d : {&w.ph_pos, &w.ph_dir, &w.s_pos, &w.s_dir, &w.q_pos, &w.q_dir, &w.q_sn, &w.q_gn, &w.q_tpn})
+        d->release();
+    for (auto* d : {&w.ph_tp, &w.s_tp}) d->release();
+    for (auto* d : {&w.ph_uid, &w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
+    w.keys_a.release();
+    w.keys_b.release();
+    w.cells.release();
+    w.path_info.release();
+    w.contrib.release();
+    w.stats_dev.release();
+    b->d_sc.release();
+    b->d_pm.release();
+    b->d_bs.release();
+    b->d_starts.release();
+    for (auto& ev : w.ev)
+        if (ev) (void)hipEventDestroy(ev);
+    for (auto& ev : w.ev_guide)
+        if (ev) (void)hipEventDestroy(ev);
+    if (w.stream) (void)hipStreamDestroy(w.stream);
+    delete b;
+    b = nullptr;
+}
+
+bool batch_prepare(Batch& b) {
+    if (b.prepared) return true;
+    if (b.scenes.empty()) {
+        set_error("batch has no scenes");
+        return false;
+    }
+    for (Engine* e : b.scenes) {
+        // one launch serves every scene: the merge arithmetic is chosen per launch
+        if (e->exact_merge != b.scenes[0]->exact_merge) {
+            set_error("batch scenes must agree on the option exact_merge");
+            return false;
+        }
+        if (!prepare(*e)) return false;
+    }
+    if (!ensure_device(b.work)) return false;
+    if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
+    size_t n = b.scenes.size();
+    if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
+    std::vector<SceneView> sc(n);
+    for (size_t i = 0; i < n; ++i) sc[i] = scene_view(*b.scenes[i]);
+    HIP_OK(hipMemcpy(b.d_sc.p, sc.data(), n * sizeof(SceneView), hipMemcpyHostToDevice));
+    b.prepared = true;
+    return true;
+}
+
+// One iteration of every scene: light pass, photon map, camera trace,
+// gather and resolve, each a single launch over all scenes.
+bool batch_iteration(Batch& b) {
+    Engine& w = b.work;
+    hipStream_t st = w.stream;
+    const size_t n = b.scenes.size();
+    std::vector<BatchScene> bs(n);
+    std::vector<PhotonMapView> pm(n);
+    std::vector<uint32_t> starts(2 * (n + 1));
+    uint64_t light = 0, camera = 0, buckets = 0;
+    for (size_t i = 0; i < n; ++i) {
+        Engine& e = *b.scenes[i];
+        float radius, r2, norm;
+        iteration_params(e, e.iterations_done, radius, r2, norm);
+        long long N = light_paths(e);
+        BatchScene& s = bs[i];
+        s.light_base = (uint32_t)light;
+        s.light_paths = (uint32_t)N;
+        s.camera_base = (uint32_t)camera;
+        s.camera_paths = (uint32_t)((long long)e.W * e.H);
+        s.k0 = (uint32_t)e.seed;
+        s.k1 = (uint32_t)(e.seed >> 32);
+        s.vm_norm = norm;
+        s.bucket_base = (uint32_t)buckets;
+        s.iteration = (uint32_t)e.iterations_done;
+        s.image = e.image.p;
+        starts[i] = (uint32_t)light;
+        starts[n + 1 + i] = (uint32_t)camera;
+        light += (uint64_t)N;
+        camera += s.camera_paths;
+        // the scene's grid, as build_map
+        uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
+        uint64_t nb = 1ull << log2_ceil(want);
+        float maxc = 0.0f;
+        for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
+        float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
+        float cell = rq * (float)std::max(0.67, e.cell_scale);
+        PhotonMapView& v = pm[i];
+        for (int k = 0; k < 3; ++k) v.origin[k] = e.bbox.lo[k] - cell;
+        v.inv_cell = 1.0f / cell;
+        v.mask = (uint32_t)(nb - 1);
+        v.r2 = r2;
+        v.r_query = rq;
+        buckets += nb;
+    }
+    starts[n] = (uint32_t)light;
+    starts[2 * n + 1] = (uint32_t)camera;
+    if (light > (1ull << 28) || camera > (1ull << 28) || buckets > (1ull << 30)) {
+        set_error("batch too large for one launch (light or camera paths > 2^28)");
+        return false;
+    }
+    int uid_bits = log2_ceil(light) + 3;
+    int bucket_bits = log2_ceil(buckets);
+    if (uid_bits + bucket_bits > 64) {
+        set_error("photon map key does not fit 64 bits");
+        return false;
+    }
+    // shared buffers
+    uint32_t cap = std::max<uint32_t>(w.capacity, (uint32_t)std::max<uint64_t>(light, 1 << 16));
+    if (!ensure_photon_buffers(w, cap) || !w.cells.alloc(buckets)) return false;
+    for (size_t i = 0; i < n; ++i) {
+        pm[i].pos = w.s_pos.p;
+        pm[i].dir_tr = w.s_dir.p;
+        pm[i].tp_gb = w.s_tp.p;
+        pm[i].cells = w.cells.p + bs[i].bucket_base;
+    }
+    HIP_OK(hipMemcpyAsync(b.d_bs.p, bs.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(b.d_pm.p, pm.data(), n * sizeof(PhotonMapView), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(b.d_starts.p, starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    BatchView view{};
+    view.n = (int)n;
+    view.sc = b.d_sc.p;
+    view.pm = b.d_pm.p;
+    view.bs = b.d_bs.p;
+    view.light_starts = b.d_starts.p;
+    view.camera_starts = b.d_starts.p + n + 1;
+    const Engine& e0 = *b.scenes[0];
+    // light pass (grow and repeat on overflow)
+    uint32_t count = 0;
+    for (int attempt = 0;; ++attempt) {
+        launch_fill_u32(w.counters.p, 0u, 4, st);
+        LightPassArgs a{};
+        a.sc = scene_view(e0);
+        a.iteration = (uint32_t)e0.iterations_done;
+        a.stream = STREAM_LIGHT;
+        a.n_paths = (uint32_t)light;
+        a.path_counter = w.counters.p;
+        a.photon_counter = w.counters.p + 1;
+        a.capacity = w.capacity;
+        a.out_pos = w.ph_pos.p;
+        a.out_dir = w.ph_dir.p;
+        a.out_tp = w.ph_tp.p;
+        a.out_uid = w.ph_uid.p;
+        a.guide = plain_guide_view();
+        a.batch = view;
+        HIP_OK(hipEventRecord(w.ev[0], st));
+        launch_light_pass(a, blocks_for(w), st);
+        HIP_OK(hipEventRecord(w.ev[1], st));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(&count, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (count <= w.capacity) break;
+        if (attempt == 1 || !ensure_photon_buffers(w, (uint32_t)std::min<double>(count * 1.25 + 1024, 4.0e9))) {
+            set_error("photon buffer overflow");
+            return false;
+        }
+    }
+    // one photon map over every scene
+    MapBuildArgs m{};
+    m.in_pos = w.ph_pos.p;
+    m.in_dir = w.ph_dir.p;
+    m.in_tp = w.ph_tp.p;
+    m.in_uid = w.ph_uid.p;
+    m.count = count;
+    m.mask = (uint32_t)(buckets - 1);
+    m.bucket_bits = bucket_bits;
+    m.uid_bits = uid_bits;
+    m.keys_a = w.keys_a.p;
+    m.keys_b = w.keys_b.p;
+    m.vals_a = w.vals_a.p;
+    m.vals_b = w.vals_b.p;
+    m.hist = w.hist.p;
+    m.hist_sums = w.hist_sums.p;
+    m.out_pos = w.s_pos.p;
+    m.out_dir = w.s_dir.p;
+    m.out_tp = w.s_tp.p;
+    m.out_index = w.s_index.p;
+    m.cells = w.cells.p;
+    HIP_OK(hipEventRecord(w.ev[2], st));
+    build_photon_map_batch(m, view, st);
+    HIP_OK(hipEventRecord(w.ev[3], st));
+    HIP_OK(hipGetLastError());
+    // camera pass
+    uint32_t qcap = std::max(w.q_capacity, (uint32_t)std::min<uint64_t>(2 * camera + 1024, 1ull << 31));
+    if (!ensure_query_buffers(w, qcap, (uint32_t)camera)) return false;
+    CameraPassArgs c{};
+    c.sc = scene_view(e0);
+    c.iteration = (uint32_t)e0.iterations_done;
+    c.stream = STREAM_CAMERA;
+    c.n_paths = (uint32_t)camera;
+    c.path_counter = w.counters.p;
+    c.path_info = w.path_info.p;
+    c.contrib = w.contrib.p;
+    c.exact_merge = e0.exact_merge;
+    c.batch = view;
+    uint32_t nq = 0;
+    for (int attempt = 0;; ++attempt) {
+        c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
+        launch_fill_u32(w.counters.p, 0u, 4, st);
+        HIP_OK(hipEventRecord(w.ev[4], st));
+        launch_camera_trace(c, blocks_for(w), st);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(&nq, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (nq <= w.q_capacity) break;
+        if (attempt == 1 ||
+            !ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
+            set_error("merge query buffer overflow");
+            return false;
+        }
+    }
+    HIP_OK(hipEventRecord(w.ev[6], st));
+    launch_gather(c, std::max(8, w.device_cus) * 16, st);
+    HIP_OK(hipEventRecord(w.ev[7], st));
+    launch_resolve(c, st);
+    HIP_OK(hipEventRecord(w.ev[5], st));
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(st));
+    float ms[8] = {0};
+    HIP_OK(hipEventElapsedTime(&ms[0], w.ev[0], w.ev[1]));
+    HIP_OK(hipEventElapsedTime(&ms[1], w.ev[2], w.ev[3]));
+    HIP_OK(hipEventElapsedTime(&ms[2], w.ev[4], w.ev[5]));
+    HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[6]));
+    HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
+    HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
+    std::fill(b.stats, b.stats + RC_STAT_COUNT, 0.0);
+    b.stats[RC_STAT_LIGHT_PATHS] = (double)light;
+    b.stats[RC_STAT_PHOTONS] = count;
+    b.stats[RC_STAT_CAMERA_PATHS] = (double)camera;
+    b.stats[RC_STAT_GATHERS] = nq;
+    b.stats[RC_STAT_MS_LIGHT] = ms[0];
+    b.stats[RC_STAT_MS_MAP] = ms[1];
+    b.stats[RC_STAT_MS_CAMERA] = ms[2];
+    b.stats[RC_STAT_MS_CAMERA_TRACE] = ms[3];
+    b.stats[RC_STAT_MS_GATHER] = ms[4];
+    b.stats[RC_STAT_MS_RESOLVE] = ms[5];
+    b.stats[RC_STAT_SORT_PASSES] = (bucket_bits + uid_bits + 7) / 8;
+    for (Engine* e : b.scenes) e->iterations_done++;
+    b.iterations++;
+    return true;
+}
+
 }  // namespace
 
 // =====================================================================
@@ -1524,6 +1787,65 @@ __attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration,
     if (!ok) set_error("rc_camera_pass_photons failed");
     return ok ? 0 : -1;
 }
+
+// ---- batches ----
+__attribute__((visibility("default"))) int rc_batch_add(void) {
+    g_error.clear();
+    if (!g || !g->initialised) {
+        set_error("rc_batch_add: no scene (call init first)");
+        return -1;
+    }
+    if (!g->init_error.empty()) {
+        set_error(g->init_error);
+        return -1;
+    }
+    if (!g_batch) g_batch = new Batch();
+    if (g_batch->prepared) {
+        set_error("rc_batch_add: the batch is already prepared");
+        return -1;
+    }
+    g_batch->scenes.push_back(g);
+    g = nullptr;
+    return (int)g_batch->scenes.size() - 1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_size(void) { return g_batch ? (int)g_batch->scenes.size() : 0; }
+
+__attribute__((visibility("default"))) int rc_batch_prepare(void) {
+    g_error.clear();
+    if (!g_batch) {
+        set_error("no batch");
+        return -1;
+    }
+    return batch_prepare(*g_batch) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_render_iteration(void) {
+    if (rc_batch_prepare() != 0) return -1;
+    if (g_cancel.load()) {
+        set_error("cancelled");
+        return -2;
+    }
+    int it = g_batch->iterations;
+    return batch_iteration(*g_batch) ? it : -1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_get_image(int scene, float* rgb, int scaled) {
+    if (!g_batch || !g_batch->prepared || scene < 0 || scene >= (int)g_batch->scenes.size() || !rgb) {
+        set_error("rc_batch_get_image: no such rendered scene");
+        return -1;
+    }
+    return get_image(*g_batch->scenes[scene], rgb, scaled != 0) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_stats(double* out, int n) {
+    if (!g_batch || !out) return 0;
+    int m = std::min(n, (int)RC_STAT_COUNT);
+    std::copy(g_batch->stats, g_batch->stats + m, out);
+    return m;
+}
+
+__attribute__((visibility("default"))) void rc_batch_release(void) { release_batch(g_batch); }
 
 // ---- learned emission proposal ----
 static GuideModel* guide_ready() {

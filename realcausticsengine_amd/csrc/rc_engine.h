@@ -75,6 +75,39 @@ struct Counters {
 };
 constexpr int kCounterFields = 9;
 
+// Several scenes rendered by the same launches: global light and camera path
+// indices are cut into per-scene ranges; each scene keeps its own device
+// arrays (SceneView), Philox key, photon-map grid (PhotonMapView, with its
+// own range of the shared bucket table) and film.
+struct BatchScene {
+    uint32_t light_base, light_paths;    // global light-path range
+    uint32_t camera_base, camera_paths;  // global camera-path range
+    uint32_t k0, k1;                     // Philox key (seed) of the scene
+    float vm_norm;                       // 1 / (pi r^2 N) of this iteration
+    uint32_t bucket_base;                // first entry of the scene in the shared bucket table
+    uint32_t iteration;                  // the scene's iteration index (Philox counter)
+    float* image;                        // the scene's film
+};
+struct BatchView {
+    int n;                           // scenes (0: single-scene launch)
+    const SceneView* sc;             // per scene
+    const PhotonMapView* pm;         // per scene (cells already offset by bucket_base)
+    const BatchScene* bs;            // per scene
+    const uint32_t* light_starts;    // n + 1 prefix sums of light paths
+    const uint32_t* camera_starts;   // n + 1 prefix sums of camera paths
+};
+
+// Scene of a global path index: the last s with starts[s] <= i.
+__device__ __forceinline__ int batch_scene_of(const uint32_t* starts, int n, uint32_t i) {
+    int lo = 0, hi = n;  // starts[lo] <= i < starts[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (starts[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 struct LightPassArgs {
     SceneView sc;
     uint32_t k0, k1, iteration, stream;
@@ -88,6 +121,7 @@ struct LightPassArgs {
     uint32_t* out_uid;
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
+    BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1 per scene; no guide)
 };
 
 // One merge query per non-delta camera vertex (80 B, SoA):
@@ -118,6 +152,7 @@ struct CameraPassArgs {
     float* image;
     Counters* counters;  // non-null selects the counting variant
     int exact_merge;     // 1: correctly rounded division/sqrt in the merge
+    BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
 };
 
 // ---- launchers (rc_kernels.hip) ----
@@ -153,6 +188,9 @@ struct MapBuildArgs {
     uint2* cells;         // mask + 1 entries
 };
 void build_photon_map(const MapBuildArgs& a, hipStream_t s);
+// Batch: the keys of every scene's photons in one sort (bucket = the scene's
+// bucket_base + its own cell bucket; uid = global light path * 8 + length).
+void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s);
 size_t photon_map_hist_entries(uint32_t max_count);
 size_t photon_map_sum_entries(uint32_t max_count);
 

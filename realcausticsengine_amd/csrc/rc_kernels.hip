@@ -81,14 +81,15 @@ struct LightState {
     F3 o, d, tp;
     float dvm;
     int pl, diffuse, last;
-    uint32_t path;
+    uint32_t path;   // global light-path index (photon uid = path * 8 + length)
+    uint32_t local;  // index within its scene (the Philox counter)
+    int scene;
 };
 
-template <bool kCount>
+template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
-    const SceneView& sc = a.sc;
     unsigned long long cnt[kCounterFields] = {0};
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
@@ -104,7 +105,18 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
                 st.path = slot;
-                rng.path = slot;
+                st.local = slot;
+                st.scene = 0;
+                if (kBatch) {
+                    st.scene = batch_scene_of(a.batch.light_starts, a.batch.n, slot);
+                    const BatchScene& bs = a.batch.bs[st.scene];
+                    st.local = slot - bs.light_base;
+                    rng.k0 = bs.k0;
+                    rng.k1 = bs.k1;
+                    rng.iter = bs.iteration;
+                }
+                rng.path = st.local;
+                const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
                 // Generate_light_sample (Vertex_Merging.cpp:18-25)
                 int bin;
                 F3 axis;
@@ -128,6 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         if (__ballot(active || !exhausted) == 0) break;
         if (__ballot(active) == 0) continue;
         n_nodes = n_tris = 0;
+        const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
         // one bounce
         bool alive = active;
         Hit h;
@@ -163,7 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
                 alive = false;
             } else {
                 // Sample_Scattering_light (Vertex_Merging.cpp:48-96)
-                rng.path = st.path;
+                rng.path = st.local;
                 U4 w = rng.block((uint32_t)bounce_block(STREAM_LIGHT, st.pl));
                 BsdfSample bs;
                 F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
@@ -193,8 +206,13 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
 }
 
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s) {
-    if (a.counters) hipLaunchKernelGGL(k_light_pass<true>, dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_light_pass<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.batch.n > 0) {
+        if (a.counters) hipLaunchKernelGGL((k_light_pass<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_light_pass<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    } else {
+        if (a.counters) hipLaunchKernelGGL((k_light_pass<true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_light_pass<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    }
 }
 
 // ---------------------------------------------------------------------
@@ -243,15 +261,17 @@ struct CameraState {
     F3 o, d, tp;
     float dvm;
     int pl;
-    uint32_t path, mask;
+    uint32_t path, mask;  // global camera-path index, vertices with a query
+    uint32_t local;       // pixel-sample index within its scene (the Philox counter)
+    int scene;
+    float vm_norm;
     int pixel;
 };
 
-template <bool kCount>
+template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
-    const SceneView& sc = a.sc;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
     unsigned long long cnt[kCounterFields] = {0};
     CameraState st;
@@ -265,12 +285,26 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
                 st.path = slot;
-                rng.path = slot;
+                st.local = slot;
+                st.scene = 0;
+                st.vm_norm = a.vm_norm;
+                if (kBatch) {
+                    st.scene = batch_scene_of(a.batch.camera_starts, a.batch.n, slot);
+                    const BatchScene& bs = a.batch.bs[st.scene];
+                    st.local = slot - bs.camera_base;
+                    st.vm_norm = bs.vm_norm;
+                    rng.k0 = bs.k0;
+                    rng.k1 = bs.k1;
+                    rng.iter = bs.iteration;
+                }
+                rng.path = st.local;
+                const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
                 const int W = sc.W, H = sc.H;
+                const uint32_t px = st.local;
                 // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
                 U4 wj = rng.block(0);
-                float sx = (float)(int)(slot % (uint32_t)W) + 0.0f + 1.0f * u24(wj.a);
-                float sy = (float)(int)(slot / (uint32_t)W) + 0.0f + 1.0f * u24(wj.b);
+                float sx = (float)(int)(px % (uint32_t)W) + 0.0f + 1.0f * u24(wj.a);
+                float sy = (float)(int)(px / (uint32_t)W) + 0.0f + 1.0f * u24(wj.b);
                 F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
                 dir = dir - ld3(sc.cam_o);
                 st.o = ld3(sc.cam_o);
@@ -294,6 +328,7 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
         }
         if (__ballot(active || !exhausted) == 0) break;
         if (__ballot(active) == 0) continue;
+        const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
         bool alive = active;
         Hit h;
         Surface sf;
@@ -318,11 +353,11 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
         bool query = alive && !bsdf_delta(b);
         uint32_t at = wave_alloc(a.path_counter + 1, query, &m);
         if (query && at < a.q.capacity) {
-            F3 tpn = st.tp * a.vm_norm;
+            F3 tpn = st.tp * st.vm_norm;
             a.q.pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
             a.q.dir[at] = make_float4(st.d.x, st.d.y, st.d.z, __int_as_float(sf.mat));
             a.q.sn[at] = make_float4(sf.sn.x, sf.sn.y, sf.sn.z, __uint_as_float(st.path * 8u + (uint32_t)st.pl));
-            a.q.gn[at] = make_float4(sf.gn.x, sf.gn.y, sf.gn.z, 0.0f);
+            a.q.gn[at] = make_float4(sf.gn.x, sf.gn.y, sf.gn.z, __int_as_float(st.scene));
             a.q.tpn[at] = make_float4(tpn.x, tpn.y, tpn.z, 0.0f);
         }
         if (query) st.mask |= 1u << st.pl;
@@ -331,7 +366,7 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
                 alive = false;
             } else {
                 // Sample_Scattering_camera (Vertex_Merging.cpp:98-135)
-                rng.path = st.path;
+                rng.path = st.local;
                 U4 w = rng.block((uint32_t)bounce_block(STREAM_CAMERA, st.pl));
                 BsdfSample bs;
                 F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
@@ -386,7 +421,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // of the image) and its L2 holds that band's photons.
 constexpr int kXcds = 8;
 
-template <bool kCount, bool kFast>
+template <bool kCount, bool kFast, bool kBatch>
 __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     __shared__ uint32_t queue_lds[kGatherWaves][128];
     __shared__ uint32_t rfirst_lds[kGatherWaves][64];
@@ -394,7 +429,6 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     const uint32_t wv = threadIdx.x >> 6;
     uint32_t* queue = queue_lds[wv];
     const uint32_t lane = lane_id();
-    const PhotonMapView& pm = a.pm;
     unsigned long long cnt[kCounterFields] = {0};
     const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
     const uint32_t xcd = blockIdx.x % kXcds;
@@ -403,10 +437,14 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
     for (uint32_t qi = q_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; qi < q_end; qi += local_waves) {
         float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
+        // the query's scene is the same in every lane
+        const int scene = kBatch ? __builtin_amdgcn_readfirstlane(__float_as_int(qg.w)) : 0;
+        const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : a.pm;
+        const DevMaterial* mats = kBatch ? a.batch.sc[scene].mats : a.sc.mats;
         F3 q = f3(qp.x, qp.y, qp.z);
         float cdvm = qp.w;
         Bsdf b;
-        bsdf_setup(b, a.sc.mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
+        bsdf_setup(b, mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
         const MergeConst mc = merge_const(b, f3(qg.x, qg.y, qg.z));
         F3 c = f3s(0.0f);
         auto eval_one = [&](uint32_t i) {
@@ -526,6 +564,7 @@ __global__ void k_resolve(CameraPassArgs a) {
     if (p >= a.n_paths) return;
     int2 info = a.path_info[p];
     if (info.x < 0) return;
+    float* image = a.batch.n > 0 ? a.batch.bs[batch_scene_of(a.batch.camera_starts, a.batch.n, p)].image : a.image;
     F3 color = f3s(0.0f);
     uint32_t mask = (uint32_t)info.y;
     while (mask) {
@@ -535,24 +574,35 @@ __global__ void k_resolve(CameraPassArgs a) {
         color = color + f3(a.contrib[s], a.contrib[s + 1], a.contrib[s + 2]);
     }
     if (color.x != 0.0f || color.y != 0.0f || color.z != 0.0f || (uint32_t)info.y != 0) {
-        unsafeAtomicAdd(&a.image[3 * (size_t)info.x], color.x);
-        unsafeAtomicAdd(&a.image[3 * (size_t)info.x + 1], color.y);
-        unsafeAtomicAdd(&a.image[3 * (size_t)info.x + 2], color.z);
+        unsafeAtomicAdd(&image[3 * (size_t)info.x], color.x);
+        unsafeAtomicAdd(&image[3 * (size_t)info.x + 1], color.y);
+        unsafeAtomicAdd(&image[3 * (size_t)info.x + 2], color.z);
     }
 }
 
 void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
-    if (a.counters) hipLaunchKernelGGL(k_camera_trace<true>, dim3(blocks), dim3(kBlock), 0, s, a);
-    else hipLaunchKernelGGL(k_camera_trace<false>, dim3(blocks), dim3(kBlock), 0, s, a);
+    if (a.batch.n > 0) {
+        if (a.counters) hipLaunchKernelGGL((k_camera_trace<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_camera_trace<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
+    } else {
+        if (a.counters) hipLaunchKernelGGL((k_camera_trace<true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_camera_trace<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
+    }
+}
+template <bool kBatch>
+static void gather_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
+    dim3 g(blocks), b(kGatherBlock);
+    if (a.exact_merge) {
+        if (a.counters) hipLaunchKernelGGL((k_gather<true, false, kBatch>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather<false, false, kBatch>), g, b, 0, s, a);
+    } else {
+        if (a.counters) hipLaunchKernelGGL((k_gather<true, true, kBatch>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather<false, true, kBatch>), g, b, 0, s, a);
+    }
 }
 void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s) {
-    if (a.exact_merge) {
-        if (a.counters) hipLaunchKernelGGL((k_gather<true, false>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_gather<false, false>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
-    } else {
-        if (a.counters) hipLaunchKernelGGL((k_gather<true, true>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
-        else hipLaunchKernelGGL((k_gather<false, true>), dim3(blocks), dim3(kGatherBlock), 0, s, a);
-    }
+    if (a.batch.n > 0) gather_variant<true>(a, blocks, s);
+    else gather_variant<false>(a, blocks, s);
 }
 void launch_resolve(const CameraPassArgs& a, hipStream_t s) {
     if (a.n_paths == 0) return;
@@ -778,6 +828,24 @@ __global__ void k_map_cells(const unsigned long long* keys, uint32_t n, int uid_
     if (i == n - 1 || (uint32_t)(keys[i + 1] >> uid_bits) != b) cells[b].y = i + 1;
 }
 
+__global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, uint32_t n, BatchView b, int uid_bits,
+                                 unsigned long long* keys, uint32_t* vals) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = pos[i];
+    uint32_t u = uid[i];
+    int sc = batch_scene_of(b.light_starts, b.n, u >> 3);
+    const PhotonMapView& pm = b.pm[sc];
+    int cx = (int)floorf((p.x - pm.origin[0]) * pm.inv_cell);
+    int cy = (int)floorf((p.y - pm.origin[1]) * pm.inv_cell);
+    int cz = (int)floorf((p.z - pm.origin[2]) * pm.inv_cell);
+    uint32_t bkt = b.bs[sc].bucket_base + cell_bucket(cx, cy, cz, pm.mask);
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)u;
+    vals[i] = i;
+}
+
+static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s);
+
 void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
     uint32_t n = a.count;
@@ -785,6 +853,25 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     unsigned g = (n + 255) / 256;
     hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, n, a.origin[0], a.origin[1],
                        a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a);
+    sort_permute_cells(a, s);
+}
+
+// a.mask + 1 = the total bucket count of all scenes.
+void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s) {
+    launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
+    uint32_t n = a.count;
+    if (n == 0) return;
+    unsigned g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, n, b, a.uid_bits, a.keys_a,
+                       a.vals_a);
+    sort_permute_cells(a, s);
+}
+
+// LSD radix sort of the keys (8-bit digits, stable), permutation of the
+// photons into key order, per-bucket [start, end).
+static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
+    uint32_t n = a.count;
+    unsigned g = (n + 255) / 256;
     uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
     int bits = a.bucket_bits + a.uid_bits;
     unsigned long long* kin = a.keys_a;

@@ -23,7 +23,8 @@ EXPORTS = [
     "rc_get_image", "rc_write_exr", "rc_release", "rc_last_error", "rc_cancel", "rc_stats", "rc_first_hit",
     "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
-    "rc_guide_records",
+    "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
+    "rc_batch_get_image", "rc_batch_stats", "rc_batch_release",
 ]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
@@ -97,6 +98,14 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_guide_table.restype = ctypes.c_longlong
     lib.rc_guide_records.argtypes = [FP, IP, ctypes.c_longlong]
     lib.rc_guide_records.restype = ctypes.c_longlong
+    for fn in (lib.rc_batch_add, lib.rc_batch_size, lib.rc_batch_prepare, lib.rc_batch_render_iteration):
+        fn.argtypes = []
+        fn.restype = ctypes.c_int
+    lib.rc_batch_get_image.argtypes = [ctypes.c_int, FP, ctypes.c_int]
+    lib.rc_batch_get_image.restype = ctypes.c_int
+    lib.rc_batch_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
+    lib.rc_batch_stats.restype = ctypes.c_int
+    lib.rc_batch_release.restype = None
     _lib = lib
     return lib
 
@@ -248,6 +257,51 @@ class Engine:
 
     def release(self):
         self.lib.rc_release()
+
+
+class Batch:
+    """Several scenes rendered by the same launches (rc_batch_*)."""
+
+    def __init__(self, scenes, options=None):
+        """scenes: SceneArgs list; options: one dict of rc_set_option values per scene (or None)."""
+        self.lib = load()
+        self.lib.rc_batch_release()
+        self.sizes = []
+        for i, args in enumerate(scenes):
+            for k, v in (options[i] if options else {}).items():
+                if self.lib.rc_set_option(k.encode(), float(v)) != 0:
+                    raise KeyError(k)
+            self.lib.init(*args.init_args())
+            if self.lib.rc_batch_add() < 0:
+                raise EngineError(f"rc_batch_add: {self.lib.rc_last_error().decode()}")
+            self.sizes.append((args.camera.width, args.camera.height))
+        self._keep = list(scenes)
+
+    def _check(self, rc, what):
+        if rc < 0:
+            raise EngineError(f"{what}: {self.lib.rc_last_error().decode()}")
+        return rc
+
+    def prepare(self):
+        self._check(self.lib.rc_batch_prepare(), "rc_batch_prepare")
+        return self
+
+    def render_iteration(self) -> int:
+        return self._check(self.lib.rc_batch_render_iteration(), "rc_batch_render_iteration")
+
+    def image(self, scene: int, scaled: bool = True) -> np.ndarray:
+        W, H = self.sizes[scene]
+        out = np.zeros(W * H * 3, dtype=np.float32)
+        self._check(self.lib.rc_batch_get_image(scene, _fp(out), int(scaled)), "rc_batch_get_image")
+        return out.reshape(H, W, 3)
+
+    def stats(self) -> dict:
+        buf = (ctypes.c_double * len(STAT_NAMES))()
+        n = self.lib.rc_batch_stats(buf, len(STAT_NAMES))
+        return {STAT_NAMES[i]: buf[i] for i in range(n)}
+
+    def release(self):
+        self.lib.rc_batch_release()
 
 
 def photons_from_oracle(records: np.ndarray) -> np.ndarray:

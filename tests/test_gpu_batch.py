@@ -1,0 +1,74 @@
+"""Scenes rendered together (rc_batch_*) against the same scenes rendered alone.
+
+Each scene of a batch keeps its own Philox key, iteration index, photon-map
+grid and film, its photons sort into its own bucket range in the same
+(bucket, path, length) order as alone, and every pixel receives one addition
+per iteration: the batched film equals the film of the scene rendered alone,
+bit for bit."""
+import numpy as np
+import pytest
+
+from realcausticsengine_amd import scenes
+from realcausticsengine_amd.engine import Batch, Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases():
+    return [
+        (scenes.water_scene(32, 24, 16, samples=2), dict(photons=20000, seed=3)),
+        (scenes.c0_scene(32, 32, subdiv=2, samples=2), dict(photons=50000, seed=5, max_path_length=4)),
+        (scenes.random_mesh_scene(seed=1, width=16, height=24), dict(photons=30011, seed=7, compat_rows=0)),
+        (scenes.slab_scene(20, 20, samples=2), dict(photons=4096, seed=11)),
+    ]
+
+
+def _alone(args, opts, iters):
+    eng = Engine(args, write_exr=0, **opts)
+    eng.prepare()
+    for _ in range(iters):
+        eng.render_iteration()
+    img = eng.image(scaled=False)
+    eng.release()
+    return img
+
+
+def test_batch_matches_scenes_rendered_alone():
+    cases = _cases()
+    iters = 2
+    alone = [_alone(a, o, iters) for a, o in cases]
+    b = Batch([a for a, _ in cases], [dict(o, write_exr=0) for _, o in cases]).prepare()
+    for i in range(iters):
+        assert b.render_iteration() == i
+    st = b.stats()
+    assert st["light_paths"] == sum(o["photons"] for _, o in cases)
+    assert st["camera_paths"] == sum(a.camera.width * a.camera.height for a, _ in cases)
+    for k, (a, _) in enumerate(cases):
+        img = b.image(k, scaled=False)
+        print(f"scene {k}: sum {img.sum():.6g} alone {alone[k].sum():.6g}", flush=True)
+        assert img.sum() > 0
+        assert np.array_equal(img, alone[k]), k
+    b.release()
+
+
+def test_batch_scene_order_does_not_matter():
+    cases = _cases()[:3]
+    b1 = Batch([a for a, _ in cases], [dict(o, write_exr=0) for _, o in cases]).prepare()
+    b1.render_iteration()
+    first = [b1.image(k, scaled=False) for k in range(3)]
+    b1.release()
+    rev = cases[::-1]
+    b2 = Batch([a for a, _ in rev], [dict(o, write_exr=0) for _, o in rev]).prepare()
+    b2.render_iteration()
+    for k in range(3):
+        assert np.array_equal(b2.image(2 - k, scaled=False), first[k])
+    b2.release()
+
+
+def test_batch_rejects_mixed_merge_arithmetic():
+    cases = _cases()[:2]
+    opts = [dict(cases[0][1], write_exr=0), dict(cases[1][1], write_exr=0, exact_merge=1)]
+    b = Batch([a for a, _ in cases], opts)
+    with pytest.raises(Exception, match="exact_merge"):
+        b.prepare()
+    b.release()
