@@ -90,6 +90,52 @@ def cpu_baseline(args, photons: int, stride: int) -> dict:
     }
 
 
+def uneven_batch(a):
+    """The uneven-scene batch (scenes.uneven_batch: seeded displaced glass blobs,
+    320 .. 327,680 triangles, 256^2 .. 1024^2, 1 .. 8 x 2^20 light paths each):
+    one batched iteration (one launch per pass for every scene) against the
+    same scenes rendered one after another with their own launches."""
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import Batch
+
+    t = time.perf_counter()
+    cases = scenes.uneven_batch(a.batch_scenes)
+    b = Batch([sc for sc, _ in cases], [dict(photons=p, seed=i, write_exr=0) for i, (_, p) in enumerate(cases)])
+    b.prepare()
+    prep_s = time.perf_counter() - t
+    print(f"[bench] uneven batch: {len(cases)} scenes ready in {prep_s:.1f} s", file=sys.stderr, flush=True)
+    b.render_iteration()  # warm-up: buffers of the batch and of every scene alone
+    for i in range(len(cases)):
+        b.render_scene(i)
+    t0 = time.perf_counter()
+    for _ in range(a.batch_steps):
+        b.render_iteration()
+    batched = (time.perf_counter() - t0) / a.batch_steps
+    st = b.stats()
+    t0 = time.perf_counter()
+    for _ in range(a.batch_steps):
+        for i in range(len(cases)):
+            b.render_scene(i)
+    sequential = (time.perf_counter() - t0) / a.batch_steps
+    b.release()
+    light = sum(p for _, p in cases)
+    out = {
+        "scenes": len(cases),
+        "triangles": int(sum(sc.total_tris for sc, _ in cases)),
+        "light_paths_per_iteration": int(light),
+        "camera_paths_per_iteration": int(sum(sc.camera.width * sc.camera.height for sc, _ in cases)),
+        "ms_per_iteration_batched": round(batched * 1e3, 2),
+        "ms_per_iteration_one_scene_at_a_time": round(sequential * 1e3, 2),
+        "speedup": round(sequential / batched, 3),
+        "photon_paths_per_s_batched": round(light / batched, 1),
+        "kernel_ms_batched": {k: round(st[k], 3) for k in ("ms_light", "ms_map", "ms_camera_trace", "ms_gather",
+                                                            "ms_resolve")},
+        "prepare_s": round(prep_s, 2),
+    }
+    print(f"[bench]   {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def time_to_rmse(a, cpu_s_per_iteration):
     """Wall time of rc_render_iteration calls until the running image reaches
     RMSE <= target against the FP64 reference (readback and scoring untimed)."""
@@ -181,6 +227,9 @@ def main() -> int:
     ap.add_argument("--ttr-max-iterations", type=int, default=3000)
     ap.add_argument("--ttr-max-seconds", type=float, default=75.0)
     ap.add_argument("--guide-train-every", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-scene batch leg")
+    ap.add_argument("--batch-scenes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=3)
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one MI355X: the path has no multi-GPU mode in this build "
@@ -240,6 +289,9 @@ def main() -> int:
     if not a.no_cpu_baseline:
         print("[bench] timing the CPU oracle sample", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args, a.photons, a.cpu_stride)
+    batch = None
+    if not a.no_batch:
+        batch = uneven_batch(a)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None)
@@ -263,6 +315,7 @@ def main() -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
+        "uneven_batch": batch,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)

@@ -149,6 +149,7 @@ int rc_batch_add(void);               /* returns the scene's index in the batch 
 int rc_batch_size(void);
 int rc_batch_prepare(void);           /* BVH build and upload of every scene */
 int rc_batch_render_iteration(void);  /* one iteration of every scene; returns its index or < 0 */
+int rc_batch_render_scene(int scene); /* one iteration of one scene alone (its own launches) */
 int rc_batch_get_image(int scene, float *rgb, int scaled);
 int rc_batch_stats(double *out, int n);  /* RC_STAT_* totals of the last batch iteration */
 void rc_batch_release(void);

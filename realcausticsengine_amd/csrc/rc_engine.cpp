@@ -11,7 +11,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/real_caustics.h"
@@ -24,8 +26,14 @@ using namespace rc;
 
 namespace rc {
 std::string g_error;
+std::mutex g_error_mutex;  // batch preparation runs scenes on several host threads
 void set_error(const std::string& e) {
+    std::lock_guard<std::mutex> lock(g_error_mutex);
     if (g_error.empty()) g_error = e;
+}
+bool has_error() {
+    std::lock_guard<std::mutex> lock(g_error_mutex);
+    return !g_error.empty();
 }
 }  // namespace rc
 
@@ -480,7 +488,7 @@ bool prepare(Engine& e) {
         set_error("main/rc_prepare called before init");
         return false;
     }
-    if (!g_error.empty()) return false;
+    if (has_error()) return false;
     if (!ensure_device(e)) return false;
     size_t ntri = e.tris.size() / 4;
     if (ntri == 0) {
@@ -1078,8 +1086,21 @@ bool batch_prepare(Batch& b) {
             set_error("batch scenes must agree on the option exact_merge");
             return false;
         }
-        if (!prepare(*e)) return false;
     }
+    // BVH builds and uploads of the scenes on parallel host threads
+    std::atomic<size_t> next{0};
+    std::atomic<bool> ok{true};
+    auto worker = [&]() {
+        for (size_t i; (i = next++) < b.scenes.size();)
+            if (!prepare(*b.scenes[i])) ok = false;
+    };
+    size_t nt = std::max<size_t>(1, std::min<size_t>(b.scenes.size(), std::thread::hardware_concurrency()));
+    nt = std::min<size_t>(nt, 16);
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto& t : pool) t.join();
+    if (!ok) return false;
     if (!ensure_device(b.work)) return false;
     if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
@@ -1138,8 +1159,9 @@ bool batch_iteration(Batch& b) {
     }
     starts[n] = (uint32_t)light;
     starts[2 * n + 1] = (uint32_t)camera;
-    if (light > (1ull << 28) || camera > (1ull << 28) || buckets > (1ull << 30)) {
-        set_error("batch too large for one launch (light or camera paths > 2^28)");
+    // uids (light) and query slots (camera) are path * 8 + length in 32 bits
+    if (light >= (1ull << 29) || camera >= (1ull << 29) || buckets > (1ull << 31)) {
+        set_error("batch too large for one launch (light or camera paths >= 2^29)");
         return false;
     }
     int uid_bits = log2_ceil(light) + 3;
@@ -1148,8 +1170,8 @@ bool batch_iteration(Batch& b) {
         set_error("photon map key does not fit 64 bits");
         return false;
     }
-    // shared buffers
-    uint32_t cap = std::max<uint32_t>(w.capacity, (uint32_t)std::max<uint64_t>(light, 1 << 16));
+    // shared buffers; photons start at a quarter of the paths and grow on overflow
+    uint32_t cap = std::max<uint32_t>(w.capacity, (uint32_t)std::max<uint64_t>(light / 4, 1 << 20));
     if (!ensure_photon_buffers(w, cap) || !w.cells.alloc(buckets)) return false;
     for (size_t i = 0; i < n; ++i) {
         pm[i].pos = w.s_pos.p;
@@ -1828,6 +1850,19 @@ __attribute__((visibility("default"))) int rc_batch_render_iteration(void) {
     }
     int it = g_batch->iterations;
     return batch_iteration(*g_batch) ? it : -1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_render_scene(int scene) {
+    if (rc_batch_prepare() != 0) return -1;
+    if (scene < 0 || scene >= (int)g_batch->scenes.size()) {
+        set_error("rc_batch_render_scene: no such scene");
+        return -1;
+    }
+    Engine& e = *g_batch->scenes[scene];
+    int it = e.iterations_done;
+    if (!render_iteration(e, it)) return -1;
+    e.iterations_done = it + 1;
+    return it;
 }
 
 __attribute__((visibility("default"))) int rc_batch_get_image(int scene, float* rgb, int scaled) {

@@ -24,7 +24,7 @@ EXPORTS = [
     "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
-    "rc_batch_get_image", "rc_batch_stats", "rc_batch_release",
+    "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release",
 ]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
@@ -101,6 +101,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     for fn in (lib.rc_batch_add, lib.rc_batch_size, lib.rc_batch_prepare, lib.rc_batch_render_iteration):
         fn.argtypes = []
         fn.restype = ctypes.c_int
+    lib.rc_batch_render_scene.argtypes = [ctypes.c_int]
+    lib.rc_batch_render_scene.restype = ctypes.c_int
     lib.rc_batch_get_image.argtypes = [ctypes.c_int, FP, ctypes.c_int]
     lib.rc_batch_get_image.restype = ctypes.c_int
     lib.rc_batch_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
@@ -288,6 +290,10 @@ class Batch:
 
     def render_iteration(self) -> int:
         return self._check(self.lib.rc_batch_render_iteration(), "rc_batch_render_iteration")
+
+    def render_scene(self, scene: int) -> int:
+        """One iteration of one scene with its own launches (the one-at-a-time baseline)."""
+        return self._check(self.lib.rc_batch_render_scene(scene), "rc_batch_render_scene")
 
     def image(self, scene: int, scaled: bool = True) -> np.ndarray:
         W, H = self.sizes[scene]

@@ -183,3 +183,62 @@ def random_mesh_scene(seed: int = 0, n_spheres: int = 3, subdiv: int = 2, width:
     lights = [LightData("A", position=(0.0, 0.0, 6.0), width=2.0, height=1.0, power=2.0, rotation=(10.0, 0.0, 20.0))]
     cam = look_at_camera(width, height, (0.0, -8.0, 5.0), (0.0, 0.0, 0.5), 40.0)
     return SceneArgs(meshes, lights, mats, cam, samples=1)
+
+
+_ICO_CACHE: dict = {}
+
+
+def _icosphere_cached(subdiv: int):
+    if subdiv not in _ICO_CACHE:
+        _ICO_CACHE[subdiv] = icosphere(subdiv)
+    return _ICO_CACHE[subdiv]
+
+
+def displaced_sphere(subdiv: int, seed: int, amplitude: float = 0.1, waves: int = 8):
+    """Icosphere with a radial displacement: amplitude * mean of `waves` seeded
+    spherical sines sin(f (n . d) + phase); normals from the displaced faces."""
+    v0, _, tris = _icosphere_cached(subdiv)
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(waves, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    f = rng.uniform(2.0, 8.0, waves)
+    ph = rng.uniform(0.0, 2.0 * np.pi, waves)
+    n = v0.astype(np.float64)
+    r = 1.0 + amplitude * np.mean(np.sin((n @ d.T) * f + ph), axis=1)
+    verts = n * r[:, None]
+    a, b, c = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    fn = np.cross(b - a, c - a)  # area-weighted face normals
+    vn = np.zeros_like(verts)
+    for k in range(3):
+        np.add.at(vn, tris[:, k], fn)
+    vn /= np.maximum(np.linalg.norm(vn, axis=1, keepdims=True), 1e-30)
+    return verts.astype(np.float32), vn, tris
+
+
+def blob_scene(seed: int, samples: int = 1):
+    """One scene of the uneven batch: a seeded displaced glass blob over a
+    diffuse plane.  The triangle count is drawn log-uniform in [1e3, 5e5] and
+    met by the icosphere level nearest to it in log scale (320 .. 327,680
+    triangles); IOR U(1.3, 1.7); resolution from {256, 512, 1024}^2; 1..8 x 2^20
+    light paths per iteration.  Returns (SceneArgs, light paths)."""
+    rng = np.random.default_rng(1000 + seed)
+    want = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
+    levels = np.arange(2, 8)
+    subdiv = int(levels[np.argmin(np.abs(np.log(20.0 * 4.0 ** levels) - np.log(want)))])
+    ior = float(rng.uniform(1.3, 1.7))
+    rough = float(rng.uniform(0.0, 0.3))  # 'G' glass is delta in the reference: recorded, not used
+    res = int(rng.choice([256, 512, 1024]))
+    photons = int(rng.integers(1, 9)) << 20
+    bv, bn, bt = displaced_sphere(subdiv, seed)
+    pv, pn, pt = plane(10.0, 10.0, 0.0)
+    meshes = [MeshData(bv, bn, bt, matrix=translation(0.0, 0.0, 1.3), material=0), MeshData(pv, pn, pt, material=1)]
+    mats = [MaterialData("G", ior=ior, roughness=rough, color=(1.0, 1.0, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.8, 0.8))]
+    lights = [LightData("A", position=(0.0, 0.0, 5.0), width=1.0, height=1.0, power=1.0)]
+    cam = look_at_camera(res, res, (0.0, -6.0, 4.0), (0.0, 0.0, 0.8), 40.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=0.01, radius_reduction=0.75), photons
+
+
+def uneven_batch(n: int = 64, samples: int = 1):
+    """The uneven-scene batch: blob_scene(seed) for seeds 0 .. n-1."""
+    return [blob_scene(s, samples) for s in range(n)]
