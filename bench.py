@@ -104,6 +104,10 @@ def uneven_batch(a):
     b.prepare()
     prep_s = time.perf_counter() - t
     print(f"[bench] uneven batch: {len(cases)} scenes ready in {prep_s:.1f} s", file=sys.stderr, flush=True)
+    tuned = None
+    if not a.no_autotune:
+        tuned = {k: round(v, 3) for k, v in b.autotune(repeats=1).items()}
+        print(f"[bench]   batch autotune: {tuned}", file=sys.stderr, flush=True)
     b.render_iteration()  # warm-up: buffers of the batch and of every scene alone
     for i in range(len(cases)):
         b.render_scene(i)
@@ -131,6 +135,7 @@ def uneven_batch(a):
         "kernel_ms_batched": {k: round(st[k], 3) for k in ("ms_light", "ms_map", "ms_camera_trace", "ms_gather",
                                                             "ms_resolve")},
         "prepare_s": round(prep_s, 2),
+        "autotune": tuned,
     }
     print(f"[bench]   {out}", file=sys.stderr, flush=True)
     return out
@@ -228,6 +233,7 @@ def main() -> int:
     ap.add_argument("--ttr-max-seconds", type=float, default=75.0)
     ap.add_argument("--guide-train-every", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-scene batch leg")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     a = ap.parse_args()
@@ -246,6 +252,10 @@ def main() -> int:
     prep_s = time.perf_counter() - t
     print(f"[bench] scene ready: {args.total_tris} triangles, BVH {eng.get('nodes'):.0f} nodes, {prep_s:.2f} s",
           file=sys.stderr, flush=True)
+    tuned = None
+    if not a.no_autotune:  # launch shapes and queue / tile sizes for this scene (untimed; renders nothing)
+        tuned = {k: round(v, 3) for k, v in eng.autotune(repeats=2).items()}
+        print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
         eng.render_iteration()
     keys = {"light_pass": "ms_light", "photon_map": "ms_map", "camera_trace": "ms_camera_trace",
@@ -274,6 +284,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom)}
     extra = {
+        "autotune": tuned,
         "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
         "kernel_achieved_GBs": {k: round(by[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern if kern[k] > 0},
         "algorithmic_bytes": {k: int(v) for k, v in by.items()},

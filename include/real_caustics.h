@@ -120,6 +120,19 @@ enum {
     RC_STAT_COUNT
 };
 
+/* Launch shapes and queue / tile sizes are options too: "path_chunk" (path
+ * indices a wave takes per queue refill, a multiple of 64), "light_blocks_per_cu",
+ * "camera_blocks_per_cu", "gather_blocks_per_cu" (persistent workgroups per
+ * compute unit) and "cell_scale" (photon-map cell size over the merge radius).
+ * rc_autotune chooses them for the current scene: every candidate renders the
+ * scene's next iteration into a scratch film (the film and the iteration count
+ * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
+ * kernels it affects.  report (up to RC_TUNE_REPORT doubles): kernel ms per
+ * iteration before and after, then path_chunk, light_blocks_per_cu,
+ * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu. */
+enum { RC_TUNE_REPORT = 7 };
+int rc_autotune(int repeats, double *report, int n);
+
 /* ---- diagnostics used by the parity tests ---- */
 
 /* Closest hit for n rays {o.xyz, d.xyz}; tri = global triangle id (mesh order,
@@ -152,6 +165,7 @@ int rc_batch_render_iteration(void);  /* one iteration of every scene; returns i
 int rc_batch_render_scene(int scene); /* one iteration of one scene alone (its own launches) */
 int rc_batch_get_image(int scene, float *rgb, int scaled);
 int rc_batch_stats(double *out, int n);  /* RC_STAT_* totals of the last batch iteration */
+int rc_batch_autotune(int repeats, double *report, int n);  /* as rc_autotune, for the whole batch */
 void rc_batch_release(void);
 
 /* ---- learned emission proposal (no counterpart in the reference) ----

@@ -11,6 +11,8 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -76,6 +78,9 @@ struct Engine {
     int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
     double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
+    // launch shapes and queue / tile sizes (options, or chosen by rc_autotune)
+    int light_blocks_per_cu = 8, camera_blocks_per_cu = 8, gather_blocks_per_cu = 16;
+    uint32_t path_chunk = 256;
     int count_events = 0;
     int exact_merge = 0;
     std::string output_path = "D:\\caustics.exr";
@@ -607,9 +612,18 @@ int log2_ceil(uint64_t v) {
     return b;
 }
 
-int blocks_for(const Engine& e) {
+int light_blocks(const Engine& e) {
     if (e.persistent_blocks > 0) return e.persistent_blocks;
-    return std::max(1, e.device_cus) * 8;
+    return std::max(1, e.device_cus) * e.light_blocks_per_cu;
+}
+int camera_blocks(const Engine& e) {
+    if (e.persistent_blocks > 0) return e.persistent_blocks;
+    return std::max(1, e.device_cus) * e.camera_blocks_per_cu;
+}
+// a multiple of the 8 XCDs (the gather assigns contiguous query segments per XCD)
+int gather_blocks(const Engine& e) {
+    int b = std::max(8, e.device_cus) * e.gather_blocks_per_cu;
+    return (b + 7) / 8 * 8;
 }
 
 bool ensure_photon_buffers(Engine& e, uint32_t cap) {
@@ -674,6 +688,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.iteration = (uint32_t)iteration;
         a.stream = STREAM_LIGHT;
         a.n_paths = (uint32_t)N;
+        a.path_chunk = e.path_chunk;
         a.path_counter = e.counters.p;
         a.photon_counter = e.counters.p + 1;
         a.capacity = e.capacity;
@@ -684,7 +699,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
         a.guide = guide_light_view(e, iteration);
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
-        launch_light_pass(a, blocks_for(e), e.stream);
+        launch_light_pass(a, light_blocks(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
         HIP_OK(hipGetLastError());
         uint32_t c = 0;
@@ -780,6 +795,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.iteration = (uint32_t)iteration;
     a.stream = STREAM_CAMERA;
     a.n_paths = n_paths;
+    a.path_chunk = e.path_chunk;
     a.path_counter = e.counters.p;
     a.vm_norm = norm;
     a.path_info = e.path_info.p;
@@ -791,7 +807,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         HIP_OK(hipEventRecord(e.ev[4], e.stream));
-        launch_camera_trace(a, blocks_for(e), e.stream);
+        launch_camera_trace(a, camera_blocks(e), e.stream);
         HIP_OK(hipGetLastError());
         uint32_t nq = 0;
         HIP_OK(hipMemcpyAsync(&nq, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
@@ -806,7 +822,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p + 1), 0u, kCounterFields * 2, e.stream);
     }
     HIP_OK(hipEventRecord(e.ev[6], e.stream));
-    launch_gather(a, std::max(8, e.device_cus) * 16, e.stream);  // a multiple of the 8 XCDs
+    launch_gather(a, gather_blocks(e), e.stream);
     HIP_OK(hipEventRecord(e.ev[7], e.stream));
     launch_resolve(a, e.stream);
     HIP_OK(hipEventRecord(e.ev[5], e.stream));
@@ -907,6 +923,122 @@ bool render_iteration(Engine& e, int iteration) {
         e.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)c[1].photons_accepted;
     }
     return true;
+}
+
+// ---------------------------------------------------------------------
+// Autotuner of the launch shapes and queue / tile sizes: every candidate
+// renders the same iteration (same photons, same queries) into a scratch
+// film; each knob keeps the value with the smallest time of the kernels it
+// affects (best of `repeats`), one knob after the other.
+// ---------------------------------------------------------------------
+enum { T_LIGHT = 0, T_MAP, T_TRACE, T_GATHER, T_PARTS };
+
+struct TuneKnob {
+    const char* name;
+    std::vector<double> candidates;
+    std::function<void(double)> set;
+    std::function<double()> get;
+    std::function<double(const double*)> cost;
+};
+
+std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::function<void(double)> set_cell_scale,
+                                 std::function<double()> get_cell_scale) {
+    auto L = [](const double* t) { return t[T_LIGHT]; };
+    auto T = [](const double* t) { return t[T_TRACE]; };
+    auto G = [](const double* t) { return t[T_GATHER]; };
+    return {
+        {"path_chunk", {64, 128, 256, 512, 1024},
+         [=](double v) { launch_owner().path_chunk = (uint32_t)v; },
+         [=]() { return (double)launch_owner().path_chunk; },
+         [](const double* t) { return t[T_LIGHT] + t[T_TRACE]; }},
+        {"light_blocks_per_cu", {4, 6, 8, 12, 16},
+         [=](double v) { launch_owner().light_blocks_per_cu = (int)v; },
+         [=]() { return (double)launch_owner().light_blocks_per_cu; }, L},
+        {"camera_blocks_per_cu", {4, 6, 8, 12, 16},
+         [=](double v) { launch_owner().camera_blocks_per_cu = (int)v; },
+         [=]() { return (double)launch_owner().camera_blocks_per_cu; }, T},
+        {"cell_scale", {0.67, 0.8, 1.0, 1.25}, set_cell_scale, get_cell_scale,
+         [](const double* t) { return t[T_MAP] + t[T_GATHER]; }},
+        {"gather_blocks_per_cu", {8, 12, 16, 24, 32},
+         [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
+         [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
+    };
+}
+
+// report: [sum of the kernel times before, after, then the chosen value of each knob]
+bool run_autotune(std::vector<TuneKnob> knobs, std::function<bool(double*)> timed, int repeats, double* report) {
+    repeats = std::max(1, repeats);
+    double t[T_PARTS];
+    if (!timed(t)) return false;  // warm-up: buffers, code objects
+    auto measure = [&](double* best) -> bool {
+        for (int k = 0; k < T_PARTS; ++k) best[k] = std::numeric_limits<double>::infinity();
+        for (int r = 0; r < repeats; ++r) {
+            if (!timed(t)) return false;
+            for (int k = 0; k < T_PARTS; ++k) best[k] = std::min(best[k], t[k]);
+        }
+        return true;
+    };
+    double before[T_PARTS];
+    if (!measure(before)) return false;
+    for (TuneKnob& kb : knobs) {
+        double keep = kb.get(), best_cost = std::numeric_limits<double>::infinity(), best_v = keep;
+        for (double v : kb.candidates) {
+            kb.set(v);
+            double m[T_PARTS];
+            if (!measure(m)) return false;
+            double c = kb.cost(m);
+            if (c < best_cost) {
+                best_cost = c;
+                best_v = v;
+            }
+        }
+        kb.set(best_v);
+    }
+    double after[T_PARTS];
+    if (!measure(after)) return false;
+    double sb = 0, sa = 0;
+    for (int k = 0; k < T_PARTS; ++k) {
+        sb += before[k];
+        sa += after[k];
+    }
+    report[0] = sb;
+    report[1] = sa;
+    for (size_t i = 0; i < knobs.size(); ++i) report[2 + i] = knobs[i].get();
+    return true;
+}
+
+// One iteration `it` into `film`, without training; kernel times in t.
+bool timed_iteration(Engine& e, int it, float* film, double* t) {
+    float radius, r2, norm;
+    iteration_params(e, it, radius, r2, norm);
+    uint32_t count = 0;
+    if (!run_light_pass(e, it, count)) return false;
+    MapInfo mi;
+    if (!build_map(e, count, radius, r2, mi)) return false;
+    if (!run_camera_pass(e, it, mi, norm, film)) return false;
+    HIP_OK(hipStreamSynchronize(e.stream));
+    float ms[T_PARTS];
+    HIP_OK(hipEventElapsedTime(&ms[T_LIGHT], e.ev[0], e.ev[1]));
+    HIP_OK(hipEventElapsedTime(&ms[T_MAP], e.ev[2], e.ev[3]));
+    HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[6]));
+    HIP_OK(hipEventElapsedTime(&ms[T_GATHER], e.ev[6], e.ev[7]));
+    for (int k = 0; k < T_PARTS; ++k) t[k] = ms[k];
+    return true;
+}
+
+bool autotune(Engine& e, int repeats, double* report) {
+    if (!prepare(e)) return false;
+    DevBuf<float> film;
+    if (!film.alloc((size_t)e.W * e.H * 3)) return false;
+    const int it = e.iterations_done, counting = e.count_events;
+    e.count_events = 0;
+    Engine* pe = &e;
+    auto knobs = tune_knobs([pe]() -> Engine& { return *pe; }, [pe](double v) { pe->cell_scale = v; },
+                            [pe]() { return pe->cell_scale; });
+    bool ok = run_autotune(knobs, [&](double* t) { return timed_iteration(e, it, film.p, t); }, repeats, report);
+    e.count_events = counting;
+    film.release();
+    return ok;
 }
 
 // ---------------------------------------------------------------------
@@ -1041,6 +1173,7 @@ struct Batch {
     DevBuf<PhotonMapView> d_pm;
     DevBuf<BatchScene> d_bs;
     DevBuf<uint32_t> d_starts;  // light starts (n + 1), then camera starts (n + 1)
+    DevBuf<float> scratch;      // films of the autotuner's iterations
     double stats[RC_STAT_COUNT] = {0};
     int iterations = 0;
 };
@@ -1065,6 +1198,7 @@ void release_batch(Batch*& b) {
     b->d_pm.release();
     b->d_bs.release();
     b->d_starts.release();
+    b->scratch.release();
     for (auto& ev : w.ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : w.ev_guide)
@@ -1113,8 +1247,9 @@ bool batch_prepare(Batch& b) {
 }
 
 // One iteration of every scene: light pass, photon map, camera trace,
-// gather and resolve, each a single launch over all scenes.
-bool batch_iteration(Batch& b) {
+// gather and resolve, each a single launch over all scenes.  With `scratch`
+// the films and iteration counts stay untouched (autotuner runs).
+bool batch_iteration(Batch& b, bool scratch = false) {
     Engine& w = b.work;
     hipStream_t st = w.stream;
     const size_t n = b.scenes.size();
@@ -1159,6 +1294,10 @@ bool batch_iteration(Batch& b) {
     }
     starts[n] = (uint32_t)light;
     starts[2 * n + 1] = (uint32_t)camera;
+    if (scratch) {  // films of the autotuner's runs: one scratch buffer, scenes at their camera offsets
+        if (!b.scratch.alloc(3 * camera)) return false;
+        for (size_t i = 0; i < n; ++i) bs[i].image = b.scratch.p + 3 * (size_t)bs[i].camera_base;
+    }
     // uids (light) and query slots (camera) are path * 8 + length in 32 bits
     if (light >= (1ull << 29) || camera >= (1ull << 29) || buckets > (1ull << 31)) {
         set_error("batch too large for one launch (light or camera paths >= 2^29)");
@@ -1199,6 +1338,7 @@ bool batch_iteration(Batch& b) {
         a.iteration = (uint32_t)e0.iterations_done;
         a.stream = STREAM_LIGHT;
         a.n_paths = (uint32_t)light;
+        a.path_chunk = w.path_chunk;
         a.path_counter = w.counters.p;
         a.photon_counter = w.counters.p + 1;
         a.capacity = w.capacity;
@@ -1209,7 +1349,7 @@ bool batch_iteration(Batch& b) {
         a.guide = plain_guide_view();
         a.batch = view;
         HIP_OK(hipEventRecord(w.ev[0], st));
-        launch_light_pass(a, blocks_for(w), st);
+        launch_light_pass(a, light_blocks(w), st);
         HIP_OK(hipEventRecord(w.ev[1], st));
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(&count, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
@@ -1253,6 +1393,7 @@ bool batch_iteration(Batch& b) {
     c.iteration = (uint32_t)e0.iterations_done;
     c.stream = STREAM_CAMERA;
     c.n_paths = (uint32_t)camera;
+    c.path_chunk = w.path_chunk;
     c.path_counter = w.counters.p;
     c.path_info = w.path_info.p;
     c.contrib = w.contrib.p;
@@ -1263,7 +1404,7 @@ bool batch_iteration(Batch& b) {
         c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
         launch_fill_u32(w.counters.p, 0u, 4, st);
         HIP_OK(hipEventRecord(w.ev[4], st));
-        launch_camera_trace(c, blocks_for(w), st);
+        launch_camera_trace(c, camera_blocks(w), st);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(&nq, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
         HIP_OK(hipStreamSynchronize(st));
@@ -1275,7 +1416,7 @@ bool batch_iteration(Batch& b) {
         }
     }
     HIP_OK(hipEventRecord(w.ev[6], st));
-    launch_gather(c, std::max(8, w.device_cus) * 16, st);
+    launch_gather(c, gather_blocks(w), st);
     HIP_OK(hipEventRecord(w.ev[7], st));
     launch_resolve(c, st);
     HIP_OK(hipEventRecord(w.ev[5], st));
@@ -1300,9 +1441,29 @@ bool batch_iteration(Batch& b) {
     b.stats[RC_STAT_MS_GATHER] = ms[4];
     b.stats[RC_STAT_MS_RESOLVE] = ms[5];
     b.stats[RC_STAT_SORT_PASSES] = (bucket_bits + uid_bits + 7) / 8;
+    if (scratch) return true;
     for (Engine* e : b.scenes) e->iterations_done++;
     b.iterations++;
     return true;
+}
+
+bool batch_autotune(Batch& b, int repeats, double* report) {
+    if (!batch_prepare(b)) return false;
+    Batch* pb = &b;
+    auto knobs = tune_knobs([pb]() -> Engine& { return pb->work; },
+                            [pb](double v) {
+                                for (Engine* e : pb->scenes) e->cell_scale = v;
+                            },
+                            [pb]() { return pb->scenes[0]->cell_scale; });
+    auto timed = [pb](double* t) {
+        if (!batch_iteration(*pb, true)) return false;
+        t[T_LIGHT] = pb->stats[RC_STAT_MS_LIGHT];
+        t[T_MAP] = pb->stats[RC_STAT_MS_MAP];
+        t[T_TRACE] = pb->stats[RC_STAT_MS_CAMERA_TRACE];
+        t[T_GATHER] = pb->stats[RC_STAT_MS_GATHER];
+        return true;
+    };
+    return run_autotune(knobs, timed, repeats, report);
 }
 
 }  // namespace
@@ -1336,6 +1497,10 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         e->photon_capacity = prev->photon_capacity;
         e->cell_scale = prev->cell_scale;
         e->persistent_blocks = prev->persistent_blocks;
+        e->light_blocks_per_cu = prev->light_blocks_per_cu;
+        e->camera_blocks_per_cu = prev->camera_blocks_per_cu;
+        e->gather_blocks_per_cu = prev->gather_blocks_per_cu;
+        e->path_chunk = prev->path_chunk;
         e->exact_merge = prev->exact_merge;
         e->output_path = prev->output_path;
         const GuideModel& pg = prev->guide;  // guide options, not its state
@@ -1536,6 +1701,10 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "photon_capacity") g->photon_capacity = v;
     else if (k == "cell_scale") g->cell_scale = v;
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
+    else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
+    else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
+    else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
+    else if (k == "path_chunk") g->path_chunk = (uint32_t)std::max(1, std::min(1024, (int)v / 64)) * 64u;
     else if (k == "count_events") g->count_events = (int)v;
     else if (k == "exact_merge") g->exact_merge = (int)v;
     else if (k == "next_iteration") g->iterations_done = std::max(0, (int)v);
@@ -1576,6 +1745,11 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "nodes") return g->n_nodes;
     if (k == "device_cus") return g->device_cus;
     if (k == "capacity") return g->capacity;
+    if (k == "light_blocks_per_cu") return g->light_blocks_per_cu;
+    if (k == "camera_blocks_per_cu") return g->camera_blocks_per_cu;
+    if (k == "gather_blocks_per_cu") return g->gather_blocks_per_cu;
+    if (k == "path_chunk") return g->path_chunk;
+    if (k == "cell_scale") return g->cell_scale;
     if (k == "guide") return g->guide.enabled;
     if (k == "guide_step") return g->guide.step;
     if (k == "guide_slots") return g->guide.slots;
@@ -1810,6 +1984,18 @@ __attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration,
     return ok ? 0 : -1;
 }
 
+__attribute__((visibility("default"))) int rc_autotune(int repeats, double* report, int n) {
+    g_error.clear();
+    if (!g) {
+        set_error("rc_autotune: no scene");
+        return -1;
+    }
+    double r[RC_TUNE_REPORT] = {0};
+    if (!autotune(*g, repeats, r)) return -1;
+    if (report) std::copy(r, r + std::max(0, std::min(n, (int)RC_TUNE_REPORT)), report);
+    return 0;
+}
+
 // ---- batches ----
 __attribute__((visibility("default"))) int rc_batch_add(void) {
     g_error.clear();
@@ -1871,6 +2057,18 @@ __attribute__((visibility("default"))) int rc_batch_get_image(int scene, float* 
         return -1;
     }
     return get_image(*g_batch->scenes[scene], rgb, scaled != 0) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_autotune(int repeats, double* report, int n) {
+    g_error.clear();
+    if (!g_batch) {
+        set_error("no batch");
+        return -1;
+    }
+    double r[RC_TUNE_REPORT] = {0};
+    if (!batch_autotune(*g_batch, repeats, r)) return -1;
+    if (report) std::copy(r, r + std::max(0, std::min(n, (int)RC_TUNE_REPORT)), report);
+    return 0;
 }
 
 __attribute__((visibility("default"))) int rc_batch_stats(double* out, int n) {

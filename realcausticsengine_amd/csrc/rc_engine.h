@@ -112,6 +112,7 @@ struct LightPassArgs {
     SceneView sc;
     uint32_t k0, k1, iteration, stream;
     uint32_t n_paths;
+    uint32_t path_chunk;  // path indices a wave takes per refill (>= 64)
     uint32_t* path_counter;
     uint32_t* photon_counter;
     uint32_t capacity;
@@ -144,6 +145,7 @@ struct CameraPassArgs {
     PhotonMapView pm;
     uint32_t k0, k1, iteration, stream;
     uint32_t n_paths;
+    uint32_t path_chunk;      // path indices a wave takes per refill (>= 64)
     uint32_t* path_counter;   // [0] paths, [1] queries, [2] gather work counter
     float vm_norm;
     QueryBuffers q;

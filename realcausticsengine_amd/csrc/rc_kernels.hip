@@ -35,11 +35,12 @@ __device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want, uin
 }
 
 // Per-wave pool of path indices taken from the global counter in chunks of
-// kPathChunk, so the refill of finished lanes costs one atomic per chunk.
-// Indices are handed out in increasing order; a wave only stops when its
-// lanes draw indices past the end, so no index below the end is skipped.
-constexpr uint32_t kPathChunk = 256;
+// `chunk` (>= 64, the autotuned queue granularity), so the refill of finished
+// lanes costs one atomic per chunk.  Indices are handed out in increasing
+// order; a wave only stops when its lanes draw indices past the end, so no
+// index below the end is skipped.
 struct WavePool {
+    uint32_t chunk;
     uint32_t next = 0, end = 0;  // wave-uniform
     __device__ __forceinline__ uint32_t draw(uint32_t* counter, bool want) {
         uint64_t m = __ballot(want);
@@ -53,11 +54,11 @@ struct WavePool {
             next += n;
         } else {
             uint32_t base = 0;
-            if (lane_id() == 0) base = atomicAdd(counter, kPathChunk);
+            if (lane_id() == 0) base = atomicAdd(counter, chunk);
             base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
             slot = r < rem ? next + r : base + (r - rem);
             next = base + (n - rem);
-            end = base + kPathChunk;
+            end = base + chunk;
         }
         return slot;
     }
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     LightState st;
     bool active = false, exhausted = false;
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
-    WavePool pool;
+    WavePool pool{a.path_chunk};
     for (;;) {
         // refill finished lanes
         bool need = !active && !exhausted;
@@ -276,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     unsigned long long cnt[kCounterFields] = {0};
     CameraState st;
     bool active = false, exhausted = false;
-    WavePool pool;
+    WavePool pool{a.path_chunk};
     for (;;) {
         bool need = !active && !exhausted;
         uint64_t m;

@@ -24,8 +24,11 @@ EXPORTS = [
     "rc_trace_photons", "rc_gather_sets", "rc_camera_pass_photons",
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
-    "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release",
+    "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
+    "rc_batch_autotune",
 ]
+TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
+             "gather_blocks_per_cu"]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
@@ -108,6 +111,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_batch_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
     lib.rc_batch_stats.restype = ctypes.c_int
     lib.rc_batch_release.restype = None
+    for fn in (lib.rc_autotune, lib.rc_batch_autotune):
+        fn.argtypes = [ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
+        fn.restype = ctypes.c_int
     _lib = lib
     return lib
 
@@ -215,6 +221,12 @@ class Engine:
         self._check(self.lib.rc_camera_pass_photons(iteration, _fp(ph), len(ph), _fp(img)), "rc_camera_pass_photons")
         return img.reshape(H, W, 3)
 
+    def autotune(self, repeats: int = 2) -> dict:
+        """Chooses launch shapes and queue / tile sizes for this scene (rc_autotune)."""
+        buf = (ctypes.c_double * len(TUNE_KEYS))()
+        self._check(self.lib.rc_autotune(repeats, buf, len(TUNE_KEYS)), "rc_autotune")
+        return dict(zip(TUNE_KEYS, list(buf)))
+
     # -- learned emission proposal -----------------------------------------
     def guide_params(self) -> np.ndarray:
         out = np.zeros(GUIDE_PARAMS, dtype=np.float32)
@@ -305,6 +317,11 @@ class Batch:
         buf = (ctypes.c_double * len(STAT_NAMES))()
         n = self.lib.rc_batch_stats(buf, len(STAT_NAMES))
         return {STAT_NAMES[i]: buf[i] for i in range(n)}
+
+    def autotune(self, repeats: int = 2) -> dict:
+        buf = (ctypes.c_double * len(TUNE_KEYS))()
+        self._check(self.lib.rc_batch_autotune(repeats, buf, len(TUNE_KEYS)), "rc_batch_autotune")
+        return dict(zip(TUNE_KEYS, list(buf)))
 
     def release(self):
         self.lib.rc_batch_release()
