@@ -141,6 +141,33 @@ def uneven_batch(a):
     return out
 
 
+def graph_leg(a):
+    """Iteration time of a launch-bound scene (64 x 64 pixels, 65,536 light
+    paths, 5,120-triangle glass sphere) with direct launches and with the
+    captured iteration graph."""
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import Engine
+
+    out = {"workload": "glass icosphere (5,120 tris) over a plane, 64x64, 65,536 light paths/iteration"}
+    for graph in (0, 1):
+        eng = Engine(scenes.c0_scene(64, 64, subdiv=4, samples=1), photons=65536, seed=0, write_exr=0, graph=graph)
+        eng.prepare()
+        for _ in range(5):
+            eng.render_iteration()
+        n = 200
+        t0 = time.perf_counter()
+        for _ in range(n):
+            eng.render_iteration()
+        out["ms_per_iteration_graph" if graph else "ms_per_iteration_direct"] = round(
+            (time.perf_counter() - t0) * 1e3 / n, 4)
+        if graph:
+            out["graph_captures"] = int(eng.get("graph_captures"))
+        eng.release()
+    out["speedup"] = round(out["ms_per_iteration_direct"] / out["ms_per_iteration_graph"], 3)
+    print(f"[bench] graph: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def time_to_rmse(a, cpu_s_per_iteration):
     """Wall time of rc_render_iteration calls until the running image reaches
     RMSE <= target against the FP64 reference (readback and scoring untimed)."""
@@ -303,6 +330,7 @@ def main() -> int:
     batch = None
     if not a.no_batch:
         batch = uneven_batch(a)
+    graph = graph_leg(a)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None)
@@ -327,6 +355,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
         "uneven_batch": batch,
+        "iteration_graph": graph,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)

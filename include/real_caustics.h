@@ -131,6 +131,12 @@ enum {
  * iteration before and after, then path_chunk, light_blocks_per_cu,
  * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu. */
 enum { RC_TUNE_REPORT = 7 };
+/* Option "graph" = 1: after the light pass (a direct launch whose photon
+ * count is checked on the host), the photon-map build and the camera pass of
+ * each iteration replay one captured hipGraph, reading counts and
+ * per-iteration values from device memory; the graph is captured again when
+ * buffers grow or the light-path count changes (rc_get_option("graph_captures")).
+ * The film is identical to the one direct launches give. */
 int rc_autotune(int repeats, double *report, int n);
 
 /* ---- diagnostics used by the parity tests ---- */

@@ -114,6 +114,13 @@ struct Engine {
     int device_cus = 0;
     uint32_t last_queries = 0;
     GuideModel guide;  // learned emission proposal (option "guide")
+    // captured iteration graph (option "graph"): photon map + camera pass
+    int graph = 0;
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;  // buffers and sizes the graph was captured with
+    DevBuf<IterParams> d_iter;
+    IterParams* h_iter = nullptr;      // pinned staging of the per-iteration values
+    int graph_captures = 0;
     hipEvent_t ev_guide[4] = {};  // guide work begin / end, network step begin / end
 };
 
@@ -857,15 +864,141 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The iteration with its photon map and camera pass replayed from a captured
+// hipGraph.  The light pass stays a direct launch: its photon count is read
+// back and checked against the capacity before the replay (a grown buffer
+// means a new capture).  Inside the graph every count comes from device
+// memory and the per-iteration values (iteration, radius, grid, VM
+// normalisation) from d_iter, copied in before each replay; the query buffer
+// holds the most queries a camera pass can make (8 per path).
+bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
+    hipStream_t st = e.stream;
+    float radius, r2, norm;
+    iteration_params(e, iteration, radius, r2, norm);
+    if (!run_light_pass(e, iteration, count)) return false;
+    long long N = light_paths(e);
+    const uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
+    if (!ensure_query_buffers(e, (uint32_t)std::min<uint64_t>(8ull * n_paths + 1024, 1ull << 31), n_paths)) return false;
+    // the grid of this iteration, as build_map
+    int uid_bits = log2_ceil((uint64_t)N) + 3;
+    uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
+    int bucket_bits = log2_ceil(want);
+    uint32_t buckets = 1u << bucket_bits;
+    if (!e.cells.alloc(buckets)) return false;
+    float maxc = 0.0f;
+    for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
+    float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
+    float cell = rq * (float)std::max(0.67, e.cell_scale);
+    if (!e.d_iter.alloc(1)) return false;
+    if (!e.h_iter) HIP_OK(hipHostMalloc((void**)&e.h_iter, sizeof(IterParams), hipHostMallocDefault));
+    IterParams& ip = *e.h_iter;
+    ip.iteration = (uint32_t)iteration;
+    ip.r2 = r2;
+    ip.r_query = rq;
+    ip.inv_cell = 1.0f / cell;
+    for (int k = 0; k < 3; ++k) ip.origin[k] = e.bbox.lo[k] - cell;
+    ip.vm_norm = norm;
+    HIP_OK(hipMemcpyAsync(e.d_iter.p, e.h_iter, sizeof(IterParams), hipMemcpyHostToDevice, st));
+    PhotonMapView& pm = mi.pm;
+    pm.pos = e.s_pos.p;
+    pm.dir_tr = e.s_dir.p;
+    pm.tp_gb = e.s_tp.p;
+    pm.cells = e.cells.p;
+    pm.mask = buckets - 1;
+    for (int k = 0; k < 3; ++k) pm.origin[k] = ip.origin[k];
+    pm.inv_cell = ip.inv_cell;
+    pm.r2 = r2;
+    pm.r_query = rq;
+    std::vector<uintptr_t> key = {(uintptr_t)e.ph_pos.p, e.capacity, (uintptr_t)e.q_pos.p, e.q_capacity,
+                                  (uintptr_t)e.cells.p, buckets, (uintptr_t)uid_bits, (uintptr_t)e.image.p,
+                                  (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
+                                  (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
+                                  (uintptr_t)e.exact_merge};
+    if (!e.graph_exec || key != e.graph_key) {
+        if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
+        e.graph_exec = nullptr;
+        MapBuildArgs m{};
+        m.in_pos = e.ph_pos.p;
+        m.in_dir = e.ph_dir.p;
+        m.in_tp = e.ph_tp.p;
+        m.in_uid = e.ph_uid.p;
+        m.count = e.capacity;
+        m.count_dev = e.counters.p + 1;
+        m.dyn = e.d_iter.p;
+        m.mask = buckets - 1;
+        m.bucket_bits = bucket_bits;
+        m.uid_bits = uid_bits;
+        m.keys_a = e.keys_a.p;
+        m.keys_b = e.keys_b.p;
+        m.vals_a = e.vals_a.p;
+        m.vals_b = e.vals_b.p;
+        m.hist = e.hist.p;
+        m.hist_sums = e.hist_sums.p;
+        m.out_pos = e.s_pos.p;
+        m.out_dir = e.s_dir.p;
+        m.out_tp = e.s_tp.p;
+        m.out_index = e.s_index.p;
+        m.cells = e.cells.p;
+        CameraPassArgs a{};
+        a.sc = scene_view(e);
+        a.pm = pm;
+        a.k0 = (uint32_t)e.seed;
+        a.k1 = (uint32_t)(e.seed >> 32);
+        a.stream = STREAM_CAMERA;
+        a.n_paths = n_paths;
+        a.path_chunk = e.path_chunk;
+        a.path_counter = e.counters.p;
+        a.path_info = e.path_info.p;
+        a.contrib = e.contrib.p;
+        a.image = e.image.p;
+        a.exact_merge = e.exact_merge;
+        a.dyn = e.d_iter.p;
+        a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
+        hipGraph_t graph = nullptr;
+        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        build_photon_map(m, st);
+        launch_fill_u32(e.counters.p, 0u, 4, st);
+        launch_camera_trace(a, camera_blocks(e), st);
+        launch_gather(a, gather_blocks(e), st);
+        launch_resolve(a, st);
+        hipError_t cap = hipStreamEndCapture(st, &graph);
+        if (cap != hipSuccess || !graph) {
+            set_error(std::string("hipGraph capture failed: ") + hipGetErrorString(cap));
+            return false;
+        }
+        hipError_t inst = hipGraphInstantiate(&e.graph_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (inst != hipSuccess) {
+            e.graph_exec = nullptr;
+            set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(inst));
+            return false;
+        }
+        e.graph_key = key;
+        e.graph_captures++;
+    }
+    HIP_OK(hipEventRecord(e.ev[2], st));
+    HIP_OK(hipGraphLaunch(e.graph_exec, st));
+    HIP_OK(hipEventRecord(e.ev[5], st));
+    HIP_OK(hipMemcpyAsync(&e.last_queries, e.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    e.stats[RC_STAT_GATHERS] = e.last_queries;
+    return true;
+}
+
 bool render_iteration(Engine& e, int iteration) {
     float radius, r2, norm;
     iteration_params(e, iteration, radius, r2, norm);
     auto t0 = std::chrono::steady_clock::now();
     uint32_t count = 0;
-    if (!run_light_pass(e, iteration, count)) return false;
     MapInfo mi;
-    if (!build_map(e, count, radius, r2, mi)) return false;
-    if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
+    const bool graphed = e.graph && !e.count_events;
+    if (graphed) {
+        if (!graph_iteration(e, iteration, count, mi)) return false;
+    } else {
+        if (!run_light_pass(e, iteration, count)) return false;
+        if (!build_map(e, count, radius, r2, mi)) return false;
+        if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
+    }
     const bool training = guide_trains(e, iteration);
     e.stats[RC_STAT_MS_GUIDE] = e.stats[RC_STAT_MS_GUIDE_STEP] = e.stats[RC_STAT_GUIDE_ROWS] = 0;
     if (training) {
@@ -878,11 +1011,16 @@ bool render_iteration(Engine& e, int iteration) {
     if (e.count_events) HIP_OK(hipMemcpyAsync(c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
     HIP_OK(hipStreamSynchronize(e.stream));
     float ms[3] = {0, 0, 0};
-    for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
     float ms_trace = 0, ms_gather = 0, ms_resolve = 0;
-    HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[6]));
-    HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
-    HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
+    if (graphed) {  // map + camera pass are one replay: its time goes to ms_camera
+        HIP_OK(hipEventElapsedTime(&ms[0], e.ev[0], e.ev[1]));
+        HIP_OK(hipEventElapsedTime(&ms[2], e.ev[2], e.ev[5]));
+    } else {
+        for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
+        HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[6]));
+        HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
+        HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
+    }
     e.stats[RC_STAT_MS_CAMERA_TRACE] = ms_trace;
     e.stats[RC_STAT_MS_GATHER] = ms_gather;
     e.stats[RC_STAT_MS_RESOLVE] = ms_resolve;
@@ -1144,6 +1282,9 @@ void release(Engine*& e) {
     for (auto& ev : e->ev_guide)
         if (ev) (void)hipEventDestroy(ev);
     e->guide.release();
+    if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    e->d_iter.release();
+    if (e->h_iter) (void)hipHostFree(e->h_iter);
     if (e->stream) (void)hipStreamDestroy(e->stream);
     delete e;
     e = nullptr;
@@ -1497,6 +1638,7 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         e->photon_capacity = prev->photon_capacity;
         e->cell_scale = prev->cell_scale;
         e->persistent_blocks = prev->persistent_blocks;
+        e->graph = prev->graph;
         e->light_blocks_per_cu = prev->light_blocks_per_cu;
         e->camera_blocks_per_cu = prev->camera_blocks_per_cu;
         e->gather_blocks_per_cu = prev->gather_blocks_per_cu;
@@ -1701,6 +1843,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "photon_capacity") g->photon_capacity = v;
     else if (k == "cell_scale") g->cell_scale = v;
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
+    else if (k == "graph") g->graph = (int)v;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -1745,6 +1888,8 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "nodes") return g->n_nodes;
     if (k == "device_cus") return g->device_cus;
     if (k == "capacity") return g->capacity;
+    if (k == "graph") return g->graph;
+    if (k == "graph_captures") return g->graph_captures;
     if (k == "light_blocks_per_cu") return g->light_blocks_per_cu;
     if (k == "camera_blocks_per_cu") return g->camera_blocks_per_cu;
     if (k == "gather_blocks_per_cu") return g->gather_blocks_per_cu;

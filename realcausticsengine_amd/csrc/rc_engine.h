@@ -125,6 +125,15 @@ struct LightPassArgs {
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1 per scene; no guide)
 };
 
+// Per-iteration values read from device memory by the kernels of a captured
+// iteration graph (a graph replays fixed kernel arguments).
+struct IterParams {
+    uint32_t iteration;
+    float r2, r_query, inv_cell;
+    float origin[3];
+    float vm_norm;
+};
+
 // One merge query per non-delta camera vertex (80 B, SoA):
 //   pos  {position, camera dVM}          dir {ray direction, material}
 //   sn   {shading normal, slot}          gn  {geometric normal, 0}
@@ -155,6 +164,7 @@ struct CameraPassArgs {
     Counters* counters;  // non-null selects the counting variant
     int exact_merge;     // 1: correctly rounded division/sqrt in the merge
     BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
+    const IterParams* dyn;  // non-null: iteration, vm_norm and the grid of pm come from here
 };
 
 // ---- launchers (rc_kernels.hip) ----
@@ -170,7 +180,9 @@ struct MapBuildArgs {
     const float4* in_dir;
     const float2* in_tp;
     const uint32_t* in_uid;
-    uint32_t count;
+    uint32_t count;           // photons (with count_dev: the capacity the grids are sized for)
+    const uint32_t* count_dev;  // non-null: the photon count is read here (captured graphs)
+    const IterParams* dyn;      // non-null: origin and inv_cell come from here
     float origin[3];
     float inv_cell;
     uint32_t mask;

@@ -273,6 +273,10 @@ template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     __shared__ int stack_lds[kLdsStack * kBlock];
     int* my_stack = stack_lds + threadIdx.x;
+    if (a.dyn) {
+        a.iteration = a.dyn->iteration;
+        a.vm_norm = a.dyn->vm_norm;
+    }
     PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
     unsigned long long cnt[kCounterFields] = {0};
     CameraState st;
@@ -431,6 +435,15 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     uint32_t* queue = queue_lds[wv];
     const uint32_t lane = lane_id();
     unsigned long long cnt[kCounterFields] = {0};
+    PhotonMapView pm0 = a.pm;
+    if (a.dyn) {
+        pm0.r2 = a.dyn->r2;
+        pm0.r_query = a.dyn->r_query;
+        pm0.inv_cell = a.dyn->inv_cell;
+        pm0.origin[0] = a.dyn->origin[0];
+        pm0.origin[1] = a.dyn->origin[1];
+        pm0.origin[2] = a.dyn->origin[2];
+    }
     const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
     const uint32_t xcd = blockIdx.x % kXcds;
     const uint32_t seg = (nq + kXcds - 1) / kXcds;
@@ -440,7 +453,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
         float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
         // the query's scene is the same in every lane
         const int scene = kBatch ? __builtin_amdgcn_readfirstlane(__float_as_int(qg.w)) : 0;
-        const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : a.pm;
+        const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : pm0;
         const DevMaterial* mats = kBatch ? a.batch.sc[scene].mats : a.sc.mats;
         F3 q = f3(qp.x, qp.y, qp.z);
         float cdvm = qp.w;
@@ -742,10 +755,21 @@ size_t photon_map_sum_entries(uint32_t max_count) {
     return (h + kScanTile - 1) / kScanTile + 1;
 }
 
-__global__ void k_map_keys(const float4* pos, const uint32_t* uid, uint32_t n, float ox, float oy, float oz,
-                           float inv_cell, uint32_t mask, int uid_bits, unsigned long long* keys, uint32_t* vals) {
+// Element count of the map kernels: n, or the count in device memory (at most n) when n_dev is given.
+__device__ __forceinline__ uint32_t live_count(const uint32_t* n_dev, uint32_t n) { return n_dev ? min(*n_dev, n) : n; }
+
+__global__ void k_map_keys(const float4* pos, const uint32_t* uid, const uint32_t* n_dev, uint32_t n,
+                           const IterParams* dyn, float ox, float oy, float oz, float inv_cell, uint32_t mask,
+                           int uid_bits, unsigned long long* keys, uint32_t* vals) {
+    n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    if (dyn) {
+        ox = dyn->origin[0];
+        oy = dyn->origin[1];
+        oz = dyn->origin[2];
+        inv_cell = dyn->inv_cell;
+    }
     float4 p = pos[i];
     int cx = (int)floorf((p.x - ox) * inv_cell);
     int cy = (int)floorf((p.y - oy) * inv_cell);
@@ -755,9 +779,10 @@ __global__ void k_map_keys(const float4* pos, const uint32_t* uid, uint32_t n, f
     vals[i] = i;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned long long* keys, uint32_t n, int shift,
-                                                            uint32_t* hist, uint32_t nblocks) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned long long* keys, const uint32_t* n_dev,
+                                                            uint32_t n, int shift, uint32_t* hist, uint32_t nblocks) {
     __shared__ uint32_t h[256];
+    n = live_count(n_dev, n);
     h[threadIdx.x] = 0;
     __syncthreads();
     uint32_t base = blockIdx.x * kSortTile;
@@ -770,10 +795,11 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned long 
 }
 
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const unsigned long long* kin, const uint32_t* vin,
-                                                               uint32_t n, int shift, const uint32_t* offsets,
-                                                               uint32_t nblocks, unsigned long long* kout,
-                                                               uint32_t* vout) {
+                                                               const uint32_t* n_dev, uint32_t n, int shift,
+                                                               const uint32_t* offsets, uint32_t nblocks,
+                                                               unsigned long long* kout, uint32_t* vout) {
     __shared__ uint32_t running[256];
+    n = live_count(n_dev, n);
     __shared__ uint32_t wcount[kSortThreads / 64][256];
     const uint32_t wave = threadIdx.x >> 6;
     running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
@@ -810,8 +836,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const unsigned lo
     }
 }
 
-__global__ void k_map_permute(const float4* pos, const float4* dir, const float2* tp, const uint32_t* vals, uint32_t n,
-                              float4* opos, float4* odir, float2* otp, uint32_t* oidx) {
+__global__ void k_map_permute(const float4* pos, const float4* dir, const float2* tp, const uint32_t* vals,
+                              const uint32_t* n_dev, uint32_t n, float4* opos, float4* odir, float2* otp,
+                              uint32_t* oidx) {
+    n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t j = vals[i];
@@ -821,7 +849,9 @@ __global__ void k_map_permute(const float4* pos, const float4* dir, const float2
     oidx[i] = j;
 }
 
-__global__ void k_map_cells(const unsigned long long* keys, uint32_t n, int uid_bits, uint2* cells) {
+__global__ void k_map_cells(const unsigned long long* keys, const uint32_t* n_dev, uint32_t n, int uid_bits,
+                            uint2* cells) {
+    n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t b = (uint32_t)(keys[i] >> uid_bits);
@@ -852,8 +882,8 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     uint32_t n = a.count;
     if (n == 0) return;
     unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, n, a.origin[0], a.origin[1],
-                       a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a);
+    hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, a.count_dev, n, a.dyn, a.origin[0],
+                       a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a);
     sort_permute_cells(a, s);
 }
 
@@ -880,10 +910,11 @@ static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
     uint32_t* vin = a.vals_a;
     uint32_t* vout = a.vals_b;
     for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, n, shift, a.hist, nblocks);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, a.count_dev, n, shift, a.hist,
+                           nblocks);
         launch_exclusive_scan_u32(a.hist, a.hist, 256 * nblocks, a.hist_sums, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, n, shift, a.hist,
-                           nblocks, kout, vout);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, a.count_dev, n, shift,
+                           a.hist, nblocks, kout, vout);
         unsigned long long* tk = kin;
         kin = kout;
         kout = tk;
@@ -891,9 +922,9 @@ static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
         vin = vout;
         vout = tv;
     }
-    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_pos, a.in_dir, a.in_tp, vin, n, a.out_pos,
-                       a.out_dir, a.out_tp, a.out_index);
-    hipLaunchKernelGGL(k_map_cells, dim3(g), dim3(256), 0, s, kin, n, a.uid_bits, a.cells);
+    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_pos, a.in_dir, a.in_tp, vin, a.count_dev, n,
+                       a.out_pos, a.out_dir, a.out_tp, a.out_index);
+    hipLaunchKernelGGL(k_map_cells, dim3(g), dim3(256), 0, s, kin, a.count_dev, n, a.uid_bits, a.cells);
 }
 
 // ---------------------------------------------------------------------
