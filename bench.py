@@ -261,6 +261,9 @@ def main() -> int:
     ap.add_argument("--guide-train-every", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-scene batch leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--launch", default="", help="fixed launch shapes instead of autotuning, "
+                    "e.g. path_chunk=512,light_blocks_per_cu=8,gather_blocks_per_cu=32")
+    ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     a = ap.parse_args()
@@ -280,7 +283,12 @@ def main() -> int:
     print(f"[bench] scene ready: {args.total_tris} triangles, BVH {eng.get('nodes'):.0f} nodes, {prep_s:.2f} s",
           file=sys.stderr, flush=True)
     tuned = None
-    if not a.no_autotune:  # launch shapes and queue / tile sizes for this scene (untimed; renders nothing)
+    if a.launch:
+        for kv in a.launch.split(","):
+            k, v = kv.split("=")
+            eng.set(k.strip(), float(v))
+        tuned = {"fixed": a.launch}
+    elif not a.no_autotune:  # launch shapes and queue / tile sizes for this scene (untimed; renders nothing)
         tuned = {k: round(v, 3) for k, v in eng.autotune(repeats=2).items()}
         print(f"[bench] autotune: {tuned}", file=sys.stderr, flush=True)
     for _ in range(a.warmup):
@@ -330,7 +338,7 @@ def main() -> int:
     batch = None
     if not a.no_batch:
         batch = uneven_batch(a)
-    graph = graph_leg(a)
+    graph = None if a.no_graph else graph_leg(a)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None)

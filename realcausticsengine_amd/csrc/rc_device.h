@@ -385,7 +385,7 @@ __device__ __forceinline__ float guide_direction(const GuideView& g, int light, 
     int cx = (int)(ax * (float)G), cy = (int)(ay * (float)G);
     cx = cx > G - 1 ? G - 1 : cx;
     cy = cy > G - 1 ? G - 1 : cy;
-    const float* cdf = g.cdf + ((size_t)(g.slot[light] * G + cy) * G + cx) * kGuideCdfStride;
+    const float* cdf = g.cdf + ((size_t)(guide_slot(g, light) * G + cy) * G + cx) * kGuideCdfStride;
     if (u24(w1.b) >= g.alpha) {
         float uc = (float)((w1.c >> 8) + 1u) * 5.9604644775390625e-08f;  // (0, 1]
         int k = find_interval(cdf, kGuideBins + 1, uc);
@@ -399,56 +399,49 @@ __device__ __forceinline__ float guide_direction(const GuideView& g, int light, 
 
 // Returns false when the light emits nothing (the reference point light).
 // bin / axis: the direction bin and emission axis of a guidable emission
-// (area and sphere lights), bin = -1 otherwise.
-__device__ bool sample_light(const SceneView& sc, const PathRng& rng, const GuideView& gv, F3& o, F3& d, F3& tp,
-                             int& bin, F3& axis) {
+// (area and sphere lights), bin = -1 otherwise.  The direction sample (block
+// 1 draw, optional guided warp, cosine lobe) is one piece of code for both
+// guidable kinds.
+__device__ __forceinline__ bool sample_light(const SceneView& sc, const PathRng& rng, const GuideView& gv, F3& o,
+                                             F3& d, F3& tp, int& bin, F3& axis) {
     U4 w0 = rng.block(0);
-    float u[5] = {u24(w0.a), u24(w0.b), u24(w0.c), u24(w0.d), 0.0f};
-    float rnd = u[0];
-    int idx = find_interval(sc.cdf, sc.nlights + 1, rnd);
+    int idx = find_interval(sc.cdf, sc.nlights + 1, u24(w0.a));
     const DevLight& L = sc.lights[idx];
     float pick = sc.compat_pdf ? L.power / ((float)sc.nlights * sc.integral) : L.power / sc.integral;
-    const bool guided = idx < kGuideMaxLights && gv.slot[idx] >= 0;
-    float q = 1.0f;
     bin = -1;
-    F3 t;
-    if (L.kind == L_AREA) {
-        float ax = u[1], ay = u[2];
-        o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
-        U4 w1 = rng.block(1);
-        uint32_t wx = w0.d, wy = w1.a;
-        if (guided) q = guide_direction(gv, idx, ax, ay, w1, wx, wy, bin);
-        F3 local = cos_hemisphere(u24(wx), u24(wy));
-        Frame fr{ld3(L.fx), ld3(L.fy), ld3(L.fz)};
-        d = to_world(fr, local);
-        axis = ld3(L.fz);
-        t = ld3(L.color) * kPI;
-    } else if (L.kind == L_SUN) {
-        float ax = u[1], ay = u[2];
-        o = (ld3(L.bl) + ld3(L.h) * ax) + ld3(L.v) * ay;
+    axis = f3s(0.0f);  // set on every path: a partly-set output keeps the compiler from promoting it to registers
+    const float u1 = u24(w0.b), u2 = u24(w0.c);
+    if (L.kind == L_SUN) {
+        o = (ld3(L.bl) + ld3(L.h) * u1) + ld3(L.v) * u2;
         d = ld3(L.normal);
-        t = ld3(L.color);
+        tp = div_each(ld3(L.color), pick);
+        return true;
+    }
+    const bool area = L.kind == L_AREA;
+    if (!area && sc.compat_point) return false;
+    F3 n = f3s(0.0f);
+    if (area) {
+        o = (ld3(L.bl) + ld3(L.h) * u1) + ld3(L.v) * u2;
     } else {
-        if (sc.compat_point) return false;
-        float u1 = u[1], u2 = u[2];
         float z = 1.0f - 2.0f * u1;
         float rr = sqrtf(smax(0.0f, 1.0f - z * z));
         float phi = kPI2 * u2;
-        F3 n = f3(rr * own_cos(phi), rr * own_sin(phi), z);
+        n = f3(rr * own_cos(phi), rr * own_sin(phi), z);
+        o = L.radius > 0.0f ? ld3(L.pos) + n * L.radius : ld3(L.pos);
+    }
+    float q = 1.0f;
+    if (area || L.radius > 0.0f) {
         U4 w1 = rng.block(1);
         uint32_t wx = w0.d, wy = w1.a;
-        if (L.radius > 0.0f) {
-            if (guided) q = guide_direction(gv, idx, u1, u2, w1, wx, wy, bin);
-            o = ld3(L.pos) + n * L.radius;
-            Frame fr = frame_from_z(n);
-            d = to_world(fr, cos_hemisphere(u24(wx), u24(wy)));
-            axis = n;
-        } else {
-            o = ld3(L.pos);
-            d = n;
-        }
-        t = (ld3(L.color) * L.power) * kPI;
+        if (guide_slot(gv, idx) >= 0) q = guide_direction(gv, idx, u1, u2, w1, wx, wy, bin);
+        F3 local = cos_hemisphere(u24(wx), u24(wy));
+        Frame fr = area ? Frame{ld3(L.fx), ld3(L.fy), ld3(L.fz)} : frame_from_z(n);
+        d = to_world(fr, local);
+        axis = area ? ld3(L.fz) : n;
+    } else {
+        d = n;
     }
+    F3 t = area ? ld3(L.color) * kPI : (ld3(L.color) * L.power) * kPI;
     tp = div_each(t, pick);
     if (q != 1.0f) tp = div_each(tp, q);
     return true;

@@ -50,7 +50,7 @@ constexpr int kGuideParams = kOffB4 + kGuideBins;       // 87,232
 struct GuideView {
     const float* cdf;        // [slot][cell][kGuideCdfStride], cdf[0] = 0, cdf[1024] = 1
     int cells;               // G
-    int slot[kGuideMaxLights];  // table slot of light i, or -1
+    uint32_t slot_mask;      // bit i: light i has a table; its slot = set bits below bit i
     float alpha;
     // training records of paths < train_paths
     uint32_t train_paths;
@@ -58,6 +58,12 @@ struct GuideView {
     int* train_bin;          // bin index, or -1 when the path has no guidable emission
     float center[3], inv_half[3];
 };
+
+// Table slot of light i (lights with a table take slots in light order), or -1.
+__host__ __device__ __forceinline__ int guide_slot(const GuideView& g, int i) {
+    if (i < 0 || i >= kGuideMaxLights || !((g.slot_mask >> i) & 1u)) return -1;
+    return __builtin_popcount(g.slot_mask & ((1u << i) - 1u));
+}
 
 // The input record of an emission point: scene-box coordinates and axis.
 __host__ __device__ __forceinline__ void guide_record(const GuideView& g, float px, float py, float pz, float ax,

@@ -64,7 +64,7 @@ bool gemm(const uint16_t* A, int lda, bool at, const uint16_t* B, int ldb, bool 
 
 GuideView plain_guide_view() {
     GuideView v{};
-    for (int i = 0; i < kGuideMaxLights; ++i) v.slot[i] = -1;
+    v.slot_mask = 0u;
     v.cells = 1;
     return v;
 }
@@ -258,7 +258,8 @@ bool GuideModel::build_table(hipStream_t s) {
 
 GuideView GuideModel::view(bool records, bool sample) const {
     GuideView gv = plain_guide_view();
-    for (int i = 0; i < kGuideMaxLights; ++i) gv.slot[i] = slot_of_light[i];
+    for (int i = 0; i < kGuideMaxLights; ++i)
+        if (slot_of_light[i] >= 0) gv.slot_mask |= 1u << i;
     gv.cells = cells_built > 0 ? cells_built : 1;
     // alpha on the 2^-24 grid of the uniform draws, so P(u < alpha) = alpha exactly
     gv.alpha = (float)(std::ceil((double)alpha * 16777216.0) / 16777216.0);
