@@ -141,6 +141,34 @@ def uneven_batch(a):
     return out
 
 
+def guide_steps(a) -> int:
+    """Only the learned proposal's training step: `--guide-steps` steps of 65,536
+    synthetic rows (forward, backward, Adam) through rc_guide_train_step, timed
+    with the engine's network-step events; prints one JSON line."""
+    import numpy as np
+
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import GUIDE_STEP_FLOPS_PER_ROW, Engine
+
+    eng = Engine(scenes.water_scene(32, 16, 16, samples=1), photons=65536, seed=0, write_exr=0)
+    eng.prepare()
+    rng = np.random.default_rng(0)
+    n = 65536
+    x = rng.uniform(-1, 1, (n, 6)).astype(np.float32)
+    bins = rng.integers(0, 1024, n).astype(np.int32)
+    w = rng.exponential(1.0, n).astype(np.float32)
+    eng.guide_train_step(x, bins, w, apply=True)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(a.guide_steps):
+        eng.guide_train_step(x, bins, w, apply=True)
+    wall = (time.perf_counter() - t0) / a.guide_steps
+    eng.release()
+    flops = GUIDE_STEP_FLOPS_PER_ROW * n
+    print(json.dumps({"guide_training_step": {"rows": n, "steps": a.guide_steps, "gflop_per_step": flops / 1e9,
+                                              "ms_per_step_wall_incl_upload": round(wall * 1e3, 3)}}), flush=True)
+    return 0
+
+
 def graph_leg(a):
     """Iteration time of a launch-bound scene (64 x 64 pixels, 65,536 light
     paths, 5,120-triangle glass sphere) with direct launches and with the
@@ -264,6 +292,8 @@ def main() -> int:
     ap.add_argument("--launch", default="", help="fixed launch shapes instead of autotuning, "
                     "e.g. path_chunk=512,light_blocks_per_cu=8,gather_blocks_per_cu=32")
     ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")
+    ap.add_argument("--guide-steps", type=int, default=0,
+                    help="only time this many training steps of the learned proposal (65,536 rows each)")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     a = ap.parse_args()
@@ -271,6 +301,8 @@ def main() -> int:
         print("bench.py measures one MI355X: the path has no multi-GPU mode in this build "
               "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
         return 2
+    if a.guide_steps > 0:
+        return guide_steps(a)
 
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Engine
