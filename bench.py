@@ -31,10 +31,11 @@ HBM_PEAK_GBS = 8000.0
 PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Vertices.h:17-27)
 
 
-def algorithmic_bytes(st: dict) -> dict:
+def algorithmic_bytes(st: dict, node_bytes: int = 128) -> dict:
     """Bytes per launch from the event counts of one counting iteration.
 
-    traversal: 32 B per box test, two boxes per node visit; 36 B per triangle
+    traversal: one node per visit (128 B: the four child boxes and refs of the
+    4-wide BVH; 64 B for the binary one); 36 B per triangle
     test (three points); 40 B per closest hit (three normals + material).
     light pass: + 40 B per stored photon.  photon map: per radix pass 32 B per
     photon, + 88 B permute and 28 B key generation per photon.  camera trace:
@@ -42,9 +43,9 @@ def algorithmic_bytes(st: dict) -> dict:
     query read, 8 B per cell lookup, 12 B per photon tested, 28 B per photon
     accepted, 12 B per result.  resolve: 8 B per path, 12 B per query, 24 B
     per film update."""
-    light = (64 * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
+    light = (node_bytes * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
              + PHOTON_BYTES * st["photons"])
-    trace = (64 * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
+    trace = (node_bytes * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
              + 80 * st["gathers"] + 8 * st["camera_paths"])
     gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
               + 28 * st["photons_accepted"])
@@ -345,7 +346,7 @@ def main() -> int:
     eng.render_iteration()
     cst = eng.stats()
     eng.set("count_events", 0)
-    by = algorithmic_bytes(cst)
+    by = algorithmic_bytes(cst, int(eng.get("node_bytes")))
     dom = max(kern, key=kern.get)
     achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -483,8 +483,8 @@ __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int i, 
 }
 
 template <bool kCount>
-__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
-                                              uint32_t& n_nodes, uint32_t& n_tris) {
+__device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+                                               uint32_t& n_nodes, uint32_t& n_tris) {
     int ext[kMaxDepth - kLdsStack];
     float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
     float tbest = INFINITY;
@@ -551,6 +551,111 @@ __device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, f
     h.v = bv;
     h.tri = best;
     return best >= 0;
+}
+
+// Closest hit through the 4-wide BVH: 8 x float4 per node = the x, y, z
+// low / high planes of the four child boxes (one float per child) and the
+// four child refs (kEmptyChild where a node has fewer children).  The four
+// slab tests are those of the binary traversal; hit children are ordered by
+// entry distance (a 5-comparator network), the nearest is taken next and the
+// others are pushed far to near.
+template <bool kCount>
+__device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+                                               uint32_t& n_nodes, uint32_t& n_tris) {
+    int ext[kMaxDepth - kLdsStack];
+    float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
+    float tbest = INFINITY;
+    int best = -1;
+    float bu = 0.0f, bv = 0.0f;
+    int sp = 0;
+    int ref = sc.root4;
+    const float4* __restrict__ nodes = sc.nodes4;
+    const float slack = 1.0000019f;
+    auto push = [&](int r) {
+        if (sp < kLdsStack) lds_stack[sp * kBlock] = r;
+        else ext[sp - kLdsStack] = r;
+        ++sp;
+    };
+    while (true) {
+        if (ref >= 0) {
+            if (kCount) ++n_nodes;
+            const float4* n = nodes + 8 * ref;
+            const float4 xl = n[0], xh = n[1], yl = n[2], yh = n[3], zl = n[4], zh = n[5];
+            const int4 cr = *reinterpret_cast<const int4*>(n + 6);
+            const float xlo[4] = {xl.x, xl.y, xl.z, xl.w}, xhi[4] = {xh.x, xh.y, xh.z, xh.w};
+            const float ylo[4] = {yl.x, yl.y, yl.z, yl.w}, yhi[4] = {yh.x, yh.y, yh.z, yh.w};
+            const float zlo[4] = {zl.x, zl.y, zl.z, zl.w}, zhi[4] = {zh.x, zh.y, zh.z, zh.w};
+            const int refs[4] = {cr.x, cr.y, cr.z, cr.w};
+            float key[4];
+            int cref[4];
+            bool hit[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                float a0 = (xlo[c] - o.x) * ix, a1 = (xhi[c] - o.x) * ix;
+                float b0 = (ylo[c] - o.y) * iy, b1 = (yhi[c] - o.y) * iy;
+                float c0 = (zlo[c] - o.z) * iz, c1 = (zhi[c] - o.z) * iz;
+                float nr = fmaxf(fmaxf(fminf(a0, a1), fminf(b0, b1)), fmaxf(fminf(c0, c1), tmin));
+                float fr = fminf(fminf(fmaxf(a0, a1), fmaxf(b0, b1)), fmaxf(c0, c1)) * slack;
+                fr = fminf(fr, tbest);
+                hit[c] = nr <= fr && refs[c] != kEmptyChild;
+                key[c] = hit[c] ? nr : INFINITY;
+                cref[c] = refs[c];
+            }
+            auto cx = [&](int i, int j) {
+                if (key[j] < key[i]) {
+                    float tk = key[i];
+                    key[i] = key[j];
+                    key[j] = tk;
+                    int tr = cref[i];
+                    cref[i] = cref[j];
+                    cref[j] = tr;
+                    bool th = hit[i];
+                    hit[i] = hit[j];
+                    hit[j] = th;
+                }
+            };
+            cx(0, 1);
+            cx(2, 3);
+            cx(0, 2);
+            cx(1, 3);
+            cx(1, 2);
+            if (hit[0]) {
+                if (hit[3]) push(cref[3]);
+                if (hit[2]) push(cref[2]);
+                if (hit[1]) push(cref[1]);
+                ref = cref[0];
+                continue;
+            }
+        } else {
+            int enc = ~ref;
+            int first = enc >> 3, count = (enc & 7) + 1;
+            if (kCount) n_tris += count;
+            for (int i = first; i < first + count; ++i) {
+                float t, u, v;
+                if (tri_test(sc.tri, i, o, d, tmin, tbest, t, u, v)) {
+                    tbest = t;
+                    best = i;
+                    bu = u;
+                    bv = v;
+                }
+            }
+        }
+        if (sp == 0) break;
+        --sp;
+        ref = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+    }
+    h.t = tbest;
+    h.u = bu;
+    h.v = bv;
+    h.tri = best;
+    return best >= 0;
+}
+
+template <bool kCount>
+__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+                                              uint32_t& n_nodes, uint32_t& n_tris) {
+    return sc.nodes4 ? trace_closest4<kCount>(sc, o, d, tmin, h, lds_stack, n_nodes, n_tris)
+                     : trace_closest2<kCount>(sc, o, d, tmin, h, lds_stack, n_nodes, n_tris);
 }
 
 // Hit record of the reference (Triangle.cpp:71-80): geometric normal,

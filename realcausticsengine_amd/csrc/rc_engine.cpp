@@ -85,7 +85,7 @@ struct Engine {
     int exact_merge = 0;
     std::string output_path = "D:\\caustics.exr";
     // device scene
-    DevBuf<float4> d_tri, d_shade, d_nodes;
+    DevBuf<float4> d_tri, d_shade, d_nodes, d_nodes4;
     DevBuf<int> d_tri_id;
     DevBuf<DevMaterial> d_mats;
     DevBuf<DevLight> d_lights;
@@ -93,6 +93,8 @@ struct Engine {
     std::vector<DevLight> lights;
     float integral = 0.0f;
     int root = 0, n_nodes = 0;
+    int root4 = 0, n_nodes4 = 0;
+    int bvh_width = 4;  // 4: traverse the 4-wide BVH, 2: the binary one
     // per-iteration buffers
     DevBuf<float4> ph_pos, ph_dir, s_pos, s_dir;
     DevBuf<float2> ph_tp, s_tp;
@@ -445,6 +447,72 @@ struct Builder {
     }
 };
 
+// ---------------------------------------------------------------------
+// 4-wide BVH: the binary tree collapsed top-down; a node takes its binary
+// node's two children and keeps opening the inner child of largest surface
+// area until it has four children or only leaves.  Node layout (8 float4):
+// x lo, x hi, y lo, y hi, z lo, z hi of the four children, then their refs
+// (a ref >= 0 is a node, < 0 the leaf encoding of the binary tree,
+// kEmptyChild for a missing child), then padding.
+// ---------------------------------------------------------------------
+struct Child4 {
+    float lo[3], hi[3];
+    int ref;
+};
+
+void binary_children(const std::vector<float4>& bn, int ref, Child4 out[2]) {
+    const float4* n = &bn[4 * (size_t)ref];
+    out[0] = Child4{{n[0].x, n[0].z, n[1].x}, {n[0].y, n[0].w, n[1].y}, __builtin_bit_cast(int, n[3].x)};
+    out[1] = Child4{{n[1].z, n[2].x, n[2].z}, {n[1].w, n[2].y, n[2].w}, __builtin_bit_cast(int, n[3].y)};
+}
+
+float child_area(const Child4& c) {
+    float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+    return dx * dy + dy * dz + dz * dx;
+}
+
+int collapse4(const std::vector<float4>& bn, int bref, std::vector<float4>& out, int depth, int& max_depth) {
+    max_depth = std::max(max_depth, depth);
+    Child4 pair[2];
+    binary_children(bn, bref, pair);
+    std::vector<Child4> c = {pair[0], pair[1]};
+    while (c.size() < 4) {
+        int pick = -1;
+        float area = -1.0f;
+        for (size_t i = 0; i < c.size(); ++i)
+            if (c[i].ref >= 0 && child_area(c[i]) > area) {
+                area = child_area(c[i]);
+                pick = (int)i;
+            }
+        if (pick < 0) break;
+        int r = c[pick].ref;
+        c.erase(c.begin() + pick);
+        binary_children(bn, r, pair);
+        c.push_back(pair[0]);
+        c.push_back(pair[1]);
+    }
+    const size_t idx = out.size() / 8;
+    out.resize(out.size() + 8);
+    int refs[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+    for (size_t i = 0; i < c.size(); ++i)
+        refs[i] = c[i].ref >= 0 ? collapse4(bn, c[i].ref, out, depth + 1, max_depth) : c[i].ref;
+    float lo[3][4], hi[3][4];
+    for (int k = 0; k < 3; ++k)
+        for (int i = 0; i < 4; ++i) {
+            lo[k][i] = i < (int)c.size() ? c[i].lo[k] : 0.0f;
+            hi[k][i] = i < (int)c.size() ? c[i].hi[k] : 0.0f;
+        }
+    float4* n = &out[8 * idx];
+    for (int k = 0; k < 3; ++k) {
+        n[2 * k] = make_float4(lo[k][0], lo[k][1], lo[k][2], lo[k][3]);
+        n[2 * k + 1] = make_float4(hi[k][0], hi[k][1], hi[k][2], hi[k][3]);
+    }
+    n[6] = make_float4(__builtin_bit_cast(float, refs[0]), __builtin_bit_cast(float, refs[1]),
+                       __builtin_bit_cast(float, refs[2]), __builtin_bit_cast(float, refs[3]));
+    n[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    return (int)idx;
+}
+
 template <typename T>
 bool upload(DevBuf<T>& b, const std::vector<T>& v) {
     if (!b.alloc(v.size())) return false;
@@ -459,6 +527,8 @@ SceneView scene_view(const Engine& e) {
     sc.tri_id = e.d_tri_id.p;
     sc.nodes = e.d_nodes.p;
     sc.root = e.root;
+    sc.nodes4 = e.bvh_width == 4 ? e.d_nodes4.p : nullptr;
+    sc.root4 = e.root4;
     sc.mats = e.d_mats.p;
     sc.lights = e.d_lights.p;
     sc.cdf = e.d_cdf.p;
@@ -549,6 +619,11 @@ bool prepare(Engine& e) {
     }
     e.root = root;
     e.n_nodes = (int)(bld.nodes.size() / 4);
+    std::vector<float4> nodes4;
+    nodes4.reserve(bld.nodes.size());
+    int depth4 = 0;
+    e.root4 = collapse4(bld.nodes, root, nodes4, 0, depth4);
+    e.n_nodes4 = (int)(nodes4.size() / 8);
     std::vector<float4> tri(3 * ntri), shade(3 * ntri);
     std::vector<int> tri_id(ntri);
     for (size_t i = 0; i < ntri; ++i) {
@@ -588,7 +663,8 @@ bool prepare(Engine& e) {
     e.scene_radius = sqrtf(len2(mn - (mn + mx) * 0.5f));
     e.base_radius *= e.scene_radius;
     if (!upload(e.d_tri, tri) || !upload(e.d_shade, shade) || !upload(e.d_tri_id, tri_id) ||
-        !upload(e.d_nodes, bld.nodes) || !upload(e.d_mats, e.mats) || !upload(e.d_lights, e.lights) ||
+        !upload(e.d_nodes, bld.nodes) || !upload(e.d_nodes4, nodes4) || !upload(e.d_mats, e.mats) ||
+        !upload(e.d_lights, e.lights) ||
         !upload(e.d_cdf, cdf))
         return false;
     if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
@@ -1258,7 +1334,8 @@ bool write_exr(const char* path, const float* rgb, int W, int H) {
 
 void release(Engine*& e) {
     if (!e) return;
-    DevBuf<float4>* f4[] = {&e->d_tri, &e->d_shade, &e->d_nodes, &e->ph_pos, &e->ph_dir, &e->s_pos, &e->s_dir};
+    DevBuf<float4>* f4[] = {&e->d_tri, &e->d_shade, &e->d_nodes, &e->d_nodes4, &e->ph_pos, &e->ph_dir, &e->s_pos,
+                            &e->s_dir};
     for (auto* b : f4) b->release();
     e->ph_tp.release();
     e->s_tp.release();
@@ -1639,6 +1716,7 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         e->cell_scale = prev->cell_scale;
         e->persistent_blocks = prev->persistent_blocks;
         e->graph = prev->graph;
+        e->bvh_width = prev->bvh_width;
         e->light_blocks_per_cu = prev->light_blocks_per_cu;
         e->camera_blocks_per_cu = prev->camera_blocks_per_cu;
         e->gather_blocks_per_cu = prev->gather_blocks_per_cu;
@@ -1844,6 +1922,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "cell_scale") g->cell_scale = v;
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
     else if (k == "graph") g->graph = (int)v;
+    else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -1889,6 +1968,9 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "device_cus") return g->device_cus;
     if (k == "capacity") return g->capacity;
     if (k == "graph") return g->graph;
+    if (k == "bvh_width") return g->bvh_width;
+    if (k == "node_bytes") return g->bvh_width == 4 ? 128 : 64;
+    if (k == "nodes4") return g->n_nodes4;
     if (k == "graph_captures") return g->graph_captures;
     if (k == "light_blocks_per_cu") return g->light_blocks_per_cu;
     if (k == "camera_blocks_per_cu") return g->camera_blocks_per_cu;

@@ -25,6 +25,7 @@ constexpr int kLeafMax = 8;
 constexpr int kBlock = 128;      // threads per block of the tracing kernels
 constexpr int kLdsStack = 24;    // traversal stack entries held in LDS per lane
 constexpr int kMaxDepth = 64;    // total stack capacity (LDS + private overflow)
+constexpr int kEmptyChild = (int)0x80000000;  // missing child of a 4-wide BVH node
 
 struct DevMaterial {
     int kind;
@@ -45,6 +46,8 @@ struct SceneView {
     const int* tri_id;
     const float4* nodes;
     int root;
+    const float4* nodes4;  // 4-wide BVH (8 float4 per node), or null: traverse the binary nodes
+    int root4;
     const DevMaterial* mats;
     const DevLight* lights;
     const float* cdf;
