@@ -88,6 +88,10 @@ int rc_prepare(void);
 /* Renders the next iteration and accumulates it (one run_iteration,
  * src/Vertex_Merging.cpp:196-366).  Returns the iteration index or <0. */
 int rc_render_iteration(void);
+/* Renders debug_test mode 2 (shading normals), 3 (facing) or 4 (Fresnel of
+ * the view direction, IOR = debug_stuff.specular) into the film, replacing it
+ * (src/Debug_scene.cpp:21-205; main() does this for debug_test 2-4). */
+int rc_render_debug(int mode);
 /* Copies the accumulated image (width*height*3 floats, buffer row order of
  * src/Image_Buffer.h).  scaled != 0 divides by the iterations rendered. */
 int rc_get_image(float *rgb, int scaled);

@@ -61,6 +61,7 @@ def load(precision: str = "f32") -> ctypes.CDLL:
     lib.orc_bsdf.argtypes = [_fp, ctypes.c_int, _fp, _fp, _fp, _fp, _fp, ctypes.c_int, _dp]
     lib.orc_light_sample.argtypes = [ctypes.c_void_p, _fp, _dp]
     lib.orc_counters.argtypes = [ctypes.c_void_p, _llp]
+    lib.orc_debug_aov.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
     lib.orc_set_guide.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double]
     lib.orc_set_guide.restype = ctypes.c_int
     _libs[precision] = lib
@@ -152,6 +153,14 @@ class OracleScene:
         out = np.zeros(9, dtype=np.float64)
         self.lib.orc_light_sample(self.h, _ptr(u, _fp), _ptr(out, _dp))
         return out
+
+    def debug_aov(self, mode: int, ior: float, brute: bool = False) -> np.ndarray:
+        """Debug render mode 2, 3 or 4 of the reference (Debug_scene.cpp)."""
+        self.prepare()
+        W, H = self.args.camera.width, self.args.camera.height
+        img = np.zeros(W * H * 3, dtype=np.float64)
+        self.lib.orc_debug_aov(self.h, mode, float(ior), int(brute), _ptr(img, _dp))
+        return img.reshape(H, W, 3)
 
     def set_guide(self, cdf, slots: int, cells: int, slot_of_light, alpha: float) -> None:
         """Emission from the engine's learned proposal table (rows of 1025-float CDFs); cdf=None: plain."""

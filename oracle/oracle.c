@@ -1800,6 +1800,45 @@ long long orc_trace_light(void *sp, int iteration, double *photons, long long ca
     return l.n;
 }
 
+/* Debug AOVs: Vertex_Merging::run_debug_normals / _facing / _materials
+ * (Debug_scene.cpp:21-119, 156-205): one unjittered camera ray per pixel,
+ * Scene::hit with tmin 1e-4 (brute: the true closest hit instead of the
+ * world-BVH traversal); adds into image (W*H*3). */
+void orc_debug_aov(void *sp, int mode, double ior_d, int brute, double *image) {
+    Scene *s = (Scene *)sp;
+    const int W = s->W, H = s->H;
+    const real ior = (real)(float)ior_d;
+    for (long long p = 0; p < (long long)W * H; ++p) {
+        real sx = (real)(int)(p % W), sy = (real)(int)(p / W);
+        V dir = vadd(vadd(s->cam_bl, vscale(s->cam_h, sx / (real)W)), vscale(s->cam_v, sy / (real)H));
+        dir = vsub(dir, s->cam_o);
+        Ray r = {s->cam_o, vnormalize(dir)};
+        Isect h;
+        if (!(brute ? scene_hit_brute(s, &r, K(0.0001f), &h) : scene_hit(s, &r, K(0.0001f), &h))) continue;
+        V c;
+        if (mode == 2) {
+            c = h.sn;                                    /* Debug_scene.cpp:59 */
+        } else if (mode == 3) {
+            int front = vdot(h.sn, r.d) < 0;             /* Triangle.cpp:75 */
+            real factor = (real)fabs(vdot(r.d, h.sn));   /* Debug_scene.cpp:110 */
+            c = vscale(front ? v3(0, 0, 1) : v3(1, 0, 0), factor);
+        } else {
+            Frame fr = frame_from_z(h.sn);               /* Debug_scene.cpp:185-199 */
+            V wl = to_local(&fr, vscale(r.d, K(-1)));
+            real eta = wl.z > 0 ? K(1) / ior : ior;
+            real f = fresnel_dielectric_eta((real)fabs(wl.z), eta);
+            c = v3(f, f, f);
+        }
+        int X = (int)sx;
+        int Y = s->compat_rows ? H - (int)sy : H - 1 - (int)sy;
+        long long at = (long long)X + (long long)Y * W;
+        if (at > (long long)W * H - 1 || at < 0) continue;
+        image[3 * at] += c.x;
+        image[3 * at + 1] += c.y;
+        image[3 * at + 2] += c.z;
+    }
+}
+
 void orc_camera_pass(void *sp, int iteration, const double *photons, long long count, double *image) {
     Scene *s = (Scene *)sp;
     size_t n = (size_t)s->W * s->H * 3;

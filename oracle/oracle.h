@@ -74,6 +74,9 @@ void orc_camera_pass(void *scene, int iteration, const double *photons, long lon
 
 /* Whole render: iterations run_iteration calls then scale (Vertex_Merging.cpp:367-379). */
 void orc_render(void *scene, int iterations, double *image);
+/* Debug render modes 2 (shading normals), 3 (facing), 4 (Fresnel, IOR ior) added into image;
+ * brute != 0 takes the true closest hit instead of the reference's world-BVH traversal. */
+void orc_debug_aov(void *scene, int mode, double ior, int brute, double *image);
 
 /* Closest hit through the reference two-level BVH (brute = 0) or over all
  * triangles (brute = 1).  rays = n * {o.xyz, d.xyz}.  tri_out is the global

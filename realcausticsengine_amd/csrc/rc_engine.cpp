@@ -61,6 +61,7 @@ struct Engine {
     int samples = 1;
     float base_radius = 0.01f, radius_alpha = 0.75f;
     int debug_test = 1;
+    float debug_ior = 1.5f;  // debug_stuff.specular: the IOR of debug mode 4 (src/Debug_scene.cpp:182)
     // geometry in world space, global triangle order (mesh, then triangle)
     std::vector<float> wv;        // 3 floats per vertex (all meshes)
     std::vector<float> wn;        // 3 floats per vertex
@@ -1367,6 +1368,56 @@ void release(Engine*& e) {
     e = nullptr;
 }
 
+// The user's options (rc_set_option) of one Engine carried into a new one.
+void copy_options(const Engine& from, Engine& to) {
+    to.seed = from.seed;
+    to.photons = from.photons;
+    to.write_exr = from.write_exr;
+    to.compat_rows = from.compat_rows;
+    to.compat_point = from.compat_point;
+    to.compat_pdf = from.compat_pdf;
+    to.max_path_length = from.max_path_length;
+    to.photon_capacity = from.photon_capacity;
+    to.cell_scale = from.cell_scale;
+    to.persistent_blocks = from.persistent_blocks;
+    to.light_blocks_per_cu = from.light_blocks_per_cu;
+    to.camera_blocks_per_cu = from.camera_blocks_per_cu;
+    to.gather_blocks_per_cu = from.gather_blocks_per_cu;
+    to.path_chunk = from.path_chunk;
+    to.count_events = from.count_events;
+    to.exact_merge = from.exact_merge;
+    to.graph = from.graph;
+    to.bvh_width = from.bvh_width;
+    to.output_path = from.output_path;
+    const GuideModel& pg = from.guide;  // the proposal's options, not its state
+    GuideModel& ng = to.guide;
+    ng.enabled = pg.enabled;
+    ng.train = pg.train;
+    ng.alpha = pg.alpha;
+    ng.lr = pg.lr;
+    ng.batch = pg.batch;
+    ng.cells = pg.cells;
+    ng.warmup = pg.warmup;
+    ng.train_every = pg.train_every;
+    ng.seed = pg.seed;
+}
+
+// Debug render modes 2-4 into the film (replacing it); one pass, so the
+// film counts as one iteration.
+bool render_debug(Engine& e, int mode) {
+    if (mode < 2 || mode > 4) {
+        set_error("debug mode must be 2 (normals), 3 (facing) or 4 (materials)");
+        return false;
+    }
+    if (!prepare(e)) return false;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(e.image.p), 0u, (size_t)e.W * e.H * 3, e.stream);
+    launch_debug_aov(scene_view(e), mode, e.debug_ior, e.image.p, e.stream);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipStreamSynchronize(e.stream));
+    e.iterations_done = 1;
+    return true;
+}
+
 bool get_image(Engine& e, float* rgb, bool scaled) {
     size_t n = (size_t)e.W * e.H * 3;
     HIP_OK(hipMemcpy(rgb, e.image.p, n * sizeof(float), hipMemcpyDeviceToHost));
@@ -1700,40 +1751,12 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
                                                  rc_python_light* lights, unsigned int number_of_lights,
                                                  rc_python_material* materials, unsigned int number_of_materials,
                                                  int** materials_indices, int debug_test, rc_debug_stuff debug_stuff) {
-    (void)debug_stuff;
+    const float debug_ior = debug_stuff.specular;
     // options survive a re-init; geometry does not
     Engine* prev = g;
     Engine* e = new Engine();
     if (prev) {
-        e->seed = prev->seed;
-        e->photons = prev->photons;
-        e->write_exr = prev->write_exr;
-        e->compat_rows = prev->compat_rows;
-        e->compat_point = prev->compat_point;
-        e->compat_pdf = prev->compat_pdf;
-        e->max_path_length = prev->max_path_length;
-        e->photon_capacity = prev->photon_capacity;
-        e->cell_scale = prev->cell_scale;
-        e->persistent_blocks = prev->persistent_blocks;
-        e->graph = prev->graph;
-        e->bvh_width = prev->bvh_width;
-        e->light_blocks_per_cu = prev->light_blocks_per_cu;
-        e->camera_blocks_per_cu = prev->camera_blocks_per_cu;
-        e->gather_blocks_per_cu = prev->gather_blocks_per_cu;
-        e->path_chunk = prev->path_chunk;
-        e->exact_merge = prev->exact_merge;
-        e->output_path = prev->output_path;
-        const GuideModel& pg = prev->guide;  // guide options, not its state
-        GuideModel& ng = e->guide;
-        ng.enabled = pg.enabled;
-        ng.train = pg.train;
-        ng.alpha = pg.alpha;
-        ng.lr = pg.lr;
-        ng.batch = pg.batch;
-        ng.cells = pg.cells;
-        ng.warmup = pg.warmup;
-        ng.train_every = pg.train_every;
-        ng.seed = pg.seed;
+        copy_options(*prev, *e);
         release(prev);
     }
     g = e;
@@ -1743,6 +1766,7 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
     e->base_radius = base_radius;
     e->radius_alpha = radius_reduction;
     e->debug_test = debug_test;
+    e->debug_ior = debug_ior;
     e->initialised = true;
     if (samples < 1) set_error("samples must be >= 1");
     if (camera_x == 0 || camera_y == 0) set_error("camera resolution must be positive");
@@ -1882,27 +1906,19 @@ __attribute__((visibility("default"))) int main(void) {
             if (!render_iteration(e, it)) return 3;
             e.iterations_done = it + 1;
         }
+    } else if (e.debug_test >= 2 && e.debug_test <= 4) {
+        // src/main.cpp:172-180: one debug pass instead of the integrator
+        if (!render_debug(e, e.debug_test)) return 3;
     }
     int rc = 0;
     if (e.debug_test != 5 && e.write_exr) {
         std::vector<float> img((size_t)e.W * e.H * 3, 0.0f);
-        if (e.debug_test == 1 && !get_image(e, img.data(), true)) rc = 4;
+        if (e.debug_test >= 1 && e.debug_test <= 4 && !get_image(e, img.data(), true)) rc = 4;
         if (rc == 0 && !write_exr(e.output_path.c_str(), img.data(), e.W, e.H)) rc = 5;
     }
     // src/main.cpp:220-221: the scene is deleted after rendering; options survive
     Engine* keep = new Engine();
-    keep->seed = e.seed;
-    keep->photons = e.photons;
-    keep->write_exr = e.write_exr;
-    keep->compat_rows = e.compat_rows;
-    keep->compat_point = e.compat_point;
-    keep->compat_pdf = e.compat_pdf;
-    keep->max_path_length = e.max_path_length;
-    keep->photon_capacity = e.photon_capacity;
-    keep->cell_scale = e.cell_scale;
-    keep->persistent_blocks = e.persistent_blocks;
-    keep->exact_merge = e.exact_merge;
-    keep->output_path = e.output_path;
+    copy_options(e, *keep);
     release(g);
     g = keep;
     return rc;
@@ -2008,6 +2024,15 @@ __attribute__((visibility("default"))) int rc_render_iteration(void) {
     if (!render_iteration(*g, it)) return -1;
     g->iterations_done = it + 1;
     return it;
+}
+
+__attribute__((visibility("default"))) int rc_render_debug(int mode) {
+    g_error.clear();
+    if (!g || !g->initialised) {
+        set_error("rc_render_debug: no scene");
+        return -1;
+    }
+    return render_debug(*g, mode) ? 0 : -1;
 }
 
 __attribute__((visibility("default"))) int rc_get_image(float* rgb, int scaled) {

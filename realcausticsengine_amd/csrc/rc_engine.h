@@ -176,6 +176,8 @@ void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_resolve(const CameraPassArgs& a, hipStream_t s);
 void launch_first_hit(const SceneView& sc, const float* rays, int n, float tmin, int* tri, float* t, hipStream_t s);
+// Debug AOV of debug_test mode 2, 3 or 4 added into image (W * H * 3).
+void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hipStream_t s);
 
 // Photon map build: keys, sort, permute, cell ranges.
 struct MapBuildArgs {

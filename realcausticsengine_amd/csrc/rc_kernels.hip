@@ -624,6 +624,54 @@ void launch_resolve(const CameraPassArgs& a, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------
+// Debug AOVs (Vertex_Merging::run_debug_normals / _facing / _materials,
+// src/Debug_scene.cpp:21-119, 156-205): one unjittered camera ray per
+// pixel from the camera position with tmin 1e-4; mode 2 the interpolated
+// shading normal, 3 blue (front) or red (back) times |d . n|, 4 the
+// dielectric Fresnel reflectance of the view direction for IOR `ior`.
+// ---------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_debug_aov(SceneView sc, int mode, float ior, float* image) {
+    __shared__ int stack_lds[kLdsStack * kBlock];
+    const int W = sc.W, H = sc.H;
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= W * H) return;
+    float sx = (float)(p % W), sy = (float)(p / W);
+    F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
+    dir = dir - ld3(sc.cam_o);
+    F3 o = ld3(sc.cam_o), d = normalize(dir);
+    Hit h;
+    uint32_t n_nodes = 0, n_tris = 0;
+    if (!trace_closest<false>(sc, o, d, 0.0001f, h, stack_lds + threadIdx.x, n_nodes, n_tris)) return;
+    Surface sf = surface_at(sc, h, o, d);
+    F3 c;
+    if (mode == 2) {
+        c = sf.sn;
+    } else if (mode == 3) {
+        bool front = dot(sf.sn, d) < 0.0f;
+        float factor = fabsf(dot(d, sf.sn));
+        c = (front ? f3(0.0f, 0.0f, 1.0f) : f3(1.0f, 0.0f, 0.0f)) * factor;
+    } else {
+        Frame fr = frame_from_z(sf.sn);
+        F3 wl = to_local(fr, -d);
+        float eta = wl.z > 0.0f ? 1.0f / ior : ior;
+        c = f3s(fresnel_eta(fabsf(wl.z), eta));
+    }
+    // Image_Buffer::add_color (Image_Buffer.h:24-37); one camera sample per pixel
+    int X = (int)sx;
+    int Y = sc.compat_rows ? H - (int)sy : H - 1 - (int)sy;
+    long long at = (long long)X + (long long)Y * W;
+    if (at < 0 || at > (long long)W * H - 1) return;
+    image[3 * at] += c.x;
+    image[3 * at + 1] += c.y;
+    image[3 * at + 2] += c.z;
+}
+
+void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hipStream_t s) {
+    int n = sc.W * sc.H;
+    if (n > 0) hipLaunchKernelGGL(k_debug_aov, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, sc, mode, ior, image);
+}
+
+// ---------------------------------------------------------------------
 // First hit (parity diagnostic)
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_first_hit(SceneView sc, const float* rays, int n, float tmin, int* tri,

@@ -25,7 +25,7 @@ EXPORTS = [
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
-    "rc_batch_autotune",
+    "rc_batch_autotune", "rc_render_debug",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu"]
@@ -111,6 +111,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_batch_stats.argtypes = [P(ctypes.c_double), ctypes.c_int]
     lib.rc_batch_stats.restype = ctypes.c_int
     lib.rc_batch_release.restype = None
+    lib.rc_render_debug.argtypes = [ctypes.c_int]
+    lib.rc_render_debug.restype = ctypes.c_int
     for fn in (lib.rc_autotune, lib.rc_batch_autotune):
         fn.argtypes = [ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
         fn.restype = ctypes.c_int
@@ -220,6 +222,11 @@ class Engine:
         ph = np.ascontiguousarray(photons, dtype=np.float32)
         self._check(self.lib.rc_camera_pass_photons(iteration, _fp(ph), len(ph), _fp(img)), "rc_camera_pass_photons")
         return img.reshape(H, W, 3)
+
+    def render_debug(self, mode: int) -> np.ndarray:
+        """Debug render mode 2, 3 or 4 (src/Debug_scene.cpp) into the film; returns it."""
+        self._check(self.lib.rc_render_debug(int(mode)), "rc_render_debug")
+        return self.image(scaled=True)
 
     def autotune(self, repeats: int = 2) -> dict:
         """Chooses launch shapes and queue / tile sizes for this scene (rc_autotune)."""

@@ -56,3 +56,30 @@ def test_world_bounds_and_radius(oracle):
     half = np.linalg.norm(np.array(lo) - (np.array(lo) + np.array(hi)) * 0.5)
     assert abs(o.get("scene_radius") - half) < 1e-5
     assert abs(o.get("base_radius") - 0.01 * half) < 1e-7
+
+
+def test_debug_aovs_on_a_plane(oracle):
+    """Debug modes 2-4 (src/Debug_scene.cpp) where the camera sees only the
+    diffuse plane z = 0 (slab scene: camera between slab and plane looking
+    down): normal (0, 0, 1); front facing, blue x |cos|; Fresnel of the view
+    direction with eta = 1/ior, at normal incidence ((ior - 1)/(ior + 1))^2."""
+    import numpy as np
+    from realcausticsengine_amd import scenes
+    args = scenes.slab_scene(24, 24)
+    o = oracle.OracleScene(args, "f64", compat_rows=0)
+    n = o.debug_aov(2, 1.5)
+    hit = np.abs(n).sum(axis=2) > 0
+    assert hit.mean() > 0.99
+    np.testing.assert_allclose(n[hit], np.tile([0.0, 0.0, 1.0], (hit.sum(), 1)), atol=1e-12)
+    f = o.debug_aov(3, 1.5)
+    assert np.all(f[hit][:, 0] == 0.0) and np.all(f[hit][:, 1] == 0.0)
+    cos = f[hit][:, 2]
+    assert cos.min() > 0.9 and cos.max() <= 1.0
+    m = o.debug_aov(4, 1.5)
+    def fresnel(c, eta):
+        g = np.sqrt(eta * eta - 1 + c * c)
+        a = (g - c) / (g + c)
+        b = (c * (g + c) - 1) / (c * (g - c) + 1)
+        return 0.5 * a * a * (1 + b * b)
+    np.testing.assert_allclose(m[hit][:, 0], fresnel(cos, 1 / 1.5), rtol=1e-9)
+    assert abs(fresnel(1.0, 1 / 1.5) - (0.5 / 2.5) ** 2) < 1e-12
