@@ -78,6 +78,12 @@ int main(void);
  * (Adam steps before the proposal is sampled, default 2), "guide_train"
  * (0 = keep the network fixed), "guide_train_every" (after warm-up, train on
  * every n-th iteration, default 1), "guide_seed", "guide_reset".
+ * Acceleration structure: "bvh_build" (0 = binned SAH on the host, the
+ * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
+ * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
+ * host tree collapsed to 4-wide nodes, the default; 2 = binary nodes).
+ * Read-only: "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
+ * "nodes", "nodes4", "node_bytes" (bytes of the node layout traversed).
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
 int rc_set_option_str(const char *key, const char *value);

@@ -96,6 +96,9 @@ struct Engine {
     int root = 0, n_nodes = 0;
     int root4 = 0, n_nodes4 = 0;
     int bvh_width = 4;  // 4: traverse the 4-wide BVH, 2: the binary one
+    int bvh_build = 0;  // 0: binned SAH on the host, 1: linear BVH on the device (binary nodes)
+    int bvh_depth = 0;
+    double bvh_build_ms = 0.0;
     // per-iteration buffers
     DevBuf<float4> ph_pos, ph_dir, s_pos, s_dir;
     DevBuf<float2> ph_tp, s_tp;
@@ -521,6 +524,8 @@ bool upload(DevBuf<T>& b, const std::vector<T>& v) {
     return true;
 }
 
+bool uses_nodes4(const Engine& e) { return e.bvh_width == 4 && e.n_nodes4 > 0; }
+
 SceneView scene_view(const Engine& e) {
     SceneView sc{};
     sc.tri = e.d_tri.p;
@@ -528,7 +533,7 @@ SceneView scene_view(const Engine& e) {
     sc.tri_id = e.d_tri_id.p;
     sc.nodes = e.d_nodes.p;
     sc.root = e.root;
-    sc.nodes4 = e.bvh_width == 4 ? e.d_nodes4.p : nullptr;
+    sc.nodes4 = uses_nodes4(e) ? e.d_nodes4.p : nullptr;
     sc.root4 = e.root4;
     sc.mats = e.d_mats.p;
     sc.lights = e.d_lights.p;
@@ -565,6 +570,56 @@ bool ensure_device(Engine& e) {
     return true;
 }
 
+// Linear BVH built on the device over the triangles uploaded in input order
+// (option bvh_build = 1); binary nodes only, root 0.
+bool gpu_bvh(Engine& e, uint32_t n) {
+    DevBuf<float4> tmp4;
+    DevBuf<int> tmp_id, parent;
+    DevBuf<unsigned long long> ka, kb;
+    DevBuf<uint32_t> va, vb, hist, sums, flags, depth;
+    DevBuf<float> boxes;
+    bool ok = e.d_nodes.alloc(4 * (size_t)(n - 1)) && tmp4.alloc(6 * (size_t)n) && tmp_id.alloc(n) &&
+              parent.alloc(2 * (size_t)n) && ka.alloc(n) && kb.alloc(n) && va.alloc(n) && vb.alloc(n) &&
+              hist.alloc(photon_map_hist_entries(n)) && sums.alloc(photon_map_sum_entries(n)) && flags.alloc(n) &&
+              depth.alloc(1) && boxes.alloc(12 * (size_t)n);
+    int d = -1;
+    if (ok) {
+        LbvhBuffers b{e.d_tri.p, e.d_shade.p, e.d_tri_id.p, tmp4.p, tmp_id.p, e.d_nodes.p, ka.p, kb.p, va.p, vb.p,
+                      hist.p, sums.p, parent.p, flags.p, boxes.p, depth.p};
+        float ms = 0.0f;
+        ok = hipEventRecord(e.ev[0], e.stream) == hipSuccess;
+        d = ok ? build_lbvh(b, n, e.bbox.lo, e.bbox.hi, e.stream) : -1;
+        ok = ok && hipEventRecord(e.ev[1], e.stream) == hipSuccess && hipEventSynchronize(e.ev[1]) == hipSuccess &&
+             hipEventElapsedTime(&ms, e.ev[0], e.ev[1]) == hipSuccess;
+        e.bvh_build_ms = ms;
+        if (!ok) set_error("GPU BVH build: HIP event error");
+    }
+    tmp4.release();
+    tmp_id.release();
+    parent.release();
+    ka.release();
+    kb.release();
+    va.release();
+    vb.release();
+    hist.release();
+    sums.release();
+    flags.release();
+    depth.release();
+    boxes.release();
+    if (!ok) return false;
+    if (d < 0) {
+        set_error("GPU BVH build failed");
+        return false;
+    }
+    if (d >= kMaxDepth) {
+        set_error("BVH deeper than the traversal stack");
+        return false;
+    }
+    e.bvh_depth = d;
+    e.d_nodes4.release();
+    return true;
+}
+
 bool prepare(Engine& e) {
     if (e.prepared) return true;
     if (!e.initialised) {
@@ -597,14 +652,26 @@ bool prepare(Engine& e) {
         }
     }
     e.bbox = sb;
-    Box rootb;
-    bld.order.reserve(ntri);
-    int root = bld.build(0, (int)ntri, 0, rootb);
-    if (bld.max_depth >= kMaxDepth) {
-        set_error("BVH deeper than the traversal stack");
+    const bool gpu_tree = e.bvh_build == 1 && ntri >= 2;
+    if (gpu_tree && ntri >= (1u << 30)) {
+        set_error("bvh_build=1 takes fewer than 2^30 triangles");
         return false;
     }
-    if (root < 0) {
+    Box rootb;
+    int root = 0;
+    auto t_build = std::chrono::steady_clock::now();
+    if (gpu_tree) {
+        bld.order.resize(ntri);  // triangles go up in input order; the GPU build reorders them
+        for (size_t t = 0; t < ntri; ++t) bld.order[t] = (uint32_t)t;
+    } else {
+        bld.order.reserve(ntri);
+        root = bld.build(0, (int)ntri, 0, rootb);
+        if (bld.max_depth >= kMaxDepth) {
+            set_error("BVH deeper than the traversal stack");
+            return false;
+        }
+    }
+    if (!gpu_tree && root < 0) {
         // whole scene in one leaf: wrap it in a node whose second child is an empty box
         bld.nodes.resize(4);
         Box lb = Builder::padded(rootb);
@@ -619,11 +686,16 @@ bool prepare(Engine& e) {
         root = 0;
     }
     e.root = root;
-    e.n_nodes = (int)(bld.nodes.size() / 4);
+    e.n_nodes = gpu_tree ? (int)ntri - 1 : (int)(bld.nodes.size() / 4);
     std::vector<float4> nodes4;
-    nodes4.reserve(bld.nodes.size());
-    int depth4 = 0;
-    e.root4 = collapse4(bld.nodes, root, nodes4, 0, depth4);
+    e.root4 = 0;
+    if (!gpu_tree) {
+        nodes4.reserve(bld.nodes.size());
+        int depth4 = 0;
+        e.root4 = collapse4(bld.nodes, root, nodes4, 0, depth4);
+        e.bvh_depth = bld.max_depth;
+        e.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
+    }
     e.n_nodes4 = (int)(nodes4.size() / 8);
     std::vector<float4> tri(3 * ntri), shade(3 * ntri);
     std::vector<int> tri_id(ntri);
@@ -668,6 +740,7 @@ bool prepare(Engine& e) {
         !upload(e.d_lights, e.lights) ||
         !upload(e.d_cdf, cdf))
         return false;
+    if (gpu_tree && !gpu_bvh(e, (uint32_t)ntri)) return false;
     if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
     HIP_OK(hipMemset(e.image.p, 0, (size_t)e.W * e.H * 3 * sizeof(float)));
     if (!e.counters.alloc(4) || !e.stats_dev.alloc(2)) return false;
@@ -1388,6 +1461,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.exact_merge = from.exact_merge;
     to.graph = from.graph;
     to.bvh_width = from.bvh_width;
+    to.bvh_build = from.bvh_build;
     to.output_path = from.output_path;
     const GuideModel& pg = from.guide;  // the proposal's options, not its state
     GuideModel& ng = to.guide;
@@ -1939,6 +2013,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "persistent_blocks") g->persistent_blocks = (int)v;
     else if (k == "graph") g->graph = (int)v;
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
+    else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -1985,7 +2060,10 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "capacity") return g->capacity;
     if (k == "graph") return g->graph;
     if (k == "bvh_width") return g->bvh_width;
-    if (k == "node_bytes") return g->bvh_width == 4 ? 128 : 64;
+    if (k == "node_bytes") return uses_nodes4(*g) ? 128 : 64;
+    if (k == "bvh_build") return g->bvh_build;
+    if (k == "bvh_depth") return g->bvh_depth;
+    if (k == "bvh_build_ms") return g->bvh_build_ms;
     if (k == "nodes4") return g->n_nodes4;
     if (k == "graph_captures") return g->graph_captures;
     if (k == "light_blocks_per_cu") return g->light_blocks_per_cu;

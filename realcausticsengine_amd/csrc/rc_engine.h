@@ -210,6 +210,33 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s);
 // Batch: the keys of every scene's photons in one sort (bucket = the scene's
 // bucket_base + its own cell bucket; uid = global light path * 8 + length).
 void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s);
+// Stable LSD radix sort (8-bit digits) of (key, value) pairs on the low `bits`
+// key bits; the sorted pairs end in *keys_out / *vals_out (one of the two buffers).
+void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
+                      uint32_t* hist, uint32_t* hist_sums, const uint32_t* count_dev, uint32_t n, int bits,
+                      unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s);
+// GPU BVH build (rc_bvh.hip): linear BVH over Morton codes (one triangle per
+// leaf) in the binary node layout; tri / shade / tri_id are reordered into
+// leaf order.  Returns the tree depth (< 0 on a launch error).
+struct LbvhBuffers {
+    float4* tri;      // 3 per triangle, reordered in place (via tmp)
+    float4* shade;
+    int* tri_id;
+    float4* tmp4;     // 6 * n float4 scratch
+    int* tmp_id;      // n
+    float4* nodes;    // 4 * max(1, n - 1) float4: the binary nodes, root 0
+    unsigned long long* keys_a;
+    unsigned long long* keys_b;
+    uint32_t* vals_a;
+    uint32_t* vals_b;
+    uint32_t* hist;
+    uint32_t* hist_sums;
+    int* parent;      // 2 * n: parents of internal nodes, then of leaves
+    uint32_t* flags;  // n
+    float* boxes;     // 6 * n: unpadded boxes of the internal nodes
+    uint32_t* depth;  // 1
+};
+int build_lbvh(const LbvhBuffers& b, uint32_t n, const float lo[3], const float hi[3], hipStream_t s);
 size_t photon_map_hist_entries(uint32_t max_count);
 size_t photon_map_sum_entries(uint32_t max_count);
 

@@ -948,21 +948,20 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
 
 // LSD radix sort of the keys (8-bit digits, stable), permutation of the
 // photons into key order, per-bucket [start, end).
-static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
-    uint32_t n = a.count;
-    unsigned g = (n + 255) / 256;
+void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
+                      uint32_t* hist, uint32_t* hist_sums, const uint32_t* count_dev, uint32_t n, int bits,
+                      unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s) {
     uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
-    int bits = a.bucket_bits + a.uid_bits;
-    unsigned long long* kin = a.keys_a;
-    unsigned long long* kout = a.keys_b;
-    uint32_t* vin = a.vals_a;
-    uint32_t* vout = a.vals_b;
-    for (int shift = 0; shift < bits; shift += 8) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, a.count_dev, n, shift, a.hist,
+    unsigned long long* kin = keys_a;
+    unsigned long long* kout = keys_b;
+    uint32_t* vin = vals_a;
+    uint32_t* vout = vals_b;
+    for (int shift = 0; shift < bits && n > 0; shift += 8) {
+        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, count_dev, n, shift, hist,
                            nblocks);
-        launch_exclusive_scan_u32(a.hist, a.hist, 256 * nblocks, a.hist_sums, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, a.count_dev, n, shift,
-                           a.hist, nblocks, kout, vout);
+        launch_exclusive_scan_u32(hist, hist, 256 * nblocks, hist_sums, s);
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev, n, shift,
+                           hist, nblocks, kout, vout);
         unsigned long long* tk = kin;
         kin = kout;
         kout = tk;
@@ -970,6 +969,17 @@ static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
         vin = vout;
         vout = tv;
     }
+    *keys_out = kin;
+    *vals_out = vin;
+}
+
+static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
+    uint32_t n = a.count;
+    unsigned g = (n + 255) / 256;
+    unsigned long long* kin;
+    uint32_t* vin;
+    radix_sort_pairs(a.keys_a, a.keys_b, a.vals_a, a.vals_b, a.hist, a.hist_sums, a.count_dev, n,
+                     a.bucket_bits + a.uid_bits, &kin, &vin, s);
     hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_pos, a.in_dir, a.in_tp, vin, a.count_dev, n,
                        a.out_pos, a.out_dir, a.out_tp, a.out_index);
     hipLaunchKernelGGL(k_map_cells, dim3(g), dim3(256), 0, s, kin, a.count_dev, n, a.uid_bits, a.cells);
