@@ -142,6 +142,70 @@ def uneven_batch(a):
     return out
 
 
+def at_scale(a) -> int:
+    """The at-scale batched + learned workload (scenes.c4_batch: displaced glass
+    meshes and water heightfields of 2M-10M triangles, four lights each, 2048^2,
+    64 x 2^20 light paths per scene and iteration, one learned proposal per
+    scene trained every iteration, trees built on the GPU).  The scenes run in
+    batches of --scale-group (a batch's light paths index 32-bit photon ids:
+    at most 2^29 per launch), one batch resident at a time; prints one JSON line."""
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import GUIDE_STEP_FLOPS_PER_ROW, Batch
+
+    n, group = a.at_scale, a.scale_group
+    opts = dict(write_exr=0, bvh_build=1, guide=1, guide_batch=65536, guide_warmup=1, compat_point_light=0,
+                compat_light_pdf=0)
+    groups, totals = [], dict(light=0, camera=0, tris=0, s_iter=0.0, ms_guide=0.0, guide_rows=0.0, prep=0.0)
+    for g0 in range(0, n, group):
+        t = time.perf_counter()
+        cases = [scenes.c4_scene(s, a.scale_res, a.scale_photons) for s in range(g0, min(n, g0 + group))]
+        gen_s = time.perf_counter() - t
+        t = time.perf_counter()
+        b = Batch([sc for sc, _ in cases], [dict(opts, photons=p, seed=g0 + i) for i, (_, p) in enumerate(cases)])
+        b.prepare()
+        prep_s = time.perf_counter() - t
+        tris = int(sum(sc.total_tris for sc, _ in cases))
+        print(f"[bench] at scale: scenes {g0}..{g0 + len(cases) - 1}, {tris} triangles, generated in {gen_s:.1f} s, "
+              f"prepared in {prep_s:.1f} s", file=sys.stderr, flush=True)
+        b.render_iteration()  # warm-up (buffers; the proposals' first step)
+        t0 = time.perf_counter()
+        ms_guide = rows = 0.0
+        for _ in range(a.scale_steps):
+            b.render_iteration()
+            st = b.stats()
+            ms_guide += st["ms_guide"]
+            rows += st["guide_rows"]
+        per_it = (time.perf_counter() - t0) / a.scale_steps
+        b.release()
+        light = sum(p for _, p in cases)
+        camera = sum(sc.camera.width * sc.camera.height for sc, _ in cases)
+        rec = {"scenes": [g0, g0 + len(cases) - 1], "triangles": tris, "light_paths": int(light),
+               "camera_paths": int(camera), "photons": int(st["photons"]), "ms_per_iteration": round(per_it * 1e3, 1),
+               "kernel_ms": {k: round(st[k], 2) for k in ("ms_light", "ms_map", "ms_camera_trace", "ms_gather",
+                                                           "ms_resolve")},
+               "ms_guide_per_iteration": round(ms_guide / a.scale_steps, 3), "prepare_s": round(prep_s, 2)}
+        print(f"[bench]   {rec}", file=sys.stderr, flush=True)
+        groups.append(rec)
+        totals["light"] += light
+        totals["camera"] += camera
+        totals["tris"] += tris
+        totals["s_iter"] += per_it
+        totals["ms_guide"] += ms_guide / a.scale_steps
+        totals["guide_rows"] += rows / a.scale_steps
+        totals["prep"] += prep_s
+    gflop = GUIDE_STEP_FLOPS_PER_ROW * totals["guide_rows"] / 1e9
+    out = {"at_scale": {
+        "scenes": n, "scenes_per_launch": group, "triangles": totals["tris"], "resolution": a.scale_res,
+        "light_paths_per_iteration": totals["light"], "camera_paths_per_iteration": totals["camera"],
+        "s_per_iteration_all_scenes": round(totals["s_iter"], 3),
+        "photon_paths_per_s": round(totals["light"] / totals["s_iter"], 1),
+        "ms_guide_per_iteration_all_scenes": round(totals["ms_guide"], 2),
+        "guide_gflop_per_iteration": round(gflop, 2),
+        "prepare_s_total": round(totals["prep"], 1), "steps": a.scale_steps, "groups": groups}}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def guide_steps(a) -> int:
     """Only the learned proposal's training step: `--guide-steps` steps of 65,536
     synthetic rows (forward, backward, Adam) through rc_guide_train_step, timed
@@ -295,6 +359,12 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")
     ap.add_argument("--guide-steps", type=int, default=0,
                     help="only time this many training steps of the learned proposal (65,536 rows each)")
+    ap.add_argument("--at-scale", type=int, default=0,
+                    help="only run the at-scale batched + learned workload over this many scenes (16 = the full set)")
+    ap.add_argument("--scale-group", type=int, default=4)
+    ap.add_argument("--scale-steps", type=int, default=3)
+    ap.add_argument("--scale-res", type=int, default=2048)
+    ap.add_argument("--scale-photons", type=int, default=64 << 20)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
     a = ap.parse_args()
@@ -304,6 +374,8 @@ def main() -> int:
         return 2
     if a.guide_steps > 0:
         return guide_steps(a)
+    if a.at_scale > 0:
+        return at_scale(a)
 
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Engine

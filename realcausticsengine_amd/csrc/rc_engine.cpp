@@ -813,15 +813,17 @@ GuideView guide_light_view(Engine& e, int iteration) {
     return v;
 }
 
-bool ensure_guide_buffers(Engine& e) {
+// s: the stream the proposal's set-up runs on (the scene's own by default)
+bool ensure_guide_buffers(Engine& e, hipStream_t s = nullptr) {
     GuideModel& gd = e.guide;
     if (!guide_active(e)) return true;
+    if (!s) s = e.stream;
     if (gd.cells_built != gd.cells &&
         !gd.setup_scene(e.lights, e.compat_point, e.bbox.lo, e.bbox.hi))
         return false;
     int n = std::max(64, guide_rows(e));
-    if (!gd.params_ready && !gd.init_params(e.stream)) return false;
-    if (!gd.table_ready && gd.step >= gd.warmup && !gd.build_table(e.stream)) return false;
+    if (!gd.params_ready && !gd.init_params(s)) return false;
+    if (!gd.table_ready && gd.step >= gd.warmup && !gd.build_table(s)) return false;
     return gd.train_x.alloc((size_t)n * kGuideRowFloats) && gd.train_bin.alloc(n) && gd.train_w.alloc(n);
 }
 
@@ -1517,8 +1519,12 @@ struct Batch {
     DevBuf<BatchScene> d_bs;
     DevBuf<uint32_t> d_starts;  // light starts (n + 1), then camera starts (n + 1)
     DevBuf<float> scratch;      // films of the autotuner's iterations
+    DevBuf<GuideView> d_guides;     // per scene proposal views of the light pass
+    DevBuf<GuideTargets> d_targets; // per scene training targets
+    DevBuf<uint8_t> vis;            // camera-visible cells of every scene's grid
     double stats[RC_STAT_COUNT] = {0};
     int iterations = 0;
+    double guide_ms = 0, guide_step_ms = 0, guide_rows = 0;  // last training of the scenes' proposals
 };
 
 Batch* g_batch = nullptr;
@@ -1542,6 +1548,9 @@ void release_batch(Batch*& b) {
     b->d_bs.release();
     b->d_starts.release();
     b->scratch.release();
+    b->d_guides.release();
+    b->d_targets.release();
+    b->vis.release();
     for (auto& ev : w.ev)
         if (ev) (void)hipEventDestroy(ev);
     for (auto& ev : w.ev_guide)
@@ -1586,6 +1595,46 @@ bool batch_prepare(Batch& b) {
     for (size_t i = 0; i < n; ++i) sc[i] = scene_view(*b.scenes[i]);
     HIP_OK(hipMemcpy(b.d_sc.p, sc.data(), n * sizeof(SceneView), hipMemcpyHostToDevice));
     b.prepared = true;
+    return true;
+}
+
+// Training step of every scene whose proposal trains this iteration:
+// camera-visible cells of each scene's grid (one launch over all queries),
+// per-path visible luminance (one launch over all photons), then each
+// scene's network step and table on the batch stream.
+bool batch_guide_training(Batch& b, const BatchView& view, uint32_t count, uint32_t nq,
+                          const std::vector<GuideTargets>& targets, hipStream_t st) {
+    Engine& w = b.work;
+    uint64_t buckets = w.cells.n;
+    if (!b.vis.alloc(buckets + 3)) return false;
+    HIP_OK(hipEventRecord(w.ev_guide[0], st));
+    launch_fill_u32(reinterpret_cast<uint32_t*>(b.vis.p), 0u, (buckets + 3) / 4, st);
+    launch_mark_visible_batch(w.q_pos.p, w.q_gn.p, std::min(nq, w.q_capacity), view, b.vis.p, st);
+    for (const GuideTargets& t : targets)
+        if (t.rows) launch_fill_u32(reinterpret_cast<uint32_t*>(t.w), 0u, t.rows, st);
+    launch_guide_targets_batch(w.ph_pos.p, w.ph_dir.p, w.ph_tp.p, w.ph_uid.p, count, view, b.vis.p, b.d_targets.p,
+                               st);
+    HIP_OK(hipEventRecord(w.ev_guide[2], st));
+    double rows = 0.0;
+    for (size_t i = 0; i < b.scenes.size(); ++i) {
+        if (!targets[i].rows) continue;
+        GuideModel& gd = b.scenes[i]->guide;
+        const int n = (int)targets[i].rows;
+        if (!gd.forward(gd.train_x.p, n, st) || !gd.backward(gd.train_bin.p, gd.train_w.p, n, st) || !gd.adam(st) ||
+            !gd.build_table(st))
+            return false;
+        HIP_OK(hipMemcpyAsync(gd.last, gd.scal.p, sizeof gd.last, hipMemcpyDeviceToHost, st));
+        rows += n;
+    }
+    HIP_OK(hipEventRecord(w.ev_guide[3], st));
+    HIP_OK(hipEventRecord(w.ev_guide[1], st));
+    HIP_OK(hipStreamSynchronize(st));
+    float ms_guide = 0, ms_step = 0;
+    HIP_OK(hipEventElapsedTime(&ms_guide, w.ev_guide[0], w.ev_guide[1]));
+    HIP_OK(hipEventElapsedTime(&ms_step, w.ev_guide[2], w.ev_guide[3]));
+    b.guide_ms = ms_guide;
+    b.guide_step_ms = ms_step;
+    b.guide_rows = rows;
     return true;
 }
 
@@ -1661,6 +1710,26 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         pm[i].tp_gb = w.s_tp.p;
         pm[i].cells = w.cells.p + bs[i].bucket_base;
     }
+    // per-scene emission proposals: sampled in the light pass, trained after the camera pass
+    std::vector<GuideView> gvs(n);
+    std::vector<GuideTargets> targets(n);
+    bool any_train = false;
+    for (size_t i = 0; i < n; ++i) {
+        Engine& e = *b.scenes[i];
+        gvs[i] = plain_guide_view();
+        targets[i] = GuideTargets{nullptr, 0u};
+        if (!guide_active(e)) continue;
+        if (!ensure_guide_buffers(e, st)) return false;
+        gvs[i] = guide_light_view(e, e.iterations_done);
+        if (scratch) gvs[i].train_paths = 0;
+        if (gvs[i].train_paths > 0) {
+            targets[i] = GuideTargets{e.guide.train_w.p, gvs[i].train_paths};
+            any_train = true;
+        }
+    }
+    if (!b.d_guides.alloc(n) || !b.d_targets.alloc(n)) return false;
+    HIP_OK(hipMemcpyAsync(b.d_guides.p, gvs.data(), n * sizeof(GuideView), hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(b.d_targets.p, targets.data(), n * sizeof(GuideTargets), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(b.d_bs.p, bs.data(), n * sizeof(BatchScene), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(b.d_pm.p, pm.data(), n * sizeof(PhotonMapView), hipMemcpyHostToDevice, st));
     HIP_OK(hipMemcpyAsync(b.d_starts.p, starts.data(), starts.size() * sizeof(uint32_t), hipMemcpyHostToDevice, st));
@@ -1671,6 +1740,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     view.bs = b.d_bs.p;
     view.light_starts = b.d_starts.p;
     view.camera_starts = b.d_starts.p + n + 1;
+    view.guides = b.d_guides.p;
     const Engine& e0 = *b.scenes[0];
     // light pass (grow and repeat on overflow)
     uint32_t count = 0;
@@ -1764,6 +1834,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     launch_resolve(c, st);
     HIP_OK(hipEventRecord(w.ev[5], st));
     HIP_OK(hipGetLastError());
+    if (any_train && !batch_guide_training(b, view, count, nq, targets, st)) return false;
     HIP_OK(hipStreamSynchronize(st));
     float ms[8] = {0};
     HIP_OK(hipEventElapsedTime(&ms[0], w.ev[0], w.ev[1]));
@@ -1784,6 +1855,11 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     b.stats[RC_STAT_MS_GATHER] = ms[4];
     b.stats[RC_STAT_MS_RESOLVE] = ms[5];
     b.stats[RC_STAT_SORT_PASSES] = (bucket_bits + uid_bits + 7) / 8;
+    if (any_train) {
+        b.stats[RC_STAT_MS_GUIDE] = b.guide_ms;
+        b.stats[RC_STAT_MS_GUIDE_STEP] = b.guide_step_ms;
+        b.stats[RC_STAT_GUIDE_ROWS] = b.guide_rows;
+    }
     if (scratch) return true;
     for (Engine* e : b.scenes) e->iterations_done++;
     b.iterations++;

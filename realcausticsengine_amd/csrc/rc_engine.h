@@ -98,6 +98,13 @@ struct BatchView {
     const BatchScene* bs;            // per scene
     const uint32_t* light_starts;    // n + 1 prefix sums of light paths
     const uint32_t* camera_starts;   // n + 1 prefix sums of camera paths
+    const GuideView* guides;         // per scene emission proposal (the plain view for unguided scenes)
+};
+
+// Training targets of one scene of a batch: w[local path] for local paths < rows.
+struct GuideTargets {
+    float* w;
+    uint32_t rows;
 };
 
 // Scene of a global path index: the last s with starts[s] <= i.
@@ -125,7 +132,7 @@ struct LightPassArgs {
     uint32_t* out_uid;
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
-    BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1 per scene; no guide)
+    BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)
 };
 
 // Per-iteration values read from device memory by the kernels of a captured
@@ -210,6 +217,15 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s);
 // Batch: the keys of every scene's photons in one sort (bucket = the scene's
 // bucket_base + its own cell bucket; uid = global light path * 8 + length).
 void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s);
+// Training targets of the scenes of a batch (rc_guide.hip): vis[bucket_base +
+// bucket] = 1 for the cell of every query (its scene in gn.w), then
+// t[scene].w[local path] += luminance of each photon of a training path in a
+// visible cell of its scene.
+void launch_mark_visible_batch(const float4* qpos, const float4* qgn, uint32_t nq, const BatchView& b, uint8_t* vis,
+                               hipStream_t s);
+void launch_guide_targets_batch(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
+                                uint32_t count, const BatchView& b, const uint8_t* vis, const GuideTargets* t,
+                                hipStream_t s);
 // Stable LSD radix sort (8-bit digits) of (key, value) pairs on the low `bits`
 // key bits; the sorted pairs end in *keys_out / *vals_out (one of the two buffers).
 void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,

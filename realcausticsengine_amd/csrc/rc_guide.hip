@@ -3,6 +3,7 @@
 // forward layers, the backward data products and the split-K weight
 // gradients), softmax cross-entropy, Adam, the proposal table (softmax ->
 // CDF per position cell) and the training targets.
+#include "rc_engine.h"
 #include "rc_guide.h"
 #include "rc_vec.h"
 
@@ -356,6 +357,36 @@ __global__ void k_guide_targets(const float4* __restrict__ pos, const float4* __
     }
 }
 
+__global__ void k_mark_visible_batch(const float4* __restrict__ qpos, const float4* __restrict__ qgn, uint32_t n,
+                                     BatchView b, uint8_t* __restrict__ vis) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int sc = __float_as_int(qgn[i].w);
+        const PhotonMapView& pm = b.pm[sc];
+        vis[b.bs[sc].bucket_base +
+            bucket_of(qpos[i], pm.origin[0], pm.origin[1], pm.origin[2], pm.inv_cell, pm.mask)] = 1;
+    }
+}
+
+__global__ void k_guide_targets_batch(const float4* __restrict__ pos, const float4* __restrict__ dir_tr,
+                                      const float2* __restrict__ tp_gb, const uint32_t* __restrict__ uid, uint32_t n,
+                                      BatchView b, const uint8_t* __restrict__ vis,
+                                      const GuideTargets* __restrict__ t) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint32_t path = uid[i] >> 3;
+        int sc = batch_scene_of(b.light_starts, b.n, path);
+        const BatchScene& bs = b.bs[sc];
+        uint32_t local = path - bs.light_base;
+        GuideTargets tg = t[sc];
+        if (local >= tg.rows) continue;
+        const PhotonMapView& pm = b.pm[sc];
+        if (!vis[bs.bucket_base + bucket_of(pos[i], pm.origin[0], pm.origin[1], pm.origin[2], pm.inv_cell, pm.mask)])
+            continue;
+        float2 gb = tp_gb[i];
+        float lum = (dir_tr[i].w + gb.x + gb.y) * (1.0f / 3.0f);
+        atomicAdd(tg.w + local, lum);
+    }
+}
+
 // ---- launchers ----
 void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_guide_encode, dim3((n + 127) / 128), dim3(128), 0, s, x, n, X);
@@ -392,6 +423,20 @@ void launch_guide_targets(const float4* pos, const float4* dir_tr, const float2*
     if (count > 0 && train_paths > 0)
         hipLaunchKernelGGL(k_guide_targets, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0, s, pos,
                            dir_tr, tp_gb, uid, count, ox, oy, oz, inv_cell, mask, vis, train_paths, w);
+}
+
+void launch_mark_visible_batch(const float4* qpos, const float4* qgn, uint32_t nq, const BatchView& b, uint8_t* vis,
+                               hipStream_t s) {
+    if (nq > 0)
+        hipLaunchKernelGGL(k_mark_visible_batch, dim3(std::min<uint32_t>(4096, (nq + 255) / 256)), dim3(256), 0, s,
+                           qpos, qgn, nq, b, vis);
+}
+void launch_guide_targets_batch(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
+                                uint32_t count, const BatchView& b, const uint8_t* vis, const GuideTargets* t,
+                                hipStream_t s) {
+    if (count > 0)
+        hipLaunchKernelGGL(k_guide_targets_batch, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0,
+                           s, pos, dir_tr, tp_gb, uid, count, b, vis, t);
 }
 
 }  // namespace rc

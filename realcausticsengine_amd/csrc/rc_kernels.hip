@@ -118,17 +118,18 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
                 }
                 rng.path = st.local;
                 const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
+                const GuideView& gv = kBatch ? a.batch.guides[st.scene] : a.guide;
                 // Generate_light_sample (Vertex_Merging.cpp:18-25)
                 int bin;
                 F3 axis;
-                active = sample_light(sc, rng, a.guide, st.o, st.d, st.tp, bin, axis);
-                if (slot < a.guide.train_paths) {
+                active = sample_light(sc, rng, gv, st.o, st.d, st.tp, bin, axis);
+                if (st.local < gv.train_paths) {
                     float rec[kGuideRowFloats];
-                    guide_record(a.guide, st.o.x, st.o.y, st.o.z, axis.x, axis.y, axis.z, rec);
-                    float4* dst = reinterpret_cast<float4*>(a.guide.train_x + (size_t)slot * kGuideRowFloats);
+                    guide_record(gv, st.o.x, st.o.y, st.o.z, axis.x, axis.y, axis.z, rec);
+                    float4* dst = reinterpret_cast<float4*>(gv.train_x + (size_t)st.local * kGuideRowFloats);
                     dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
                     dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
-                    a.guide.train_bin[slot] = active ? bin : -1;
+                    gv.train_bin[st.local] = active ? bin : -1;
                 }
                 st.pl = 0;
                 st.diffuse = 0;

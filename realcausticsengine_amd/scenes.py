@@ -242,3 +242,105 @@ def blob_scene(seed: int, samples: int = 1):
 def uneven_batch(n: int = 64, samples: int = 1):
     """The uneven-scene batch: blob_scene(seed) for seeds 0 .. n-1."""
     return [blob_scene(s, samples) for s in range(n)]
+
+
+def displaced_cube_sphere(m: int, seed: int, amplitude: float = 0.1, waves: int = 8):
+    """Unit sphere made of the six faces of a cube, m x m quads per face
+    (12 m^2 triangles, any count, unlike icosphere levels), radially displaced
+    by amplitude * mean of `waves` seeded spherical sines sin(f (n . d) + phase).
+    Face-edge vertices come from the same grid values on both faces, so the
+    seams close exactly; normals are the analytic normals of the displaced
+    surface, n - grad_t(r) / r (continuous across seams)."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(waves, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    f = rng.uniform(2.0, 8.0, waves)
+    ph = rng.uniform(0.0, 2.0 * np.pi, waves)
+    g = np.linspace(-1.0, 1.0, m + 1)
+    U, V = np.meshgrid(g, g, indexing="xy")
+    i = np.arange(m)
+    I, J = np.meshgrid(i, i, indexing="xy")
+    a = (J * (m + 1) + I).reshape(-1)
+    b, c = a + 1, a + (m + 1)
+    dd = c + 1
+    verts, norms, tris = [], [], []
+    base = 0
+    for axis in range(3):
+        for sign in (-1.0, 1.0):
+            P = np.empty((m + 1, m + 1, 3))
+            P[..., axis] = sign
+            P[..., (axis + 1) % 3] = U
+            P[..., (axis + 2) % 3] = V
+            n = (P / np.sqrt((P * P).sum(-1, keepdims=True))).reshape(-1, 3)
+            arg = (n @ d.T) * f + ph
+            r = 1.0 + amplitude * np.mean(np.sin(arg), axis=1)
+            grad = amplitude * (np.cos(arg) * f) @ d / waves
+            grad_t = grad - (grad * n).sum(1, keepdims=True) * n
+            nn = n - grad_t / r[:, None]
+            nn /= np.linalg.norm(nn, axis=1, keepdims=True)
+            verts.append((n * r[:, None]).astype(np.float32))
+            norms.append(nn)
+            if sign > 0:  # e_(axis+1) x e_(axis+2) = e_axis: outward winding
+                t = np.concatenate([np.stack([a, b, dd], 1), np.stack([a, dd, c], 1)])
+            else:
+                t = np.concatenate([np.stack([a, dd, b], 1), np.stack([a, c, dd], 1)])
+            tris.append((t + base).astype(np.uint32))
+            base += (m + 1) * (m + 1)
+    return np.concatenate(verts), np.concatenate(norms), np.concatenate(tris)
+
+
+def _loguniform(rng, lo, hi):
+    return float(np.exp(rng.uniform(np.log(lo), np.log(hi))))
+
+
+def c4_scene(seed: int, res: int = 2048, photons: int = 64 << 20, samples: int = 1):
+    """One scene of the at-scale batch: a seeded displaced glass mesh (even
+    seeds) or water heightfield (odd seeds) of 2M-10M triangles (log-uniform),
+    IOR U(1.3, 1.7) for glass and 1.33 for water, roughness U(0, 0.2) (glass is
+    a delta interface in the reference: recorded, not used), over a diffuse
+    floor; four lights -- a point light, a quad, a sphere and one more of a
+    random kind among those three -- with log-uniform powers in [0.25, 4];
+    res^2 camera; `photons` light paths per iteration.  Point lights emit only
+    with compat_point_light = 0 (isotropic emission).  Returns (SceneArgs, photons)."""
+    rng = np.random.default_rng(4000 + seed)
+    want = _loguniform(rng, 2e6, 1e7)
+    rough = float(rng.uniform(0.0, 0.2))
+    if seed % 2 == 0:
+        m = int(round(np.sqrt(want / 12.0)))
+        bv, bn, bt = displaced_cube_sphere(m, seed)
+        ior = float(rng.uniform(1.3, 1.7))
+        obj = MeshData(bv, bn, bt, matrix=translation(0.0, 0.0, 1.3), material=0)
+        floor = plane(10.0, 10.0, 0.0)
+        look, cam_pos = (0.0, 0.0, 0.8), (0.0, -6.0, 4.0)
+    else:
+        n = int(round(np.sqrt(want / 2.0))) + 1
+        hv, hn, ht = heightfield(n, 32, seed)
+        ior = 1.33
+        obj = MeshData(hv, hn, ht, material=0)
+        floor = plane(4.0, 4.0, -0.5)
+        look, cam_pos = (0.0, 0.0, 0.0), (0.0, -1.6, 1.6)
+    fv, fn, ft = floor
+    meshes = [obj, MeshData(fv, fn, ft, material=1)]
+    mats = [MaterialData("G", ior=ior, roughness=rough, color=(1.0, 1.0, 1.0)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.8, 0.8))]
+    z = 5.0 if seed % 2 == 0 else 2.0
+    kinds = ["point", "quad", "sphere", ["point", "quad", "sphere"][int(rng.integers(0, 3))]]
+    lights = []
+    for kind in kinds:
+        pos = (float(rng.uniform(-1.0, 1.0)), float(rng.uniform(-1.0, 1.0)), z + float(rng.uniform(-0.3, 0.3)))
+        power = _loguniform(rng, 0.25, 4.0)
+        if kind == "point":
+            lights.append(LightData("P", position=pos, power=power))
+        elif kind == "sphere":
+            lights.append(LightData("P", position=pos, power=power, radius=float(rng.uniform(0.1, 0.4))))
+        else:
+            s = float(rng.uniform(0.5, 2.0))
+            tilt = (float(rng.uniform(-15.0, 15.0)), float(rng.uniform(-15.0, 15.0)), 0.0)
+            lights.append(LightData("A", position=pos, width=s, height=s, power=power, rotation=tilt))
+    cam = look_at_camera(res, res, cam_pos, look, 40.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=0.01, radius_reduction=0.75), photons
+
+
+def c4_batch(n: int = 16, res: int = 2048, photons: int = 64 << 20, samples: int = 1):
+    """The at-scale batch: c4_scene(seed) for seeds 0 .. n-1."""
+    return [c4_scene(s, res, photons, samples) for s in range(n)]

@@ -24,7 +24,7 @@ def _cases():
 
 
 def _alone(args, opts, iters):
-    eng = Engine(args, write_exr=0, **opts)
+    eng = Engine(args, **dict(opts, write_exr=0))
     eng.prepare()
     for _ in range(iters):
         eng.render_iteration()
@@ -71,4 +71,57 @@ def test_batch_rejects_mixed_merge_arithmetic():
     b = Batch([a for a, _ in cases], opts)
     with pytest.raises(Exception, match="exact_merge"):
         b.prepare()
+    b.release()
+
+
+def _guided_cases():
+    return [
+        (scenes.water_scene(32, 24, 16, samples=3), dict(photons=40000, seed=3)),
+        (scenes.c0_scene(32, 32, subdiv=2, samples=3), dict(photons=50000, seed=5, max_path_length=4)),
+        (scenes.slab_scene(20, 20, samples=3), dict(photons=4096, seed=11)),  # no proposal: plain emission
+    ]
+
+
+def test_batch_with_fixed_proposals_matches_scenes_rendered_alone():
+    """Per-scene proposals (fixed networks: same initial parameters, no
+    training) sample the same directions in the batch as alone."""
+    guide = dict(guide=1, guide_train=0, guide_warmup=0, guide_batch=4096)
+    cases = _guided_cases()
+    opts = [dict(o, write_exr=0, **(guide if k < 2 else {})) for k, (_, o) in enumerate(cases)]
+    iters = 2
+    alone = [_alone(a, o, iters) for (a, _), o in zip(cases, opts)]
+    b = Batch([a for a, _ in cases], opts).prepare()
+    for _ in range(iters):
+        b.render_iteration()
+    for k in range(len(cases)):
+        img = b.image(k, scaled=False)
+        assert img.sum() > 0
+        assert np.array_equal(img, alone[k]), k
+    b.release()
+
+
+def test_batch_trains_each_scene_proposal():
+    """Training inside the batch: every guided scene takes its own step on its
+    own first guide_batch paths each iteration; the films stay close to the
+    films of the scenes trained alone (same photons and targets; float atomics
+    in the targets and gradients reorder sums, so not bit for bit)."""
+    guide = dict(guide=1, guide_warmup=1, guide_batch=8192)
+    cases = _guided_cases()
+    opts = [dict(o, write_exr=0, **(guide if k < 2 else {})) for k, (_, o) in enumerate(cases)]
+    iters = 3
+    alone = [_alone(a, o, iters) for (a, _), o in zip(cases, opts)]
+    b = Batch([a for a, _ in cases], opts).prepare()
+    for _ in range(iters):
+        b.render_iteration()
+        st = b.stats()
+        assert st["guide_rows"] == 2 * 8192
+        assert st["ms_guide"] > 0
+    for k in range(len(cases)):
+        img = b.image(k, scaled=False)
+        assert np.all(np.isfinite(img)) and img.sum() > 0
+        rel = abs(img.sum() - alone[k].sum()) / alone[k].sum()
+        print(f"scene {k}: batch {img.sum():.6g} alone {alone[k].sum():.6g} rel {rel:.2e}", flush=True)
+        assert rel < 0.05, k
+        if k == 2:
+            assert np.array_equal(img, alone[k])  # the unguided scene is untouched by its neighbours' training
     b.release()
