@@ -176,13 +176,24 @@ def at_scale(a) -> int:
             ms_guide += st["ms_guide"]
             rows += st["guide_rows"]
         per_it = (time.perf_counter() - t0) / a.scale_steps
+        # one counting iteration after the timed ones (its radius is a step smaller, so its counts are a
+        # slight underestimate of the timed iterations'): algorithmic bytes over the last timed kernel times
+        b.set(0, "count_events", 1)
+        b.render_iteration()
+        cst = b.stats()
         b.release()
+        cst["sort_passes"] = st["sort_passes"]
+        alg = algorithmic_bytes(cst, 64)  # binary nodes of the GPU-built trees
         light = sum(p for _, p in cases)
         camera = sum(sc.camera.width * sc.camera.height for sc, _ in cases)
         rec = {"scenes": [g0, g0 + len(cases) - 1], "triangles": tris, "light_paths": int(light),
                "camera_paths": int(camera), "photons": int(st["photons"]), "ms_per_iteration": round(per_it * 1e3, 1),
                "kernel_ms": {k: round(st[k], 2) for k in ("ms_light", "ms_map", "ms_camera_trace", "ms_gather",
                                                            "ms_resolve")},
+               "gather": {"queries": int(cst["gathers"]), "photons_tested_per_query":
+                          round(cst["photons_tested"] / max(1, cst["gathers"]), 1),
+                          "algorithmic_gb_per_s": round(alg["gather"] / (st["ms_gather"] * 1e-3) / 1e9, 1)},
+               "light_pass_algorithmic_gb_per_s": round(alg["light_pass"] / (st["ms_light"] * 1e-3) / 1e9, 1),
                "ms_guide_per_iteration": round(ms_guide / a.scale_steps, 3), "prepare_s": round(prep_s, 2)}
         print(f"[bench]   {rec}", file=sys.stderr, flush=True)
         groups.append(rec)

@@ -82,6 +82,9 @@ int main(void);
  * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
  * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
  * host tree collapsed to 4-wide nodes, the default; 2 = binary nodes).
+ * Merge order: "query_order" (1 = the merge takes the queries sorted by the
+ * Morton code of their photon-map cell, the default; 0 = in trace order);
+ * the film is the same either way.
  * Read-only: "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
  * "nodes", "nodes4", "node_bytes" (bytes of the node layout traversed).
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
@@ -127,6 +130,8 @@ enum {
      * the network step alone (forward, backward, Adam) and its row count */
     RC_STAT_MS_GUIDE, RC_STAT_GUIDE_VISIBLE, RC_STAT_GUIDE_HIT_FRACTION, RC_STAT_GUIDE_LOSS,
     RC_STAT_MS_GUIDE_STEP, RC_STAT_GUIDE_ROWS,
+    /* sort of the merge queries into spatial order (option "query_order") */
+    RC_STAT_MS_QUERY_SORT,
     RC_STAT_COUNT
 };
 
@@ -182,6 +187,9 @@ int rc_batch_render_scene(int scene); /* one iteration of one scene alone (its o
 int rc_batch_get_image(int scene, float *rgb, int scaled);
 int rc_batch_stats(double *out, int n);  /* RC_STAT_* totals of the last batch iteration */
 int rc_batch_autotune(int repeats, double *report, int n);  /* as rc_autotune, for the whole batch */
+/* rc_set_option for one scene of the batch (e.g. "count_events" on scene 0,
+ * which switches the event counters of the batched launches on). */
+int rc_batch_set_option(int scene, const char *key, double value);
 void rc_batch_release(void);
 
 /* ---- learned emission proposal (no counterpart in the reference) ----

@@ -11,15 +11,6 @@
 
 namespace rc {
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v) {
-    v &= 1023u;
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
-    return v;
-}
-
 __device__ __forceinline__ void tri_box(const float4* tri, uint32_t i, float lo[3], float hi[3]) {
     float4 a = tri[3 * i], b = tri[3 * i + 1], c = tri[3 * i + 2];
     float p0[3] = {a.x, a.y, a.z};

@@ -111,7 +111,10 @@ struct Engine {
     DevBuf<float> contrib;
     uint32_t q_capacity = 0;
     DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
-    hipEvent_t ev[8] = {};  // light, map, camera (trace..resolve), trace end, gather end
+    hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
+    int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
+    DevBuf<unsigned long long> qk_a, qk_b;  // query sort buffers (q_capacity each)
+    DevBuf<uint32_t> qv_a, qv_b, q_hist, q_hist_sums;
     int sort_passes = 0;
     uint32_t capacity = 0;
     int iterations_done = 0;
@@ -936,9 +939,21 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
 
 bool ensure_query_buffers(Engine& e, uint32_t cap, uint32_t n_paths) {
     if (!e.path_info.alloc(n_paths) || !e.contrib.alloc((size_t)n_paths * 8 * 3)) return false;
+    if (e.query_order && !(e.qk_a.alloc(std::max(cap, e.q_capacity)) && e.qk_b.alloc(std::max(cap, e.q_capacity)) &&
+                           e.qv_a.alloc(std::max(cap, e.q_capacity)) && e.qv_b.alloc(std::max(cap, e.q_capacity)) &&
+                           e.q_hist.alloc(photon_map_hist_entries(std::max(cap, e.q_capacity))) &&
+                           e.q_hist_sums.alloc(photon_map_sum_entries(std::max(cap, e.q_capacity)))))
+        return false;
     if (cap <= e.q_capacity && e.q_pos.p) return true;
     e.q_capacity = cap;
     return e.q_pos.alloc(cap) && e.q_dir.alloc(cap) && e.q_sn.alloc(cap) && e.q_gn.alloc(cap) && e.q_tpn.alloc(cap);
+}
+
+// Spatial order of this pass's queries (option query_order), or null.
+const uint32_t* query_order(Engine& e, const CameraPassArgs& a, uint32_t n, hipStream_t s) {
+    if (!e.query_order) return nullptr;
+    QuerySortBuffers qb{e.qk_a.p, e.qk_b.p, e.qv_a.p, e.qv_b.p, e.q_hist.p, e.q_hist_sums.p};
+    return sort_queries(a, qb, std::min(n, e.q_capacity), s);
 }
 
 // Camera pass: trace (queries), merge, resolve into `image`.
@@ -980,6 +995,8 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
         }
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p + 1), 0u, kCounterFields * 2, e.stream);
     }
+    HIP_OK(hipEventRecord(e.ev[8], e.stream));
+    a.q_order = query_order(e, a, e.last_queries, e.stream);
     HIP_OK(hipEventRecord(e.ev[6], e.stream));
     launch_gather(a, gather_blocks(e), e.stream);
     HIP_OK(hipEventRecord(e.ev[7], e.stream));
@@ -1111,6 +1128,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         build_photon_map(m, st);
         launch_fill_u32(e.counters.p, 0u, 4, st);
         launch_camera_trace(a, camera_blocks(e), st);
+        a.q_order = query_order(e, a, e.q_capacity, st);  // count read from the device
         launch_gather(a, gather_blocks(e), st);
         launch_resolve(a, st);
         hipError_t cap = hipStreamEndCapture(st, &graph);
@@ -1163,19 +1181,21 @@ bool render_iteration(Engine& e, int iteration) {
     if (e.count_events) HIP_OK(hipMemcpyAsync(c, e.stats_dev.p, sizeof c, hipMemcpyDeviceToHost, e.stream));
     HIP_OK(hipStreamSynchronize(e.stream));
     float ms[3] = {0, 0, 0};
-    float ms_trace = 0, ms_gather = 0, ms_resolve = 0;
+    float ms_trace = 0, ms_gather = 0, ms_resolve = 0, ms_sort = 0;
     if (graphed) {  // map + camera pass are one replay: its time goes to ms_camera
         HIP_OK(hipEventElapsedTime(&ms[0], e.ev[0], e.ev[1]));
         HIP_OK(hipEventElapsedTime(&ms[2], e.ev[2], e.ev[5]));
     } else {
         for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
-        HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[6]));
+        HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[8]));
+        HIP_OK(hipEventElapsedTime(&ms_sort, e.ev[8], e.ev[6]));
         HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
         HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
     }
     e.stats[RC_STAT_MS_CAMERA_TRACE] = ms_trace;
     e.stats[RC_STAT_MS_GATHER] = ms_gather;
     e.stats[RC_STAT_MS_RESOLVE] = ms_resolve;
+    e.stats[RC_STAT_MS_QUERY_SORT] = ms_sort;
     if (training) {
         float ms_guide = 0;
         HIP_OK(hipEventElapsedTime(&ms_guide, e.ev_guide[0], e.ev_guide[1]));
@@ -1423,6 +1443,8 @@ void release(Engine*& e) {
     for (auto* b : u) b->release();
     e->keys_a.release();
     e->keys_b.release();
+    for (auto* d : {&e->qk_a, &e->qk_b}) d->release();
+    for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
     e->cells.release();
     e->image.release();
     e->stats_dev.release();
@@ -1464,6 +1486,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.graph = from.graph;
     to.bvh_width = from.bvh_width;
     to.bvh_build = from.bvh_build;
+    to.query_order = from.query_order;
     to.output_path = from.output_path;
     const GuideModel& pg = from.guide;  // the proposal's options, not its state
     GuideModel& ng = to.guide;
@@ -1539,6 +1562,8 @@ void release_batch(Batch*& b) {
     for (auto* d : {&w.ph_uid, &w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
     w.keys_a.release();
     w.keys_b.release();
+    for (auto* d : {&w.qk_a, &w.qk_b}) d->release();
+    for (auto* d : {&w.qv_a, &w.qv_b, &w.q_hist, &w.q_hist_sums}) d->release();
     w.cells.release();
     w.path_info.release();
     w.contrib.release();
@@ -1588,6 +1613,7 @@ bool batch_prepare(Batch& b) {
     for (auto& t : pool) t.join();
     if (!ok) return false;
     if (!ensure_device(b.work)) return false;
+    b.work.query_order = b.scenes[0]->query_order;
     if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
@@ -1742,10 +1768,12 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     view.camera_starts = b.d_starts.p + n + 1;
     view.guides = b.d_guides.p;
     const Engine& e0 = *b.scenes[0];
+    const bool counting = e0.count_events != 0;  // event counters of the first scene's option
     // light pass (grow and repeat on overflow)
     uint32_t count = 0;
     for (int attempt = 0;; ++attempt) {
         launch_fill_u32(w.counters.p, 0u, 4, st);
+        if (counting) launch_fill_u32(reinterpret_cast<uint32_t*>(w.stats_dev.p), 0u, 2 * kCounterFields * 2, st);
         LightPassArgs a{};
         a.sc = scene_view(e0);
         a.iteration = (uint32_t)e0.iterations_done;
@@ -1761,6 +1789,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.out_uid = w.ph_uid.p;
         a.guide = plain_guide_view();
         a.batch = view;
+        a.counters = counting ? w.stats_dev.p : nullptr;
         HIP_OK(hipEventRecord(w.ev[0], st));
         launch_light_pass(a, light_blocks(w), st);
         HIP_OK(hipEventRecord(w.ev[1], st));
@@ -1812,10 +1841,13 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.contrib = w.contrib.p;
     c.exact_merge = e0.exact_merge;
     c.batch = view;
+    c.counters = counting ? w.stats_dev.p + 1 : nullptr;
     uint32_t nq = 0;
     for (int attempt = 0;; ++attempt) {
         c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
         launch_fill_u32(w.counters.p, 0u, 4, st);
+        if (counting && attempt > 0)
+            launch_fill_u32(reinterpret_cast<uint32_t*>(w.stats_dev.p + 1), 0u, kCounterFields * 2, st);
         HIP_OK(hipEventRecord(w.ev[4], st));
         launch_camera_trace(c, camera_blocks(w), st);
         HIP_OK(hipGetLastError());
@@ -1828,6 +1860,8 @@ bool batch_iteration(Batch& b, bool scratch = false) {
             return false;
         }
     }
+    HIP_OK(hipEventRecord(w.ev[8], st));
+    c.q_order = query_order(w, c, nq, st);
     HIP_OK(hipEventRecord(w.ev[6], st));
     launch_gather(c, gather_blocks(w), st);
     HIP_OK(hipEventRecord(w.ev[7], st));
@@ -1835,12 +1869,15 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     HIP_OK(hipEventRecord(w.ev[5], st));
     HIP_OK(hipGetLastError());
     if (any_train && !batch_guide_training(b, view, count, nq, targets, st)) return false;
+    Counters cnt[2] = {};
+    if (counting) HIP_OK(hipMemcpyAsync(cnt, w.stats_dev.p, sizeof cnt, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     float ms[8] = {0};
     HIP_OK(hipEventElapsedTime(&ms[0], w.ev[0], w.ev[1]));
     HIP_OK(hipEventElapsedTime(&ms[1], w.ev[2], w.ev[3]));
     HIP_OK(hipEventElapsedTime(&ms[2], w.ev[4], w.ev[5]));
-    HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[6]));
+    HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[8]));
+    HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
     HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
     HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
     std::fill(b.stats, b.stats + RC_STAT_COUNT, 0.0);
@@ -1854,7 +1891,19 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     b.stats[RC_STAT_MS_CAMERA_TRACE] = ms[3];
     b.stats[RC_STAT_MS_GATHER] = ms[4];
     b.stats[RC_STAT_MS_RESOLVE] = ms[5];
+    b.stats[RC_STAT_MS_QUERY_SORT] = ms[6];
     b.stats[RC_STAT_SORT_PASSES] = (bucket_bits + uid_bits + 7) / 8;
+    if (counting) {
+        b.stats[RC_STAT_LIGHT_NODE_VISITS] = (double)cnt[0].nodes;
+        b.stats[RC_STAT_LIGHT_TRI_TESTS] = (double)cnt[0].tris;
+        b.stats[RC_STAT_LIGHT_HITS] = (double)cnt[0].hits;
+        b.stats[RC_STAT_CAMERA_NODE_VISITS] = (double)cnt[1].nodes;
+        b.stats[RC_STAT_CAMERA_TRI_TESTS] = (double)cnt[1].tris;
+        b.stats[RC_STAT_CAMERA_HITS] = (double)cnt[1].hits;
+        b.stats[RC_STAT_CELL_VISITS] = (double)cnt[1].cells;
+        b.stats[RC_STAT_PHOTONS_TESTED] = (double)cnt[1].photons_tested;
+        b.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)cnt[1].photons_accepted;
+    }
     if (any_train) {
         b.stats[RC_STAT_MS_GUIDE] = b.guide_ms;
         b.stats[RC_STAT_MS_GUIDE_STEP] = b.guide_step_ms;
@@ -2090,6 +2139,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "graph") g->graph = (int)v;
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
+    else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -2138,6 +2188,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "bvh_width") return g->bvh_width;
     if (k == "node_bytes") return uses_nodes4(*g) ? 128 : 64;
     if (k == "bvh_build") return g->bvh_build;
+    if (k == "query_order") return g->query_order;
     if (k == "bvh_depth") return g->bvh_depth;
     if (k == "bvh_build_ms") return g->bvh_build_ms;
     if (k == "nodes4") return g->n_nodes4;
@@ -2421,6 +2472,18 @@ __attribute__((visibility("default"))) int rc_batch_add(void) {
     g_batch->scenes.push_back(g);
     g = nullptr;
     return (int)g_batch->scenes.size() - 1;
+}
+
+__attribute__((visibility("default"))) int rc_batch_set_option(int scene, const char* key, double value) {
+    if (!g_batch || scene < 0 || scene >= (int)g_batch->scenes.size()) {
+        set_error("rc_batch_set_option: no such scene");
+        return -1;
+    }
+    Engine* saved = g;
+    g = g_batch->scenes[scene];
+    int rc = rc_set_option(key, value);
+    g = saved;
+    return rc;
 }
 
 __attribute__((visibility("default"))) int rc_batch_size(void) { return g_batch ? (int)g_batch->scenes.size() : 0; }

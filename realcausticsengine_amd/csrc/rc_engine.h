@@ -175,6 +175,7 @@ struct CameraPassArgs {
     int exact_merge;     // 1: correctly rounded division/sqrt in the merge
     BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
     const IterParams* dyn;  // non-null: iteration, vm_norm and the grid of pm come from here
+    const uint32_t* q_order;  // non-null: the gather takes the queries in this order (spatially sorted)
 };
 
 // ---- launchers (rc_kernels.hip) ----
@@ -182,6 +183,19 @@ void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s);
 void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_resolve(const CameraPassArgs& a, hipStream_t s);
+// Spatial order of the merge queries: key = scene << 32 | Morton code of the
+// query's photon-map cell (10 bits per axis), one stable radix sort; the query
+// count comes from a.path_counter[1] (at most n).  Returns the order array
+// (one of vals_a / vals_b).
+struct QuerySortBuffers {
+    unsigned long long* keys_a;
+    unsigned long long* keys_b;
+    uint32_t* vals_a;
+    uint32_t* vals_b;
+    uint32_t* hist;
+    uint32_t* hist_sums;
+};
+const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s);
 void launch_first_hit(const SceneView& sc, const float* rays, int n, float tmin, int* tri, float* t, hipStream_t s);
 // Debug AOV of debug_test mode 2, 3 or 4 added into image (W * H * 3).
 void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hipStream_t s);

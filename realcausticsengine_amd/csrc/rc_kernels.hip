@@ -450,7 +450,8 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     const uint32_t seg = (nq + kXcds - 1) / kXcds;
     const uint32_t q_begin = min(nq, xcd * seg), q_end = min(nq, q_begin + seg);
     const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
-    for (uint32_t qi = q_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; qi < q_end; qi += local_waves) {
+    for (uint32_t qj = q_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; qj < q_end; qj += local_waves) {
+        const uint32_t qi = a.q_order ? a.q_order[qj] : qj;
         float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
         // the query's scene is the same in every lane
         const int scene = kBatch ? __builtin_amdgcn_readfirstlane(__float_as_int(qg.w)) : 0;
@@ -972,6 +973,56 @@ void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, ui
     }
     *keys_out = kin;
     *vals_out = vin;
+}
+
+__global__ void k_query_keys(CameraPassArgs a, uint32_t n, unsigned long long* keys, uint32_t* vals) {
+    n = live_count(a.path_counter + 1, min(n, a.q.capacity));
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 p = a.q.pos[i];
+    int scene = 0;
+    float o[3], inv;
+    if (a.batch.n > 0) {
+        scene = __float_as_int(a.q.gn[i].w);
+        const PhotonMapView& pm = a.batch.pm[scene];
+        o[0] = pm.origin[0];
+        o[1] = pm.origin[1];
+        o[2] = pm.origin[2];
+        inv = pm.inv_cell;
+    } else if (a.dyn) {
+        o[0] = a.dyn->origin[0];
+        o[1] = a.dyn->origin[1];
+        o[2] = a.dyn->origin[2];
+        inv = a.dyn->inv_cell;
+    } else {
+        o[0] = a.pm.origin[0];
+        o[1] = a.pm.origin[1];
+        o[2] = a.pm.origin[2];
+        inv = a.pm.inv_cell;
+    }
+    // cell coordinates modulo 1024: neighbouring cells keep neighbouring codes
+    uint32_t cx = (uint32_t)(int)floorf((p.x - o[0]) * inv);
+    uint32_t cy = (uint32_t)(int)floorf((p.y - o[1]) * inv);
+    uint32_t cz = (uint32_t)(int)floorf((p.z - o[2]) * inv);
+    uint32_t code = (spread10(cx) << 2) | (spread10(cy) << 1) | spread10(cz);
+    keys[i] = ((unsigned long long)(uint32_t)scene << 32) | code;
+    vals[i] = i;
+}
+
+const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s) {
+    if (n == 0) return nullptr;
+    int bits = 30;
+    if (a.batch.n > 1) {
+        int sb = 0;
+        while ((1 << sb) < a.batch.n) ++sb;
+        bits = 32 + sb;
+    }
+    hipLaunchKernelGGL(k_query_keys, dim3((n + 255) / 256), dim3(256), 0, s, a, n, b.keys_a, b.vals_a);
+    unsigned long long* kout;
+    uint32_t* vout;
+    radix_sort_pairs(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, b.hist_sums, a.path_counter + 1, n, bits, &kout,
+                     &vout, s);
+    return vout;
 }
 
 static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {

@@ -15,6 +15,16 @@ struct F3 {
     float x, y, z;
 };
 
+// Low 10 bits of v spread to every third bit (3-D Morton codes).
+RC_HD uint32_t spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
 RC_HD F3 f3(float x, float y, float z) { return F3{x, y, z}; }
 RC_HD F3 f3s(float a) { return F3{a, a, a}; }
 RC_HD F3 operator+(F3 a, F3 b) { return F3{a.x + b.x, a.y + b.y, a.z + b.z}; }

@@ -25,7 +25,7 @@ EXPORTS = [
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
-    "rc_batch_autotune", "rc_render_debug",
+    "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu"]
@@ -35,7 +35,8 @@ STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_test
               "ms_light", "ms_map", "ms_camera", "ms_total", "radius", "nodes", "tris",
               "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
               "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve",
-              "ms_guide", "guide_visible", "guide_hit_fraction", "guide_loss", "ms_guide_step", "guide_rows"]
+              "ms_guide", "guide_visible", "guide_hit_fraction", "guide_loss", "ms_guide_step", "guide_rows",
+              "ms_query_sort"]
 # Matrix-core FLOPs of one training step per row: forward 2*(192*64 + 2*64*64 + 64*1024), data
 # gradients of the three upper layers 2*(1024*64 + 2*64*64), weight gradients 2*(64*1024 + 2*64*64 + 192*64).
 GUIDE_STEP_FLOPS_PER_ROW = 2 * (192 * 64 + 2 * 64 * 64 + 64 * 1024) + 2 * (1024 * 64 + 2 * 64 * 64) + \
@@ -104,6 +105,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     for fn in (lib.rc_batch_add, lib.rc_batch_size, lib.rc_batch_prepare, lib.rc_batch_render_iteration):
         fn.argtypes = []
         fn.restype = ctypes.c_int
+    lib.rc_batch_set_option.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_double]
+    lib.rc_batch_set_option.restype = ctypes.c_int
     lib.rc_batch_render_scene.argtypes = [ctypes.c_int]
     lib.rc_batch_render_scene.restype = ctypes.c_int
     lib.rc_batch_get_image.argtypes = [ctypes.c_int, FP, ctypes.c_int]
@@ -309,6 +312,10 @@ class Batch:
 
     def render_iteration(self) -> int:
         return self._check(self.lib.rc_batch_render_iteration(), "rc_batch_render_iteration")
+
+    def set(self, scene: int, key: str, value) -> None:
+        if self.lib.rc_batch_set_option(scene, key.encode(), float(value)) != 0:
+            raise EngineError(f"rc_batch_set_option({scene}, {key}): {self.lib.rc_last_error().decode() or 'unknown key'}")
 
     def render_scene(self, scene: int) -> int:
         """One iteration of one scene with its own launches (the one-at-a-time baseline)."""
