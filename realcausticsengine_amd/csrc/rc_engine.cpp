@@ -113,6 +113,7 @@ struct Engine {
     DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
     hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
+    int group_queries = 1;  // 1: queries with the same cell window share photon loads (grouped merge)
     DevBuf<unsigned long long> qk_a, qk_b;  // query sort buffers (q_capacity each)
     DevBuf<uint32_t> qv_a, qv_b, q_hist, q_hist_sums;
     int sort_passes = 0;
@@ -977,6 +978,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.image = image;
     a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
     a.exact_merge = e.exact_merge;
+    a.group_queries = e.group_queries;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
@@ -1082,7 +1084,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)e.cells.p, buckets, (uintptr_t)uid_bits, (uintptr_t)e.image.p,
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
-                                  (uintptr_t)e.exact_merge};
+                                  (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         e.graph_exec = nullptr;
@@ -1121,6 +1123,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.contrib = e.contrib.p;
         a.image = e.image.p;
         a.exact_merge = e.exact_merge;
+        a.group_queries = e.group_queries;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         hipGraph_t graph = nullptr;
@@ -1487,6 +1490,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.bvh_width = from.bvh_width;
     to.bvh_build = from.bvh_build;
     to.query_order = from.query_order;
+    to.group_queries = from.group_queries;
     to.output_path = from.output_path;
     const GuideModel& pg = from.guide;  // the proposal's options, not its state
     GuideModel& ng = to.guide;
@@ -1840,6 +1844,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.path_info = w.path_info.p;
     c.contrib = w.contrib.p;
     c.exact_merge = e0.exact_merge;
+    c.group_queries = e0.group_queries;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
     uint32_t nq = 0;
@@ -2140,6 +2145,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
+    else if (k == "group_queries") g->group_queries = (int)v ? 1 : 0;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -2189,6 +2195,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "node_bytes") return uses_nodes4(*g) ? 128 : 64;
     if (k == "bvh_build") return g->bvh_build;
     if (k == "query_order") return g->query_order;
+    if (k == "group_queries") return g->group_queries;
     if (k == "bvh_depth") return g->bvh_depth;
     if (k == "bvh_build_ms") return g->bvh_build_ms;
     if (k == "nodes4") return g->n_nodes4;

@@ -176,6 +176,7 @@ struct CameraPassArgs {
     BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
     const IterParams* dyn;  // non-null: iteration, vm_norm and the grid of pm come from here
     const uint32_t* q_order;  // non-null: the gather takes the queries in this order (spatially sorted)
+    int group_queries;        // 1: grouped merge (queries with the same cell window share photon loads)
 };
 
 // ---- launchers (rc_kernels.hip) ----
@@ -183,8 +184,9 @@ void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s);
 void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_gather(const CameraPassArgs& a, int blocks, hipStream_t s);
 void launch_resolve(const CameraPassArgs& a, hipStream_t s);
-// Spatial order of the merge queries: key = scene << 32 | Morton code of the
-// query's photon-map cell (10 bits per axis), one stable radix sort; the query
+// Spatial order of the merge queries: key = scene << 36 | Morton code of the
+// low corner of the query's photon-map cell window (10 bits per axis) << 6 |
+// window extents, one stable radix sort; the query
 // count comes from a.path_counter[1] (at most n).  Returns the order array
 // (one of vals_a / vals_b).
 struct QuerySortBuffers {

@@ -573,6 +573,209 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     if (kCount) flush_counters(a.counters, cnt);
 }
 
+// ---------------------------------------------------------------------
+// Grouped merge: a wave takes kGroupQ consecutive queries of the spatial
+// order.  Queries whose candidate cells are the same window (sub-runs of the
+// group) share one pass over the window's photons: each lane loads one
+// photon (position, direction, throughput) per step and tests it against
+// every query of the sub-run, evaluating the accepted ones.  A photon is read
+// once per sub-run instead of once per query.  Per-query state (BSDF frame,
+// merge constants) sits in LDS and is read as broadcasts; the per-lane
+// partial sums of the sub-run's queries are registers (the query loop is
+// unrolled), reduced with the fixed butterfly when the sub-run ends.  A
+// query's sum depends only on its window's photon stream (photon s goes to
+// lane s mod 64), not on which queries share its wave; the terms are the
+// per-query kernel's, added in another order.
+// ---------------------------------------------------------------------
+constexpr int kGroupQ = 8;
+
+struct GroupQuery {
+    Bsdf b;
+    MergeConst mc;
+    F3 q;
+    float cdvm;
+};
+
+// The cell window of a query: cells [lo, hi] per axis overlapping its ball.
+__device__ __forceinline__ void query_window(const PhotonMapView& pm, F3 q, int lo[3], int hi[3]) {
+    float qq[3] = {q.x, q.y, q.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        lo[k] = (int)floorf(((qq[k] - pm.r_query) - pm.origin[k]) * pm.inv_cell);
+        hi[k] = (int)floorf(((qq[k] + pm.r_query) - pm.origin[k]) * pm.inv_cell);
+    }
+}
+
+template <bool kCount, bool kFast, bool kBatch>
+__global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs a) {
+    __shared__ uint32_t rfirst_lds[kGatherWaves][64];
+    __shared__ uint32_t rbase_lds[kGatherWaves][64];
+    __shared__ GroupQuery gq_lds[kGatherWaves][kGroupQ];
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t lane = lane_id();
+    unsigned long long cnt[kCounterFields] = {0};
+    PhotonMapView pm0 = a.pm;
+    if (a.dyn) {
+        pm0.r2 = a.dyn->r2;
+        pm0.r_query = a.dyn->r_query;
+        pm0.inv_cell = a.dyn->inv_cell;
+        pm0.origin[0] = a.dyn->origin[0];
+        pm0.origin[1] = a.dyn->origin[1];
+        pm0.origin[2] = a.dyn->origin[2];
+    }
+    const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
+    const uint32_t nblk = (nq + kGroupQ - 1) / kGroupQ;
+    const uint32_t xcd = blockIdx.x % kXcds;
+    const uint32_t seg = (nblk + kXcds - 1) / kXcds;
+    const uint32_t b_begin = min(nblk, xcd * seg), b_end = min(nblk, b_begin + seg);
+    const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
+    for (uint32_t bj = b_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; bj < b_end; bj += local_waves) {
+        const uint32_t j0 = bj * kGroupQ;
+        const uint32_t nj = min((uint32_t)kGroupQ, nq - j0);
+        // lane j < nj sets up query j of the group
+        const bool mine = lane < nj;
+        uint32_t qi = 0;
+        int scene = -1;
+        int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
+        if (mine) {
+            qi = a.q_order ? a.q_order[j0 + lane] : j0 + lane;
+            float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
+            scene = kBatch ? __float_as_int(qg.w) : 0;
+            const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : pm0;
+            const DevMaterial* mats = kBatch ? a.batch.sc[scene].mats : a.sc.mats;
+            GroupQuery g;
+            g.q = f3(qp.x, qp.y, qp.z);
+            g.cdvm = qp.w;
+            bsdf_setup(g.b, mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
+            g.mc = merge_const(g.b, f3(qg.x, qg.y, qg.z));
+            gq_lds[wv][lane] = g;
+            query_window(pm, g.q, lo, hi);
+        }
+        // sub-runs: query j starts one when its window (or scene) differs from query j - 1's
+        bool same = lane > 0;
+        {
+            int p;
+            p = __shfl_up(scene, 1);
+            same = same && p == scene;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                p = __shfl_up(lo[k], 1);
+                same = same && p == lo[k];
+                p = __shfl_up(hi[k], 1);
+                same = same && p == hi[k];
+            }
+        }
+        uint64_t starts = __ballot(mine && !same);
+        __builtin_amdgcn_wave_barrier();
+        while (starts) {
+            const uint32_t s0 = (uint32_t)__builtin_ctzll(starts);
+            starts &= starts - 1;
+            const uint32_t e0 = starts ? (uint32_t)__builtin_ctzll(starts) : nj;
+            const uint32_t nrun = e0 - s0;
+            const int sc = __builtin_amdgcn_readlane(scene, (int)s0);
+            const PhotonMapView& pm = kBatch ? a.batch.pm[sc] : pm0;
+            int wlo[3], whi[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                wlo[k] = __builtin_amdgcn_readlane(lo[k], (int)s0);
+                whi[k] = __builtin_amdgcn_readlane(hi[k], (int)s0);
+            }
+            // lane l takes cell (l & 3, l >> 2 & 3, l >> 4) of the window, as the per-query merge
+            int cx = wlo[0] + (int)(lane & 3u), cy = wlo[1] + (int)((lane >> 2) & 3u), cz = wlo[2] + (int)(lane >> 4);
+            bool valid = cx <= whi[0] && cy <= whi[1] && cz <= whi[2];
+            uint2 range = valid ? pm.cells[cell_bucket(cx, cy, cz, pm.mask)] : make_uint2(0u, 0u);
+            uint32_t len = range.y - range.x;
+            uint64_t ne = __ballot(len > 0u);
+            const uint32_t nr = (uint32_t)__builtin_popcountll(ne);
+            uint32_t incl = wave_incl_scan(len);
+            const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+            if (len > 0u) {
+                uint32_t r = lanes_below(ne);
+                rfirst_lds[wv][r] = range.x;
+                rbase_lds[wv][r] = incl - len;
+            }
+            __builtin_amdgcn_wave_barrier();
+            const uint32_t r_first = rfirst_lds[wv][lane];
+            const uint32_t r_base = rbase_lds[wv][lane];
+            __builtin_amdgcn_wave_barrier();
+            if (kCount) {
+                const uint64_t lookups = __ballot(valid);
+                if (lane == 0) {
+                    cnt[C_CELLS] += (uint32_t)__builtin_popcountll(lookups);
+                    cnt[C_TESTED] += (unsigned long long)total * nrun;
+                }
+            }
+            F3 c[kGroupQ];
+#pragma unroll
+            for (int jj = 0; jj < kGroupQ; ++jj) c[jj] = f3s(0.0f);
+            uint32_t next_r = 0, cur_first = 0, cur_base = 0;
+            for (uint32_t sb = 0; sb < total; sb += 64) {
+                const uint32_t sp = sb + lane;
+                for (uint32_t r = next_r; r < nr; ++r) {
+                    uint32_t bnd = (uint32_t)__builtin_amdgcn_readlane((int)r_base, (int)r);
+                    if (bnd >= sb + 64u) break;
+                    uint32_t fst = (uint32_t)__builtin_amdgcn_readlane((int)r_first, (int)r);
+                    if (sp >= bnd) {
+                        cur_first = fst;
+                        cur_base = bnd;
+                    }
+                    if (bnd <= sb) next_r = r + 1;
+                }
+                const bool v = sp < total;
+                const uint32_t i = cur_first + (sp - cur_base);
+                float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dt = p;
+                float2 gb = make_float2(0.0f, 0.0f);
+                if (v) {
+                    p = pm.pos[i];
+                    dt = pm.dir_tr[i];
+                    gb = pm.tp_gb[i];
+                }
+#pragma unroll
+                for (int jj = 0; jj < kGroupQ; ++jj) {
+                    if ((uint32_t)jj < nrun) {
+                        const GroupQuery& g = gq_lds[wv][s0 + jj];
+                        const bool acc = v && len2(g.q - f3(p.x, p.y, p.z)) < pm.r2;
+                        if (kCount) {
+                            const uint64_t am = __ballot(acc);
+                            if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                        }
+                        if (acc) {
+                            float fwd, rev;
+                            F3 e = bsdf_eval_pdf<kFast>(g.b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
+                            float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + p.w * fwd);
+                            c[jj] = c[jj] + (e * f3(dt.w, gb.x, gb.y)) * w;
+                        }
+                    }
+                }
+            }
+            // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
+#pragma unroll
+            for (int jj = 0; jj < kGroupQ; ++jj) {
+                if ((uint32_t)jj < nrun) {
+                    F3 r = c[jj];
+                    for (int off = 32; off > 0; off >>= 1) {
+                        r.x += __shfl_xor(r.x, off);
+                        r.y += __shfl_xor(r.y, off);
+                        r.z += __shfl_xor(r.z, off);
+                    }
+                    const uint32_t qj = (uint32_t)__builtin_amdgcn_readlane((int)qi, (int)(s0 + jj));
+                    if (lane == 0) {
+                        float4 t = a.q.tpn[qj];
+                        F3 o = f3(t.x, t.y, t.z) * r;
+                        uint32_t slot = __float_as_uint(a.q.sn[qj].w);
+                        a.contrib[3 * (size_t)slot] = o.x;
+                        a.contrib[3 * (size_t)slot + 1] = o.y;
+                        a.contrib[3 * (size_t)slot + 2] = o.z;
+                        if (kCount) cnt[C_GATHERS]++;
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    if (kCount) flush_counters(a.counters, cnt);
+}
+
 // Camera pass, part 3: color = sum of the path's vertex contributions in
 // vertex order, added to the film (Vertex_Merging.cpp:347, :361).
 __global__ void k_resolve(CameraPassArgs a) {
@@ -608,6 +811,16 @@ void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
 template <bool kBatch>
 static void gather_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
     dim3 g(blocks), b(kGatherBlock);
+    if (a.group_queries) {
+        if (a.exact_merge) {
+            if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, false, kBatch>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_gather_grouped<false, false, kBatch>), g, b, 0, s, a);
+        } else {
+            if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, true, kBatch>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_gather_grouped<false, true, kBatch>), g, b, 0, s, a);
+        }
+        return;
+    }
     if (a.exact_merge) {
         if (a.counters) hipLaunchKernelGGL((k_gather<true, false, kBatch>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_gather<false, false, kBatch>), g, b, 0, s, a);
@@ -981,41 +1194,36 @@ __global__ void k_query_keys(CameraPassArgs a, uint32_t n, unsigned long long* k
     if (i >= n) return;
     float4 p = a.q.pos[i];
     int scene = 0;
-    float o[3], inv;
+    PhotonMapView pm = a.pm;
     if (a.batch.n > 0) {
         scene = __float_as_int(a.q.gn[i].w);
-        const PhotonMapView& pm = a.batch.pm[scene];
-        o[0] = pm.origin[0];
-        o[1] = pm.origin[1];
-        o[2] = pm.origin[2];
-        inv = pm.inv_cell;
+        pm = a.batch.pm[scene];
     } else if (a.dyn) {
-        o[0] = a.dyn->origin[0];
-        o[1] = a.dyn->origin[1];
-        o[2] = a.dyn->origin[2];
-        inv = a.dyn->inv_cell;
-    } else {
-        o[0] = a.pm.origin[0];
-        o[1] = a.pm.origin[1];
-        o[2] = a.pm.origin[2];
-        inv = a.pm.inv_cell;
+        pm.r_query = a.dyn->r_query;
+        pm.inv_cell = a.dyn->inv_cell;
+        pm.origin[0] = a.dyn->origin[0];
+        pm.origin[1] = a.dyn->origin[1];
+        pm.origin[2] = a.dyn->origin[2];
     }
-    // cell coordinates modulo 1024: neighbouring cells keep neighbouring codes
-    uint32_t cx = (uint32_t)(int)floorf((p.x - o[0]) * inv);
-    uint32_t cy = (uint32_t)(int)floorf((p.y - o[1]) * inv);
-    uint32_t cz = (uint32_t)(int)floorf((p.z - o[2]) * inv);
-    uint32_t code = (spread10(cx) << 2) | (spread10(cy) << 1) | spread10(cz);
-    keys[i] = ((unsigned long long)(uint32_t)scene << 32) | code;
+    // the query's cell window: Morton code of its low corner (cell coordinates
+    // modulo 1024, so neighbouring windows keep neighbouring codes) above the
+    // window extents, so queries with the same window are adjacent
+    int lo[3], hi[3];
+    query_window(pm, f3(p.x, p.y, p.z), lo, hi);
+    uint32_t code = (spread10((uint32_t)lo[0]) << 2) | (spread10((uint32_t)lo[1]) << 1) | spread10((uint32_t)lo[2]);
+    uint32_t ext = (uint32_t)((hi[0] - lo[0]) & 3) | (uint32_t)((hi[1] - lo[1]) & 3) << 2 |
+                   (uint32_t)((hi[2] - lo[2]) & 3) << 4;
+    keys[i] = ((unsigned long long)(uint32_t)scene << 36) | ((unsigned long long)code << 6) | ext;
     vals[i] = i;
 }
 
 const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s) {
     if (n == 0) return nullptr;
-    int bits = 30;
+    int bits = 36;
     if (a.batch.n > 1) {
         int sb = 0;
         while ((1 << sb) < a.batch.n) ++sb;
-        bits = 32 + sb;
+        bits = 36 + sb;
     }
     hipLaunchKernelGGL(k_query_keys, dim3((n + 255) / 256), dim3(256), 0, s, a, n, b.keys_a, b.vals_a);
     unsigned long long* kout;

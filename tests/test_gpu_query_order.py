@@ -1,10 +1,13 @@
 """The merge in spatial query order (option query_order = 1, the default)
-against the merge in trace order.
+against the merge in trace order, and the grouped merge (group_queries = 1,
+the default) against the per-query merge.
 
-Each query's sum is formed by one wave in the same photon order whichever
-query comes first, and the film adds every query's contribution through its
-path slot, so the films must be bit-identical; only the L2 reuse between
-neighbouring queries changes."""
+In either merge a query's sum is formed by one wave in an order fixed by the
+query's cell window alone (whichever queries share its wave), and the film
+adds every query's contribution through its path slot, so the films of the
+two query orders must be bit-identical; only the L2 reuse between
+neighbouring queries changes.  The grouped and per-query merges add the same
+terms in different orders: films within float rounding of each other."""
 import numpy as np
 import pytest
 
@@ -14,9 +17,9 @@ from realcausticsengine_amd.engine import Batch, Engine
 pytestmark = pytest.mark.gpu
 
 
-def _films(order: int, graph: int = 0, iters: int = 2):
+def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1):
     args = scenes.water_scene(128, 96, 80, samples=iters)
-    eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph)
+    eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group)
     eng.prepare()
     films = []
     for _ in range(iters):
@@ -27,9 +30,10 @@ def _films(order: int, graph: int = 0, iters: int = 2):
     return films, st
 
 
-def test_sorted_merge_matches_trace_order():
-    plain, st0 = _films(0)
-    ordered, st1 = _films(1)
+@pytest.mark.parametrize("group", [0, 1])
+def test_sorted_merge_matches_trace_order(group):
+    plain, st0 = _films(0, group=group)
+    ordered, st1 = _films(1, group=group)
     assert st1["ms_query_sort"] > st0["ms_query_sort"]  # order 0: two events back to back
     for a, b in zip(plain, ordered):
         assert a.sum() > 0
@@ -41,6 +45,29 @@ def test_sorted_merge_in_graph_matches_trace_order():
     graphed, _ = _films(1, graph=1)
     for a, b in zip(plain, graphed):
         assert np.array_equal(a, b)
+
+
+def test_grouped_merge_matches_per_query_merge():
+    per_query, s0 = _films(1, group=0)
+    grouped, s1 = _films(1, group=1)
+    for a, b in zip(per_query, grouped):
+        assert a.sum() > 0
+        np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6 * float(a.max()))
+    assert s0["gathers"] == s1["gathers"]
+
+
+def test_grouped_merge_counts_match_per_query_merge():
+    args = scenes.water_scene(128, 96, 80, samples=1)
+    st = []
+    for group in (0, 1):
+        eng = Engine(args, photons=200000, seed=4, write_exr=0, count_events=1, group_queries=group)
+        eng.prepare()
+        eng.render_iteration()
+        st.append(eng.stats())
+        eng.release()
+    for k in ("gathers", "photons_tested", "photons_accepted"):
+        assert st[0][k] == st[1][k], k
+    assert st[1]["cell_visits"] <= st[0]["cell_visits"]  # one window lookup per sub-run of a group
 
 
 def test_sorted_merge_in_batch_matches_trace_order():
