@@ -83,8 +83,11 @@ int main(void);
  * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
  * host tree collapsed to 4-wide nodes, the default; 2 = binary nodes).
  * Merge order: "query_order" (1 = the merge takes the queries sorted by the
- * Morton code of their photon-map cell, the default; 0 = in trace order);
- * the film is the same either way.
+ * Morton code of their photon-map cell window, the default; 0 = in trace
+ * order); the film is the same either way.  "group_queries" (8 = the default):
+ * queries per wave of the grouped merge, in which queries with the same cell
+ * window share the photon loads (same terms as one query per wave, summed in
+ * another order).
  * Read-only: "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
  * "nodes", "nodes4", "node_bytes" (bytes of the node layout traversed).
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
@@ -138,14 +141,15 @@ enum {
 /* Launch shapes and queue / tile sizes are options too: "path_chunk" (path
  * indices a wave takes per queue refill, a multiple of 64), "light_blocks_per_cu",
  * "camera_blocks_per_cu", "gather_blocks_per_cu" (persistent workgroups per
- * compute unit) and "cell_scale" (photon-map cell size over the merge radius).
+ * compute unit), "cell_scale" (photon-map cell size over the merge radius) and
+ * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects.  report (up to RC_TUNE_REPORT doubles): kernel ms per
  * iteration before and after, then path_chunk, light_blocks_per_cu,
- * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu. */
-enum { RC_TUNE_REPORT = 7 };
+ * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu, group_queries. */
+enum { RC_TUNE_REPORT = 8 };
 /* Option "graph" = 1: after the light pass (a direct launch whose photon
  * count is checked on the host), the photon-map build and the camera pass of
  * each iteration replay one captured hipGraph, reading counts and

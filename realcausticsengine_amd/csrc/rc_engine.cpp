@@ -113,7 +113,7 @@ struct Engine {
     DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
     hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
-    int group_queries = 1;  // 1: queries with the same cell window share photon loads (grouped merge)
+    int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
     DevBuf<unsigned long long> qk_a, qk_b;  // query sort buffers (q_capacity each)
     DevBuf<uint32_t> qv_a, qv_b, q_hist, q_hist_sums;
     int sort_passes = 0;
@@ -1275,6 +1275,9 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"gather_blocks_per_cu", {8, 12, 16, 24, 32},
          [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
          [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
+        {"group_queries", {4, 8, 16},
+         [=](double v) { launch_owner().group_queries = (int)v; },
+         [=]() { return (double)launch_owner().group_queries; }, G},
     };
 }
 
@@ -1618,6 +1621,7 @@ bool batch_prepare(Batch& b) {
     if (!ok) return false;
     if (!ensure_device(b.work)) return false;
     b.work.query_order = b.scenes[0]->query_order;
+    b.work.group_queries = b.scenes[0]->group_queries;
     if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
@@ -1844,7 +1848,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.path_info = w.path_info.p;
     c.contrib = w.contrib.p;
     c.exact_merge = e0.exact_merge;
-    c.group_queries = e0.group_queries;
+    c.group_queries = w.group_queries;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
     uint32_t nq = 0;
@@ -2145,7 +2149,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
-    else if (k == "group_queries") g->group_queries = (int)v ? 1 : 0;
+    else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));

@@ -176,7 +176,8 @@ struct CameraPassArgs {
     BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
     const IterParams* dyn;  // non-null: iteration, vm_norm and the grid of pm come from here
     const uint32_t* q_order;  // non-null: the gather takes the queries in this order (spatially sorted)
-    int group_queries;        // 1: grouped merge (queries with the same cell window share photon loads)
+    int group_queries;        // 4, 8 or 16: grouped merge of that many queries per wave (queries with the
+                              // same cell window share photon loads); 0: one query per wave
 };
 
 // ---- launchers (rc_kernels.hip) ----

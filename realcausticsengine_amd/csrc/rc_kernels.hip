@@ -574,7 +574,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
 }
 
 // ---------------------------------------------------------------------
-// Grouped merge: a wave takes kGroupQ consecutive queries of the spatial
+// Grouped merge: a wave takes kQ (4, 8 or 16) consecutive queries of the spatial
 // order.  Queries whose candidate cells are the same window (sub-runs of the
 // group) share one pass over the window's photons: each lane loads one
 // photon (position, direction, throughput) per step and tests it against
@@ -587,8 +587,6 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
 // lane s mod 64), not on which queries share its wave; the terms are the
 // per-query kernel's, added in another order.
 // ---------------------------------------------------------------------
-constexpr int kGroupQ = 8;
-
 struct GroupQuery {
     Bsdf b;
     MergeConst mc;
@@ -606,11 +604,11 @@ __device__ __forceinline__ void query_window(const PhotonMapView& pm, F3 q, int 
     }
 }
 
-template <bool kCount, bool kFast, bool kBatch>
+template <bool kCount, bool kFast, bool kBatch, int kQ>
 __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs a) {
     __shared__ uint32_t rfirst_lds[kGatherWaves][64];
     __shared__ uint32_t rbase_lds[kGatherWaves][64];
-    __shared__ GroupQuery gq_lds[kGatherWaves][kGroupQ];
+    __shared__ GroupQuery gq_lds[kGatherWaves][kQ];
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     unsigned long long cnt[kCounterFields] = {0};
@@ -624,14 +622,14 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
         pm0.origin[2] = a.dyn->origin[2];
     }
     const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
-    const uint32_t nblk = (nq + kGroupQ - 1) / kGroupQ;
+    const uint32_t nblk = (nq + kQ - 1) / kQ;
     const uint32_t xcd = blockIdx.x % kXcds;
     const uint32_t seg = (nblk + kXcds - 1) / kXcds;
     const uint32_t b_begin = min(nblk, xcd * seg), b_end = min(nblk, b_begin + seg);
     const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
     for (uint32_t bj = b_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; bj < b_end; bj += local_waves) {
-        const uint32_t j0 = bj * kGroupQ;
-        const uint32_t nj = min((uint32_t)kGroupQ, nq - j0);
+        const uint32_t j0 = bj * kQ;
+        const uint32_t nj = min((uint32_t)kQ, nq - j0);
         // lane j < nj sets up query j of the group
         const bool mine = lane < nj;
         uint32_t qi = 0;
@@ -705,9 +703,9 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     cnt[C_TESTED] += (unsigned long long)total * nrun;
                 }
             }
-            F3 c[kGroupQ];
+            F3 c[kQ];
 #pragma unroll
-            for (int jj = 0; jj < kGroupQ; ++jj) c[jj] = f3s(0.0f);
+            for (int jj = 0; jj < kQ; ++jj) c[jj] = f3s(0.0f);
             uint32_t next_r = 0, cur_first = 0, cur_base = 0;
             for (uint32_t sb = 0; sb < total; sb += 64) {
                 const uint32_t sp = sb + lane;
@@ -731,7 +729,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     gb = pm.tp_gb[i];
                 }
 #pragma unroll
-                for (int jj = 0; jj < kGroupQ; ++jj) {
+                for (int jj = 0; jj < kQ; ++jj) {
                     if ((uint32_t)jj < nrun) {
                         const GroupQuery& g = gq_lds[wv][s0 + jj];
                         const bool acc = v && len2(g.q - f3(p.x, p.y, p.z)) < pm.r2;
@@ -750,7 +748,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
             }
             // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
 #pragma unroll
-            for (int jj = 0; jj < kGroupQ; ++jj) {
+            for (int jj = 0; jj < kQ; ++jj) {
                 if ((uint32_t)jj < nrun) {
                     F3 r = c[jj];
                     for (int off = 32; off > 0; off >>= 1) {
@@ -808,19 +806,24 @@ void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
         else hipLaunchKernelGGL((k_camera_trace<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
     }
 }
+template <bool kBatch, int kQ>
+static void grouped_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
+    dim3 g(blocks), b(kGatherBlock);
+    if (a.exact_merge) {
+        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, false, kBatch, kQ>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather_grouped<false, false, kBatch, kQ>), g, b, 0, s, a);
+    } else {
+        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, true, kBatch, kQ>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather_grouped<false, true, kBatch, kQ>), g, b, 0, s, a);
+    }
+}
 template <bool kBatch>
 static void gather_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
     dim3 g(blocks), b(kGatherBlock);
-    if (a.group_queries) {
-        if (a.exact_merge) {
-            if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, false, kBatch>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_gather_grouped<false, false, kBatch>), g, b, 0, s, a);
-        } else {
-            if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, true, kBatch>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_gather_grouped<false, true, kBatch>), g, b, 0, s, a);
-        }
-        return;
-    }
+    if (a.group_queries == 4) grouped_variant<kBatch, 4>(a, blocks, s);
+    else if (a.group_queries == 16) grouped_variant<kBatch, 16>(a, blocks, s);
+    else if (a.group_queries) grouped_variant<kBatch, 8>(a, blocks, s);
+    if (a.group_queries) return;
     if (a.exact_merge) {
         if (a.counters) hipLaunchKernelGGL((k_gather<true, false, kBatch>), g, b, 0, s, a);
         else hipLaunchKernelGGL((k_gather<false, false, kBatch>), g, b, 0, s, a);

@@ -3,7 +3,8 @@
 It must leave the film and the iteration count alone, pick values from its
 candidate lists, and the tuned launch shapes must render the same image: the
 photon list and the merge sets do not depend on them; only the summation
-order inside a merge changes with the cell size (relative 1e-5)."""
+order inside a merge changes with the cell size and the merge group size
+(relative 1e-5)."""
 import numpy as np
 import pytest
 
@@ -13,7 +14,8 @@ from realcausticsengine_amd.engine import Batch, Engine
 pytestmark = pytest.mark.gpu
 
 CANDIDATES = {"path_chunk": {64, 128, 256, 512, 1024}, "light_blocks_per_cu": {4, 6, 8, 12, 16},
-              "camera_blocks_per_cu": {4, 6, 8, 12, 16}, "gather_blocks_per_cu": {8, 12, 16, 24, 32}}
+              "camera_blocks_per_cu": {4, 6, 8, 12, 16}, "gather_blocks_per_cu": {8, 12, 16, 24, 32},
+              "group_queries": {4, 8, 16}}
 
 
 def _check_report(r):

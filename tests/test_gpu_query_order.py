@@ -1,6 +1,6 @@
 """The merge in spatial query order (option query_order = 1, the default)
-against the merge in trace order, and the grouped merge (group_queries = 1,
-the default) against the per-query merge.
+against the merge in trace order, and the grouped merge (group_queries = 4,
+8 -- the default -- or 16 queries per wave) against the per-query merge.
 
 In either merge a query's sum is formed by one wave in an order fixed by the
 query's cell window alone (whichever queries share its wave), and the film
@@ -30,7 +30,7 @@ def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1):
     return films, st
 
 
-@pytest.mark.parametrize("group", [0, 1])
+@pytest.mark.parametrize("group", [0, 8])
 def test_sorted_merge_matches_trace_order(group):
     plain, st0 = _films(0, group=group)
     ordered, st1 = _films(1, group=group)
@@ -47,19 +47,28 @@ def test_sorted_merge_in_graph_matches_trace_order():
         assert np.array_equal(a, b)
 
 
-def test_grouped_merge_matches_per_query_merge():
+@pytest.mark.parametrize("group", [4, 8, 16])
+def test_grouped_merge_matches_per_query_merge(group):
     per_query, s0 = _films(1, group=0)
-    grouped, s1 = _films(1, group=1)
+    grouped, s1 = _films(1, group=group)
     for a, b in zip(per_query, grouped):
         assert a.sum() > 0
         np.testing.assert_allclose(b, a, rtol=1e-5, atol=1e-6 * float(a.max()))
     assert s0["gathers"] == s1["gathers"]
 
 
+def test_grouped_merge_sizes_agree_bitwise():
+    """A query's sum depends only on its window's photon stream, not on the group size."""
+    films = [_films(1, group=g)[0] for g in (4, 8, 16)]
+    for f in films[1:]:
+        for a, b in zip(films[0], f):
+            assert np.array_equal(a, b)
+
+
 def test_grouped_merge_counts_match_per_query_merge():
     args = scenes.water_scene(128, 96, 80, samples=1)
     st = []
-    for group in (0, 1):
+    for group in (0, 8):
         eng = Engine(args, photons=200000, seed=4, write_exr=0, count_events=1, group_queries=group)
         eng.prepare()
         eng.render_iteration()
