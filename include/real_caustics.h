@@ -135,21 +135,28 @@ enum {
     RC_STAT_MS_GUIDE_STEP, RC_STAT_GUIDE_ROWS,
     /* sort of the merge queries into spatial order (option "query_order") */
     RC_STAT_MS_QUERY_SORT,
+    /* photon records the merge streamed (count_events): once per query with
+     * group_queries = 0, once per group of queries sharing a window otherwise */
+    RC_STAT_PHOTONS_STREAMED,
     RC_STAT_COUNT
 };
 
 /* Launch shapes and queue / tile sizes are options too: "path_chunk" (path
  * indices a wave takes per queue refill, a multiple of 64), "light_blocks_per_cu",
  * "camera_blocks_per_cu", "gather_blocks_per_cu" (persistent workgroups per
- * compute unit), "cell_scale" (photon-map cell size over the merge radius) and
- * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave).
+ * compute unit), "cell_scale" (photon-map cell size over the merge radius),
+ * "light_refill" / "camera_refill" (a persistent tracing wave takes new paths
+ * once at least this many of its 64 lanes are idle, or none is live; 1 = at
+ * once) and "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per
+ * wave).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects.  report (up to RC_TUNE_REPORT doubles): kernel ms per
  * iteration before and after, then path_chunk, light_blocks_per_cu,
- * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu, group_queries. */
-enum { RC_TUNE_REPORT = 8 };
+ * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu, light_refill,
+ * camera_refill, group_queries. */
+enum { RC_TUNE_REPORT = 10 };
 /* Option "graph" = 1: after the light pass (a direct launch whose photon
  * count is checked on the host), the photon-map build and the camera pass of
  * each iteration replay one captured hipGraph, reading counts and

@@ -82,6 +82,7 @@ struct Engine {
     // launch shapes and queue / tile sizes (options, or chosen by rc_autotune)
     int light_blocks_per_cu = 8, camera_blocks_per_cu = 8, gather_blocks_per_cu = 16;
     uint32_t path_chunk = 256;
+    uint32_t light_refill = 1, camera_refill = 1;  // idle lanes a tracing wave waits for before it refills
     int count_events = 0;
     int exact_merge = 0;
     std::string output_path = "D:\\caustics.exr";
@@ -852,6 +853,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.stream = STREAM_LIGHT;
         a.n_paths = (uint32_t)N;
         a.path_chunk = e.path_chunk;
+        a.refill_min = e.light_refill;
         a.path_counter = e.counters.p;
         a.photon_counter = e.counters.p + 1;
         a.capacity = e.capacity;
@@ -971,6 +973,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.stream = STREAM_CAMERA;
     a.n_paths = n_paths;
     a.path_chunk = e.path_chunk;
+    a.refill_min = e.camera_refill;
     a.path_counter = e.counters.p;
     a.vm_norm = norm;
     a.path_info = e.path_info.p;
@@ -1084,7 +1087,8 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)e.cells.p, buckets, (uintptr_t)uid_bits, (uintptr_t)e.image.p,
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
-                                  (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries};
+                                  (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
+                                  (uintptr_t)e.camera_refill};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         e.graph_exec = nullptr;
@@ -1118,6 +1122,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.stream = STREAM_CAMERA;
         a.n_paths = n_paths;
         a.path_chunk = e.path_chunk;
+        a.refill_min = e.camera_refill;
         a.path_counter = e.counters.p;
         a.path_info = e.path_info.p;
         a.contrib = e.contrib.p;
@@ -1234,6 +1239,7 @@ bool render_iteration(Engine& e, int iteration) {
         e.stats[RC_STAT_GATHERS] = (double)c[1].gathers;
         e.stats[RC_STAT_PHOTONS_TESTED] = (double)c[1].photons_tested;
         e.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)c[1].photons_accepted;
+        e.stats[RC_STAT_PHOTONS_STREAMED] = (double)c[1].photons_streamed;
     }
     return true;
 }
@@ -1275,6 +1281,12 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"gather_blocks_per_cu", {8, 12, 16, 24, 32},
          [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
          [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
+        {"light_refill", {1, 8, 16, 32},
+         [=](double v) { launch_owner().light_refill = (uint32_t)v; },
+         [=]() { return (double)launch_owner().light_refill; }, L},
+        {"camera_refill", {1, 8, 16, 32},
+         [=](double v) { launch_owner().camera_refill = (uint32_t)v; },
+         [=]() { return (double)launch_owner().camera_refill; }, T},
         {"group_queries", {4, 8, 16},
          [=](double v) { launch_owner().group_queries = (int)v; },
          [=]() { return (double)launch_owner().group_queries; }, G},
@@ -1487,6 +1499,8 @@ void copy_options(const Engine& from, Engine& to) {
     to.camera_blocks_per_cu = from.camera_blocks_per_cu;
     to.gather_blocks_per_cu = from.gather_blocks_per_cu;
     to.path_chunk = from.path_chunk;
+    to.light_refill = from.light_refill;
+    to.camera_refill = from.camera_refill;
     to.count_events = from.count_events;
     to.exact_merge = from.exact_merge;
     to.graph = from.graph;
@@ -1788,6 +1802,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.stream = STREAM_LIGHT;
         a.n_paths = (uint32_t)light;
         a.path_chunk = w.path_chunk;
+        a.refill_min = w.light_refill;
         a.path_counter = w.counters.p;
         a.photon_counter = w.counters.p + 1;
         a.capacity = w.capacity;
@@ -1844,6 +1859,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.stream = STREAM_CAMERA;
     c.n_paths = (uint32_t)camera;
     c.path_chunk = w.path_chunk;
+    c.refill_min = w.camera_refill;
     c.path_counter = w.counters.p;
     c.path_info = w.path_info.p;
     c.contrib = w.contrib.p;
@@ -1912,6 +1928,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         b.stats[RC_STAT_CELL_VISITS] = (double)cnt[1].cells;
         b.stats[RC_STAT_PHOTONS_TESTED] = (double)cnt[1].photons_tested;
         b.stats[RC_STAT_PHOTONS_ACCEPTED] = (double)cnt[1].photons_accepted;
+        b.stats[RC_STAT_PHOTONS_STREAMED] = (double)cnt[1].photons_streamed;
     }
     if (any_train) {
         b.stats[RC_STAT_MS_GUIDE] = b.guide_ms;
@@ -2153,6 +2170,8 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
+    else if (k == "light_refill") g->light_refill = (uint32_t)std::max(1, std::min(64, (int)v));
+    else if (k == "camera_refill") g->camera_refill = (uint32_t)std::max(1, std::min(64, (int)v));
     else if (k == "path_chunk") g->path_chunk = (uint32_t)std::max(1, std::min(1024, (int)v / 64)) * 64u;
     else if (k == "count_events") g->count_events = (int)v;
     else if (k == "exact_merge") g->exact_merge = (int)v;
@@ -2208,6 +2227,8 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "camera_blocks_per_cu") return g->camera_blocks_per_cu;
     if (k == "gather_blocks_per_cu") return g->gather_blocks_per_cu;
     if (k == "path_chunk") return g->path_chunk;
+    if (k == "light_refill") return g->light_refill;
+    if (k == "camera_refill") return g->camera_refill;
     if (k == "cell_scale") return g->cell_scale;
     if (k == "guide") return g->guide.enabled;
     if (k == "guide_step") return g->guide.step;

@@ -75,8 +75,9 @@ struct PhotonMapView {
 // algorithmic bytes of the roofline figures.
 struct Counters {
     unsigned long long paths, nodes, tris, hits, photons_stored, gathers, cells, photons_tested, photons_accepted;
+    unsigned long long photons_streamed;  // photon records read by the merge (once per shared window)
 };
-constexpr int kCounterFields = 9;
+constexpr int kCounterFields = 10;
 
 // Several scenes rendered by the same launches: global light and camera path
 // indices are cut into per-scene ranges; each scene keeps its own device
@@ -123,6 +124,7 @@ struct LightPassArgs {
     uint32_t k0, k1, iteration, stream;
     uint32_t n_paths;
     uint32_t path_chunk;  // path indices a wave takes per refill (>= 64)
+    uint32_t refill_min;  // a wave refills when at least this many lanes are idle (1..64) or none is live
     uint32_t* path_counter;
     uint32_t* photon_counter;
     uint32_t capacity;
@@ -165,6 +167,7 @@ struct CameraPassArgs {
     uint32_t k0, k1, iteration, stream;
     uint32_t n_paths;
     uint32_t path_chunk;      // path indices a wave takes per refill (>= 64)
+    uint32_t refill_min;      // a wave refills when at least this many lanes are idle (1..64) or none is live
     uint32_t* path_counter;   // [0] paths, [1] queries, [2] gather work counter
     float vm_norm;
     QueryBuffers q;

@@ -73,7 +73,7 @@ __device__ __forceinline__ void flush_counters(Counters* dst, unsigned long long
         if (lane_id() == 0 && v) atomicAdd(reinterpret_cast<unsigned long long*>(dst) + f, v);
     }
 }
-enum { C_PATHS, C_NODES, C_TRIS, C_HITS, C_STORED, C_GATHERS, C_CELLS, C_TESTED, C_ACCEPTED };
+enum { C_PATHS, C_NODES, C_TRIS, C_HITS, C_STORED, C_GATHERS, C_CELLS, C_TESTED, C_ACCEPTED, C_STREAMED };
 
 // ---------------------------------------------------------------------
 // Light pass
@@ -100,6 +100,9 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     for (;;) {
         // refill finished lanes
         bool need = !active && !exhausted;
+        // refill once at least refill_min lanes are idle (or none is live):
+        // the refill code then runs for many lanes at once
+        if ((uint32_t)__builtin_popcountll(__ballot(need)) < a.refill_min && __ballot(active) != 0) need = false;
         uint64_t m;
         uint32_t slot = pool.draw(a.path_counter, need);
         if (need) {
@@ -285,6 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
     WavePool pool{a.path_chunk};
     for (;;) {
         bool need = !active && !exhausted;
+        // refill once at least refill_min lanes are idle (or none is live):
+        // the refill code then runs for many lanes at once
+        if ((uint32_t)__builtin_popcountll(__ballot(need)) < a.refill_min && __ballot(active) != 0) need = false;
         uint64_t m;
         uint32_t slot = pool.draw(a.path_counter, need);
         if (need) {
@@ -504,6 +510,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
             if (lane == 0) {
                 cnt[C_CELLS] += (uint32_t)__builtin_popcountll(lookups);
                 cnt[C_TESTED] += total;
+                cnt[C_STREAMED] += total;
             }
         }
         uint32_t qhead = 0, qcount = 0;  // wave-uniform queue state
@@ -701,6 +708,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                 if (lane == 0) {
                     cnt[C_CELLS] += (uint32_t)__builtin_popcountll(lookups);
                     cnt[C_TESTED] += (unsigned long long)total * nrun;
+                    cnt[C_STREAMED] += total;
                 }
             }
             F3 c[kQ];

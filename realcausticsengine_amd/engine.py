@@ -28,7 +28,7 @@ EXPORTS = [
     "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
-             "gather_blocks_per_cu", "group_queries"]
+             "gather_blocks_per_cu", "light_refill", "camera_refill", "group_queries"]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",
@@ -36,7 +36,7 @@ STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_test
               "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
               "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve",
               "ms_guide", "guide_visible", "guide_hit_fraction", "guide_loss", "ms_guide_step", "guide_rows",
-              "ms_query_sort"]
+              "ms_query_sort", "photons_streamed"]
 # Matrix-core FLOPs of one training step per row: forward 2*(192*64 + 2*64*64 + 64*1024), data
 # gradients of the three upper layers 2*(1024*64 + 2*64*64), weight gradients 2*(64*1024 + 2*64*64 + 192*64).
 GUIDE_STEP_FLOPS_PER_ROW = 2 * (192 * 64 + 2 * 64 * 64 + 64 * 1024) + 2 * (1024 * 64 + 2 * 64 * 64) + \

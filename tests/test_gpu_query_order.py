@@ -93,3 +93,20 @@ def test_sorted_merge_in_batch_matches_trace_order():
     for a, b in zip(*out):
         assert a.sum() > 0
         assert np.array_equal(a, b)
+
+
+def test_refill_thresholds_keep_the_film():
+    """Waves that wait for idle lanes before taking new paths trace the same
+    paths with the same keys; photons and queries land in other buffer slots,
+    which the photon-map sort and the merge order undo."""
+    args = scenes.water_scene(128, 96, 80, samples=2)
+    films = []
+    for r in (1, 32):
+        eng = Engine(args, photons=200000, seed=4, write_exr=0, light_refill=r, camera_refill=r)
+        eng.prepare()
+        eng.render_iteration()
+        eng.render_iteration()
+        films.append(eng.image(scaled=False))
+        eng.release()
+    assert films[0].sum() > 0
+    assert np.array_equal(films[0], films[1])
