@@ -78,6 +78,13 @@ int main(void);
  * (Adam steps before the proposal is sampled, default 2), "guide_train"
  * (0 = keep the network fixed), "guide_train_every" (after warm-up, train on
  * every n-th iteration, default 1), "guide_seed", "guide_reset".
+ * Light-pass estimator: "mlt" (1 = the reference's adaptive MLT light pass,
+ * Vertex_Merging.cpp:231-282: for each light-path index the uniform sample,
+ * and when it stores no photon a mutation of a Markov chain's state, kept
+ * when it stores one; the merge is scaled by visible uniform paths / N), with
+ * min("mlt_chains", N / 64) independent chains (default at most 262144, so
+ * a chain takes at least 64 indices; chain c takes c, c + chains, ...).
+ * Single scenes, without the learned proposal.
  * Acceleration structure: "bvh_build" (0 = binned SAH on the host, the
  * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
  * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
@@ -88,7 +95,8 @@ int main(void);
  * queries per wave of the grouped merge, in which queries with the same cell
  * window share the photon loads (same terms as one query per wave, summed in
  * another order).
- * Read-only: "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
+ * Read-only: "mlt_visible", "mlt_mutations", "mlt_accepted" (counts of the
+ * last MLT light pass), "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
  * "nodes", "nodes4", "node_bytes" (bytes of the node layout traversed).
  * String keys: "output_path". Return 0, or -1 for an unknown key. */
 int rc_set_option(const char *key, double value);
@@ -138,6 +146,8 @@ enum {
     /* photon records the merge streamed (count_events): once per query with
      * group_queries = 0, once per group of queries sharing a window otherwise */
     RC_STAT_PHOTONS_STREAMED,
+    /* adaptive MLT light pass (option "mlt"): visible uniform paths, mutations, accepted mutations */
+    RC_STAT_MLT_VISIBLE, RC_STAT_MLT_MUTATIONS, RC_STAT_MLT_ACCEPTED,
     RC_STAT_COUNT
 };
 

@@ -288,8 +288,9 @@ static real get1d(Sampler *s) {
 
 /* Source of the draws that the reference takes from its serial stream:
  * Russian roulette and camera jitter. */
-typedef struct { int serial; Xorshift *xs; PathKey key; } Draws;
+typedef struct { int serial; Xorshift *xs; PathKey key; const uint32_t *words; } Draws;
 static real draw_rr(Draws *d, int path_length) {
+    if (d->words) return u24(d->words[4 * bounce_block(STREAM_LIGHT, path_length) + 3]);
     return d->serial ? random_float_0_1(d->xs) : path_u(&d->key, 4 * bounce_block((int)d->key.stream, path_length) + 3);
 }
 
@@ -952,6 +953,8 @@ typedef struct {
     Sampler serial_smp; /* the iteration's MLT_Sampler, shared by both passes */
     Counters ctr;
     long long visible_paths;
+    long long mlt_mutations, mlt_accepted; /* adaptive MLT light pass of the last iteration */
+    long long mlt_chains;                  /* rng_serial = 2: chains of the engine's MLT light pass */
     /* learned emission proposal: per-cell CDFs given by the caller */
     real *guide_cdf;            /* [slot][cell][GUIDE_BINS + 1] */
     int guide_cells, guide_slot[GUIDE_MAX_LIGHTS];
@@ -970,6 +973,7 @@ void *orc_create(int samples, float base_radius, float radius_reduction, const l
     s->radius_alpha = radius_reduction;
     s->max_path_length = 6; /* main.cpp:112 */
     s->camera_stride = 1;
+    s->mlt_chains = 262144;
     s->compat_point = 1; s->compat_pdf = 1; s->compat_rows = 1;
     s->xs.a = 132141u;
     s->nmeshes = number_of_meshes;
@@ -1080,6 +1084,7 @@ void orc_free(void *sp) {
 int orc_set_option(void *sp, const char *key, double v) {
     Scene *s = (Scene *)sp;
     if (!strcmp(key, "rng_serial")) s->rng_serial = (int)v;
+    else if (!strcmp(key, "mlt_chains")) s->mlt_chains = (long long)v;
     else if (!strcmp(key, "seed")) s->seed = (uint64_t)v;
     else if (!strcmp(key, "photons")) s->photons = (long long)v;
     else if (!strcmp(key, "compat_point_light")) s->compat_point = (int)v;
@@ -1098,6 +1103,8 @@ double orc_get(void *sp, const char *key) {
     if (!strcmp(key, "scene_radius")) return s->scene_radius;
     if (!strcmp(key, "base_radius")) return s->base_radius;
     if (!strcmp(key, "visible_paths")) return (double)s->visible_paths;
+    if (!strcmp(key, "mlt_mutations")) return (double)s->mlt_mutations;
+    if (!strcmp(key, "mlt_accepted")) return (double)s->mlt_accepted;
     if (!strcmp(key, "bbox_min_x")) return s->bbox.mn.x;
     if (!strcmp(key, "bbox_min_y")) return s->bbox.mn.y;
     if (!strcmp(key, "bbox_min_z")) return s->bbox.mn.z;
@@ -1468,9 +1475,121 @@ void orc_iteration_params(void *sp, int iteration, double *radius_o, double *r2_
     if (norm_o) *norm_o = norm;
 }
 
+/* ------------------------------------------------------------------ */
+/* Chain-parallel adaptive MLT light pass (rng_serial = 2): the engine's   */
+/* k_light_pass_mlt restated.  Chain c takes the indices c, c + C, ...,    */
+/* C = clamp(N / 64, 1, mlt_chains).  Primary samples are Philox words in  */
+/* the engine's layout (blocks 0-1: light sample, block 2 + k: bounce k    */
+/* with its Russian-roulette word); a chain's state is the words of its    */
+/* last visible uniform sample, or of its last accepted mutation; a        */
+/* mutation moves each 24-bit value by +-e * exp(-1/size) modulo 1 (by     */
+/* +-e when size <= 0), e and the sign from the mutation stream.  The      */
+/* chain and size arithmetic is float, as on the GPU, so the f32 build     */
+/* reproduces the engine's photons; photons keep their throughput (the     */
+/* merge carries visible / N).                                             */
+/* ------------------------------------------------------------------ */
+enum { STREAM_MLT_DELTA = 2, STREAM_MLT_INIT = 3, MLT_BLOCKS = 10, MLT_WORDS = 4 * MLT_BLOCKS };
+
+static float mlt_exp_neg(float x) {
+    if (x < -80.0f) return 0.0f;
+    float k = rintf(x * 1.44269502f);
+    float r = (x - k * 0.693145752f) - k * 1.42860677e-06f;
+    float p = 1.38888889e-03f;
+    p = p * r + 8.33333377e-03f;
+    p = p * r + 4.16666679e-02f;
+    p = p * r + 1.66666672e-01f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)k);
+}
+static float mlt_step(float size) { return size > 0.0f ? mlt_exp_neg(-1.0f / size) : 1.0f; }
+
+static void mlt_words(const uint32_t key[2], uint32_t path, uint32_t iter, uint32_t stream, uint32_t *w) {
+    for (uint32_t b = 0; b < MLT_BLOCKS; ++b) {
+        uint32_t ctr[4] = {b, path, iter, stream};
+        orc_philox(ctr, key, w + 4 * b);
+    }
+}
+static uint32_t mlt_moved(uint32_t w, uint32_t e_word, uint32_t sign_word, float step) {
+    float dlt = (float)u24(e_word) * step;
+    dlt -= floorf(dlt);
+    uint32_t d24 = (uint32_t)(dlt * 16777216.0f) & 0xFFFFFFu;
+    uint32_t v = w >> 8;
+    v = (float)u24(sign_word) < 0.5f ? v + d24 : v - d24;
+    return (v & 0xFFFFFFu) << 8;
+}
+static void mlt_propose(const uint32_t key[2], uint32_t path, uint32_t iter, const uint32_t *state, float step,
+                        uint32_t *prop) {
+    for (uint32_t b = 0; b < MLT_BLOCKS; ++b) {
+        uint32_t c0[4] = {2 * b, path, iter, STREAM_MLT_DELTA}, c1[4] = {2 * b + 1, path, iter, STREAM_MLT_DELTA};
+        uint32_t r0[4], r1[4];
+        orc_philox(c0, key, r0);
+        orc_philox(c1, key, r1);
+        const uint32_t *s4 = state + 4 * b;
+        prop[4 * b + 0] = mlt_moved(s4[0], r0[0], r0[1], step);
+        prop[4 * b + 1] = mlt_moved(s4[1], r0[2], r0[3], step);
+        prop[4 * b + 2] = mlt_moved(s4[2], r1[0], r1[1], step);
+        prop[4 * b + 3] = mlt_moved(s4[3], r1[2], r1[3], step);
+    }
+}
+/* The trace's dimensions from the words: light pick, position, direction
+ * (block 0 words a-d, block 1 word a), then three per bounce (block 2 + k). */
+static void sampler_from_words(Sampler *smp, const uint32_t *w) {
+    smp->index = 0;
+    for (int d = 0; d < 4; ++d) smp->value[d] = u24(w[d]);
+    smp->value[4] = u24(w[4]);
+    for (int b = 0; b < 8; ++b)
+        for (int j = 0; j < 3; ++j) smp->value[5 + 3 * b + j] = u24(w[4 * (2 + b) + j]);
+    smp->n = 29;
+}
+
+static void light_pass_mlt_chains(Scene *s, int iteration, PList *out) {
+    long long N = light_paths_count(s);
+    const uint32_t key[2] = {(uint32_t)s->seed, (uint32_t)(s->seed >> 32)};
+    long long C = N / 64 < s->mlt_chains ? N / 64 : s->mlt_chains;
+    if (C < 1) C = 1;
+    long long visible = 0, mutations = 0, accepted = 0;
+    uint32_t state[MLT_WORDS], w[MLT_WORDS], prop[MLT_WORDS];
+    for (long long c = 0; c < C && c < N; ++c) {
+        float size = 1.0f; /* MLT_Sampler(max_path_length, 1.0f) */
+        int acc = 1, mut = 0; /* Vertex_Merging.cpp:227-228 */
+        mlt_words(key, (uint32_t)c, (uint32_t)iteration, STREAM_MLT_INIT, state);
+        for (long long p = c; p < N; p += C) {
+            Draws dr = {0, NULL, {{key[0], key[1]}, (uint32_t)p, (uint32_t)iteration, STREAM_LIGHT}, w};
+            Sampler smp;
+            mlt_words(key, (uint32_t)p, (uint32_t)iteration, STREAM_LIGHT, w);
+            sampler_from_words(&smp, w);
+            if (trace_light_path(s, &smp, &dr, (uint32_t)p, out)) {
+                ++visible;
+                memcpy(state, w, sizeof state);
+                continue;
+            }
+            mlt_propose(key, (uint32_t)p, (uint32_t)iteration, state, mlt_step(size), prop);
+            sampler_from_words(&smp, prop);
+            dr.words = prop;
+            ++mut;
+            ++mutations;
+            if (trace_light_path(s, &smp, &dr, (uint32_t)p, out)) {
+                ++acc;
+                ++accepted;
+                memcpy(state, prop, sizeof state);
+            }
+            size += ((float)acc / (float)mut - 0.234f) / (float)mut;
+        }
+    }
+    s->visible_paths = visible;
+    s->mlt_mutations = mutations;
+    s->mlt_accepted = accepted;
+}
+
 static void light_pass(Scene *s, int iteration, PList *out) {
     long long N = light_paths_count(s);
     s->visible_paths = 0;
+    if (s->rng_serial == 2) {
+        light_pass_mlt_chains(s, iteration, out);
+        return;
+    }
     if (!s->rng_serial) {
         Draws dr = {0, NULL, {{(uint32_t)s->seed, (uint32_t)(s->seed >> 32)}, 0, (uint32_t)iteration, STREAM_LIGHT}};
         for (long long p = 0; p < N; ++p) {
@@ -1515,6 +1634,8 @@ static void light_pass(Scene *s, int iteration, PList *out) {
         }
     }
     s->visible_paths = visible_uniform;
+    s->mlt_mutations = mutated;
+    s->mlt_accepted = accepted - 1; /* the reference's count starts at 1 (:227) */
     real c = (real)visible_uniform / (real)N;
     for (long long i = 0; i < out->n; ++i)
         for (int k = 6; k < 9; ++k) out->p[(size_t)i * ORC_PHOTON_STRIDE + k] = (double)((real)out->p[(size_t)i * ORC_PHOTON_STRIDE + k] * c);
@@ -1856,6 +1977,12 @@ void orc_render(void *sp, int iterations, double *image) {
     for (int it = 0; it < iterations; ++it) {
         PList l = {0};
         light_pass(s, it, &l);
+        if (s->rng_serial == 2) { /* visibility normalisation of the chain-parallel MLT pass */
+            real c = (real)s->visible_paths / (real)light_paths_count(s);
+            for (long long i = 0; i < l.n; ++i)
+                for (int k = 6; k < 9; ++k)
+                    l.p[(size_t)i * ORC_PHOTON_STRIDE + k] = (double)((real)l.p[(size_t)i * ORC_PHOTON_STRIDE + k] * c);
+        }
         camera_pass(s, it, l.p, l.n, img);
         free(l.p);
     }

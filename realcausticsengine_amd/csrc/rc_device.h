@@ -402,7 +402,8 @@ __device__ __forceinline__ float guide_direction(const GuideView& g, int light, 
 // (area and sphere lights), bin = -1 otherwise.  The direction sample (block
 // 1 draw, optional guided warp, cosine lobe) is one piece of code for both
 // guidable kinds.
-__device__ __forceinline__ bool sample_light(const SceneView& sc, const PathRng& rng, const GuideView& gv, F3& o,
+template <class Rng>
+__device__ __forceinline__ bool sample_light(const SceneView& sc, const Rng& rng, const GuideView& gv, F3& o,
                                              F3& d, F3& tp, int& bin, F3& axis) {
     U4 w0 = rng.block(0);
     int idx = find_interval(sc.cdf, sc.nlights + 1, u24(w0.a));

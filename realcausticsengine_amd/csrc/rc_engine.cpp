@@ -115,6 +115,12 @@ struct Engine {
     hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
     int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
+    // adaptive MLT light pass (option "mlt"): chains, their explicit states and counts
+    int mlt = 0;
+    uint32_t mlt_chains = 262144;
+    DevBuf<uint4> mlt_state;
+    DevBuf<uint32_t> mlt_counts;
+    uint32_t mlt_last[3] = {0, 0, 0};  // visible uniform paths, mutations, accepted mutations
     DevBuf<unsigned long long> qk_a, qk_b;  // query sort buffers (q_capacity each)
     DevBuf<uint32_t> qv_a, qv_b, q_hist, q_hist_sums;
     int sort_passes = 0;
@@ -757,6 +763,10 @@ bool prepare(Engine& e) {
 
 long long light_paths(const Engine& e) { return e.photons > 0 ? e.photons : (long long)e.W * e.H; }
 
+// Visibility normalisation of the adaptive MLT light pass (Vertex_Merging.cpp:278-282):
+// every photon's throughput times (visible uniform paths) / N, applied to the merge.
+float mlt_factor(const Engine& e) { return e.mlt ? (float)e.mlt_last[0] / (float)light_paths(e) : 1.0f; }
+
 // Vertex_Merging.cpp:201-212, in float with the C library powf.
 void iteration_params(const Engine& e, int iteration, float& radius, float& r2, float& norm) {
     float count = (float)light_paths(e);
@@ -842,8 +852,16 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
     uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
     if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
     if (!ensure_guide_buffers(e)) return false;
+    if (e.mlt) {
+        if (guide_active(e)) {
+            set_error("options mlt and guide are exclusive");
+            return false;
+        }
+        if (!e.mlt_state.alloc((size_t)kMltBlocks * e.mlt_chains) || !e.mlt_counts.alloc(4)) return false;
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
         LightPassArgs a{};
         a.sc = scene_view(e);
@@ -863,12 +881,19 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         a.out_uid = e.ph_uid.p;
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
         a.guide = guide_light_view(e, iteration);
+        if (e.mlt) {
+            a.mlt_state = e.mlt_state.p;
+            // at most mlt_chains chains of at least 64 indices each (the mutation size adapts along a chain)
+            a.mlt_chains = (uint32_t)std::max<long long>(1, std::min<long long>(e.mlt_chains, N / 64));
+            a.mlt_counts = e.mlt_counts.p;
+        }
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
         launch_light_pass(a, light_blocks(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
         HIP_OK(hipGetLastError());
         uint32_t c = 0;
         HIP_OK(hipMemcpyAsync(&c, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        if (e.mlt) HIP_OK(hipMemcpyAsync(e.mlt_last, e.mlt_counts.p, sizeof e.mlt_last, hipMemcpyDeviceToHost, e.stream));
         HIP_OK(hipStreamSynchronize(e.stream));
         if (c <= e.capacity) {
             count = c;
@@ -1071,7 +1096,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     ip.r_query = rq;
     ip.inv_cell = 1.0f / cell;
     for (int k = 0; k < 3; ++k) ip.origin[k] = e.bbox.lo[k] - cell;
-    ip.vm_norm = norm;
+    ip.vm_norm = norm * mlt_factor(e);
     HIP_OK(hipMemcpyAsync(e.d_iter.p, e.h_iter, sizeof(IterParams), hipMemcpyHostToDevice, st));
     PhotonMapView& pm = mi.pm;
     pm.pos = e.s_pos.p;
@@ -1175,7 +1200,7 @@ bool render_iteration(Engine& e, int iteration) {
     } else {
         if (!run_light_pass(e, iteration, count)) return false;
         if (!build_map(e, count, radius, r2, mi)) return false;
-        if (!run_camera_pass(e, iteration, mi, norm, e.image.p)) return false;
+        if (!run_camera_pass(e, iteration, mi, norm * mlt_factor(e), e.image.p)) return false;
     }
     const bool training = guide_trains(e, iteration);
     e.stats[RC_STAT_MS_GUIDE] = e.stats[RC_STAT_MS_GUIDE_STEP] = e.stats[RC_STAT_GUIDE_ROWS] = 0;
@@ -1218,6 +1243,9 @@ bool render_iteration(Engine& e, int iteration) {
         e.stats[RC_STAT_GUIDE_LOSS] = e.guide.last[2];
     }
     e.stats[RC_STAT_LIGHT_PATHS] = (double)light_paths(e);
+    e.stats[RC_STAT_MLT_VISIBLE] = e.mlt ? e.mlt_last[0] : 0.0;
+    e.stats[RC_STAT_MLT_MUTATIONS] = e.mlt ? e.mlt_last[1] : 0.0;
+    e.stats[RC_STAT_MLT_ACCEPTED] = e.mlt ? e.mlt_last[2] : 0.0;
     e.stats[RC_STAT_PHOTONS] = count;
     e.stats[RC_STAT_CAMERA_PATHS] = (double)e.W * e.H;
     e.stats[RC_STAT_MS_LIGHT] = ms[0];
@@ -1343,7 +1371,7 @@ bool timed_iteration(Engine& e, int it, float* film, double* t) {
     if (!run_light_pass(e, it, count)) return false;
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
-    if (!run_camera_pass(e, it, mi, norm, film)) return false;
+    if (!run_camera_pass(e, it, mi, norm * mlt_factor(e), film)) return false;
     HIP_OK(hipStreamSynchronize(e.stream));
     float ms[T_PARTS];
     HIP_OK(hipEventElapsedTime(&ms[T_LIGHT], e.ev[0], e.ev[1]));
@@ -1462,6 +1490,8 @@ void release(Engine*& e) {
     e->keys_a.release();
     e->keys_b.release();
     for (auto* d : {&e->qk_a, &e->qk_b}) d->release();
+    e->mlt_state.release();
+    e->mlt_counts.release();
     for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
     e->cells.release();
     e->image.release();
@@ -1508,6 +1538,8 @@ void copy_options(const Engine& from, Engine& to) {
     to.bvh_build = from.bvh_build;
     to.query_order = from.query_order;
     to.group_queries = from.group_queries;
+    to.mlt = from.mlt;
+    to.mlt_chains = from.mlt_chains;
     to.output_path = from.output_path;
     const GuideModel& pg = from.guide;  // the proposal's options, not its state
     GuideModel& ng = to.guide;
@@ -1616,6 +1648,10 @@ bool batch_prepare(Batch& b) {
         // one launch serves every scene: the merge arithmetic is chosen per launch
         if (e->exact_merge != b.scenes[0]->exact_merge) {
             set_error("batch scenes must agree on the option exact_merge");
+            return false;
+        }
+        if (e->mlt) {
+            set_error("the adaptive MLT light pass (option mlt) renders single scenes only");
             return false;
         }
     }
@@ -2166,6 +2202,8 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
+    else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
+    else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
@@ -2219,6 +2257,11 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "bvh_build") return g->bvh_build;
     if (k == "query_order") return g->query_order;
     if (k == "group_queries") return g->group_queries;
+    if (k == "mlt") return g->mlt;
+    if (k == "mlt_chains") return g->mlt_chains;
+    if (k == "mlt_visible") return g->mlt_last[0];  // counts of the last MLT light pass
+    if (k == "mlt_mutations") return g->mlt_last[1];
+    if (k == "mlt_accepted") return g->mlt_last[2];
     if (k == "bvh_depth") return g->bvh_depth;
     if (k == "bvh_build_ms") return g->bvh_build_ms;
     if (k == "nodes4") return g->n_nodes4;

@@ -135,7 +135,13 @@ struct LightPassArgs {
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)
+    // adaptive MLT light pass (non-null mlt_state selects it; single scene, plain proposal)
+    uint4* mlt_state;     // kMltBlocks * mlt_chains explicit chain states, block-major
+    uint32_t mlt_chains;  // Markov chains; chain c takes the path indices c, c + chains, ...
+    uint32_t* mlt_counts; // [0] visible uniform paths, [1] mutations, [2] accepted mutations
 };
+// Philox blocks of a light path's primary sample (light sample 0-1, bounces 2-9).
+constexpr int kMltBlocks = 10;
 
 // Per-iteration values read from device memory by the kernels of a captured
 // iteration graph (a graph replays fixed kernel arguments).

@@ -210,7 +210,227 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     if (kCount) flush_counters(a.counters, cnt);
 }
 
+// ---------------------------------------------------------------------
+// Adaptive MLT light pass (Vertex_Merging.cpp:231-282, MLT_Sampler.cpp:7-67)
+// with one Markov chain per lane instead of the reference's single serial
+// chain.  Chain c takes the path indices c, c + chains, c + 2 chains, ...;
+// for each index it traces the uniform sample of the index (the plain
+// estimator's path); a path that stores no photon is "invisible" and is
+// followed by a mutation of the chain's current state, accepted when the
+// mutated path stores a photon.  Visible uniform samples become the chain's
+// state; every path that stores photons keeps them.  The mutation size adapts
+// towards an acceptance rate of 0.234 as in the reference, per chain.  The
+// host scales the merge by (visible uniform paths) / N, the reference's
+// visibility normalisation.
+//
+// Primary samples are Philox words; the chain's state is either the Philox
+// sample of (base_path, base_stream) -- its last accepted uniform sample or
+// its initial state -- or, after an accepted mutation, explicit words in
+// memory.  A mutation moves each 24-bit value by +-e (1/e)^(1/size) modulo 1
+// (MLT_Sampler::Mutate), e and the sign drawn per dimension; a size that has
+// adapted to zero or below moves by up to +-e (the reference's huge steps
+// modulo 1 are likewise jumps across the whole interval).
+// ---------------------------------------------------------------------
+struct MltRng {
+    uint32_t k0, k1, iter, path;
+    bool mutate;
+    const uint4* state;  // explicit state (block b at state[b * stride]) or null
+    uint32_t stride, base_path, base_stream;
+    float step;  // powf(1/e, 1/mutation_size)
+    __device__ __forceinline__ U4 current(uint32_t b) const {
+        if (state) {
+            uint4 v = state[(size_t)b * stride];
+            return U4{v.x, v.y, v.z, v.w};
+        }
+        return philox(b, base_path, iter, base_stream, k0, k1);
+    }
+    __device__ __forceinline__ uint32_t moved(uint32_t w, uint32_t e_word, uint32_t sign_word) const {
+        float dlt = u24(e_word) * step;
+        dlt -= floorf(dlt);
+        uint32_t d24 = (uint32_t)(dlt * 16777216.0f) & 0xFFFFFFu;
+        uint32_t v = w >> 8;
+        v = u24(sign_word) < 0.5f ? v + d24 : v - d24;
+        return (v & 0xFFFFFFu) << 8;
+    }
+    __device__ __forceinline__ U4 block(uint32_t b) const {
+        if (!mutate) return philox(b, path, iter, STREAM_LIGHT, k0, k1);
+        U4 s = current(b);
+        U4 r0 = philox(2 * b, path, iter, STREAM_MLT_DELTA, k0, k1);
+        U4 r1 = philox(2 * b + 1, path, iter, STREAM_MLT_DELTA, k0, k1);
+        return U4{moved(s.a, r0.a, r0.b), moved(s.b, r0.c, r0.d), moved(s.c, r1.a, r1.b), moved(s.d, r1.c, r1.d)};
+    }
+};
+
+template <bool kCount>
+__global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
+    __shared__ int stack_lds[kLdsStack * kBlock];
+    int* my_stack = stack_lds + threadIdx.x;
+    unsigned long long cnt[kCounterFields] = {0};
+    uint32_t n_nodes = 0, n_tris = 0;
+    LightState st;
+    const uint32_t lanes = gridDim.x * blockDim.x;
+    const uint32_t C = a.mlt_chains;
+    GuideView plain{};  // no proposal: every light samples plain emission
+    plain.cells = 1;
+    uint32_t chain = blockIdx.x * blockDim.x + threadIdx.x;  // this lane's first chain
+    uint32_t step_k = 0;                                      // position of the current index in the chain
+    float msize = 1.0f;                                       // MLT_Sampler(max_path_length, 1.0f)
+    int acc = 1, mut = 0;                                     // Vertex_Merging.cpp:227-228
+    bool expl = false;
+    uint32_t base_path = chain, base_stream = STREAM_MLT_INIT;
+    bool mutating = false, started = false, visible = false;
+    bool active = false, exhausted = chain >= C || chain >= a.n_paths;
+    uint32_t n_visible = 0, n_mut = 0, n_acc = 0;
+    MltRng rng{a.k0, a.k1, a.iteration, 0u, false, nullptr, C, 0u, 0u, 1.0f};
+    for (;;) {
+        if (!active && !exhausted) {
+            bool next = started;
+            if (started) {
+                if (!mutating) {
+                    if (visible) {  // the uniform sample becomes the chain's state
+                        ++n_visible;
+                        expl = false;
+                        base_path = st.path;
+                        base_stream = STREAM_LIGHT;
+                    } else {  // Reject + Mutate: a mutation of the state for the same index
+                        mutating = true;
+                        next = false;
+                    }
+                } else {
+                    ++mut;
+                    ++n_mut;
+                    if (visible) {  // accepted: the proposal becomes explicit state
+                        ++acc;
+                        ++n_acc;
+#pragma unroll 1
+                        for (uint32_t b = 0; b < (uint32_t)kMltBlocks; ++b) {
+                            U4 w = rng.block(b);
+                            a.mlt_state[(size_t)b * C + chain] = make_uint4(w.a, w.b, w.c, w.d);
+                        }
+                        expl = true;
+                    }
+                    msize += ((float)acc / (float)mut - 0.234f) / (float)mut;
+                    mutating = false;
+                }
+            }
+            if (next) {
+                ++step_k;
+                if ((uint64_t)chain + (uint64_t)step_k * C >= a.n_paths) {  // the lane's next chain
+                    chain += lanes;
+                    step_k = 0;
+                    msize = 1.0f;
+                    acc = 1;
+                    mut = 0;
+                    expl = false;
+                    base_path = chain;
+                    base_stream = STREAM_MLT_INIT;
+                    if (chain >= C || chain >= a.n_paths) exhausted = true;
+                }
+            }
+            if (!exhausted) {
+                if (kCount) cnt[C_PATHS]++;
+                st.path = chain + step_k * C;
+                st.local = st.path;
+                st.scene = 0;
+                rng.path = st.path;
+                rng.mutate = mutating;
+                rng.state = expl ? a.mlt_state + chain : nullptr;
+                rng.base_path = base_path;
+                rng.base_stream = base_stream;
+                rng.step = mlt_step(msize);
+                int bin;
+                F3 axis;
+                active = sample_light(a.sc, rng, plain, st.o, st.d, st.tp, bin, axis);
+                st.pl = 0;
+                st.diffuse = 0;
+                st.last = EV_DIFFUSE;
+                st.dvm = 0.0f;
+                started = true;
+                visible = false;
+            }
+        }
+        if (__ballot(active || !exhausted) == 0) break;
+        if (__ballot(active) == 0) continue;
+        n_nodes = n_tris = 0;
+        const SceneView& sc = a.sc;
+        // one bounce, as k_light_pass
+        bool alive = active;
+        Hit h;
+        Surface sf;
+        Bsdf b;
+        if (alive) {
+            F3 ro = st.o + st.d * 0.001f;
+            bool hit = trace_closest<kCount>(sc, ro, st.d, 0.001f, h, my_stack, n_nodes, n_tris);
+            if (kCount) {
+                cnt[C_NODES] += n_nodes;
+                cnt[C_TRIS] += n_tris;
+                cnt[C_HITS] += hit;
+            }
+            if (!hit) {
+                alive = false;
+            } else {
+                sf = surface_at(sc, h, ro, st.d);
+                if (!bsdf_setup(b, sc.mats[sf.mat], sf.sn, st.d, false)) alive = false;
+                else st.dvm /= fabsf(fabsf(b.wo.z));
+            }
+        }
+        bool store = alive && !bsdf_delta(b) && (st.last & EV_GLOSSY);
+        uint64_t m;
+        uint32_t at = wave_alloc(a.photon_counter, store, &m);
+        if (kCount) cnt[C_STORED] += store;
+        if (store) visible = true;
+        if (store && at < a.capacity) {
+            a.out_pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
+            a.out_dir[at] = make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x);
+            a.out_tp[at] = make_float2(st.tp.y, st.tp.z);
+            a.out_uid[at] = st.path * 8u + (uint32_t)st.pl;
+        }
+        if (alive) {
+            if (st.pl > sc.max_path_length || st.diffuse > 2) {
+                alive = false;
+            } else {
+                U4 w = rng.block((uint32_t)bounce_block(STREAM_LIGHT, st.pl));
+                BsdfSample bs;
+                F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
+                if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {
+                    alive = false;
+                } else {
+                    if (st.pl > 2) {
+                        if (u24(w.d) > bs.cont) alive = false;
+                        else f = div_each(f, bs.cont);
+                    }
+                    if (alive) {
+                        st.o = sf.pos;
+                        st.d = bs.wi;
+                        if (bs.event & EV_DIFFUSE) st.diffuse++;
+                        st.tp = st.tp * f;
+                        st.last = bs.event;
+                        if (!store) st.dvm *= (bs.cos_out / bs.fwd) * bs.rev;
+                        else st.dvm = (bs.cos_out / bs.fwd) * (1.0f + bs.rev * st.dvm);
+                        st.pl++;
+                    }
+                }
+            }
+        }
+        active = alive;
+    }
+    // chain statistics: one atomic per wave and counter
+    unsigned long long v3[3] = {n_visible, n_mut, n_acc};
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        unsigned long long v = v3[f];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+        if (lane_id() == 0 && v) atomicAdd(a.mlt_counts + f, (uint32_t)v);
+    }
+    if (kCount) flush_counters(a.counters, cnt);
+}
+
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s) {
+    if (a.mlt_state) {
+        if (a.counters) hipLaunchKernelGGL((k_light_pass_mlt<true>), dim3(blocks), dim3(kBlock), 0, s, a);
+        else hipLaunchKernelGGL((k_light_pass_mlt<false>), dim3(blocks), dim3(kBlock), 0, s, a);
+        return;
+    }
     if (a.batch.n > 0) {
         if (a.counters) hipLaunchKernelGGL((k_light_pass<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
         else hipLaunchKernelGGL((k_light_pass<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);

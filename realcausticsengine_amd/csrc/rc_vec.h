@@ -180,12 +180,35 @@ RC_HD U4 philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0,
     return U4{c0, c1, c2, c3};
 }
 RC_HD float u24(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
+
+// exp(x) for x <= 0 in float operations only (the same bits on the host and
+// the GPU): Cody-Waite reduction by ln 2 (two-part constant) and the degree-6
+// Taylor polynomial on |r| <= ln2 / 2; 0 below -80, where the result would
+// approach the denormal range.
+RC_HD float own_exp_neg(float x) {
+    if (x < -80.0f) return 0.0f;
+    float k = rintf(x * 1.44269502f);
+    float r = (x - k * 0.693145752f) - k * 1.42860677e-06f;
+    float p = 1.38888889e-03f;
+    p = p * r + 8.33333377e-03f;
+    p = p * r + 4.16666679e-02f;
+    p = p * r + 1.66666672e-01f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    return ldexpf(p, (int)k);
+}
+// Mutation scale of the adaptive MLT light pass: (1/e)^(1/size), and 1 (a
+// jump of up to the whole unit interval) when the adapted size is not positive.
+RC_HD float mlt_step(float size) { return size > 0.0f ? own_exp_neg(-1.0f / size) : 1.0f; }
 RC_HD uint32_t pick_word(U4 w, int i) { return i == 0 ? w.a : (i == 1 ? w.b : (i == 2 ? w.c : w.d)); }
 
 // Light paths: dimensions 0..4 are the light sample, bounce k uses Philox
 // block 2 + k ({BSDF u0, u1, u2, Russian roulette}).  Camera paths:
 // dimensions 0, 1 are the pixel jitter, bounce k uses block 1 + k.
-enum { STREAM_LIGHT = 0, STREAM_CAMERA = 1 };
+// The adaptive MLT light pass adds a stream of mutation draws and one of
+// initial chain states.
+enum { STREAM_LIGHT = 0, STREAM_CAMERA = 1, STREAM_MLT_DELTA = 2, STREAM_MLT_INIT = 3 };
 RC_HD int bounce_block(int stream, int path_length) { return (stream == STREAM_LIGHT ? 2 : 1) + (path_length & 7); }
 
 struct PathRng {

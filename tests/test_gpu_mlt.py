@@ -1,0 +1,84 @@
+"""The adaptive MLT light pass (option mlt) on the GPU.
+
+The chains are deterministic (Philox draws, float adaptation), so their
+photons must equal the oracle's restatement of the same chains (rng_serial =
+2) path for path; that restatement is itself compared with the reference's
+serial loop statistically on the CPU (test_oracle_mlt.py).  The uniform
+samples are exactly the plain estimator's paths, so the visible count equals
+the plain run's and the film minus the plain film (both scaled) is the
+accepted mutations' photons, never negative."""
+import numpy as np
+import pytest
+
+from realcausticsengine_amd import scenes
+from realcausticsengine_amd.engine import Engine, photons_from_oracle
+
+pytestmark = pytest.mark.gpu
+
+N = 65536
+
+
+def _scene():
+    return scenes.c0_scene(64, 64, subdiv=3, samples=4)
+
+
+def _gpu(iters=2, **opts):
+    eng = Engine(_scene(), photons=N, seed=9, write_exr=0, **opts)
+    eng.prepare()
+    for _ in range(iters):
+        eng.render_iteration()
+    img = eng.image(scaled=False)
+    st = eng.stats()
+    eng.release()
+    return img, st
+
+
+@pytest.mark.parametrize("chains", [1, 262144])
+def test_mlt_photons_match_the_oracle_restatement(oracle, chains):
+    """Photon lists of the GPU chains against the oracle's restatement of the
+    same chains (rng_serial = 2): bit-exact per path (closest-hit ties aside)
+    and the same visible, mutation and acceptance counts."""
+    eng = Engine(_scene(), photons=N, seed=9, write_exr=0, mlt=1, mlt_chains=chains)
+    ph_g, uid_g = eng.trace_photons(1)
+    counts = [eng.get(k) for k in ("mlt_visible", "mlt_mutations", "mlt_accepted")]
+    eng.release()
+    o = oracle.OracleScene(_scene(), photons=N, seed=9, rng_serial=2, mlt_chains=chains)
+    ph_o = o.trace_light(1)
+    counts_o = [o.get(k) for k in ("visible_paths", "mlt_mutations", "mlt_accepted")]
+    print(f"chains {chains}: gpu {counts} oracle {counts_o} photons {len(uid_g)} / {len(ph_o)}", flush=True)
+    uid_o = ph_o[:, 10].astype(np.uint32)
+    common, ig, io = np.intersect1d(uid_g, uid_o, return_indices=True)
+    assert len(common) >= 0.999 * max(len(uid_g), len(uid_o))
+    exact = np.all(ph_g[ig].view(np.uint32) == photons_from_oracle(ph_o[io]).view(np.uint32), axis=1)
+    assert exact.mean() >= 0.999, exact.mean()
+    assert abs(counts[0] - counts_o[0]) <= 1e-3 * N and abs(counts[2] - counts_o[2]) <= 1e-3 * N
+
+
+def test_chains_keep_the_plain_paths_and_add_mutations():
+    plain, _ = _gpu(iters=1)
+    eng = Engine(_scene(), photons=N, seed=9, write_exr=0)
+    _, uid = eng.trace_photons(0)
+    eng.release()
+    visible_plain = len(np.unique(uid >> 3))
+    mlt, st = _gpu(iters=1, mlt=1)
+    vis, mut, acc = st["mlt_visible"], st["mlt_mutations"], st["mlt_accepted"]
+    print(f"visible {vis:.0f} (plain {visible_plain}) mutations {mut:.0f} accepted {acc:.0f}", flush=True)
+    assert vis == visible_plain and vis + mut == N and 0 < acc < mut
+    extra = mlt / (vis / N) - plain  # the accepted mutations' photons
+    assert extra.min() >= -1e-4 * plain.max()
+    assert extra.sum() > 0
+
+
+def test_mlt_graph_matches_direct_launches():
+    direct, _ = _gpu(iters=3, mlt=1)
+    graphed, _ = _gpu(iters=3, mlt=1, graph=1)
+    assert direct.sum() > 0
+    assert np.array_equal(direct, graphed)
+
+
+def test_mlt_excludes_the_learned_proposal():
+    eng = Engine(_scene(), photons=N, seed=9, write_exr=0, mlt=1, guide=1)
+    eng.prepare()
+    with pytest.raises(Exception, match="exclusive"):
+        eng.render_iteration()
+    eng.release()
