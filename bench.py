@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Vertices.h:17-27)
 
 
-def algorithmic_bytes(st: dict, node_bytes: int = 128) -> dict:
+def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> dict:
     """Bytes per launch from the event counts of one counting iteration.
 
     traversal: one node per visit (128 B: the four child boxes and refs of the
@@ -40,15 +40,20 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128) -> dict:
     light pass: + 40 B per stored photon.  photon map: per radix pass 32 B per
     photon, + 88 B permute and 28 B key generation per photon.  camera trace:
     + 80 B per merge query written + 8 B per path record.  gather: 80 B per
-    query read, 8 B per cell lookup, 12 B per photon tested, 28 B per photon
-    accepted, 12 B per result.  resolve: 8 B per path, 12 B per query, 24 B
-    per film update."""
+    query read, 12 B per result, 8 B per cell lookup, and per photon: with one
+    query per wave (grouped = False) 12 B per photon tested and 28 B per photon
+    accepted; grouped, the 40 B record once per photon streamed (a group of
+    queries sharing a window reads it once).  resolve: 8 B per path, 12 B per
+    query, 24 B per film update."""
     light = (node_bytes * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
              + PHOTON_BYTES * st["photons"])
     trace = (node_bytes * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
              + 80 * st["gathers"] + 8 * st["camera_paths"])
-    gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
-              + 28 * st["photons_accepted"])
+    if grouped:
+        gather = 92 * st["gathers"] + 8 * st["cell_visits"] + PHOTON_BYTES * st["photons_streamed"]
+    else:
+        gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
+                  + 28 * st["photons_accepted"])
     resolve = 32 * st["camera_paths"] + 12 * st["gathers"]
     photon_map = st["photons"] * (32 * st["sort_passes"] + 88 + 28)
     return {"light_pass": light, "photon_map": photon_map, "camera_trace": trace, "gather": gather,
@@ -306,7 +311,7 @@ def time_to_rmse(a, cpu_s_per_iteration):
         eng = engine(guided, a.photons)
         tuner = convergence.PhotonBudgetTuner(4 << 20, 1 << 20, pilots=6)
         run = convergence.run_tuned(eng, ref, target, a.ttr_max_iterations, tuner, label=f"{name} tuned",
-                                    max_seconds=a.ttr_max_seconds)
+                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune)
         mfma = None
         if guided:
             eng.set("guide_train_every", 1)
@@ -429,7 +434,7 @@ def main() -> int:
     eng.render_iteration()
     cst = eng.stats()
     eng.set("count_events", 0)
-    by = algorithmic_bytes(cst, int(eng.get("node_bytes")))
+    by = algorithmic_bytes(cst, int(eng.get("node_bytes")), eng.get("group_queries") > 0)
     dom = max(kern, key=kern.get)
     achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -444,7 +449,8 @@ def main() -> int:
         "photons_tested_per_gather": cst["photons_tested"] / max(1.0, cst["gathers"]),
         "counts": {k: int(cst[k]) for k in ("light_node_visits", "light_tri_tests", "light_hits",
                                              "camera_node_visits", "camera_tri_tests", "camera_hits",
-                                             "gathers", "cell_visits", "photons_tested", "photons_accepted")},
+                                             "gathers", "cell_visits", "photons_tested", "photons_accepted",
+                                             "photons_streamed")},
     }
     eng.release()
     cpu = None

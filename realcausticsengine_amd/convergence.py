@@ -58,6 +58,7 @@ class ConvergenceRun:
     rmse: list = field(default_factory=list)      # running-image RMSE after each iteration
     reached_at: int | None = None                  # 1-based iteration that first reached the target
     stats: list = field(default_factory=list)
+    autotune: dict | None = None                   # launch shapes chosen inside the run (run_tuned)
 
     def summary(self, target: float) -> dict:
         it = self.reached_at
@@ -65,6 +66,8 @@ class ConvergenceRun:
                "seconds": round(self.seconds[it - 1], 4) if it else None,
                "final_rmse": round(self.rmse[-1], 5) if self.rmse else None,
                "iterations_run": len(self.rmse), "light_paths_per_iteration": sorted(set(self.photons))}
+        if self.autotune:
+            out["autotune"] = {k: round(v, 3) for k, v in self.autotune.items()}
         if not it and len(self.rmse) >= 8:
             # not reached: extrapolate rmse^2 = a / k from the last quarter of the run
             k = np.arange(1, len(self.rmse) + 1, dtype=np.float64)
@@ -185,14 +188,23 @@ class PhotonBudgetTuner:
 
 
 def run_tuned(eng: Engine, ref: SubsetReference, target: float, max_iterations: int, tuner: PhotonBudgetTuner,
-              label: str = "", max_seconds: float = 1e9) -> ConvergenceRun:
-    """run_to_target with the tuner choosing the light-path count."""
+              label: str = "", max_seconds: float = 1e9, autotune: bool = False) -> ConvergenceRun:
+    """run_to_target with the tuner choosing the light-path count.  With
+    `autotune`, rc_autotune picks the launch shapes for the chosen count once
+    the pilots are done; its wall time counts in the run's time (its renders
+    go to a scratch film and add nothing to the image)."""
     run = ConvergenceRun(label)
     spent = 0.0
+    tuned = False
     for i in range(max_iterations):
         n = tuner.schedule(i)
         if n is not None:
             eng.set("photons", int(n))
+        if autotune and not tuned and tuner.fit is not None:
+            t0 = time.perf_counter()
+            run.autotune = eng.autotune(repeats=1)
+            spent += time.perf_counter() - t0
+            tuned = True
         t0 = time.perf_counter()
         eng.render_iteration()
         spent += time.perf_counter() - t0

@@ -1315,6 +1315,10 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"camera_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().camera_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().camera_refill; }, T},
+        {"query_order", {1, 0},
+         [=](double v) { launch_owner().query_order = (int)v; },
+         [=]() { return (double)launch_owner().query_order; },
+         [](const double* t) { return t[T_TRACE] + t[T_GATHER]; }},
         {"group_queries", {4, 8, 16},
          [=](double v) { launch_owner().group_queries = (int)v; },
          [=]() { return (double)launch_owner().group_queries; }, G},
