@@ -157,16 +157,17 @@ enum {
  * compute unit), "cell_scale" (photon-map cell size over the merge radius),
  * "light_refill" / "camera_refill" (a persistent tracing wave takes new paths
  * once at least this many of its 64 lanes are idle, or none is live; 1 = at
- * once), "query_order" and "group_queries" (merge queries per wave: 4, 8 or
- * 16; 0 = one per wave).
+ * once), "light_order" (1 = trace the light paths sorted by their emission
+ * samples, for ray coherence; the film is the same), "query_order" and
+ * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects.  report (up to RC_TUNE_REPORT doubles): kernel ms per
  * iteration before and after, then path_chunk, light_blocks_per_cu,
  * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu, light_refill,
- * camera_refill, query_order, group_queries. */
-enum { RC_TUNE_REPORT = 11 };
+ * light_order, camera_refill, query_order, group_queries. */
+enum { RC_TUNE_REPORT = 12 };
 /* Option "graph" = 1: after the light pass (a direct launch whose photon
  * count is checked on the host), the photon-map build and the camera pass of
  * each iteration replay one captured hipGraph, reading counts and

@@ -114,6 +114,9 @@ struct Engine {
     DevBuf<Counters> stats_dev;  // [0] light pass, [1] camera pass
     hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
+    int light_order = 0;    // 1: the light pass traces its paths sorted by emission position and direction
+    DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
+    DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
     int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
     // adaptive MLT light pass (option "mlt"): chains, their explicit states and counts
     int mlt = 0;
@@ -842,6 +845,17 @@ bool ensure_guide_buffers(Engine& e, hipStream_t s = nullptr) {
     return gd.train_x.alloc((size_t)n * kGuideRowFloats) && gd.train_bin.alloc(n) && gd.train_w.alloc(n);
 }
 
+// Coherent order of this launch's light paths (option light_order), or null.
+const uint32_t* light_path_order(Engine& e, const LightPassArgs& a, hipStream_t s) {
+    if (!e.light_order || a.mlt_state) return nullptr;
+    size_t n = a.n_paths;
+    if (!e.lk_a.alloc(n) || !e.lk_b.alloc(n) || !e.lv_a.alloc(n) || !e.lv_b.alloc(n) ||
+        !e.l_hist.alloc(photon_map_hist_entries((uint32_t)n)) || !e.l_hist_sums.alloc(photon_map_sum_entries((uint32_t)n)))
+        return nullptr;
+    QuerySortBuffers b{e.lk_a.p, e.lk_b.p, e.lv_a.p, e.lv_b.p, e.l_hist.p, e.l_hist_sums.p};
+    return sort_light_paths(a, b, s);
+}
+
 // Light pass into the unsorted photon buffers; returns the photon count.
 bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
     long long N = light_paths(e);
@@ -888,6 +902,8 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
             a.mlt_counts = e.mlt_counts.p;
         }
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
+        a.path_order = light_path_order(e, a, e.stream);  // inside the light pass's time
+        if (e.light_order && !a.path_order && !e.mlt) return false;
         launch_light_pass(a, light_blocks(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
         HIP_OK(hipGetLastError());
@@ -1312,6 +1328,9 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"light_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().light_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().light_refill; }, L},
+        {"light_order", {0, 1},
+         [=](double v) { launch_owner().light_order = (int)v; },
+         [=]() { return (double)launch_owner().light_order; }, L},
         {"camera_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().camera_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().camera_refill; }, T},
@@ -1493,7 +1512,8 @@ void release(Engine*& e) {
     for (auto* b : u) b->release();
     e->keys_a.release();
     e->keys_b.release();
-    for (auto* d : {&e->qk_a, &e->qk_b}) d->release();
+    for (auto* d : {&e->qk_a, &e->qk_b, &e->lk_a, &e->lk_b}) d->release();
+    for (auto* d : {&e->lv_a, &e->lv_b, &e->l_hist, &e->l_hist_sums}) d->release();
     e->mlt_state.release();
     e->mlt_counts.release();
     for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
@@ -1541,6 +1561,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.bvh_width = from.bvh_width;
     to.bvh_build = from.bvh_build;
     to.query_order = from.query_order;
+    to.light_order = from.light_order;
     to.group_queries = from.group_queries;
     to.mlt = from.mlt;
     to.mlt_chains = from.mlt_chains;
@@ -1619,7 +1640,8 @@ void release_batch(Batch*& b) {
     for (auto* d : {&w.ph_uid, &w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
     w.keys_a.release();
     w.keys_b.release();
-    for (auto* d : {&w.qk_a, &w.qk_b}) d->release();
+    for (auto* d : {&w.qk_a, &w.qk_b, &w.lk_a, &w.lk_b}) d->release();
+    for (auto* d : {&w.lv_a, &w.lv_b, &w.l_hist, &w.l_hist_sums}) d->release();
     for (auto* d : {&w.qv_a, &w.qv_b, &w.q_hist, &w.q_hist_sums}) d->release();
     w.cells.release();
     w.path_info.release();
@@ -1675,6 +1697,7 @@ bool batch_prepare(Batch& b) {
     if (!ok) return false;
     if (!ensure_device(b.work)) return false;
     b.work.query_order = b.scenes[0]->query_order;
+    b.work.light_order = b.scenes[0]->light_order;
     b.work.group_queries = b.scenes[0]->group_queries;
     if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
@@ -1854,6 +1877,8 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.batch = view;
         a.counters = counting ? w.stats_dev.p : nullptr;
         HIP_OK(hipEventRecord(w.ev[0], st));
+        a.path_order = light_path_order(w, a, st);
+        if (w.light_order && !a.path_order) return false;
         launch_light_pass(a, light_blocks(w), st);
         HIP_OK(hipEventRecord(w.ev[1], st));
         HIP_OK(hipGetLastError());
@@ -2206,6 +2231,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
+    else if (k == "light_order") g->light_order = (int)v ? 1 : 0;
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
@@ -2260,6 +2286,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "node_bytes") return uses_nodes4(*g) ? 128 : 64;
     if (k == "bvh_build") return g->bvh_build;
     if (k == "query_order") return g->query_order;
+    if (k == "light_order") return g->light_order;
     if (k == "group_queries") return g->group_queries;
     if (k == "mlt") return g->mlt;
     if (k == "mlt_chains") return g->mlt_chains;

@@ -135,6 +135,7 @@ struct LightPassArgs {
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)
+    const uint32_t* path_order;  // non-null: slot s traces path path_order[s] (paths sorted for coherence)
     // adaptive MLT light pass (non-null mlt_state selects it; single scene, plain proposal)
     uint4* mlt_state;     // kMltBlocks * mlt_chains explicit chain states, block-major
     uint32_t mlt_chains;  // Markov chains; chain c takes the path indices c, c + chains, ...
@@ -208,6 +209,9 @@ struct QuerySortBuffers {
     uint32_t* hist_sums;
 };
 const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s);
+// Coherent order of the light paths: key = (scene, top bits of the light-pick word, 4-D Morton code of the
+// emission position and direction words), one stable radix sort.  Returns the order (one of the vals buffers).
+const uint32_t* sort_light_paths(const LightPassArgs& a, const QuerySortBuffers& b, hipStream_t s);
 void launch_first_hit(const SceneView& sc, const float* rays, int n, float tmin, int* tri, float* t, hipStream_t s);
 // Debug AOV of debug_test mode 2, 3 or 4 added into image (W * H * 3).
 void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hipStream_t s);

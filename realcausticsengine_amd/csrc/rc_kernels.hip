@@ -108,13 +108,14 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         if (need) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
-                st.path = slot;
-                st.local = slot;
+                const uint32_t path = a.path_order ? a.path_order[slot] : slot;
+                st.path = path;
+                st.local = path;
                 st.scene = 0;
                 if (kBatch) {
-                    st.scene = batch_scene_of(a.batch.light_starts, a.batch.n, slot);
+                    st.scene = batch_scene_of(a.batch.light_starts, a.batch.n, path);
                     const BatchScene& bs = a.batch.bs[st.scene];
-                    st.local = slot - bs.light_base;
+                    st.local = path - bs.light_base;
                     rng.k0 = bs.k0;
                     rng.k1 = bs.k1;
                     rng.iter = bs.iteration;
@@ -1446,6 +1447,52 @@ __global__ void k_query_keys(CameraPassArgs a, uint32_t n, unsigned long long* k
                    (uint32_t)((hi[2] - lo[2]) & 3) << 4;
     keys[i] = ((unsigned long long)(uint32_t)scene << 36) | ((unsigned long long)code << 6) | ext;
     vals[i] = i;
+}
+
+// 7 low bits of v spread to every fourth bit (4-D Morton codes).
+__device__ __forceinline__ uint32_t spread7x4(uint32_t v) {
+    uint32_t r = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) r |= ((v >> k) & 1u) << (4 * k);
+    return r;
+}
+
+__global__ void k_light_keys(LightPassArgs a, unsigned long long* keys, uint32_t* vals) {
+    uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n_paths) return;
+    PathRng rng{a.k0, a.k1, p, a.iteration, STREAM_LIGHT};
+    uint32_t scene = 0;
+    if (a.batch.n > 0) {
+        scene = (uint32_t)batch_scene_of(a.batch.light_starts, a.batch.n, p);
+        const BatchScene& bs = a.batch.bs[scene];
+        rng.k0 = bs.k0;
+        rng.k1 = bs.k1;
+        rng.iter = bs.iteration;
+        rng.path = p - bs.light_base;
+    }
+    // the plain emission words: light pick, position (2), direction (2)
+    U4 w0 = rng.block(0);
+    uint32_t wy = rng.block(1).a;
+    uint32_t code = (spread7x4(w0.b >> 25) << 3) | (spread7x4(w0.c >> 25) << 2) | (spread7x4(w0.d >> 25) << 1) |
+                    spread7x4(wy >> 25);
+    keys[p] = ((unsigned long long)scene << 32) | ((unsigned long long)(w0.a >> 28) << 28) | code;
+    vals[p] = p;
+}
+
+const uint32_t* sort_light_paths(const LightPassArgs& a, const QuerySortBuffers& b, hipStream_t s) {
+    const uint32_t n = a.n_paths;
+    if (n == 0) return nullptr;
+    int bits = 32;
+    if (a.batch.n > 1) {
+        int sb = 0;
+        while ((1 << sb) < a.batch.n) ++sb;
+        bits = 32 + sb;
+    }
+    hipLaunchKernelGGL(k_light_keys, dim3((n + 255) / 256), dim3(256), 0, s, a, b.keys_a, b.vals_a);
+    unsigned long long* kout;
+    uint32_t* vout;
+    radix_sort_pairs(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, b.hist_sums, nullptr, n, bits, &kout, &vout, s);
+    return vout;
 }
 
 const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s) {

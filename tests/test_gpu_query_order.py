@@ -110,3 +110,34 @@ def test_refill_thresholds_keep_the_film():
         eng.release()
     assert films[0].sum() > 0
     assert np.array_equal(films[0], films[1])
+
+
+def test_light_path_order_keeps_the_film():
+    """Light paths traced in the order of their emission samples: each path is
+    computed from its own index and key, so the photons are the same and the
+    photon-map sort restores their order."""
+    args = scenes.water_scene(128, 96, 80, samples=2)
+    films = []
+    for order in (0, 1):
+        eng = Engine(args, photons=200000, seed=4, write_exr=0, light_order=order)
+        eng.prepare()
+        eng.render_iteration()
+        eng.render_iteration()
+        films.append(eng.image(scaled=False))
+        eng.release()
+    assert films[0].sum() > 0
+    assert np.array_equal(films[0], films[1])
+
+
+def test_light_path_order_in_batch_keeps_the_films():
+    cases = [(scenes.water_scene(32, 24, 16, samples=2), dict(photons=20000, seed=3)),
+             (scenes.c0_scene(32, 32, subdiv=2, samples=2), dict(photons=50000, seed=5, max_path_length=4))]
+    out = []
+    for order in (0, 1):
+        b = Batch([a for a, _ in cases], [dict(o, write_exr=0, light_order=order) for _, o in cases]).prepare()
+        b.render_iteration()
+        out.append([b.image(k, scaled=False) for k in range(len(cases))])
+        b.release()
+    for a, b in zip(*out):
+        assert a.sum() > 0
+        assert np.array_equal(a, b)
