@@ -287,33 +287,35 @@ def time_to_rmse(a, cpu_s_per_iteration):
     target = a.ttr_target
     legs = {}
 
-    def engine(guided, photons):
+    def engine(guided, photons, qmc=False):
         args = scenes.water_scene(a.grid, 1024, 1024, alpha=1.0, samples=1, base_radius=0.01)
-        opts = dict(photons=photons, seed=0, write_exr=0, compat_rows=0)
+        opts = dict(photons=photons, seed=0, write_exr=0, compat_rows=0, camera_qmc=int(qmc))
         if guided:
             opts.update(guide=1, guide_warmup=2, guide_train_every=a.guide_train_every)
         e = Engine(args, **opts)
         e.prepare()
         return e
 
-    for guided in (False, True):
-        name = "guided" if guided else "plain"
-        print(f"[bench] time-to-RMSE: {name}, 16,777,216 light paths/iteration", file=sys.stderr, flush=True)
-        eng = engine(guided, a.photons)
-        run = convergence.run_to_target(eng, ref, target, a.ttr_max_iterations, label=f"{name} 16M",
-                                        max_seconds=a.ttr_max_seconds)
-        eng.release()
-        legs[f"{name}_16M"] = run.summary(target)
-        if guided:
-            legs[f"{name}_16M"]["guide_hit_fraction_last"] = round(run.stats[-1].get("guide_hit_fraction", 0), 4)
-        print(f"[bench]   {legs[f'{name}_16M']}", file=sys.stderr, flush=True)
+    # (guided, stratified camera samples): the 16M legs only for independent draws
+    for guided, qmc in ((False, False), (True, False), (False, True), (True, True)):
+        name = ("guided" if guided else "plain") + ("_qmc" if qmc else "")
+        if not qmc:
+            print(f"[bench] time-to-RMSE: {name}, 16,777,216 light paths/iteration", file=sys.stderr, flush=True)
+            eng = engine(guided, a.photons)
+            run = convergence.run_to_target(eng, ref, target, a.ttr_max_iterations, label=f"{name} 16M",
+                                            max_seconds=a.ttr_max_seconds)
+            eng.release()
+            legs[f"{name}_16M"] = run.summary(target)
+            if guided:
+                legs[f"{name}_16M"]["guide_hit_fraction_last"] = round(run.stats[-1].get("guide_hit_fraction", 0), 4)
+            print(f"[bench]   {legs[f'{name}_16M']}", file=sys.stderr, flush=True)
         print(f"[bench] time-to-RMSE: {name}, tuned light-path count", file=sys.stderr, flush=True)
-        eng = engine(guided, a.photons)
+        eng = engine(guided, a.photons, qmc)
         tuner = convergence.PhotonBudgetTuner(4 << 20, 1 << 20, pilots=6)
         run = convergence.run_tuned(eng, ref, target, a.ttr_max_iterations, tuner, label=f"{name} tuned",
                                     max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune)
         mfma = None
-        if guided:
+        if guided and not qmc:
             eng.set("guide_train_every", 1)
             eng.render_iteration()  # one training iteration to time the network step
             st = eng.stats()

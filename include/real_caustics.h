@@ -89,6 +89,11 @@ int main(void);
  * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
  * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
  * host tree collapsed to 4-wide nodes, the default; 2 = binary nodes).
+ * Camera sampling: "camera_qmc" (1 = pixel jitter and the first bounce's
+ * reflect / refract choice follow a randomly shifted low-discrepancy sequence
+ * over iterations -- iteration i takes point i -- instead of independent
+ * draws; every iteration keeps its expectation, the running image converges
+ * faster).
  * Merge order: "query_order" (1 = the merge takes the queries sorted by the
  * Morton code of their photon-map cell window, the default; 0 = in trace
  * order); the film is the same either way.  "group_queries" (8 = the default):

@@ -64,6 +64,8 @@ def load(precision: str = "f32") -> ctypes.CDLL:
     lib.orc_debug_aov.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]
     lib.orc_set_guide.argtypes = [ctypes.c_void_p, _fp, ctypes.c_int, ctypes.c_int, _ip, ctypes.c_double]
     lib.orc_set_guide.restype = ctypes.c_int
+    lib.orc_qmc_words.argtypes = [ctypes.c_ulonglong, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(ctypes.c_uint)]
+    lib.orc_qmc_words.restype = None
     _libs[precision] = lib
     return lib
 
@@ -221,4 +223,11 @@ def bsdf(mat, mode: int, shade_n, geo_n, direction, u3=(0.5, 0.5, 0.5), wi=(0.0,
     arrs = [np.ascontiguousarray(x, dtype=np.float32) for x in (shade_n, geo_n, direction, u3, wi)]
     out = np.zeros(13, dtype=np.float64)
     lib.orc_bsdf(_ptr(m, _fp), mode, *[_ptr(a, _fp) for a in arrs], op, _ptr(out, _dp))
+    return out
+
+
+def qmc_words(seed: int, pixel: int, iteration: int) -> np.ndarray:
+    """The three stratified camera words (option camera_qmc) of one pixel sample."""
+    out = np.zeros(3, dtype=np.uint32)
+    load("f32").orc_qmc_words(seed, pixel, iteration, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)))
     return out

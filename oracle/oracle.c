@@ -955,6 +955,7 @@ typedef struct {
     long long visible_paths;
     long long mlt_mutations, mlt_accepted; /* adaptive MLT light pass of the last iteration */
     long long mlt_chains;                  /* rng_serial = 2: chains of the engine's MLT light pass */
+    int camera_qmc;                        /* stratified camera jitter and first-bounce choice */
     /* learned emission proposal: per-cell CDFs given by the caller */
     real *guide_cdf;            /* [slot][cell][GUIDE_BINS + 1] */
     int guide_cells, guide_slot[GUIDE_MAX_LIGHTS];
@@ -1085,6 +1086,7 @@ int orc_set_option(void *sp, const char *key, double v) {
     Scene *s = (Scene *)sp;
     if (!strcmp(key, "rng_serial")) s->rng_serial = (int)v;
     else if (!strcmp(key, "mlt_chains")) s->mlt_chains = (long long)v;
+    else if (!strcmp(key, "camera_qmc")) s->camera_qmc = (int)v;
     else if (!strcmp(key, "seed")) s->seed = (uint64_t)v;
     else if (!strcmp(key, "photons")) s->photons = (long long)v;
     else if (!strcmp(key, "compat_point_light")) s->compat_point = (int)v;
@@ -1815,6 +1817,46 @@ long long orc_gather_sets_kd(const float *pos, long long P, const float *q, long
 /* ------------------------------------------------------------------ */
 /* Camera pass (Vertex_Merging.cpp:27-46, 98-135, 309-363)             */
 /* ------------------------------------------------------------------ */
+/* ------------------------------------------------------------------ */
+/* Stratified camera samples (camera_qmc), the engine's rc_vec.h restated:  */
+/* sample i of a pixel = point i of the (0,2)-sequence (van der Corput,     */
+/* Sobol's second dimension) for the jitter and the base-3 radical inverse  */
+/* for the first bounce's first dimension, each shifted modulo 1 by the     */
+/* pixel's Philox offset (stream 4, iteration-independent counter).         */
+/* ------------------------------------------------------------------ */
+enum { STREAM_QMC_SHIFT = 4 };
+static uint32_t qmc_vdc(uint32_t i) {
+    uint32_t r = 0;
+    for (int b = 0; b < 32; ++b) r |= ((i >> b) & 1u) << (31 - b);
+    return r;
+}
+static uint32_t qmc_sobol2(uint32_t i) {
+    uint32_t r = 0, v = 1u << 31;
+    for (; i; i >>= 1, v ^= v >> 1)
+        if (i & 1u) r ^= v;
+    return r;
+}
+static uint32_t qmc_ri3_24(uint32_t i) {
+    uint64_t num = 0, den = 1;
+    for (; i; i /= 3u) { num = num * 3u + i % 3u; den *= 3u; }
+    return (uint32_t)((num << 24) / den);
+}
+static uint32_t qmc_word(uint32_t v24, uint32_t shift_word) { return ((v24 + (shift_word >> 8)) & 0xFFFFFFu) << 8; }
+static void qmc_camera_words(const uint32_t key[2], uint32_t pixel, uint32_t iteration, uint32_t w[3]) {
+    uint32_t ctr[4] = {0u, pixel, 0xFFFFFFFFu, STREAM_QMC_SHIFT}, sh[4];
+    orc_philox(ctr, key, sh);
+    w[0] = qmc_word(qmc_vdc(iteration) >> 8, sh[0]);
+    w[1] = qmc_word(qmc_sobol2(iteration) >> 8, sh[1]);
+    w[2] = qmc_word(qmc_ri3_24(iteration), sh[2]);
+}
+/* The three camera words of a pixel's sample `iteration` (seed = Philox key). */
+void orc_qmc_words(unsigned long long seed, unsigned int pixel, unsigned int iteration, unsigned int *out3) {
+    const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t w[3];
+    qmc_camera_words(key, pixel, iteration, w);
+    for (int k = 0; k < 3; ++k) out3[k] = w[k];
+}
+
 static void camera_pass(Scene *s, int iteration, const double *ph, long long P, real *image) {
     double dr_, dr2, dnorm;
     orc_iteration_params(s, iteration, &dr_, &dr2, &dnorm);
@@ -1841,6 +1883,13 @@ static void camera_pass(Scene *s, int iteration, const double *ph, long long P, 
             jx = path_u(&dr.key, 0);
             jy = path_u(&dr.key, 1);
             sampler_plain(smp, &dr.key);
+            if (s->camera_qmc) {
+                uint32_t w[3];
+                qmc_camera_words(dr.key.key, (uint32_t)p, (uint32_t)iteration, w);
+                jx = u24(w[0]);
+                jy = u24(w[1]);
+                smp->value[0] = u24(w[2]); /* bounce 0, first dimension */
+            }
         }
         smp->index = 0;
         real sx = (real)(int)(p % W) + K(0.0f) + K(1.0f) * jx;

@@ -55,6 +55,9 @@ void orc_free(void *scene);
  * "max_path_length", "gather" (0 = KD tree, 1 = brute force). */
 int orc_set_option(void *scene, const char *key, double value);
 double orc_get(void *scene, const char *key);
+/* Stratified camera words (option camera_qmc) of a pixel's sample `iteration`:
+ * jitter x, jitter y, first bounce's first dimension. */
+void orc_qmc_words(unsigned long long seed, unsigned int pixel, unsigned int iteration, unsigned int *out3);
 
 /* Prelude of main(): BVH build, lights, radius scale (main.cpp:157-168). */
 int orc_prepare(void *scene);

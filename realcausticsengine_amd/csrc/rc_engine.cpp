@@ -115,6 +115,7 @@ struct Engine {
     hipEvent_t ev[9] = {};  // light, map, camera (trace..resolve), trace end, gather end, query sort begin
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
     int light_order = 0;    // 1: the light pass traces its paths sorted by emission position and direction
+    int camera_qmc = 0;     // 1: camera jitter and first-bounce choice stratified across iterations
     DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
     DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
     int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
@@ -1023,6 +1024,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
     a.exact_merge = e.exact_merge;
     a.group_queries = e.group_queries;
+    a.camera_qmc = e.camera_qmc;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
@@ -1129,7 +1131,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
-                                  (uintptr_t)e.camera_refill};
+                                  (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         e.graph_exec = nullptr;
@@ -1170,6 +1172,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.image = e.image.p;
         a.exact_merge = e.exact_merge;
         a.group_queries = e.group_queries;
+        a.camera_qmc = e.camera_qmc;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         hipGraph_t graph = nullptr;
@@ -1562,6 +1565,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.bvh_build = from.bvh_build;
     to.query_order = from.query_order;
     to.light_order = from.light_order;
+    to.camera_qmc = from.camera_qmc;
     to.group_queries = from.group_queries;
     to.mlt = from.mlt;
     to.mlt_chains = from.mlt_chains;
@@ -1930,6 +1934,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.contrib = w.contrib.p;
     c.exact_merge = e0.exact_merge;
     c.group_queries = w.group_queries;
+    c.camera_qmc = e0.camera_qmc;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
     uint32_t nq = 0;
@@ -2232,6 +2237,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
     else if (k == "light_order") g->light_order = (int)v ? 1 : 0;
+    else if (k == "camera_qmc") g->camera_qmc = (int)v ? 1 : 0;
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
@@ -2287,6 +2293,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "bvh_build") return g->bvh_build;
     if (k == "query_order") return g->query_order;
     if (k == "light_order") return g->light_order;
+    if (k == "camera_qmc") return g->camera_qmc;
     if (k == "group_queries") return g->group_queries;
     if (k == "mlt") return g->mlt;
     if (k == "mlt_chains") return g->mlt_chains;

@@ -186,6 +186,7 @@ struct CameraPassArgs {
     BatchView batch;     // batch.n > 0: several scenes (queries carry their scene in gn.w)
     const IterParams* dyn;  // non-null: iteration, vm_norm and the grid of pm come from here
     const uint32_t* q_order;  // non-null: the gather takes the queries in this order (spatially sorted)
+    int camera_qmc;           // 1: jitter and first-bounce choice stratified across iterations
     int group_queries;        // 4, 8 or 16: grouped merge of that many queries per wave (queries with the
                               // same cell window share photon loads); 0: one query per wave
 };

@@ -492,6 +492,7 @@ struct CameraState {
     int scene;
     float vm_norm;
     int pixel;
+    uint32_t qmc_u0;  // camera_qmc: the first bounce's stratified first word
 };
 
 template <bool kCount, bool kBatch>
@@ -536,6 +537,12 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
                 const uint32_t px = st.local;
                 // Generate_camera_sample (Vertex_Merging.cpp:27-46), Camera::get_ray (Camera.h:29-34)
                 U4 wj = rng.block(0);
+                if (a.camera_qmc) {  // stratified across iterations (sample index = iteration)
+                    U4 sh = philox(0u, px, 0xFFFFFFFFu, STREAM_QMC_SHIFT, rng.k0, rng.k1);
+                    wj.a = qmc_word(qmc_vdc(rng.iter) >> 8, sh.a);
+                    wj.b = qmc_word(qmc_sobol2(rng.iter) >> 8, sh.b);
+                    st.qmc_u0 = qmc_word(qmc_ri3_24(rng.iter), sh.c);
+                }
                 float sx = (float)(int)(px % (uint32_t)W) + 0.0f + 1.0f * u24(wj.a);
                 float sy = (float)(int)(px / (uint32_t)W) + 0.0f + 1.0f * u24(wj.b);
                 F3 dir = (ld3(sc.cam_bl) + ld3(sc.cam_h) * (sx / (float)W)) + ld3(sc.cam_v) * (sy / (float)H);
@@ -601,6 +608,7 @@ __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
                 // Sample_Scattering_camera (Vertex_Merging.cpp:98-135)
                 rng.path = st.local;
                 U4 w = rng.block((uint32_t)bounce_block(STREAM_CAMERA, st.pl));
+                if (a.camera_qmc && st.pl == 0) w.a = st.qmc_u0;
                 BsdfSample bs;
                 F3 f = bsdf_sample(b, sf.gn, u24(w.a), u24(w.b), u24(w.c), bs);
                 if (f.x == 0.0f && f.y == 0.0f && f.z == 0.0f) {

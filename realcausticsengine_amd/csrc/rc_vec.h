@@ -198,6 +198,39 @@ RC_HD float own_exp_neg(float x) {
     p = p * r + 1.0f;
     return ldexpf(p, (int)k);
 }
+// Progressive camera samples stratified across iterations (option
+// camera_qmc): sample i of a pixel takes point i of the (0,2)-sequence (van der
+// Corput, Sobol's second dimension) for the jitter and the base-3 radical
+// inverse for the first bounce's first dimension (the reflect / refract
+// choice at a dielectric), each shifted modulo 1 by a per-pixel random offset
+// that does not change across iterations (Cranley-Patterson rotation), so
+// every iteration is still uniformly distributed.  Values are 24-bit, in the
+// word form u24() reads.
+enum { STREAM_QMC_SHIFT = 4 };
+RC_HD uint32_t qmc_vdc(uint32_t i) {
+    i = (i << 16) | (i >> 16);
+    i = ((i & 0x00FF00FFu) << 8) | ((i & 0xFF00FF00u) >> 8);
+    i = ((i & 0x0F0F0F0Fu) << 4) | ((i & 0xF0F0F0F0u) >> 4);
+    i = ((i & 0x33333333u) << 2) | ((i & 0xCCCCCCCCu) >> 2);
+    i = ((i & 0x55555555u) << 1) | ((i & 0xAAAAAAAAu) >> 1);
+    return i;
+}
+RC_HD uint32_t qmc_sobol2(uint32_t i) {
+    uint32_t r = 0, v = 1u << 31;
+    for (; i; i >>= 1, v ^= v >> 1)
+        if (i & 1u) r ^= v;
+    return r;
+}
+RC_HD uint32_t qmc_ri3_24(uint32_t i) {  // floor(2^24 * base-3 radical inverse of i)
+    uint64_t num = 0, den = 1;
+    for (; i; i /= 3u) {
+        num = num * 3u + i % 3u;
+        den *= 3u;
+    }
+    return (uint32_t)((num << 24) / den);
+}
+RC_HD uint32_t qmc_word(uint32_t v24, uint32_t shift_word) { return ((v24 + (shift_word >> 8)) & 0xFFFFFFu) << 8; }
+
 // Mutation scale of the adaptive MLT light pass: (1/e)^(1/size), and 1 (a
 // jump of up to the whole unit interval) when the adapted size is not positive.
 RC_HD float mlt_step(float size) { return size > 0.0f ? own_exp_neg(-1.0f / size) : 1.0f; }
