@@ -484,9 +484,9 @@ __device__ __forceinline__ bool tri_test(const float4* __restrict__ tri, int i, 
 }
 
 template <bool kCount>
-__device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+__device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int2* lds_stack,
                                                uint32_t& n_nodes, uint32_t& n_tris) {
-    int ext[kMaxDepth - kLdsStack];
+    int2 ext[kMaxDepth - kLdsStack];
     float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
     float tbest = INFINITY;
     int best = -1;
@@ -519,8 +519,9 @@ __device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, 
             int cl = __float_as_int(n3.x), cr = __float_as_int(n3.y);
             if (hl && hr) {
                 int nearc = ln <= rn ? cl : cr, farc = ln <= rn ? cr : cl;
-                if (sp < kLdsStack) lds_stack[sp * kBlock] = farc;
-                else ext[sp - kLdsStack] = farc;
+                const int2 e = make_int2(farc, __float_as_int(ln <= rn ? rn : ln));
+                if (sp < kLdsStack) lds_stack[sp * kBlock] = e;
+                else ext[sp - kLdsStack] = e;
                 ++sp;
                 ref = nearc;
                 continue;
@@ -543,9 +544,19 @@ __device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, 
                 }
             }
         }
-        if (sp == 0) break;
-        --sp;
-        ref = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+        // pop the nearest pending entry whose box can still hold a closer hit
+        // (the test a box passes when it is pushed: entry distance <= tbest)
+        bool more = false;
+        while (sp > 0) {
+            --sp;
+            const int2 e = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+            if (__int_as_float(e.y) <= tbest) {
+                ref = e.x;
+                more = true;
+                break;
+            }
+        }
+        if (!more) break;
     }
     h.t = tbest;
     h.u = bu;
@@ -561,9 +572,9 @@ __device__ __forceinline__ bool trace_closest2(const SceneView& sc, F3 o, F3 d, 
 // entry distance (a 5-comparator network), the nearest is taken next and the
 // others are pushed far to near.
 template <bool kCount>
-__device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+__device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int2* lds_stack,
                                                uint32_t& n_nodes, uint32_t& n_tris) {
-    int ext[kMaxDepth - kLdsStack];
+    int2 ext[kMaxDepth - kLdsStack];
     float ix = 1.0f / d.x, iy = 1.0f / d.y, iz = 1.0f / d.z;
     float tbest = INFINITY;
     int best = -1;
@@ -572,9 +583,10 @@ __device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, 
     int ref = sc.root4;
     const float4* __restrict__ nodes = sc.nodes4;
     const float slack = 1.0000019f;
-    auto push = [&](int r) {
-        if (sp < kLdsStack) lds_stack[sp * kBlock] = r;
-        else ext[sp - kLdsStack] = r;
+    auto push = [&](int r, float dist) {
+        const int2 e = make_int2(r, __float_as_int(dist));
+        if (sp < kLdsStack) lds_stack[sp * kBlock] = e;
+        else ext[sp - kLdsStack] = e;
         ++sp;
     };
     while (true) {
@@ -621,9 +633,9 @@ __device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, 
             cx(1, 3);
             cx(1, 2);
             if (hit[0]) {
-                if (hit[3]) push(cref[3]);
-                if (hit[2]) push(cref[2]);
-                if (hit[1]) push(cref[1]);
+                if (hit[3]) push(cref[3], key[3]);
+                if (hit[2]) push(cref[2], key[2]);
+                if (hit[1]) push(cref[1], key[1]);
                 ref = cref[0];
                 continue;
             }
@@ -641,9 +653,19 @@ __device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, 
                 }
             }
         }
-        if (sp == 0) break;
-        --sp;
-        ref = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+        // pop the nearest pending entry whose box can still hold a closer hit
+        // (the test a box passes when it is pushed: entry distance <= tbest)
+        bool more = false;
+        while (sp > 0) {
+            --sp;
+            const int2 e = sp < kLdsStack ? lds_stack[sp * kBlock] : ext[sp - kLdsStack];
+            if (__int_as_float(e.y) <= tbest) {
+                ref = e.x;
+                more = true;
+                break;
+            }
+        }
+        if (!more) break;
     }
     h.t = tbest;
     h.u = bu;
@@ -653,7 +675,7 @@ __device__ __forceinline__ bool trace_closest4(const SceneView& sc, F3 o, F3 d, 
 }
 
 template <bool kCount>
-__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int* lds_stack,
+__device__ __forceinline__ bool trace_closest(const SceneView& sc, F3 o, F3 d, float tmin, Hit& h, int2* lds_stack,
                                               uint32_t& n_nodes, uint32_t& n_tris) {
     return sc.nodes4 ? trace_closest4<kCount>(sc, o, d, tmin, h, lds_stack, n_nodes, n_tris)
                      : trace_closest2<kCount>(sc, o, d, tmin, h, lds_stack, n_nodes, n_tris);

@@ -89,8 +89,8 @@ struct LightState {
 
 template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
-    __shared__ int stack_lds[kLdsStack * kBlock];
-    int* my_stack = stack_lds + threadIdx.x;
+    __shared__ int2 stack_lds[kLdsStack * kBlock];
+    int2* my_stack = stack_lds + threadIdx.x;
     unsigned long long cnt[kCounterFields] = {0};
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
@@ -264,8 +264,8 @@ struct MltRng {
 
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
-    __shared__ int stack_lds[kLdsStack * kBlock];
-    int* my_stack = stack_lds + threadIdx.x;
+    __shared__ int2 stack_lds[kLdsStack * kBlock];
+    int2* my_stack = stack_lds + threadIdx.x;
     unsigned long long cnt[kCounterFields] = {0};
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
@@ -497,8 +497,8 @@ struct CameraState {
 
 template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_camera_trace(CameraPassArgs a) {
-    __shared__ int stack_lds[kLdsStack * kBlock];
-    int* my_stack = stack_lds + threadIdx.x;
+    __shared__ int2 stack_lds[kLdsStack * kBlock];
+    int2* my_stack = stack_lds + threadIdx.x;
     if (a.dyn) {
         a.iteration = a.dyn->iteration;
         a.vm_norm = a.dyn->vm_norm;
@@ -1086,7 +1086,7 @@ void launch_resolve(const CameraPassArgs& a, hipStream_t s) {
 // dielectric Fresnel reflectance of the view direction for IOR `ior`.
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_debug_aov(SceneView sc, int mode, float ior, float* image) {
-    __shared__ int stack_lds[kLdsStack * kBlock];
+    __shared__ int2 stack_lds[kLdsStack * kBlock];
     const int W = sc.W, H = sc.H;
     int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= W * H) return;
@@ -1131,7 +1131,7 @@ void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hi
 // ---------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void k_first_hit(SceneView sc, const float* rays, int n, float tmin, int* tri,
                                                       float* t) {
-    __shared__ int stack_lds[kLdsStack * kBlock];
+    __shared__ int2 stack_lds[kLdsStack * kBlock];
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     F3 o = f3(rays[6 * i], rays[6 * i + 1], rays[6 * i + 2]);
