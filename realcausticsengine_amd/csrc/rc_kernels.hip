@@ -64,6 +64,60 @@ struct WavePool {
     }
 };
 
+// Photons of a wave are gathered in an LDS ring and written out 64 at a time:
+// one atomic on the shared photon counter per 64 photons instead of one per
+// bounce step (the counter is the light pass's most contended address), and
+// coalesced stores.  A ring holds fewer than 64 entries before a push, so 128
+// entries never overflow.
+constexpr uint32_t kPhotonRing = 128;
+struct PhotonRing {
+    float4 pos[kPhotonRing];
+    float4 dir[kPhotonRing];
+    float2 tp[kPhotonRing];
+    uint32_t uid[kPhotonRing];
+};
+struct PhotonQueue {
+    PhotonRing* r;
+    uint32_t head = 0, count = 0;  // wave-uniform
+    __device__ __forceinline__ void flush(const LightPassArgs& a, uint32_t n) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(a.photon_counter, n);
+        base = (uint32_t)__shfl((int)base, 0);
+        const uint32_t l = lane_id();
+        if (l < n) {
+            const uint32_t e = (head + l) & (kPhotonRing - 1u);
+            const uint32_t at = base + l;
+            if (at < a.capacity) {
+                a.out_pos[at] = r->pos[e];
+                a.out_dir[at] = r->dir[e];
+                a.out_tp[at] = r->tp[e];
+                a.out_uid[at] = r->uid[e];
+            }
+        }
+        head = (head + n) & (kPhotonRing - 1u);
+        count -= n;
+        __builtin_amdgcn_wave_barrier();
+    }
+    __device__ __forceinline__ void push(const LightPassArgs& a, bool store, float4 pos, float4 dir, float2 tp,
+                                         uint32_t uid) {
+        const uint64_t m = __ballot(store);
+        if (m == 0) return;
+        if (store) {
+            const uint32_t e = (head + count + lanes_below(m)) & (kPhotonRing - 1u);
+            r->pos[e] = pos;
+            r->dir[e] = dir;
+            r->tp[e] = tp;
+            r->uid[e] = uid;
+        }
+        count += (uint32_t)__builtin_popcountll(m);
+        __builtin_amdgcn_wave_barrier();
+        if (count >= 64u) flush(a, 64u);
+    }
+    __device__ __forceinline__ void drain(const LightPassArgs& a) {
+        while (count > 0) flush(a, count < 64u ? count : 64u);
+    }
+};
+
 // Adds per-lane event counts into a global Counters with one atomic per wave and field.
 __device__ __forceinline__ void flush_counters(Counters* dst, unsigned long long (&c)[kCounterFields]) {
 #pragma unroll
@@ -90,7 +144,9 @@ struct LightState {
 template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     __shared__ int2 stack_lds[kLdsStack * kBlock];
+    __shared__ PhotonRing rings[kBlock / 64];
     int2* my_stack = stack_lds + threadIdx.x;
+    PhotonQueue photons{&rings[threadIdx.x >> 6]};
     unsigned long long cnt[kCounterFields] = {0};
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
@@ -103,7 +159,6 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         // refill once at least refill_min lanes are idle (or none is live):
         // the refill code then runs for many lanes at once
         if ((uint32_t)__builtin_popcountll(__ballot(need)) < a.refill_min && __ballot(active) != 0) need = false;
-        uint64_t m;
         uint32_t slot = pool.draw(a.path_counter, need);
         if (need) {
             if (slot < a.n_paths) {
@@ -169,14 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
             }
         }
         bool store = alive && !bsdf_delta(b) && (st.last & EV_GLOSSY);
-        uint32_t at = wave_alloc(a.photon_counter, store, &m);
         if (kCount) cnt[C_STORED] += store;
-        if (store && at < a.capacity) {
-            a.out_pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
-            a.out_dir[at] = make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x);
-            a.out_tp[at] = make_float2(st.tp.y, st.tp.z);
-            a.out_uid[at] = st.path * 8u + (uint32_t)st.pl;
-        }
+        photons.push(a, store, make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm),
+                     make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x), make_float2(st.tp.y, st.tp.z),
+                     st.path * 8u + (uint32_t)st.pl);
         if (alive) {
             if (st.pl > sc.max_path_length || st.diffuse > 2) {
                 alive = false;
@@ -208,6 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         }
         active = alive;
     }
+    photons.drain(a);
     if (kCount) flush_counters(a.counters, cnt);
 }
 
@@ -265,7 +317,9 @@ struct MltRng {
 template <bool kCount>
 __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
     __shared__ int2 stack_lds[kLdsStack * kBlock];
+    __shared__ PhotonRing rings[kBlock / 64];
     int2* my_stack = stack_lds + threadIdx.x;
+    PhotonQueue photons{&rings[threadIdx.x >> 6]};
     unsigned long long cnt[kCounterFields] = {0};
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
@@ -376,16 +430,11 @@ __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
             }
         }
         bool store = alive && !bsdf_delta(b) && (st.last & EV_GLOSSY);
-        uint64_t m;
-        uint32_t at = wave_alloc(a.photon_counter, store, &m);
         if (kCount) cnt[C_STORED] += store;
         if (store) visible = true;
-        if (store && at < a.capacity) {
-            a.out_pos[at] = make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm);
-            a.out_dir[at] = make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x);
-            a.out_tp[at] = make_float2(st.tp.y, st.tp.z);
-            a.out_uid[at] = st.path * 8u + (uint32_t)st.pl;
-        }
+        photons.push(a, store, make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm),
+                     make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x), make_float2(st.tp.y, st.tp.z),
+                     st.path * 8u + (uint32_t)st.pl);
         if (alive) {
             if (st.pl > sc.max_path_length || st.diffuse > 2) {
                 alive = false;
@@ -415,6 +464,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
         }
         active = alive;
     }
+    photons.drain(a);
     // chain statistics: one atomic per wave and counter
     unsigned long long v3[3] = {n_visible, n_mut, n_acc};
 #pragma unroll
