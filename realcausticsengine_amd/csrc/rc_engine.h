@@ -23,7 +23,7 @@ enum { L_AREA = 0, L_SUN = 1, L_POINT = 2 };
 
 constexpr int kLeafMax = 8;
 constexpr int kBlock = 128;      // threads per block of the tracing kernels
-constexpr int kLdsStack = 12;    // traversal stack entries ({node ref, entry distance}) held in LDS per lane
+constexpr int kLdsStack = 8;     // traversal stack entries ({node ref, entry distance}) held in LDS per lane
 constexpr int kMaxDepth = 64;    // total stack capacity (LDS + private overflow)
 constexpr int kEmptyChild = (int)0x80000000;  // missing child of a 4-wide BVH node
 
