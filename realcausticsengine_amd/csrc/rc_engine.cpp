@@ -80,7 +80,7 @@ struct Engine {
     double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
     // launch shapes and queue / tile sizes (options, or chosen by rc_autotune)
-    int light_blocks_per_cu = 8, camera_blocks_per_cu = 8, gather_blocks_per_cu = 16;
+    int light_blocks_per_cu = 8, camera_blocks_per_cu = 8, gather_blocks_per_cu = 8;
     uint32_t path_chunk = 256;
     uint32_t light_refill = 1, camera_refill = 1;  // idle lanes a tracing wave waits for before it refills
     int count_events = 0;
@@ -118,7 +118,8 @@ struct Engine {
     int camera_qmc = 0;     // 1: camera jitter and first-bounce choice stratified across iterations
     DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
     DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
-    int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
+    int group_queries = 16;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
+    uint32_t gather_tile = 1024;  // grouped merge: queries per XCD tile of the order (0: one segment per XCD)
     // adaptive MLT light pass (option "mlt"): chains, their explicit states and counts
     int mlt = 0;
     uint32_t mlt_chains = 262144;
@@ -758,7 +759,7 @@ bool prepare(Engine& e) {
     if (gpu_tree && !gpu_bvh(e, (uint32_t)ntri)) return false;
     if (!e.image.alloc((size_t)e.W * e.H * 3)) return false;
     HIP_OK(hipMemset(e.image.p, 0, (size_t)e.W * e.H * 3 * sizeof(float)));
-    if (!e.counters.alloc(4) || !e.stats_dev.alloc(2)) return false;
+    if (!e.counters.alloc(kPathCounters) || !e.stats_dev.alloc(2)) return false;
     if (!e.guide.setup_scene(e.lights, e.compat_point, e.bbox.lo, e.bbox.hi)) return false;
     e.iterations_done = 0;
     e.prepared = true;
@@ -1024,10 +1025,11 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
     a.exact_merge = e.exact_merge;
     a.group_queries = e.group_queries;
+    a.gather_tile = e.gather_tile;
     a.camera_qmc = e.camera_qmc;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
-        launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        launch_fill_u32(e.counters.p, 0u, kPathCounters, e.stream);
         HIP_OK(hipEventRecord(e.ev[4], e.stream));
         launch_camera_trace(a, camera_blocks(e), e.stream);
         HIP_OK(hipGetLastError());
@@ -1131,7 +1133,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
-                                  (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc};
+                                  (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         e.graph_exec = nullptr;
@@ -1172,13 +1174,14 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.image = e.image.p;
         a.exact_merge = e.exact_merge;
         a.group_queries = e.group_queries;
+        a.gather_tile = e.gather_tile;
         a.camera_qmc = e.camera_qmc;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         hipGraph_t graph = nullptr;
         HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         build_photon_map(m, st);
-        launch_fill_u32(e.counters.p, 0u, 4, st);
+        launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
         launch_camera_trace(a, camera_blocks(e), st);
         a.q_order = query_order(e, a, e.q_capacity, st);  // count read from the device
         launch_gather(a, gather_blocks(e), st);
@@ -1344,6 +1347,9 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"group_queries", {4, 8, 16},
          [=](double v) { launch_owner().group_queries = (int)v; },
          [=]() { return (double)launch_owner().group_queries; }, G},
+        {"gather_tile", {1024, 256, 0},
+         [=](double v) { launch_owner().gather_tile = (uint32_t)v; },
+         [=]() { return (double)launch_owner().gather_tile; }, G},
     };
 }
 
@@ -1567,6 +1573,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.light_order = from.light_order;
     to.camera_qmc = from.camera_qmc;
     to.group_queries = from.group_queries;
+    to.gather_tile = from.gather_tile;
     to.mlt = from.mlt;
     to.mlt_chains = from.mlt_chains;
     to.output_path = from.output_path;
@@ -1703,7 +1710,8 @@ bool batch_prepare(Batch& b) {
     b.work.query_order = b.scenes[0]->query_order;
     b.work.light_order = b.scenes[0]->light_order;
     b.work.group_queries = b.scenes[0]->group_queries;
-    if (!b.work.counters.alloc(4) || !b.work.stats_dev.alloc(2)) return false;
+    b.work.gather_tile = b.scenes[0]->gather_tile;
+    if (!b.work.counters.alloc(kPathCounters) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
     std::vector<SceneView> sc(n);
@@ -1934,13 +1942,14 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.contrib = w.contrib.p;
     c.exact_merge = e0.exact_merge;
     c.group_queries = w.group_queries;
+    c.gather_tile = w.gather_tile;
     c.camera_qmc = e0.camera_qmc;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
     uint32_t nq = 0;
     for (int attempt = 0;; ++attempt) {
         c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
-        launch_fill_u32(w.counters.p, 0u, 4, st);
+        launch_fill_u32(w.counters.p, 0u, kPathCounters, st);
         if (counting && attempt > 0)
             launch_fill_u32(reinterpret_cast<uint32_t*>(w.stats_dev.p + 1), 0u, kCounterFields * 2, st);
         HIP_OK(hipEventRecord(w.ev[4], st));
@@ -2241,6 +2250,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
+    else if (k == "gather_tile") g->gather_tile = (uint32_t)std::max(0.0, std::min(v, 1048576.0));
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "gather_blocks_per_cu") g->gather_blocks_per_cu = std::max(1, std::min(128, (int)v));
@@ -2295,6 +2305,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "light_order") return g->light_order;
     if (k == "camera_qmc") return g->camera_qmc;
     if (k == "group_queries") return g->group_queries;
+    if (k == "gather_tile") return g->gather_tile;
     if (k == "mlt") return g->mlt;
     if (k == "mlt_chains") return g->mlt_chains;
     if (k == "mlt_visible") return g->mlt_last[0];  // counts of the last MLT light pass

@@ -168,6 +168,10 @@ struct QueryBuffers {
     uint32_t capacity;
 };
 
+// Path counters of a pass: [0] paths taken, [1] queries written, [2..3] pass
+// bookkeeping, [4..11] the grouped merge's per-XCD group counters.
+constexpr int kPathCounters = 12;
+
 struct CameraPassArgs {
     SceneView sc;
     PhotonMapView pm;
@@ -189,6 +193,8 @@ struct CameraPassArgs {
     int camera_qmc;           // 1: jitter and first-bounce choice stratified across iterations
     int group_queries;        // 4, 8 or 16: grouped merge of that many queries per wave (queries with the
                               // same cell window share photon loads); 0: one query per wave
+    uint32_t gather_tile;     // grouped merge: XCD x takes the tiles t = x mod 8 of this many queries of the
+                              // order; 0: XCD x takes the x-th contiguous eighth
 };
 
 // ---- launchers (rc_kernels.hip) ----

@@ -880,6 +880,40 @@ struct GroupQuery {
     float cdvm;
 };
 
+// What the merge needs of a query whose surface is a diffuse-only plastic
+// (Fresnel-microfacet lobe with reflectance 0, fast arithmetic): the lobe is
+// the constant R / pi with pdf |cos| / pi, so a photon needs only its cosine
+// against the shading normal and the same-hemisphere tests, and R / pi
+// multiplies the query's sum once, after the reduction.  k1 = 1 + camera
+// dVM x reverse pdf is the left part of the MIS denominator, rounded as
+// there.  Every query carries its position here; `fast` marks the ones that
+// take this form.
+struct DiffuseQuery {
+    F3 q;
+    F3 n;
+    F3 gn;
+    float side_wo, woz, k1;
+    int fast;
+};
+
+__device__ __forceinline__ float uniform_f(float x) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ F3 uniform_f3(F3 v) { return f3(uniform_f(v.x), uniform_f(v.y), uniform_f(v.z)); }
+
+// Wave-uniform copy of a DiffuseQuery read from LDS: scalar registers.
+__device__ __forceinline__ DiffuseQuery uniform_dq(const DiffuseQuery& d) {
+    DiffuseQuery u;
+    u.q = uniform_f3(d.q);
+    u.n = uniform_f3(d.n);
+    u.gn = uniform_f3(d.gn);
+    u.side_wo = uniform_f(d.side_wo);
+    u.woz = uniform_f(d.woz);
+    u.k1 = uniform_f(d.k1);
+    u.fast = __builtin_amdgcn_readfirstlane(d.fast);
+    return u;
+}
+
 // The cell window of a query: cells [lo, hi] per axis overlapping its ball.
 __device__ __forceinline__ void query_window(const PhotonMapView& pm, F3 q, int lo[3], int hi[3]) {
     float qq[3] = {q.x, q.y, q.z};
@@ -895,6 +929,9 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
     __shared__ uint32_t rfirst_lds[kGatherWaves][64];
     __shared__ uint32_t rbase_lds[kGatherWaves][64];
     __shared__ GroupQuery gq_lds[kGatherWaves][kQ];
+    __shared__ DiffuseQuery dq_lds[kGatherWaves][kQ];
+    // up to 4 queries' DiffuseQuery constants stay in scalar registers over a sub-run
+    constexpr bool kScalarQ = kQ <= 4;
     const uint32_t wv = threadIdx.x >> 6;
     const uint32_t lane = lane_id();
     unsigned long long cnt[kCounterFields] = {0};
@@ -910,10 +947,27 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
     const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
     const uint32_t nblk = (nq + kQ - 1) / kQ;
     const uint32_t xcd = blockIdx.x % kXcds;
-    const uint32_t seg = (nblk + kXcds - 1) / kXcds;
-    const uint32_t b_begin = min(nblk, xcd * seg), b_end = min(nblk, b_begin + seg);
+    // XCD x takes groups [b_begin, b_end) of the order, dealt to its waves in
+    // turn (gather_tile = 0), or the tiles t = x, x + 8, ... of `tile` groups
+    // each, taken one group at a time from the XCD's counter by whichever of its
+    // waves is free (the counters start at zero with the pass's path counters)
+    const bool dyn = a.gather_tile != 0;
+    const uint32_t tile = dyn ? max(1u, a.gather_tile / kQ) : (nblk + kXcds - 1) / kXcds;
+    const uint32_t b_begin = dyn ? 0u : min(nblk, xcd * tile);
+    const uint32_t b_end = dyn ? 0xffffffffu : min(nblk, b_begin + tile);
     const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
-    for (uint32_t bj = b_begin + (blockIdx.x / kXcds) * kGatherWaves + wv; bj < b_end; bj += local_waves) {
+    uint32_t* take = a.path_counter + 4 + xcd;
+    uint32_t k = (blockIdx.x / kXcds) * kGatherWaves + wv;
+    if (dyn) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(take, 1u);
+        k = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    }
+    for (;;) {
+        const uint32_t bj = dyn ? ((k / tile) * kXcds + xcd) * tile + k % tile : b_begin + k;
+        if (bj >= b_end || bj >= nblk) break;
+        uint32_t k_next = 0;
+        if (dyn && lane == 0) k_next = atomicAdd(take, 1u);
         const uint32_t j0 = bj * kQ;
         const uint32_t nj = min((uint32_t)kQ, nq - j0);
         // lane j < nj sets up query j of the group
@@ -933,6 +987,15 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
             bsdf_setup(g.b, mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
             g.mc = merge_const(g.b, f3(qg.x, qg.y, qg.z));
             gq_lds[wv][lane] = g;
+            DiffuseQuery d;
+            d.q = g.q;
+            d.n = g.b.fr.z;
+            d.gn = g.mc.gn;
+            d.side_wo = g.mc.side_wo;
+            d.woz = g.b.wo.z;
+            d.k1 = 1.0f + g.cdvm * g.mc.rd;
+            d.fast = kFast && g.b.kind == BK_FMG && g.mc.f == 0.0f;
+            dq_lds[wv][lane] = d;
             query_window(pm, g.q, lo, hi);
         }
         // sub-runs: query j starts one when its window (or scene) differs from query j - 1's
@@ -991,8 +1054,12 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                 }
             }
             F3 c[kQ];
+            DiffuseQuery sq[kQ];
 #pragma unroll
-            for (int jj = 0; jj < kQ; ++jj) c[jj] = f3s(0.0f);
+            for (int jj = 0; jj < kQ; ++jj) {
+                c[jj] = f3s(0.0f);
+                if (kScalarQ && (uint32_t)jj < nrun) sq[jj] = uniform_dq(dq_lds[wv][s0 + jj]);
+            }
             uint32_t next_r = 0, cur_first = 0, cur_base = 0;
             for (uint32_t sb = 0; sb < total; sb += 64) {
                 const uint32_t sp = sb + lane;
@@ -1018,15 +1085,27 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
 #pragma unroll
                 for (int jj = 0; jj < kQ; ++jj) {
                     if ((uint32_t)jj < nrun) {
-                        const GroupQuery& g = gq_lds[wv][s0 + jj];
-                        const bool acc = v && len2(g.q - f3(p.x, p.y, p.z)) < pm.r2;
+                        const DiffuseQuery& d = kScalarQ ? sq[jj] : dq_lds[wv][s0 + jj];
+                        const bool acc = v && len2(d.q - f3(p.x, p.y, p.z)) < pm.r2;
                         if (kCount) {
                             const uint64_t am = __ballot(acc);
                             if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
                         }
-                        if (acc) {
+                        if (kFast && (kScalarQ ? d.fast : __builtin_amdgcn_readfirstlane(d.fast))) {
+                            // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
+                            // the wrong side adds nothing (its term is 0 there)
+                            const F3 wi = f3(dt.x, dt.y, dt.z);
+                            const float wz = dot(wi, d.n);
+                            if (acc && !(dot(wi, d.gn) * d.side_wo <= 0.0f) && !(wz * d.woz <= 0.0f)) {
+                                const float w = __builtin_amdgcn_rcpf(d.k1 + p.w * (fabsf(wz) * kInvPI));
+                                c[jj] = c[jj] + f3(dt.w, gb.x, gb.y) * w;
+                            }
+                        } else if (acc) {
+                            const GroupQuery& g = gq_lds[wv][s0 + jj];
+                            Bsdf b = g.b;
+                            b.kind = __builtin_amdgcn_readfirstlane(b.kind);
                             float fwd, rev;
-                            F3 e = bsdf_eval_pdf<kFast>(g.b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
+                            F3 e = bsdf_eval_pdf<kFast>(b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
                             float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + p.w * fwd);
                             c[jj] = c[jj] + (e * f3(dt.w, gb.x, gb.y)) * w;
                         }
@@ -1045,6 +1124,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     }
                     const uint32_t qj = (uint32_t)__builtin_amdgcn_readlane((int)qi, (int)(s0 + jj));
                     if (lane == 0) {
+                        if (kFast && dq_lds[wv][s0 + jj].fast) r = gq_lds[wv][s0 + jj].mc.diffuse * r;
                         float4 t = a.q.tpn[qj];
                         F3 o = f3(t.x, t.y, t.z) * r;
                         uint32_t slot = __float_as_uint(a.q.sn[qj].w);
@@ -1057,6 +1137,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
             }
         }
         __builtin_amdgcn_wave_barrier();
+        k = dyn ? (uint32_t)__builtin_amdgcn_readfirstlane((int)k_next) : k + local_waves;
     }
     if (kCount) flush_counters(a.counters, cnt);
 }

@@ -1,6 +1,6 @@
 """The merge in spatial query order (option query_order = 1, the default)
 against the merge in trace order, and the grouped merge (group_queries = 4,
-8 -- the default -- or 16 queries per wave) against the per-query merge.
+8 or 16 -- the default -- queries per wave) against the per-query merge.
 
 In either merge a query's sum is formed by one wave in an order fixed by the
 query's cell window alone (whichever queries share its wave), and the film
@@ -17,9 +17,10 @@ from realcausticsengine_amd.engine import Batch, Engine
 pytestmark = pytest.mark.gpu
 
 
-def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1):
+def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int = 1024):
     args = scenes.water_scene(128, 96, 80, samples=iters)
-    eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group)
+    eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group,
+                 gather_tile=tile)
     eng.prepare()
     films = []
     for _ in range(iters):
@@ -62,6 +63,18 @@ def test_grouped_merge_sizes_agree_bitwise():
     films = [_films(1, group=g)[0] for g in (4, 8, 16)]
     for f in films[1:]:
         for a, b in zip(films[0], f):
+            assert np.array_equal(a, b)
+
+
+def test_gather_tiles_keep_the_film():
+    """XCD tiles and the dealing of groups to waves change which wave merges a
+    group, not a query's sum: one contiguous segment per XCD in a fixed order
+    (tile 0) and tiles of 64 or 1024 queries taken as waves free up give the
+    same film, in a graph too."""
+    base = _films(1, group=16, tile=0)[0]
+    for tile, graph in ((64, 0), (1024, 0), (1024, 1)):
+        for a, b in zip(base, _films(1, group=16, tile=tile, graph=graph)[0]):
+            assert a.sum() > 0
             assert np.array_equal(a, b)
 
 
