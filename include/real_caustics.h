@@ -96,14 +96,19 @@ int main(void);
  * faster).
  * Merge order: "query_order" (1 = the merge takes the queries sorted by the
  * Morton code of their photon-map cell window, the default; 0 = in trace
- * order); the film is the same either way.  "group_queries" (16 = the default):
+ * order); the film is the same either way.  "group_queries" (8 = the default):
  * queries per wave of the grouped merge, in which queries with the same cell
  * window share the photon loads (same terms as one query per wave, summed in
  * another order).  "gather_tile" (1024 = the default): the grouped merge deals
  * tiles of this many consecutive queries of the order to the eight XCDs in
  * turn, and each XCD's waves take its groups one at a time as they free up;
  * 0 = each XCD takes one contiguous eighth, dealt to its waves in a fixed
- * order.  The film is the same for every value.
+ * order.  The film is the same for every value.  "gather_block" (2 = the
+ * default): the grouped merge takes a sub-run's photons 64 x this many (1, 2
+ * or 4) at a time into registers and runs a scalar loop over the queries,
+ * whose per-lane sums wait in LDS; 0 = the query loop unrolled, sums in
+ * registers.  Each query adds its terms in the same order either way: the
+ * film is the same.
  * Read-only: "mlt_visible", "mlt_mutations", "mlt_accepted" (counts of the
  * last MLT light pass), "bvh_depth", "bvh_build_ms" (build time of the last rc_prepare),
  * "nodes", "nodes4", "node_bytes" (bytes of the node layout traversed).
@@ -169,15 +174,17 @@ enum {
  * once), "light_order" (1 = trace the light paths sorted by their emission
  * samples, for ray coherence; the film is the same), "query_order" and
  * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave),
- * "gather_tile" (0, 256 or 1024 queries per XCD tile of the merge).
+ * "gather_tile" (0, 256 or 1024 queries per XCD tile of the merge),
+ * "gather_block" (0, 1, 2 or 4 photon steps per block of the merge).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects.  report (up to RC_TUNE_REPORT doubles): kernel ms per
  * iteration before and after, then path_chunk, light_blocks_per_cu,
  * camera_blocks_per_cu, cell_scale, gather_blocks_per_cu, light_refill,
- * light_order, camera_refill, query_order, group_queries, gather_tile. */
-enum { RC_TUNE_REPORT = 13 };
+ * light_order, camera_refill, group_queries, gather_tile, gather_block,
+ * query_order. */
+enum { RC_TUNE_REPORT = 14 };
 /* Option "graph" = 1: after the light pass (a direct launch whose photon
  * count is checked on the host), the photon-map build and the camera pass of
  * each iteration replay one captured hipGraph, reading counts and

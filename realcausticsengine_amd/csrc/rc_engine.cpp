@@ -118,8 +118,9 @@ struct Engine {
     int camera_qmc = 0;     // 1: camera jitter and first-bounce choice stratified across iterations
     DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
     DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
-    int group_queries = 16;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
+    int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
     uint32_t gather_tile = 1024;  // grouped merge: queries per XCD tile of the order (0: one segment per XCD)
+    int gather_block = 2;  // grouped merge: 1 / 2 / 4 = rolled query loop over 64 x this many photons, 0 = unrolled
     // adaptive MLT light pass (option "mlt"): chains, their explicit states and counts
     int mlt = 0;
     uint32_t mlt_chains = 262144;
@@ -1026,6 +1027,7 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.exact_merge = e.exact_merge;
     a.group_queries = e.group_queries;
     a.gather_tile = e.gather_tile;
+    a.gather_block = e.gather_block;
     a.camera_qmc = e.camera_qmc;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
@@ -1133,7 +1135,8 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
-                                  (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile};
+                                  (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile,
+                                  (uintptr_t)e.gather_block};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         e.graph_exec = nullptr;
@@ -1175,6 +1178,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.exact_merge = e.exact_merge;
         a.group_queries = e.group_queries;
         a.gather_tile = e.gather_tile;
+        a.gather_block = e.gather_block;
         a.camera_qmc = e.camera_qmc;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
@@ -1328,7 +1332,7 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
          [=]() { return (double)launch_owner().camera_blocks_per_cu; }, T},
         {"cell_scale", {0.67, 0.8, 1.0, 1.25}, set_cell_scale, get_cell_scale,
          [](const double* t) { return t[T_MAP] + t[T_GATHER]; }},
-        {"gather_blocks_per_cu", {8, 12, 16, 24, 32},
+        {"gather_blocks_per_cu", {5, 8, 12, 16, 24},
          [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
          [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
         {"light_refill", {1, 8, 16, 32},
@@ -1340,16 +1344,19 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"camera_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().camera_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().camera_refill; }, T},
-        {"query_order", {1, 0},
-         [=](double v) { launch_owner().query_order = (int)v; },
-         [=]() { return (double)launch_owner().query_order; },
-         [](const double* t) { return t[T_TRACE] + t[T_GATHER]; }},
         {"group_queries", {4, 8, 16},
          [=](double v) { launch_owner().group_queries = (int)v; },
          [=]() { return (double)launch_owner().group_queries; }, G},
         {"gather_tile", {1024, 256, 0},
          [=](double v) { launch_owner().gather_tile = (uint32_t)v; },
          [=]() { return (double)launch_owner().gather_tile; }, G},
+        {"gather_block", {2, 4, 1, 0},
+         [=](double v) { launch_owner().gather_block = (int)v; },
+         [=]() { return (double)launch_owner().gather_block; }, G},
+        {"query_order", {1, 0},
+         [=](double v) { launch_owner().query_order = (int)v; },
+         [=]() { return (double)launch_owner().query_order; },
+         [](const double* t) { return t[T_TRACE] + t[T_GATHER]; }},
     };
 }
 
@@ -1574,6 +1581,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.camera_qmc = from.camera_qmc;
     to.group_queries = from.group_queries;
     to.gather_tile = from.gather_tile;
+    to.gather_block = from.gather_block;
     to.mlt = from.mlt;
     to.mlt_chains = from.mlt_chains;
     to.output_path = from.output_path;
@@ -1711,6 +1719,7 @@ bool batch_prepare(Batch& b) {
     b.work.light_order = b.scenes[0]->light_order;
     b.work.group_queries = b.scenes[0]->group_queries;
     b.work.gather_tile = b.scenes[0]->gather_tile;
+    b.work.gather_block = b.scenes[0]->gather_block;
     if (!b.work.counters.alloc(kPathCounters) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
@@ -1943,6 +1952,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.exact_merge = e0.exact_merge;
     c.group_queries = w.group_queries;
     c.gather_tile = w.gather_tile;
+    c.gather_block = w.gather_block;
     c.camera_qmc = e0.camera_qmc;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
@@ -2250,6 +2260,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
+    else if (k == "gather_block") g->gather_block = (int)v >= 4 ? 4 : (int)v >= 2 ? 2 : (int)v >= 1 ? 1 : 0;
     else if (k == "gather_tile") g->gather_tile = (uint32_t)std::max(0.0, std::min(v, 1048576.0));
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
     else if (k == "camera_blocks_per_cu") g->camera_blocks_per_cu = std::max(1, std::min(64, (int)v));
@@ -2306,6 +2317,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "camera_qmc") return g->camera_qmc;
     if (k == "group_queries") return g->group_queries;
     if (k == "gather_tile") return g->gather_tile;
+    if (k == "gather_block") return g->gather_block;
     if (k == "mlt") return g->mlt;
     if (k == "mlt_chains") return g->mlt_chains;
     if (k == "mlt_visible") return g->mlt_last[0];  // counts of the last MLT light pass

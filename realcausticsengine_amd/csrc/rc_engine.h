@@ -195,6 +195,8 @@ struct CameraPassArgs {
                               // same cell window share photon loads); 0: one query per wave
     uint32_t gather_tile;     // grouped merge: XCD x takes the tiles t = x mod 8 of this many queries of the
                               // order; 0: XCD x takes the x-th contiguous eighth
+    int gather_block;         // grouped merge: 2 or 4 = photons taken 64 x this many at a time and the query
+                              // loop rolled (sums in LDS); 0 = the query loop unrolled (sums in registers)
 };
 
 // ---- launchers (rc_kernels.hip) ----

@@ -886,32 +886,23 @@ struct GroupQuery {
 // against the shading normal and the same-hemisphere tests, and R / pi
 // multiplies the query's sum once, after the reduction.  k1 = 1 + camera
 // dVM x reverse pdf is the left part of the MIS denominator, rounded as
-// there.  Every query carries its position here; `fast` marks the ones that
-// take this form.
+// there.  Every query carries its position here; a per-group bit mask marks
+// the ones that take this form.  Three float4 (one LDS read each):
+// {q, k1}, {shading normal, side_wo}, {geometric normal, wo.z}.
 struct DiffuseQuery {
-    F3 q;
-    F3 n;
-    F3 gn;
-    float side_wo, woz, k1;
-    int fast;
+    float4 qk, ns, gw;
 };
 
 __device__ __forceinline__ float uniform_f(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
 }
-__device__ __forceinline__ F3 uniform_f3(F3 v) { return f3(uniform_f(v.x), uniform_f(v.y), uniform_f(v.z)); }
 
+__device__ __forceinline__ float4 uniform_f4(float4 v) {
+    return make_float4(uniform_f(v.x), uniform_f(v.y), uniform_f(v.z), uniform_f(v.w));
+}
 // Wave-uniform copy of a DiffuseQuery read from LDS: scalar registers.
 __device__ __forceinline__ DiffuseQuery uniform_dq(const DiffuseQuery& d) {
-    DiffuseQuery u;
-    u.q = uniform_f3(d.q);
-    u.n = uniform_f3(d.n);
-    u.gn = uniform_f3(d.gn);
-    u.side_wo = uniform_f(d.side_wo);
-    u.woz = uniform_f(d.woz);
-    u.k1 = uniform_f(d.k1);
-    u.fast = __builtin_amdgcn_readfirstlane(d.fast);
-    return u;
+    return DiffuseQuery{uniform_f4(d.qk), uniform_f4(d.ns), uniform_f4(d.gw)};
 }
 
 // The cell window of a query: cells [lo, hi] per axis overlapping its ball.
@@ -924,12 +915,19 @@ __device__ __forceinline__ void query_window(const PhotonMapView& pm, F3 q, int 
     }
 }
 
-template <bool kCount, bool kFast, bool kBatch, int kQ>
+// kB = 0: the query loop is unrolled and each query's per-lane sums are
+// registers.  kB > 0: the sub-run's photons are taken kB steps (64 kB
+// photons) at a time into registers and the query loop is a scalar loop over
+// them; each query's per-lane sums live in LDS between blocks.  A lane adds a
+// query's terms in the same order either way (photon sb + lane, sb rising), so
+// the two give the same bits.
+template <bool kCount, bool kFast, bool kBatch, int kQ, int kB>
 __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs a) {
     __shared__ uint32_t rfirst_lds[kGatherWaves][64];
     __shared__ uint32_t rbase_lds[kGatherWaves][64];
     __shared__ GroupQuery gq_lds[kGatherWaves][kQ];
     __shared__ DiffuseQuery dq_lds[kGatherWaves][kQ];
+    __shared__ float acc_lds[kGatherWaves][kB > 0 ? 3 * kQ : 1][64];
     // up to 4 queries' DiffuseQuery constants stay in scalar registers over a sub-run
     constexpr bool kScalarQ = kQ <= 4;
     const uint32_t wv = threadIdx.x >> 6;
@@ -956,28 +954,36 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
     const uint32_t b_begin = dyn ? 0u : min(nblk, xcd * tile);
     const uint32_t b_end = dyn ? 0xffffffffu : min(nblk, b_begin + tile);
     const uint32_t local_waves = (gridDim.x / kXcds) * kGatherWaves;
+    // a wave takes `grab` consecutive groups per atomic on its XCD's counter
+    // (about eight takes per wave), so that the counter is not the critical
+    // path when groups are short
     uint32_t* take = a.path_counter + 4 + xcd;
-    uint32_t k = (blockIdx.x / kXcds) * kGatherWaves + wv;
-    if (dyn) {
+    const uint32_t grab = max(1u, nblk / (kXcds * local_waves * 8u));
+    auto take_groups = [&]() -> uint32_t {
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(take, 1u);
-        k = (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
-    }
+        if (lane == 0) t = atomicAdd(take, grab);
+        return (uint32_t)__builtin_amdgcn_readfirstlane((int)t);
+    };
+    uint32_t k = dyn ? take_groups() : (blockIdx.x / kXcds) * kGatherWaves + wv;
+    uint32_t left = grab;
     for (;;) {
         const uint32_t bj = dyn ? ((k / tile) * kXcds + xcd) * tile + k % tile : b_begin + k;
         if (bj >= b_end || bj >= nblk) break;
-        uint32_t k_next = 0;
-        if (dyn && lane == 0) k_next = atomicAdd(take, 1u);
         const uint32_t j0 = bj * kQ;
         const uint32_t nj = min((uint32_t)kQ, nq - j0);
         // lane j < nj sets up query j of the group
         const bool mine = lane < nj;
-        uint32_t qi = 0;
+        uint32_t qi = 0, slot = 0;
+        F3 tpn = f3s(0.0f);
         int scene = -1;
+        bool fast = false;
         int lo[3] = {0, 0, 0}, hi[3] = {-1, -1, -1};
         if (mine) {
             qi = a.q_order ? a.q_order[j0 + lane] : j0 + lane;
             float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi];
+            const float4 t = a.q.tpn[qi];
+            tpn = f3(t.x, t.y, t.z);
+            slot = __float_as_uint(qs.w);
             scene = kBatch ? __float_as_int(qg.w) : 0;
             const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : pm0;
             const DevMaterial* mats = kBatch ? a.batch.sc[scene].mats : a.sc.mats;
@@ -988,16 +994,15 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
             g.mc = merge_const(g.b, f3(qg.x, qg.y, qg.z));
             gq_lds[wv][lane] = g;
             DiffuseQuery d;
-            d.q = g.q;
-            d.n = g.b.fr.z;
-            d.gn = g.mc.gn;
-            d.side_wo = g.mc.side_wo;
-            d.woz = g.b.wo.z;
-            d.k1 = 1.0f + g.cdvm * g.mc.rd;
-            d.fast = kFast && g.b.kind == BK_FMG && g.mc.f == 0.0f;
+            d.qk = make_float4(g.q.x, g.q.y, g.q.z, 1.0f + g.cdvm * g.mc.rd);
+            d.ns = make_float4(g.b.fr.z.x, g.b.fr.z.y, g.b.fr.z.z, g.mc.side_wo);
+            d.gw = make_float4(g.mc.gn.x, g.mc.gn.y, g.mc.gn.z, g.b.wo.z);
             dq_lds[wv][lane] = d;
+            fast = kFast && g.b.kind == BK_FMG && g.mc.f == 0.0f;
             query_window(pm, g.q, lo, hi);
         }
+        // bit j: query j of the group takes the diffuse-only form
+        const uint64_t fastmask = __ballot(fast);
         // sub-runs: query j starts one when its window (or scene) differs from query j - 1's
         bool same = lane > 0;
         {
@@ -1053,91 +1058,204 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     cnt[C_STREAMED] += total;
                 }
             }
-            F3 c[kQ];
-            DiffuseQuery sq[kQ];
-#pragma unroll
-            for (int jj = 0; jj < kQ; ++jj) {
-                c[jj] = f3s(0.0f);
-                if (kScalarQ && (uint32_t)jj < nrun) sq[jj] = uniform_dq(dq_lds[wv][s0 + jj]);
-            }
-            uint32_t next_r = 0, cur_first = 0, cur_base = 0;
-            for (uint32_t sb = 0; sb < total; sb += 64) {
-                const uint32_t sp = sb + lane;
-                for (uint32_t r = next_r; r < nr; ++r) {
-                    uint32_t bnd = (uint32_t)__builtin_amdgcn_readlane((int)r_base, (int)r);
-                    if (bnd >= sb + 64u) break;
-                    uint32_t fst = (uint32_t)__builtin_amdgcn_readlane((int)r_first, (int)r);
-                    if (sp >= bnd) {
-                        cur_first = fst;
-                        cur_base = bnd;
-                    }
-                    if (bnd <= sb) next_r = r + 1;
+            if constexpr (kB > 0) {
+                for (uint32_t jj = 0; jj < nrun; ++jj) {
+                    acc_lds[wv][3 * jj][lane] = 0.0f;
+                    acc_lds[wv][3 * jj + 1][lane] = 0.0f;
+                    acc_lds[wv][3 * jj + 2][lane] = 0.0f;
                 }
-                const bool v = sp < total;
-                const uint32_t i = cur_first + (sp - cur_base);
-                float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dt = p;
-                float2 gb = make_float2(0.0f, 0.0f);
-                if (v) {
-                    p = pm.pos[i];
-                    dt = pm.dir_tr[i];
-                    gb = pm.tp_gb[i];
-                }
+                uint32_t next_r = 0, cur_first = 0, cur_base = 0;
+                for (uint32_t sb0 = 0; sb0 < total; sb0 += 64u * kB) {
+                    float4 P[kB], DT[kB];
+                    float2 GB[kB];
+                    bool V[kB];
 #pragma unroll
-                for (int jj = 0; jj < kQ; ++jj) {
-                    if ((uint32_t)jj < nrun) {
-                        const DiffuseQuery& d = kScalarQ ? sq[jj] : dq_lds[wv][s0 + jj];
-                        const bool acc = v && len2(d.q - f3(p.x, p.y, p.z)) < pm.r2;
-                        if (kCount) {
-                            const uint64_t am = __ballot(acc);
-                            if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
-                        }
-                        if (kFast && (kScalarQ ? d.fast : __builtin_amdgcn_readfirstlane(d.fast))) {
-                            // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
-                            // the wrong side adds nothing (its term is 0 there)
-                            const F3 wi = f3(dt.x, dt.y, dt.z);
-                            const float wz = dot(wi, d.n);
-                            if (acc && !(dot(wi, d.gn) * d.side_wo <= 0.0f) && !(wz * d.woz <= 0.0f)) {
-                                const float w = __builtin_amdgcn_rcpf(d.k1 + p.w * (fabsf(wz) * kInvPI));
-                                c[jj] = c[jj] + f3(dt.w, gb.x, gb.y) * w;
+                    for (int bb = 0; bb < kB; ++bb) {
+                        const uint32_t sb = sb0 + 64u * bb;
+                        const uint32_t sp = sb + lane;
+                        V[bb] = false;
+                        P[bb] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        DT[bb] = P[bb];
+                        GB[bb] = make_float2(0.0f, 0.0f);
+                        if (sb < total) {
+                            for (uint32_t r = next_r; r < nr; ++r) {
+                                uint32_t bnd = (uint32_t)__builtin_amdgcn_readlane((int)r_base, (int)r);
+                                if (bnd >= sb + 64u) break;
+                                uint32_t fst = (uint32_t)__builtin_amdgcn_readlane((int)r_first, (int)r);
+                                if (sp >= bnd) {
+                                    cur_first = fst;
+                                    cur_base = bnd;
+                                }
+                                if (bnd <= sb) next_r = r + 1;
                             }
-                        } else if (acc) {
-                            const GroupQuery& g = gq_lds[wv][s0 + jj];
+                            V[bb] = sp < total;
+                            const uint32_t i = cur_first + (sp - cur_base);
+                            if (V[bb]) {
+                                P[bb] = pm.pos[i];
+                                DT[bb] = pm.dir_tr[i];
+                                GB[bb] = pm.tp_gb[i];
+                            }
+                        }
+                    }
+                    for (uint32_t jj = 0; jj < nrun; ++jj) {
+                        const uint32_t qj = s0 + jj;
+                        F3 c = f3(acc_lds[wv][3 * jj][lane], acc_lds[wv][3 * jj + 1][lane], acc_lds[wv][3 * jj + 2][lane]);
+                        const DiffuseQuery d = dq_lds[wv][qj];
+                        const F3 dq = f3(d.qk.x, d.qk.y, d.qk.z);
+                        if (kFast && ((fastmask >> qj) & 1u)) {
+                            const F3 dn = f3(d.ns.x, d.ns.y, d.ns.z), dg = f3(d.gw.x, d.gw.y, d.gw.z);
+#pragma unroll
+                            for (int bb = 0; bb < kB; ++bb) {
+                                const bool acc = V[bb] && len2(dq - f3(P[bb].x, P[bb].y, P[bb].z)) < pm.r2;
+                                if (kCount) {
+                                    const uint64_t am = __ballot(acc);
+                                    if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                                }
+                                // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
+                                // the wrong side adds nothing (its term is 0 there)
+                                const F3 wi = f3(DT[bb].x, DT[bb].y, DT[bb].z);
+                                const float wz = dot(wi, dn);
+                                if (acc && !(dot(wi, dg) * d.ns.w <= 0.0f) && !(wz * d.gw.w <= 0.0f)) {
+                                    const float w = __builtin_amdgcn_rcpf(d.qk.w + P[bb].w * (fabsf(wz) * kInvPI));
+                                    c = c + f3(DT[bb].w, GB[bb].x, GB[bb].y) * w;
+                                }
+                            }
+                        } else {
+                            const GroupQuery& g = gq_lds[wv][qj];
                             Bsdf b = g.b;
                             b.kind = __builtin_amdgcn_readfirstlane(b.kind);
-                            float fwd, rev;
-                            F3 e = bsdf_eval_pdf<kFast>(b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
-                            float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + p.w * fwd);
-                            c[jj] = c[jj] + (e * f3(dt.w, gb.x, gb.y)) * w;
+#pragma unroll
+                            for (int bb = 0; bb < kB; ++bb) {
+                                const bool acc = V[bb] && len2(dq - f3(P[bb].x, P[bb].y, P[bb].z)) < pm.r2;
+                                if (kCount) {
+                                    const uint64_t am = __ballot(acc);
+                                    if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                                }
+                                if (acc) {
+                                    float fwd, rev;
+                                    F3 e = bsdf_eval_pdf<kFast>(b, g.mc, f3(DT[bb].x, DT[bb].y, DT[bb].z), fwd, rev);
+                                    float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + P[bb].w * fwd);
+                                    c = c + (e * f3(DT[bb].w, GB[bb].x, GB[bb].y)) * w;
+                                }
+                            }
                         }
+                        acc_lds[wv][3 * jj][lane] = c.x;
+                        acc_lds[wv][3 * jj + 1][lane] = c.y;
+                        acc_lds[wv][3 * jj + 2][lane] = c.z;
                     }
                 }
-            }
-            // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
-#pragma unroll
-            for (int jj = 0; jj < kQ; ++jj) {
-                if ((uint32_t)jj < nrun) {
-                    F3 r = c[jj];
+                // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
+                for (uint32_t jj = 0; jj < nrun; ++jj) {
+                    F3 r = f3(acc_lds[wv][3 * jj][lane], acc_lds[wv][3 * jj + 1][lane], acc_lds[wv][3 * jj + 2][lane]);
                     for (int off = 32; off > 0; off >>= 1) {
                         r.x += __shfl_xor(r.x, off);
                         r.y += __shfl_xor(r.y, off);
                         r.z += __shfl_xor(r.z, off);
                     }
-                    const uint32_t qj = (uint32_t)__builtin_amdgcn_readlane((int)qi, (int)(s0 + jj));
-                    if (lane == 0) {
-                        if (kFast && dq_lds[wv][s0 + jj].fast) r = gq_lds[wv][s0 + jj].mc.diffuse * r;
-                        float4 t = a.q.tpn[qj];
-                        F3 o = f3(t.x, t.y, t.z) * r;
-                        uint32_t slot = __float_as_uint(a.q.sn[qj].w);
+                    // every lane holds the same total: the query's own lane weights and writes it
+                    if (lane == s0 + jj) {
+                        if (kFast && ((fastmask >> lane) & 1u)) r = gq_lds[wv][lane].mc.diffuse * r;
+                        const F3 o = tpn * r;
                         a.contrib[3 * (size_t)slot] = o.x;
                         a.contrib[3 * (size_t)slot + 1] = o.y;
                         a.contrib[3 * (size_t)slot + 2] = o.z;
                         if (kCount) cnt[C_GATHERS]++;
                     }
                 }
+                __builtin_amdgcn_wave_barrier();
+            } else {
+                F3 c[kQ];
+                DiffuseQuery sq[kQ];
+#pragma unroll
+                for (int jj = 0; jj < kQ; ++jj) {
+                    c[jj] = f3s(0.0f);
+                    if (kScalarQ && (uint32_t)jj < nrun) sq[jj] = uniform_dq(dq_lds[wv][s0 + jj]);
+                }
+                uint32_t next_r = 0, cur_first = 0, cur_base = 0;
+                for (uint32_t sb = 0; sb < total; sb += 64) {
+                    const uint32_t sp = sb + lane;
+                    for (uint32_t r = next_r; r < nr; ++r) {
+                        uint32_t bnd = (uint32_t)__builtin_amdgcn_readlane((int)r_base, (int)r);
+                        if (bnd >= sb + 64u) break;
+                        uint32_t fst = (uint32_t)__builtin_amdgcn_readlane((int)r_first, (int)r);
+                        if (sp >= bnd) {
+                            cur_first = fst;
+                            cur_base = bnd;
+                        }
+                        if (bnd <= sb) next_r = r + 1;
+                    }
+                    const bool v = sp < total;
+                    const uint32_t i = cur_first + (sp - cur_base);
+                    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f), dt = p;
+                    float2 gb = make_float2(0.0f, 0.0f);
+                    if (v) {
+                        p = pm.pos[i];
+                        dt = pm.dir_tr[i];
+                        gb = pm.tp_gb[i];
+                    }
+#pragma unroll
+                    for (int jj = 0; jj < kQ; ++jj) {
+                        if ((uint32_t)jj < nrun) {
+                            const DiffuseQuery& d = kScalarQ ? sq[jj] : dq_lds[wv][s0 + jj];
+                            const bool acc = v && len2(f3(d.qk.x, d.qk.y, d.qk.z) - f3(p.x, p.y, p.z)) < pm.r2;
+                            if (kCount) {
+                                const uint64_t am = __ballot(acc);
+                                if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                            }
+                            if (kFast && ((fastmask >> (s0 + jj)) & 1u)) {
+                                // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
+                                // the wrong side adds nothing (its term is 0 there)
+                                const F3 wi = f3(dt.x, dt.y, dt.z);
+                                const float wz = dot(wi, f3(d.ns.x, d.ns.y, d.ns.z));
+                                if (acc && !(dot(wi, f3(d.gw.x, d.gw.y, d.gw.z)) * d.ns.w <= 0.0f) &&
+                                    !(wz * d.gw.w <= 0.0f)) {
+                                    const float w = __builtin_amdgcn_rcpf(d.qk.w + p.w * (fabsf(wz) * kInvPI));
+                                    c[jj] = c[jj] + f3(dt.w, gb.x, gb.y) * w;
+                                }
+                            } else if (acc) {
+                                const GroupQuery& g = gq_lds[wv][s0 + jj];
+                                Bsdf b = g.b;
+                                b.kind = __builtin_amdgcn_readfirstlane(b.kind);
+                                float fwd, rev;
+                                F3 e = bsdf_eval_pdf<kFast>(b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
+                                float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + p.w * fwd);
+                                c[jj] = c[jj] + (e * f3(dt.w, gb.x, gb.y)) * w;
+                            }
+                        }
+                    }
+                }
+                // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
+#pragma unroll
+                for (int jj = 0; jj < kQ; ++jj) {
+                    if ((uint32_t)jj < nrun) {
+                        F3 r = c[jj];
+                        for (int off = 32; off > 0; off >>= 1) {
+                            r.x += __shfl_xor(r.x, off);
+                            r.y += __shfl_xor(r.y, off);
+                            r.z += __shfl_xor(r.z, off);
+                        }
+                        // every lane holds the same total: the query's own lane weights and writes it
+                        if (lane == s0 + jj) {
+                            if (kFast && ((fastmask >> lane) & 1u)) r = gq_lds[wv][lane].mc.diffuse * r;
+                            const F3 o = tpn * r;
+                            a.contrib[3 * (size_t)slot] = o.x;
+                            a.contrib[3 * (size_t)slot + 1] = o.y;
+                            a.contrib[3 * (size_t)slot + 2] = o.z;
+                            if (kCount) cnt[C_GATHERS]++;
+                        }
+                    }
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();
-        k = dyn ? (uint32_t)__builtin_amdgcn_readfirstlane((int)k_next) : k + local_waves;
+        if (!dyn) {
+            k += local_waves;
+        } else if (--left == 0) {
+            k = take_groups();
+            left = grab;
+        } else {
+            ++k;
+        }
     }
     if (kCount) flush_counters(a.counters, cnt);
 }
@@ -1174,23 +1292,30 @@ void launch_camera_trace(const CameraPassArgs& a, int blocks, hipStream_t s) {
         else hipLaunchKernelGGL((k_camera_trace<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
     }
 }
-template <bool kBatch, int kQ>
+template <bool kBatch, int kQ, int kB>
 static void grouped_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
     dim3 g(blocks), b(kGatherBlock);
     if (a.exact_merge) {
-        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, false, kBatch, kQ>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_gather_grouped<false, false, kBatch, kQ>), g, b, 0, s, a);
+        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, false, kBatch, kQ, kB>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather_grouped<false, false, kBatch, kQ, kB>), g, b, 0, s, a);
     } else {
-        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, true, kBatch, kQ>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_gather_grouped<false, true, kBatch, kQ>), g, b, 0, s, a);
+        if (a.counters) hipLaunchKernelGGL((k_gather_grouped<true, true, kBatch, kQ, kB>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_gather_grouped<false, true, kBatch, kQ, kB>), g, b, 0, s, a);
     }
+}
+template <bool kBatch, int kQ>
+static void blocked_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
+    if (a.gather_block == 4) grouped_variant<kBatch, kQ, 4>(a, blocks, s);
+    else if (a.gather_block == 2) grouped_variant<kBatch, kQ, 2>(a, blocks, s);
+    else if (a.gather_block == 1) grouped_variant<kBatch, kQ, 1>(a, blocks, s);
+    else grouped_variant<kBatch, kQ, 0>(a, blocks, s);
 }
 template <bool kBatch>
 static void gather_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
     dim3 g(blocks), b(kGatherBlock);
-    if (a.group_queries == 4) grouped_variant<kBatch, 4>(a, blocks, s);
-    else if (a.group_queries == 16) grouped_variant<kBatch, 16>(a, blocks, s);
-    else if (a.group_queries) grouped_variant<kBatch, 8>(a, blocks, s);
+    if (a.group_queries == 4) blocked_variant<kBatch, 4>(a, blocks, s);
+    else if (a.group_queries == 16) blocked_variant<kBatch, 16>(a, blocks, s);
+    else if (a.group_queries) blocked_variant<kBatch, 8>(a, blocks, s);
     if (a.group_queries) return;
     if (a.exact_merge) {
         if (a.counters) hipLaunchKernelGGL((k_gather<true, false, kBatch>), g, b, 0, s, a);

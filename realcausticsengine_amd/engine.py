@@ -28,8 +28,8 @@ EXPORTS = [
     "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
-             "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "query_order", "group_queries",
-             "gather_tile"]
+             "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
+             "gather_block", "query_order"]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",

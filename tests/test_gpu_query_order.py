@@ -1,6 +1,6 @@
 """The merge in spatial query order (option query_order = 1, the default)
 against the merge in trace order, and the grouped merge (group_queries = 4,
-8 or 16 -- the default -- queries per wave) against the per-query merge.
+8 -- the default -- or 16 queries per wave) against the per-query merge.
 
 In either merge a query's sum is formed by one wave in an order fixed by the
 query's cell window alone (whichever queries share its wave), and the film
@@ -17,10 +17,11 @@ from realcausticsengine_amd.engine import Batch, Engine
 pytestmark = pytest.mark.gpu
 
 
-def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int = 1024):
+def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int = 1024, block: int = 0,
+           exact: int = 0):
     args = scenes.water_scene(128, 96, 80, samples=iters)
     eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group,
-                 gather_tile=tile)
+                 gather_tile=tile, gather_block=block, exact_merge=exact)
     eng.prepare()
     films = []
     for _ in range(iters):
@@ -78,18 +79,33 @@ def test_gather_tiles_keep_the_film():
             assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("exact", [0, 1])
+def test_rolled_query_loop_keeps_the_film(exact):
+    """The rolled query loop over 2 or 4 photon steps at a time (sums in LDS)
+    adds each query's terms in the unrolled loop's order: the same film, for
+    every group size, with fast and exact merge arithmetic."""
+    for group in (4, 8, 16):
+        base = _films(1, group=group, exact=exact)[0]
+        for block in (2, 4):
+            for a, b in zip(base, _films(1, group=group, block=block, exact=exact)[0]):
+                assert a.sum() > 0
+                assert np.array_equal(a, b), (group, block)
+
+
 def test_grouped_merge_counts_match_per_query_merge():
     args = scenes.water_scene(128, 96, 80, samples=1)
     st = []
-    for group in (0, 8):
-        eng = Engine(args, photons=200000, seed=4, write_exr=0, count_events=1, group_queries=group)
+    for group, block in ((0, 0), (8, 0), (8, 4)):
+        eng = Engine(args, photons=200000, seed=4, write_exr=0, count_events=1, group_queries=group,
+                     gather_block=block)
         eng.prepare()
         eng.render_iteration()
         st.append(eng.stats())
         eng.release()
     for k in ("gathers", "photons_tested", "photons_accepted"):
-        assert st[0][k] == st[1][k], k
+        assert st[0][k] == st[1][k] == st[2][k], k
     assert st[1]["cell_visits"] <= st[0]["cell_visits"]  # one window lookup per sub-run of a group
+    assert st[2]["cell_visits"] == st[1]["cell_visits"] and st[2]["photons_streamed"] == st[1]["photons_streamed"]
 
 
 def test_sorted_merge_in_batch_matches_trace_order():
