@@ -153,13 +153,20 @@ def at_scale(a) -> int:
     64 x 2^20 light paths per scene and iteration, one learned proposal per
     scene trained every iteration, trees built on the GPU).  The scenes run in
     batches of --scale-group (a batch's light paths index 32-bit photon ids:
-    at most 2^29 per launch), one batch resident at a time; prints one JSON line."""
+    at most 2^29 per launch), one batch resident at a time; each batch's launch
+    shapes are autotuned (untimed) unless --launch fixes them; prints one JSON
+    line."""
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import GUIDE_STEP_FLOPS_PER_ROW, Batch
 
     n, group = a.at_scale, a.scale_group
     opts = dict(write_exr=0, bvh_build=1, guide=1, guide_batch=65536, guide_warmup=1, compat_point_light=0,
                 compat_light_pdf=0, light_order=1)  # coherent light rays: -22 % light pass on these scenes
+    fixed = {}
+    for kv in filter(None, a.launch.split(",")):  # fixed launch shapes (every scene; scene 0's serve the batch)
+        k, v = kv.split("=")
+        fixed[k.strip()] = float(v)
+    opts.update(fixed)
     groups, totals = [], dict(light=0, camera=0, tris=0, s_iter=0.0, ms_guide=0.0, guide_rows=0.0, prep=0.0)
     for g0 in range(0, n, group):
         t = time.perf_counter()
@@ -173,6 +180,11 @@ def at_scale(a) -> int:
         print(f"[bench] at scale: scenes {g0}..{g0 + len(cases) - 1}, {tris} triangles, generated in {gen_s:.1f} s, "
               f"prepared in {prep_s:.1f} s", file=sys.stderr, flush=True)
         b.render_iteration()  # warm-up (buffers; the proposals' first step)
+        tuned = {"fixed": a.launch} if fixed else None
+        if not fixed and not a.no_autotune:  # launch shapes for this batch (untimed; renders into scratch films)
+            t = time.perf_counter()
+            tuned = {k: round(v, 3) for k, v in b.autotune(repeats=1).items()}
+            print(f"[bench]   batch autotune ({time.perf_counter() - t:.1f} s): {tuned}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ms_guide = rows = 0.0
         for _ in range(a.scale_steps):
@@ -199,7 +211,8 @@ def at_scale(a) -> int:
                           round(cst["photons_tested"] / max(1, cst["gathers"]), 1),
                           "algorithmic_gb_per_s": round(alg["gather"] / (st["ms_gather"] * 1e-3) / 1e9, 1)},
                "light_pass_algorithmic_gb_per_s": round(alg["light_pass"] / (st["ms_light"] * 1e-3) / 1e9, 1),
-               "ms_guide_per_iteration": round(ms_guide / a.scale_steps, 3), "prepare_s": round(prep_s, 2)}
+               "ms_guide_per_iteration": round(ms_guide / a.scale_steps, 3), "prepare_s": round(prep_s, 2),
+               "autotune": tuned}
         print(f"[bench]   {rec}", file=sys.stderr, flush=True)
         groups.append(rec)
         totals["light"] += light
@@ -355,6 +368,22 @@ def time_to_rmse(a, cpu_s_per_iteration):
     return out
 
 
+def copy_calibration() -> int:
+    """Three copies of 1 GiB between device buffers: what FETCH_SIZE and
+    WRITE_SIZE read for a known number of bytes moved."""
+    import torch
+
+    n = 1 << 28
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copies": 3, "bytes_per_copy": n * 4}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,11 +414,14 @@ def main() -> int:
     ap.add_argument("--scale-photons", type=int, default=64 << 20)
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run three 1 GiB device-to-device copies (the reference point of the HBM counters)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one MI355X: the path has no multi-GPU mode in this build "
-              "(sharding is described in DESIGN.md); run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one MI355X: the engine renders on one GPU; run with --gpus 1", file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        return copy_calibration()
     if a.guide_steps > 0:
         return guide_steps(a)
     if a.at_scale > 0:

@@ -1715,6 +1715,13 @@ bool batch_prepare(Batch& b) {
     for (auto& t : pool) t.join();
     if (!ok) return false;
     if (!ensure_device(b.work)) return false;
+    // one launch per pass serves every scene: its shapes are scene 0's options
+    b.work.path_chunk = b.scenes[0]->path_chunk;
+    b.work.light_blocks_per_cu = b.scenes[0]->light_blocks_per_cu;
+    b.work.camera_blocks_per_cu = b.scenes[0]->camera_blocks_per_cu;
+    b.work.gather_blocks_per_cu = b.scenes[0]->gather_blocks_per_cu;
+    b.work.light_refill = b.scenes[0]->light_refill;
+    b.work.camera_refill = b.scenes[0]->camera_refill;
     b.work.query_order = b.scenes[0]->query_order;
     b.work.light_order = b.scenes[0]->light_order;
     b.work.group_queries = b.scenes[0]->group_queries;
