@@ -868,7 +868,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
 // once per sub-run instead of once per query.  Per-query state (BSDF frame,
 // merge constants) sits in LDS and is read as broadcasts; the per-lane
 // partial sums of the sub-run's queries are registers (the query loop is
-// unrolled), reduced with the fixed butterfly when the sub-run ends.  A
+// unrolled), summed over the lanes in a fixed order when the sub-run ends.  A
 // query's sum depends only on its window's photon stream (photon s goes to
 // lane s mod 64), not on which queries share its wave; the terms are the
 // per-query kernel's, added in another order.
@@ -892,6 +892,27 @@ struct GroupQuery {
 struct DiffuseQuery {
     float4 qk, ns, gw;
 };
+
+// Sum of a float over the 64 lanes of a wave, the same bits in every lane, in
+// a fixed order: pairs and quads (quad permutes), eights (half-row mirror),
+// rows of 16 (row mirror) -- each step adds a lane's value and its partner's,
+// so partners agree bit for bit -- then the four row sums read from lanes 15,
+// 31, 47 and 63 and added as (r0 + r1) + (r2 + r3).  Every lane must be active.
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), kCtrl, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float wave_total(float v) {
+    v += dpp_f<0xB1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_f<0x4E>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_f<0x141>(v);  // row_half_mirror
+    v += dpp_f<0x140>(v);  // row_mirror
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    return (r0 + r1) + (r2 + r3);
+}
 
 __device__ __forceinline__ float uniform_f(float x) {
     return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
@@ -1144,14 +1165,10 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                         acc_lds[wv][3 * jj + 2][lane] = c.z;
                     }
                 }
-                // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
+                // the sub-run's queries: fixed-order sum over the lanes, then the weighted result into its slot
                 for (uint32_t jj = 0; jj < nrun; ++jj) {
-                    F3 r = f3(acc_lds[wv][3 * jj][lane], acc_lds[wv][3 * jj + 1][lane], acc_lds[wv][3 * jj + 2][lane]);
-                    for (int off = 32; off > 0; off >>= 1) {
-                        r.x += __shfl_xor(r.x, off);
-                        r.y += __shfl_xor(r.y, off);
-                        r.z += __shfl_xor(r.z, off);
-                    }
+                    F3 r = f3(wave_total(acc_lds[wv][3 * jj][lane]), wave_total(acc_lds[wv][3 * jj + 1][lane]),
+                              wave_total(acc_lds[wv][3 * jj + 2][lane]));
                     // every lane holds the same total: the query's own lane weights and writes it
                     if (lane == s0 + jj) {
                         if (kFast && ((fastmask >> lane) & 1u)) r = gq_lds[wv][lane].mc.diffuse * r;
@@ -1224,16 +1241,11 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                         }
                     }
                 }
-                // the sub-run's queries: fixed butterfly over the lanes, then the weighted result into its slot
+                // the sub-run's queries: fixed-order sum over the lanes, then the weighted result into its slot
 #pragma unroll
                 for (int jj = 0; jj < kQ; ++jj) {
                     if ((uint32_t)jj < nrun) {
-                        F3 r = c[jj];
-                        for (int off = 32; off > 0; off >>= 1) {
-                            r.x += __shfl_xor(r.x, off);
-                            r.y += __shfl_xor(r.y, off);
-                            r.z += __shfl_xor(r.z, off);
-                        }
+                        F3 r = f3(wave_total(c[jj].x), wave_total(c[jj].y), wave_total(c[jj].z));
                         // every lane holds the same total: the query's own lane weights and writes it
                         if (lane == s0 + jj) {
                             if (kFast && ((fastmask >> lane) & 1u)) r = gq_lds[wv][lane].mc.diffuse * r;
