@@ -71,21 +71,40 @@ def profiled_traffic(kernel: str):
         return None
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(args, photons: int, stride: int) -> dict:
-    """The CPU oracle (reference-mode fp32 restatement, one core) on one
-    iteration of the same scene: the full light pass, its photon map, and the
-    camera pass over every `stride`-th pixel, extrapolated to all pixels."""
+    """The CPU oracle (reference-mode fp32 restatement) on one iteration of the
+    same scene: the full light pass, its photon map, and the camera pass over
+    every `stride`-th pixel, extrapolated to all pixels -- on one core (the
+    reference is single-threaded), then with its OpenMP passes on the host's
+    cores (at most 16, one GPU's share of the host), whose photon lists and
+    films equal the one-core run's."""
     from oracle import binding
 
-    o = binding.OracleScene(args, "f32", photons=photons, seed=0, camera_stride=stride)
-    o.prepare()
-    t0 = time.perf_counter()
-    ph = o.trace_light(0)
-    t_light = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    o.camera_pass(0, ph)
-    t_cam = time.perf_counter() - t1
-    total = t_light + stride * t_cam
+    def one(threads):
+        o = binding.OracleScene(args, "f32", photons=photons, seed=0, camera_stride=stride, threads=threads)
+        o.prepare()
+        t0 = time.perf_counter()
+        ph = o.trace_light(0)
+        t_light = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        o.camera_pass(0, ph)
+        t_cam = time.perf_counter() - t1
+        return t_light, t_cam, t_light + stride * t_cam
+
+    t_light, t_cam, total = one(1)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    tl_n, tc_n, total_n = one(threads)
     return {
         "value": photons / total,
         "unit": "photon paths/s",
@@ -93,6 +112,11 @@ def cpu_baseline(args, photons: int, stride: int) -> dict:
         "kind": "port",
         "sample": (f"oracle f32, 1 iteration: {photons} light paths ({t_light:.1f} s), KD photon map + camera "
                    f"pass over 1/{stride} of the pixels ({t_cam:.1f} s, x{stride} extrapolated)"),
+        "all_cores": {"value": photons / total_n, "cores": threads,
+                      "sample": f"the same iteration on {threads} OpenMP threads: light pass {tl_n:.2f} s, "
+                                f"camera pass over 1/{stride} of the pixels {tc_n:.2f} s"},
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
     }
 
 
@@ -290,7 +314,7 @@ def graph_leg(a):
     return out
 
 
-def time_to_rmse(a, cpu_s_per_iteration):
+def time_to_rmse(a, cpu_s_per_iteration, cpu_all_s_per_iteration=None):
     """Wall time of rc_render_iteration calls until the running image reaches
     RMSE <= target against the FP64 reference (readback and scoring untimed)."""
     from realcausticsengine_amd import convergence, scenes
@@ -354,6 +378,8 @@ def time_to_rmse(a, cpu_s_per_iteration):
            "legs": legs}
     if cpu_s_per_iteration and it_plain:
         out["cpu_reference_mode_estimate_s"] = round(cpu_s_per_iteration * it_plain, 1)
+    if cpu_all_s_per_iteration and it_plain:
+        out["cpu_all_cores_estimate_s"] = round(cpu_all_s_per_iteration * it_plain, 1)
 
     def secs(leg):
         return leg["seconds"] if leg["seconds"] is not None else leg.get("extrapolated_seconds")
@@ -365,6 +391,8 @@ def time_to_rmse(a, cpu_s_per_iteration):
         if "cpu_reference_mode_estimate_s" in out:
             best = min(secs(v) for v in legs.values() if isinstance(v, dict) and "iterations_run" in v and secs(v))
             out["speedup_best_vs_cpu"] = round(out["cpu_reference_mode_estimate_s"] / best, 1)
+            if "cpu_all_cores_estimate_s" in out:
+                out["speedup_best_vs_cpu_all_cores"] = round(out["cpu_all_cores_estimate_s"] / best, 1)
     return out
 
 
@@ -497,7 +525,8 @@ def main() -> int:
     graph = None if a.no_graph else graph_leg(a)
     ttr = None
     if not a.no_ttr:
-        ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None)
+        ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None,
+                           a.photons / cpu["all_cores"]["value"] if cpu else None)
     value = a.photons / (ms_step * 1e-3)
     line = {
         "metric": "time-to-RMSE<=0.01 vs FP64 (guided vs plain) + photon paths/s, 1 MI355X",

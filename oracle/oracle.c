@@ -35,6 +35,9 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #ifdef ORC_DOUBLE
 typedef double real;
@@ -961,6 +964,7 @@ typedef struct {
     int guide_cells, guide_slot[GUIDE_MAX_LIGHTS];
     real guide_alpha;
     uint32_t guide_words[4];    /* this path's {dim 3, dim 4, mixture, bin} words */
+    int threads;                /* > 1: plain light and camera passes on this many OpenMP threads */
 } Scene;
 
 void *orc_create(int samples, float base_radius, float radius_reduction, const long long *meshes_pointers,
@@ -1096,6 +1100,7 @@ int orc_set_option(void *sp, const char *key, double v) {
     else if (!strcmp(key, "camera_stride")) s->camera_stride = v < 1 ? 1 : (int)v;
     else if (!strcmp(key, "max_path_length")) s->max_path_length = (int)v;
     else if (!strcmp(key, "gather")) s->gather_brute = (int)v;
+    else if (!strcmp(key, "threads")) s->threads = v < 1 ? 1 : (int)v;
     else return -1;
     return 0;
 }
@@ -1592,6 +1597,54 @@ static void light_pass(Scene *s, int iteration, PList *out) {
         light_pass_mlt_chains(s, iteration, out);
         return;
     }
+#ifdef _OPENMP
+    if (!s->rng_serial && s->threads > 1 && !s->guide_cdf) {
+        /* Paths are independent (counter-based draws): thread t takes the
+         * contiguous range [N t / T, N (t + 1) / T) with its own photon list and
+         * counters on a shallow copy of the scene, and the lists are joined in
+         * thread order -- the serial loop's list, photon for photon. */
+        const int T = s->threads;
+        PList *lists = (PList *)calloc((size_t)T, sizeof(PList));
+        long long *vis = (long long *)calloc((size_t)T, sizeof(long long));
+        Counters *ctrs = (Counters *)calloc((size_t)T, sizeof(Counters));
+#pragma omp parallel num_threads(T)
+        {
+            const int t = omp_get_thread_num(), nt = omp_get_num_threads();
+            const long long lo = N * t / nt, hi = N * (t + 1) / nt;
+            Scene local = *s;
+            local.ctr.box_tests = local.ctr.tri_tests = 0;
+            Draws dr = {0, NULL, {{(uint32_t)s->seed, (uint32_t)(s->seed >> 32)}, 0, (uint32_t)iteration, STREAM_LIGHT}};
+            long long v = 0;
+            for (long long p = lo; p < hi; ++p) {
+                dr.key.path = (uint32_t)p;
+                Sampler smp;
+                sampler_plain(&smp, &dr.key);
+                v += trace_light_path(&local, &smp, &dr, (uint32_t)p, &lists[t]);
+            }
+            vis[t] = v;
+            ctrs[t] = local.ctr;
+        }
+        for (int t = 0; t < T; ++t) {
+            for (long long i = 0; i < lists[t].n; ++i) {
+                if (out->n == out->cap) {
+                    out->cap = out->cap ? 2 * out->cap : 4096;
+                    out->p = realloc(out->p, (size_t)out->cap * ORC_PHOTON_STRIDE * sizeof(double));
+                }
+                memcpy(out->p + (size_t)out->n * ORC_PHOTON_STRIDE, lists[t].p + (size_t)i * ORC_PHOTON_STRIDE,
+                       ORC_PHOTON_STRIDE * sizeof(double));
+                out->n++;
+            }
+            free(lists[t].p);
+            s->visible_paths += vis[t];
+            s->ctr.box_tests += ctrs[t].box_tests;
+            s->ctr.tri_tests += ctrs[t].tri_tests;
+        }
+        free(lists);
+        free(vis);
+        free(ctrs);
+        return;
+    }
+#endif
     if (!s->rng_serial) {
         Draws dr = {0, NULL, {{(uint32_t)s->seed, (uint32_t)(s->seed >> 32)}, 0, (uint32_t)iteration, STREAM_LIGHT}};
         for (long long p = 0; p < N; ++p) {
@@ -1857,99 +1910,164 @@ void orc_qmc_words(unsigned long long seed, unsigned int pixel, unsigned int ite
     for (int k = 0; k < 3; ++k) out3[k] = w[k];
 }
 
+/* What a camera path's merges read: the photon list, its positions and KD
+ * tree, the radius and the VM normalisation of the iteration. */
+typedef struct {
+    const double *ph;
+    const real *pos;
+    long long P;
+    const KD *kd;
+    real r2, norm;
+} GatherCtx;
+
+/* The camera path of pixel sample p (Vertex_Merging.cpp:309-363): its colour
+ * and the film element it adds to (*at = -1: none). */
+static void camera_path(Scene *s, int iteration, const GatherCtx *g, long long p, Draws *dr, Sampler *smp,
+                        IList *found, V *color_out, long long *at_out) {
+    const int W = s->W, H = s->H;
+    *at_out = -1;
+    real jx, jy;
+    if (s->rng_serial) {
+        (void)random_float_0_1(&s->xs); /* progress draw (:311) */
+        jx = random_float_0_1(&s->xs);  /* argument order unspecified in C++; taken left to right */
+        jy = random_float_0_1(&s->xs);
+        sampler_init_uniform(smp, &s->xs);
+    } else {
+        dr->key.path = (uint32_t)p;
+        jx = path_u(&dr->key, 0);
+        jy = path_u(&dr->key, 1);
+        sampler_plain(smp, &dr->key);
+        if (s->camera_qmc) {
+            uint32_t w[3];
+            qmc_camera_words(dr->key.key, (uint32_t)p, (uint32_t)iteration, w);
+            jx = u24(w[0]);
+            jy = u24(w[1]);
+            smp->value[0] = u24(w[2]); /* bounce 0, first dimension */
+        }
+    }
+    smp->index = 0;
+    real sx = (real)(int)(p % W) + K(0.0f) + K(1.0f) * jx;
+    real sy = (real)(int)(p / W) + K(0.0f) + K(1.0f) * jy;
+    /* Camera::get_ray (Camera.h:29-34) */
+    V dir = vadd(vadd(s->cam_bl, vscale(s->cam_h, sx / (real)W)), vscale(s->cam_v, sy / (real)H));
+    dir = vsub(dir, s->cam_o);
+    V o = s->cam_o, d = vnormalize(dir), tp = vs(1), color = vs(0);
+    real dvm = 0;
+    for (int pl = 0;; ++pl) {
+        Ray r = {vadd(o, vscale(d, K(0.001f))), d};
+        Isect h;
+        if (!scene_hit(s, &r, K(0.001f), &h)) break;
+        Bsdf b;
+        bsdf_init(&b, &s->mats[h.mat], h.sn, h.dir, 1);
+        if (!b.valid) break;
+        dvm /= (real)fabs(b.wo.z);
+        if (!bsdf_is_delta(&b)) {
+            /* Path_Vertices::get_contrib (Path_Vertices.h:46-68) */
+            found->n = 0;
+            if (g->P > 0) {
+                if (s->gather_brute) brute_query(g->pos, g->P, h.pos, g->r2, found);
+                else kd_query(g->kd, h.pos, g->r2, 1, found);
+            }
+            V c = vs(0);
+            for (long long i = 0; i < found->n; ++i) {
+                const double *pr = g->ph + (size_t)found->idx[i] * ORC_PHOTON_STRIDE;
+                V pdir = v3((real)pr[3], (real)pr[4], (real)pr[5]);
+                V ptp = v3((real)pr[6], (real)pr[7], (real)pr[8]);
+                real ldvm = (real)pr[9];
+                real fwd, rev;
+                bsdf_pdf(&b, pdir, &fwd, &rev);
+                real w = K(1.0f) / (K(1.0f) + dvm * rev + ldvm * fwd);
+                V e = bsdf_eval(&b, pdir, h.gn);
+                c = vadd(c, vscale(vmul(e, ptp), w));
+            }
+            color = vadd(color, vmul(vscale(tp, g->norm), c));
+        }
+        if (pl > s->max_path_length) break;
+        real u0 = get1d(smp), u1 = get1d(smp), u2 = get1d(smp);
+        Sample sm;
+        V f = bsdf_sample(&b, h.gn, u0, u1, u2, &sm);
+        if (f.x == 0 && f.y == 0 && f.z == 0) break;
+        if (pl > 2) {
+            if (draw_rr(dr, pl) > sm.cont) break;
+            f = vdiv_each(f, sm.cont);
+        }
+        o = h.pos;
+        d = sm.wi;
+        tp = vmul(tp, f);
+        if (sm.event & EV_SPECULAR) dvm *= (sm.cos_out / sm.fwd) * sm.rev;
+        else dvm = (sm.cos_out / sm.fwd) * (K(1) + sm.rev * dvm);
+    }
+    *color_out = color;
+    /* Image_Buffer::add_color (Image_Buffer.h:24-37) */
+    if (sx < 0 || sx >= (real)W) return;
+    if (sy < 0 || sy >= (real)H) return;
+    int X = (int)sx;
+    int Y = s->compat_rows ? H - (int)sy : H - 1 - (int)sy;
+    long long at = (long long)X + (long long)Y * W;
+    if (at > (long long)W * H - 1 || at < 0) return;
+    *at_out = at;
+}
+
 static void camera_pass(Scene *s, int iteration, const double *ph, long long P, real *image) {
     double dr_, dr2, dnorm;
     orc_iteration_params(s, iteration, &dr_, &dr2, &dnorm);
-    real r2 = (real)dr2, norm = (real)dnorm;
     real *pos = (real *)malloc(sizeof(real) * 3 * (size_t)(P > 0 ? P : 1));
     for (long long i = 0; i < P; ++i)
         for (int c = 0; c < 3; ++c) pos[3 * i + c] = (real)ph[(size_t)i * ORC_PHOTON_STRIDE + c];
     KD kd;
     if (!s->gather_brute) kd_build(&kd, pos, P);
+    const GatherCtx g = {ph, pos, P, &kd, (real)dr2, (real)dnorm};
+    const long long pixels = (long long)s->W * s->H, stride = s->camera_stride;
+#ifdef _OPENMP
+    if (!s->rng_serial && s->threads > 1) {
+        /* Camera paths are independent: sample k = p / stride is traced on any
+         * thread (shallow scene copy for the counters), its colour kept, and
+         * the colours are added to the film in sample order as the serial loop
+         * adds them. */
+        const int T = s->threads;
+        const long long n = (pixels + stride - 1) / stride;
+        V *col = (V *)malloc(sizeof(V) * (size_t)(n > 0 ? n : 1));
+        long long *at = (long long *)malloc(sizeof(long long) * (size_t)(n > 0 ? n : 1));
+        Counters *ctrs = (Counters *)calloc((size_t)T, sizeof(Counters));
+#pragma omp parallel num_threads(T)
+        {
+            Scene local = *s;
+            local.ctr.box_tests = local.ctr.tri_tests = 0;
+            Draws dr = {0, NULL, {{(uint32_t)s->seed, (uint32_t)(s->seed >> 32)}, 0, (uint32_t)iteration, STREAM_CAMERA}};
+            Sampler plain;
+            IList found = {0};
+#pragma omp for schedule(static)
+            for (long long k = 0; k < n; ++k) camera_path(&local, iteration, &g, k * stride, &dr, &plain, &found, &col[k], &at[k]);
+            free(found.idx);
+            ctrs[omp_get_thread_num()] = local.ctr;
+        }
+        for (long long k = 0; k < n; ++k) {
+            if (at[k] < 0) continue;
+            image[3 * at[k]] += col[k].x;
+            image[3 * at[k] + 1] += col[k].y;
+            image[3 * at[k] + 2] += col[k].z;
+        }
+        for (int t = 0; t < T; ++t) {
+            s->ctr.box_tests += ctrs[t].box_tests;
+            s->ctr.tri_tests += ctrs[t].tri_tests;
+        }
+        free(col);
+        free(at);
+        free(ctrs);
+        if (!s->gather_brute) kd_free(&kd);
+        free(pos);
+        return;
+    }
+#endif
     IList found = {0};
-    const int W = s->W, H = s->H;
     Draws dr = {s->rng_serial, &s->xs, {{(uint32_t)s->seed, (uint32_t)(s->seed >> 32)}, 0, (uint32_t)iteration, STREAM_CAMERA}};
     Sampler *smp = &s->serial_smp, plain;
     if (!s->rng_serial) smp = &plain;
-    for (long long p = 0; p < (long long)W * H; p += s->camera_stride) {
-        real jx, jy;
-        if (s->rng_serial) {
-            (void)random_float_0_1(&s->xs); /* progress draw (:311) */
-            jx = random_float_0_1(&s->xs);  /* argument order unspecified in C++; taken left to right */
-            jy = random_float_0_1(&s->xs);
-            sampler_init_uniform(smp, &s->xs);
-        } else {
-            dr.key.path = (uint32_t)p;
-            jx = path_u(&dr.key, 0);
-            jy = path_u(&dr.key, 1);
-            sampler_plain(smp, &dr.key);
-            if (s->camera_qmc) {
-                uint32_t w[3];
-                qmc_camera_words(dr.key.key, (uint32_t)p, (uint32_t)iteration, w);
-                jx = u24(w[0]);
-                jy = u24(w[1]);
-                smp->value[0] = u24(w[2]); /* bounce 0, first dimension */
-            }
-        }
-        smp->index = 0;
-        real sx = (real)(int)(p % W) + K(0.0f) + K(1.0f) * jx;
-        real sy = (real)(int)(p / W) + K(0.0f) + K(1.0f) * jy;
-        /* Camera::get_ray (Camera.h:29-34) */
-        V dir = vadd(vadd(s->cam_bl, vscale(s->cam_h, sx / (real)W)), vscale(s->cam_v, sy / (real)H));
-        dir = vsub(dir, s->cam_o);
-        V o = s->cam_o, d = vnormalize(dir), tp = vs(1), color = vs(0);
-        real dvm = 0;
-        for (int pl = 0;; ++pl) {
-            Ray r = {vadd(o, vscale(d, K(0.001f))), d};
-            Isect h;
-            if (!scene_hit(s, &r, K(0.001f), &h)) break;
-            Bsdf b;
-            bsdf_init(&b, &s->mats[h.mat], h.sn, h.dir, 1);
-            if (!b.valid) break;
-            dvm /= (real)fabs(b.wo.z);
-            if (!bsdf_is_delta(&b)) {
-                /* Path_Vertices::get_contrib (Path_Vertices.h:46-68) */
-                found.n = 0;
-                if (P > 0) {
-                    if (s->gather_brute) brute_query(pos, P, h.pos, r2, &found);
-                    else kd_query(&kd, h.pos, r2, 1, &found);
-                }
-                V c = vs(0);
-                for (long long i = 0; i < found.n; ++i) {
-                    const double *pr = ph + (size_t)found.idx[i] * ORC_PHOTON_STRIDE;
-                    V pdir = v3((real)pr[3], (real)pr[4], (real)pr[5]);
-                    V ptp = v3((real)pr[6], (real)pr[7], (real)pr[8]);
-                    real ldvm = (real)pr[9];
-                    real fwd, rev;
-                    bsdf_pdf(&b, pdir, &fwd, &rev);
-                    real w = K(1.0f) / (K(1.0f) + dvm * rev + ldvm * fwd);
-                    V e = bsdf_eval(&b, pdir, h.gn);
-                    c = vadd(c, vscale(vmul(e, ptp), w));
-                }
-                color = vadd(color, vmul(vscale(tp, norm), c));
-            }
-            if (pl > s->max_path_length) break;
-            real u0 = get1d(smp), u1 = get1d(smp), u2 = get1d(smp);
-            Sample sm;
-            V f = bsdf_sample(&b, h.gn, u0, u1, u2, &sm);
-            if (f.x == 0 && f.y == 0 && f.z == 0) break;
-            if (pl > 2) {
-                if (draw_rr(&dr, pl) > sm.cont) break;
-                f = vdiv_each(f, sm.cont);
-            }
-            o = h.pos;
-            d = sm.wi;
-            tp = vmul(tp, f);
-            if (sm.event & EV_SPECULAR) dvm *= (sm.cos_out / sm.fwd) * sm.rev;
-            else dvm = (sm.cos_out / sm.fwd) * (K(1) + sm.rev * dvm);
-        }
-        /* Image_Buffer::add_color (Image_Buffer.h:24-37) */
-        if (sx < 0 || sx >= (real)W) continue;
-        if (sy < 0 || sy >= (real)H) continue;
-        int X = (int)sx;
-        int Y = s->compat_rows ? H - (int)sy : H - 1 - (int)sy;
-        long long at = (long long)X + (long long)Y * W;
-        if (at > (long long)W * H - 1 || at < 0) continue;
+    for (long long p = 0; p < pixels; p += stride) {
+        V color;
+        long long at;
+        camera_path(s, iteration, &g, p, &dr, smp, &found, &color, &at);
+        if (at < 0) continue;
         image[3 * at] += color.x;
         image[3 * at + 1] += color.y;
         image[3 * at + 2] += color.z;
