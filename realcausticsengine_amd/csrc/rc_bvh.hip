@@ -165,6 +165,86 @@ __global__ void k_lbvh_depth(int n, const int* parent, uint32_t* depth) {
     atomicMax(depth, d);
 }
 
+// 4-wide nodes of the linear BVH: the internal nodes at even depth (edges
+// from the root).  A 4-wide node takes its binary node's children, and in
+// place of an internal child (odd depth) that child's two children, so every
+// internal node of odd depth is opened by its parent and every grandchild
+// that is internal is again a 4-wide node.  Layout as the host-collapsed tree.
+__global__ void k_lbvh_wide_flags(int n, const int* parent, uint32_t* wide) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    uint32_t d = 0;
+    for (int p = parent[i]; p >= 0 && d <= (uint32_t)(2 * kMaxDepth); p = parent[p]) ++d;
+    wide[i] = (d & 1u) ? 0u : 1u;
+}
+
+// Child `side` (0 left, 1 right) of binary node `node`: box and ref.
+__device__ __forceinline__ int binary_child(const float4* nodes, int node, int side, float lo[3], float hi[3]) {
+    const float4* b = nodes + 4 * node;
+    if (side == 0) {
+        lo[0] = b[0].x; hi[0] = b[0].y; lo[1] = b[0].z; hi[1] = b[0].w; lo[2] = b[1].x; hi[2] = b[1].y;
+        return __float_as_int(b[3].x);
+    }
+    lo[0] = b[1].z; hi[0] = b[1].w; lo[1] = b[2].x; hi[1] = b[2].y; lo[2] = b[2].z; hi[2] = b[2].w;
+    return __float_as_int(b[3].y);
+}
+
+__global__ void k_lbvh_collapse4(int n, const float4* nodes, const uint32_t* wide, const uint32_t* wide_idx,
+                                 float4* nodes4) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1 || !wide[i]) return;
+    float lo[3][4] = {}, hi[3][4] = {};
+    int refs[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+    int c = 0;
+    for (int side = 0; side < 2; ++side) {
+        float bl[3], bh[3];
+        const int ref = binary_child(nodes, i, side, bl, bh);
+        if (ref < 0) {
+            for (int k = 0; k < 3; ++k) {
+                lo[k][c] = bl[k];
+                hi[k][c] = bh[k];
+            }
+            refs[c++] = ref;
+            continue;
+        }
+        for (int s2 = 0; s2 < 2; ++s2) {
+            const int g = binary_child(nodes, ref, s2, bl, bh);
+            for (int k = 0; k < 3; ++k) {
+                lo[k][c] = bl[k];
+                hi[k][c] = bh[k];
+            }
+            refs[c++] = g < 0 ? g : (int)wide_idx[g];
+        }
+    }
+    float4* o = nodes4 + 8 * (size_t)wide_idx[i];
+    for (int k = 0; k < 3; ++k) {
+        o[2 * k] = make_float4(lo[k][0], lo[k][1], lo[k][2], lo[k][3]);
+        o[2 * k + 1] = make_float4(hi[k][0], hi[k][1], hi[k][2], hi[k][3]);
+    }
+    o[6] = make_float4(__int_as_float(refs[0]), __int_as_float(refs[1]), __int_as_float(refs[2]),
+                       __int_as_float(refs[3]));
+    o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+int lbvh_wide_count(const LbvhBuffers& b, uint32_t n, hipStream_t s) {
+    if (n < 2) return -1;
+    unsigned g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_lbvh_wide_flags, dim3(g), dim3(256), 0, s, (int)n, b.parent, b.flags);
+    launch_exclusive_scan_u32(b.flags, b.vals_a, n - 1, b.hist, s);
+    uint32_t last[2] = {0, 0};
+    if (hipMemcpyAsync(&last[0], b.vals_a + (n - 2), 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&last[1], b.flags + (n - 2), 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess || hipGetLastError() != hipSuccess)
+        return -1;
+    return (int)(last[0] + last[1]);
+}
+
+bool lbvh_collapse4(const LbvhBuffers& b, uint32_t n, float4* nodes4, hipStream_t s) {
+    unsigned g = (n + 255) / 256;
+    hipLaunchKernelGGL(k_lbvh_collapse4, dim3(g), dim3(256), 0, s, (int)n, b.nodes, b.flags, b.vals_a, nodes4);
+    return hipStreamSynchronize(s) == hipSuccess && hipGetLastError() == hipSuccess;
+}
+
 int build_lbvh(const LbvhBuffers& b, uint32_t n, const float lo[3], const float hi[3], hipStream_t s) {
     if (n < 2) return -1;
     unsigned g = (n + 255) / 256;

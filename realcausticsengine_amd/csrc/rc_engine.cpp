@@ -606,10 +606,20 @@ bool gpu_bvh(Engine& e, uint32_t n) {
         float ms = 0.0f;
         ok = hipEventRecord(e.ev[0], e.stream) == hipSuccess;
         d = ok ? build_lbvh(b, n, e.bbox.lo, e.bbox.hi, e.stream) : -1;
+        // bvh_width 4: the binary tree collapsed to 4-wide nodes on the GPU
+        e.n_nodes4 = 0;
+        e.root4 = 0;
+        bool collapsed = true;
+        if (ok && d >= 0 && d < kMaxDepth && e.bvh_width == 4) {
+            const int n4 = lbvh_wide_count(b, n, e.stream);
+            collapsed = n4 > 0 && e.d_nodes4.alloc(8 * (size_t)n4) && lbvh_collapse4(b, n, e.d_nodes4.p, e.stream);
+            if (collapsed) e.n_nodes4 = n4;
+            ok = collapsed;
+        }
         ok = ok && hipEventRecord(e.ev[1], e.stream) == hipSuccess && hipEventSynchronize(e.ev[1]) == hipSuccess &&
              hipEventElapsedTime(&ms, e.ev[0], e.ev[1]) == hipSuccess;
         e.bvh_build_ms = ms;
-        if (!ok) set_error("GPU BVH build: HIP event error");
+        if (!ok) set_error(collapsed ? "GPU BVH build: HIP event error" : "GPU BVH build: 4-wide collapse failed");
     }
     tmp4.release();
     tmp_id.release();
@@ -633,7 +643,7 @@ bool gpu_bvh(Engine& e, uint32_t n) {
         return false;
     }
     e.bvh_depth = d;
-    e.d_nodes4.release();
+    if (e.n_nodes4 == 0) e.d_nodes4.release();
     return true;
 }
 

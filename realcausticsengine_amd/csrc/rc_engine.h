@@ -292,6 +292,10 @@ struct LbvhBuffers {
     uint32_t* depth;  // 1
 };
 int build_lbvh(const LbvhBuffers& b, uint32_t n, const float lo[3], const float hi[3], hipStream_t s);
+// After build_lbvh (same buffers): the number of 4-wide nodes of the collapsed
+// tree (-1 on error), then the collapse into nodes4 (8 float4 per node, root 0).
+int lbvh_wide_count(const LbvhBuffers& b, uint32_t n, hipStream_t s);
+bool lbvh_collapse4(const LbvhBuffers& b, uint32_t n, float4* nodes4, hipStream_t s);
 size_t photon_map_hist_entries(uint32_t max_count);
 size_t photon_map_sum_entries(uint32_t max_count);
 

@@ -1,5 +1,6 @@
-"""The GPU-built linear BVH (option bvh_build = 1) against brute force, the
-oracle and the host-built SAH tree.
+"""The GPU-built linear BVH (option bvh_build = 1), traversed as binary nodes
+(bvh_width = 2) and collapsed on the GPU to 4-wide nodes (bvh_width = 4, the
+default), against brute force, the oracle and the host-built SAH tree.
 
 The tree changes only which nodes a ray visits: the closest hit is the same
 triangle with the same Moller-Trumbore distance bits, except where two
@@ -42,13 +43,20 @@ def _check_hits(tri_g, t_g, tri_o, t_o):
     return hit.sum()
 
 
+@pytest.mark.parametrize("width", [2, 4])
 @pytest.mark.parametrize("scene", ["c0", "random"])
-def test_lbvh_first_hit_matches_brute_force(oracle, scene):
+def test_lbvh_first_hit_matches_brute_force(oracle, scene, width):
     args = scenes.c0_scene(64, 64, subdiv=3) if scene == "c0" else scenes.random_mesh_scene(seed=2)
-    eng = Engine(args, bvh_build=1)
+    eng = Engine(args, bvh_build=1, bvh_width=width)
     eng.prepare()
-    assert eng.get("bvh_build") == 1 and eng.get("nodes4") == 0 and eng.get("node_bytes") == 64
-    assert eng.get("nodes") == len(np.concatenate([m.tris for m in args.meshes])) - 1
+    ntri = len(np.concatenate([m.tris for m in args.meshes]))
+    assert eng.get("bvh_build") == 1 and eng.get("node_bytes") == 32 * width
+    assert eng.get("nodes") == ntri - 1
+    if width == 4:
+        # the internal nodes of even depth
+        assert (ntri - 1) / 4 <= eng.get("nodes4") < ntri - 1
+    else:
+        assert eng.get("nodes4") == 0
     orc = oracle.OracleScene(args)
     rays = _rays(args, 20000, 1)
     tri_g, t_g = eng.first_hit(rays)
@@ -57,16 +65,17 @@ def test_lbvh_first_hit_matches_brute_force(oracle, scene):
     eng.release()
 
 
-def test_lbvh_two_triangles(oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_lbvh_two_triangles(oracle, width):
     """The smallest tree the GPU build makes: one internal node over two leaves."""
     pv, pn, pt = scenes.plane(4.0, 4.0, 0.0)
     args = SceneArgs([MeshData(pv, pn, pt, material=0)], [LightData("A", position=(0.0, 0.0, 3.0), width=1.0,
                                                                     height=1.0, power=1.0)],
                      [MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.8, 0.8))],
                      look_at_camera(16, 16, (0.0, -3.0, 3.0), (0.0, 0.0, 0.0), 40.0), samples=1)
-    eng = Engine(args, bvh_build=1)
+    eng = Engine(args, bvh_build=1, bvh_width=width)
     eng.prepare()
-    assert eng.get("nodes") == 1 and eng.get("bvh_depth") == 1
+    assert eng.get("nodes") == 1 and eng.get("bvh_depth") == 1 and eng.get("nodes4") == (1 if width == 4 else 0)
     orc = oracle.OracleScene(args)
     rays = _rays(args, 4000, 5, spread=1.5, z=1.0)
     tri_g, t_g = eng.first_hit(rays)
@@ -75,9 +84,10 @@ def test_lbvh_two_triangles(oracle):
     eng.release()
 
 
-def test_lbvh_photon_list_matches_oracle(oracle):
+@pytest.mark.parametrize("width", [2, 4])
+def test_lbvh_photon_list_matches_oracle(oracle, width):
     args = scenes.c0_scene(64, 64, subdiv=3)
-    eng = Engine(args, photons=32768, seed=7, bvh_build=1)
+    eng = Engine(args, photons=32768, seed=7, bvh_build=1, bvh_width=width)
     orc = oracle.OracleScene(args, photons=32768, seed=7)
     ph_g, uid_g = eng.trace_photons(0)
     ph_o = orc.trace_light(0)
@@ -90,27 +100,30 @@ def test_lbvh_photon_list_matches_oracle(oracle):
 
 
 def test_lbvh_two_million_triangles_match_host_tree():
-    """A 1024 x 1024-vertex water surface (2.09 M triangles): GPU-built and
-    host-built trees give the same first hits and the same photons."""
+    """A 1024 x 1024-vertex water surface (2.09 M triangles): GPU-built trees
+    (binary and collapsed 4-wide) and the host-built tree give the same first
+    hits and the same photons."""
     args = scenes.water_scene(1024, 64, 64, samples=1)
     ntri = sum(len(m.tris) for m in args.meshes)
     assert ntri > 2_000_000
     rays = _rays(args, 100000, 9, spread=1.5, z=0.2)
     out = {}
-    for build in (0, 1):
-        eng = Engine(args, photons=1 << 18, seed=3, bvh_build=build, bvh_width=2)
+    for build, width in ((0, 2), (1, 2), (1, 4)):
+        eng = Engine(args, photons=1 << 18, seed=3, bvh_build=build, bvh_width=width)
         eng.prepare()
         depth, ms = eng.get("bvh_depth"), eng.get("bvh_build_ms")
-        print(f"bvh_build={build}: depth {depth:.0f}, build {ms:.1f} ms, nodes {eng.get('nodes'):.0f}", flush=True)
+        print(f"bvh_build={build} width={width}: depth {depth:.0f}, build {ms:.1f} ms, nodes {eng.get('nodes'):.0f}, "
+              f"4-wide nodes {eng.get('nodes4'):.0f}", flush=True)
         assert 0 < depth < 64
         tri, t = eng.first_hit(rays)
         ph, uid = eng.trace_photons(0)
-        out[build] = (tri, t, ph, uid)
+        out[(build, width)] = (tri, t, ph, uid)
         eng.release()
-    tri_h, t_h, ph_h, uid_h = out[0]
-    tri_l, t_l, ph_l, uid_l = out[1]
-    assert _check_hits(tri_l, t_l, tri_h, t_h) > 50000
-    common, ia, ib = np.intersect1d(uid_l, uid_h, return_indices=True)
-    assert len(common) >= 0.999 * max(len(uid_l), len(uid_h))
-    exact = np.all(ph_l[ia].view(np.uint32) == ph_h[ib].view(np.uint32), axis=1)
-    assert exact.mean() >= 0.999, exact.mean()
+    tri_h, t_h, ph_h, uid_h = out[(0, 2)]
+    for key in ((1, 2), (1, 4)):
+        tri_l, t_l, ph_l, uid_l = out[key]
+        assert _check_hits(tri_l, t_l, tri_h, t_h) > 50000
+        common, ia, ib = np.intersect1d(uid_l, uid_h, return_indices=True)
+        assert len(common) >= 0.999 * max(len(uid_l), len(uid_h))
+        exact = np.all(ph_l[ia].view(np.uint32) == ph_h[ib].view(np.uint32), axis=1)
+        assert exact.mean() >= 0.999, (key, exact.mean())
