@@ -87,8 +87,9 @@ int main(void);
  * Single scenes, without the learned proposal.
  * Acceleration structure: "bvh_build" (0 = binned SAH on the host, the
  * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
- * millions of triangles, traversed as a binary tree), "bvh_width" (4 = the
- * host tree collapsed to 4-wide nodes, the default; 2 = binary nodes).
+ * millions of triangles), "bvh_width" (4 = the tree collapsed to 4-wide
+ * nodes -- on the host for the SAH tree, on the GPU for the linear BVH --
+ * the default; 2 = binary nodes).
  * Camera sampling: "camera_qmc" (1 = pixel jitter and the first bounce's
  * reflect / refract choice follow a randomly shifted low-discrepancy sequence
  * over iterations -- iteration i takes point i -- instead of independent
