@@ -3,9 +3,10 @@
 // engine's binary node layout: Morton codes of the triangle box centres
 // (10 bits per axis) above the triangle index (so that every key is
 // distinct), one radix sort, one thread per internal node for the hierarchy, and a bottom-up pass
-// in which the second child to arrive at a node forms its box.  One triangle
-// per leaf.  The reference builds its BVH on the CPU (BVH_mesh.cpp:28-155);
-// this build serves scenes of millions of triangles.
+// in which the second child to arrive at a node forms its box; then,
+// optionally, the collapse to 4-wide nodes.  One triangle per leaf.  The
+// reference builds its BVH on the CPU (BVH_mesh.cpp:28-155); the GPU build
+// serves scenes of millions of triangles.
 #include "rc_engine.h"
 #include "rc_vec.h"
 
