@@ -134,12 +134,23 @@ struct Engine {
     int iterations_done = 0;
     double stats[RC_STAT_COUNT] = {0};
     hipStream_t stream = nullptr;
+    // overlap = 1: the camera trace runs on stream2 (its own path counters)
+    // while the light pass and the photon map run on stream; the merge waits
+    // for both
+    int overlap = 1;
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_cam = nullptr;   // the overlapped trace is done (stream waits on it)
+    hipEvent_t ev_join = nullptr;  // stream passed the wait: the query sort's start
+    hipEvent_t ev_fork = nullptr;  // stream2 starts after the work already queued on stream
+    DevBuf<uint32_t> cam_counters;
+    uint32_t* h_nq = nullptr;  // pinned: the overlapped trace's query count
     int device_cus = 0;
     uint32_t last_queries = 0;
     GuideModel guide;  // learned emission proposal (option "guide")
     // captured iteration graph (option "graph"): photon map + camera pass
     int graph = 0;
     hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec2 = nullptr;  // with overlap: query sort, merge, resolve (graph_exec: the map)
     std::vector<uintptr_t> graph_key;  // buffers and sizes the graph was captured with
     DevBuf<IterParams> d_iter;
     IterParams* h_iter = nullptr;      // pinned staging of the per-iteration values
@@ -587,6 +598,17 @@ bool ensure_device(Engine& e) {
     return true;
 }
 
+// The second stream, its join event and counters (overlapped camera trace), on first use.
+bool ensure_stream2(Engine& e) {
+    if (e.stream2) return true;
+    HIP_OK(hipStreamCreate(&e.stream2));
+    HIP_OK(hipEventCreateWithFlags(&e.ev_cam, hipEventDisableTiming));
+    HIP_OK(hipEventCreate(&e.ev_join));
+    HIP_OK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_nq), sizeof(uint32_t), hipHostMallocDefault));
+    return e.cam_counters.alloc(kPathCounters);
+}
+
 // Linear BVH built on the device over the triangles uploaded in input order
 // (option bvh_build = 1); binary nodes only, root 0.
 bool gpu_bvh(Engine& e, uint32_t n) {
@@ -1014,6 +1036,94 @@ const uint32_t* query_order(Engine& e, const CameraPassArgs& a, uint32_t n, hipS
 }
 
 // Camera pass: trace (queries), merge, resolve into `image`.
+CameraPassArgs camera_args(Engine& e, int iteration, float norm, float* image) {
+    uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
+    CameraPassArgs a{};
+    a.sc = scene_view(e);
+    a.k0 = (uint32_t)e.seed;
+    a.k1 = (uint32_t)(e.seed >> 32);
+    a.iteration = (uint32_t)iteration;
+    a.stream = STREAM_CAMERA;
+    a.n_paths = n_paths;
+    a.path_chunk = e.path_chunk;
+    a.refill_min = e.camera_refill;
+    a.path_counter = e.counters.p;
+    a.vm_norm = norm;
+    a.path_info = e.path_info.p;
+    a.contrib = e.contrib.p;
+    a.image = image;
+    a.counters = e.count_events ? e.stats_dev.p + 1 : nullptr;
+    a.exact_merge = e.exact_merge;
+    a.group_queries = e.group_queries;
+    a.gather_tile = e.gather_tile;
+    a.gather_block = e.gather_block;
+    a.camera_qmc = e.camera_qmc;
+    a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
+    return a;
+}
+
+// Overlapped camera pass, part 1: the camera trace on stream2 with its own
+// path counters, its query count copied to pinned memory.
+bool camera_trace_async(Engine& e, int iteration, float norm, float* image, CameraPassArgs& a) {
+    uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
+    uint32_t cap = std::max(e.q_capacity, std::max<uint32_t>(2 * n_paths, 1024));
+    if (!ensure_query_buffers(e, cap, n_paths) || !ensure_stream2(e)) return false;
+    a = camera_args(e, iteration, norm, image);
+    a.path_counter = e.cam_counters.p;
+    HIP_OK(hipEventRecord(e.ev_fork, e.stream));
+    HIP_OK(hipStreamWaitEvent(e.stream2, e.ev_fork, 0));
+    launch_fill_u32(e.cam_counters.p, 0u, kPathCounters, e.stream2);
+    HIP_OK(hipEventRecord(e.ev[4], e.stream2));
+    launch_camera_trace(a, camera_blocks(e), e.stream2);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(e.h_nq, e.cam_counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream2));
+    HIP_OK(hipEventRecord(e.ev[8], e.stream2));
+    HIP_OK(hipEventRecord(e.ev_cam, e.stream2));
+    return true;
+}
+
+// Part 2, after the photon map on stream: the query count (a trace that
+// overflowed the query buffer is traced again, serially, into larger
+// buffers), then stream waits for the trace and runs the query sort, the
+// merge and the resolve.
+bool camera_finish(Engine& e, int iteration, const MapInfo& mi, float norm, float* image, CameraPassArgs a) {
+    HIP_OK(hipEventSynchronize(e.ev_cam));
+    uint32_t nq = *e.h_nq;
+    if (nq > e.q_capacity) {
+        uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
+        if (!ensure_query_buffers(e, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), n_paths)) {
+            set_error("merge query buffer overflow");
+            return false;
+        }
+        a = camera_args(e, iteration, norm, image);
+        a.path_counter = e.cam_counters.p;
+        launch_fill_u32(e.cam_counters.p, 0u, kPathCounters, e.stream);
+        HIP_OK(hipEventRecord(e.ev[4], e.stream));
+        launch_camera_trace(a, camera_blocks(e), e.stream);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(&nq, e.cam_counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        HIP_OK(hipStreamSynchronize(e.stream));
+        HIP_OK(hipEventRecord(e.ev[8], e.stream));
+        if (nq > e.q_capacity) {
+            set_error("merge query buffer overflow");
+            return false;
+        }
+    }
+    e.stats[RC_STAT_GATHERS] = nq;
+    e.last_queries = nq;
+    HIP_OK(hipStreamWaitEvent(e.stream, e.ev_cam, 0));
+    HIP_OK(hipEventRecord(e.ev_join, e.stream));
+    a.pm = mi.pm;
+    a.q_order = query_order(e, a, e.last_queries, e.stream);
+    HIP_OK(hipEventRecord(e.ev[6], e.stream));
+    launch_gather(a, gather_blocks(e), e.stream);
+    HIP_OK(hipEventRecord(e.ev[7], e.stream));
+    launch_resolve(a, e.stream);
+    HIP_OK(hipEventRecord(e.ev[5], e.stream));
+    HIP_OK(hipGetLastError());
+    return true;
+}
+
 bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, float* image) {
     uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
     uint32_t cap = std::max(e.q_capacity, std::max<uint32_t>(2 * n_paths, 1024));
@@ -1095,21 +1205,47 @@ double ms_since(std::chrono::steady_clock::time_point t0) {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// The launches `body` queues on st, captured and instantiated into `out`.
+bool capture_graph(hipStream_t st, const std::function<void()>& body, hipGraphExec_t& out) {
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    body();
+    hipError_t cap = hipStreamEndCapture(st, &graph);
+    if (cap != hipSuccess || !graph) {
+        set_error(std::string("hipGraph capture failed: ") + hipGetErrorString(cap));
+        return false;
+    }
+    hipError_t inst = hipGraphInstantiate(&out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (inst != hipSuccess) {
+        out = nullptr;
+        set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(inst));
+        return false;
+    }
+    return true;
+}
+
 // The iteration with its photon map and camera pass replayed from a captured
 // hipGraph.  The light pass stays a direct launch: its photon count is read
 // back and checked against the capacity before the replay (a grown buffer
 // means a new capture).  Inside the graph every count comes from device
 // memory and the per-iteration values (iteration, radius, grid, VM
 // normalisation) from d_iter, copied in before each replay; the query buffer
-// holds the most queries a camera pass can make (8 per path).
+// holds the most queries a camera pass can make (8 per path).  With option
+// overlap the camera trace is a direct launch on stream2 beside the light
+// pass and two graphs replay: the photon map after the light pass, then,
+// once stream has waited for the trace, the query sort, merge and resolve.
 bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     hipStream_t st = e.stream;
     float radius, r2, norm;
     iteration_params(e, iteration, radius, r2, norm);
-    if (!run_light_pass(e, iteration, count)) return false;
-    long long N = light_paths(e);
     const uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
     if (!ensure_query_buffers(e, (uint32_t)std::min<uint64_t>(8ull * n_paths + 1024, 1ull << 31), n_paths)) return false;
+    const bool overlapped = e.overlap && !e.mlt;
+    CameraPassArgs ca{};
+    if (overlapped && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
+    if (!run_light_pass(e, iteration, count)) return false;
+    long long N = light_paths(e);
     // the grid of this iteration, as build_map
     int uid_bits = log2_ceil((uint64_t)N) + 3;
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
@@ -1146,10 +1282,11 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
                                   (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile,
-                                  (uintptr_t)e.gather_block};
+                                  (uintptr_t)e.gather_block, (uintptr_t)overlapped};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
-        e.graph_exec = nullptr;
+        if (e.graph_exec2) HIP_OK(hipGraphExecDestroy(e.graph_exec2));
+        e.graph_exec = e.graph_exec2 = nullptr;
         MapBuildArgs m{};
         m.in_pos = e.ph_pos.p;
         m.in_dir = e.ph_dir.p;
@@ -1181,7 +1318,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.n_paths = n_paths;
         a.path_chunk = e.path_chunk;
         a.refill_min = e.camera_refill;
-        a.path_counter = e.counters.p;
+        a.path_counter = overlapped ? e.cam_counters.p : e.counters.p;
         a.path_info = e.path_info.p;
         a.contrib = e.contrib.p;
         a.image = e.image.p;
@@ -1192,24 +1329,28 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.camera_qmc = e.camera_qmc;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
-        hipGraph_t graph = nullptr;
-        HIP_OK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        build_photon_map(m, st);
-        launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
-        launch_camera_trace(a, camera_blocks(e), st);
-        a.q_order = query_order(e, a, e.q_capacity, st);  // count read from the device
-        launch_gather(a, gather_blocks(e), st);
-        launch_resolve(a, st);
-        hipError_t cap = hipStreamEndCapture(st, &graph);
-        if (cap != hipSuccess || !graph) {
-            set_error(std::string("hipGraph capture failed: ") + hipGetErrorString(cap));
-            return false;
+        auto camera = [&]() {
+            a.q_order = query_order(e, a, e.q_capacity, st);  // count read from the device
+            launch_gather(a, gather_blocks(e), st);
+            launch_resolve(a, st);
+        };
+        bool ok;
+        if (overlapped) {
+            ok = capture_graph(st, [&]() { build_photon_map(m, st); }, e.graph_exec) &&
+                 capture_graph(st, camera, e.graph_exec2);
+        } else {
+            ok = capture_graph(st,
+                               [&]() {
+                                   build_photon_map(m, st);
+                                   launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
+                                   launch_camera_trace(a, camera_blocks(e), st);
+                                   camera();
+                               },
+                               e.graph_exec);
         }
-        hipError_t inst = hipGraphInstantiate(&e.graph_exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (inst != hipSuccess) {
+        if (!ok) {
+            if (e.graph_exec) (void)hipGraphExecDestroy(e.graph_exec);
             e.graph_exec = nullptr;
-            set_error(std::string("hipGraphInstantiate failed: ") + hipGetErrorString(inst));
             return false;
         }
         e.graph_key = key;
@@ -1217,8 +1358,13 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     }
     HIP_OK(hipEventRecord(e.ev[2], st));
     HIP_OK(hipGraphLaunch(e.graph_exec, st));
+    if (overlapped) {
+        HIP_OK(hipStreamWaitEvent(st, e.ev_cam, 0));
+        HIP_OK(hipGraphLaunch(e.graph_exec2, st));
+    }
     HIP_OK(hipEventRecord(e.ev[5], st));
-    HIP_OK(hipMemcpyAsync(&e.last_queries, e.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(&e.last_queries, (overlapped ? e.cam_counters.p : e.counters.p) + 1, 4,
+                          hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     e.stats[RC_STAT_GATHERS] = e.last_queries;
     return true;
@@ -1231,12 +1377,22 @@ bool render_iteration(Engine& e, int iteration) {
     uint32_t count = 0;
     MapInfo mi;
     const bool graphed = e.graph && !e.count_events;
+    bool overlapped = false;
     if (graphed) {
         if (!graph_iteration(e, iteration, count, mi)) return false;
     } else {
+        // the camera trace needs neither the photons nor the light pass's results
+        // (plain estimator), so it can run beside them
+        overlapped = e.overlap && !e.count_events && !e.mlt;
+        CameraPassArgs ca{};
+        if (overlapped && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
         if (!run_light_pass(e, iteration, count)) return false;
         if (!build_map(e, count, radius, r2, mi)) return false;
-        if (!run_camera_pass(e, iteration, mi, norm * mlt_factor(e), e.image.p)) return false;
+        if (overlapped) {
+            if (!camera_finish(e, iteration, mi, norm, e.image.p, ca)) return false;
+        } else if (!run_camera_pass(e, iteration, mi, norm * mlt_factor(e), e.image.p)) {
+            return false;
+        }
     }
     const bool training = guide_trains(e, iteration);
     e.stats[RC_STAT_MS_GUIDE] = e.stats[RC_STAT_MS_GUIDE_STEP] = e.stats[RC_STAT_GUIDE_ROWS] = 0;
@@ -1257,7 +1413,7 @@ bool render_iteration(Engine& e, int iteration) {
     } else {
         for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
         HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[8]));
-        HIP_OK(hipEventElapsedTime(&ms_sort, e.ev[8], e.ev[6]));
+        HIP_OK(hipEventElapsedTime(&ms_sort, overlapped ? e.ev_join : e.ev[8], e.ev[6]));
         HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
         HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
     }
@@ -1312,9 +1468,13 @@ bool render_iteration(Engine& e, int iteration) {
 // Autotuner of the launch shapes and queue / tile sizes: every candidate
 // renders the same iteration (same photons, same queries) into a scratch
 // film; each knob keeps the value with the smallest time of the kernels it
-// affects (best of `repeats`), one knob after the other.
+// affects (best of `repeats`), one knob after the other.  The last knob,
+// overlap, is judged by the iteration's wall time: the camera trace beside
+// the light pass shortens the iteration only while the light pass leaves
+// compute units idle, and it is kept only when it saves at least 3 % (beside
+// the trace the light pass's own time no longer measures that kernel alone).
 // ---------------------------------------------------------------------
-enum { T_LIGHT = 0, T_MAP, T_TRACE, T_GATHER, T_PARTS };
+enum { T_LIGHT = 0, T_MAP, T_TRACE, T_GATHER, T_WALL, T_PARTS };
 
 struct TuneKnob {
     const char* name;
@@ -1322,10 +1482,12 @@ struct TuneKnob {
     std::function<void(double)> set;
     std::function<double()> get;
     std::function<double(const double*)> cost;
+    double margin = 0.0;  // a later candidate must beat the best so far by this fraction
 };
 
+// overlap_candidates: {0, 1} for one scene, {0} for a batch (its camera trace is serial).
 std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::function<void(double)> set_cell_scale,
-                                 std::function<double()> get_cell_scale) {
+                                 std::function<double()> get_cell_scale, std::vector<double> overlap_candidates) {
     auto L = [](const double* t) { return t[T_LIGHT]; };
     auto T = [](const double* t) { return t[T_TRACE]; };
     auto G = [](const double* t) { return t[T_GATHER]; };
@@ -1367,10 +1529,14 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
          [=](double v) { launch_owner().query_order = (int)v; },
          [=]() { return (double)launch_owner().query_order; },
          [](const double* t) { return t[T_TRACE] + t[T_GATHER]; }},
+        {"overlap", overlap_candidates,
+         [=](double v) { launch_owner().overlap = (int)v; },
+         [=]() { return (double)launch_owner().overlap; },
+         [](const double* t) { return t[T_WALL]; }, 0.03},
     };
 }
 
-// report: [sum of the kernel times before, after, then the chosen value of each knob]
+// report: [wall ms of the iteration before, after, then the chosen value of each knob]
 bool run_autotune(std::vector<TuneKnob> knobs, std::function<bool(double*)> timed, int repeats, double* report) {
     repeats = std::max(1, repeats);
     double t[T_PARTS];
@@ -1392,7 +1558,7 @@ bool run_autotune(std::vector<TuneKnob> knobs, std::function<bool(double*)> time
             double m[T_PARTS];
             if (!measure(m)) return false;
             double c = kb.cost(m);
-            if (c < best_cost) {
+            if (c < best_cost * (1.0 - kb.margin)) {
                 best_cost = c;
                 best_v = v;
             }
@@ -1401,33 +1567,44 @@ bool run_autotune(std::vector<TuneKnob> knobs, std::function<bool(double*)> time
     }
     double after[T_PARTS];
     if (!measure(after)) return false;
-    double sb = 0, sa = 0;
-    for (int k = 0; k < T_PARTS; ++k) {
-        sb += before[k];
-        sa += after[k];
-    }
-    report[0] = sb;
-    report[1] = sa;
+    report[0] = before[T_WALL];
+    report[1] = after[T_WALL];
     for (size_t i = 0; i < knobs.size(); ++i) report[2 + i] = knobs[i].get();
     return true;
 }
 
-// One iteration `it` into `film`, without training; kernel times in t.
+// One iteration `it` into `film`, without training; kernel times and the
+// host's wall time in t.
 bool timed_iteration(Engine& e, int it, float* film, double* t) {
     float radius, r2, norm;
     iteration_params(e, it, radius, r2, norm);
+    auto w0 = std::chrono::steady_clock::now();
+    const bool overlapped = e.overlap && !e.mlt;
+    CameraPassArgs ca{};
+    if (overlapped && !camera_trace_async(e, it, norm, film, ca)) return false;
     uint32_t count = 0;
     if (!run_light_pass(e, it, count)) return false;
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
-    if (!run_camera_pass(e, it, mi, norm * mlt_factor(e), film)) return false;
+    if (overlapped) {
+        if (!camera_finish(e, it, mi, norm, film, ca)) return false;
+    } else if (!run_camera_pass(e, it, mi, norm * mlt_factor(e), film)) {
+        return false;
+    }
     HIP_OK(hipStreamSynchronize(e.stream));
-    float ms[T_PARTS];
+    t[T_WALL] = ms_since(w0);
+    float ms[4], sort_ms = 0;
     HIP_OK(hipEventElapsedTime(&ms[T_LIGHT], e.ev[0], e.ev[1]));
     HIP_OK(hipEventElapsedTime(&ms[T_MAP], e.ev[2], e.ev[3]));
-    HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[6]));
+    if (overlapped) {  // the trace, then the query sort after the join
+        HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[8]));
+        HIP_OK(hipEventElapsedTime(&sort_ms, e.ev_join, e.ev[6]));
+        ms[T_TRACE] += sort_ms;
+    } else {
+        HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[6]));
+    }
     HIP_OK(hipEventElapsedTime(&ms[T_GATHER], e.ev[6], e.ev[7]));
-    for (int k = 0; k < T_PARTS; ++k) t[k] = ms[k];
+    for (int k = 0; k < 4; ++k) t[k] = ms[k];
     return true;
 }
 
@@ -1435,13 +1612,15 @@ bool autotune(Engine& e, int repeats, double* report) {
     if (!prepare(e)) return false;
     DevBuf<float> film;
     if (!film.alloc((size_t)e.W * e.H * 3)) return false;
-    const int it = e.iterations_done, counting = e.count_events;
+    const int it = e.iterations_done, counting = e.count_events, overlap = e.overlap;
     e.count_events = 0;
+    e.overlap = 0;  // the other knobs are timed with the camera trace serial
     Engine* pe = &e;
     auto knobs = tune_knobs([pe]() -> Engine& { return *pe; }, [pe](double v) { pe->cell_scale = v; },
-                            [pe]() { return pe->cell_scale; });
+                            [pe]() { return pe->cell_scale; }, {0, 1});
     bool ok = run_autotune(knobs, [&](double* t) { return timed_iteration(e, it, film.p, t); }, repeats, report);
     e.count_events = counting;
+    if (!ok) e.overlap = overlap;
     film.release();
     return ok;
 }
@@ -1556,9 +1735,16 @@ void release(Engine*& e) {
         if (ev) (void)hipEventDestroy(ev);
     e->guide.release();
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
+    if (e->graph_exec2) (void)hipGraphExecDestroy(e->graph_exec2);
     e->d_iter.release();
     if (e->h_iter) (void)hipHostFree(e->h_iter);
     if (e->stream) (void)hipStreamDestroy(e->stream);
+    e->cam_counters.release();
+    if (e->h_nq) (void)hipHostFree(e->h_nq);
+    if (e->ev_cam) (void)hipEventDestroy(e->ev_cam);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->stream2) (void)hipStreamDestroy(e->stream2);
     delete e;
     e = nullptr;
 }
@@ -1592,6 +1778,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.group_queries = from.group_queries;
     to.gather_tile = from.gather_tile;
     to.gather_block = from.gather_block;
+    to.overlap = from.overlap;
     to.mlt = from.mlt;
     to.mlt_chains = from.mlt_chains;
     to.output_path = from.output_path;
@@ -2054,9 +2241,11 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
                             [pb](double v) {
                                 for (Engine* e : pb->scenes) e->cell_scale = v;
                             },
-                            [pb]() { return pb->scenes[0]->cell_scale; });
+                            [pb]() { return pb->scenes[0]->cell_scale; }, {0});
     auto timed = [pb](double* t) {
+        auto w0 = std::chrono::steady_clock::now();
         if (!batch_iteration(*pb, true)) return false;
+        t[T_WALL] = ms_since(w0);
         t[T_LIGHT] = pb->stats[RC_STAT_MS_LIGHT];
         t[T_MAP] = pb->stats[RC_STAT_MS_MAP];
         t[T_TRACE] = pb->stats[RC_STAT_MS_CAMERA_TRACE];
@@ -2277,6 +2466,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
+    else if (k == "overlap") g->overlap = (int)v ? 1 : 0;
     else if (k == "gather_block") g->gather_block = (int)v >= 4 ? 4 : (int)v >= 2 ? 2 : (int)v >= 1 ? 1 : 0;
     else if (k == "gather_tile") g->gather_tile = (uint32_t)std::max(0.0, std::min(v, 1048576.0));
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));
@@ -2335,6 +2525,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "group_queries") return g->group_queries;
     if (k == "gather_tile") return g->gather_tile;
     if (k == "gather_block") return g->gather_block;
+    if (k == "overlap") return g->overlap;
     if (k == "mlt") return g->mlt;
     if (k == "mlt_chains") return g->mlt_chains;
     if (k == "mlt_visible") return g->mlt_last[0];  // counts of the last MLT light pass

@@ -29,7 +29,7 @@ EXPORTS = [
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
-             "gather_block", "query_order"]
+             "gather_block", "query_order", "overlap"]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",

@@ -18,7 +18,7 @@ CANDIDATES = {"path_chunk": {64, 128, 256, 512, 1024}, "light_blocks_per_cu": {4
               "light_refill": {1, 8, 16, 32}, "light_order": {0, 1}, "camera_refill": {1, 8, 16, 32},
               "query_order": {0, 1},
               "group_queries": {4, 8, 16}, "gather_tile": {0, 256, 1024},
-              "gather_block": {0, 1, 2, 4}}
+              "gather_block": {0, 1, 2, 4}, "overlap": {0, 1}}
 
 
 def _check_report(r):
@@ -62,6 +62,7 @@ def test_batch_autotune_keeps_films():
     before = [b.image(k, scaled=False) for k in range(2)]
     r = b.autotune(repeats=1)
     _check_report(r)
+    assert r["overlap"] == 0
     for k in range(2):
         assert np.array_equal(b.image(k, scaled=False), before[k])
     assert b.render_iteration() == 1

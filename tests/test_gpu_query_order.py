@@ -18,10 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int = 1024, block: int = 0,
-           exact: int = 0):
+           exact: int = 0, overlap: int = 1):
     args = scenes.water_scene(128, 96, 80, samples=iters)
     eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group,
-                 gather_tile=tile, gather_block=block, exact_merge=exact)
+                 gather_tile=tile, gather_block=block, exact_merge=exact, overlap=overlap)
     eng.prepare()
     films = []
     for _ in range(iters):
@@ -90,6 +90,20 @@ def test_rolled_query_loop_keeps_the_film(exact):
             for a, b in zip(base, _films(1, group=group, block=block, exact=exact)[0]):
                 assert a.sum() > 0
                 assert np.array_equal(a, b), (group, block)
+
+
+@pytest.mark.parametrize("order,graph", [(0, 0), (1, 0), (1, 1)])
+def test_overlapped_camera_trace_keeps_the_film(order, graph):
+    """The camera trace on a second stream, beside the light pass and the
+    photon map (option overlap = 1, the default), traces the same queries from
+    the same keys as the serial trace: the same film and query count, with
+    direct launches and with the graph replay after the trace."""
+    serial, s0 = _films(order, group=8, overlap=0, iters=3)
+    overlapped, s1 = _films(order, group=8, overlap=1, iters=3, graph=graph)
+    for a, b in zip(serial, overlapped):
+        assert a.sum() > 0
+        assert np.array_equal(a, b)
+    assert s0["gathers"] == s1["gathers"] > 0
 
 
 def test_grouped_merge_counts_match_per_query_merge():
