@@ -140,10 +140,12 @@ struct Engine {
     int overlap = 1;
     hipStream_t stream2 = nullptr;
     hipEvent_t ev_cam = nullptr;   // the overlapped trace is done (stream waits on it)
-    hipEvent_t ev_join = nullptr;  // stream passed the wait: the query sort's start
+    hipEvent_t ev_join = nullptr;   // the query sort's start
+    hipEvent_t ev_qdone = nullptr;  // the query sort's end (stream waits on it before the merge)
     hipEvent_t ev_fork = nullptr;  // stream2 starts after the work already queued on stream
     DevBuf<uint32_t> cam_counters;
     uint32_t* h_nq = nullptr;  // pinned: the overlapped trace's query count
+    uint32_t* h_count = nullptr;  // pinned: the light pass's photon count
     int device_cus = 0;
     uint32_t last_queries = 0;
     GuideModel guide;  // learned emission proposal (option "guide")
@@ -604,6 +606,7 @@ bool ensure_stream2(Engine& e) {
     HIP_OK(hipStreamCreate(&e.stream2));
     HIP_OK(hipEventCreateWithFlags(&e.ev_cam, hipEventDisableTiming));
     HIP_OK(hipEventCreate(&e.ev_join));
+    HIP_OK(hipEventCreate(&e.ev_qdone));
     HIP_OK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
     HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_nq), sizeof(uint32_t), hipHostMallocDefault));
     return e.cam_counters.alloc(kPathCounters);
@@ -892,7 +895,9 @@ const uint32_t* light_path_order(Engine& e, const LightPassArgs& a, hipStream_t 
 }
 
 // Light pass into the unsorted photon buffers; returns the photon count.
-bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
+// after_launch (if set) queues other work once the first light pass is queued
+// and before the host waits for its photon count.
+bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::function<bool()>& after_launch = nullptr) {
     long long N = light_paths(e);
     if (N <= 0 || N > (1ll << 28)) {
         set_error("light path count out of range (1 .. 2^28)");
@@ -908,6 +913,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         }
         if (!e.mlt_state.alloc((size_t)kMltBlocks * e.mlt_chains) || !e.mlt_counts.alloc(4)) return false;
     }
+    if (!e.h_count) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_count), sizeof(uint32_t), hipHostMallocDefault));
     for (int attempt = 0; attempt < 2; ++attempt) {
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
@@ -942,10 +948,11 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count) {
         launch_light_pass(a, light_blocks(e), e.stream);
         HIP_OK(hipEventRecord(e.ev[1], e.stream));
         HIP_OK(hipGetLastError());
-        uint32_t c = 0;
-        HIP_OK(hipMemcpyAsync(&c, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        HIP_OK(hipMemcpyAsync(e.h_count, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        if (attempt == 0 && after_launch && !after_launch()) return false;
         if (e.mlt) HIP_OK(hipMemcpyAsync(e.mlt_last, e.mlt_counts.p, sizeof e.mlt_last, hipMemcpyDeviceToHost, e.stream));
         HIP_OK(hipStreamSynchronize(e.stream));
+        const uint32_t c = *e.h_count;
         if (c <= e.capacity) {
             count = c;
             return true;
@@ -960,31 +967,58 @@ struct MapInfo {
     PhotonMapView pm;
 };
 
-bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
+// The photon map's grid for this iteration (host arithmetic, no launch):
+// key widths, bucket table, cell size and origin, and the view the merge and
+// the query sort read.
+struct MapGrid {
+    int uid_bits = 0, bucket_bits = 0;
+    uint32_t buckets = 0;
+    float cell = 0.0f;
+    PhotonMapView pm{};
+};
+
+bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
     long long N = light_paths(e);
-    int uid_bits = log2_ceil((uint64_t)N) + 3;
+    g.uid_bits = log2_ceil((uint64_t)N) + 3;
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
-    int bucket_bits = log2_ceil(want);
-    if (bucket_bits + uid_bits > 64) {
+    g.bucket_bits = log2_ceil(want);
+    if (g.bucket_bits + g.uid_bits > 64) {
         set_error("photon map key does not fit 64 bits");
         return false;
     }
-    uint32_t buckets = 1u << bucket_bits;
-    if (!e.cells.alloc(buckets)) return false;
+    g.buckets = 1u << g.bucket_bits;
+    if (!e.cells.alloc(g.buckets)) return false;
     float maxc = 0.0f;
     for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
     float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
     // cells of at least 2/3 of the query radius: a query touches at most 4 per axis
-    float cell = rq * (float)std::max(0.67, e.cell_scale);
+    g.cell = rq * (float)std::max(0.67, e.cell_scale);
+    PhotonMapView& pm = g.pm;
+    pm.pos = e.s_pos.p;
+    pm.dir_tr = e.s_dir.p;
+    pm.tp_gb = e.s_tp.p;
+    pm.cells = e.cells.p;
+    pm.mask = g.buckets - 1;
+    for (int k = 0; k < 3; ++k) pm.origin[k] = e.bbox.lo[k] - g.cell;
+    pm.inv_cell = 1.0f / g.cell;
+    pm.r2 = r2;
+    pm.r_query = rq;
+    return true;
+}
+
+bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
+    MapGrid g;
+    if (!map_grid(e, radius, r2, g)) return false;
+    const int uid_bits = g.uid_bits, bucket_bits = g.bucket_bits;
     MapBuildArgs a{};
     a.in_pos = e.ph_pos.p;
     a.in_dir = e.ph_dir.p;
     a.in_tp = e.ph_tp.p;
     a.in_uid = e.ph_uid.p;
     a.count = count;
-    for (int k = 0; k < 3; ++k) a.origin[k] = e.bbox.lo[k] - cell;
-    a.inv_cell = 1.0f / cell;
-    a.mask = buckets - 1;
+    for (int k = 0; k < 3; ++k) a.origin[k] = g.pm.origin[k];
+    a.inv_cell = g.pm.inv_cell;
+    a.mask = g.pm.mask;
     a.bucket_bits = bucket_bits;
     a.uid_bits = uid_bits;
     a.keys_a = e.keys_a.p;
@@ -1003,16 +1037,7 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     build_photon_map(a, e.stream);
     HIP_OK(hipEventRecord(e.ev[3], e.stream));
     HIP_OK(hipGetLastError());
-    PhotonMapView& pm = mi.pm;
-    pm.pos = e.s_pos.p;
-    pm.dir_tr = e.s_dir.p;
-    pm.tp_gb = e.s_tp.p;
-    pm.cells = e.cells.p;
-    pm.mask = a.mask;
-    for (int k = 0; k < 3; ++k) pm.origin[k] = a.origin[k];
-    pm.inv_cell = a.inv_cell;
-    pm.r2 = r2;
-    pm.r_query = rq;
+    mi.pm = g.pm;
     return true;
 }
 
@@ -1082,14 +1107,30 @@ bool camera_trace_async(Engine& e, int iteration, float norm, float* image, Came
     return true;
 }
 
-// Part 2, after the photon map on stream: the query count (a trace that
-// overflowed the query buffer is traced again, serially, into larger
-// buffers), then stream waits for the trace and runs the query sort, the
-// merge and the resolve.
+// Part 1b, queued once the light pass is: the query sort after the trace on
+// stream2, sized for the buffer and reading the count from device memory.
+// Its keys need only this iteration's map grid, which the host knows before
+// any photon exists.
+bool queue_query_sort(Engine& e, CameraPassArgs& a, float radius, float r2) {
+    MapGrid g;
+    if (!map_grid(e, radius, r2, g)) return false;
+    a.pm = g.pm;
+    HIP_OK(hipEventRecord(e.ev_join, e.stream2));
+    a.q_order = query_order(e, a, e.q_capacity, e.stream2);
+    HIP_OK(hipEventRecord(e.ev_qdone, e.stream2));
+    return true;
+}
+
+// Part 2, once the photon map is queued on stream: the query count (a trace
+// that overflowed the query buffer is traced again, serially, into larger
+// buffers, and its queries sorted on stream), then stream waits for the query
+// sort and runs the merge and the resolve.
 bool camera_finish(Engine& e, int iteration, const MapInfo& mi, float norm, float* image, CameraPassArgs a) {
     HIP_OK(hipEventSynchronize(e.ev_cam));
     uint32_t nq = *e.h_nq;
-    if (nq > e.q_capacity) {
+    const bool overflow = nq > e.q_capacity;
+    if (overflow) {
+        HIP_OK(hipStreamSynchronize(e.stream2));  // the sort of the overflowed trace, before its buffers move
         uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
         if (!ensure_query_buffers(e, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), n_paths)) {
             set_error("merge query buffer overflow");
@@ -1111,10 +1152,13 @@ bool camera_finish(Engine& e, int iteration, const MapInfo& mi, float norm, floa
     }
     e.stats[RC_STAT_GATHERS] = nq;
     e.last_queries = nq;
-    HIP_OK(hipStreamWaitEvent(e.stream, e.ev_cam, 0));
-    HIP_OK(hipEventRecord(e.ev_join, e.stream));
     a.pm = mi.pm;
-    a.q_order = query_order(e, a, e.last_queries, e.stream);
+    if (overflow) {
+        HIP_OK(hipEventRecord(e.ev_join, e.stream));
+        a.q_order = query_order(e, a, e.last_queries, e.stream);
+        HIP_OK(hipEventRecord(e.ev_qdone, e.stream));
+    }
+    HIP_OK(hipStreamWaitEvent(e.stream, e.ev_qdone, 0));  // after the trace too (stream2's order)
     HIP_OK(hipEventRecord(e.ev[6], e.stream));
     launch_gather(a, gather_blocks(e), e.stream);
     HIP_OK(hipEventRecord(e.ev[7], e.stream));
@@ -1386,7 +1430,9 @@ bool render_iteration(Engine& e, int iteration) {
         overlapped = e.overlap && !e.count_events && !e.mlt;
         CameraPassArgs ca{};
         if (overlapped && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
-        if (!run_light_pass(e, iteration, count)) return false;
+        auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
+        if (!run_light_pass(e, iteration, count, overlapped ? sort_queries_next : std::function<bool()>()))
+            return false;
         if (!build_map(e, count, radius, r2, mi)) return false;
         if (overlapped) {
             if (!camera_finish(e, iteration, mi, norm, e.image.p, ca)) return false;
@@ -1413,7 +1459,8 @@ bool render_iteration(Engine& e, int iteration) {
     } else {
         for (int k = 0; k < 3; ++k) HIP_OK(hipEventElapsedTime(&ms[k], e.ev[2 * k], e.ev[2 * k + 1]));
         HIP_OK(hipEventElapsedTime(&ms_trace, e.ev[4], e.ev[8]));
-        HIP_OK(hipEventElapsedTime(&ms_sort, overlapped ? e.ev_join : e.ev[8], e.ev[6]));
+        if (overlapped) HIP_OK(hipEventElapsedTime(&ms_sort, e.ev_join, e.ev_qdone));
+        else HIP_OK(hipEventElapsedTime(&ms_sort, e.ev[8], e.ev[6]));
         HIP_OK(hipEventElapsedTime(&ms_gather, e.ev[6], e.ev[7]));
         HIP_OK(hipEventElapsedTime(&ms_resolve, e.ev[7], e.ev[5]));
     }
@@ -1583,7 +1630,8 @@ bool timed_iteration(Engine& e, int it, float* film, double* t) {
     CameraPassArgs ca{};
     if (overlapped && !camera_trace_async(e, it, norm, film, ca)) return false;
     uint32_t count = 0;
-    if (!run_light_pass(e, it, count)) return false;
+    auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
+    if (!run_light_pass(e, it, count, overlapped ? sort_queries_next : std::function<bool()>())) return false;
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi)) return false;
     if (overlapped) {
@@ -1596,9 +1644,9 @@ bool timed_iteration(Engine& e, int it, float* film, double* t) {
     float ms[4], sort_ms = 0;
     HIP_OK(hipEventElapsedTime(&ms[T_LIGHT], e.ev[0], e.ev[1]));
     HIP_OK(hipEventElapsedTime(&ms[T_MAP], e.ev[2], e.ev[3]));
-    if (overlapped) {  // the trace, then the query sort after the join
+    if (overlapped) {  // the trace, then the query sort
         HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[8]));
-        HIP_OK(hipEventElapsedTime(&sort_ms, e.ev_join, e.ev[6]));
+        HIP_OK(hipEventElapsedTime(&sort_ms, e.ev_join, e.ev_qdone));
         ms[T_TRACE] += sort_ms;
     } else {
         HIP_OK(hipEventElapsedTime(&ms[T_TRACE], e.ev[4], e.ev[6]));
@@ -1741,8 +1789,10 @@ void release(Engine*& e) {
     if (e->stream) (void)hipStreamDestroy(e->stream);
     e->cam_counters.release();
     if (e->h_nq) (void)hipHostFree(e->h_nq);
+    if (e->h_count) (void)hipHostFree(e->h_count);
     if (e->ev_cam) (void)hipEventDestroy(e->ev_cam);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
+    if (e->ev_qdone) (void)hipEventDestroy(e->ev_qdone);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
     if (e->stream2) (void)hipStreamDestroy(e->stream2);
     delete e;

@@ -1565,6 +1565,13 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned long 
     hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
+// Sorts of at most this many tiles skip the separate scan: each scatter block
+// forms its offsets from the raw digit counts (nblocks reads per thread, from L2).
+constexpr uint32_t kSelfScanBlocks = 128;
+
+// kSelfScan = false: offsets = exclusive scan of the digit-major counts;
+// true: offsets holds the raw counts hist[digit * nblocks + block].
+template <bool kSelfScan>
 __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const unsigned long long* kin, const uint32_t* vin,
                                                                const uint32_t* n_dev, uint32_t n, int shift,
                                                                const uint32_t* offsets, uint32_t nblocks,
@@ -1573,7 +1580,27 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const unsigned lo
     n = live_count(n_dev, n);
     __shared__ uint32_t wcount[kSortThreads / 64][256];
     const uint32_t wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+    if (kSelfScan) {
+        // this digit's count over all tiles and over the tiles before this one
+        const uint32_t* row = offsets + (size_t)threadIdx.x * nblocks;
+        uint32_t total = 0, before = 0;
+        for (uint32_t b = 0; b < nblocks; ++b) {
+            uint32_t v = row[b];
+            total += v;
+            before += b < blockIdx.x ? v : 0u;
+        }
+        running[threadIdx.x] = total;
+        __syncthreads();
+        for (uint32_t off = 1; off < 256; off <<= 1) {  // inclusive scan of the digit totals
+            uint32_t t = threadIdx.x >= off ? running[threadIdx.x - off] : 0u;
+            __syncthreads();
+            running[threadIdx.x] += t;
+            __syncthreads();
+        }
+        running[threadIdx.x] += before - total;
+    } else {
+        running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+    }
     uint32_t base = blockIdx.x * kSortTile;
     for (int r = 0; r < kSortRounds; ++r) {
         for (int k = 0; k < kSortThreads / 64; ++k) wcount[k][threadIdx.x] = 0;
@@ -1679,12 +1706,18 @@ void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, ui
     unsigned long long* kout = keys_b;
     uint32_t* vin = vals_a;
     uint32_t* vout = vals_b;
+    const bool self_scan = nblocks <= kSelfScanBlocks;
     for (int shift = 0; shift < bits && n > 0; shift += 8) {
         hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, count_dev, n, shift, hist,
                            nblocks);
-        launch_exclusive_scan_u32(hist, hist, 256 * nblocks, hist_sums, s);
-        hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev, n, shift,
-                           hist, nblocks, kout, vout);
+        if (self_scan) {
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev, n,
+                               shift, hist, nblocks, kout, vout);
+        } else {
+            launch_exclusive_scan_u32(hist, hist, 256 * nblocks, hist_sums, s);
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev,
+                               n, shift, hist, nblocks, kout, vout);
+        }
         unsigned long long* tk = kin;
         kin = kout;
         kout = tk;
