@@ -152,7 +152,8 @@ struct Engine {
     // captured iteration graph (option "graph"): photon map + camera pass
     int graph = 0;
     hipGraphExec_t graph_exec = nullptr;
-    hipGraphExec_t graph_exec2 = nullptr;  // with overlap: query sort, merge, resolve (graph_exec: the map)
+    hipGraphExec_t graph_exec2 = nullptr;  // with overlap: merge and resolve (graph_exec: the map)
+    hipGraphExec_t graph_exec3 = nullptr;  // with overlap: the query sort, replayed on stream2
     std::vector<uintptr_t> graph_key;  // buffers and sizes the graph was captured with
     DevBuf<IterParams> d_iter;
     IterParams* h_iter = nullptr;      // pinned staging of the per-iteration values
@@ -1277,18 +1278,15 @@ bool capture_graph(hipStream_t st, const std::function<void()>& body, hipGraphEx
 // normalisation) from d_iter, copied in before each replay; the query buffer
 // holds the most queries a camera pass can make (8 per path).  With option
 // overlap the camera trace is a direct launch on stream2 beside the light
-// pass and two graphs replay: the photon map after the light pass, then,
-// once stream has waited for the trace, the query sort, merge and resolve.
+// pass, a graph of the query sort replays behind it on stream2, and on stream
+// a graph of the photon map replays after the light pass, then, once stream
+// has waited for the sort, a graph of the merge and the resolve.
 bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     hipStream_t st = e.stream;
     float radius, r2, norm;
     iteration_params(e, iteration, radius, r2, norm);
     const uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
     if (!ensure_query_buffers(e, (uint32_t)std::min<uint64_t>(8ull * n_paths + 1024, 1ull << 31), n_paths)) return false;
-    const bool overlapped = e.overlap && !e.mlt;
-    CameraPassArgs ca{};
-    if (overlapped && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
-    if (!run_light_pass(e, iteration, count)) return false;
     long long N = light_paths(e);
     // the grid of this iteration, as build_map
     int uid_bits = log2_ceil((uint64_t)N) + 3;
@@ -1308,8 +1306,18 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     ip.r_query = rq;
     ip.inv_cell = 1.0f / cell;
     for (int k = 0; k < 3; ++k) ip.origin[k] = e.bbox.lo[k] - cell;
-    ip.vm_norm = norm * mlt_factor(e);
-    HIP_OK(hipMemcpyAsync(e.d_iter.p, e.h_iter, sizeof(IterParams), hipMemcpyHostToDevice, st));
+    // copied in before the overlapped trace forks (the query sort behind it reads the grid), else after the
+    // light pass (the MLT light pass sets the normalisation)
+    auto copy_iter = [&]() -> bool {
+        ip.vm_norm = norm * mlt_factor(e);
+        HIP_OK(hipMemcpyAsync(e.d_iter.p, e.h_iter, sizeof(IterParams), hipMemcpyHostToDevice, st));
+        return true;
+    };
+    const bool overlapped = e.overlap && !e.mlt;
+    CameraPassArgs ca{};
+    if (overlapped && !(copy_iter() && camera_trace_async(e, iteration, norm, e.image.p, ca))) return false;
+    if (!run_light_pass(e, iteration, count)) return false;
+    if (!overlapped && !copy_iter()) return false;
     PhotonMapView& pm = mi.pm;
     pm.pos = e.s_pos.p;
     pm.dir_tr = e.s_dir.p;
@@ -1330,7 +1338,8 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         if (e.graph_exec2) HIP_OK(hipGraphExecDestroy(e.graph_exec2));
-        e.graph_exec = e.graph_exec2 = nullptr;
+        if (e.graph_exec3) HIP_OK(hipGraphExecDestroy(e.graph_exec3));
+        e.graph_exec = e.graph_exec2 = e.graph_exec3 = nullptr;
         MapBuildArgs m{};
         m.in_pos = e.ph_pos.p;
         m.in_dir = e.ph_dir.p;
@@ -1373,15 +1382,19 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         a.camera_qmc = e.camera_qmc;
         a.dyn = e.d_iter.p;
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
-        auto camera = [&]() {
-            a.q_order = query_order(e, a, e.q_capacity, st);  // count read from the device
+        auto sort = [&]() { a.q_order = query_order(e, a, e.q_capacity, st); };  // count read from the device
+        auto merge = [&]() {
             launch_gather(a, gather_blocks(e), st);
             launch_resolve(a, st);
+        };
+        auto camera = [&]() {
+            sort();
+            merge();
         };
         bool ok;
         if (overlapped) {
             ok = capture_graph(st, [&]() { build_photon_map(m, st); }, e.graph_exec) &&
-                 capture_graph(st, camera, e.graph_exec2);
+                 capture_graph(st, sort, e.graph_exec3) && capture_graph(st, merge, e.graph_exec2);
         } else {
             ok = capture_graph(st,
                                [&]() {
@@ -1394,16 +1407,21 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         }
         if (!ok) {
             if (e.graph_exec) (void)hipGraphExecDestroy(e.graph_exec);
-            e.graph_exec = nullptr;
+            if (e.graph_exec3) (void)hipGraphExecDestroy(e.graph_exec3);
+            e.graph_exec = e.graph_exec3 = nullptr;
             return false;
         }
         e.graph_key = key;
         e.graph_captures++;
     }
+    if (overlapped) {  // the query sort behind the trace
+        HIP_OK(hipGraphLaunch(e.graph_exec3, e.stream2));
+        HIP_OK(hipEventRecord(e.ev_qdone, e.stream2));
+    }
     HIP_OK(hipEventRecord(e.ev[2], st));
     HIP_OK(hipGraphLaunch(e.graph_exec, st));
     if (overlapped) {
-        HIP_OK(hipStreamWaitEvent(st, e.ev_cam, 0));
+        HIP_OK(hipStreamWaitEvent(st, e.ev_qdone, 0));
         HIP_OK(hipGraphLaunch(e.graph_exec2, st));
     }
     HIP_OK(hipEventRecord(e.ev[5], st));
@@ -1784,6 +1802,7 @@ void release(Engine*& e) {
     e->guide.release();
     if (e->graph_exec) (void)hipGraphExecDestroy(e->graph_exec);
     if (e->graph_exec2) (void)hipGraphExecDestroy(e->graph_exec2);
+    if (e->graph_exec3) (void)hipGraphExecDestroy(e->graph_exec3);
     e->d_iter.release();
     if (e->h_iter) (void)hipHostFree(e->h_iter);
     if (e->stream) (void)hipStreamDestroy(e->stream);
