@@ -179,19 +179,19 @@ enum {
  * "gather_block" (0, 1, 2 or 4 photon steps per block of the merge),
  * "overlap" (1, the default: the camera trace runs on a second stream beside
  * the light pass and the photon-map build, which it does not read; the merge
- * waits for both.  Direct launches of one scene without "mlt" or
- * "count_events"; the film is the same).
+ * waits for both.  Direct launches and graphs of one scene without "mlt",
+ * and batches (scene 0's option); not with "count_events"; the film is the
+ * same).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects, "overlap" last, by the iteration's wall time (kept only
- * when it saves at least 3 %; the other knobs are timed with the camera trace
+ * when it saves at least 5 %; the other knobs are timed with the camera trace
  * serial).  report (up to RC_TUNE_REPORT
  * doubles): wall ms of one iteration before (camera trace serial) and after,
  * then path_chunk, light_blocks_per_cu, camera_blocks_per_cu, cell_scale,
  * gather_blocks_per_cu, light_refill, light_order, camera_refill,
- * group_queries, gather_tile, gather_block, query_order, overlap (always 0
- * from rc_batch_autotune: a batch traces its cameras serially). */
+ * group_queries, gather_tile, gather_block, query_order, overlap. */
 enum { RC_TUNE_REPORT = 15 };
 /* Option "graph" = 1: after the light pass (a direct launch whose photon
  * count is checked on the host), the photon-map build and the camera pass of

@@ -603,14 +603,15 @@ bool ensure_device(Engine& e) {
 
 // The second stream, its join event and counters (overlapped camera trace), on first use.
 bool ensure_stream2(Engine& e) {
-    if (e.stream2) return true;
+    if (e.stream2) return true;  // created last: everything else is in place
+    if (!e.ev_cam) HIP_OK(hipEventCreateWithFlags(&e.ev_cam, hipEventDisableTiming));
+    if (!e.ev_join) HIP_OK(hipEventCreate(&e.ev_join));
+    if (!e.ev_qdone) HIP_OK(hipEventCreate(&e.ev_qdone));
+    if (!e.ev_fork) HIP_OK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    if (!e.h_nq) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_nq), sizeof(uint32_t), hipHostMallocDefault));
+    if (!e.cam_counters.alloc(kPathCounters)) return false;
     HIP_OK(hipStreamCreate(&e.stream2));
-    HIP_OK(hipEventCreateWithFlags(&e.ev_cam, hipEventDisableTiming));
-    HIP_OK(hipEventCreate(&e.ev_join));
-    HIP_OK(hipEventCreate(&e.ev_qdone));
-    HIP_OK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
-    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_nq), sizeof(uint32_t), hipHostMallocDefault));
-    return e.cam_counters.alloc(kPathCounters);
+    return true;
 }
 
 // Linear BVH built on the device over the triangles uploaded in input order
@@ -1536,7 +1537,7 @@ bool render_iteration(Engine& e, int iteration) {
 // affects (best of `repeats`), one knob after the other.  The last knob,
 // overlap, is judged by the iteration's wall time: the camera trace beside
 // the light pass shortens the iteration only while the light pass leaves
-// compute units idle, and it is kept only when it saves at least 3 % (beside
+// compute units idle, and it is kept only when it saves at least 5 % (beside
 // the trace the light pass's own time no longer measures that kernel alone).
 // ---------------------------------------------------------------------
 enum { T_LIGHT = 0, T_MAP, T_TRACE, T_GATHER, T_WALL, T_PARTS };
@@ -1550,7 +1551,7 @@ struct TuneKnob {
     double margin = 0.0;  // a later candidate must beat the best so far by this fraction
 };
 
-// overlap_candidates: {0, 1} for one scene, {0} for a batch (its camera trace is serial).
+// overlap_candidates: {0, 1}, tried after the other knobs (timed with the camera trace serial).
 std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::function<void(double)> set_cell_scale,
                                  std::function<double()> get_cell_scale, std::vector<double> overlap_candidates) {
     auto L = [](const double* t) { return t[T_LIGHT]; };
@@ -1597,7 +1598,7 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"overlap", overlap_candidates,
          [=](double v) { launch_owner().overlap = (int)v; },
          [=]() { return (double)launch_owner().overlap; },
-         [](const double* t) { return t[T_WALL]; }, 0.03},
+         [](const double* t) { return t[T_WALL]; }, 0.05},
     };
 }
 
@@ -1993,6 +1994,7 @@ bool batch_prepare(Batch& b) {
     b.work.group_queries = b.scenes[0]->group_queries;
     b.work.gather_tile = b.scenes[0]->gather_tile;
     b.work.gather_block = b.scenes[0]->gather_block;
+    b.work.overlap = b.scenes[0]->overlap;
     if (!b.work.counters.alloc(kPathCounters) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
@@ -2148,6 +2150,47 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     view.guides = b.d_guides.p;
     const Engine& e0 = *b.scenes[0];
     const bool counting = e0.count_events != 0;  // event counters of the first scene's option
+    // camera trace and query sort on stream2 beside the light pass and the map (option overlap of the
+    // batch's launch options; not while counting events): the trace reads only the scene tables above
+    const bool overlapped = w.overlap && !counting;
+    uint32_t qcap = std::max(w.q_capacity, (uint32_t)std::min<uint64_t>(2 * camera + 1024, 1ull << 31));
+    if (!ensure_query_buffers(w, qcap, (uint32_t)camera)) return false;
+    CameraPassArgs c{};
+    c.sc = scene_view(e0);
+    c.iteration = (uint32_t)e0.iterations_done;
+    c.stream = STREAM_CAMERA;
+    c.n_paths = (uint32_t)camera;
+    c.path_chunk = w.path_chunk;
+    c.refill_min = w.camera_refill;
+    c.path_counter = w.counters.p;
+    c.path_info = w.path_info.p;
+    c.contrib = w.contrib.p;
+    c.exact_merge = e0.exact_merge;
+    c.group_queries = w.group_queries;
+    c.gather_tile = w.gather_tile;
+    c.gather_block = w.gather_block;
+    c.camera_qmc = e0.camera_qmc;
+    c.batch = view;
+    c.counters = counting ? w.stats_dev.p + 1 : nullptr;
+    if (overlapped) {
+        if (!ensure_stream2(w)) return false;
+        hipStream_t s2 = w.stream2;
+        c.path_counter = w.cam_counters.p;
+        c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
+        HIP_OK(hipEventRecord(w.ev_fork, st));
+        HIP_OK(hipStreamWaitEvent(s2, w.ev_fork, 0));
+        launch_fill_u32(w.cam_counters.p, 0u, kPathCounters, s2);
+        HIP_OK(hipEventRecord(w.ev[4], s2));
+        launch_camera_trace(c, camera_blocks(w), s2);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(w.h_nq, w.cam_counters.p + 1, 4, hipMemcpyDeviceToHost, s2));
+        HIP_OK(hipEventRecord(w.ev[8], s2));
+        HIP_OK(hipEventRecord(w.ev_cam, s2));
+        HIP_OK(hipEventRecord(w.ev_join, s2));
+        c.q_order = query_order(w, c, w.q_capacity, s2);  // count read from the device
+        HIP_OK(hipEventRecord(w.ev_qdone, s2));
+    }
+
     // light pass (grow and repeat on overflow)
     uint32_t count = 0;
     for (int attempt = 0;; ++attempt) {
@@ -2209,28 +2252,24 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     build_photon_map_batch(m, view, st);
     HIP_OK(hipEventRecord(w.ev[3], st));
     HIP_OK(hipGetLastError());
-    // camera pass
-    uint32_t qcap = std::max(w.q_capacity, (uint32_t)std::min<uint64_t>(2 * camera + 1024, 1ull << 31));
-    if (!ensure_query_buffers(w, qcap, (uint32_t)camera)) return false;
-    CameraPassArgs c{};
-    c.sc = scene_view(e0);
-    c.iteration = (uint32_t)e0.iterations_done;
-    c.stream = STREAM_CAMERA;
-    c.n_paths = (uint32_t)camera;
-    c.path_chunk = w.path_chunk;
-    c.refill_min = w.camera_refill;
-    c.path_counter = w.counters.p;
-    c.path_info = w.path_info.p;
-    c.contrib = w.contrib.p;
-    c.exact_merge = e0.exact_merge;
-    c.group_queries = w.group_queries;
-    c.gather_tile = w.gather_tile;
-    c.gather_block = w.gather_block;
-    c.camera_qmc = e0.camera_qmc;
-    c.batch = view;
-    c.counters = counting ? w.stats_dev.p + 1 : nullptr;
+    // camera pass: the overlapped trace's count, or (serial, or after an overflow) the trace here
     uint32_t nq = 0;
-    for (int attempt = 0;; ++attempt) {
+    bool serial = !overlapped;
+    if (overlapped) {
+        HIP_OK(hipEventSynchronize(w.ev_cam));
+        nq = *w.h_nq;
+        if (nq > w.q_capacity) {  // trace again, serially, into larger buffers
+            HIP_OK(hipStreamSynchronize(w.stream2));
+            if (!ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
+                set_error("merge query buffer overflow");
+                return false;
+            }
+            c.path_counter = w.counters.p;
+            c.q_order = nullptr;
+            serial = true;
+        }
+    }
+    for (int attempt = 0; serial; ++attempt) {
         c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
         launch_fill_u32(w.counters.p, 0u, kPathCounters, st);
         if (counting && attempt > 0)
@@ -2247,8 +2286,14 @@ bool batch_iteration(Batch& b, bool scratch = false) {
             return false;
         }
     }
-    HIP_OK(hipEventRecord(w.ev[8], st));
-    c.q_order = query_order(w, c, nq, st);
+    if (serial) {
+        HIP_OK(hipEventRecord(w.ev[8], st));
+        if (overlapped) HIP_OK(hipEventRecord(w.ev_join, st));
+        c.q_order = query_order(w, c, nq, st);
+        if (overlapped) HIP_OK(hipEventRecord(w.ev_qdone, st));
+    } else {
+        HIP_OK(hipStreamWaitEvent(st, w.ev_qdone, 0));  // the trace and the query sort on stream2
+    }
     HIP_OK(hipEventRecord(w.ev[6], st));
     launch_gather(c, gather_blocks(w), st);
     HIP_OK(hipEventRecord(w.ev[7], st));
@@ -2264,7 +2309,8 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     HIP_OK(hipEventElapsedTime(&ms[1], w.ev[2], w.ev[3]));
     HIP_OK(hipEventElapsedTime(&ms[2], w.ev[4], w.ev[5]));
     HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[8]));
-    HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
+    if (overlapped) HIP_OK(hipEventElapsedTime(&ms[6], w.ev_join, w.ev_qdone));
+    else HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
     HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
     HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
     std::fill(b.stats, b.stats + RC_STAT_COUNT, 0.0);
@@ -2310,7 +2356,9 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
                             [pb](double v) {
                                 for (Engine* e : pb->scenes) e->cell_scale = v;
                             },
-                            [pb]() { return pb->scenes[0]->cell_scale; }, {0});
+                            [pb]() { return pb->scenes[0]->cell_scale; }, {0, 1});
+    const int overlap = b.work.overlap;
+    b.work.overlap = 0;  // the other knobs are timed with the camera trace serial
     auto timed = [pb](double* t) {
         auto w0 = std::chrono::steady_clock::now();
         if (!batch_iteration(*pb, true)) return false;
@@ -2321,7 +2369,9 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
         t[T_GATHER] = pb->stats[RC_STAT_MS_GATHER];
         return true;
     };
-    return run_autotune(knobs, timed, repeats, report);
+    bool ok = run_autotune(knobs, timed, repeats, report);
+    if (!ok) b.work.overlap = overlap;
+    return ok;
 }
 
 }  // namespace
