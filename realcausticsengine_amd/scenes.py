@@ -185,6 +185,50 @@ def random_mesh_scene(seed: int = 0, n_spheres: int = 3, subdiv: int = 2, width:
     return SceneArgs(meshes, lights, mats, cam, samples=1)
 
 
+def materials_scene(width: int = 48, height: int = 32, subdiv: int = 2, lights=None) -> SceneArgs:
+    """One sphere of each BSDF class of the reference in a row over a glossy floor.
+
+    Spheres, left to right: glass (``Fresnel_Specular``), mirror metal
+    (``Specular_Reflection``, roughness < 0.01), rough metal
+    (``Microfacet_Reflection_GGX``), glossy plastic with specular > 0
+    (``Fresnel_Microfacet_GGX``), smooth plastic (``Fresnel_Specular_Microfacet_GGX``,
+    roughness^2 < 0.01) and a diffuse plastic (specular 0); the floor is a
+    glossy plastic with specular > 0, so merge queries on it take the general
+    BSDF evaluation (``src/materials.h:59-146``).  A 2 x 1 area light rotated
+    by (10, 0, 20) degrees sits above, unless `lights` replaces it."""
+    mats = [MaterialData("G", ior=1.45, color=(0.9, 0.95, 1.0)),
+            MaterialData("M", roughness=0.0, color=(0.9, 0.9, 0.9)),
+            MaterialData("M", roughness=0.35, color=(0.9, 0.8, 0.7)),
+            MaterialData("P", roughness=0.5, specular=0.6, color=(0.3, 0.6, 0.3)),
+            MaterialData("P", roughness=0.05, specular=0.5, color=(0.5, 0.6, 0.7)),
+            MaterialData("P", roughness=1.0, specular=0.0, color=(0.8, 0.7, 0.6)),
+            MaterialData("P", roughness=0.6, specular=0.8, color=(0.7, 0.7, 0.7))]
+    sv, sn, st = icosphere(subdiv)
+    meshes = []
+    for k in range(6):
+        m = np.eye(4, dtype=np.float32) * 0.7
+        m[3, 3] = 1.0
+        m[:3, 3] = (1.8 * (k - 2.5), 0.3 * (k % 2), 0.75)
+        meshes.append(MeshData(sv, sn, st, matrix=m, material=k))
+    pv, pn, pt = plane(14.0, 8.0, 0.0)
+    meshes.append(MeshData(pv, pn, pt, material=6))
+    if lights is None:
+        lights = [LightData("A", position=(0.0, 0.0, 5.0), width=2.0, height=1.0, power=2.0,
+                            rotation=(10.0, 0.0, 20.0))]
+    cam = look_at_camera(width, height, (0.0, -8.0, 4.5), (0.0, 0.0, 0.5), 50.0)
+    return SceneArgs(meshes, lights, mats, cam, samples=1, base_radius=0.01, radius_reduction=0.75)
+
+
+def mixed_lights() -> list:
+    """A rotated area light, a sun, a point light and a sphere light of
+    different powers and colours (``src/Lights.cpp``, ``src/Lights.h:78-81``)."""
+    return [LightData("A", position=(-1.0, 0.5, 5.0), width=1.5, height=0.8, power=1.5, color=(1.0, 0.9, 0.8),
+                      rotation=(15.0, -10.0, 30.0)),
+            LightData("S", power=0.4, color=(1.0, 1.0, 0.9), rotation=(0.35, -0.25, 0.6)),
+            LightData("P", position=(1.5, -0.5, 4.0), power=3.0, color=(0.8, 0.9, 1.0)),
+            LightData("P", position=(0.5, 1.0, 3.5), power=2.0, color=(0.9, 0.7, 1.0), radius=0.3)]
+
+
 _ICO_CACHE: dict = {}
 
 
