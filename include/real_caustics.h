@@ -203,6 +203,16 @@ int rc_autotune(int repeats, double *report, int n);
 
 /* ---- diagnostics used by the parity tests ---- */
 
+/* Shape of the BVH that rc_prepare builds on the host (binned SAH, then the
+ * 4-wide collapse) over n world-space triangles of 9 floats each; host
+ * memory only, no GPU call.  out4 = {internal ancestors of the deepest leaf
+ * of the binary tree, depth of the deepest internal 4-wide node, traversal
+ * width kept (requested_width 4 or 2; 2 when the 4-wide traversal's stack,
+ * 3 entries per 4-wide level, could exceed its 64 entries; 0 when neither
+ * fits), stack entries the kept traversal can need}.  Returns 0, or -1 when
+ * the tree cannot be traversed (rc_prepare then fails the same way). */
+int rc_bvh_shape(const float *tris, long long n, int requested_width, int *out4);
+
 /* Closest hit for n rays {o.xyz, d.xyz}; tri = global triangle id (mesh order,
  * then triangle order) or -1; t = distance. */
 int rc_first_hit(const float *rays, long long n, float tmin, int *tri, float *t);

@@ -98,7 +98,8 @@ struct Engine {
     int root4 = 0, n_nodes4 = 0;
     int bvh_width = 4;  // 4: traverse the 4-wide BVH, 2: the binary one
     int bvh_build = 0;  // 0: binned SAH on the host, 1: linear BVH on the device (binary nodes)
-    int bvh_depth = 0;
+    int bvh_depth = 0;   // internal ancestors of the deepest leaf of the binary tree
+    int bvh_depth4 = 0;  // depth of the deepest internal 4-wide node (root 0)
     double bvh_build_ms = 0.0;
     // per-iteration buffers
     DevBuf<float4> ph_pos, ph_dir, s_pos, s_dir;
@@ -637,7 +638,9 @@ bool gpu_bvh(Engine& e, uint32_t n) {
         e.n_nodes4 = 0;
         e.root4 = 0;
         bool collapsed = true;
-        if (ok && d >= 0 && d < kMaxDepth && e.bvh_width == 4) {
+        // the collapse keeps the internal nodes of even depth: the deepest
+        // internal node (depth d - 1) lies under (d - 1) / 2 4-wide ancestors
+        if (ok && d >= 1 && traversal_width(e.bvh_width, d, (d - 1) / 2) == 4) {
             const int n4 = lbvh_wide_count(b, n, e.stream);
             collapsed = n4 > 0 && e.d_nodes4.alloc(8 * (size_t)n4) && lbvh_collapse4(b, n, e.d_nodes4.p, e.stream);
             if (collapsed) e.n_nodes4 = n4;
@@ -665,11 +668,12 @@ bool gpu_bvh(Engine& e, uint32_t n) {
         set_error("GPU BVH build failed");
         return false;
     }
-    if (d >= kMaxDepth) {
+    if (traversal_width(2, d, 0) == 0) {
         set_error("BVH deeper than the traversal stack");
         return false;
     }
     e.bvh_depth = d;
+    e.bvh_depth4 = d >= 1 ? (d - 1) / 2 : 0;
     if (e.n_nodes4 == 0) e.d_nodes4.release();
     return true;
 }
@@ -720,10 +724,6 @@ bool prepare(Engine& e) {
     } else {
         bld.order.reserve(ntri);
         root = bld.build(0, (int)ntri, 0, rootb);
-        if (bld.max_depth >= kMaxDepth) {
-            set_error("BVH deeper than the traversal stack");
-            return false;
-        }
     }
     if (!gpu_tree && root < 0) {
         // whole scene in one leaf: wrap it in a node whose second child is an empty box
@@ -747,7 +747,19 @@ bool prepare(Engine& e) {
         nodes4.reserve(bld.nodes.size());
         int depth4 = 0;
         e.root4 = collapse4(bld.nodes, root, nodes4, 0, depth4);
-        e.bvh_depth = bld.max_depth;
+        e.bvh_depth = std::max(1, bld.max_depth);
+        e.bvh_depth4 = depth4;
+        // a 4-wide tree whose traversal stack could exceed kMaxDepth entries is
+        // dropped for the binary traversal (one push per level)
+        const int width = traversal_width(e.bvh_width, e.bvh_depth, depth4);
+        if (width == 0) {
+            set_error("BVH deeper than the traversal stack");
+            return false;
+        }
+        if (width != 4) {
+            nodes4.clear();
+            e.root4 = 0;
+        }
         e.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     }
     e.n_nodes4 = (int)(nodes4.size() / 8);
@@ -2651,6 +2663,8 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "mlt_mutations") return g->mlt_last[1];
     if (k == "mlt_accepted") return g->mlt_last[2];
     if (k == "bvh_depth") return g->bvh_depth;
+    if (k == "bvh_depth4") return g->bvh_depth4;
+    if (k == "bvh_traversal_width") return uses_nodes4(*g) ? 4 : 2;
     if (k == "bvh_build_ms") return g->bvh_build_ms;
     if (k == "nodes4") return g->n_nodes4;
     if (k == "graph_captures") return g->graph_captures;
@@ -2735,6 +2749,40 @@ __attribute__((visibility("default"))) int rc_stats(double* out, int n) {
     int m = std::min(n, (int)RC_STAT_COUNT);
     for (int i = 0; i < m; ++i) out[i] = g->stats[i];
     return m;
+}
+
+__attribute__((visibility("default"))) int rc_bvh_shape(const float* tris, long long n, int requested_width,
+                                                        int* out4) {
+    // the host build of rc_prepare (binned SAH, then the 4-wide collapse) on
+    // the given triangles; host memory only, no device call
+    if (!tris || n <= 0 || n >= (1LL << 28) || !out4) return -1;
+    Builder bld;
+    bld.items.resize((size_t)n);
+    for (long long t = 0; t < n; ++t) {
+        BuildItem& it = bld.items[(size_t)t];
+        it.id = (uint32_t)t;
+        for (int k = 0; k < 3; ++k) {
+            const float a = tris[9 * t + k], b = tris[9 * t + 3 + k], c = tris[9 * t + 6 + k];
+            it.lo[k] = smin(smin(a, b), c);
+            it.hi[k] = smax(smax(a, b), c);
+            it.c[k] = (it.lo[k] + it.hi[k]) * 0.5f;
+        }
+    }
+    bld.order.reserve((size_t)n);
+    Box rootb;
+    const int root = bld.build(0, (int)n, 0, rootb);
+    int depth4 = 0;
+    if (root >= 0) {
+        std::vector<float4> nodes4;
+        collapse4(bld.nodes, root, nodes4, 0, depth4);
+    }
+    const int depth = std::max(1, bld.max_depth);
+    const int width = traversal_width(requested_width, depth, depth4);
+    out4[0] = depth;
+    out4[1] = depth4;
+    out4[2] = width;
+    out4[3] = width == 4 ? stack_need_wide(depth4) : stack_need_binary(depth);
+    return width == 0 ? -1 : 0;
 }
 
 __attribute__((visibility("default"))) int rc_first_hit(const float* rays, long long n, float tmin, int* tri, float* t) {

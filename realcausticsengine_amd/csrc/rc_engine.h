@@ -27,6 +27,23 @@ constexpr int kLdsStack = 8;     // traversal stack entries ({node ref, entry di
 constexpr int kMaxDepth = 64;    // total stack capacity (LDS + private overflow)
 constexpr int kEmptyChild = (int)0x80000000;  // missing child of a 4-wide BVH node
 
+// Traversal stack entries a tree can need.  The binary traversal pushes at
+// most one entry (the far child) per internal ancestor of the node it is in,
+// so a tree whose deepest leaf has `leaf_depth` internal ancestors needs that
+// many.  The 4-wide traversal pushes at most three entries per 4-wide node on
+// the path, so a tree whose deepest internal 4-wide node sits at depth
+// `depth4` (root 0) needs 3 (depth4 + 1).  The builders keep a tree only when
+// the traversal that runs it fits kMaxDepth; the kernels do not check.
+constexpr int stack_need_binary(int leaf_depth) { return leaf_depth; }
+constexpr int stack_need_wide(int depth4) { return 3 * (depth4 + 1); }
+// Traversal width for a tree: `requested` (4 or 2) when its stack fits,
+// else 2 when the binary stack fits, else 0 (the tree cannot be traversed).
+constexpr int traversal_width(int requested, int leaf_depth, int depth4) {
+    return (requested == 4 && stack_need_wide(depth4) <= kMaxDepth)
+               ? 4
+               : (stack_need_binary(leaf_depth) <= kMaxDepth ? 2 : 0);
+}
+
 struct DevMaterial {
     int kind;
     float R[3];

@@ -25,7 +25,7 @@ EXPORTS = [
     "rc_guide_get_params", "rc_guide_set_params", "rc_guide_forward", "rc_guide_train_step", "rc_guide_table",
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
-    "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug",
+    "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug", "rc_bvh_shape",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
@@ -117,6 +117,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_batch_release.restype = None
     lib.rc_render_debug.argtypes = [ctypes.c_int]
     lib.rc_render_debug.restype = ctypes.c_int
+    lib.rc_bvh_shape.argtypes = [P(ctypes.c_float), ctypes.c_longlong, ctypes.c_int, P(ctypes.c_int)]
+    lib.rc_bvh_shape.restype = ctypes.c_int
     for fn in (lib.rc_autotune, lib.rc_batch_autotune):
         fn.argtypes = [ctypes.c_int, P(ctypes.c_double), ctypes.c_int]
         fn.restype = ctypes.c_int
@@ -340,6 +342,17 @@ class Batch:
 
     def release(self):
         self.lib.rc_batch_release()
+
+
+def bvh_shape(tris: np.ndarray, width: int = 4) -> dict:
+    """Shape of the host-built BVH over (n, 3, 3) world-space triangles
+    (rc_bvh_shape; no GPU needed): binary depth, 4-wide depth, traversal width
+    kept and the stack entries it can need (ok False: the tree cannot be traversed)."""
+    t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    out = np.zeros(4, dtype=np.int32)
+    rc = load().rc_bvh_shape(_fp(t), len(t), int(width), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return {"depth": int(out[0]), "depth4": int(out[1]), "width": int(out[2]), "stack_need": int(out[3]),
+            "ok": rc == 0}
 
 
 def photons_from_oracle(records: np.ndarray) -> np.ndarray:

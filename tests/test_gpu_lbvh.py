@@ -127,3 +127,47 @@ def test_lbvh_two_million_triangles_match_host_tree():
         assert len(common) >= 0.999 * max(len(uid_l), len(uid_h))
         exact = np.all(ph_l[ia].view(np.uint32) == ph_h[ib].view(np.uint32), axis=1)
         assert exact.mean() >= 0.999, (key, exact.mean())
+
+
+@pytest.mark.parametrize("build", [0, 1])
+def test_deep_tree_traverses_within_its_stack(oracle, build):
+    """A tree whose 4-wide traversal could need more than the 64 stack
+    entries the kernels hold (tests/test_bvh_stack_cpu.py builds it): the
+    host build keeps the binary traversal, and first hits still equal brute
+    force on both builds."""
+    import sys
+    import os
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_bvh_stack_cpu import _deep_spine
+    from realcausticsengine_amd.engine import bvh_shape
+
+    tris = _deep_spine()
+    shape = bvh_shape(tris)
+    assert shape["width"] == 2 and shape["stack_need"] <= 64
+    verts = tris.reshape(-1, 3)
+    idx = np.arange(len(verts), dtype=np.uint32).reshape(-1, 3)
+    normals = np.tile(np.asarray([0.0, 0.0, 1.0]), (len(verts), 1))
+    args = SceneArgs([MeshData(verts, normals, idx, material=0)],
+                     [LightData("A", position=(0.0, 0.0, 5.0), width=1.0, height=1.0, power=1.0)],
+                     [MaterialData("P", roughness=1.0, specular=0.0)],
+                     look_at_camera(8, 8, (0.0, -6.0, 4.0), (0.0, 0.0, 0.0), 40.0), samples=1)
+    eng = Engine(args, bvh_build=build)
+    eng.prepare()
+    print(f"build {build}: depth {eng.get('bvh_depth'):.0f}, depth4 {eng.get('bvh_depth4'):.0f}, "
+          f"width {eng.get('bvh_traversal_width'):.0f}", flush=True)
+    if build == 0:
+        assert eng.get("bvh_traversal_width") == 2 and eng.get("bvh_depth") == shape["depth"]
+    w = eng.get("bvh_traversal_width")
+    need = 3 * (eng.get("bvh_depth4") + 1) if w == 4 else eng.get("bvh_depth")
+    assert need <= 64
+    # vertical rays onto the centroids of the triangles from 17^-1 to 17^11
+    # (Triangle::hit rejects the smaller ones by its determinant epsilon)
+    c = tris.astype(np.float64).mean(axis=1)[32 * 11:32 * 24]
+    o = c + np.asarray([0.0, 0.0, 1.0]) * np.maximum(1e-3, 0.1 * np.abs(c[:, :1]))
+    d = np.tile(np.asarray([0.0, 0.0, -1.0]), (len(c), 1))
+    rays = np.concatenate([o, d], axis=1).astype(np.float32)
+    orc = oracle.OracleScene(args)
+    tri_g, t_g = eng.first_hit(rays, tmin=0.0)
+    tri_o, t_o = orc.first_hit(rays, tmin=0.0, brute=True)
+    assert _check_hits(tri_g, t_g, tri_o, t_o) >= 0.9 * len(rays)
+    eng.release()
