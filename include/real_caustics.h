@@ -225,6 +225,19 @@ long long rc_trace_photons(int iteration, float *photons, unsigned int *uid, lon
  * CSR output; returns the total (indices beyond cap are not written). */
 long long rc_gather_sets(const float *positions, long long count, const float *queries, long long nq,
                          float r2, long long *offsets, int *indices, long long cap);
+/* rc_gather_sets without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture set-up, tight serving loops, harnesses that forbid
+ * hipMalloc).  positions, queries, offsets (nq + 1 int64) and indices (cap)
+ * are DEVICE pointers; the photon map and the per-query counts use a fixed
+ * scratch array of about 25 MB that the library holds on each GPU until the
+ * process ends, so count and nq are at most 262144 each (larger sizes are an
+ * error).  The scene must be prepared (rc_prepare) first; the grid is the one
+ * rc_gather_sets builds, so the sets are the same.  Returns the total set
+ * size; indices are written only when it is <= cap.  Synchronous.  Calls that
+ * overlap on one GPU share its scratch array and race. */
+long long rc_gather_sets_noalloc(const float *positions, long long count, const float *queries, long long nq,
+                                 float r2, long long *offsets, int *indices, long long cap);
 /* Camera pass of one iteration against a given photon list (10 floats each,
  * as rc_trace_photons); adds into image (width*height*3). */
 int rc_camera_pass_photons(int iteration, const float *photons, long long count, float *image);

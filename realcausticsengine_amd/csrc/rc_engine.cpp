@@ -991,8 +991,9 @@ struct MapGrid {
     PhotonMapView pm{};
 };
 
-bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
-    long long N = light_paths(e);
+// The photon-map grid for N light paths (uids below 8 N) and a radius,
+// over the scene's bounds; allocates nothing, sets no array of g.pm.
+bool map_grid_params(const Engine& e, long long N, float radius, float r2, MapGrid& g) {
     g.uid_bits = log2_ceil((uint64_t)N) + 3;
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
     g.bucket_bits = log2_ceil(want);
@@ -1001,22 +1002,28 @@ bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
         return false;
     }
     g.buckets = 1u << g.bucket_bits;
-    if (!e.cells.alloc(g.buckets)) return false;
     float maxc = 0.0f;
     for (int k = 0; k < 3; ++k) maxc = std::max(maxc, std::max(std::fabs(e.bbox.lo[k]), std::fabs(e.bbox.hi[k])));
     float rq = radius + std::max(radius * 1e-3f, 8.0f * maxc * 1.1920929e-07f);
     // cells of at least 2/3 of the query radius: a query touches at most 4 per axis
     g.cell = rq * (float)std::max(0.67, e.cell_scale);
     PhotonMapView& pm = g.pm;
-    pm.pos = e.s_pos.p;
-    pm.dir_tr = e.s_dir.p;
-    pm.tp_gb = e.s_tp.p;
-    pm.cells = e.cells.p;
     pm.mask = g.buckets - 1;
     for (int k = 0; k < 3; ++k) pm.origin[k] = e.bbox.lo[k] - g.cell;
     pm.inv_cell = 1.0f / g.cell;
     pm.r2 = r2;
     pm.r_query = rq;
+    return true;
+}
+
+bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
+    if (!map_grid_params(e, light_paths(e), radius, r2, g)) return false;
+    if (!e.cells.alloc(g.buckets)) return false;
+    PhotonMapView& pm = g.pm;
+    pm.pos = e.s_pos.p;
+    pm.dir_tr = e.s_dir.p;
+    pm.tp_gb = e.s_tp.p;
+    pm.cells = e.cells.p;
     return true;
 }
 
@@ -2923,6 +2930,45 @@ __attribute__((visibility("default"))) long long rc_gather_sets(const float* pos
     doff.release();
     dout.release();
     if (total < 0) set_error("rc_gather_sets failed");
+    return total;
+}
+
+__attribute__((visibility("default"))) long long rc_gather_sets_noalloc(const float* positions, long long count,
+                                                                        const float* queries, long long nq, float r2,
+                                                                        long long* offsets, int* indices,
+                                                                        long long cap) {
+    // no device allocation: the scene must be prepared already, and the
+    // photon map and counts go into the library's fixed scratch array
+    if (!g || !g->prepared) {
+        set_error("rc_gather_sets_noalloc: the scene is not prepared (call rc_prepare first)");
+        return -1;
+    }
+    if (count < 0 || nq < 0 || count > (long long)kNoAllocMaxPhotons || nq > (long long)kNoAllocMaxQueries) {
+        set_error("rc_gather_sets_noalloc: at most 262144 photons and 262144 queries");
+        return -1;
+    }
+    if ((count > 0 && !positions) || (nq > 0 && !queries) || !offsets || (cap > 0 && !indices) || cap < 0) {
+        set_error("rc_gather_sets_noalloc: null buffer");
+        return -1;
+    }
+    Engine& e = *g;
+    MapGrid gr;
+    // the grid rc_gather_sets builds for the same input (uid bits covering the indices)
+    if (!map_grid_params(e, std::max<long long>(1, (count + 7) / 8 + 1), sqrtf(r2), r2, gr)) return -1;
+    NoAllocGather a{};
+    a.positions = positions;
+    a.count = (uint32_t)count;
+    a.queries = queries;
+    a.nq = (uint32_t)nq;
+    a.pm = gr.pm;
+    a.bucket_bits = gr.bucket_bits;
+    a.uid_bits = gr.uid_bits;
+    a.offsets = reinterpret_cast<unsigned long long*>(offsets);
+    a.indices = indices;
+    a.cap = cap;
+    const char* err = nullptr;
+    long long total = gather_sets_noalloc(a, e.stream, &err);
+    if (total < 0) set_error(err ? err : "rc_gather_sets_noalloc failed");
     return total;
 }
 

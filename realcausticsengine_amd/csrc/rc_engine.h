@@ -317,6 +317,27 @@ size_t photon_map_hist_entries(uint32_t max_count);
 size_t photon_map_sum_entries(uint32_t max_count);
 
 // Radius sets through the photon map (count then fill).
+// Radius sets without device allocation (rc_gather_sets_noalloc): the
+// photon-map build's scratch and the per-query counts live in one
+// file-scope device array sized for these maxima.
+constexpr uint32_t kNoAllocMaxPhotons = 1u << 18;
+constexpr uint32_t kNoAllocMaxQueries = 1u << 18;
+constexpr uint32_t kNoAllocMaxBuckets = 1u << 15;
+struct NoAllocGather {
+    const float* positions;  // device, 3 floats per photon
+    uint32_t count;
+    const float* queries;  // device, 3 floats per query
+    uint32_t nq;
+    PhotonMapView pm;  // the grid (origin, inv_cell, mask, r2, r_query); the launcher sets its arrays
+    int bucket_bits, uid_bits;
+    unsigned long long* offsets;  // device, nq + 1
+    int* indices;                 // device, cap
+    long long cap;
+};
+// Returns the total set size (indices written only when it is <= cap), or -1
+// with *err set.  Synchronous on `s`.
+long long gather_sets_noalloc(const NoAllocGather& a, hipStream_t s, const char** err);
+size_t noalloc_scratch_bytes();
 void launch_gather_count(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig, uint32_t* counts,
                          hipStream_t s);
 void launch_gather_fill(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig,

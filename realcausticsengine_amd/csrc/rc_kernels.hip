@@ -13,6 +13,8 @@
 //  k_gather       the merge of each query (Path_Vertices.h:46-68), one wave
 //                 per query with ballot-compacted BSDF evaluation.
 //  k_resolve      per-path sum of the vertex contributions into the film.
+#include <mutex>
+
 #include "rc_device.h"
 
 namespace rc {
@@ -1862,6 +1864,157 @@ void launch_gather_fill(const PhotonMapView& pm, const float* q, int nq, const u
                         const unsigned long long* offsets, int* out, hipStream_t s) {
     if (nq <= 0) return;
     hipLaunchKernelGGL(k_gather_fill, dim3((nq + 127) / 128), dim3(128), 0, s, pm, q, nq, orig, offsets, out);
+}
+
+// ---------------------------------------------------------------------
+// Radius sets without device allocation.  Every array beyond the caller's
+// inputs and outputs is a slice of `scratch`, one file-scope device array
+// per GPU, held until the process ends.  Slices (256-byte aligned):
+// photons {pos, 0} and their indices as uids, the sort's keys and values
+// (two of each), its histograms and scan partials, the sorted photons and
+// their input indices, the bucket ranges, and the per-query counts.  The
+// permute's direction and throughput inputs alias the position slice (the
+// sets read positions only).  Calls that overlap on one GPU race on it.
+// ---------------------------------------------------------------------
+namespace {
+constexpr size_t slice(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+constexpr size_t kNaP = kNoAllocMaxPhotons;
+constexpr size_t kNaHist = 256 * ((kNaP + kSortTile - 1) / kSortTile);
+constexpr size_t kNaSums = (kNaHist + kScanTile - 1) / kScanTile + 1;
+constexpr size_t kNaInPos = 0;
+constexpr size_t kNaInUid = kNaInPos + slice(16 * kNaP);
+constexpr size_t kNaKeysA = kNaInUid + slice(4 * kNaP);
+constexpr size_t kNaKeysB = kNaKeysA + slice(8 * kNaP);
+constexpr size_t kNaValsA = kNaKeysB + slice(8 * kNaP);
+constexpr size_t kNaValsB = kNaValsA + slice(4 * kNaP);
+constexpr size_t kNaHistAt = kNaValsB + slice(4 * kNaP);
+constexpr size_t kNaSumsAt = kNaHistAt + slice(4 * kNaHist);
+constexpr size_t kNaOutPos = kNaSumsAt + slice(4 * kNaSums);
+constexpr size_t kNaOutDir = kNaOutPos + slice(16 * kNaP);
+constexpr size_t kNaOutTp = kNaOutDir + slice(16 * kNaP);
+constexpr size_t kNaOutIdx = kNaOutTp + slice(8 * kNaP);
+constexpr size_t kNaCells = kNaOutIdx + slice(4 * kNaP);
+constexpr size_t kNaCounts = kNaCells + slice(8 * (size_t)kNoAllocMaxBuckets);
+constexpr size_t kNaBytes = kNaCounts + slice(4 * ((size_t)kNoAllocMaxQueries + 1));
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+size_t noalloc_scratch_bytes() { return kNaBytes; }
+
+// Device address of `scratch` on the current device, looked up once per
+// device (the host-side name is only a stand-in for it).
+static unsigned char* scratch_address(const char** err) {
+    static std::mutex mu;
+    static unsigned char* addr[64] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        *err = "no HIP device";
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            *err = "hipGetSymbolAddress of the scratch array failed";
+            return nullptr;
+        }
+        addr[dev] = static_cast<unsigned char*>(p);
+    }
+    return addr[dev];
+}
+
+__global__ void k_na_photons(const float* p, uint32_t n, float4* pos, uint32_t* uid) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    pos[i] = make_float4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0f);
+    uid[i] = i;
+}
+
+// offsets[i] = counts[0] + ... + counts[i - 1] for i <= n (one workgroup:
+// each thread sums a contiguous run, a scan over the runs in LDS, then each
+// thread writes its run's offsets).
+constexpr int kNaScanThreads = 1024;
+__global__ __launch_bounds__(kNaScanThreads) void k_na_offsets(const uint32_t* counts, uint32_t n,
+                                                               unsigned long long* offsets) {
+    __shared__ unsigned long long run[kNaScanThreads];
+    const uint32_t per = (n + kNaScanThreads - 1) / kNaScanThreads;
+    const uint32_t b = threadIdx.x * per, e = min(n, b + per);
+    unsigned long long sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += counts[i];
+    run[threadIdx.x] = sum;
+    __syncthreads();
+    for (uint32_t off = 1; off < kNaScanThreads; off <<= 1) {
+        unsigned long long t = threadIdx.x >= off ? run[threadIdx.x - off] : 0ull;
+        __syncthreads();
+        run[threadIdx.x] += t;
+        __syncthreads();
+    }
+    unsigned long long at = run[threadIdx.x] - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        offsets[i] = at;
+        at += counts[i];
+    }
+    if (threadIdx.x == kNaScanThreads - 1) offsets[n] = run[kNaScanThreads - 1];
+}
+
+long long gather_sets_noalloc(const NoAllocGather& a, hipStream_t s, const char** err) {
+    if (a.count > kNoAllocMaxPhotons || a.nq > kNoAllocMaxQueries || a.pm.mask + 1 > kNoAllocMaxBuckets) {
+        *err = "rc_gather_sets_noalloc: size beyond the scratch array";
+        return -1;
+    }
+    unsigned char* base = scratch_address(err);
+    if (!base) return -1;
+    float4* in_pos = reinterpret_cast<float4*>(base + kNaInPos);
+    uint32_t* in_uid = reinterpret_cast<uint32_t*>(base + kNaInUid);
+    MapBuildArgs m{};
+    m.in_pos = in_pos;
+    m.in_dir = in_pos;
+    m.in_tp = reinterpret_cast<const float2*>(in_pos);
+    m.in_uid = in_uid;
+    m.count = a.count;
+    for (int k = 0; k < 3; ++k) m.origin[k] = a.pm.origin[k];
+    m.inv_cell = a.pm.inv_cell;
+    m.mask = a.pm.mask;
+    m.bucket_bits = a.bucket_bits;
+    m.uid_bits = a.uid_bits;
+    m.keys_a = reinterpret_cast<unsigned long long*>(base + kNaKeysA);
+    m.keys_b = reinterpret_cast<unsigned long long*>(base + kNaKeysB);
+    m.vals_a = reinterpret_cast<uint32_t*>(base + kNaValsA);
+    m.vals_b = reinterpret_cast<uint32_t*>(base + kNaValsB);
+    m.hist = reinterpret_cast<uint32_t*>(base + kNaHistAt);
+    m.hist_sums = reinterpret_cast<uint32_t*>(base + kNaSumsAt);
+    m.out_pos = reinterpret_cast<float4*>(base + kNaOutPos);
+    m.out_dir = reinterpret_cast<float4*>(base + kNaOutDir);
+    m.out_tp = reinterpret_cast<float2*>(base + kNaOutTp);
+    m.out_index = reinterpret_cast<uint32_t*>(base + kNaOutIdx);
+    m.cells = reinterpret_cast<uint2*>(base + kNaCells);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(base + kNaCounts);
+    if (a.count) hipLaunchKernelGGL(k_na_photons, dim3((a.count + 255) / 256), dim3(256), 0, s, a.positions, a.count,
+                                    in_pos, in_uid);
+    build_photon_map(m, s);  // clears every bucket range with a fill kernel first
+    PhotonMapView pm = a.pm;
+    pm.pos = m.out_pos;
+    pm.dir_tr = m.out_dir;
+    pm.tp_gb = m.out_tp;
+    pm.cells = m.cells;
+    launch_gather_count(pm, a.queries, (int)a.nq, m.out_index, counts, s);
+    hipLaunchKernelGGL(k_na_offsets, dim3(1), dim3(kNaScanThreads), 0, s, counts, a.nq, a.offsets);
+    unsigned long long total = 0;
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(&total, a.offsets + a.nq, sizeof total, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+        *err = "rc_gather_sets_noalloc: HIP error";
+        return -1;
+    }
+    if ((long long)total <= a.cap && total > 0) {
+        launch_gather_fill(pm, a.queries, (int)a.nq, m.out_index, a.offsets, a.indices, s);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess) {
+            *err = "rc_gather_sets_noalloc: HIP error";
+            return -1;
+        }
+    }
+    return (long long)total;
 }
 
 }  // namespace rc

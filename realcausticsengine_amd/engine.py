@@ -26,6 +26,7 @@ EXPORTS = [
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
     "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug", "rc_bvh_shape",
+    "rc_gather_sets_noalloc",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
@@ -117,6 +118,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_batch_release.restype = None
     lib.rc_render_debug.argtypes = [ctypes.c_int]
     lib.rc_render_debug.restype = ctypes.c_int
+    lib.rc_gather_sets_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
+                                           ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
+    lib.rc_gather_sets_noalloc.restype = ctypes.c_longlong
     lib.rc_bvh_shape.argtypes = [P(ctypes.c_float), ctypes.c_longlong, ctypes.c_int, P(ctypes.c_int)]
     lib.rc_bvh_shape.restype = ctypes.c_int
     for fn in (lib.rc_autotune, lib.rc_batch_autotune):
@@ -221,6 +225,12 @@ class Engine:
             if n <= cap:
                 return off, idx[:n].copy()
             cap = int(n)
+
+    def gather_sets_noalloc(self, d_positions: int, count: int, d_queries: int, nq: int, r2: float, d_offsets: int,
+                            d_indices: int, cap: int) -> int:
+        """rc_gather_sets_noalloc on device addresses (ints); returns the total set size."""
+        return self._check(self.lib.rc_gather_sets_noalloc(d_positions, count, d_queries, nq, float(r2), d_offsets,
+                                                           d_indices, cap), "rc_gather_sets_noalloc")
 
     def camera_pass(self, iteration: int, photons: np.ndarray) -> np.ndarray:
         W, H = self.args.camera.width, self.args.camera.height
