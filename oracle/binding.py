@@ -152,7 +152,7 @@ class OracleScene:
     def light_sample(self, u5) -> np.ndarray:
         self.prepare()
         u = np.ascontiguousarray(u5, dtype=np.float32)
-        out = np.zeros(9, dtype=np.float64)
+        out = np.zeros(10, dtype=np.float64)
         self.lib.orc_light_sample(self.h, _ptr(u, _fp), _ptr(out, _dp))
         return out
 

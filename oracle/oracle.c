@@ -1326,7 +1326,7 @@ static real guide_direction(const Scene *s, int light, real ax, real ay, real *e
 }
 
 /* Lights_list::sample_lights (Light_list.cpp:10-30) and the per-light sample_light. */
-static void sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
+static int sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
     real rng = get1d(smp);
     int idx = find_interval(s->cdf, (int)s->nlights + 1, rng);
     const Light *L = &s->lights[idx];
@@ -1378,6 +1378,7 @@ static void sample_light(Scene *s, Sampler *smp, V *origin, V *dir, V *tp) {
     }
     *tp = vdiv_each(t, pick);
     if (q != K(1.0f)) *tp = vdiv_each(*tp, q);
+    return idx;
 }
 
 int orc_set_guide(void *sp, const float *cdf, int slots, int cells, const int *slot_of_light, double alpha) {
@@ -1402,8 +1403,8 @@ void orc_light_sample(void *sp, const float *u5, double *out) {
     smp.n = 5; smp.index = 0;
     for (int i = 0; i < 5; ++i) smp.value[i] = u5[i];
     V o, d, t;
-    sample_light(s, &smp, &o, &d, &t);
-    double r[9] = {o.x, o.y, o.z, d.x, d.y, d.z, t.x, t.y, t.z};
+    int idx = sample_light(s, &smp, &o, &d, &t);
+    double r[10] = {o.x, o.y, o.z, d.x, d.y, d.z, t.x, t.y, t.z, (double)idx};
     memcpy(out, r, sizeof r);
 }
 

@@ -112,7 +112,8 @@ double orc_math(int which, double x);
 void orc_bsdf(const float *mat, int mode, const float *shade_n, const float *geo_n,
               const float *dir, const float *u3, const float *wi, int op, double *out);
 
-/* One light emission sample: u5 = 5 primary samples; out = origin.xyz, dir.xyz, throughput.rgb. */
+/* One light emission sample: u5 = 5 primary samples; out = origin.xyz, dir.xyz, throughput.rgb,
+ * index of the light picked. */
 void orc_light_sample(void *scene, const float *u5, double *out);
 /* Learned emission proposal of the engine: per-cell CDFs (slots * cells^2
  * rows of 1025 floats), the slot of each of the first 8 lights (-1: plain)
