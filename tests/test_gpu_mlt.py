@@ -82,3 +82,28 @@ def test_mlt_excludes_the_learned_proposal():
     with pytest.raises(Exception, match="exclusive"):
         eng.render_iteration()
     eng.release()
+
+
+def test_every_path_visible_mlt_equals_plain():
+    """Where every uniform path is visible the MLT pass never mutates: a small
+    light inside a closed sphere of nearly smooth GGX metal (the oracle's
+    special-case scene, test_oracle_mlt.py) gives the plain photon list bit
+    for bit, with zero mutations and all paths visible."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from test_oracle_mlt import _enclosed_light_scene
+
+    args = _enclosed_light_scene()
+    plain = Engine(args, photons=8192, seed=2)
+    ph_p, uid_p = plain.trace_photons(0)
+    plain.release()
+    assert len(np.unique(uid_p >> 3)) == 8192
+    eng = Engine(args, photons=8192, seed=2, mlt=1)
+    ph_m, uid_m = eng.trace_photons(0)
+    counts = [eng.get(k) for k in ("mlt_visible", "mlt_mutations", "mlt_accepted")]
+    eng.release()
+    assert counts == [8192, 0, 0], counts
+    op, om = np.argsort(uid_p), np.argsort(uid_m)
+    assert np.array_equal(uid_p[op], uid_m[om])
+    assert np.array_equal(ph_p[op].view(np.uint32), ph_m[om].view(np.uint32))
