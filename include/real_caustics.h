@@ -225,6 +225,15 @@ long long rc_trace_photons(int iteration, float *photons, unsigned int *uid, lon
  * CSR output; returns the total (indices beyond cap are not written). */
 long long rc_gather_sets(const float *positions, long long count, const float *queries, long long nq,
                          float r2, long long *offsets, int *indices, long long cap);
+/* Radius sets of the production merge: the camera pass of `iteration`
+ * against a given photon list (10 floats each, as rc_trace_photons) with the
+ * merge kernel the options select, in its counting form.  Per merge query in
+ * trace order (up to cap): position (3 floats), contribution slot (camera
+ * path * 8 + vertex), the number of photons the merge accepted and the sum,
+ * modulo 2^64, of the SplitMix64 finaliser of their input indices; r2_out =
+ * the squared merge radius.  Returns the query count. */
+long long rc_merge_sets(int iteration, const float *photons, long long count, float *query_pos, long long *query_slot,
+                        unsigned long long *set_count, unsigned long long *set_hash, long long cap, float *r2_out);
 /* rc_gather_sets without any device allocation.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture set-up, tight serving loops, harnesses that forbid

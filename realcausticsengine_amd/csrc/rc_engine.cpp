@@ -98,6 +98,7 @@ struct Engine {
     int root4 = 0, n_nodes4 = 0;
     int bvh_width = 4;  // 4: traverse the 4-wide BVH, 2: the binary one
     int bvh_build = 0;  // 0: binned SAH on the host, 1: linear BVH on the device (binary nodes)
+    unsigned long long* merge_sets = nullptr;  // rc_merge_sets: per-slot set fingerprints (counting launches)
     int bvh_depth = 0;   // internal ancestors of the deepest leaf of the binary tree
     int bvh_depth4 = 0;  // depth of the deepest internal 4-wide node (root 0)
     double bvh_build_ms = 0.0;
@@ -1214,6 +1215,8 @@ bool run_camera_pass(Engine& e, int iteration, const MapInfo& mi, float norm, fl
     a.gather_tile = e.gather_tile;
     a.gather_block = e.gather_block;
     a.camera_qmc = e.camera_qmc;
+    a.merge_sets = e.count_events ? e.merge_sets : nullptr;
+    a.map_index = e.s_index.p;
     for (int attempt = 0; attempt < 2; ++attempt) {
         a.q = QueryBuffers{e.q_pos.p, e.q_dir.p, e.q_sn.p, e.q_gn.p, e.q_tpn.p, e.q_capacity};
         launch_fill_u32(e.counters.p, 0u, kPathCounters, e.stream);
@@ -2970,6 +2973,63 @@ __attribute__((visibility("default"))) long long rc_gather_sets_noalloc(const fl
     long long total = gather_sets_noalloc(a, e.stream, &err);
     if (total < 0) set_error(err ? err : "rc_gather_sets_noalloc failed");
     return total;
+}
+
+__attribute__((visibility("default"))) long long rc_merge_sets(int iteration, const float* photons, long long count,
+                                                               float* query_pos, long long* query_slot,
+                                                               unsigned long long* set_count,
+                                                               unsigned long long* set_hash, long long cap,
+                                                               float* r2_out) {
+    if (!g || !prepare(*g)) return -1;
+    Engine& e = *g;
+    if (!load_photons(photons, count)) return -1;
+    float radius, r2, norm;
+    iteration_params(e, iteration, radius, r2, norm);
+    long long saved = e.photons;
+    e.photons = std::max<long long>(light_paths(e), (count + 7) / 8 + 1);
+    MapInfo mi;
+    bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
+    e.photons = saved;
+    if (!ok) return -1;
+    const size_t slots = (size_t)e.W * e.H * 8;
+    DevBuf<unsigned long long> sets;
+    DevBuf<float> img;
+    if (!sets.alloc(2 * slots) || !img.alloc((size_t)e.W * e.H * 3)) return -1;
+    launch_fill_u32(reinterpret_cast<uint32_t*>(sets.p), 0u, 4 * slots, e.stream);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(img.p), 0u, (size_t)e.W * e.H * 3, e.stream);
+    const int saved_count = e.count_events;
+    e.count_events = 1;
+    e.merge_sets = sets.p;
+    ok = run_camera_pass(e, iteration, mi, norm, img.p) && hipStreamSynchronize(e.stream) == hipSuccess;
+    e.merge_sets = nullptr;
+    e.count_events = saved_count;
+    const long long nq = ok ? (long long)e.last_queries : -1;
+    if (ok && nq > 0) {
+        const long long n = std::min(nq, cap);
+        std::vector<float4> qp((size_t)nq), qs((size_t)nq);
+        std::vector<unsigned long long> fs(2 * slots);
+        ok = hipMemcpy(qp.data(), e.q_pos.p, (size_t)nq * 16, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(qs.data(), e.q_sn.p, (size_t)nq * 16, hipMemcpyDeviceToHost) == hipSuccess &&
+             hipMemcpy(fs.data(), sets.p, 2 * slots * 8, hipMemcpyDeviceToHost) == hipSuccess;
+        for (long long i = 0; ok && i < n; ++i) {
+            uint32_t slot;
+            memcpy(&slot, &qs[(size_t)i].w, 4);
+            query_pos[3 * i] = qp[(size_t)i].x;
+            query_pos[3 * i + 1] = qp[(size_t)i].y;
+            query_pos[3 * i + 2] = qp[(size_t)i].z;
+            query_slot[i] = slot;
+            set_count[i] = fs[2 * (size_t)slot];
+            set_hash[i] = fs[2 * (size_t)slot + 1];
+        }
+    }
+    sets.release();
+    img.release();
+    if (r2_out) *r2_out = mi.pm.r2;
+    if (!ok) {
+        set_error("rc_merge_sets failed");
+        return -1;
+    }
+    return nq;
 }
 
 __attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration, const float* photons, long long count,

@@ -214,7 +214,21 @@ struct CameraPassArgs {
                               // order; 0: XCD x takes the x-th contiguous eighth
     int gather_block;         // grouped merge: 2 or 4 = photons taken 64 x this many at a time and the query
                               // loop rolled (sums in LDS); 0 = the query loop unrolled (sums in registers)
+    // counting launches (counters non-null) with merge_sets non-null: per
+    // contribution slot, {photons the merge accepted, sum of mix64 of their
+    // input indices}; map_index = input index of each sorted photon
+    unsigned long long* merge_sets;
+    const uint32_t* map_index;
 };
+
+// SplitMix64 finaliser (Steele, Lea and Flood 2014): the per-photon term of
+// the merge-set fingerprints.
+__host__ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
 
 // ---- launchers (rc_kernels.hip) ----
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s);

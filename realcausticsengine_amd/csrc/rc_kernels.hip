@@ -714,6 +714,13 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
 // of the image) and its L2 holds that band's photons.
 constexpr int kXcds = 8;
 
+// Fingerprint of a merge's radius set (counting launches): photon count and
+// the sum of mix64 of the input indices, per contribution slot.
+__device__ __forceinline__ void record_merge(const CameraPassArgs& a, uint32_t slot, uint32_t sorted_i) {
+    atomicAdd(a.merge_sets + 2 * (size_t)slot, 1ull);
+    atomicAdd(a.merge_sets + 2 * (size_t)slot + 1, mix64((unsigned long long)a.map_index[sorted_i]));
+}
+
 template <bool kCount, bool kFast, bool kBatch>
 __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
     __shared__ uint32_t queue_lds[kGatherWaves][128];
@@ -819,6 +826,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather(CameraPassArgs a) {
             if (acc) queue[(qhead + qcount + lanes_below(mk)) & 127u] = i;
             qcount += n;
             if (kCount && lane == 0) cnt[C_ACCEPTED] += n;
+            if (kCount && a.merge_sets && acc) record_merge(a, __float_as_uint(qs.w), i);
             __builtin_amdgcn_wave_barrier();
             if (qcount >= 64) {
                 eval_one(queue[(qhead + lane) & 127u]);
@@ -1092,11 +1100,13 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     float4 P[kB], DT[kB];
                     float2 GB[kB];
                     bool V[kB];
+                    uint32_t I[kB];
 #pragma unroll
                     for (int bb = 0; bb < kB; ++bb) {
                         const uint32_t sb = sb0 + 64u * bb;
                         const uint32_t sp = sb + lane;
                         V[bb] = false;
+                        I[bb] = 0u;
                         P[bb] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
                         DT[bb] = P[bb];
                         GB[bb] = make_float2(0.0f, 0.0f);
@@ -1113,6 +1123,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                             }
                             V[bb] = sp < total;
                             const uint32_t i = cur_first + (sp - cur_base);
+                            I[bb] = i;
                             if (V[bb]) {
                                 P[bb] = pm.pos[i];
                                 DT[bb] = pm.dir_tr[i];
@@ -1122,6 +1133,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                     }
                     for (uint32_t jj = 0; jj < nrun; ++jj) {
                         const uint32_t qj = s0 + jj;
+                        const uint32_t qslot = kCount ? (uint32_t)__builtin_amdgcn_readlane((int)slot, (int)qj) : 0u;
                         F3 c = f3(acc_lds[wv][3 * jj][lane], acc_lds[wv][3 * jj + 1][lane], acc_lds[wv][3 * jj + 2][lane]);
                         const DiffuseQuery d = dq_lds[wv][qj];
                         const F3 dq = f3(d.qk.x, d.qk.y, d.qk.z);
@@ -1133,6 +1145,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                                 if (kCount) {
                                     const uint64_t am = __ballot(acc);
                                     if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                                    if (a.merge_sets && acc) record_merge(a, qslot, I[bb]);
                                 }
                                 // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
                                 // the wrong side adds nothing (its term is 0 there)
@@ -1153,6 +1166,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                                 if (kCount) {
                                     const uint64_t am = __ballot(acc);
                                     if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                                    if (a.merge_sets && acc) record_merge(a, qslot, I[bb]);
                                 }
                                 if (acc) {
                                     float fwd, rev;
@@ -1220,6 +1234,8 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                             if (kCount) {
                                 const uint64_t am = __ballot(acc);
                                 if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
+                                const uint32_t qslot = (uint32_t)__builtin_amdgcn_readlane((int)slot, (int)(s0 + jj));
+                                if (a.merge_sets && acc) record_merge(a, qslot, i);
                             }
                             if (kFast && ((fastmask >> (s0 + jj)) & 1u)) {
                                 // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on

@@ -26,7 +26,7 @@ EXPORTS = [
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
     "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug", "rc_bvh_shape",
-    "rc_gather_sets_noalloc",
+    "rc_gather_sets_noalloc", "rc_merge_sets",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
@@ -121,6 +121,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.rc_gather_sets_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
                                            ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong]
     lib.rc_gather_sets_noalloc.restype = ctypes.c_longlong
+    lib.rc_merge_sets.argtypes = [ctypes.c_int, P(ctypes.c_float), ctypes.c_longlong, P(ctypes.c_float),
+                                  P(ctypes.c_longlong), P(ctypes.c_ulonglong), P(ctypes.c_ulonglong),
+                                  ctypes.c_longlong, P(ctypes.c_float)]
+    lib.rc_merge_sets.restype = ctypes.c_longlong
     lib.rc_bvh_shape.argtypes = [P(ctypes.c_float), ctypes.c_longlong, ctypes.c_int, P(ctypes.c_int)]
     lib.rc_bvh_shape.restype = ctypes.c_int
     for fn in (lib.rc_autotune, lib.rc_batch_autotune):
@@ -225,6 +229,24 @@ class Engine:
             if n <= cap:
                 return off, idx[:n].copy()
             cap = int(n)
+
+    def merge_sets(self, iteration: int, photons: np.ndarray, cap: int = 1 << 20):
+        """Set fingerprints of the production merge (rc_merge_sets): per query
+        its position, slot, accepted count and SplitMix64 hash sum, and r2."""
+        ph = np.ascontiguousarray(photons, dtype=np.float32)
+        qp = np.zeros((cap, 3), dtype=np.float32)
+        sl = np.zeros(cap, dtype=np.int64)
+        cnt = np.zeros(cap, dtype=np.uint64)
+        hs = np.zeros(cap, dtype=np.uint64)
+        r2 = np.zeros(1, dtype=np.float32)
+        P = ctypes.POINTER
+        n = self._check(self.lib.rc_merge_sets(iteration, _fp(ph), len(ph), _fp(qp), sl.ctypes.data_as(P(ctypes.c_longlong)),
+                                               cnt.ctypes.data_as(P(ctypes.c_ulonglong)),
+                                               hs.ctypes.data_as(P(ctypes.c_ulonglong)), cap, _fp(r2)),
+                        "rc_merge_sets")
+        if n > cap:
+            return self.merge_sets(iteration, photons, int(n))
+        return qp[:n], sl[:n], cnt[:n], hs[:n], float(r2[0])
 
     def gather_sets_noalloc(self, d_positions: int, count: int, d_queries: int, nq: int, r2: float, d_offsets: int,
                             d_indices: int, cap: int) -> int:
