@@ -130,8 +130,14 @@ int rc_render_debug(int mode);
 /* Copies the accumulated image (width*height*3 floats, buffer row order of
  * src/Image_Buffer.h).  scaled != 0 divides by the iterations rendered. */
 int rc_get_image(float *rgb, int scaled);
-/* Writes the current image as an uncompressed fp32 RGB OpenEXR file. */
+/* Writes the current image as the fp32 RGB OpenEXR file main() writes:
+ * tinyexr's layout for the reference's SaveEXR call (src/main.cpp:192-207,
+ * lib/tinyexr.h SaveEXR) -- single-part scanline, FLOAT channels B, G, R,
+ * ZIP compression in 16-line blocks (none when both sides are below 16). */
 int rc_write_exr(const char *path);
+/* The same writer for a host image (width * height * 3 floats, buffer row
+ * order); needs no scene and no GPU. */
+int rc_write_exr_image(const char *path, const float *rgb, int width, int height);
 /* Frees the scene and device memory (the end of main, src/main.cpp:220-221). */
 void rc_release(void);
 /* Last error message, "" when none. */

@@ -50,7 +50,7 @@ def build(verbose: bool = False) -> str:
     if failed:
         raise RuntimeError("hipcc failed on " + ", ".join(failed))
     tmp = LIB + ".tmp"
-    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-lz", "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

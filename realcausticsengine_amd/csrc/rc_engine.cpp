@@ -18,6 +18,8 @@
 #include <thread>
 #include <vector>
 
+#include <zlib.h>
+
 #include "../../include/real_caustics.h"
 #include "rc_engine.h"
 #include "rc_guide_model.h"
@@ -1715,8 +1717,29 @@ bool autotune(Engine& e, int repeats, double* report) {
 }
 
 // ---------------------------------------------------------------------
-// OpenEXR writer: single-part scanline, uncompressed, FLOAT channels B G R.
+// OpenEXR writer with the layout tinyexr's SaveEXR gives the reference's
+// image (src/main.cpp:192-207): single-part scanline, FLOAT channels B G R,
+// ZIP compression in blocks of 16 scanlines (no compression when both sides
+// are below 16 pixels).  A ZIP block is the block's lines (each line: its B,
+// G and R pixels) with the bytes split into even and odd halves and then
+// delta-coded (OpenEXR's ZIP predictor), deflated into a zlib stream; a block
+// whose stream would not be smaller is stored raw (its size then equals the
+// raw size, which tells readers it is not compressed).
 // ---------------------------------------------------------------------
+static void zip_block(const unsigned char* src, size_t n, std::vector<unsigned char>& out) {
+    std::vector<unsigned char> t(n);
+    size_t a = 0, b = (n + 1) / 2;
+    for (size_t i = 0; i < n; ++i) t[(i & 1) ? b++ : a++] = src[i];
+    for (size_t i = n; i-- > 1;) t[i] = (unsigned char)(t[i] - t[i - 1] + 128);
+    uLongf cap = compressBound((uLong)n);
+    out.resize(cap);
+    if (compress(out.data(), &cap, t.data(), (uLong)n) != Z_OK || cap >= n) {
+        out.assign(src, src + n);
+        return;
+    }
+    out.resize(cap);
+}
+
 bool write_exr(const char* path, const float* rgb, int W, int H) {
     std::vector<unsigned char> out;
     auto put = [&](const void* p, size_t n) {
@@ -1732,6 +1755,8 @@ bool write_exr(const char* path, const float* rgb, int W, int H) {
         put_str(type);
         put_u32(size);
     };
+    const bool zip = !(W < 16 && H < 16);
+    const int lines = zip ? 16 : 1;  // scanlines per block
     put_u32(20000630u);  // magic
     put_u32(2u);         // version 2, scanline
     attr("channels", "chlist", 3 * (2 + 16) + 1);
@@ -1745,7 +1770,7 @@ bool write_exr(const char* path, const float* rgb, int W, int H) {
     }
     out.push_back(0);
     attr("compression", "compression", 1);
-    out.push_back(0);  // NO_COMPRESSION
+    out.push_back(zip ? 3 : 0);  // ZIP_COMPRESSION or NO_COMPRESSION
     attr("dataWindow", "box2i", 16);
     put_i32(0);
     put_i32(0);
@@ -1766,16 +1791,27 @@ bool write_exr(const char* path, const float* rgb, int W, int H) {
     attr("screenWindowWidth", "float", 4);
     put_f32(1.0f);
     out.push_back(0);  // end of header
+    const int blocks = (H + lines - 1) / lines;
     size_t table = out.size();
-    out.resize(out.size() + 8 * (size_t)H);
-    uint32_t line_bytes = (uint32_t)(3 * 4 * W);
-    for (int y = 0; y < H; ++y) {
+    out.resize(out.size() + 8 * (size_t)blocks);
+    std::vector<unsigned char> raw, packed;
+    for (int k = 0; k < blocks; ++k) {
+        const int y0 = k * lines, y1 = std::min(H, y0 + lines);
+        raw.clear();
+        for (int y = y0; y < y1; ++y)
+            for (int ch : {2, 1, 0})
+                for (int x = 0; x < W; ++x) {
+                    const float v = rgb[3 * ((size_t)y * W + x) + ch];
+                    const unsigned char* c = (const unsigned char*)&v;
+                    raw.insert(raw.end(), c, c + 4);
+                }
+        if (zip) zip_block(raw.data(), raw.size(), packed);
+        const std::vector<unsigned char>& data = zip ? packed : raw;
         uint64_t off = out.size();
-        memcpy(&out[table + 8 * (size_t)y], &off, 8);
-        put_i32(y);
-        put_u32(line_bytes);
-        for (int ch : {2, 1, 0})
-            for (int x = 0; x < W; ++x) put_f32(rgb[3 * ((size_t)y * W + x) + ch]);
+        memcpy(&out[table + 8 * (size_t)k], &off, 8);
+        put_i32(y0);
+        put_u32((uint32_t)data.size());
+        put(data.data(), data.size());
     }
     FILE* f = fopen(path, "wb");
     if (!f) {
@@ -2733,6 +2769,15 @@ __attribute__((visibility("default"))) int rc_get_image(float* rgb, int scaled) 
         return -1;
     }
     return get_image(*g, rgb, scaled != 0) ? 0 : -1;
+}
+
+__attribute__((visibility("default"))) int rc_write_exr_image(const char* path, const float* rgb, int width,
+                                                              int height) {
+    if (!path || !rgb || width <= 0 || height <= 0) {
+        set_error("rc_write_exr_image: bad arguments");
+        return -1;
+    }
+    return write_exr(path, rgb, width, height) ? 0 : -1;
 }
 
 __attribute__((visibility("default"))) int rc_write_exr(const char* path) {

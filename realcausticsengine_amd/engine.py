@@ -26,7 +26,7 @@ EXPORTS = [
     "rc_guide_records", "rc_batch_add", "rc_batch_size", "rc_batch_prepare", "rc_batch_render_iteration",
     "rc_batch_render_scene", "rc_batch_get_image", "rc_batch_stats", "rc_batch_release", "rc_autotune",
     "rc_batch_autotune", "rc_batch_set_option", "rc_render_debug", "rc_bvh_shape",
-    "rc_gather_sets_noalloc", "rc_merge_sets",
+    "rc_gather_sets_noalloc", "rc_merge_sets", "rc_write_exr_image",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
              "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
@@ -125,6 +125,8 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
                                   P(ctypes.c_longlong), P(ctypes.c_ulonglong), P(ctypes.c_ulonglong),
                                   ctypes.c_longlong, P(ctypes.c_float)]
     lib.rc_merge_sets.restype = ctypes.c_longlong
+    lib.rc_write_exr_image.argtypes = [ctypes.c_char_p, P(ctypes.c_float), ctypes.c_int, ctypes.c_int]
+    lib.rc_write_exr_image.restype = ctypes.c_int
     lib.rc_bvh_shape.argtypes = [P(ctypes.c_float), ctypes.c_longlong, ctypes.c_int, P(ctypes.c_int)]
     lib.rc_bvh_shape.restype = ctypes.c_int
     for fn in (lib.rc_autotune, lib.rc_batch_autotune):
@@ -374,6 +376,15 @@ class Batch:
 
     def release(self):
         self.lib.rc_batch_release()
+
+
+def write_exr_image(path: str, rgb: np.ndarray) -> None:
+    """Writes an (H, W, 3) float image with the library's EXR writer (rc_write_exr_image; no GPU needed)."""
+    img = np.ascontiguousarray(rgb, dtype=np.float32)
+    H, W = img.shape[:2]
+    lib = load()
+    if lib.rc_write_exr_image(path.encode(), _fp(img), W, H) != 0:
+        raise EngineError(f"rc_write_exr_image: {lib.rc_last_error().decode()}")
 
 
 def bvh_shape(tris: np.ndarray, width: int = 4) -> dict:

@@ -31,14 +31,34 @@ def test_debug_modes_match_oracle(oracle, mode):
     assert same.mean() >= 0.998
 
 
-def test_main_writes_debug_exr(tmp_path):
+@pytest.mark.parametrize("mode", [2, 3, 4])
+def test_main_writes_debug_exr(tmp_path, mode):
+    """main() with debug_test 2-4 writes the debug image (src/main.cpp:172-207):
+    the decoded EXR equals rc_render_debug's image bit for bit."""
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from exr_reader import read_exr, read_rgb
+
     args = scenes.c0_scene(32, 24, subdiv=2, samples=1)
-    args.debug_test = 3
-    out = str(tmp_path / "facing.exr")
+    args.debug_stuff.specular = 1.33
+    eng = Engine(args, write_exr=0)
+    want = eng.render_debug(mode)
+    eng.release()
+    args.debug_test = mode
+    out = str(tmp_path / f"debug{mode}.exr")
     eng = Engine(args, output_path=out)
     eng.main()
-    data = open(out, "rb").read()
-    assert data[:4] == bytes([0x76, 0x2F, 0x31, 0x01])  # OpenEXR magic
-    # 32 x 24 x 3 floats of pixel data follow the header and offset table
-    assert len(data) > 32 * 24 * 3 * 4
+    e = read_exr(out)
+    assert e["compression"] == 3 and e["channel_order"] == ["B", "G", "R"]
+    got = read_rgb(out)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
     os.remove(out)
+
+
+def test_main_debug_mode_5_writes_no_image(tmp_path):
+    args = scenes.c0_scene(16, 16, subdiv=1, samples=1)
+    args.debug_test = 5
+    out = str(tmp_path / "none.exr")
+    eng = Engine(args, output_path=out)
+    eng.main()
+    assert not os.path.exists(out)
