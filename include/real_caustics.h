@@ -66,7 +66,11 @@ int main(void);
 
 /* Options.  Numeric keys: "seed", "photons" (light paths per iteration,
  * 0 = width*height as the reference), "write_exr", "compat_rows",
- * "compat_point_light", "compat_light_pdf", "max_path_length",
+ * "compat_point_light", "compat_light_pdf", "compat_glass" (1, the default:
+ * glass is a delta interface whatever its roughness, as src/materials.h:104;
+ * 0: glass of roughness >= 0.01 is a GGX microfacet dielectric of
+ * a2 = roughness^2 -- reflection or refraction through a sampled microfacet,
+ * non-delta, so photons are stored and merged on it), "max_path_length",
  * "photon_capacity", "cell_scale", "persistent_blocks", "count_events",
  * "exact_merge" (1 = correctly rounded division and sqrt in the merge),
  * "next_iteration" (index of the next iteration rc_render_iteration renders).

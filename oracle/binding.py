@@ -211,7 +211,7 @@ def own_math(which: int, x: float) -> float:
     return load("f32").orc_math(which, float(x))
 
 
-MAT_CODES = {"P": 80, "G": 71, "M": 77}
+MAT_CODES = {"P": 80, "G": 71, "M": 77, "R": 82}  # "R": glass as the rough dielectric (compat_glass = 0)
 
 
 def bsdf(mat, mode: int, shade_n, geo_n, direction, u3=(0.5, 0.5, 0.5), wi=(0.0, 0.0, 1.0), op: int = 0,

@@ -334,9 +334,13 @@ static V sample_cos_hemisphere(real ex, real ey) {
 /* Materials and BSDFs (materials.h:59-146, BXDF.cpp, BXDF.h:461-505)  */
 /* ------------------------------------------------------------------ */
 enum { EV_REFLECTION = 1, EV_TRANSMISSION = 2, EV_DIFFUSE = 4, EV_GLOSSY = 8, EV_SPECULAR = 16 };
-enum { BK_SPEC_REFL, BK_FRESNEL_SPEC, BK_FMG, BK_FSMG, BK_MRG };
+/* BK_RGLASS: glass of roughness >= 0.01 as a GGX microfacet dielectric
+ * (option compat_glass = 0; the reference's glass is always delta,
+ * materials.h:104).  Walter, Marschner, Li, Torrance 2007, with the
+ * reference's GGX sampling, D and height-correlated G2 (BXDF.cpp:286-311). */
+enum { BK_SPEC_REFL, BK_FRESNEL_SPEC, BK_FMG, BK_FSMG, BK_MRG, BK_RGLASS };
 
-typedef struct { char type; V color; real ior, roughness, specular; } Material;
+typedef struct { char type; V color; real ior, roughness, specular; int rough_glass; } Material;
 
 typedef struct {
     int kind, radiance, valid;
@@ -355,6 +359,7 @@ static Material material_from_python(const unsigned char *pm) {
     memcpy(&spec, pm + 12, 4);
     memcpy(col, pm + 16, 12);
     m.type = (char)pm[0];
+    m.rough_glass = 0;
     m.color = v3(col[0], col[1], col[2]);
     m.ior = ior;
     if (m.type == 'P') { m.roughness = (real)rough * (real)rough; m.specular = (real)spec * K(0.08f); }
@@ -375,7 +380,9 @@ static void bsdf_init(Bsdf *b, const Material *m, V shade_n, V dir, int radiance
         if (m->roughness < K(0.01f)) { b->kind = BK_FSMG; b->cont = 1; }
         else { b->kind = BK_FMG; b->a2 = m->roughness * m->roughness; b->cont = 1; }
     } else if (m->type == 'G') {
-        b->kind = BK_FRESNEL_SPEC; b->R = m->color; b->ior = m->ior; b->cont = cont_of(m->color);
+        b->kind = m->rough_glass ? BK_RGLASS : BK_FRESNEL_SPEC; b->R = m->color; b->ior = m->ior;
+        b->cont = cont_of(m->color);
+        if (m->rough_glass) b->a2 = m->roughness * m->roughness;
     } else {
         b->R = m->color;
         if (m->roughness < K(0.01f)) { b->kind = BK_SPEC_REFL; }
@@ -488,6 +495,51 @@ static V bsdf_sample(const Bsdf *b, V gn, real u0, real u1, real u2, Sample *o) 
         real g2 = ggx_G2((real)fabs(wo.z), (real)fabs(wil.z), b->a2);
         return vdivs(vscale(vscale(b->R, g2), vabsdot(wo, h)), (real)fabs(wo.z) * chn);
     }
+    case BK_RGLASS: {
+        /* a GGX microfacet normal on wo's side; the Fresnel term of wo at the
+         * microfacet picks reflection or refraction through it; weight
+         * G2 (wo.m) / (|wo.n| |m.n|), times eta^2 for refraction in radiance
+         * mode (Fresnel_Specular's weights as the roughness goes to zero) */
+        V m = ggx_half(b->a2, u1, u2);
+        if (wo.z < 0) m.z = -m.z;
+        real wom = vdot(wo, m);
+        if (!(wom > K(0))) return zero;
+        real eta = b->ior;
+        if (wo.z > 0) eta = K(1) / b->ior;
+        real f = fresnel_dielectric_eta(wom, K(1) / eta);
+        real mz = (real)fabs(m.z);
+        real d = ggx_D(mz, b->a2);
+        o->cont = b->cont;
+        if (u0 < f) {
+            o->event = EV_GLOSSY | EV_REFLECTION;
+            V wil = vreflect(vneg(wo), m);
+            o->cos_out = (real)fabs(wil.z);
+            if (o->cos_out < K(1e-6f)) return zero;
+            o->wi = to_world(&b->fr, wil);
+            if (!same_side(b, o->wi, gn) || wil.z * wo.z <= K(0)) return zero;
+            o->fwd = o->rev = ((f * d) * mz) * K(0.25f) / wom;
+            real g2 = ggx_G2((real)fabs(wo.z), (real)fabs(wil.z), b->a2);
+            return vscale(vscale(b->R, g2), wom / ((real)fabs(wo.z) * mz));
+        }
+        o->event = EV_GLOSSY | EV_TRANSMISSION;
+        V wil;
+        if (!vrefract(wo, m, eta, &wil)) return zero;
+        o->cos_out = (real)fabs(wil.z);
+        if (o->cos_out < K(1e-6f) || wil.z * wo.z >= K(0)) return zero;
+        o->wi = to_world(&b->fr, wil);
+        if (vdot(o->wi, gn) * vdot(to_world(&b->fr, wo), gn) >= K(0)) return zero;
+        /* pdfs with the Jacobian of the refracted direction (Walter et al. eq. 17) */
+        real wim = vdot(wil, m);
+        real sq = wim + eta * wom;
+        real den = sq * sq;
+        real t = ((K(1) - f) * d) * mz;
+        o->fwd = (t * (real)fabs(wim)) / den;
+        o->rev = (t * ((eta * eta) * wom)) / den;
+        real g2 = ggx_G2((real)fabs(wo.z), (real)fabs(wil.z), b->a2);
+        V w = vscale(vscale(b->R, g2), wom / ((real)fabs(wo.z) * mz));
+        if (b->radiance) w = vscale(vscale(w, eta), eta);
+        return w;
+    }
     case BK_FMG: { /* BXDF.cpp:384-489 */
         real f = fresnel_dielectric_reflectance((real)fabs(wo.z), b->s);
         if (u0 < f) {
@@ -554,6 +606,28 @@ static V bsdf_sample(const Bsdf *b, V gn, real u0, real u1, real u2, Sample *o) 
     }
 }
 
+/* The rough dielectric's half vector for a pair (wo, wi): wo + wi for a
+ * reflection, -(wi + eta wo) for a refraction (the generalised half vector,
+ * Walter et al. eq. 16), turned to wo's side; with its cosines, Fresnel term
+ * and D. */
+typedef struct { int refl; real eta, wom, wim, f, mz, d; } RoughPair;
+static RoughPair rough_pair(const Bsdf *b, V wil) {
+    RoughPair r;
+    V wo = b->wo;
+    r.eta = b->ior;
+    if (wo.z > 0) r.eta = K(1) / b->ior;
+    r.refl = wil.z * wo.z > K(0);
+    V mh = r.refl ? vadd(wo, wil) : vneg(vadd(wil, vscale(wo, r.eta)));
+    mh = vnormalize(mh);
+    if (mh.z * wo.z < K(0)) mh = vneg(mh);
+    r.wom = vdot(wo, mh);
+    r.wim = vdot(wil, mh);
+    r.f = fresnel_dielectric_eta(r.wom, K(1) / r.eta);
+    r.mz = (real)fabs(mh.z);
+    r.d = ggx_D(r.mz, b->a2);
+    return r;
+}
+
 /* BxDF::evaluate */
 static V bsdf_eval(const Bsdf *b, V wi, V gn) {
     const V zero = vs(0);
@@ -571,6 +645,24 @@ static V bsdf_eval(const Bsdf *b, V wi, V gn) {
         real chn = (real)fabs(h.z);
         real d = ggx_D(chn, b->a2);
         return vdivs(vscale(vscale(vscale(b->R, d), g2), K(0.25f)), (real)fabs(wo.z));
+    }
+    case BK_RGLASS: { /* f |cos wi| of the rough dielectric */
+        V wil = to_local(&b->fr, wi);
+        RoughPair r = rough_pair(b, wil);
+        real g2 = ggx_G2((real)fabs(wo.z), (real)fabs(wil.z), b->a2);
+        if (r.refl) {
+            if (!same_side(b, wi, gn) || !(r.wom > K(0))) return zero;
+            return vscale(b->R, ((r.f * r.d) * g2) * K(0.25f) / (real)fabs(wo.z));
+        }
+        real sq = r.wim + r.eta * r.wom;
+        real den = sq * sq;
+        if (!(vdot(wi, gn) * vdot(to_world(&b->fr, wo), gn) < K(0)) || !(r.wom > K(0)) || !(r.wim < K(0)))
+            return zero;
+        /* the eta^2 of radiance transport, as the sampler's weight (the merge evaluates in radiance mode) */
+        real v = (((K(1) - r.f) * r.d) * g2) * (r.wom * (real)fabs(r.wim));
+        if (b->radiance) v = v * (r.eta * r.eta);
+        v = v / ((real)fabs(wo.z) * den);
+        return vscale(b->R, v);
     }
     case BK_FMG: { /* BXDF.cpp:491-519 */
         if (!same_side(b, wi, gn)) return zero;
@@ -606,6 +698,21 @@ static void bsdf_pdf(const Bsdf *b, V wi, real *fwd, real *rev) {
         real chn = (real)fabs(h.z);
         real d = ggx_D(chn, b->a2);
         *fwd = *rev = d * chn * K(0.25f) / vabsdot(h, wo);
+        return;
+    }
+    case BK_RGLASS: {
+        RoughPair r = rough_pair(b, to_local(&b->fr, wi));
+        /* directions no microfacet normal on wo's side sends wo to have density 0 */
+        if (!(r.wom > K(0)) || (!r.refl && !(r.wim < K(0)))) { *fwd = *rev = 0; return; }
+        if (r.refl) {
+            *fwd = *rev = ((r.f * r.d) * r.mz) * K(0.25f) / r.wom;
+            return;
+        }
+        real sq = r.wim + r.eta * r.wom;
+        real den = sq * sq;
+        real t = ((K(1) - r.f) * r.d) * r.mz;
+        *fwd = (t * (real)fabs(r.wim)) / den;
+        *rev = (t * ((r.eta * r.eta) * r.wom)) / den;
         return;
     }
     case BK_FMG: { /* BXDF.cpp:521-548 */
@@ -948,6 +1055,7 @@ typedef struct {
     int max_path_length;
     /* options */
     int rng_serial, compat_point, compat_pdf, compat_rows, camera_stride, gather_brute;
+    int compat_glass; /* 0: glass of roughness >= 0.01 is the rough dielectric */
     uint64_t seed;
     long long photons;
     int prepared;
@@ -979,7 +1087,7 @@ void *orc_create(int samples, float base_radius, float radius_reduction, const l
     s->max_path_length = 6; /* main.cpp:112 */
     s->camera_stride = 1;
     s->mlt_chains = 262144;
-    s->compat_point = 1; s->compat_pdf = 1; s->compat_rows = 1;
+    s->compat_point = 1; s->compat_pdf = 1; s->compat_rows = 1; s->compat_glass = 1;
     s->xs.a = 132141u;
     s->nmeshes = number_of_meshes;
     s->meshes = (Mesh *)calloc(number_of_meshes ? number_of_meshes : 1, sizeof(Mesh));
@@ -1095,6 +1203,7 @@ int orc_set_option(void *sp, const char *key, double v) {
     else if (!strcmp(key, "photons")) s->photons = (long long)v;
     else if (!strcmp(key, "compat_point_light")) s->compat_point = (int)v;
     else if (!strcmp(key, "compat_light_pdf")) s->compat_pdf = (int)v;
+    else if (!strcmp(key, "compat_glass")) s->compat_glass = (int)v;
     else if (!strcmp(key, "compat_rows")) s->compat_rows = (int)v;
     else if (!strcmp(key, "libm")) g_libm = (int)v;
     else if (!strcmp(key, "camera_stride")) s->camera_stride = v < 1 ? 1 : (int)v;
@@ -1194,6 +1303,8 @@ static void init_lights(Scene *s) {
 
 int orc_prepare(void *sp) {
     Scene *s = (Scene *)sp;
+    for (uint32_t i = 0; i < s->nmats; ++i)
+        s->mats[i].rough_glass = s->mats[i].type == 'G' && !s->compat_glass && s->mats[i].roughness >= K(0.01f);
     /* Scene::update_bvh (Scene.cpp:110-152) */
     Item *witems = (Item *)malloc(sizeof(Item) * (s->nmeshes ? s->nmeshes : 1));
     for (uint32_t i = 0; i < s->nmeshes; ++i) {
@@ -2170,12 +2281,14 @@ void orc_counters(void *sp, long long *out3) {
 void orc_bsdf(const float *mat, int mode, const float *shade_n, const float *geo_n, const float *dir,
               const float *u3, const float *wi, int op, double *out) {
     unsigned char pm[28] = {0};
-    pm[0] = (unsigned char)(int)mat[0];
+    const int rough_glass = (int)mat[0] == 'R'; /* test code: glass as the rough dielectric */
+    pm[0] = rough_glass ? (unsigned char)'G' : (unsigned char)(int)mat[0];
     memcpy(pm + 4, &mat[1], 4);
     memcpy(pm + 8, &mat[2], 4);
     memcpy(pm + 12, &mat[3], 4);
     memcpy(pm + 16, &mat[4], 12);
     Material m = material_from_python(pm);
+    m.rough_glass = rough_glass;
     Bsdf b;
     bsdf_init(&b, &m, v3(shade_n[0], shade_n[1], shade_n[2]), v3(dir[0], dir[1], dir[2]), mode == 0);
     V gn = v3(geo_n[0], geo_n[1], geo_n[2]);

@@ -52,7 +52,8 @@ void orc_free(void *scene);
  * (light paths per iteration; 0 = W*H as the reference), "compat_point_light",
  * "compat_light_pdf", "compat_rows", "libm" (1 = C library sin/cos/acos instead
  * of the engine's own), "camera_stride" (trace every k-th pixel only),
- * "max_path_length", "gather" (0 = KD tree, 1 = brute force). */
+ * "max_path_length", "gather" (0 = KD tree, 1 = brute force), "compat_glass" (0 = glass of
+ * roughness >= 0.01 is a GGX microfacet dielectric; 1, the default, the reference's delta glass). */
 int orc_set_option(void *scene, const char *key, double value);
 double orc_get(void *scene, const char *key);
 /* Stratified camera words (option camera_qmc) of a pixel's sample `iteration`:
@@ -104,7 +105,8 @@ void orc_philox(const uint32_t *ctr, const uint32_t *key, uint32_t *out);
 double orc_math(int which, double x);
 
 /* One BSDF evaluation, for unit tests.
- * mat = {type char code, ior, roughness, specular, r, g, b}; mode 0 radiance / 1 importance.
+ * mat = {type char code, ior, roughness, specular, r, g, b} ('R': glass as the rough GGX
+ * dielectric of option compat_glass = 0); mode 0 radiance / 1 importance.
  * shade_n, geo_n, dir (incoming ray direction) are 3 floats each.
  * op 0 = sample with u3 -> out[0..2] factor, out[3..5] wi, out[6] fwd, out[7] rev,
  *        out[8] cos_out, out[9] cont, out[10] event, out[11] valid, out[12] delta

@@ -169,6 +169,54 @@ __device__ F3 bsdf_sample(const Bsdf& b, F3 gn, float u0, float u1, float u2, Bs
         float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
         return div_recip((b.R * g2) * absdot(wo, h), fabsf(wo.z) * chn);
     }
+    if (b.kind == BK_RGLASS) {
+        // rough dielectric (Walter, Marschner, Li, Torrance 2007): a GGX
+        // microfacet normal on wo's side; the Fresnel term of wo at the
+        // microfacet picks reflection or refraction through it.  The weight
+        // (f |cos wi| / pdf) is G2 (wo.m) / (|wo.n| |m.n|) for both, times
+        // eta^2 for refraction in radiance mode -- Fresnel_Specular's weights
+        // in the limit of zero roughness.
+        F3 m = ggx_half(b.a2, u1, u2);
+        if (wo.z < 0.0f) m.z = -m.z;
+        const float wom = dot(wo, m);
+        if (!(wom > 0.0f)) return zero;
+        float eta = b.ior;
+        if (wo.z > 0.0f) eta = 1.0f / b.ior;
+        const float f = fresnel_eta(wom, 1.0f / eta);
+        const float mz = fabsf(m.z);
+        const float d = ggx_D(mz, b.a2);
+        o.cont = b.cont;
+        if (u0 < f) {
+            o.event = EV_GLOSSY | EV_REFLECTION;
+            F3 wil = reflect(-wo, m);
+            o.cos_out = fabsf(wil.z);
+            if (o.cos_out < 1e-6f) return zero;
+            o.wi = to_world(b.fr, wil);
+            if (!same_side(b, o.wi, gn) || wil.z * wo.z <= 0.0f) return zero;
+            o.fwd = o.rev = ((f * d) * mz) * 0.25f / wom;
+            const float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+            return (b.R * g2) * (wom / (fabsf(wo.z) * mz));
+        }
+        o.event = EV_GLOSSY | EV_TRANSMISSION;
+        F3 wil;
+        if (!refract(wo, m, eta, &wil)) return zero;
+        o.cos_out = fabsf(wil.z);
+        if (o.cos_out < 1e-6f || wil.z * wo.z >= 0.0f) return zero;
+        o.wi = to_world(b.fr, wil);
+        if (dot(o.wi, gn) * dot(to_world(b.fr, wo), gn) >= 0.0f) return zero;
+        // pdfs with the Jacobian of the refracted direction, wi.m + eta wo.m
+        // being the generalised half vector's length over the wi-side index
+        const float wim = dot(wil, m);
+        const float sq = wim + eta * wom;
+        const float den = sq * sq;
+        const float t = ((1.0f - f) * d) * mz;
+        o.fwd = (t * fabsf(wim)) / den;
+        o.rev = (t * ((eta * eta) * wom)) / den;
+        const float g2 = ggx_G2(fabsf(wo.z), fabsf(wil.z), b.a2);
+        F3 w = (b.R * g2) * (wom / (fabsf(wo.z) * mz));
+        if (b.radiance) w = (w * eta) * eta;
+        return w;
+    }
     // plastics
     float f = fresnel_refl(fabsf(wo.z), b.s);
     if (b.kind == BK_FMG) {
@@ -281,6 +329,17 @@ __device__ __forceinline__ F3 mnormalize(F3 a) {
     return a * s;
 }
 template <bool kFast>
+__device__ __forceinline__ float mfresnel_eta(float c, float eta) {
+    float g = eta * eta - 1.0f + c * c;
+    if (g > 0.0f) {
+        g = msqrt<kFast>(g);
+        float A = mdiv<kFast>(g - c, g + c);
+        float B = mdiv<kFast>(c * (g + c) - 1.0f, c * (g - c) + 1.0f);
+        return 0.5f * A * A * (1.0f + B * B);
+    }
+    return 1.0f;
+}
+template <bool kFast>
 __device__ __forceinline__ float mggx_D(float ndoth, float a2) {
     float den = ndoth * ndoth * (a2 - 1.0f) + 1.0f;
     return mdiv<kFast>(a2, kPI * den * den);
@@ -335,6 +394,36 @@ __device__ __forceinline__ F3 bsdf_eval_pdf(const Bsdf& b, const MergeConst& m, 
         float g2 = mggx_G2<kFast>(m.awz, fabsf(wil.z), b.a2);
         float spec = mdiv<kFast>(f * d * g2 * 0.25f, m.awz);
         return f3(m.diffuse.x + spec, m.diffuse.y + spec, m.diffuse.z + spec);
+    }
+    if (b.kind == BK_RGLASS) {
+        // the rough dielectric's evaluate (f |cos wi|) and pdfs for a photon
+        // direction: a reflection about m = wo + wi, or a refraction through
+        // m = -(wi + eta wo) (the generalised half vector), m turned to wo's side
+        float eta = b.ior;
+        if (wo.z > 0.0f) eta = 1.0f / b.ior;
+        const bool refl = wil.z * wo.z > 0.0f;
+        F3 mh = refl ? wo + wil : -(wil + wo * eta);
+        mh = mnormalize<kFast>(mh);
+        if (mh.z * wo.z < 0.0f) mh = -mh;
+        const float wom = dot(wo, mh), wim = dot(wil, mh);
+        const float f = mfresnel_eta<kFast>(wom, mdiv<kFast>(1.0f, eta));
+        const float mz = fabsf(mh.z);
+        const float d = mggx_D<kFast>(mz, b.a2);
+        const float g2 = mggx_G2<kFast>(m.awz, fabsf(wil.z), b.a2);
+        if (refl) {
+            fwd = rev = mdiv<kFast>(((f * d) * mz) * 0.25f, wom);
+            if (!side || !(wom > 0.0f)) return zero;
+            return b.R * mdiv<kFast>(((f * d) * g2) * 0.25f, m.awz);
+        }
+        const float sq = wim + eta * wom;
+        const float den = sq * sq;
+        const float t = ((1.0f - f) * d) * mz;
+        fwd = mdiv<kFast>(t * fabsf(wim), den);
+        rev = mdiv<kFast>(t * ((eta * eta) * wom), den);
+        if (!(dot(wi, m.gn) * m.side_wo < 0.0f) || !(wom > 0.0f) || !(wim < 0.0f)) return zero;
+        // radiance mode (the merge's camera vertices): the eta^2 of radiance transport
+        const float v = mdiv<kFast>(((((1.0f - f) * d) * g2) * (wom * fabsf(wim))) * (eta * eta), m.awz * den);
+        return b.R * v;
     }
     if (b.kind == BK_FSMG) {
         float f = m.f;

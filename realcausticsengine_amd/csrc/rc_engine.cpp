@@ -79,6 +79,8 @@ struct Engine {
     uint64_t seed = 0;
     long long photons = 0;
     int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
+    int compat_glass = 1;  // 0: glass of roughness >= 0.01 is a GGX dielectric (BK_RGLASS)
+    std::vector<rc_python_material> py_mats;  // as given to init; converted again at rc_prepare
     double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
     // launch shapes and queue / tile sizes (options, or chosen by rc_autotune)
@@ -197,8 +199,10 @@ void inverse3x3_of4(const float* M, float inv[3][3]) {
     inv[2][2] = (float)(det * (a[0][0] * A1313 - a[0][1] * A0313 + a[0][3] * A0113));
 }
 
-DevMaterial convert_material(const rc_python_material& pm) {
-    // Scene::init_materials (Scene.cpp:45-68) and materials.h:59-146
+DevMaterial convert_material(const rc_python_material& pm, int compat_glass) {
+    // Scene::init_materials (Scene.cpp:45-68) and materials.h:59-146; with
+    // compat_glass = 0 glass of roughness >= 0.01 is a GGX dielectric of
+    // a2 = roughness^2 (the metal's mapping, materials.h:127-137)
     DevMaterial m{};
     F3 c = f3(pm.color[0], pm.color[1], pm.color[2]);
     char t = pm.type;
@@ -223,7 +227,8 @@ DevMaterial convert_material(const rc_python_material& pm) {
             m.a2 = rough * rough;
         }
     } else if (t == 'G') {
-        m.kind = BK_FRESNEL_SPEC;
+        m.kind = (!compat_glass && pm.roughness >= 0.01f) ? BK_RGLASS : BK_FRESNEL_SPEC;
+        if (m.kind == BK_RGLASS) m.a2 = pm.roughness * pm.roughness;
         m.ior = pm.ior;
         m.cont = cont_of(c);
     } else {
@@ -561,6 +566,13 @@ bool upload(DevBuf<T>& b, const std::vector<T>& v) {
 
 bool uses_nodes4(const Engine& e) { return e.bvh_width == 4 && e.n_nodes4 > 0; }
 
+// Materials converted again with the options in force at rc_prepare (compat_glass).
+bool reconvert_materials(Engine& e) {
+    for (size_t i = 0; i < e.py_mats.size() && i < e.mats.size(); ++i)
+        e.mats[i] = convert_material(e.py_mats[i], e.compat_glass);
+    return true;
+}
+
 SceneView scene_view(const Engine& e) {
     SceneView sc{};
     sc.tri = e.d_tri.p;
@@ -805,7 +817,8 @@ bool prepare(Engine& e) {
     e.scene_radius = sqrtf(len2(mn - (mn + mx) * 0.5f));
     e.base_radius *= e.scene_radius;
     if (!upload(e.d_tri, tri) || !upload(e.d_shade, shade) || !upload(e.d_tri_id, tri_id) ||
-        !upload(e.d_nodes, bld.nodes) || !upload(e.d_nodes4, nodes4) || !upload(e.d_mats, e.mats) ||
+        !upload(e.d_nodes, bld.nodes) || !upload(e.d_nodes4, nodes4) || !reconvert_materials(e) ||
+        !upload(e.d_mats, e.mats) ||
         !upload(e.d_lights, e.lights) ||
         !upload(e.d_cdf, cdf))
         return false;
@@ -1885,6 +1898,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.compat_rows = from.compat_rows;
     to.compat_point = from.compat_point;
     to.compat_pdf = from.compat_pdf;
+    to.compat_glass = from.compat_glass;
     to.max_path_length = from.max_path_length;
     to.photon_capacity = from.photon_capacity;
     to.cell_scale = from.cell_scale;
@@ -2473,7 +2487,10 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
     if (number_of_materials > 0 && !materials) set_error("null material array");
     if (number_of_lights > 0 && !lights) set_error("null light array");
     if (!g_error.empty()) return;
-    for (unsigned i = 0; i < number_of_materials; ++i) e->mats.push_back(convert_material(materials[i]));
+    for (unsigned i = 0; i < number_of_materials; ++i) {
+        e->py_mats.push_back(materials[i]);
+        e->mats.push_back(convert_material(materials[i], e->compat_glass));
+    }
     for (unsigned i = 0; i < number_of_lights; ++i) {
         HostLight h;
         memcpy(&h, &lights[i], sizeof h);
@@ -2629,6 +2646,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "write_exr") g->write_exr = (int)v;
     else if (k == "compat_rows") g->compat_rows = (int)v;
     else if (k == "compat_point_light") g->compat_point = (int)v;
+    else if (k == "compat_glass") g->compat_glass = (int)v;
     else if (k == "compat_light_pdf") g->compat_pdf = (int)v;
     else if (k == "max_path_length") g->max_path_length = std::max(0, std::min(7, (int)v));
     else if (k == "photon_capacity") g->photon_capacity = v;
@@ -2708,6 +2726,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "mlt_visible") return g->mlt_last[0];  // counts of the last MLT light pass
     if (k == "mlt_mutations") return g->mlt_last[1];
     if (k == "mlt_accepted") return g->mlt_last[2];
+    if (k == "compat_glass") return g->compat_glass;
     if (k == "bvh_depth") return g->bvh_depth;
     if (k == "bvh_depth4") return g->bvh_depth4;
     if (k == "bvh_traversal_width") return uses_nodes4(*g) ? 4 : 2;

@@ -17,7 +17,10 @@
 
 namespace rc {
 
-enum { BK_SPEC_REFL = 0, BK_FRESNEL_SPEC = 1, BK_FMG = 2, BK_FSMG = 3, BK_MRG = 4 };
+// BSDF classes of the reference (materials.h:59-146), plus BK_RGLASS: glass of
+// roughness >= 0.01 as a GGX microfacet dielectric (option compat_glass = 0;
+// the reference's glass is always a delta interface).
+enum { BK_SPEC_REFL = 0, BK_FRESNEL_SPEC = 1, BK_FMG = 2, BK_FSMG = 3, BK_MRG = 4, BK_RGLASS = 5 };
 enum { EV_REFLECTION = 1, EV_TRANSMISSION = 2, EV_DIFFUSE = 4, EV_GLOSSY = 8, EV_SPECULAR = 16 };
 enum { L_AREA = 0, L_SUN = 1, L_POINT = 2 };
 

@@ -185,7 +185,8 @@ def random_mesh_scene(seed: int = 0, n_spheres: int = 3, subdiv: int = 2, width:
     return SceneArgs(meshes, lights, mats, cam, samples=1)
 
 
-def materials_scene(width: int = 48, height: int = 32, subdiv: int = 2, lights=None) -> SceneArgs:
+def materials_scene(width: int = 48, height: int = 32, subdiv: int = 2, lights=None,
+                    glass_roughness: float = 0.0) -> SceneArgs:
     """One sphere of each BSDF class of the reference in a row over a glossy floor.
 
     Spheres, left to right: glass (``Fresnel_Specular``), mirror metal
@@ -195,8 +196,10 @@ def materials_scene(width: int = 48, height: int = 32, subdiv: int = 2, lights=N
     roughness^2 < 0.01) and a diffuse plastic (specular 0); the floor is a
     glossy plastic with specular > 0, so merge queries on it take the general
     BSDF evaluation (``src/materials.h:59-146``).  A 2 x 1 area light rotated
-    by (10, 0, 20) degrees sits above, unless `lights` replaces it."""
-    mats = [MaterialData("G", ior=1.45, color=(0.9, 0.95, 1.0)),
+    by (10, 0, 20) degrees sits above, unless `lights` replaces it.  The
+    glass sphere takes `glass_roughness` (a rough dielectric with option
+    compat_glass = 0; the reference ignores it)."""
+    mats = [MaterialData("G", ior=1.45, roughness=glass_roughness, color=(0.9, 0.95, 1.0)),
             MaterialData("M", roughness=0.0, color=(0.9, 0.9, 0.9)),
             MaterialData("M", roughness=0.35, color=(0.9, 0.8, 0.7)),
             MaterialData("P", roughness=0.5, specular=0.6, color=(0.3, 0.6, 0.3)),
