@@ -60,15 +60,28 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
             "resolve": resolve}
 
 
+TRAFFIC_FILE = os.path.join("profiles", "traffic_water.json")
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` on this workload, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/), or None."""
-    path = os.path.join(ROOT, "profiles", "traffic_water.json")
     try:
-        with open(path) as f:
+        with open(os.path.join(ROOT, TRAFFIC_FILE)) as f:
             return json.load(f).get(kernel)
     except (OSError, ValueError):
         return None
+
+
+# What limits each kernel beyond its bytes, from the SQ / TCC counter passes
+# of profiles/water_sq_counters.md (VALU lanes active per instruction, VALU
+# instructions per vector memory instruction, L2 hit rate).
+LIMITER = {
+    "light_pass": "instruction issue across divergent lanes (about 17 of 64 lanes active per VALU instruction, "
+                  "27 VALU per VMEM instruction, 59 % L2 hits): algorithmic bytes mostly come from L2 / MALL",
+    "gather": "instruction issue (distance tests and BSDF evaluations per photon-query pair)",
+    "camera_trace": "instruction issue across divergent lanes",
+}
 
 
 def cpu_model() -> str:
@@ -396,6 +409,43 @@ def time_to_rmse(a, cpu_s_per_iteration, cpu_all_s_per_iteration=None):
     return out
 
 
+def rough_glass_leg(a, launch: dict) -> dict:
+    """The same water workload with the water as a rough GGX dielectric
+    (roughness 0.02, option compat_glass = 0; the reference renders its glass
+    as delta whatever the roughness): photon paths/s and kernel times with the
+    main leg's launch shapes (photons are now also stored on the water and
+    camera paths merge there)."""
+    from realcausticsengine_amd import scenes
+    from realcausticsengine_amd.engine import Engine
+
+    args = scenes.water_scene(a.grid, a.res, a.res, roughness=0.02, samples=64)
+    eng = Engine(args, photons=a.photons, seed=0, write_exr=0, compat_glass=0)
+    eng.prepare()
+    for k, v in launch.items():
+        eng.set(k, v)
+    for _ in range(2):
+        eng.render_iteration()
+    n = max(3, a.steps // 2)
+    kern = {"ms_light": 0.0, "ms_map": 0.0, "ms_camera_trace": 0.0, "ms_gather": 0.0}
+    photons = gathers = 0.0
+    t0 = time.perf_counter()
+    for _ in range(n):
+        eng.render_iteration()
+        st = eng.stats()
+        for k in kern:
+            kern[k] += st[k] / n
+        photons += st["photons"] / n
+        gathers += st["gathers"] / n
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    eng.release()
+    out = {"workload": "the water workload with roughness 0.02 as a GGX dielectric (compat_glass = 0)",
+           "ms_per_step": round(ms, 3), "photon_paths_per_s": round(a.photons / (ms * 1e-3), 1),
+           "photons_per_iteration": int(photons), "queries_per_iteration": int(gathers),
+           "kernel_ms": {k: round(v, 3) for k, v in kern.items()}}
+    print(f"[bench] rough glass: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def copy_calibration() -> int:
     """Three copies of 1 GiB between device buffers: what FETCH_SIZE and
     WRITE_SIZE read for a known number of bytes moved."""
@@ -432,6 +482,7 @@ def main() -> int:
     ap.add_argument("--launch", default="", help="fixed launch shapes instead of autotuning, "
                     "e.g. path_chunk=512,light_blocks_per_cu=8,gather_blocks_per_cu=32")
     ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")
+    ap.add_argument("--no-rough", action="store_true", help="skip the rough-glass leg")
     ap.add_argument("--guide-steps", type=int, default=0,
                     help="only time this many training steps of the learned proposal (65,536 rows each)")
     ap.add_argument("--at-scale", type=int, default=0,
@@ -458,7 +509,9 @@ def main() -> int:
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Engine
 
-    args = scenes.water_scene(a.grid, a.res, a.res, roughness=0.0, samples=a.warmup + a.steps + 1)
+    # BASELINE's water is GGX roughness 0.02; with compat_glass = 1 (the default) glass is a delta interface
+    # whatever its roughness, as in the reference (src/materials.h:104), so this is the reference's rendering
+    args = scenes.water_scene(a.grid, a.res, a.res, roughness=0.02, samples=a.warmup + 2 * a.steps + 4)
     eng = Engine(args, photons=a.photons, seed=0, write_exr=0)
     t = time.perf_counter()
     eng.prepare()
@@ -488,7 +541,25 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     ms_step = elapsed * 1000.0 / a.steps
     kern = {k: v / a.steps for k, v in kern.items()}
-    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernel ms {kern}", file=sys.stderr, flush=True)
+    overlap = int(eng.get("overlap"))
+    print(f"[bench] {a.steps} steps, {ms_step:.2f} ms/step, kernel ms {kern} (overlap {overlap})",
+          file=sys.stderr, flush=True)
+    kern_timed = dict(kern)
+    if overlap:
+        # beside the light pass the camera trace shares the GPU: the per-kernel times of the roofline
+        # come from serial iterations (untimed for the step rate)
+        eng.set("overlap", 0)
+        kern = {k: 0.0 for k in keys}
+        for _ in range(a.steps):
+            eng.render_iteration()
+            st = eng.stats()
+            for k, sk in keys.items():
+                kern[k] += st[sk] / a.steps
+        eng.set("overlap", 1)
+        print(f"[bench] serial kernel ms {kern}", file=sys.stderr, flush=True)
+    launch = {k: eng.get(k) for k in ("path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
+                                      "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill",
+                                      "group_queries", "gather_tile", "gather_block", "query_order", "overlap")}
     # counting launch (not timed) of the middle timed iteration: its event
     # counts give the algorithmic bytes (the merge radius shrinks per iteration)
     eng.set("next_iteration", a.warmup + a.steps // 2)
@@ -499,11 +570,20 @@ def main() -> int:
     by = algorithmic_bytes(cst, int(eng.get("node_bytes")), eng.get("group_queries") > 0)
     dom = max(kern, key=kern.get)
     achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
+    traffic = profiled_traffic(dom)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": profiled_traffic(dom)}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": f"{TRAFFIC_FILE}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this kernel on this "
+                                  "workload, scaled by a 1 GiB copy (profiles/water_kernels.md)",
+                "traffic_frac": (round(traffic / (kern[dom] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4) if traffic else None),
+                "kernel_ms": round(kern[dom], 4), "timing": "serial iterations (overlap 0), HIP events on the "
+                                                           "engine's stream",
+                "limiter": LIMITER.get(dom)}
     extra = {
         "autotune": tuned,
         "kernel_ms": {k: round(v, 3) for k, v in kern.items()},
+        "kernel_ms_in_timed_steps": {k: round(v, 3) for k, v in kern_timed.items()},
+        "overlap_in_timed_steps": overlap,
         "kernel_achieved_GBs": {k: round(by[k] / (kern[k] * 1e-3) / 1e9, 1) for k in kern if kern[k] > 0},
         "algorithmic_bytes": {k: int(v) for k, v in by.items()},
         "photons_per_iteration": int(cst["photons"]),
@@ -523,6 +603,7 @@ def main() -> int:
     if not a.no_batch:
         batch = uneven_batch(a)
     graph = None if a.no_graph else graph_leg(a)
+    rough = None if a.no_rough else rough_glass_leg(a, launch)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None,
@@ -541,14 +622,19 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "water caustic: 512x512-vertex sine heightfield (522,242 tris, IOR 1.33) over "
-                               "diffuse floor, 2x2 area light, 1024x1024, 16,777,216 light paths/iteration",
+        "config": {"workload": "water caustic: 512x512-vertex sine heightfield (522,242 tris, IOR 1.33, GGX "
+                               "roughness 0.02) over diffuse floor, 2x2 area light, 1024x1024, 16,777,216 light "
+                               "paths/iteration",
+                   "glass": "delta interface (compat_glass = 1): the reference ignores glass roughness "
+                            "(src/materials.h:104), so this is its rendering of the configuration; the rough "
+                            "dielectric (compat_glass = 0) is the rough_glass leg",
                    "estimator": "plain vertex merging", "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "time_to_rmse": ttr,
         "uneven_batch": batch,
         "iteration_graph": graph,
+        "rough_glass": rough,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)

@@ -1524,6 +1524,9 @@ bool render_iteration(Engine& e, int iteration) {
     e.stats[RC_STAT_MS_GATHER] = ms_gather;
     e.stats[RC_STAT_MS_RESOLVE] = ms_resolve;
     e.stats[RC_STAT_MS_QUERY_SORT] = ms_sort;
+    // overlapped: the span from the trace's start to the resolve's end covers
+    // the light pass and the map build too, so the camera pass is its own spans
+    if (overlapped && !graphed) ms[2] = ms_trace + ms_sort + ms_gather + ms_resolve;
     if (training) {
         float ms_guide = 0;
         HIP_OK(hipEventElapsedTime(&ms_guide, e.ev_guide[0], e.ev_guide[1]));
@@ -2385,6 +2388,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     else HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
     HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
     HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
+    if (overlapped) ms[2] = ms[3] + ms[6] + ms[4] + ms[5];  // the camera pass's own spans (see run_iteration)
     std::fill(b.stats, b.stats + RC_STAT_COUNT, 0.0);
     b.stats[RC_STAT_LIGHT_PATHS] = (double)light;
     b.stats[RC_STAT_PHOTONS] = count;
