@@ -57,3 +57,37 @@ def test_overlapped_trace_retraces_on_query_overflow():
     for a, b in zip(serial, overlapped):
         assert a.sum() > 0
         assert np.array_equal(a, b)
+
+
+def _render_batch(overlap, iters=3):
+    """The two-plane scene twice in one batch (the second with another seed
+    and a smaller film) beside a glass-sphere scene: the batch's first query
+    buffer (2 queries per camera path + 1024) overflows in the first iteration."""
+    from realcausticsengine_amd.engine import Batch
+
+    cases = [_two_planes(samples=iters), _two_planes(40, 24, samples=iters), scenes.c0_scene(32, 32, subdiv=2,
+                                                                                            samples=iters)]
+    opts = [dict(photons=65536, seed=9, write_exr=0, overlap=overlap),
+            dict(photons=32768, seed=4, write_exr=0, overlap=overlap),
+            dict(photons=16384, seed=2, write_exr=0, overlap=overlap)]
+    b = Batch(cases, opts)
+    b.prepare()
+    films, gathers = [], []
+    for _ in range(iters):
+        b.render_iteration()
+        films.append([b.image(i, scaled=False) for i in range(len(cases))])
+        gathers.append(b.stats()["gathers"])
+    paths = b.stats()["camera_paths"]
+    b.release()
+    return films, gathers, paths
+
+
+def test_overlapped_batch_trace_retraces_on_query_overflow():
+    serial, g0, paths = _render_batch(0)
+    overlapped, g1, _ = _render_batch(1)
+    assert g0[0] > 2 * paths, (g0[0], paths)  # the batch's first trace overflowed its first buffer
+    assert g0 == g1
+    for fs, fo in zip(serial, overlapped):
+        for a, b in zip(fs, fo):
+            assert a.sum() > 0
+            assert np.array_equal(a, b)
