@@ -173,6 +173,8 @@ enum {
     RC_STAT_PHOTONS_STREAMED,
     /* adaptive MLT light pass (option "mlt"): visible uniform paths, mutations, accepted mutations */
     RC_STAT_MLT_VISIBLE, RC_STAT_MLT_MUTATIONS, RC_STAT_MLT_ACCEPTED,
+    /* graphs captured so far (option "graph"; for a batch, scene 0's option) */
+    RC_STAT_GRAPH_CAPTURES,
     RC_STAT_COUNT
 };
 
@@ -208,7 +210,10 @@ enum { RC_TUNE_REPORT = 15 };
  * each iteration replay one captured hipGraph, reading counts and
  * per-iteration values from device memory; the graph is captured again when
  * buffers grow or the light-path count changes (rc_get_option("graph_captures")).
- * The film is identical to the one direct launches give. */
+ * The film is identical to the one direct launches give.  In a batch, scene
+ * 0's "graph" captures the batched map and camera pass of all scenes (the
+ * per-scene values come from device tables refilled before each replay;
+ * RC_STAT_GRAPH_CAPTURES counts the captures). */
 int rc_autotune(int repeats, double *report, int n);
 
 /* ---- diagnostics used by the parity tests ---- */

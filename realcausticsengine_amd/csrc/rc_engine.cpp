@@ -1554,6 +1554,7 @@ bool render_iteration(Engine& e, int iteration) {
     e.stats[RC_STAT_NODES] = e.n_nodes;
     e.stats[RC_STAT_TRIS] = (double)(e.tris.size() / 4);
     e.stats[RC_STAT_SORT_PASSES] = e.sort_passes;
+    e.stats[RC_STAT_GRAPH_CAPTURES] = e.graph_captures;
     if (e.count_events) {
         e.stats[RC_STAT_LIGHT_NODE_VISITS] = (double)c[0].nodes;
         e.stats[RC_STAT_LIGHT_TRI_TESTS] = (double)c[0].tris;
@@ -1976,6 +1977,10 @@ struct Batch {
     std::vector<Engine*> scenes;
     Engine work;
     bool prepared = false;
+    // captured batch iteration (scene 0's option "graph"): photon map + camera pass
+    hipGraphExec_t graph_exec = nullptr;
+    std::vector<uintptr_t> graph_key;
+    int graph_captures = 0;
     DevBuf<SceneView> d_sc;
     DevBuf<PhotonMapView> d_pm;
     DevBuf<BatchScene> d_bs;
@@ -1993,6 +1998,8 @@ Batch* g_batch = nullptr;
 
 void release_batch(Batch*& b) {
     if (!b) return;
+    if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
+    b->graph_exec = nullptr;
     for (Engine*& e : b->scenes) release(e);
     Engine& w = b->work;
     for (auto* d : {&w.ph_pos, &w.ph_dir, &w.s_pos, &w.s_dir, &w.q_pos, &w.q_dir, &w.q_sn, &w.q_gn, &w.q_tpn})
@@ -2120,6 +2127,82 @@ bool batch_guide_training(Batch& b, const BatchView& view, uint32_t count, uint3
     return true;
 }
 
+// The batch's photon map and camera pass replayed from one captured graph
+// (scene 0's option graph).  The per-scene values live in the device tables
+// the caller filled for this iteration; the photon and query counts are read
+// from device memory, so the sort grids are sized for the buffers' capacity;
+// the graph is captured again when a buffer, a table, the scene set or a
+// launch shape changes.  nq = this pass's query count.
+bool batch_graph_pass(Batch& b, const BatchView& view, CameraPassArgs c, uint32_t count, uint64_t buckets,
+                      int bucket_bits, int uid_bits, uint32_t& nq) {
+    Engine& w = b.work;
+    hipStream_t st = w.stream;
+    (void)count;
+    c.path_counter = w.counters.p;
+    c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
+    std::vector<uintptr_t> key = {(uintptr_t)w.ph_pos.p, w.capacity, (uintptr_t)w.q_pos.p, w.q_capacity,
+                                  (uintptr_t)w.cells.p, (uintptr_t)buckets, (uintptr_t)bucket_bits,
+                                  (uintptr_t)uid_bits, (uintptr_t)w.path_info.p, (uintptr_t)w.contrib.p,
+                                  (uintptr_t)view.n, (uintptr_t)view.sc, (uintptr_t)view.pm, (uintptr_t)view.bs,
+                                  (uintptr_t)view.light_starts, (uintptr_t)view.guides, (uintptr_t)c.n_paths,
+                                  (uintptr_t)w.path_chunk, (uintptr_t)gather_blocks(w), (uintptr_t)camera_blocks(w),
+                                  (uintptr_t)c.exact_merge, (uintptr_t)w.query_order, (uintptr_t)w.group_queries,
+                                  (uintptr_t)w.camera_refill, (uintptr_t)c.camera_qmc, (uintptr_t)w.gather_tile,
+                                  (uintptr_t)w.gather_block};
+    if (!b.graph_exec || key != b.graph_key) {
+        if (b.graph_exec) HIP_OK(hipGraphExecDestroy(b.graph_exec));
+        b.graph_exec = nullptr;
+        MapBuildArgs m{};
+        m.in_pos = w.ph_pos.p;
+        m.in_dir = w.ph_dir.p;
+        m.in_tp = w.ph_tp.p;
+        m.in_uid = w.ph_uid.p;
+        m.count = w.capacity;
+        m.count_dev = w.counters.p + 1;
+        m.mask = (uint32_t)(buckets - 1);
+        m.bucket_bits = bucket_bits;
+        m.uid_bits = uid_bits;
+        m.keys_a = w.keys_a.p;
+        m.keys_b = w.keys_b.p;
+        m.vals_a = w.vals_a.p;
+        m.vals_b = w.vals_b.p;
+        m.hist = w.hist.p;
+        m.hist_sums = w.hist_sums.p;
+        m.out_pos = w.s_pos.p;
+        m.out_dir = w.s_dir.p;
+        m.out_tp = w.s_tp.p;
+        m.out_index = w.s_index.p;
+        m.cells = w.cells.p;
+        bool ok = capture_graph(st,
+                                [&]() {
+                                    build_photon_map_batch(m, view, st);
+                                    launch_fill_u32(w.counters.p, 0u, kPathCounters, st);  // after the map read [1]
+                                    launch_camera_trace(c, camera_blocks(w), st);
+                                    c.q_order = query_order(w, c, w.q_capacity, st);  // count from the device
+                                    launch_gather(c, gather_blocks(w), st);
+                                    launch_resolve(c, st);
+                                },
+                                b.graph_exec);
+        if (!ok) {
+            if (b.graph_exec) (void)hipGraphExecDestroy(b.graph_exec);
+            b.graph_exec = nullptr;
+            return false;
+        }
+        b.graph_key = key;
+        b.graph_captures++;
+    }
+    HIP_OK(hipEventRecord(w.ev[2], st));
+    HIP_OK(hipGraphLaunch(b.graph_exec, st));
+    HIP_OK(hipEventRecord(w.ev[5], st));
+    HIP_OK(hipMemcpyAsync(&nq, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (nq > w.q_capacity) {  // cannot happen: at most 8 queries per camera path
+        set_error("merge query buffer overflow in the batch graph");
+        return false;
+    }
+    return true;
+}
+
 // One iteration of every scene: light pass, photon map, camera trace,
 // gather and resolve, each a single launch over all scenes.  With `scratch`
 // the films and iteration counts stay untouched (autotuner runs).
@@ -2225,10 +2308,16 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     view.guides = b.d_guides.p;
     const Engine& e0 = *b.scenes[0];
     const bool counting = e0.count_events != 0;  // event counters of the first scene's option
+    // scene 0's option "graph": after the light pass (a direct launch whose photon count is checked),
+    // the photon map and the camera pass replay one captured graph; every per-scene value comes from
+    // the tables above, every count from device memory, and the query buffer holds the most queries
+    // a camera pass can make (8 per path)
+    const bool graphed = e0.graph && !counting;
     // camera trace and query sort on stream2 beside the light pass and the map (option overlap of the
-    // batch's launch options; not while counting events): the trace reads only the scene tables above
-    const bool overlapped = w.overlap && !counting;
-    uint32_t qcap = std::max(w.q_capacity, (uint32_t)std::min<uint64_t>(2 * camera + 1024, 1ull << 31));
+    // batch's launch options; not while counting events or graphed): the trace reads only the scene tables
+    const bool overlapped = w.overlap && !counting && !graphed;
+    uint32_t qcap = std::max(w.q_capacity,
+                             (uint32_t)std::min<uint64_t>((graphed ? 8 : 2) * camera + 1024, 1ull << 31));
     if (!ensure_query_buffers(w, qcap, (uint32_t)camera)) return false;
     CameraPassArgs c{};
     c.sc = scene_view(e0);
@@ -2302,93 +2391,101 @@ bool batch_iteration(Batch& b, bool scratch = false) {
             return false;
         }
     }
-    // one photon map over every scene
-    MapBuildArgs m{};
-    m.in_pos = w.ph_pos.p;
-    m.in_dir = w.ph_dir.p;
-    m.in_tp = w.ph_tp.p;
-    m.in_uid = w.ph_uid.p;
-    m.count = count;
-    m.mask = (uint32_t)(buckets - 1);
-    m.bucket_bits = bucket_bits;
-    m.uid_bits = uid_bits;
-    m.keys_a = w.keys_a.p;
-    m.keys_b = w.keys_b.p;
-    m.vals_a = w.vals_a.p;
-    m.vals_b = w.vals_b.p;
-    m.hist = w.hist.p;
-    m.hist_sums = w.hist_sums.p;
-    m.out_pos = w.s_pos.p;
-    m.out_dir = w.s_dir.p;
-    m.out_tp = w.s_tp.p;
-    m.out_index = w.s_index.p;
-    m.cells = w.cells.p;
-    HIP_OK(hipEventRecord(w.ev[2], st));
-    build_photon_map_batch(m, view, st);
-    HIP_OK(hipEventRecord(w.ev[3], st));
-    HIP_OK(hipGetLastError());
-    // camera pass: the overlapped trace's count, or (serial, or after an overflow) the trace here
     uint32_t nq = 0;
-    bool serial = !overlapped;
-    if (overlapped) {
-        HIP_OK(hipEventSynchronize(w.ev_cam));
-        nq = *w.h_nq;
-        if (nq > w.q_capacity) {  // trace again, serially, into larger buffers
-            HIP_OK(hipStreamSynchronize(w.stream2));
-            if (!ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
+    if (graphed) {
+        if (!batch_graph_pass(b, view, c, count, buckets, bucket_bits, uid_bits, nq)) return false;
+    } else {
+        // one photon map over every scene
+        MapBuildArgs m{};
+        m.in_pos = w.ph_pos.p;
+        m.in_dir = w.ph_dir.p;
+        m.in_tp = w.ph_tp.p;
+        m.in_uid = w.ph_uid.p;
+        m.count = count;
+        m.mask = (uint32_t)(buckets - 1);
+        m.bucket_bits = bucket_bits;
+        m.uid_bits = uid_bits;
+        m.keys_a = w.keys_a.p;
+        m.keys_b = w.keys_b.p;
+        m.vals_a = w.vals_a.p;
+        m.vals_b = w.vals_b.p;
+        m.hist = w.hist.p;
+        m.hist_sums = w.hist_sums.p;
+        m.out_pos = w.s_pos.p;
+        m.out_dir = w.s_dir.p;
+        m.out_tp = w.s_tp.p;
+        m.out_index = w.s_index.p;
+        m.cells = w.cells.p;
+        HIP_OK(hipEventRecord(w.ev[2], st));
+        build_photon_map_batch(m, view, st);
+        HIP_OK(hipEventRecord(w.ev[3], st));
+        HIP_OK(hipGetLastError());
+        // camera pass: the overlapped trace's count, or (serial, or after an overflow) the trace here
+        bool serial = !overlapped;
+        if (overlapped) {
+            HIP_OK(hipEventSynchronize(w.ev_cam));
+            nq = *w.h_nq;
+            if (nq > w.q_capacity) {  // trace again, serially, into larger buffers
+                HIP_OK(hipStreamSynchronize(w.stream2));
+                if (!ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
+                    set_error("merge query buffer overflow");
+                    return false;
+                }
+                c.path_counter = w.counters.p;
+                c.q_order = nullptr;
+                serial = true;
+            }
+        }
+        for (int attempt = 0; serial; ++attempt) {
+            c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
+            launch_fill_u32(w.counters.p, 0u, kPathCounters, st);
+            if (counting && attempt > 0)
+                launch_fill_u32(reinterpret_cast<uint32_t*>(w.stats_dev.p + 1), 0u, kCounterFields * 2, st);
+            HIP_OK(hipEventRecord(w.ev[4], st));
+            launch_camera_trace(c, camera_blocks(w), st);
+            HIP_OK(hipGetLastError());
+            HIP_OK(hipMemcpyAsync(&nq, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (nq <= w.q_capacity) break;
+            if (attempt == 1 ||
+                !ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
                 set_error("merge query buffer overflow");
                 return false;
             }
-            c.path_counter = w.counters.p;
-            c.q_order = nullptr;
-            serial = true;
         }
-    }
-    for (int attempt = 0; serial; ++attempt) {
-        c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
-        launch_fill_u32(w.counters.p, 0u, kPathCounters, st);
-        if (counting && attempt > 0)
-            launch_fill_u32(reinterpret_cast<uint32_t*>(w.stats_dev.p + 1), 0u, kCounterFields * 2, st);
-        HIP_OK(hipEventRecord(w.ev[4], st));
-        launch_camera_trace(c, camera_blocks(w), st);
+        if (serial) {
+            HIP_OK(hipEventRecord(w.ev[8], st));
+            if (overlapped) HIP_OK(hipEventRecord(w.ev_join, st));
+            c.q_order = query_order(w, c, nq, st);
+            if (overlapped) HIP_OK(hipEventRecord(w.ev_qdone, st));
+        } else {
+            HIP_OK(hipStreamWaitEvent(st, w.ev_qdone, 0));  // the trace and the query sort on stream2
+        }
+        HIP_OK(hipEventRecord(w.ev[6], st));
+        launch_gather(c, gather_blocks(w), st);
+        HIP_OK(hipEventRecord(w.ev[7], st));
+        launch_resolve(c, st);
+        HIP_OK(hipEventRecord(w.ev[5], st));
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(&nq, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
-        if (nq <= w.q_capacity) break;
-        if (attempt == 1 ||
-            !ensure_query_buffers(w, (uint32_t)std::min<double>(nq * 1.25 + 1024, 4.0e9), (uint32_t)camera)) {
-            set_error("merge query buffer overflow");
-            return false;
-        }
     }
-    if (serial) {
-        HIP_OK(hipEventRecord(w.ev[8], st));
-        if (overlapped) HIP_OK(hipEventRecord(w.ev_join, st));
-        c.q_order = query_order(w, c, nq, st);
-        if (overlapped) HIP_OK(hipEventRecord(w.ev_qdone, st));
-    } else {
-        HIP_OK(hipStreamWaitEvent(st, w.ev_qdone, 0));  // the trace and the query sort on stream2
-    }
-    HIP_OK(hipEventRecord(w.ev[6], st));
-    launch_gather(c, gather_blocks(w), st);
-    HIP_OK(hipEventRecord(w.ev[7], st));
-    launch_resolve(c, st);
-    HIP_OK(hipEventRecord(w.ev[5], st));
-    HIP_OK(hipGetLastError());
     if (any_train && !batch_guide_training(b, view, count, nq, targets, st)) return false;
     Counters cnt[2] = {};
     if (counting) HIP_OK(hipMemcpyAsync(cnt, w.stats_dev.p, sizeof cnt, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     float ms[8] = {0};
     HIP_OK(hipEventElapsedTime(&ms[0], w.ev[0], w.ev[1]));
-    HIP_OK(hipEventElapsedTime(&ms[1], w.ev[2], w.ev[3]));
-    HIP_OK(hipEventElapsedTime(&ms[2], w.ev[4], w.ev[5]));
-    HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[8]));
-    if (overlapped) HIP_OK(hipEventElapsedTime(&ms[6], w.ev_join, w.ev_qdone));
-    else HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
-    HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
-    HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
-    if (overlapped) ms[2] = ms[3] + ms[6] + ms[4] + ms[5];  // the camera pass's own spans (see run_iteration)
+    if (graphed) {  // map + camera pass are one replay: its time goes to ms_camera
+        HIP_OK(hipEventElapsedTime(&ms[2], w.ev[2], w.ev[5]));
+    } else {
+        HIP_OK(hipEventElapsedTime(&ms[1], w.ev[2], w.ev[3]));
+        HIP_OK(hipEventElapsedTime(&ms[2], w.ev[4], w.ev[5]));
+        HIP_OK(hipEventElapsedTime(&ms[3], w.ev[4], w.ev[8]));
+        if (overlapped) HIP_OK(hipEventElapsedTime(&ms[6], w.ev_join, w.ev_qdone));
+        else HIP_OK(hipEventElapsedTime(&ms[6], w.ev[8], w.ev[6]));
+        HIP_OK(hipEventElapsedTime(&ms[4], w.ev[6], w.ev[7]));
+        HIP_OK(hipEventElapsedTime(&ms[5], w.ev[7], w.ev[5]));
+        if (overlapped) ms[2] = ms[3] + ms[6] + ms[4] + ms[5];  // the camera pass's own spans (see run_iteration)
+    }
     std::fill(b.stats, b.stats + RC_STAT_COUNT, 0.0);
     b.stats[RC_STAT_LIGHT_PATHS] = (double)light;
     b.stats[RC_STAT_PHOTONS] = count;
@@ -2419,6 +2516,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         b.stats[RC_STAT_MS_GUIDE_STEP] = b.guide_step_ms;
         b.stats[RC_STAT_GUIDE_ROWS] = b.guide_rows;
     }
+    b.stats[RC_STAT_GRAPH_CAPTURES] = b.graph_captures;
     if (scratch) return true;
     for (Engine* e : b.scenes) e->iterations_done++;
     b.iterations++;
@@ -2435,6 +2533,13 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
                             [pb]() { return pb->scenes[0]->cell_scale; }, {0, 1});
     const int overlap = b.work.overlap;
     b.work.overlap = 0;  // the other knobs are timed with the camera trace serial
+    const int graph = b.scenes[0]->graph;
+    b.scenes[0]->graph = 0;  // per-kernel times need direct launches
+    struct Restore {
+        Engine* e;
+        int g;
+        ~Restore() { e->graph = g; }
+    } restore{b.scenes[0], graph};
     auto timed = [pb](double* t) {
         auto w0 = std::chrono::steady_clock::now();
         if (!batch_iteration(*pb, true)) return false;

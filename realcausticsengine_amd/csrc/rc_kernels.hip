@@ -1675,8 +1675,9 @@ __global__ void k_map_cells(const unsigned long long* keys, const uint32_t* n_de
     if (i == n - 1 || (uint32_t)(keys[i + 1] >> uid_bits) != b) cells[b].y = i + 1;
 }
 
-__global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, uint32_t n, BatchView b, int uid_bits,
-                                 unsigned long long* keys, uint32_t* vals) {
+__global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, const uint32_t* n_dev, uint32_t n,
+                                 BatchView b, int uid_bits, unsigned long long* keys, uint32_t* vals) {
+    n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 p = pos[i];
@@ -1709,8 +1710,8 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
     uint32_t n = a.count;
     if (n == 0) return;
     unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, n, b, a.uid_bits, a.keys_a,
-                       a.vals_a);
+    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, a.count_dev, n, b,
+                       a.uid_bits, a.keys_a, a.vals_a);
     sort_permute_cells(a, s);
 }
 

@@ -38,7 +38,8 @@ STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_test
               "light_node_visits", "light_tri_tests", "light_hits", "camera_node_visits", "camera_tri_tests",
               "camera_hits", "cell_visits", "sort_passes", "ms_camera_trace", "ms_gather", "ms_resolve",
               "ms_guide", "guide_visible", "guide_hit_fraction", "guide_loss", "ms_guide_step", "guide_rows",
-              "ms_query_sort", "photons_streamed", "mlt_visible", "mlt_mutations", "mlt_accepted"]
+              "ms_query_sort", "photons_streamed", "mlt_visible", "mlt_mutations", "mlt_accepted",
+              "graph_captures"]
 # Matrix-core FLOPs of one training step per row: forward 2*(192*64 + 2*64*64 + 64*1024), data
 # gradients of the three upper layers 2*(1024*64 + 2*64*64), weight gradients 2*(64*1024 + 2*64*64 + 192*64).
 GUIDE_STEP_FLOPS_PER_ROW = 2 * (192 * 64 + 2 * 64 * 64 + 64 * 1024) + 2 * (1024 * 64 + 2 * 64 * 64) + \

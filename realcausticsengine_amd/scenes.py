@@ -262,21 +262,22 @@ def displaced_sphere(subdiv: int, seed: int, amplitude: float = 0.1, waves: int 
     return verts.astype(np.float32), vn, tris
 
 
-def blob_scene(seed: int, samples: int = 1):
+def blob_scene(seed: int, samples: int = 1, res_scale: int = 1, photon_scale: int = 1):
     """One scene of the uneven batch: a seeded displaced glass blob over a
     diffuse plane.  The triangle count is drawn log-uniform in [1e3, 5e5] and
-    met by the icosphere level nearest to it in log scale (320 .. 327,680
-    triangles); IOR U(1.3, 1.7); resolution from {256, 512, 1024}^2; 1..8 x 2^20
-    light paths per iteration.  Returns (SceneArgs, light paths)."""
+    met by a displaced cube sphere of m x m quads per face (12 m^2 triangles,
+    the nearest such count); IOR U(1.3, 1.7); GGX roughness U(0, 0.3) (a rough
+    dielectric with option compat_glass = 0; the reference renders glass as
+    delta); resolution from {256, 512, 1024}^2 / res_scale; 1..8 x 2^20 /
+    photon_scale light paths per iteration.  Returns (SceneArgs, light paths)."""
     rng = np.random.default_rng(1000 + seed)
     want = float(np.exp(rng.uniform(np.log(1e3), np.log(5e5))))
-    levels = np.arange(2, 8)
-    subdiv = int(levels[np.argmin(np.abs(np.log(20.0 * 4.0 ** levels) - np.log(want)))])
+    m = min(204, max(10, int(round(np.sqrt(want / 12.0)))))  # 1,200 .. 499,392 triangles
     ior = float(rng.uniform(1.3, 1.7))
-    rough = float(rng.uniform(0.0, 0.3))  # 'G' glass is delta in the reference: recorded, not used
-    res = int(rng.choice([256, 512, 1024]))
-    photons = int(rng.integers(1, 9)) << 20
-    bv, bn, bt = displaced_sphere(subdiv, seed)
+    rough = float(rng.uniform(0.0, 0.3))
+    res = int(rng.choice([256, 512, 1024])) // res_scale
+    photons = (int(rng.integers(1, 9)) << 20) // photon_scale
+    bv, bn, bt = displaced_cube_sphere(m, seed)
     pv, pn, pt = plane(10.0, 10.0, 0.0)
     meshes = [MeshData(bv, bn, bt, matrix=translation(0.0, 0.0, 1.3), material=0), MeshData(pv, pn, pt, material=1)]
     mats = [MaterialData("G", ior=ior, roughness=rough, color=(1.0, 1.0, 1.0)),
@@ -286,9 +287,9 @@ def blob_scene(seed: int, samples: int = 1):
     return SceneArgs(meshes, lights, mats, cam, samples=samples, base_radius=0.01, radius_reduction=0.75), photons
 
 
-def uneven_batch(n: int = 64, samples: int = 1):
+def uneven_batch(n: int = 64, samples: int = 1, res_scale: int = 1, photon_scale: int = 1):
     """The uneven-scene batch: blob_scene(seed) for seeds 0 .. n-1."""
-    return [blob_scene(s, samples) for s in range(n)]
+    return [blob_scene(s, samples, res_scale, photon_scale) for s in range(n)]
 
 
 def displaced_cube_sphere(m: int, seed: int, amplitude: float = 0.1, waves: int = 8):
