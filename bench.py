@@ -135,15 +135,19 @@ def cpu_baseline(args, photons: int, stride: int) -> dict:
 
 def uneven_batch(a):
     """The uneven-scene batch (scenes.uneven_batch: seeded displaced glass blobs,
-    320 .. 327,680 triangles, 256^2 .. 1024^2, 1 .. 8 x 2^20 light paths each):
-    one batched iteration (one launch per pass for every scene) against the
-    same scenes rendered one after another with their own launches."""
+    1,200 .. 499,392 triangles log-uniform, GGX roughness U(0, 0.3) as the rough
+    dielectric (compat_glass = 0), 256^2 .. 1024^2, 1 .. 8 x 2^20 light paths
+    each): one batched iteration (one launch per pass for every scene) against
+    the same scenes rendered one after another with their own launches, and
+    the batched iteration with its photon map and camera pass replayed from a
+    captured graph."""
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Batch
 
     t = time.perf_counter()
     cases = scenes.uneven_batch(a.batch_scenes)
-    b = Batch([sc for sc, _ in cases], [dict(photons=p, seed=i, write_exr=0) for i, (_, p) in enumerate(cases)])
+    b = Batch([sc for sc, _ in cases], [dict(photons=p, seed=i, write_exr=0, compat_glass=0)
+                                        for i, (_, p) in enumerate(cases)])
     b.prepare()
     prep_s = time.perf_counter() - t
     print(f"[bench] uneven batch: {len(cases)} scenes ready in {prep_s:.1f} s", file=sys.stderr, flush=True)
@@ -164,6 +168,13 @@ def uneven_batch(a):
         for i in range(len(cases)):
             b.render_scene(i)
     sequential = (time.perf_counter() - t0) / a.batch_steps
+    b.set(0, "graph", 1)
+    b.render_iteration()  # capture
+    t0 = time.perf_counter()
+    for _ in range(a.batch_steps):
+        b.render_iteration()
+    graphed = (time.perf_counter() - t0) / a.batch_steps
+    captures = int(b.stats()["graph_captures"])
     b.release()
     light = sum(p for _, p in cases)
     out = {
@@ -173,6 +184,8 @@ def uneven_batch(a):
         "camera_paths_per_iteration": int(sum(sc.camera.width * sc.camera.height for sc, _ in cases)),
         "ms_per_iteration_batched": round(batched * 1e3, 2),
         "ms_per_iteration_one_scene_at_a_time": round(sequential * 1e3, 2),
+        "ms_per_iteration_batched_graph": round(graphed * 1e3, 2),
+        "graph_captures": captures,
         "speedup": round(sequential / batched, 3),
         "photon_paths_per_s_batched": round(light / batched, 1),
         "kernel_ms_batched": {k: round(st[k], 3) for k in ("ms_light", "ms_map", "ms_camera_trace", "ms_gather",
