@@ -80,6 +80,8 @@ struct Engine {
     long long photons = 0;
     int write_exr = 1, compat_rows = 1, compat_point = 1, compat_pdf = 1, max_path_length = 6;
     int compat_glass = 1;  // 0: glass of roughness >= 0.01 is a GGX dielectric (BK_RGLASS)
+    int light_emit = 0;    // 1: the light samples of every path in a coherent kernel before the light pass
+    DevBuf<float4> emitted;
     std::vector<rc_python_material> py_mats;  // as given to init; converted again at rc_prepare
     double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
@@ -972,6 +974,10 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
             a.mlt_chains = (uint32_t)std::max<long long>(1, std::min<long long>(e.mlt_chains, N / 64));
             a.mlt_counts = e.mlt_counts.p;
         }
+        if (e.light_emit && !e.mlt) {
+            if (!e.emitted.alloc(3 * (size_t)N)) return false;
+            a.emitted = e.emitted.p;
+        }
         HIP_OK(hipEventRecord(e.ev[0], e.stream));
         a.path_order = light_path_order(e, a, e.stream);  // inside the light pass's time
         if (e.light_order && !a.path_order && !e.mlt) return false;
@@ -1010,7 +1016,9 @@ struct MapGrid {
 // The photon-map grid for N light paths (uids below 8 N) and a radius,
 // over the scene's bounds; allocates nothing, sets no array of g.pm.
 bool map_grid_params(const Engine& e, long long N, float radius, float r2, MapGrid& g) {
-    g.uid_bits = log2_ceil((uint64_t)N) + 3;
+    // keys sort by (bucket, light path): a path's photons come from one lane in bounce order,
+    // so the stable sort keeps them in that order without the three length bits of the uid
+    g.uid_bits = log2_ceil((uint64_t)N);
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
     g.bucket_bits = log2_ceil(want);
     if (g.bucket_bits + g.uid_bits > 64) {
@@ -1327,7 +1335,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     if (!ensure_query_buffers(e, (uint32_t)std::min<uint64_t>(8ull * n_paths + 1024, 1ull << 31), n_paths)) return false;
     long long N = light_paths(e);
     // the grid of this iteration, as build_map
-    int uid_bits = log2_ceil((uint64_t)N) + 3;
+    int uid_bits = log2_ceil((uint64_t)N);  // (bucket, light path) keys, as map_grid_params
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
     int bucket_bits = log2_ceil(want);
     uint32_t buckets = 1u << bucket_bits;
@@ -1862,6 +1870,7 @@ void release(Engine*& e) {
     for (auto* d : {&e->qk_a, &e->qk_b, &e->lk_a, &e->lk_b}) d->release();
     for (auto* d : {&e->lv_a, &e->lv_b, &e->l_hist, &e->l_hist_sums}) d->release();
     e->mlt_state.release();
+    e->emitted.release();
     e->mlt_counts.release();
     for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
     e->cells.release();
@@ -1903,6 +1912,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.compat_point = from.compat_point;
     to.compat_pdf = from.compat_pdf;
     to.compat_glass = from.compat_glass;
+    to.light_emit = from.light_emit;
     to.max_path_length = from.max_path_length;
     to.photon_capacity = from.photon_capacity;
     to.cell_scale = from.cell_scale;
@@ -2005,6 +2015,7 @@ void release_batch(Batch*& b) {
     for (auto* d : {&w.ph_pos, &w.ph_dir, &w.s_pos, &w.s_dir, &w.q_pos, &w.q_dir, &w.q_sn, &w.q_gn, &w.q_tpn})
         d->release();
     for (auto* d : {&w.ph_tp, &w.s_tp}) d->release();
+    w.emitted.release();
     for (auto* d : {&w.ph_uid, &w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
     w.keys_a.release();
     w.keys_b.release();
@@ -2077,6 +2088,7 @@ bool batch_prepare(Batch& b) {
     b.work.gather_tile = b.scenes[0]->gather_tile;
     b.work.gather_block = b.scenes[0]->gather_block;
     b.work.overlap = b.scenes[0]->overlap;
+    b.work.light_emit = b.scenes[0]->light_emit;
     if (!b.work.counters.alloc(kPathCounters) || !b.work.stats_dev.alloc(2)) return false;
     size_t n = b.scenes.size();
     if (!b.d_sc.alloc(n) || !b.d_pm.alloc(n) || !b.d_bs.alloc(n) || !b.d_starts.alloc(2 * (n + 1))) return false;
@@ -2260,7 +2272,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         set_error("batch too large for one launch (light or camera paths >= 2^29)");
         return false;
     }
-    int uid_bits = log2_ceil(light) + 3;
+    int uid_bits = log2_ceil(light);  // (bucket, global light path) keys, as map_grid_params
     int bucket_bits = log2_ceil(buckets);
     if (uid_bits + bucket_bits > 64) {
         set_error("photon map key does not fit 64 bits");
@@ -2377,6 +2389,10 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.guide = plain_guide_view();
         a.batch = view;
         a.counters = counting ? w.stats_dev.p : nullptr;
+        if (w.light_emit) {
+            if (!w.emitted.alloc(3 * (size_t)light)) return false;
+            a.emitted = w.emitted.p;
+        }
         HIP_OK(hipEventRecord(w.ev[0], st));
         a.path_order = light_path_order(w, a, st);
         if (w.light_order && !a.path_order) return false;
@@ -2756,6 +2772,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "compat_rows") g->compat_rows = (int)v;
     else if (k == "compat_point_light") g->compat_point = (int)v;
     else if (k == "compat_glass") g->compat_glass = (int)v;
+    else if (k == "light_emit") g->light_emit = (int)v;
     else if (k == "compat_light_pdf") g->compat_pdf = (int)v;
     else if (k == "max_path_length") g->max_path_length = std::max(0, std::min(7, (int)v));
     else if (k == "photon_capacity") g->photon_capacity = v;
@@ -2836,6 +2853,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "mlt_mutations") return g->mlt_last[1];
     if (k == "mlt_accepted") return g->mlt_last[2];
     if (k == "compat_glass") return g->compat_glass;
+    if (k == "light_emit") return g->light_emit;
     if (k == "bvh_depth") return g->bvh_depth;
     if (k == "bvh_depth4") return g->bvh_depth4;
     if (k == "bvh_traversal_width") return uses_nodes4(*g) ? 4 : 2;

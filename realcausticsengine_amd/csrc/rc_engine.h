@@ -156,6 +156,9 @@ struct LightPassArgs {
     GuideView guide;     // learned emission proposal (all slots -1: plain)
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)
     const uint32_t* path_order;  // non-null: slot s traces path path_order[s] (paths sorted for coherence)
+    // non-null: every slot's emission (light sample and proposal record) is made by a coherent kernel
+    // before the persistent pass: 3 float4 per slot {o, tp.x}, {d, tp.y}, {tp.z, emits, 0, 0}
+    float4* emitted;
     // adaptive MLT light pass (non-null mlt_state selects it; single scene, plain proposal)
     uint4* mlt_state;     // kMltBlocks * mlt_chains explicit chain states, block-major
     uint32_t mlt_chains;  // Markov chains; chain c takes the path indices c, c + chains, ...

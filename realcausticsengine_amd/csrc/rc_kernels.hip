@@ -143,6 +143,64 @@ struct LightState {
     int scene;
 };
 
+// Generate_light_sample (Vertex_Merging.cpp:18-25) for the light-path slot
+// `slot`: the path, its scene and Philox counter, the emitted ray and the
+// proposal's training record.  Returns whether the light emits.
+template <bool kBatch>
+__device__ __forceinline__ bool emit_light_path(const LightPassArgs& a, uint32_t slot, LightState& st, PathRng& rng) {
+    const uint32_t path = a.path_order ? a.path_order[slot] : slot;
+    st.path = path;
+    st.local = path;
+    st.scene = 0;
+    if (kBatch) {
+        st.scene = batch_scene_of(a.batch.light_starts, a.batch.n, path);
+        const BatchScene& bs = a.batch.bs[st.scene];
+        st.local = path - bs.light_base;
+        rng.k0 = bs.k0;
+        rng.k1 = bs.k1;
+        rng.iter = bs.iteration;
+    }
+    rng.path = st.local;
+    st.pl = 0;
+    st.diffuse = 0;
+    st.last = EV_DIFFUSE;
+    st.dvm = 0.0f;
+    if (a.emitted) return false;  // the emission kernel made this slot's ray: the caller loads it
+    const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
+    const GuideView& gv = kBatch ? a.batch.guides[st.scene] : a.guide;
+    int bin;
+    F3 axis;
+    const bool emits = sample_light(sc, rng, gv, st.o, st.d, st.tp, bin, axis);
+    if (st.local < gv.train_paths) {
+        float rec[kGuideRowFloats];
+        guide_record(gv, st.o.x, st.o.y, st.o.z, axis.x, axis.y, axis.z, rec);
+        float4* dst = reinterpret_cast<float4*>(gv.train_x + (size_t)st.local * kGuideRowFloats);
+        dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
+        dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
+        gv.train_bin[st.local] = emits ? bin : -1;
+    }
+    return emits;
+}
+
+// The emission of every light-path slot in one coherent pass (option
+// light_emit): the persistent pass then loads its refills instead of
+// sampling the lights on the few lanes a refill serves.
+template <bool kBatch>
+__global__ __launch_bounds__(256) void k_light_emit(LightPassArgs a) {
+    LightPassArgs b = a;
+    b.emitted = nullptr;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.n_paths; slot += stride) {
+        LightState st;
+        PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
+        const bool emits = emit_light_path<kBatch>(b, slot, st, rng);
+        float4* e = a.emitted + 3 * (size_t)slot;
+        e[0] = make_float4(st.o.x, st.o.y, st.o.z, st.tp.x);
+        e[1] = make_float4(st.d.x, st.d.y, st.d.z, st.tp.y);
+        e[2] = make_float4(st.tp.z, emits ? 1.0f : 0.0f, 0.0f, 0.0f);
+    }
+}
+
 template <bool kCount, bool kBatch>
 __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     __shared__ int2 stack_lds[kLdsStack * kBlock];
@@ -165,37 +223,15 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         if (need) {
             if (slot < a.n_paths) {
                 if (kCount) cnt[C_PATHS]++;
-                const uint32_t path = a.path_order ? a.path_order[slot] : slot;
-                st.path = path;
-                st.local = path;
-                st.scene = 0;
-                if (kBatch) {
-                    st.scene = batch_scene_of(a.batch.light_starts, a.batch.n, path);
-                    const BatchScene& bs = a.batch.bs[st.scene];
-                    st.local = path - bs.light_base;
-                    rng.k0 = bs.k0;
-                    rng.k1 = bs.k1;
-                    rng.iter = bs.iteration;
+                active = emit_light_path<kBatch>(a, slot, st, rng);
+                if (a.emitted) {
+                    const float4* e = a.emitted + 3 * (size_t)slot;
+                    const float4 e0 = e[0], e1 = e[1], e2 = e[2];
+                    st.o = f3(e0.x, e0.y, e0.z);
+                    st.d = f3(e1.x, e1.y, e1.z);
+                    st.tp = f3(e0.w, e1.w, e2.x);
+                    active = e2.y != 0.0f;
                 }
-                rng.path = st.local;
-                const SceneView& sc = kBatch ? a.batch.sc[st.scene] : a.sc;
-                const GuideView& gv = kBatch ? a.batch.guides[st.scene] : a.guide;
-                // Generate_light_sample (Vertex_Merging.cpp:18-25)
-                int bin;
-                F3 axis;
-                active = sample_light(sc, rng, gv, st.o, st.d, st.tp, bin, axis);
-                if (st.local < gv.train_paths) {
-                    float rec[kGuideRowFloats];
-                    guide_record(gv, st.o.x, st.o.y, st.o.z, axis.x, axis.y, axis.z, rec);
-                    float4* dst = reinterpret_cast<float4*>(gv.train_x + (size_t)st.local * kGuideRowFloats);
-                    dst[0] = make_float4(rec[0], rec[1], rec[2], rec[3]);
-                    dst[1] = make_float4(rec[4], rec[5], rec[6], rec[7]);
-                    gv.train_bin[st.local] = active ? bin : -1;
-                }
-                st.pl = 0;
-                st.diffuse = 0;
-                st.last = EV_DIFFUSE;
-                st.dvm = 0.0f;
             } else {
                 exhausted = true;
             }
@@ -478,6 +514,8 @@ __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
     if (kCount) flush_counters(a.counters, cnt);
 }
 
+static unsigned emit_blocks(uint32_t n) { return std::max(1u, std::min((n + 255u) / 256u, 8192u)); }
+
 void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s) {
     if (a.mlt_state) {
         if (a.counters) hipLaunchKernelGGL((k_light_pass_mlt<true>), dim3(blocks), dim3(kBlock), 0, s, a);
@@ -485,9 +523,11 @@ void launch_light_pass(const LightPassArgs& a, int blocks, hipStream_t s) {
         return;
     }
     if (a.batch.n > 0) {
+        if (a.emitted) hipLaunchKernelGGL(k_light_emit<true>, dim3(emit_blocks(a.n_paths)), dim3(256), 0, s, a);
         if (a.counters) hipLaunchKernelGGL((k_light_pass<true, true>), dim3(blocks), dim3(kBlock), 0, s, a);
         else hipLaunchKernelGGL((k_light_pass<false, true>), dim3(blocks), dim3(kBlock), 0, s, a);
     } else {
+        if (a.emitted) hipLaunchKernelGGL(k_light_emit<false>, dim3(emit_blocks(a.n_paths)), dim3(256), 0, s, a);
         if (a.counters) hipLaunchKernelGGL((k_light_pass<true, false>), dim3(blocks), dim3(kBlock), 0, s, a);
         else hipLaunchKernelGGL((k_light_pass<false, false>), dim3(blocks), dim3(kBlock), 0, s, a);
     }
@@ -1564,7 +1604,7 @@ __global__ void k_map_keys(const float4* pos, const uint32_t* uid, const uint32_
     int cy = (int)floorf((p.y - oy) * inv_cell);
     int cz = (int)floorf((p.z - oz) * inv_cell);
     uint32_t bkt = cell_bucket(cx, cy, cz, mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)uid[i];
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)(uid[i] >> 3);
     vals[i] = i;
 }
 
@@ -1688,7 +1728,7 @@ __global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, const u
     int cy = (int)floorf((p.y - pm.origin[1]) * pm.inv_cell);
     int cz = (int)floorf((p.z - pm.origin[2]) * pm.inv_cell);
     uint32_t bkt = b.bs[sc].bucket_base + cell_bucket(cx, cy, cz, pm.mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)u;
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)(u >> 3);
     vals[i] = i;
 }
 

@@ -184,3 +184,34 @@ def test_light_path_order_in_batch_keeps_the_films():
     for a, b in zip(*out):
         assert a.sum() > 0
         assert np.array_equal(a, b)
+
+
+def test_light_emission_kernel_keeps_photons_and_films():
+    """Option light_emit (the light samples of every path made by a coherent
+    kernel before the persistent light pass): the same photons, bit for bit,
+    and the same film -- single scene and batch."""
+    from realcausticsengine_amd.engine import Batch
+
+    args = scenes.c0_scene(48, 48, subdiv=3, samples=2)
+    out = []
+    for emit in (0, 1):
+        eng = Engine(args, photons=65536, seed=5, write_exr=0, light_emit=emit)
+        ph, uid = eng.trace_photons(0)
+        eng.render_iteration()
+        eng.render_iteration()
+        out.append((ph, uid, eng.image(scaled=False)))
+        eng.release()
+    (p0, u0, f0), (p1, u1, f1) = out
+    o0, o1 = np.argsort(u0), np.argsort(u1)
+    assert np.array_equal(u0[o0], u1[o1]) and np.array_equal(p0[o0].view(np.uint32), p1[o1].view(np.uint32))
+    assert np.array_equal(f0, f1) and f0.sum() > 0
+    films = []
+    for emit in (0, 1):
+        cases = [scenes.c0_scene(32, 32, subdiv=2, samples=2), scenes.water_scene(32, 24, 16, samples=2)]
+        b = Batch(cases, [dict(photons=20000, seed=3, write_exr=0, light_emit=emit),
+                          dict(photons=30000, seed=4, write_exr=0, light_emit=emit)]).prepare()
+        b.render_iteration()
+        films.append([b.image(k, scaled=False) for k in range(2)])
+        b.release()
+    for a, c in zip(*films):
+        assert np.array_equal(a, c)
