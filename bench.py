@@ -202,8 +202,8 @@ def at_scale(a) -> int:
     meshes and water heightfields of 2M-10M triangles, four lights each, 2048^2,
     64 x 2^20 light paths per scene and iteration, one learned proposal per
     scene trained every iteration, trees built on the GPU).  The scenes run in
-    batches of --scale-group (a batch's light paths index 32-bit photon ids:
-    at most 2^29 per launch), one batch resident at a time; each batch's launch
+    batches of --scale-group (default 16: all scenes in one launch per pass; a
+    launch holds at most 2^31 light paths), one batch resident at a time; each batch's launch
     shapes are autotuned (untimed) unless --launch fixes them; prints one JSON
     line."""
     from realcausticsengine_amd import scenes
@@ -500,7 +500,7 @@ def main() -> int:
                     help="only time this many training steps of the learned proposal (65,536 rows each)")
     ap.add_argument("--at-scale", type=int, default=0,
                     help="only run the at-scale batched + learned workload over this many scenes (16 = the full set)")
-    ap.add_argument("--scale-group", type=int, default=4)
+    ap.add_argument("--scale-group", type=int, default=16)
     ap.add_argument("--scale-steps", type=int, default=3)
     ap.add_argument("--scale-res", type=int, default=2048)
     ap.add_argument("--scale-photons", type=int, default=64 << 20)

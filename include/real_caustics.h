@@ -232,7 +232,9 @@ int rc_bvh_shape(const float *tris, long long n, int requested_width, int *out4)
  * then triangle order) or -1; t = distance. */
 int rc_first_hit(const float *rays, long long n, float tmin, int *tri, float *t);
 /* Light pass of one iteration; photons in path order as 10 floats
- * {pos.xyz, dir.xyz, throughput.rgb, dVM} and uid = path*8 + path_length.
+ * {pos.xyz, dir.xyz, throughput.rgb, dVM} and uid = path*8 + path_length
+ * (uid non-null needs fewer than 2^29 light paths; the engine's own photon
+ * ids are the light path indices, up to 2^31 paths per launch).
  * Returns the photon count (records beyond cap are not written). */
 long long rc_trace_photons(int iteration, float *photons, unsigned int *uid, long long cap);
 /* Radius sets through the engine's photon map: for each query, the input

@@ -82,6 +82,9 @@ struct Engine {
     int compat_glass = 1;  // 0: glass of roughness >= 0.01 is a GGX dielectric (BK_RGLASS)
     int light_emit = 0;    // 1: the light samples of every path in a coherent kernel before the light pass
     DevBuf<float4> emitted;
+    DevBuf<uint8_t> ph_len;   // path length of each photon (written only when read_lengths)
+    bool read_lengths = false;
+    long long map_ids = 0;    // > 0: photon ids of the next map are indices below this (caller photon lists)
     std::vector<rc_python_material> py_mats;  // as given to init; converted again at rc_prepare
     double photon_capacity = 0, cell_scale = 0.67;
     int persistent_blocks = 0;
@@ -931,8 +934,8 @@ const uint32_t* light_path_order(Engine& e, const LightPassArgs& a, hipStream_t 
 // and before the host waits for its photon count.
 bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::function<bool()>& after_launch = nullptr) {
     long long N = light_paths(e);
-    if (N <= 0 || N > (1ll << 28)) {
-        set_error("light path count out of range (1 .. 2^28)");
+    if (N <= 0 || N > (1ll << 31)) {
+        set_error("light path count out of range (1 .. 2^31)");
         return false;
     }
     uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
@@ -950,6 +953,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
+        if (e.read_lengths && !e.ph_len.alloc(e.capacity)) return false;
         LightPassArgs a{};
         a.sc = scene_view(e);
         a.k0 = (uint32_t)e.seed;
@@ -966,6 +970,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
         a.out_dir = e.ph_dir.p;
         a.out_tp = e.ph_tp.p;
         a.out_uid = e.ph_uid.p;
+        a.out_len = e.read_lengths ? e.ph_len.p : nullptr;
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
         a.guide = guide_light_view(e, iteration);
         if (e.mlt) {
@@ -1013,12 +1018,13 @@ struct MapGrid {
     PhotonMapView pm{};
 };
 
-// The photon-map grid for N light paths (uids below 8 N) and a radius,
-// over the scene's bounds; allocates nothing, sets no array of g.pm.
-bool map_grid_params(const Engine& e, long long N, float radius, float r2, MapGrid& g) {
-    // keys sort by (bucket, light path): a path's photons come from one lane in bounce order,
-    // so the stable sort keeps them in that order without the three length bits of the uid
-    g.uid_bits = log2_ceil((uint64_t)N);
+// The photon-map grid for N light paths and a radius, over the scene's
+// bounds, for photon ids below `ids` (the light path of each photon, or the
+// index of a caller's photon); allocates nothing, sets no array of g.pm.
+bool map_grid_params(const Engine& e, long long N, long long ids, float radius, float r2, MapGrid& g) {
+    // keys sort by (bucket, photon id): a path's photons come from one lane in bounce order,
+    // so the stable sort keeps them in that order
+    g.uid_bits = log2_ceil((uint64_t)std::max<long long>(1, ids));
     uint64_t want = std::max<uint64_t>(4096, std::min<uint64_t>(1ull << 24, (uint64_t)N / 2));
     g.bucket_bits = log2_ceil(want);
     if (g.bucket_bits + g.uid_bits > 64) {
@@ -1041,7 +1047,7 @@ bool map_grid_params(const Engine& e, long long N, float radius, float r2, MapGr
 }
 
 bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
-    if (!map_grid_params(e, light_paths(e), radius, r2, g)) return false;
+    if (!map_grid_params(e, light_paths(e), e.map_ids > 0 ? e.map_ids : light_paths(e), radius, r2, g)) return false;
     if (!e.cells.alloc(g.buckets)) return false;
     PhotonMapView& pm = g.pm;
     pm.pos = e.s_pos.p;
@@ -1871,6 +1877,7 @@ void release(Engine*& e) {
     for (auto* d : {&e->lv_a, &e->lv_b, &e->l_hist, &e->l_hist_sums}) d->release();
     e->mlt_state.release();
     e->emitted.release();
+    e->ph_len.release();
     e->mlt_counts.release();
     for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
     e->cells.release();
@@ -2267,9 +2274,9 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         if (!b.scratch.alloc(3 * camera)) return false;
         for (size_t i = 0; i < n; ++i) bs[i].image = b.scratch.p + 3 * (size_t)bs[i].camera_base;
     }
-    // uids (light) and query slots (camera) are path * 8 + length in 32 bits
-    if (light >= (1ull << 29) || camera >= (1ull << 29) || buckets > (1ull << 31)) {
-        set_error("batch too large for one launch (light or camera paths >= 2^29)");
+    // photon ids are global light path indices (32 bits); query slots are camera path * 8 + length
+    if (light > (1ull << 31) || camera >= (1ull << 29) || buckets > (1ull << 31)) {
+        set_error("batch too large for one launch (light paths > 2^31 or camera paths >= 2^29)");
         return false;
     }
     int uid_bits = log2_ceil(light);  // (bucket, global light path) keys, as map_grid_params
@@ -3009,20 +3016,32 @@ __attribute__((visibility("default"))) int rc_first_hit(const float* rays, long 
 __attribute__((visibility("default"))) long long rc_trace_photons(int iteration, float* photons, unsigned int* uid,
                                                                   long long cap) {
     if (!g || !prepare(*g)) return -1;
+    if (uid && light_paths(*g) > (1ll << 29)) {
+        set_error("uid = path * 8 + length needs fewer than 2^29 light paths");
+        return -1;
+    }
     uint32_t count = 0;
-    if (!run_light_pass(*g, iteration, count)) return -1;
+    g->read_lengths = true;
+    bool traced = run_light_pass(*g, iteration, count);
+    g->read_lengths = false;
+    if (!traced) return -1;
     std::vector<float4> pos(count), dir(count);
     std::vector<float2> tp(count);
-    std::vector<uint32_t> u(count);
+    std::vector<uint32_t> path(count);
+    std::vector<uint8_t> len(count);
     if (count) {
         if (hipMemcpy(pos.data(), g->ph_pos.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(dir.data(), g->ph_dir.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
             hipMemcpy(tp.data(), g->ph_tp.p, count * 8, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(u.data(), g->ph_uid.p, count * 4, hipMemcpyDeviceToHost) != hipSuccess) {
+            hipMemcpy(path.data(), g->ph_uid.p, count * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(len.data(), g->ph_len.p, count, hipMemcpyDeviceToHost) != hipSuccess) {
             set_error("photon readback failed");
             return -1;
         }
     }
+    g->ph_len.release();
+    std::vector<uint32_t> u(count);
+    for (uint32_t i = 0; i < count; ++i) u[i] = path[i] * 8u + len[i];
     // path order (the order of the reference's sequential light loop)
     std::vector<uint32_t> ord(count);
     for (uint32_t i = 0; i < count; ++i) ord[i] = i;
@@ -3078,13 +3097,15 @@ __attribute__((visibility("default"))) long long rc_gather_sets(const float* pos
     for (long long i = 0; i < count; ++i)
         for (int k = 0; k < 3; ++k) ph[10 * i + k] = positions[3 * i + k];
     if (!load_photons(ph.data(), count)) return -1;
-    // a photon map whose uid bits cover the input indices
+    // a photon map whose id bits cover the input indices
     long long saved = e.photons;
     e.photons = std::max<long long>(1, (count + 7) / 8 + 1);
+    e.map_ids = count;
     float radius = sqrtf(r2);
     MapInfo mi;
     bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
     e.photons = saved;
+    e.map_ids = 0;
     if (!ok) return -1;
     mi.pm.r2 = r2;
     DevBuf<float> dq;
@@ -3148,7 +3169,7 @@ __attribute__((visibility("default"))) long long rc_gather_sets_noalloc(const fl
     Engine& e = *g;
     MapGrid gr;
     // the grid rc_gather_sets builds for the same input (uid bits covering the indices)
-    if (!map_grid_params(e, std::max<long long>(1, (count + 7) / 8 + 1), sqrtf(r2), r2, gr)) return -1;
+    if (!map_grid_params(e, std::max<long long>(1, (count + 7) / 8 + 1), count, sqrtf(r2), r2, gr)) return -1;
     NoAllocGather a{};
     a.positions = positions;
     a.count = (uint32_t)count;
@@ -3176,11 +3197,10 @@ __attribute__((visibility("default"))) long long rc_merge_sets(int iteration, co
     if (!load_photons(photons, count)) return -1;
     float radius, r2, norm;
     iteration_params(e, iteration, radius, r2, norm);
-    long long saved = e.photons;
-    e.photons = std::max<long long>(light_paths(e), (count + 7) / 8 + 1);
+    e.map_ids = count;  // photon ids are the input indices
     MapInfo mi;
     bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
-    e.photons = saved;
+    e.map_ids = 0;
     if (!ok) return -1;
     const size_t slots = (size_t)e.W * e.H * 8;
     DevBuf<unsigned long long> sets;
@@ -3230,13 +3250,10 @@ __attribute__((visibility("default"))) int rc_camera_pass_photons(int iteration,
     if (!load_photons(photons, count)) return -1;
     float radius, r2, norm;
     iteration_params(e, iteration, radius, r2, norm);
-    // uids are the input indices; size the key for them
-    long long saved = e.photons;
-    long long n_paths = light_paths(e);
-    e.photons = std::max<long long>(n_paths, (count + 7) / 8 + 1);
+    e.map_ids = count;  // photon ids are the input indices
     MapInfo mi;
     bool ok = build_map(e, (uint32_t)count, radius, r2, mi);
-    e.photons = saved;
+    e.map_ids = 0;
     if (!ok) return -1;
     size_t n = (size_t)e.W * e.H * 3;
     DevBuf<float> img;

@@ -151,7 +151,8 @@ struct LightPassArgs {
     float4* out_pos;
     float4* out_dir;
     float2* out_tp;
-    uint32_t* out_uid;
+    uint32_t* out_uid;   // light path index of each photon (the map's sort key below the bucket)
+    uint8_t* out_len;    // non-null: path length (vertices) of each photon, for readback
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)

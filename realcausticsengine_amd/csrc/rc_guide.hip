@@ -348,7 +348,7 @@ __global__ void k_guide_targets(const float4* __restrict__ pos, const float4* __
                                 float ox, float oy, float oz, float inv_cell, uint32_t mask,
                                 const uint8_t* __restrict__ vis, uint32_t train_paths, float* __restrict__ w) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t path = uid[i] >> 3;
+        uint32_t path = uid[i];
         if (path >= train_paths) continue;
         if (!vis[bucket_of(pos[i], ox, oy, oz, inv_cell, mask)]) continue;
         float2 gb = tp_gb[i];
@@ -372,7 +372,7 @@ __global__ void k_guide_targets_batch(const float4* __restrict__ pos, const floa
                                       BatchView b, const uint8_t* __restrict__ vis,
                                       const GuideTargets* __restrict__ t) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t path = uid[i] >> 3;
+        uint32_t path = uid[i];
         int sc = batch_scene_of(b.light_starts, b.n, path);
         const BatchScene& bs = b.bs[sc];
         uint32_t local = path - bs.light_base;

@@ -76,7 +76,8 @@ struct PhotonRing {
     float4 pos[kPhotonRing];
     float4 dir[kPhotonRing];
     float2 tp[kPhotonRing];
-    uint32_t uid[kPhotonRing];
+    uint32_t path[kPhotonRing];
+    uint8_t len[kPhotonRing];
 };
 struct PhotonQueue {
     PhotonRing* r;
@@ -93,7 +94,8 @@ struct PhotonQueue {
                 a.out_pos[at] = r->pos[e];
                 a.out_dir[at] = r->dir[e];
                 a.out_tp[at] = r->tp[e];
-                a.out_uid[at] = r->uid[e];
+                a.out_uid[at] = r->path[e];
+                if (a.out_len) a.out_len[at] = r->len[e];
             }
         }
         head = (head + n) & (kPhotonRing - 1u);
@@ -101,7 +103,7 @@ struct PhotonQueue {
         __builtin_amdgcn_wave_barrier();
     }
     __device__ __forceinline__ void push(const LightPassArgs& a, bool store, float4 pos, float4 dir, float2 tp,
-                                         uint32_t uid) {
+                                         uint32_t path, int len) {
         const uint64_t m = __ballot(store);
         if (m == 0) return;
         if (store) {
@@ -109,7 +111,8 @@ struct PhotonQueue {
             r->pos[e] = pos;
             r->dir[e] = dir;
             r->tp[e] = tp;
-            r->uid[e] = uid;
+            r->path[e] = path;
+            r->len[e] = (uint8_t)len;
         }
         count += (uint32_t)__builtin_popcountll(m);
         __builtin_amdgcn_wave_barrier();
@@ -265,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
         if (kCount) cnt[C_STORED] += store;
         photons.push(a, store, make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm),
                      make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x), make_float2(st.tp.y, st.tp.z),
-                     st.path * 8u + (uint32_t)st.pl);
+                     st.path, st.pl);
         if (alive) {
             if (st.pl > sc.max_path_length || st.diffuse > 2) {
                 alive = false;
@@ -472,7 +475,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass_mlt(LightPassArgs a) {
         if (store) visible = true;
         photons.push(a, store, make_float4(sf.pos.x, sf.pos.y, sf.pos.z, st.dvm),
                      make_float4(-st.d.x, -st.d.y, -st.d.z, st.tp.x), make_float2(st.tp.y, st.tp.z),
-                     st.path * 8u + (uint32_t)st.pl);
+                     st.path, st.pl);
         if (alive) {
             if (st.pl > sc.max_path_length || st.diffuse > 2) {
                 alive = false;
@@ -1604,7 +1607,7 @@ __global__ void k_map_keys(const float4* pos, const uint32_t* uid, const uint32_
     int cy = (int)floorf((p.y - oy) * inv_cell);
     int cz = (int)floorf((p.z - oz) * inv_cell);
     uint32_t bkt = cell_bucket(cx, cy, cz, mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)(uid[i] >> 3);
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)uid[i];
     vals[i] = i;
 }
 
@@ -1722,13 +1725,13 @@ __global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, const u
     if (i >= n) return;
     float4 p = pos[i];
     uint32_t u = uid[i];
-    int sc = batch_scene_of(b.light_starts, b.n, u >> 3);
+    int sc = batch_scene_of(b.light_starts, b.n, u);
     const PhotonMapView& pm = b.pm[sc];
     int cx = (int)floorf((p.x - pm.origin[0]) * pm.inv_cell);
     int cy = (int)floorf((p.y - pm.origin[1]) * pm.inv_cell);
     int cz = (int)floorf((p.z - pm.origin[2]) * pm.inv_cell);
     uint32_t bkt = b.bs[sc].bucket_base + cell_bucket(cx, cy, cz, pm.mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)(u >> 3);
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)u;
     vals[i] = i;
 }
 

@@ -2,9 +2,10 @@
 
 Each scene of a batch keeps its own Philox key, iteration index, photon-map
 grid and film, its photons sort into its own bucket range in the same
-(bucket, path, length) order as alone, and every pixel receives one addition
-per iteration: the batched film equals the film of the scene rendered alone,
-bit for bit."""
+(bucket, light path) order as alone (a path's photons in bounce order), and
+every pixel receives one addition per iteration: the batched film equals the
+film of the scene rendered alone, bit for bit.  A launch holds up to 2^31
+light paths (photon ids are 32-bit light path indices)."""
 import numpy as np
 import pytest
 
@@ -44,6 +45,23 @@ def test_batch_matches_scenes_rendered_alone():
     assert st["light_paths"] == sum(o["photons"] for _, o in cases)
     assert st["camera_paths"] == sum(a.camera.width * a.camera.height for a, _ in cases)
     for k, (a, _) in enumerate(cases):
+        img = b.image(k, scaled=False)
+        print(f"scene {k}: sum {img.sum():.6g} alone {alone[k].sum():.6g}", flush=True)
+        assert img.sum() > 0
+        assert np.array_equal(img, alone[k]), k
+    b.release()
+
+
+def test_batch_beyond_2_29_light_paths_matches_scenes_alone():
+    # 2^28 + k light paths per scene: the second scene's photon ids (global light path indices) pass 2^29
+    cases = [(scenes.c0_scene(16, 16, subdiv=2, samples=1), dict(photons=(1 << 28) + 12345, seed=21)),
+             (scenes.c0_scene(24, 16, subdiv=3, samples=1), dict(photons=(1 << 28) + 777, seed=22))]
+    alone = [_alone(a, o, 1) for a, o in cases]
+    b = Batch([a for a, _ in cases], [dict(o, write_exr=0) for _, o in cases]).prepare()
+    b.render_iteration()
+    st = b.stats()
+    assert st["light_paths"] == sum(o["photons"] for _, o in cases) > (1 << 29)
+    for k in range(len(cases)):
         img = b.image(k, scaled=False)
         print(f"scene {k}: sum {img.sum():.6g} alone {alone[k].sum():.6g}", flush=True)
         assert img.sum() > 0
