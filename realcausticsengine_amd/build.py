@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libRealCaustics.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["rc_kernels.hip", "rc_guide.hip", "rc_bvh.hip", "rc_engine.cpp", "rc_guide_host.cpp"]
+SOURCES = ["rc_kernels.hip", "rc_guide.hip", "rc_bvh.hip", "rc_ingest.hip", "rc_engine.cpp", "rc_guide_host.cpp"]
 HEADERS = ["rc_device.h", "rc_engine.h", "rc_guide.h", "rc_guide_model.h", "rc_host.h", "rc_vec.h"]
 
 COMMON = [

@@ -107,6 +107,9 @@ struct Engine {
     int root4 = 0, n_nodes4 = 0;
     int bvh_width = 4;  // 4: traverse the 4-wide BVH, 2: the binary one
     int bvh_build = 0;  // 0: binned SAH on the host, 1: linear BVH on the device (binary nodes)
+    int device_ingest = 1;  // with bvh_build 1: init converts the meshes on the GPU (d_tri, d_shade, d_tri_id, bbox)
+    bool ingested_on_device = false;
+    size_t n_tris = 0;      // triangles of the scene (either ingest)
     unsigned long long* merge_sets = nullptr;  // rc_merge_sets: per-slot set fingerprints (counting launches)
     int bvh_depth = 0;   // internal ancestors of the deepest leaf of the binary tree
     int bvh_depth4 = 0;  // depth of the deepest internal 4-wide node (root 0)
@@ -706,16 +709,22 @@ bool prepare(Engine& e) {
     }
     if (has_error()) return false;
     if (!ensure_device(e)) return false;
-    size_t ntri = e.tris.size() / 4;
+    const bool dev = e.ingested_on_device;
+    size_t ntri = dev ? e.n_tris : e.tris.size() / 4;
+    e.n_tris = ntri;
     if (ntri == 0) {
         set_error("scene has no triangles");
         return false;
     }
-    // scene bounds (union of triangle boxes, as the root of BVH_world.cpp)
+    if (dev && e.bvh_build != 1) {
+        set_error("the meshes were ingested on the GPU (option device_ingest); set bvh_build = 1 or device_ingest = 0 before init");
+        return false;
+    }
+    // scene bounds (union of triangle boxes, as the root of BVH_world.cpp); the device ingest made them
     Builder bld;
-    bld.items.resize(ntri);
-    Box sb = Builder::empty();
-    for (size_t t = 0; t < ntri; ++t) {
+    if (!dev) bld.items.resize(ntri);
+    Box sb = dev ? e.bbox : Builder::empty();
+    for (size_t t = 0; t < (dev ? 0 : ntri); ++t) {
         BuildItem& it = bld.items[t];
         it.id = (uint32_t)t;
         for (int k = 0; k < 3; ++k) {
@@ -739,8 +748,10 @@ bool prepare(Engine& e) {
     int root = 0;
     auto t_build = std::chrono::steady_clock::now();
     if (gpu_tree) {
-        bld.order.resize(ntri);  // triangles go up in input order; the GPU build reorders them
-        for (size_t t = 0; t < ntri; ++t) bld.order[t] = (uint32_t)t;
+        if (!dev) {
+            bld.order.resize(ntri);  // triangles go up in input order; the GPU build reorders them
+            for (size_t t = 0; t < ntri; ++t) bld.order[t] = (uint32_t)t;
+        }
     } else {
         bld.order.reserve(ntri);
         root = bld.build(0, (int)ntri, 0, rootb);
@@ -783,9 +794,9 @@ bool prepare(Engine& e) {
         e.bvh_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_build).count();
     }
     e.n_nodes4 = (int)(nodes4.size() / 8);
-    std::vector<float4> tri(3 * ntri), shade(3 * ntri);
-    std::vector<int> tri_id(ntri);
-    for (size_t i = 0; i < ntri; ++i) {
+    std::vector<float4> tri(dev ? 0 : 3 * ntri), shade(dev ? 0 : 3 * ntri);
+    std::vector<int> tri_id(dev ? 0 : ntri);
+    for (size_t i = 0; i < (dev ? 0 : ntri); ++i) {
         uint32_t t = bld.order[i];
         const uint32_t* T = &e.tris[4 * (size_t)t];
         F3 p0 = f3(e.wv[3 * T[0]], e.wv[3 * T[0] + 1], e.wv[3 * T[0] + 2]);
@@ -821,7 +832,7 @@ bool prepare(Engine& e) {
     F3 mn = f3(sb.lo[0], sb.lo[1], sb.lo[2]), mx = f3(sb.hi[0], sb.hi[1], sb.hi[2]);
     e.scene_radius = sqrtf(len2(mn - (mn + mx) * 0.5f));
     e.base_radius *= e.scene_radius;
-    if (!upload(e.d_tri, tri) || !upload(e.d_shade, shade) || !upload(e.d_tri_id, tri_id) ||
+    if ((!dev && (!upload(e.d_tri, tri) || !upload(e.d_shade, shade) || !upload(e.d_tri_id, tri_id))) ||
         !upload(e.d_nodes, bld.nodes) || !upload(e.d_nodes4, nodes4) || !reconvert_materials(e) ||
         !upload(e.d_mats, e.mats) ||
         !upload(e.d_lights, e.lights) ||
@@ -1566,7 +1577,7 @@ bool render_iteration(Engine& e, int iteration) {
     e.stats[RC_STAT_MS_TOTAL] = ms_since(t0);
     e.stats[RC_STAT_RADIUS] = radius;
     e.stats[RC_STAT_NODES] = e.n_nodes;
-    e.stats[RC_STAT_TRIS] = (double)(e.tris.size() / 4);
+    e.stats[RC_STAT_TRIS] = (double)e.n_tris;
     e.stats[RC_STAT_SORT_PASSES] = e.sort_passes;
     e.stats[RC_STAT_GRAPH_CAPTURES] = e.graph_captures;
     if (e.count_events) {
@@ -1935,6 +1946,7 @@ void copy_options(const Engine& from, Engine& to) {
     to.graph = from.graph;
     to.bvh_width = from.bvh_width;
     to.bvh_build = from.bvh_build;
+    to.device_ingest = from.device_ingest;
     to.query_order = from.query_order;
     to.light_order = from.light_order;
     to.camera_qmc = from.camera_qmc;
@@ -2578,6 +2590,159 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
     return ok;
 }
 
+
+// Blender DNA offsets of the Mesh struct's mpoly, mloop and mvert pointers.
+constexpr size_t kMeshMpoly = 200, kMeshMloop = 208, kMeshMvert = 256;
+
+// Scene ingest on the GPU (option device_ingest with bvh_build = 1): the
+// host checks each polygon (three loops at least, a used material slot that
+// maps into the material array) while it finds how many loops and material
+// slots the polygons reach, copies the mesh's raw MVert, MPoly and MLoop
+// bytes and its slot map to the device, and the ingest kernels make d_tri,
+// d_shade, d_tri_id and the bounds there.  Non-finite coordinates and loop
+// vertex indices beyond the mesh come back as flags.  Sets the error and
+// returns false on a rejected scene.
+bool ingest_on_device(Engine& e, const long long* meshes_pointers, unsigned n_meshes, const rc_matrix4x4* matrices,
+                      const unsigned* n_verts, const unsigned* n_tris, int* const* materials_indices,
+                      unsigned n_materials) {
+    if (!ensure_device(e)) return false;
+    struct Piece {
+        const void* src;
+        size_t bytes, at;
+    };
+    std::vector<Piece> pieces;
+    std::vector<std::vector<int>> maps(n_meshes);
+    std::vector<IngestMesh> meshes(n_meshes);
+    size_t total = 0;
+    auto place = [&](const void* src, size_t bytes) {
+        size_t at = total;
+        pieces.push_back(Piece{src, bytes, at});
+        total += (bytes + 255) / 256 * 256;
+        return at;
+    };
+    std::vector<size_t> at_vert(n_meshes), at_poly(n_meshes), at_loop(n_meshes), at_map(n_meshes);
+    uint64_t vbase = 0, tbase = 0;
+    for (unsigned i = 0; i < n_meshes; ++i) {
+        const unsigned char* mb = reinterpret_cast<const unsigned char*>((intptr_t)meshes_pointers[i]);
+        const unsigned nv = n_verts[i], nt = n_tris[i];
+        e.mesh_tri_offset.push_back((uint32_t)tbase);
+        if (!mb) {
+            set_error("null Blender mesh pointer");
+            return false;
+        }
+        const unsigned char *mpoly, *mloop, *mvert;
+        memcpy(&mpoly, mb + kMeshMpoly, sizeof(void*));
+        memcpy(&mloop, mb + kMeshMloop, sizeof(void*));
+        memcpy(&mvert, mb + kMeshMvert, sizeof(void*));
+        if ((nt > 0 && (!mpoly || !mloop || !mvert)) || (nv > 0 && !mvert)) {
+            set_error("Blender mesh without mpoly/mloop/mvert");
+            return false;
+        }
+        if (!materials_indices[i]) {
+            set_error("null materials_indices entry");
+            return false;
+        }
+        // polygons: the loops they reach and the material slots they use
+        int64_t loops = 0;
+        std::vector<uint8_t> used(1 << 15, 0);
+        int max_slot = 0;
+        for (unsigned t = 0; t < nt; ++t) {
+            int32_t loopstart, totloop;
+            int16_t mat_nr;
+            memcpy(&loopstart, mpoly + 12 * (size_t)t, 4);
+            memcpy(&totloop, mpoly + 12 * (size_t)t + 4, 4);
+            memcpy(&mat_nr, mpoly + 12 * (size_t)t + 8, 2);
+            if (totloop < 3 || loopstart < 0) {
+                set_error("polygon with fewer than 3 loops");
+                return false;
+            }
+            loops = std::max<int64_t>(loops, (int64_t)loopstart + 3);
+            const int slot = mat_nr < 0 ? 0 : mat_nr;
+            used[slot] = 1;
+            max_slot = std::max(max_slot, slot);
+        }
+        std::vector<int>& map = maps[i];
+        map.assign(max_slot + 1, 0);
+        for (int k = 0; k <= max_slot; ++k) {
+            if (!used[k]) continue;
+            const int mat = materials_indices[i][k];
+            if (mat < 0 || (unsigned)mat >= n_materials) {
+                set_error("material index out of range");
+                return false;
+            }
+            map[k] = mat;
+        }
+        at_vert[i] = place(mvert, 20 * (size_t)nv);
+        at_poly[i] = place(mpoly, 12 * (size_t)nt);
+        at_loop[i] = place(mloop, 8 * (size_t)loops);
+        at_map[i] = place(map.data(), 4 * map.size());
+        IngestMesh& m = meshes[i];
+        m.nv = nv;
+        m.nt = nt;
+        m.vbase = (uint32_t)vbase;
+        m.tbase = (uint32_t)tbase;
+        const float* M = matrices[i].t;
+        for (int k = 0; k < 12; ++k) m.M[k] = M[k];
+        float inv[3][3];
+        inverse3x3_of4(M, inv);
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) m.inv[3 * r + c] = inv[r][c];
+        vbase += nv;
+        tbase += nt;
+        if (vbase >= (1ull << 32) || tbase >= (1ull << 30)) {
+            set_error("device ingest takes fewer than 2^32 vertices and 2^30 triangles");
+            return false;
+        }
+    }
+    const uint32_t nv = (uint32_t)vbase, nt = (uint32_t)tbase;
+    e.n_tris = nt;
+    if (nt == 0) return true;  // prepare reports the empty scene
+    DevBuf<unsigned char> raw;
+    DevBuf<IngestMesh> table;
+    DevBuf<float4> wv, wn;
+    DevBuf<int> small;  // bounds[6], flags
+    if (!raw.alloc(std::max<size_t>(total, 1)) || !table.alloc(n_meshes) || !wv.alloc(std::max(nv, 1u)) ||
+        !wn.alloc(std::max(nv, 1u)) || !small.alloc(8) || !e.d_tri.alloc(3 * (size_t)nt) ||
+        !e.d_shade.alloc(3 * (size_t)nt) || !e.d_tri_id.alloc(nt))
+        return false;
+    for (const Piece& p : pieces)
+        if (p.bytes) HIP_OK(hipMemcpyAsync(raw.p + p.at, p.src, p.bytes, hipMemcpyHostToDevice, e.stream));
+    for (unsigned i = 0; i < n_meshes; ++i) {
+        meshes[i].mvert = raw.p + at_vert[i];
+        meshes[i].mpoly = raw.p + at_poly[i];
+        meshes[i].mloop = raw.p + at_loop[i];
+        meshes[i].mat_map = reinterpret_cast<const int*>(raw.p + at_map[i]);
+    }
+    const int init_small[8] = {ordered_float_bits(0x7f800000), ordered_float_bits(0x7f800000),
+                               ordered_float_bits(0x7f800000), ordered_float_bits((int)0xff800000u),
+                               ordered_float_bits((int)0xff800000u), ordered_float_bits((int)0xff800000u), 0, 0};
+    HIP_OK(hipMemcpyAsync(table.p, meshes.data(), n_meshes * sizeof(IngestMesh), hipMemcpyHostToDevice, e.stream));
+    HIP_OK(hipMemcpyAsync(small.p, init_small, sizeof init_small, hipMemcpyHostToDevice, e.stream));
+    launch_ingest(table.p, (int)n_meshes, nv, nt, wv.p, wn.p, e.d_tri.p, e.d_shade.p, e.d_tri_id.p, small.p,
+                  reinterpret_cast<uint32_t*>(small.p + 6), e.stream);
+    HIP_OK(hipGetLastError());
+    int out[8];
+    HIP_OK(hipMemcpyAsync(out, small.p, sizeof out, hipMemcpyDeviceToHost, e.stream));
+    HIP_OK(hipStreamSynchronize(e.stream));
+    if (out[6] & (int)kIngestNonFinite) {
+        set_error("non-finite vertex coordinate");
+        return false;
+    }
+    if (out[6] & (int)kIngestLoopVertex) {
+        set_error("loop vertex index out of range");
+        return false;
+    }
+    for (int k = 0; k < 3; ++k) {  // +0.0f: a zero bound is +0
+        int lo = ordered_float_bits(out[k]), hi = ordered_float_bits(out[3 + k]);
+        float l, h;
+        memcpy(&l, &lo, 4);
+        memcpy(&h, &hi, 4);
+        e.bbox.lo[k] = l + 0.0f;
+        e.bbox.hi[k] = h + 0.0f;
+    }
+    e.ingested_on_device = true;
+    return true;
+}
 }  // namespace
 
 // =====================================================================
@@ -2628,8 +2793,13 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
         memcpy(&h, &lights[i], sizeof h);
         e->py_lights.push_back(h);
     }
+    if (e->bvh_build == 1 && e->device_ingest) {
+        if (!ingest_on_device(*e, meshes_pointers, number_of_meshes, mesh_matrices, meshes_number_of_verts,
+                              meshes_number_of_tris, materials_indices, number_of_materials))
+            return;
+    }
     uint32_t vbase = 0;
-    for (unsigned i = 0; i < number_of_meshes; ++i) {
+    for (unsigned i = 0; i < (e->ingested_on_device ? 0u : number_of_meshes); ++i) {
         const unsigned char* mb = reinterpret_cast<const unsigned char*>((intptr_t)meshes_pointers[i]);
         unsigned nv = meshes_number_of_verts[i], nt = meshes_number_of_tris[i];
         e->mesh_tri_offset.push_back((uint32_t)(e->tris.size() / 4));
@@ -2638,9 +2808,9 @@ static void init_impl(int samples, float base_radius, float radius_reduction,
             return;
         }
         const unsigned char *mpoly, *mloop, *mvert;
-        memcpy(&mpoly, mb + 200, sizeof(void*));
-        memcpy(&mloop, mb + 208, sizeof(void*));
-        memcpy(&mvert, mb + 256, sizeof(void*));
+        memcpy(&mpoly, mb + kMeshMpoly, sizeof(void*));
+        memcpy(&mloop, mb + kMeshMloop, sizeof(void*));
+        memcpy(&mvert, mb + kMeshMvert, sizeof(void*));
         if (nt > 0 && (!mpoly || !mloop || !mvert)) {
             set_error("Blender mesh without mpoly/mloop/mvert");
             return;
@@ -2788,6 +2958,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "graph") g->graph = (int)v;
     else if (k == "bvh_width") g->bvh_width = (int)v == 2 ? 2 : 4;
     else if (k == "bvh_build") g->bvh_build = (int)v == 1 ? 1 : 0;
+    else if (k == "device_ingest") g->device_ingest = v != 0.0 ? 1 : 0;
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
     else if (k == "light_order") g->light_order = (int)v ? 1 : 0;
     else if (k == "camera_qmc") g->camera_qmc = (int)v ? 1 : 0;
@@ -2847,6 +3018,7 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "bvh_width") return g->bvh_width;
     if (k == "node_bytes") return uses_nodes4(*g) ? 128 : 64;
     if (k == "bvh_build") return g->bvh_build;
+    if (k == "device_ingest") return g->device_ingest;
     if (k == "query_order") return g->query_order;
     if (k == "light_order") return g->light_order;
     if (k == "camera_qmc") return g->camera_qmc;

@@ -364,6 +364,33 @@ void launch_gather_count(const PhotonMapView& pm, const float* q, int nq, const 
 void launch_gather_fill(const PhotonMapView& pm, const float* q, int nq, const uint32_t* orig,
                         const unsigned long long* offsets, int* out, hipStream_t s);
 
+// Scene ingest on the GPU: one mesh's Blender arrays, copied to the device as
+// raw bytes (MVert 20 B: float co[3], short no[3], flag, bweight; MPoly 12 B:
+// int loopstart, int totloop, short mat_nr, flag, pad; MLoop 8 B: unsigned v,
+// unsigned e), with its object matrix.
+struct IngestMesh {
+    const unsigned char* mvert;
+    const unsigned char* mpoly;
+    const unsigned char* mloop;
+    const int* mat_map;  // material slot (mat_nr, negative as 0) -> scene material
+    uint32_t nv, nt;     // vertices, polygons (triangles: the first three loops of each)
+    uint32_t vbase, tbase;  // first global vertex and triangle of the mesh
+    float M[12];         // rows 0-2 of the row-major object matrix
+    float inv[9];        // inverse of its upper 3x3, row-major (normals: inv^T n)
+};
+// Bits of the ingest's flag word.
+constexpr uint32_t kIngestNonFinite = 1u;    // a vertex coordinate is not finite
+constexpr uint32_t kIngestLoopVertex = 2u;   // a loop's vertex index is >= the mesh's vertex count
+// World-space triangles {p0, material bits}, {e1, 0}, {e2, 0}, shading normals
+// (3 per triangle), ids (input order) and the scene bounds (6 order-preserving
+// int encodings of lo.xyz, hi.xyz; bounds[0..5] start at +inf / -inf) of
+// n_meshes meshes holding nv vertices and nt triangles; wv and wn hold nv
+// float4 each (scratch).  Asynchronous on s.
+void launch_ingest(const IngestMesh* meshes, int n_meshes, uint32_t nv, uint32_t nt, float4* wv, float4* wn,
+                   float4* tri, float4* shade, int* tri_id, int* bounds, uint32_t* flags, hipStream_t s);
+// The order-preserving int of a float (the bounds' encoding; its own inverse).
+__host__ __device__ inline int ordered_float_bits(int bits) { return bits >= 0 ? bits : bits ^ 0x7fffffff; }
+
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
 void launch_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
 
