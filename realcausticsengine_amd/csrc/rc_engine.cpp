@@ -82,8 +82,6 @@ struct Engine {
     int compat_glass = 1;  // 0: glass of roughness >= 0.01 is a GGX dielectric (BK_RGLASS)
     int light_emit = 0;    // 1: the light samples of every path in a coherent kernel before the light pass
     DevBuf<float4> emitted;
-    DevBuf<uint8_t> ph_len;   // path length of each photon (written only when read_lengths)
-    bool read_lengths = false;
     long long map_ids = 0;    // > 0: photon ids of the next map are indices below this (caller photon lists)
     std::vector<rc_python_material> py_mats;  // as given to init; converted again at rc_prepare
     double photon_capacity = 0, cell_scale = 0.67;
@@ -115,9 +113,9 @@ struct Engine {
     int bvh_depth4 = 0;  // depth of the deepest internal 4-wide node (root 0)
     double bvh_build_ms = 0.0;
     // per-iteration buffers
-    DevBuf<float4> ph_pos, ph_dir, s_pos, s_dir;
-    DevBuf<float2> ph_tp, s_tp;
-    DevBuf<uint32_t> ph_uid, s_index, vals_a, vals_b, hist, hist_sums, counters;
+    DevBuf<float4> ph_rec, s_pos, s_dir;  // ph_rec: kPhotonRecord float4 per unsorted photon
+    DevBuf<float2> s_tp;
+    DevBuf<uint32_t> s_index, vals_a, vals_b, hist, hist_sums, counters;
     DevBuf<unsigned long long> keys_a, keys_b;
     DevBuf<uint2> cells;
     DevBuf<float> image;
@@ -886,9 +884,9 @@ int gather_blocks(const Engine& e) {
 }
 
 bool ensure_photon_buffers(Engine& e, uint32_t cap) {
-    if (cap <= e.capacity && e.ph_pos.p) return true;
+    if (cap <= e.capacity && e.ph_rec.p) return true;
     e.capacity = cap;
-    return e.ph_pos.alloc(cap) && e.ph_dir.alloc(cap) && e.ph_tp.alloc(cap) && e.ph_uid.alloc(cap) &&
+    return e.ph_rec.alloc(kPhotonRecord * (size_t)cap) &&
            e.s_pos.alloc(cap) && e.s_dir.alloc(cap) && e.s_tp.alloc(cap) && e.s_index.alloc(cap) &&
            e.keys_a.alloc(cap) && e.keys_b.alloc(cap) && e.vals_a.alloc(cap) && e.vals_b.alloc(cap) &&
            e.hist.alloc(photon_map_hist_entries(cap)) && e.hist_sums.alloc(photon_map_sum_entries(cap));
@@ -964,7 +962,6 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
-        if (e.read_lengths && !e.ph_len.alloc(e.capacity)) return false;
         LightPassArgs a{};
         a.sc = scene_view(e);
         a.k0 = (uint32_t)e.seed;
@@ -977,11 +974,7 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
         a.path_counter = e.counters.p;
         a.photon_counter = e.counters.p + 1;
         a.capacity = e.capacity;
-        a.out_pos = e.ph_pos.p;
-        a.out_dir = e.ph_dir.p;
-        a.out_tp = e.ph_tp.p;
-        a.out_uid = e.ph_uid.p;
-        a.out_len = e.read_lengths ? e.ph_len.p : nullptr;
+        a.out_rec = e.ph_rec.p;
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
         a.guide = guide_light_view(e, iteration);
         if (e.mlt) {
@@ -1073,10 +1066,7 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     if (!map_grid(e, radius, r2, g)) return false;
     const int uid_bits = g.uid_bits, bucket_bits = g.bucket_bits;
     MapBuildArgs a{};
-    a.in_pos = e.ph_pos.p;
-    a.in_dir = e.ph_dir.p;
-    a.in_tp = e.ph_tp.p;
-    a.in_uid = e.ph_uid.p;
+    a.in_rec = e.ph_rec.p;
     a.count = count;
     for (int k = 0; k < 3; ++k) a.origin[k] = g.pm.origin[k];
     a.inv_cell = g.pm.inv_cell;
@@ -1301,7 +1291,7 @@ bool guide_iteration(Engine& e, uint32_t photon_count, const MapInfo& mi) {
     launch_mark_visible(e.q_pos.p, std::min(e.last_queries, e.q_capacity), pm.origin[0], pm.origin[1], pm.origin[2],
                         pm.inv_cell, pm.mask, gd.vis.p, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(gd.train_w.p), 0u, (size_t)n, s);
-    launch_guide_targets(e.ph_pos.p, e.ph_dir.p, e.ph_tp.p, e.ph_uid.p, photon_count, pm.origin[0], pm.origin[1],
+    launch_guide_targets(e.ph_rec.p, photon_count, pm.origin[0], pm.origin[1],
                          pm.origin[2], pm.inv_cell, pm.mask, gd.vis.p, (uint32_t)n, gd.train_w.p, s);
     HIP_OK(hipEventRecord(e.ev_guide[2], s));
     bool ok = gd.forward(gd.train_x.p, n, s) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) && gd.adam(s);
@@ -1391,7 +1381,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     pm.inv_cell = ip.inv_cell;
     pm.r2 = r2;
     pm.r_query = rq;
-    std::vector<uintptr_t> key = {(uintptr_t)e.ph_pos.p, e.capacity, (uintptr_t)e.q_pos.p, e.q_capacity,
+    std::vector<uintptr_t> key = {(uintptr_t)e.ph_rec.p, e.capacity, (uintptr_t)e.q_pos.p, e.q_capacity,
                                   (uintptr_t)e.cells.p, buckets, (uintptr_t)uid_bits, (uintptr_t)e.image.p,
                                   (uintptr_t)e.path_info.p, (uintptr_t)e.contrib.p, (uintptr_t)e.path_chunk,
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
@@ -1404,10 +1394,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         if (e.graph_exec3) HIP_OK(hipGraphExecDestroy(e.graph_exec3));
         e.graph_exec = e.graph_exec2 = e.graph_exec3 = nullptr;
         MapBuildArgs m{};
-        m.in_pos = e.ph_pos.p;
-        m.in_dir = e.ph_dir.p;
-        m.in_tp = e.ph_tp.p;
-        m.in_uid = e.ph_uid.p;
+        m.in_rec = e.ph_rec.p;
         m.count = e.capacity;
         m.count_dev = e.counters.p + 1;
         m.dyn = e.d_iter.p;
@@ -1871,16 +1858,14 @@ bool write_exr(const char* path, const float* rgb, int W, int H) {
 
 void release(Engine*& e) {
     if (!e) return;
-    DevBuf<float4>* f4[] = {&e->d_tri, &e->d_shade, &e->d_nodes, &e->d_nodes4, &e->ph_pos, &e->ph_dir, &e->s_pos,
-                            &e->s_dir};
+    DevBuf<float4>* f4[] = {&e->d_tri, &e->d_shade, &e->d_nodes, &e->d_nodes4, &e->ph_rec, &e->s_pos, &e->s_dir};
     for (auto* b : f4) b->release();
-    e->ph_tp.release();
     e->s_tp.release();
     DevBuf<float4>* q4[] = {&e->q_pos, &e->q_dir, &e->q_sn, &e->q_gn, &e->q_tpn};
     for (auto* b : q4) b->release();
     e->path_info.release();
     e->contrib.release();
-    DevBuf<uint32_t>* u[] = {&e->ph_uid, &e->s_index, &e->vals_a, &e->vals_b, &e->hist, &e->hist_sums, &e->counters};
+    DevBuf<uint32_t>* u[] = {&e->s_index, &e->vals_a, &e->vals_b, &e->hist, &e->hist_sums, &e->counters};
     for (auto* b : u) b->release();
     e->keys_a.release();
     e->keys_b.release();
@@ -1888,7 +1873,6 @@ void release(Engine*& e) {
     for (auto* d : {&e->lv_a, &e->lv_b, &e->l_hist, &e->l_hist_sums}) d->release();
     e->mlt_state.release();
     e->emitted.release();
-    e->ph_len.release();
     e->mlt_counts.release();
     for (auto* d : {&e->qv_a, &e->qv_b, &e->q_hist, &e->q_hist_sums}) d->release();
     e->cells.release();
@@ -2031,11 +2015,11 @@ void release_batch(Batch*& b) {
     b->graph_exec = nullptr;
     for (Engine*& e : b->scenes) release(e);
     Engine& w = b->work;
-    for (auto* d : {&w.ph_pos, &w.ph_dir, &w.s_pos, &w.s_dir, &w.q_pos, &w.q_dir, &w.q_sn, &w.q_gn, &w.q_tpn})
+    for (auto* d : {&w.ph_rec, &w.s_pos, &w.s_dir, &w.q_pos, &w.q_dir, &w.q_sn, &w.q_gn, &w.q_tpn})
         d->release();
-    for (auto* d : {&w.ph_tp, &w.s_tp}) d->release();
+    w.s_tp.release();
     w.emitted.release();
-    for (auto* d : {&w.ph_uid, &w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
+    for (auto* d : {&w.s_index, &w.vals_a, &w.vals_b, &w.hist, &w.hist_sums, &w.counters}) d->release();
     w.keys_a.release();
     w.keys_b.release();
     for (auto* d : {&w.qk_a, &w.qk_b, &w.lk_a, &w.lk_b}) d->release();
@@ -2132,7 +2116,7 @@ bool batch_guide_training(Batch& b, const BatchView& view, uint32_t count, uint3
     launch_mark_visible_batch(w.q_pos.p, w.q_gn.p, std::min(nq, w.q_capacity), view, b.vis.p, st);
     for (const GuideTargets& t : targets)
         if (t.rows) launch_fill_u32(reinterpret_cast<uint32_t*>(t.w), 0u, t.rows, st);
-    launch_guide_targets_batch(w.ph_pos.p, w.ph_dir.p, w.ph_tp.p, w.ph_uid.p, count, view, b.vis.p, b.d_targets.p,
+    launch_guide_targets_batch(w.ph_rec.p, count, view, b.vis.p, b.d_targets.p,
                                st);
     HIP_OK(hipEventRecord(w.ev_guide[2], st));
     double rows = 0.0;
@@ -2171,7 +2155,7 @@ bool batch_graph_pass(Batch& b, const BatchView& view, CameraPassArgs c, uint32_
     (void)count;
     c.path_counter = w.counters.p;
     c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
-    std::vector<uintptr_t> key = {(uintptr_t)w.ph_pos.p, w.capacity, (uintptr_t)w.q_pos.p, w.q_capacity,
+    std::vector<uintptr_t> key = {(uintptr_t)w.ph_rec.p, w.capacity, (uintptr_t)w.q_pos.p, w.q_capacity,
                                   (uintptr_t)w.cells.p, (uintptr_t)buckets, (uintptr_t)bucket_bits,
                                   (uintptr_t)uid_bits, (uintptr_t)w.path_info.p, (uintptr_t)w.contrib.p,
                                   (uintptr_t)view.n, (uintptr_t)view.sc, (uintptr_t)view.pm, (uintptr_t)view.bs,
@@ -2184,10 +2168,7 @@ bool batch_graph_pass(Batch& b, const BatchView& view, CameraPassArgs c, uint32_
         if (b.graph_exec) HIP_OK(hipGraphExecDestroy(b.graph_exec));
         b.graph_exec = nullptr;
         MapBuildArgs m{};
-        m.in_pos = w.ph_pos.p;
-        m.in_dir = w.ph_dir.p;
-        m.in_tp = w.ph_tp.p;
-        m.in_uid = w.ph_uid.p;
+        m.in_rec = w.ph_rec.p;
         m.count = w.capacity;
         m.count_dev = w.counters.p + 1;
         m.mask = (uint32_t)(buckets - 1);
@@ -2401,10 +2382,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.path_counter = w.counters.p;
         a.photon_counter = w.counters.p + 1;
         a.capacity = w.capacity;
-        a.out_pos = w.ph_pos.p;
-        a.out_dir = w.ph_dir.p;
-        a.out_tp = w.ph_tp.p;
-        a.out_uid = w.ph_uid.p;
+        a.out_rec = w.ph_rec.p;
         a.guide = plain_guide_view();
         a.batch = view;
         a.counters = counting ? w.stats_dev.p : nullptr;
@@ -2432,10 +2410,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     } else {
         // one photon map over every scene
         MapBuildArgs m{};
-        m.in_pos = w.ph_pos.p;
-        m.in_dir = w.ph_dir.p;
-        m.in_tp = w.ph_tp.p;
-        m.in_uid = w.ph_uid.p;
+        m.in_rec = w.ph_rec.p;
         m.count = count;
         m.mask = (uint32_t)(buckets - 1);
         m.bucket_bits = bucket_bits;
@@ -3193,46 +3168,39 @@ __attribute__((visibility("default"))) long long rc_trace_photons(int iteration,
         return -1;
     }
     uint32_t count = 0;
-    g->read_lengths = true;
-    bool traced = run_light_pass(*g, iteration, count);
-    g->read_lengths = false;
-    if (!traced) return -1;
-    std::vector<float4> pos(count), dir(count);
-    std::vector<float2> tp(count);
-    std::vector<uint32_t> path(count);
-    std::vector<uint8_t> len(count);
-    if (count) {
-        if (hipMemcpy(pos.data(), g->ph_pos.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(dir.data(), g->ph_dir.p, count * 16, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(tp.data(), g->ph_tp.p, count * 8, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(path.data(), g->ph_uid.p, count * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-            hipMemcpy(len.data(), g->ph_len.p, count, hipMemcpyDeviceToHost) != hipSuccess) {
-            set_error("photon readback failed");
-            return -1;
-        }
+    if (!run_light_pass(*g, iteration, count)) return -1;
+    std::vector<float4> rec(kPhotonRecord * (size_t)count);
+    if (count && hipMemcpy(rec.data(), g->ph_rec.p, rec.size() * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("photon readback failed");
+        return -1;
     }
-    g->ph_len.release();
     std::vector<uint32_t> u(count);
-    for (uint32_t i = 0; i < count; ++i) u[i] = path[i] * 8u + len[i];
+    for (uint32_t i = 0; i < count; ++i) {
+        const float4 r2 = rec[kPhotonRecord * (size_t)i + 2];
+        uint32_t path, len;
+        memcpy(&path, &r2.z, 4);
+        memcpy(&len, &r2.w, 4);
+        u[i] = path * 8u + len;
+    }
     // path order (the order of the reference's sequential light loop)
     std::vector<uint32_t> ord(count);
     for (uint32_t i = 0; i < count; ++i) ord[i] = i;
     std::sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return u[a] < u[b]; });
     long long m = std::min<long long>(cap, count);
     for (long long k = 0; k < m; ++k) {
-        uint32_t i = ord[k];
+        const float4* p = &rec[kPhotonRecord * (size_t)ord[k]];
         float* r = photons + 10 * k;
-        r[0] = pos[i].x;
-        r[1] = pos[i].y;
-        r[2] = pos[i].z;
-        r[3] = dir[i].x;
-        r[4] = dir[i].y;
-        r[5] = dir[i].z;
-        r[6] = dir[i].w;
-        r[7] = tp[i].x;
-        r[8] = tp[i].y;
-        r[9] = pos[i].w;
-        if (uid) uid[k] = u[i];
+        r[0] = p[0].x;
+        r[1] = p[0].y;
+        r[2] = p[0].z;
+        r[3] = p[1].x;
+        r[4] = p[1].y;
+        r[5] = p[1].z;
+        r[6] = p[1].w;
+        r[7] = p[2].x;
+        r[8] = p[2].y;
+        r[9] = p[0].w;
+        if (uid) uid[k] = u[ord[k]];
     }
     return count;
 }
@@ -3241,22 +3209,17 @@ __attribute__((visibility("default"))) long long rc_trace_photons(int iteration,
 static bool load_photons(const float* photons, long long count) {
     Engine& e = *g;
     if (!ensure_photon_buffers(e, (uint32_t)std::max<long long>(count, std::max<uint32_t>(e.capacity, 1024)))) return false;
-    std::vector<float4> pos(count), dir(count);
-    std::vector<float2> tp(count);
-    std::vector<uint32_t> u(count);
-    for (long long i = 0; i < count; ++i) {
+    std::vector<float4> rec(kPhotonRecord * (size_t)count);
+    for (long long i = 0; i < count; ++i) {  // photon id = input index, path length 0
         const float* r = photons + 10 * i;
-        pos[i] = make_float4(r[0], r[1], r[2], r[9]);
-        dir[i] = make_float4(r[3], r[4], r[5], r[6]);
-        tp[i] = make_float2(r[7], r[8]);
-        u[i] = (uint32_t)i;
+        float id;
+        const uint32_t u = (uint32_t)i;
+        memcpy(&id, &u, 4);
+        rec[kPhotonRecord * i] = make_float4(r[0], r[1], r[2], r[9]);
+        rec[kPhotonRecord * i + 1] = make_float4(r[3], r[4], r[5], r[6]);
+        rec[kPhotonRecord * i + 2] = make_float4(r[7], r[8], id, 0.0f);
     }
-    if (count) {
-        HIP_OK(hipMemcpy(e.ph_pos.p, pos.data(), count * 16, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(e.ph_dir.p, dir.data(), count * 16, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(e.ph_tp.p, tp.data(), count * 8, hipMemcpyHostToDevice));
-        HIP_OK(hipMemcpy(e.ph_uid.p, u.data(), count * 4, hipMemcpyHostToDevice));
-    }
+    if (count) HIP_OK(hipMemcpy(e.ph_rec.p, rec.data(), rec.size() * sizeof(float4), hipMemcpyHostToDevice));
     return true;
 }
 

@@ -139,6 +139,14 @@ __device__ __forceinline__ int batch_scene_of(const uint32_t* starts, int n, uin
     return lo;
 }
 
+// An unsorted photon as the light pass writes it: kPhotonRecord float4,
+//   {position, dVM}  {direction towards the light, throughput.r}
+//   {throughput.g, throughput.b, light path index (bits), path length (bits)}
+// one record per photon, so that the map's permutation gathers one record
+// (one or two cache lines) per photon; the light path index is the map's sort
+// key below the bucket.
+constexpr int kPhotonRecord = 3;
+
 struct LightPassArgs {
     SceneView sc;
     uint32_t k0, k1, iteration, stream;
@@ -148,11 +156,7 @@ struct LightPassArgs {
     uint32_t* path_counter;
     uint32_t* photon_counter;
     uint32_t capacity;
-    float4* out_pos;
-    float4* out_dir;
-    float2* out_tp;
-    uint32_t* out_uid;   // light path index of each photon (the map's sort key below the bucket)
-    uint8_t* out_len;    // non-null: path length (vertices) of each photon, for readback
+    float4* out_rec;     // kPhotonRecord float4 per photon (layout below)
     Counters* counters;  // non-null selects the counting variant
     GuideView guide;     // learned emission proposal (all slots -1: plain)
     BatchView batch;     // batch.n > 0: several scenes (sc, k0, k1, proposal per scene)
@@ -265,10 +269,7 @@ void launch_debug_aov(const SceneView& sc, int mode, float ior, float* image, hi
 
 // Photon map build: keys, sort, permute, cell ranges.
 struct MapBuildArgs {
-    const float4* in_pos;
-    const float4* in_dir;
-    const float2* in_tp;
-    const uint32_t* in_uid;
+    const float4* in_rec;     // unsorted photon records (kPhotonRecord float4 each)
     uint32_t count;           // photons (with count_dev: the capacity the grids are sized for)
     const uint32_t* count_dev;  // non-null: the photon count is read here (captured graphs)
     const IterParams* dyn;      // non-null: origin and inv_cell come from here
@@ -292,7 +293,7 @@ struct MapBuildArgs {
 };
 void build_photon_map(const MapBuildArgs& a, hipStream_t s);
 // Batch: the keys of every scene's photons in one sort (bucket = the scene's
-// bucket_base + its own cell bucket; uid = global light path * 8 + length).
+// bucket_base + its own cell bucket, above the global light path index).
 void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s);
 // Training targets of the scenes of a batch (rc_guide.hip): vis[bucket_base +
 // bucket] = 1 for the cell of every query (its scene in gn.w), then
@@ -300,11 +301,12 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
 // visible cell of its scene.
 void launch_mark_visible_batch(const float4* qpos, const float4* qgn, uint32_t nq, const BatchView& b, uint8_t* vis,
                                hipStream_t s);
-void launch_guide_targets_batch(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
-                                uint32_t count, const BatchView& b, const uint8_t* vis, const GuideTargets* t,
-                                hipStream_t s);
-// Stable LSD radix sort (8-bit digits) of (key, value) pairs on the low `bits`
-// key bits; the sorted pairs end in *keys_out / *vals_out (one of the two buffers).
+void launch_guide_targets_batch(const float4* rec, uint32_t count, const BatchView& b, const uint8_t* vis,
+                                const GuideTargets* t, hipStream_t s);
+// Stable LSD radix sort (8-bit digits, one kernel per digit) of (key, value)
+// pairs on the low `bits` key bits; the sorted pairs end in *keys_out /
+// *vals_out (one of the two buffers).  hist: photon_map_hist_entries(n)
+// entries of scratch; hist_sums is not used.
 void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
                       uint32_t* hist, uint32_t* hist_sums, const uint32_t* count_dev, uint32_t n, int bits,
                       unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s);

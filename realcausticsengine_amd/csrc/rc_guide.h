@@ -121,8 +121,7 @@ void launch_guide_table(const float* Z, int rows, float* cdf, hipStream_t s);
 void launch_mark_visible(const float4* qpos, uint32_t nq, float ox, float oy, float oz, float inv_cell, uint32_t mask,
                          uint8_t* vis, hipStream_t s);
 // w[path] += luminance of each stored photon of a training path whose cell is visible.
-void launch_guide_targets(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
-                          uint32_t count, float ox, float oy, float oz, float inv_cell, uint32_t mask,
-                          const uint8_t* vis, uint32_t train_paths, float* w, hipStream_t s);
+void launch_guide_targets(const float4* rec, uint32_t count, float ox, float oy, float oz, float inv_cell,
+                          uint32_t mask, const uint8_t* vis, uint32_t train_paths, float* w, hipStream_t s);
 
 }  // namespace rc

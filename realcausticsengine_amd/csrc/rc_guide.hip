@@ -343,16 +343,16 @@ __global__ void k_mark_visible(const float4* __restrict__ qpos, uint32_t n, floa
         vis[bucket_of(qpos[i], ox, oy, oz, inv_cell, mask)] = 1;
 }
 
-__global__ void k_guide_targets(const float4* __restrict__ pos, const float4* __restrict__ dir_tr,
-                                const float2* __restrict__ tp_gb, const uint32_t* __restrict__ uid, uint32_t n,
+__global__ void k_guide_targets(const float4* __restrict__ rec, uint32_t n,
                                 float ox, float oy, float oz, float inv_cell, uint32_t mask,
                                 const uint8_t* __restrict__ vis, uint32_t train_paths, float* __restrict__ w) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t path = uid[i];
+        const float4* r = rec + kPhotonRecord * (size_t)i;
+        const float4 r2 = r[2];
+        uint32_t path = __float_as_uint(r2.z);
         if (path >= train_paths) continue;
-        if (!vis[bucket_of(pos[i], ox, oy, oz, inv_cell, mask)]) continue;
-        float2 gb = tp_gb[i];
-        float lum = (dir_tr[i].w + gb.x + gb.y) * (1.0f / 3.0f);
+        if (!vis[bucket_of(r[0], ox, oy, oz, inv_cell, mask)]) continue;
+        float lum = (r[1].w + r2.x + r2.y) * (1.0f / 3.0f);
         atomicAdd(w + path, lum);
     }
 }
@@ -367,22 +367,22 @@ __global__ void k_mark_visible_batch(const float4* __restrict__ qpos, const floa
     }
 }
 
-__global__ void k_guide_targets_batch(const float4* __restrict__ pos, const float4* __restrict__ dir_tr,
-                                      const float2* __restrict__ tp_gb, const uint32_t* __restrict__ uid, uint32_t n,
+__global__ void k_guide_targets_batch(const float4* __restrict__ rec, uint32_t n,
                                       BatchView b, const uint8_t* __restrict__ vis,
                                       const GuideTargets* __restrict__ t) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        uint32_t path = uid[i];
+        const float4* r = rec + kPhotonRecord * (size_t)i;
+        const float4 r2 = r[2];
+        uint32_t path = __float_as_uint(r2.z);
         int sc = batch_scene_of(b.light_starts, b.n, path);
         const BatchScene& bs = b.bs[sc];
         uint32_t local = path - bs.light_base;
         GuideTargets tg = t[sc];
         if (local >= tg.rows) continue;
         const PhotonMapView& pm = b.pm[sc];
-        if (!vis[bs.bucket_base + bucket_of(pos[i], pm.origin[0], pm.origin[1], pm.origin[2], pm.inv_cell, pm.mask)])
+        if (!vis[bs.bucket_base + bucket_of(r[0], pm.origin[0], pm.origin[1], pm.origin[2], pm.inv_cell, pm.mask)])
             continue;
-        float2 gb = tp_gb[i];
-        float lum = (dir_tr[i].w + gb.x + gb.y) * (1.0f / 3.0f);
+        float lum = (r[1].w + r2.x + r2.y) * (1.0f / 3.0f);
         atomicAdd(tg.w + local, lum);
     }
 }
@@ -417,12 +417,11 @@ void launch_mark_visible(const float4* qpos, uint32_t nq, float ox, float oy, fl
         hipLaunchKernelGGL(k_mark_visible, dim3(std::min<uint32_t>(4096, (nq + 255) / 256)), dim3(256), 0, s, qpos, nq,
                            ox, oy, oz, inv_cell, mask, vis);
 }
-void launch_guide_targets(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
-                          uint32_t count, float ox, float oy, float oz, float inv_cell, uint32_t mask,
-                          const uint8_t* vis, uint32_t train_paths, float* w, hipStream_t s) {
+void launch_guide_targets(const float4* rec, uint32_t count, float ox, float oy, float oz, float inv_cell,
+                          uint32_t mask, const uint8_t* vis, uint32_t train_paths, float* w, hipStream_t s) {
     if (count > 0 && train_paths > 0)
-        hipLaunchKernelGGL(k_guide_targets, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0, s, pos,
-                           dir_tr, tp_gb, uid, count, ox, oy, oz, inv_cell, mask, vis, train_paths, w);
+        hipLaunchKernelGGL(k_guide_targets, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0, s, rec,
+                           count, ox, oy, oz, inv_cell, mask, vis, train_paths, w);
 }
 
 void launch_mark_visible_batch(const float4* qpos, const float4* qgn, uint32_t nq, const BatchView& b, uint8_t* vis,
@@ -431,12 +430,11 @@ void launch_mark_visible_batch(const float4* qpos, const float4* qgn, uint32_t n
         hipLaunchKernelGGL(k_mark_visible_batch, dim3(std::min<uint32_t>(4096, (nq + 255) / 256)), dim3(256), 0, s,
                            qpos, qgn, nq, b, vis);
 }
-void launch_guide_targets_batch(const float4* pos, const float4* dir_tr, const float2* tp_gb, const uint32_t* uid,
-                                uint32_t count, const BatchView& b, const uint8_t* vis, const GuideTargets* t,
-                                hipStream_t s) {
+void launch_guide_targets_batch(const float4* rec, uint32_t count, const BatchView& b, const uint8_t* vis,
+                                const GuideTargets* t, hipStream_t s) {
     if (count > 0)
         hipLaunchKernelGGL(k_guide_targets_batch, dim3(std::min<uint32_t>(8192, (count + 255) / 256)), dim3(256), 0,
-                           s, pos, dir_tr, tp_gb, uid, count, b, vis, t);
+                           s, rec, count, b, vis, t);
 }
 
 }  // namespace rc

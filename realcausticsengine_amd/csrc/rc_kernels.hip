@@ -91,11 +91,11 @@ struct PhotonQueue {
             const uint32_t e = (head + l) & (kPhotonRing - 1u);
             const uint32_t at = base + l;
             if (at < a.capacity) {
-                a.out_pos[at] = r->pos[e];
-                a.out_dir[at] = r->dir[e];
-                a.out_tp[at] = r->tp[e];
-                a.out_uid[at] = r->path[e];
-                if (a.out_len) a.out_len[at] = r->len[e];
+                float4* o = a.out_rec + kPhotonRecord * (size_t)at;
+                const float2 tp = r->tp[e];
+                o[0] = r->pos[e];
+                o[1] = r->dir[e];
+                o[2] = make_float4(tp.x, tp.y, __uint_as_float(r->path[e]), __uint_as_float((uint32_t)r->len[e]));
             }
         }
         head = (head + n) & (kPhotonRing - 1u);
@@ -1573,24 +1573,36 @@ void launch_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, ui
 // Photon map build: keys, LSD radix sort (8-bit digits, stable), permute,
 // bucket ranges.
 // ---------------------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 16;
-constexpr int kSortTile = kSortThreads * kSortRounds;
+// The sort (radix_sort_pairs): one-sweep LSD radix sort of (key, value)
+// pairs on 8-bit digits (Adinets and Merrill, "Onesweep: A Faster Least
+// Significant Digit Radix Sort for GPUs", 2022).  One kernel counts the
+// digits of every pass, one scans them, then each pass is one kernel: a
+// workgroup takes the next tile by ticket, ranks its keys by digit (stable:
+// wave-ordered chunks, ballot matches within a wave), publishes its digit
+// counts and finds the counts of the tiles before it by decoupled look-back,
+// reorders the tile by digit in LDS and writes each digit's run contiguously.
+// Scratch (u32 units): two look-back status arrays of tiles x 256 64-bit
+// words (a pass reads one and clears the other for the next pass), the digit
+// offsets of 8 passes, and the tickets.
+constexpr int kOsThreads = 256;
+constexpr int kOsItems = 16;
+constexpr uint32_t kOsTile = kOsThreads * kOsItems;
+constexpr int kOsWaves = kOsThreads / 64;
+constexpr int kOsMaxPasses = 8;
+constexpr uint32_t kOsChunk = kOsTile / kOsWaves;  // keys per wave, in index order
 
-size_t photon_map_hist_entries(uint32_t max_count) {
-    size_t blocks = (max_count + kSortTile - 1) / kSortTile;
-    if (blocks == 0) blocks = 1;
-    return 256 * blocks;
+__host__ __device__ constexpr size_t onesweep_tiles(size_t n) { return n == 0 ? 1 : (n + kOsTile - 1) / kOsTile; }
+__host__ __device__ constexpr size_t onesweep_scratch_u32(size_t n) {
+    return 2 * (2 * 256 * onesweep_tiles(n)) + kOsMaxPasses * 256 + 16;
 }
-size_t photon_map_sum_entries(uint32_t max_count) {
-    size_t h = photon_map_hist_entries(max_count);
-    return (h + kScanTile - 1) / kScanTile + 1;
-}
+
+size_t photon_map_hist_entries(uint32_t max_count) { return onesweep_scratch_u32(max_count); }
+size_t photon_map_sum_entries(uint32_t) { return 1; }  // the one-sweep sort needs no scan partials
 
 // Element count of the map kernels: n, or the count in device memory (at most n) when n_dev is given.
 __device__ __forceinline__ uint32_t live_count(const uint32_t* n_dev, uint32_t n) { return n_dev ? min(*n_dev, n) : n; }
 
-__global__ void k_map_keys(const float4* pos, const uint32_t* uid, const uint32_t* n_dev, uint32_t n,
+__global__ void k_map_keys(const float4* rec, const uint32_t* n_dev, uint32_t n,
                            const IterParams* dyn, float ox, float oy, float oz, float inv_cell, uint32_t mask,
                            int uid_bits, unsigned long long* keys, uint32_t* vals) {
     n = live_count(n_dev, n);
@@ -1602,109 +1614,218 @@ __global__ void k_map_keys(const float4* pos, const uint32_t* uid, const uint32_
         oz = dyn->origin[2];
         inv_cell = dyn->inv_cell;
     }
-    float4 p = pos[i];
+    const float4 p = rec[kPhotonRecord * (size_t)i];
+    const uint32_t path = __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z);
     int cx = (int)floorf((p.x - ox) * inv_cell);
     int cy = (int)floorf((p.y - oy) * inv_cell);
     int cz = (int)floorf((p.z - oz) * inv_cell);
     uint32_t bkt = cell_bucket(cx, cy, cz, mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)uid[i];
+    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)path;
     vals[i] = i;
 }
 
-__global__ __launch_bounds__(kSortThreads) void k_sort_hist(const unsigned long long* keys, const uint32_t* n_dev,
-                                                            uint32_t n, int shift, uint32_t* hist, uint32_t nblocks) {
-    __shared__ uint32_t h[256];
-    n = live_count(n_dev, n);
-    h[threadIdx.x] = 0;
+// Digit counts of every pass: dig[p * 256 + d] += keys whose digit p is d
+// (per-wave LDS counts, then one global atomic per digit and workgroup).
+// Also clears the first pass's look-back status rows of the tiles that hold keys.
+__global__ __launch_bounds__(kOsThreads) void k_os_hist(const unsigned long long* keys, const uint32_t* n_dev,
+                                                        uint32_t n, int passes, uint32_t* dig,
+                                                        unsigned long long* status0) {
+    __shared__ uint32_t h[kOsWaves][kOsMaxPasses][256];
+    const uint32_t wave = threadIdx.x >> 6;
+    for (int w = 0; w < kOsWaves; ++w)
+        for (int p = 0; p < passes; ++p) h[w][p][threadIdx.x] = 0;
     __syncthreads();
-    uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; ++r) {
-        uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    n = live_count(n_dev, n);
+    const size_t rows = 256 * ((size_t)(n + kOsTile - 1) / kOsTile);
+    for (size_t i = (size_t)blockIdx.x * kOsThreads + threadIdx.x; i < rows; i += (size_t)gridDim.x * kOsThreads)
+        status0[i] = 0ull;
+    // a digit shared by the whole wave (common in the high digits) is added once by the first lane
+    for (uint32_t i0 = blockIdx.x * kOsThreads + (threadIdx.x & ~63u); i0 < n; i0 += gridDim.x * kOsThreads) {
+        const uint32_t i = i0 + (threadIdx.x & 63u);
+        const bool valid = i < n;
+        const unsigned long long k = valid ? keys[i] : 0ull;
+        const uint64_t live = __ballot(valid);
+        for (int p = 0; p < passes; ++p) {
+            const uint32_t d = (uint32_t)(k >> (8 * p)) & 255u;
+            const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+            if (__ballot(valid && d == d0) == live) {
+                if ((threadIdx.x & 63u) == 0) h[wave][p][d0] += (uint32_t)__builtin_popcountll(live);
+            } else if (valid) {
+                atomicAdd(&h[wave][p][d], 1u);
+            }
+        }
     }
     __syncthreads();
-    hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+    for (int p = 0; p < passes; ++p) {
+        uint32_t c = 0;
+        for (int w = 0; w < kOsWaves; ++w) c += h[w][p][threadIdx.x];
+        if (c) atomicAdd(&dig[p * 256 + threadIdx.x], c);
+    }
 }
 
-// Sorts of at most this many tiles skip the separate scan: each scatter block
-// forms its offsets from the raw digit counts (nblocks reads per thread, from L2).
-constexpr uint32_t kSelfScanBlocks = 128;
-
-// kSelfScan = false: offsets = exclusive scan of the digit-major counts;
-// true: offsets holds the raw counts hist[digit * nblocks + block].
-template <bool kSelfScan>
-__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(const unsigned long long* kin, const uint32_t* vin,
-                                                               const uint32_t* n_dev, uint32_t n, int shift,
-                                                               const uint32_t* offsets, uint32_t nblocks,
-                                                               unsigned long long* kout, uint32_t* vout) {
-    __shared__ uint32_t running[256];
-    n = live_count(n_dev, n);
-    __shared__ uint32_t wcount[kSortThreads / 64][256];
-    const uint32_t wave = threadIdx.x >> 6;
-    if (kSelfScan) {
-        // this digit's count over all tiles and over the tiles before this one
-        const uint32_t* row = offsets + (size_t)threadIdx.x * nblocks;
-        uint32_t total = 0, before = 0;
-        for (uint32_t b = 0; b < nblocks; ++b) {
-            uint32_t v = row[b];
-            total += v;
-            before += b < blockIdx.x ? v : 0u;
-        }
-        running[threadIdx.x] = total;
+// Exclusive scan of each pass's 256 digit counts, in place (one workgroup).
+__global__ __launch_bounds__(256) void k_os_scan(uint32_t* dig, int passes) {
+    __shared__ uint32_t t[256];
+    for (int p = 0; p < passes; ++p) {
+        const uint32_t v = dig[p * 256 + threadIdx.x];
+        t[threadIdx.x] = v;
         __syncthreads();
-        for (uint32_t off = 1; off < 256; off <<= 1) {  // inclusive scan of the digit totals
-            uint32_t t = threadIdx.x >= off ? running[threadIdx.x - off] : 0u;
+        for (uint32_t off = 1; off < 256; off <<= 1) {
+            const uint32_t a = threadIdx.x >= off ? t[threadIdx.x - off] : 0u;
             __syncthreads();
-            running[threadIdx.x] += t;
+            t[threadIdx.x] += a;
             __syncthreads();
         }
-        running[threadIdx.x] += before - total;
-    } else {
-        running[threadIdx.x] = offsets[threadIdx.x * nblocks + blockIdx.x];
+        dig[p * 256 + threadIdx.x] = t[threadIdx.x] - v;
+        __syncthreads();
     }
-    uint32_t base = blockIdx.x * kSortTile;
-    for (int r = 0; r < kSortRounds; ++r) {
-        for (int k = 0; k < kSortThreads / 64; ++k) wcount[k][threadIdx.x] = 0;
-        __syncthreads();
-        uint32_t i = base + r * kSortThreads + threadIdx.x;
-        bool valid = i < n;
-        unsigned long long key = valid ? kin[i] : 0ull;
-        uint32_t val = valid ? vin[i] : 0u;
-        uint32_t digit = (uint32_t)(key >> shift) & 255u;
+}
+
+// Look-back status word of (tile, digit): flag in the high half (1: the
+// tile's own count, 2: the inclusive count of every tile up to it), count in
+// the low half; 0 = not yet published.
+constexpr unsigned long long kOsAggregate = 1ull << 32, kOsInclusive = 2ull << 32;
+// Polls of a predecessor's status before a tile gives up (about 2^24 short
+// sleeps): a bound on the wait.  A tile that gives up flags the failure, and
+// every write of the pass stays below n.
+constexpr uint32_t kOsSpinLimit = 1u << 24;
+
+__global__ __launch_bounds__(kOsThreads) void k_os_pass(const unsigned long long* __restrict__ kin,
+                                                        const uint32_t* __restrict__ vin, const uint32_t* n_dev,
+                                                        uint32_t n, int shift, const uint32_t* __restrict__ dig,
+                                                        unsigned long long* status, unsigned long long* status_next,
+                                                        uint32_t* ticket, uint32_t* fail,
+                                                        unsigned long long* __restrict__ kout,
+                                                        uint32_t* __restrict__ vout) {
+    __shared__ unsigned long long s_key[kOsTile];
+    __shared__ uint32_t s_val[kOsTile];
+    __shared__ uint32_t s_cnt[kOsWaves][256];  // per wave: digit counts, then the waves-before prefix
+    __shared__ uint32_t s_excl[256];           // tile-exclusive digit offsets
+    __shared__ uint32_t s_glob[256];           // output index of this tile's first key of each digit
+    __shared__ uint32_t s_tile;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    if (tid == 0) s_tile = atomicAdd(ticket, 1u);
+    for (int w = 0; w < kOsWaves; ++w) s_cnt[w][tid] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    n = live_count(n_dev, n);
+    const uint32_t base = tile * kOsTile;
+    if (base >= n) return;  // no keys: no tile holding keys reads this one's status
+    status_next[(size_t)tile * 256 + tid] = 0ull;  // the next pass's status row (read after this kernel ends)
+    // 1. ranks within the wave's chunk: kOsItems rounds of 64 keys in index order
+    unsigned long long key[kOsItems];
+    uint32_t val[kOsItems], rank[kOsItems];
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {  // every load issued before the ranking's wave barriers
+        const uint32_t i = base + wave * kOsChunk + j * 64 + lane;
+        key[j] = i < n ? kin[i] : 0ull;
+        val[j] = i < n ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        const bool valid = base + wave * kOsChunk + j * 64 + lane < n;
+        const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
         for (int bit = 0; bit < 8; ++bit) {
-            bool b = (digit >> bit) & 1u;
-            uint64_t bb = __ballot(valid && b);
+            const bool b = (d >> bit) & 1u;
+            const uint64_t bb = __ballot(valid && b);
             peers &= b ? bb : ~bb;
         }
-        uint32_t rank = lanes_below(peers);
-        if (valid && rank == 0) wcount[wave][digit] = (uint32_t)__builtin_popcountll(peers);
+        const uint32_t r = lanes_below(peers);
+        const uint32_t before = valid ? s_cnt[wave][d] : 0u;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && r == 0) s_cnt[wave][d] = before + (uint32_t)__builtin_popcountll(peers);
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = before + r;
+    }
+    __syncthreads();
+    // 2. thread tid = digit tid: the tile's count, the waves-before prefixes, the tile-exclusive offsets
+    uint32_t h = 0;
+#pragma unroll
+    for (int w = 0; w < kOsWaves; ++w) {
+        const uint32_t c = s_cnt[w][tid];
+        s_cnt[w][tid] = h;
+        h += c;
+    }
+    s_excl[tid] = h;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t a = tid >= off ? s_excl[tid - off] : 0u;
         __syncthreads();
-        if (valid) {
-            uint32_t pos = running[digit] + rank;
-            for (uint32_t w = 0; w < wave; ++w) pos += wcount[w][digit];
-            kout[pos] = key;
-            vout[pos] = val;
+        s_excl[tid] += a;
+        __syncthreads();
+    }
+    const uint32_t excl = s_excl[tid] - h;
+    __syncthreads();
+    s_excl[tid] = excl;
+    // 3. decoupled look-back for digit tid
+    unsigned long long* st = status + (size_t)tile * 256 + tid;
+    __hip_atomic_store(st, (tile == 0 ? kOsInclusive : kOsAggregate) | h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t prefix = 0;
+    bool gave_up = false;
+    if (tile > 0) {
+        int j = (int)tile - 1;
+        uint32_t spins = 0;
+        while (j >= 0) {
+            const unsigned long long w = __hip_atomic_load(status + (size_t)j * 256 + tid, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            if ((w >> 32) == 0ull) {
+                if (++spins >= kOsSpinLimit) {
+                    gave_up = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+                continue;
+            }
+            prefix += (uint32_t)w;
+            if ((w >> 32) == 2ull) break;
+            --j;
         }
-        __syncthreads();
-        uint32_t add = 0;
-        for (int k = 0; k < kSortThreads / 64; ++k) add += wcount[k][threadIdx.x];
-        running[threadIdx.x] += add;
-        __syncthreads();
+        __hip_atomic_store(st, kOsInclusive | (prefix + h), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (gave_up) atomicOr(fail, 1u);
+    s_glob[tid] = dig[tid] + prefix;
+    __syncthreads();
+    // 4. the tile in digit order in LDS, then each digit's run written contiguously
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        const uint32_t i = base + wave * kOsChunk + j * 64 + lane;
+        if (i < n) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+            const uint32_t at = s_excl[d] + s_cnt[wave][d] + rank[j];
+            s_key[at] = key[j];
+            s_val[at] = val[j];
+        }
+    }
+    __syncthreads();
+    const uint32_t cnt = min(kOsTile, n - base);
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        const uint32_t e = j * kOsThreads + tid;
+        if (e < cnt) {
+            const unsigned long long k = s_key[e];
+            const uint32_t d = (uint32_t)(k >> shift) & 255u;
+            const uint32_t out = s_glob[d] + (e - s_excl[d]);
+            if (out < n) {
+                kout[out] = k;
+                vout[out] = s_val[e];
+            }
+        }
     }
 }
 
-__global__ void k_map_permute(const float4* pos, const float4* dir, const float2* tp, const uint32_t* vals,
-                              const uint32_t* n_dev, uint32_t n, float4* opos, float4* odir, float2* otp,
-                              uint32_t* oidx) {
+__global__ void k_map_permute(const float4* rec, const uint32_t* vals, const uint32_t* n_dev, uint32_t n,
+                              float4* opos, float4* odir, float2* otp, uint32_t* oidx) {
     n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t j = vals[i];
-    opos[i] = pos[j];
-    odir[i] = dir[j];
-    otp[i] = tp[j];
+    const uint32_t j = vals[i];
+    const float4* r = rec + kPhotonRecord * (size_t)j;
+    const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+    opos[i] = r0;
+    odir[i] = r1;
+    otp[i] = make_float2(r2.x, r2.y);
     oidx[i] = j;
 }
 
@@ -1718,13 +1839,13 @@ __global__ void k_map_cells(const unsigned long long* keys, const uint32_t* n_de
     if (i == n - 1 || (uint32_t)(keys[i + 1] >> uid_bits) != b) cells[b].y = i + 1;
 }
 
-__global__ void k_map_keys_batch(const float4* pos, const uint32_t* uid, const uint32_t* n_dev, uint32_t n,
+__global__ void k_map_keys_batch(const float4* rec, const uint32_t* n_dev, uint32_t n,
                                  BatchView b, int uid_bits, unsigned long long* keys, uint32_t* vals) {
     n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    float4 p = pos[i];
-    uint32_t u = uid[i];
+    const float4 p = rec[kPhotonRecord * (size_t)i];
+    const uint32_t u = __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z);
     int sc = batch_scene_of(b.light_starts, b.n, u);
     const PhotonMapView& pm = b.pm[sc];
     int cx = (int)floorf((p.x - pm.origin[0]) * pm.inv_cell);
@@ -1742,7 +1863,7 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     uint32_t n = a.count;
     if (n == 0) return;
     unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, a.count_dev, n, a.dyn, a.origin[0],
+    hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_rec, a.count_dev, n, a.dyn, a.origin[0],
                        a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a);
     sort_permute_cells(a, s);
 }
@@ -1753,7 +1874,7 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
     uint32_t n = a.count;
     if (n == 0) return;
     unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_pos, a.in_uid, a.count_dev, n, b,
+    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_rec, a.count_dev, n, b,
                        a.uid_bits, a.keys_a, a.vals_a);
     sort_permute_cells(a, s);
 }
@@ -1761,31 +1882,32 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
 // LSD radix sort of the keys (8-bit digits, stable), permutation of the
 // photons into key order, per-bucket [start, end).
 void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
-                      uint32_t* hist, uint32_t* hist_sums, const uint32_t* count_dev, uint32_t n, int bits,
+                      uint32_t* hist, uint32_t*, const uint32_t* count_dev, uint32_t n, int bits,
                       unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s) {
-    uint32_t nblocks = (n + kSortTile - 1) / kSortTile;
+    const int passes = std::min(kOsMaxPasses, (bits + 7) / 8);
+    *keys_out = keys_a;
+    *vals_out = vals_a;
+    if (n == 0 || passes == 0) return;
+    const size_t tiles = onesweep_tiles(n);
+    unsigned long long* status[2] = {reinterpret_cast<unsigned long long*>(hist),
+                                     reinterpret_cast<unsigned long long*>(hist) + 256 * tiles};
+    uint32_t* dig = hist + 2 * (2 * 256 * tiles);
+    uint32_t* tickets = dig + kOsMaxPasses * 256;  // one per pass, then the failure flag
+    // the digit counts, the tickets and the flag start at zero; k_os_hist clears the first status rows in use
+    launch_fill_u32(dig, 0u, kOsMaxPasses * 256 + 16, s);
+    hipLaunchKernelGGL(k_os_hist, dim3((unsigned)std::min<size_t>(tiles, 512)), dim3(kOsThreads), 0, s, keys_a,
+                       count_dev, n, passes, dig, status[0]);
+    hipLaunchKernelGGL(k_os_scan, dim3(1), dim3(256), 0, s, dig, passes);
     unsigned long long* kin = keys_a;
     unsigned long long* kout = keys_b;
     uint32_t* vin = vals_a;
     uint32_t* vout = vals_b;
-    const bool self_scan = nblocks <= kSelfScanBlocks;
-    for (int shift = 0; shift < bits && n > 0; shift += 8) {
-        hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, s, kin, count_dev, n, shift, hist,
-                           nblocks);
-        if (self_scan) {
-            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev, n,
-                               shift, hist, nblocks, kout, vout);
-        } else {
-            launch_exclusive_scan_u32(hist, hist, 256 * nblocks, hist_sums, s);
-            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(nblocks), dim3(kSortThreads), 0, s, kin, vin, count_dev,
-                               n, shift, hist, nblocks, kout, vout);
-        }
-        unsigned long long* tk = kin;
-        kin = kout;
-        kout = tk;
-        uint32_t* tv = vin;
-        vin = vout;
-        vout = tv;
+    for (int p = 0; p < passes; ++p) {
+        hipLaunchKernelGGL(k_os_pass, dim3((unsigned)tiles), dim3(kOsThreads), 0, s, kin, vin, count_dev, n, 8 * p,
+                           dig + 256 * p, status[p & 1], status[(p + 1) & 1], tickets + p, tickets + kOsMaxPasses,
+                           kout, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
     }
     *keys_out = kin;
     *vals_out = vin;
@@ -1889,8 +2011,8 @@ static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
     uint32_t* vin;
     radix_sort_pairs(a.keys_a, a.keys_b, a.vals_a, a.vals_b, a.hist, a.hist_sums, a.count_dev, n,
                      a.bucket_bits + a.uid_bits, &kin, &vin, s);
-    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_pos, a.in_dir, a.in_tp, vin, a.count_dev, n,
-                       a.out_pos, a.out_dir, a.out_tp, a.out_index);
+    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_rec, vin, a.count_dev, n, a.out_pos, a.out_dir,
+                       a.out_tp, a.out_index);
     hipLaunchKernelGGL(k_map_cells, dim3(g), dim3(256), 0, s, kin, a.count_dev, n, a.uid_bits, a.cells);
 }
 
@@ -1939,11 +2061,10 @@ void launch_gather_fill(const PhotonMapView& pm, const float* q, int nq, const u
 namespace {
 constexpr size_t slice(size_t bytes) { return (bytes + 255) & ~size_t(255); }
 constexpr size_t kNaP = kNoAllocMaxPhotons;
-constexpr size_t kNaHist = 256 * ((kNaP + kSortTile - 1) / kSortTile);
-constexpr size_t kNaSums = (kNaHist + kScanTile - 1) / kScanTile + 1;
-constexpr size_t kNaInPos = 0;
-constexpr size_t kNaInUid = kNaInPos + slice(16 * kNaP);
-constexpr size_t kNaKeysA = kNaInUid + slice(4 * kNaP);
+constexpr size_t kNaHist = onesweep_scratch_u32(kNaP);
+constexpr size_t kNaSums = 1;
+constexpr size_t kNaInRec = 0;
+constexpr size_t kNaKeysA = kNaInRec + slice(16 * kPhotonRecord * kNaP);
 constexpr size_t kNaKeysB = kNaKeysA + slice(8 * kNaP);
 constexpr size_t kNaValsA = kNaKeysB + slice(8 * kNaP);
 constexpr size_t kNaValsB = kNaValsA + slice(4 * kNaP);
@@ -1984,11 +2105,13 @@ static unsigned char* scratch_address(const char** err) {
     return addr[dev];
 }
 
-__global__ void k_na_photons(const float* p, uint32_t n, float4* pos, uint32_t* uid) {
+__global__ void k_na_photons(const float* p, uint32_t n, float4* rec) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    pos[i] = make_float4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0f);
-    uid[i] = i;
+    float4* r = rec + kPhotonRecord * (size_t)i;
+    r[0] = make_float4(p[3 * (size_t)i], p[3 * (size_t)i + 1], p[3 * (size_t)i + 2], 0.0f);
+    r[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    r[2] = make_float4(0.0f, 0.0f, __uint_as_float(i), 0.0f);
 }
 
 // offsets[i] = counts[0] + ... + counts[i - 1] for i <= n (one workgroup:
@@ -2025,13 +2148,9 @@ long long gather_sets_noalloc(const NoAllocGather& a, hipStream_t s, const char*
     }
     unsigned char* base = scratch_address(err);
     if (!base) return -1;
-    float4* in_pos = reinterpret_cast<float4*>(base + kNaInPos);
-    uint32_t* in_uid = reinterpret_cast<uint32_t*>(base + kNaInUid);
+    float4* in_rec = reinterpret_cast<float4*>(base + kNaInRec);
     MapBuildArgs m{};
-    m.in_pos = in_pos;
-    m.in_dir = in_pos;
-    m.in_tp = reinterpret_cast<const float2*>(in_pos);
-    m.in_uid = in_uid;
+    m.in_rec = in_rec;
     m.count = a.count;
     for (int k = 0; k < 3; ++k) m.origin[k] = a.pm.origin[k];
     m.inv_cell = a.pm.inv_cell;
@@ -2051,7 +2170,7 @@ long long gather_sets_noalloc(const NoAllocGather& a, hipStream_t s, const char*
     m.cells = reinterpret_cast<uint2*>(base + kNaCells);
     uint32_t* counts = reinterpret_cast<uint32_t*>(base + kNaCounts);
     if (a.count) hipLaunchKernelGGL(k_na_photons, dim3((a.count + 255) / 256), dim3(256), 0, s, a.positions, a.count,
-                                    in_pos, in_uid);
+                                    in_rec);
     build_photon_map(m, s);  // clears every bucket range with a fill kernel first
     PhotonMapView pm = a.pm;
     pm.pos = m.out_pos;
