@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Vertices.h:17-27)
+PHOTON_RECORD_BYTES = 48  # the light pass's photon record: the 40 B + light path index and length
 
 
 def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> dict:
@@ -37,8 +38,11 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
     traversal: one node per visit (128 B: the four child boxes and refs of the
     4-wide BVH; 64 B for the binary one); 36 B per triangle
     test (three points); 40 B per closest hit (three normals + material).
-    light pass: + 40 B per stored photon.  photon map: per radix pass 32 B per
-    photon, + 88 B permute and 28 B key generation per photon.  camera trace:
+    light pass: + 48 B per stored photon (its record).  photon map, per photon:
+    44 B key generation (position and path read, key and value written), 8 B
+    digit counting, 24 B per radix pass (the pair read and written), 96 B
+    permutation (record and index read, sorted photon and index written),
+    8 B cell ranges.  camera trace:
     + 80 B per merge query written + 8 B per path record.  gather: 80 B per
     query read, 12 B per result, 8 B per cell lookup, and per photon: with one
     query per wave (grouped = False) 12 B per photon tested and 28 B per photon
@@ -46,7 +50,7 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
     queries sharing a window reads it once).  resolve: 8 B per path, 12 B per
     query, 24 B per film update."""
     light = (node_bytes * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
-             + PHOTON_BYTES * st["photons"])
+             + PHOTON_RECORD_BYTES * st["photons"])
     trace = (node_bytes * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
              + 80 * st["gathers"] + 8 * st["camera_paths"])
     if grouped:
@@ -55,7 +59,7 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
         gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
                   + 28 * st["photons_accepted"])
     resolve = 32 * st["camera_paths"] + 12 * st["gathers"]
-    photon_map = st["photons"] * (32 * st["sort_passes"] + 88 + 28)
+    photon_map = st["photons"] * (44 + 8 + 24 * st["sort_passes"] + 96 + 8)
     return {"light_pass": light, "photon_map": photon_map, "camera_trace": trace, "gather": gather,
             "resolve": resolve}
 
@@ -314,28 +318,35 @@ def guide_steps(a) -> int:
 
 
 def graph_leg(a):
-    """Iteration time of a launch-bound scene (64 x 64 pixels, 65,536 light
-    paths, 5,120-triangle glass sphere) with direct launches and with the
-    captured iteration graph."""
+    """Iteration time of a small scene (64 x 64 pixels, 65,536 light paths,
+    5,120-triangle glass sphere) with direct launches and with the captured
+    iteration graph, without and with the overlapped camera trace.  Without
+    overlap the whole iteration is one graph; with it the light pass and the
+    trace stay direct launches beside each other.  At this size the iteration
+    is bounded by the latency of the longest light and camera paths (paths
+    caught in the glass), not by launches."""
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Engine
 
     out = {"workload": "glass icosphere (5,120 tris) over a plane, 64x64, 65,536 light paths/iteration"}
-    for graph in (0, 1):
-        eng = Engine(scenes.c0_scene(64, 64, subdiv=4, samples=1), photons=65536, seed=0, write_exr=0, graph=graph)
-        eng.prepare()
-        for _ in range(5):
-            eng.render_iteration()
-        n = 200
-        t0 = time.perf_counter()
-        for _ in range(n):
-            eng.render_iteration()
-        out["ms_per_iteration_graph" if graph else "ms_per_iteration_direct"] = round(
-            (time.perf_counter() - t0) * 1e3 / n, 4)
-        if graph:
-            out["graph_captures"] = int(eng.get("graph_captures"))
-        eng.release()
-    out["speedup"] = round(out["ms_per_iteration_direct"] / out["ms_per_iteration_graph"], 3)
+    for overlap in (0, 1):
+        for graph in (0, 1):
+            eng = Engine(scenes.c0_scene(64, 64, subdiv=4, samples=1), photons=65536, seed=0, write_exr=0,
+                         graph=graph, overlap=overlap)
+            eng.prepare()
+            for _ in range(5):
+                eng.render_iteration()
+            n = 200
+            t0 = time.perf_counter()
+            for _ in range(n):
+                eng.render_iteration()
+            key = f"ms_per_iteration_{'graph' if graph else 'direct'}_overlap{overlap}"
+            out[key] = round((time.perf_counter() - t0) * 1e3 / n, 4)
+            if graph:
+                out[f"graph_captures_overlap{overlap}"] = int(eng.get("graph_captures"))
+            eng.release()
+        out[f"speedup_overlap{overlap}"] = round(out[f"ms_per_iteration_direct_overlap{overlap}"] /
+                                                out[f"ms_per_iteration_graph_overlap{overlap}"], 3)
     print(f"[bench] graph: {out}", file=sys.stderr, flush=True)
     return out
 

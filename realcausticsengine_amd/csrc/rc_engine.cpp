@@ -158,6 +158,8 @@ struct Engine {
     DevBuf<uint32_t> cam_counters;
     uint32_t* h_nq = nullptr;  // pinned: the overlapped trace's query count
     uint32_t* h_count = nullptr;  // pinned: the light pass's photon count
+    DevBuf<float> film_prev;         // whole-iteration graph: the film before the replay
+    DevBuf<uint32_t> graph_count;    // whole-iteration graph: the light pass's photon count
     int device_cus = 0;
     uint32_t last_queries = 0;
     GuideModel guide;  // learned emission proposal (option "guide")
@@ -927,13 +929,33 @@ bool ensure_guide_buffers(Engine& e, hipStream_t s = nullptr) {
     return gd.train_x.alloc((size_t)n * kGuideRowFloats) && gd.train_bin.alloc(n) && gd.train_w.alloc(n);
 }
 
+// The light pass's arguments for one iteration (single scene, no event counts).
+LightPassArgs light_pass_args(Engine& e, int iteration) {
+    LightPassArgs a{};
+    a.sc = scene_view(e);
+    a.k0 = (uint32_t)e.seed;
+    a.k1 = (uint32_t)(e.seed >> 32);
+    a.iteration = (uint32_t)iteration;
+    a.stream = STREAM_LIGHT;
+    a.n_paths = (uint32_t)light_paths(e);
+    a.path_chunk = e.path_chunk;
+    a.refill_min = e.light_refill;
+    a.path_counter = e.counters.p;
+    a.photon_counter = e.counters.p + 1;
+    a.capacity = e.capacity;
+    a.out_rec = e.ph_rec.p;
+    a.guide = guide_light_view(e, iteration);
+    return a;
+}
+
 // Coherent order of this launch's light paths (option light_order), or null.
+bool light_order_buffers(Engine& e, size_t n) {
+    return e.lk_a.alloc(n) && e.lk_b.alloc(n) && e.lv_a.alloc(n) && e.lv_b.alloc(n) &&
+           e.l_hist.alloc(photon_map_hist_entries((uint32_t)n)) && e.l_hist_sums.alloc(photon_map_sum_entries((uint32_t)n));
+}
 const uint32_t* light_path_order(Engine& e, const LightPassArgs& a, hipStream_t s) {
     if (!e.light_order || a.mlt_state) return nullptr;
-    size_t n = a.n_paths;
-    if (!e.lk_a.alloc(n) || !e.lk_b.alloc(n) || !e.lv_a.alloc(n) || !e.lv_b.alloc(n) ||
-        !e.l_hist.alloc(photon_map_hist_entries((uint32_t)n)) || !e.l_hist_sums.alloc(photon_map_sum_entries((uint32_t)n)))
-        return nullptr;
+    if (!light_order_buffers(e, a.n_paths)) return nullptr;
     QuerySortBuffers b{e.lk_a.p, e.lk_b.p, e.lv_a.p, e.lv_b.p, e.l_hist.p, e.l_hist_sums.p};
     return sort_light_paths(a, b, s);
 }
@@ -962,21 +984,8 @@ bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::functi
         launch_fill_u32(e.counters.p, 0u, 4, e.stream);
         if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
         if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
-        LightPassArgs a{};
-        a.sc = scene_view(e);
-        a.k0 = (uint32_t)e.seed;
-        a.k1 = (uint32_t)(e.seed >> 32);
-        a.iteration = (uint32_t)iteration;
-        a.stream = STREAM_LIGHT;
-        a.n_paths = (uint32_t)N;
-        a.path_chunk = e.path_chunk;
-        a.refill_min = e.light_refill;
-        a.path_counter = e.counters.p;
-        a.photon_counter = e.counters.p + 1;
-        a.capacity = e.capacity;
-        a.out_rec = e.ph_rec.p;
+        LightPassArgs a = light_pass_args(e, iteration);
         a.counters = e.count_events ? e.stats_dev.p : nullptr;
-        a.guide = guide_light_view(e, iteration);
         if (e.mlt) {
             a.mlt_state = e.mlt_state.p;
             // at most mlt_chains chains of at least 64 indices each (the mutation size adapts along a chain)
@@ -1323,13 +1332,18 @@ bool capture_graph(hipStream_t st, const std::function<void()>& body, hipGraphEx
     return true;
 }
 
-// The iteration with its photon map and camera pass replayed from a captured
-// hipGraph.  The light pass stays a direct launch: its photon count is read
-// back and checked against the capacity before the replay (a grown buffer
-// means a new capture).  Inside the graph every count comes from device
-// memory and the per-iteration values (iteration, radius, grid, VM
-// normalisation) from d_iter, copied in before each replay; the query buffer
-// holds the most queries a camera pass can make (8 per path).  With option
+// The iteration replayed from captured hipGraphs.  Inside a graph every count
+// comes from device memory and the per-iteration values (iteration, radius,
+// grid, VM normalisation) from d_iter, copied in before each replay; the
+// query buffer holds the most queries a camera pass can make (8 per path).
+// Plain light pass, no overlap: the whole iteration is one graph -- a copy of
+// the film, the light pass, the photon map, the camera pass and the resolve
+// -- and the host waits once, after it, for the photon count; when the
+// photons overflowed their buffer the film is restored from the copy, the
+// buffers grow (a new capture) and the iteration runs again.  Otherwise the
+// light pass stays a direct launch (MLT and the learned proposal set
+// per-iteration host values), its photon count is read back and checked
+// before the replay of the photon map and the camera pass.  With option
 // overlap the camera trace is a direct launch on stream2 beside the light
 // pass, a graph of the query sort replays behind it on stream2, and on stream
 // a graph of the photon map replays after the light pass, then, once stream
@@ -1367,10 +1381,27 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         return true;
     };
     const bool overlapped = e.overlap && !e.mlt;
+    const bool whole = !overlapped && !e.mlt && !guide_active(e);
     CameraPassArgs ca{};
     if (overlapped && !(copy_iter() && camera_trace_async(e, iteration, norm, e.image.p, ca))) return false;
-    if (!run_light_pass(e, iteration, count)) return false;
-    if (!overlapped && !copy_iter()) return false;
+    const size_t film = (size_t)e.W * e.H * 3;
+    if (whole) {  // the buffers the captured light pass writes, sized as run_light_pass sizes them
+        if (N <= 0 || N > (1ll << 31)) {
+            set_error("light path count out of range (1 .. 2^31)");
+            return false;
+        }
+        const uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
+        if (!ensure_photon_buffers(e, std::max(cap, e.capacity)) || !e.film_prev.alloc(film) ||
+            !e.graph_count.alloc(1) || (e.light_order && !light_order_buffers(e, (size_t)N)) ||
+            (e.light_emit && !e.emitted.alloc(3 * (size_t)N)))
+            return false;
+        if (!e.h_count) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_count), sizeof(uint32_t), hipHostMallocDefault));
+        if (!copy_iter()) return false;
+    } else {
+        if (!run_light_pass(e, iteration, count)) return false;
+        if (!overlapped && !copy_iter()) return false;
+    }
+    for (int attempt = 0;; ++attempt) {
     PhotonMapView& pm = mi.pm;
     pm.pos = e.s_pos.p;
     pm.dir_tr = e.s_dir.p;
@@ -1387,7 +1418,10 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
                                   (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile,
-                                  (uintptr_t)e.gather_block, (uintptr_t)overlapped};
+                                  (uintptr_t)e.gather_block, (uintptr_t)overlapped, (uintptr_t)whole,
+                                  (uintptr_t)light_blocks(e), (uintptr_t)e.light_refill, (uintptr_t)e.light_order,
+                                  (uintptr_t)e.light_emit, (uintptr_t)e.emitted.p, (uintptr_t)e.film_prev.p,
+                                  (uintptr_t)e.lk_a.p};
     if (!e.graph_exec || key != e.graph_key) {
         if (e.graph_exec) HIP_OK(hipGraphExecDestroy(e.graph_exec));
         if (e.graph_exec2) HIP_OK(hipGraphExecDestroy(e.graph_exec2));
@@ -1445,6 +1479,24 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         if (overlapped) {
             ok = capture_graph(st, [&]() { build_photon_map(m, st); }, e.graph_exec) &&
                  capture_graph(st, sort, e.graph_exec3) && capture_graph(st, merge, e.graph_exec2);
+        } else if (whole) {
+            ok = capture_graph(st,
+                               [&]() {
+                                   launch_copy_u32(reinterpret_cast<uint32_t*>(e.film_prev.p),
+                                                   reinterpret_cast<const uint32_t*>(e.image.p), film, st);
+                                   launch_fill_u32(e.counters.p, 0u, 4, st);
+                                   LightPassArgs la = light_pass_args(e, iteration);
+                                   la.dyn = e.d_iter.p;
+                                   if (e.light_emit) la.emitted = e.emitted.p;
+                                   la.path_order = light_path_order(e, la, st);
+                                   launch_light_pass(la, light_blocks(e), st);
+                                   launch_copy_u32(e.graph_count.p, e.counters.p + 1, 1, st);
+                                   build_photon_map(m, st);
+                                   launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
+                                   launch_camera_trace(a, camera_blocks(e), st);
+                                   camera();
+                               },
+                               e.graph_exec);
         } else {
             ok = capture_graph(st,
                                [&]() {
@@ -1468,6 +1520,10 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         HIP_OK(hipGraphLaunch(e.graph_exec3, e.stream2));
         HIP_OK(hipEventRecord(e.ev_qdone, e.stream2));
     }
+    if (whole) {  // no separate light pass: its span is empty
+        HIP_OK(hipEventRecord(e.ev[0], st));
+        HIP_OK(hipEventRecord(e.ev[1], st));
+    }
     HIP_OK(hipEventRecord(e.ev[2], st));
     HIP_OK(hipGraphLaunch(e.graph_exec, st));
     if (overlapped) {
@@ -1477,8 +1533,21 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     HIP_OK(hipEventRecord(e.ev[5], st));
     HIP_OK(hipMemcpyAsync(&e.last_queries, (overlapped ? e.cam_counters.p : e.counters.p) + 1, 4,
                           hipMemcpyDeviceToHost, st));
+    if (whole) HIP_OK(hipMemcpyAsync(e.h_count, e.graph_count.p, 4, hipMemcpyDeviceToHost, st));
     HIP_OK(hipStreamSynchronize(st));
     e.stats[RC_STAT_GATHERS] = e.last_queries;
+    if (!whole || *e.h_count <= e.capacity) break;
+    if (attempt > 0) {
+        set_error("photon buffer overflow");
+        return false;
+    }
+    // the photons overflowed their buffer: this iteration's film contribution is undone and the
+    // iteration runs again with larger buffers (a new capture)
+    launch_copy_u32(reinterpret_cast<uint32_t*>(e.image.p), reinterpret_cast<const uint32_t*>(e.film_prev.p), film,
+                    st);
+    if (!ensure_photon_buffers(e, (uint32_t)std::min<double>((double)*e.h_count * 1.25 + 1024, 4.0e9))) return false;
+    }
+    if (whole) count = *e.h_count;
     return true;
 }
 
@@ -1896,6 +1965,8 @@ void release(Engine*& e) {
     e->cam_counters.release();
     if (e->h_nq) (void)hipHostFree(e->h_nq);
     if (e->h_count) (void)hipHostFree(e->h_count);
+    e->film_prev.release();
+    e->graph_count.release();
     if (e->ev_cam) (void)hipEventDestroy(e->ev_cam);
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->ev_qdone) (void)hipEventDestroy(e->ev_qdone);

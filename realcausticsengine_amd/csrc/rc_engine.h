@@ -147,6 +147,8 @@ __device__ __forceinline__ int batch_scene_of(const uint32_t* starts, int n, uin
 // key below the bucket.
 constexpr int kPhotonRecord = 3;
 
+struct IterParams;
+
 struct LightPassArgs {
     SceneView sc;
     uint32_t k0, k1, iteration, stream;
@@ -164,6 +166,8 @@ struct LightPassArgs {
     // non-null: every slot's emission (light sample and proposal record) is made by a coherent kernel
     // before the persistent pass: 3 float4 per slot {o, tp.x}, {d, tp.y}, {tp.z, emits, 0, 0}
     float4* emitted;
+    // non-null: the iteration (Philox counter) is read here (a captured iteration graph)
+    const IterParams* dyn;
     // adaptive MLT light pass (non-null mlt_state selects it; single scene, plain proposal)
     uint4* mlt_state;     // kMltBlocks * mlt_chains explicit chain states, block-major
     uint32_t mlt_chains;  // Markov chains; chain c takes the path indices c, c + chains, ...
@@ -394,6 +398,8 @@ void launch_ingest(const IngestMesh* meshes, int n_meshes, uint32_t nv, uint32_t
 __host__ __device__ inline int ordered_float_bits(int bits) { return bits >= 0 ? bits : bits ^ 0x7fffffff; }
 
 void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s);
+// dst[i] = src[i] for n 32-bit words (a kernel: graph captures hold no memory-copy nodes).
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s);
 void launch_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* partials, hipStream_t s);
 
 }  // namespace rc

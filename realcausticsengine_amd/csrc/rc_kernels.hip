@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_light_emit(LightPassArgs a) {
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x; slot < a.n_paths; slot += stride) {
         LightState st;
-        PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
+        PathRng rng{a.k0, a.k1, 0u, a.dyn ? a.dyn->iteration : a.iteration, a.stream};
         const bool emits = emit_light_path<kBatch>(b, slot, st, rng);
         float4* e = a.emitted + 3 * (size_t)slot;
         e[0] = make_float4(st.o.x, st.o.y, st.o.z, st.tp.x);
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(LightPassArgs a) {
     uint32_t n_nodes = 0, n_tris = 0;
     LightState st;
     bool active = false, exhausted = false;
-    PathRng rng{a.k0, a.k1, 0u, a.iteration, a.stream};
+    PathRng rng{a.k0, a.k1, 0u, a.dyn ? a.dyn->iteration : a.iteration, a.stream};
     WavePool pool{a.path_chunk};
     for (;;) {
         // refill finished lanes
@@ -1491,6 +1491,17 @@ void launch_fill_u32(uint32_t* p, uint32_t v, size_t n, hipStream_t s) {
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, s, p, v, n);
 }
+__global__ void k_copy_u32(uint32_t* dst, const uint32_t* src, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+void launch_copy_u32(uint32_t* dst, const uint32_t* src, size_t n, hipStream_t s) {
+    if (!n) return;
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_copy_u32, dim3((unsigned)blocks), dim3(256), 0, s, dst, src, n);
+}
 
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
@@ -1953,7 +1964,7 @@ __device__ __forceinline__ uint32_t spread7x4(uint32_t v) {
 __global__ void k_light_keys(LightPassArgs a, unsigned long long* keys, uint32_t* vals) {
     uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n_paths) return;
-    PathRng rng{a.k0, a.k1, p, a.iteration, STREAM_LIGHT};
+    PathRng rng{a.k0, a.k1, p, a.dyn ? a.dyn->iteration : a.iteration, STREAM_LIGHT};
     uint32_t scene = 0;
     if (a.batch.n > 0) {
         scene = (uint32_t)batch_scene_of(a.batch.light_starts, a.batch.n, p);

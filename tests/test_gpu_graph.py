@@ -1,11 +1,13 @@
 """The captured iteration graph (option "graph") against direct launches.
 
-The graph replays the photon-map build and the camera pass with every count
-read from device memory and the per-iteration values from a device struct;
-the kernels and their data are the same as in direct launches, so each
-replay must leave the film bit-identical to the direct run's film.  Every
-replay is checked, with the direct light pass and a readback between
-replays."""
+With the plain light pass the whole iteration is one graph (film copy, light
+pass, photon map, camera pass, resolve) with every count read from device
+memory and the per-iteration values from a device struct; with the learned
+proposal the light pass stays a direct launch before the graph.  The kernels
+and their data are the same as in direct launches, so each replay must leave
+the film bit-identical to the direct run's film.  Every replay is checked,
+with a readback between replays; a photon buffer that overflows inside the
+graph undoes the iteration's film and runs it again with larger buffers."""
 import numpy as np
 import pytest
 
@@ -30,9 +32,10 @@ def _run(graph: int, iters: int, schedule=None, **opts):
     return films, caps
 
 
-def test_graph_replays_match_direct_launches():
-    direct, _ = _run(0, 6, photons=65536)
-    graphed, caps = _run(1, 6, photons=65536)
+@pytest.mark.parametrize("light_order", [0, 1])
+def test_graph_replays_match_direct_launches(light_order):
+    direct, _ = _run(0, 6, photons=65536, light_order=light_order)
+    graphed, caps = _run(1, 6, photons=65536, light_order=light_order)
     assert caps == 1
     for i in range(6):
         assert direct[i].sum() > 0
