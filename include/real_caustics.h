@@ -93,7 +93,10 @@ int main(void);
  * default; 1 = linear BVH over Morton codes built on the GPU, for scenes of
  * millions of triangles), "bvh_width" (4 = the tree collapsed to 4-wide
  * nodes -- on the host for the SAH tree, on the GPU for the linear BVH --
- * the default; 2 = binary nodes).
+ * the default; 2 = binary nodes), "device_ingest" (1 = the default: with
+ * bvh_build 1, init copies the raw Blender arrays to the GPU and converts
+ * them there -- the same triangles as the host ingest; init then needs the
+ * GPU; 0 = convert on the host).
  * Camera sampling: "camera_qmc" (1 = pixel jitter and the first bounce's
  * reflect / refract choice follow a randomly shifted low-discrepancy sequence
  * over iterations -- iteration i takes point i -- instead of independent
@@ -205,12 +208,17 @@ enum {
  * gather_blocks_per_cu, light_refill, light_order, camera_refill,
  * group_queries, gather_tile, gather_block, query_order, overlap. */
 enum { RC_TUNE_REPORT = 15 };
-/* Option "graph" = 1: after the light pass (a direct launch whose photon
- * count is checked on the host), the photon-map build and the camera pass of
- * each iteration replay one captured hipGraph, reading counts and
- * per-iteration values from device memory; the graph is captured again when
- * buffers grow or the light-path count changes (rc_get_option("graph_captures")).
- * The film is identical to the one direct launches give.  In a batch, scene
+/* Option "graph" = 1: with the plain light pass and the camera trace serial
+ * the whole iteration (light pass, photon map, camera pass, resolve) replays
+ * one captured hipGraph and the host waits once, after it: photons beyond
+ * the buffer restore the film from a copy taken in the graph, grow the
+ * buffers and run the iteration again.  Otherwise (MLT, a learned proposal,
+ * option overlap) the light pass is a direct launch whose photon count is
+ * checked on the host, then the photon-map build and the camera pass replay.
+ * Graphs read counts and per-iteration values from device memory and are
+ * captured again when buffers grow or the light-path count changes
+ * (rc_get_option("graph_captures")).  The film is identical to the one
+ * direct launches give.  In a batch, scene
  * 0's "graph" captures the batched map and camera pass of all scenes (the
  * per-scene values come from device tables refilled before each replay;
  * RC_STAT_GRAPH_CAPTURES counts the captures). */
@@ -256,7 +264,7 @@ long long rc_merge_sets(int iteration, const float *photons, long long count, fl
  * HIP graph capture set-up, tight serving loops, harnesses that forbid
  * hipMalloc).  positions, queries, offsets (nq + 1 int64) and indices (cap)
  * are DEVICE pointers; the photon map and the per-query counts use a fixed
- * scratch array of about 25 MB that the library holds on each GPU until the
+ * scratch array of about 32 MB that the library holds on each GPU until the
  * process ends, so count and nq are at most 262144 each (larger sizes are an
  * error).  The scene must be prepared (rc_prepare) first; the grid is the one
  * rc_gather_sets builds, so the sets are the same.  Returns the total set
