@@ -201,6 +201,12 @@ def uneven_batch(a):
     return out
 
 
+# Launch shapes the batch autotuner chose for the 16 at-scale scenes in one launch (the sample leg uses them).
+AT_SCALE_LAUNCH = ("path_chunk=512,light_blocks_per_cu=12,camera_blocks_per_cu=16,cell_scale=0.67,"
+                   "gather_blocks_per_cu=24,light_refill=16,light_order=1,camera_refill=32,group_queries=8,"
+                   "gather_tile=0,gather_block=2,query_order=1,overlap=0")
+
+
 def at_scale(a) -> int:
     """The at-scale batched + learned workload (scenes.c4_batch: displaced glass
     meshes and water heightfields of 2M-10M triangles, four lights each, 2048^2,
@@ -210,21 +216,36 @@ def at_scale(a) -> int:
     launch holds at most 2^31 light paths), one batch resident at a time; each batch's launch
     shapes are autotuned (untimed) unless --launch fixes them; prints one JSON
     line."""
+    print(json.dumps({"at_scale": at_scale_run(a.at_scale, a.scale_group, a.scale_res, a.scale_photons,
+                                               a.scale_steps, a.launch, a.no_autotune)}), flush=True)
+    return 0
+
+
+def at_scale_sample(a) -> dict:
+    """The first four at-scale scenes at their full size (13.9 M triangles,
+    2048^2, 64 x 2^20 light paths each, proposals trained every iteration) in
+    one launch per pass, with the launch shapes tuned for the whole set (no
+    tuning here): a sample of the at-scale workload that fits the default run."""
+    out = at_scale_run(4, 4, 2048, 64 << 20, 2, AT_SCALE_LAUNCH, True)
+    out["sample"] = "scenes 0-3 of the 16 at-scale scenes, launch shapes fixed (bench.py --at-scale 16 runs all)"
+    return out
+
+
+def at_scale_run(n, group, res, photons, steps, launch, no_autotune) -> dict:
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import GUIDE_STEP_FLOPS_PER_ROW, Batch
 
-    n, group = a.at_scale, a.scale_group
     opts = dict(write_exr=0, bvh_build=1, guide=1, guide_batch=65536, guide_warmup=1, compat_point_light=0,
                 compat_light_pdf=0, light_order=1)  # coherent light rays: -22 % light pass on these scenes
     fixed = {}
-    for kv in filter(None, a.launch.split(",")):  # fixed launch shapes (every scene; scene 0's serve the batch)
+    for kv in filter(None, launch.split(",")):  # fixed launch shapes (every scene; scene 0's serve the batch)
         k, v = kv.split("=")
         fixed[k.strip()] = float(v)
     opts.update(fixed)
     groups, totals = [], dict(light=0, camera=0, tris=0, s_iter=0.0, ms_guide=0.0, guide_rows=0.0, prep=0.0)
     for g0 in range(0, n, group):
         t = time.perf_counter()
-        cases = [scenes.c4_scene(s, a.scale_res, a.scale_photons) for s in range(g0, min(n, g0 + group))]
+        cases = [scenes.c4_scene(s, res, photons) for s in range(g0, min(n, g0 + group))]
         gen_s = time.perf_counter() - t
         t = time.perf_counter()
         b = Batch([sc for sc, _ in cases], [dict(opts, photons=p, seed=g0 + i) for i, (_, p) in enumerate(cases)])
@@ -234,19 +255,19 @@ def at_scale(a) -> int:
         print(f"[bench] at scale: scenes {g0}..{g0 + len(cases) - 1}, {tris} triangles, generated in {gen_s:.1f} s, "
               f"prepared in {prep_s:.1f} s", file=sys.stderr, flush=True)
         b.render_iteration()  # warm-up (buffers; the proposals' first step)
-        tuned = {"fixed": a.launch} if fixed else None
-        if not fixed and not a.no_autotune:  # launch shapes for this batch (untimed; renders into scratch films)
+        tuned = {"fixed": launch} if fixed else None
+        if not fixed and not no_autotune:  # launch shapes for this batch (untimed; renders into scratch films)
             t = time.perf_counter()
             tuned = {k: round(v, 3) for k, v in b.autotune(repeats=1).items()}
             print(f"[bench]   batch autotune ({time.perf_counter() - t:.1f} s): {tuned}", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         ms_guide = rows = 0.0
-        for _ in range(a.scale_steps):
+        for _ in range(steps):
             b.render_iteration()
             st = b.stats()
             ms_guide += st["ms_guide"]
             rows += st["guide_rows"]
-        per_it = (time.perf_counter() - t0) / a.scale_steps
+        per_it = (time.perf_counter() - t0) / steps
         # one counting iteration after the timed ones (its radius is a step smaller, so its counts are a
         # slight underestimate of the timed iterations'): algorithmic bytes over the last timed kernel times
         b.set(0, "count_events", 1)
@@ -265,7 +286,7 @@ def at_scale(a) -> int:
                           round(cst["photons_tested"] / max(1, cst["gathers"]), 1),
                           "algorithmic_gb_per_s": round(alg["gather"] / (st["ms_gather"] * 1e-3) / 1e9, 1)},
                "light_pass_algorithmic_gb_per_s": round(alg["light_pass"] / (st["ms_light"] * 1e-3) / 1e9, 1),
-               "ms_guide_per_iteration": round(ms_guide / a.scale_steps, 3), "prepare_s": round(prep_s, 2),
+               "ms_guide_per_iteration": round(ms_guide / steps, 3), "prepare_s": round(prep_s, 2),
                "autotune": tuned}
         print(f"[bench]   {rec}", file=sys.stderr, flush=True)
         groups.append(rec)
@@ -273,20 +294,18 @@ def at_scale(a) -> int:
         totals["camera"] += camera
         totals["tris"] += tris
         totals["s_iter"] += per_it
-        totals["ms_guide"] += ms_guide / a.scale_steps
-        totals["guide_rows"] += rows / a.scale_steps
+        totals["ms_guide"] += ms_guide / steps
+        totals["guide_rows"] += rows / steps
         totals["prep"] += prep_s
     gflop = GUIDE_STEP_FLOPS_PER_ROW * totals["guide_rows"] / 1e9
-    out = {"at_scale": {
-        "scenes": n, "scenes_per_launch": group, "triangles": totals["tris"], "resolution": a.scale_res,
+    return {
+        "scenes": n, "scenes_per_launch": group, "triangles": totals["tris"], "resolution": res,
         "light_paths_per_iteration": totals["light"], "camera_paths_per_iteration": totals["camera"],
         "s_per_iteration_all_scenes": round(totals["s_iter"], 3),
         "photon_paths_per_s": round(totals["light"] / totals["s_iter"], 1),
         "ms_guide_per_iteration_all_scenes": round(totals["ms_guide"], 2),
         "guide_gflop_per_iteration": round(gflop, 2),
-        "prepare_s_total": round(totals["prep"], 1), "steps": a.scale_steps, "groups": groups}}
-    print(json.dumps(out), flush=True)
-    return 0
+        "prepare_s_total": round(totals["prep"], 1), "steps": steps, "groups": groups}
 
 
 def guide_steps(a) -> int:
@@ -507,6 +526,7 @@ def main() -> int:
                     "e.g. path_chunk=512,light_blocks_per_cu=8,gather_blocks_per_cu=32")
     ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")
     ap.add_argument("--no-rough", action="store_true", help="skip the rough-glass leg")
+    ap.add_argument("--no-scale", action="store_true", help="skip the at-scale sample leg (four at-scale scenes)")
     ap.add_argument("--guide-steps", type=int, default=0,
                     help="only time this many training steps of the learned proposal (65,536 rows each)")
     ap.add_argument("--at-scale", type=int, default=0,
@@ -628,6 +648,7 @@ def main() -> int:
         batch = uneven_batch(a)
     graph = None if a.no_graph else graph_leg(a)
     rough = None if a.no_rough else rough_glass_leg(a, launch)
+    scale = None if a.no_scale else at_scale_sample(a)
     ttr = None
     if not a.no_ttr:
         ttr = time_to_rmse(a, a.photons / cpu["value"] if cpu else None,
@@ -659,6 +680,7 @@ def main() -> int:
         "uneven_batch": batch,
         "iteration_graph": graph,
         "rough_glass": rough,
+        "at_scale_sample": scale,
         "detail": extra,
     }
     print(json.dumps(line), flush=True)
