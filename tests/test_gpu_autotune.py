@@ -66,3 +66,18 @@ def test_batch_autotune_keeps_films():
         assert np.array_equal(b.image(k, scaled=False), before[k])
     assert b.render_iteration() == 1
     b.release()
+
+
+def test_tuned_render_matches_the_oracle(oracle):
+    # the tuned launch shapes against the oracle's render of the same two iterations
+    args = scenes.c0_scene(32, 32, subdiv=3, samples=2)
+    eng = Engine(args, photons=16384, seed=11, write_exr=0)
+    eng.prepare()
+    eng.render_iteration()
+    _check_report(eng.autotune(repeats=1))
+    eng.render_iteration()
+    img_g = eng.image(scaled=True)
+    eng.release()
+    img_o = oracle.OracleScene(args, photons=16384, seed=11).render(2)
+    bad = np.abs(img_g - img_o) > 2e-4 * np.abs(img_o) + 1e-7
+    assert img_o.sum() > 0 and bad.mean() <= 0.002, bad.mean()

@@ -143,3 +143,20 @@ def test_batch_trains_each_scene_proposal():
         if k == 2:
             assert np.array_equal(img, alone[k])  # the unguided scene is untouched by its neighbours' training
     b.release()
+
+
+def test_batched_scenes_match_the_oracle(oracle):
+    # each scene of a batch against the oracle's own render of it (not only against the scene alone)
+    cases = [(scenes.c0_scene(32, 32, subdiv=3, samples=2), dict(photons=16384, seed=11)),
+             (scenes.materials_scene(40, 24), dict(photons=16384, seed=12)),
+             (scenes.water_scene(32, 24, 16, samples=2), dict(photons=8192, seed=13))]
+    b = Batch([a for a, _ in cases], [dict(o, write_exr=0) for _, o in cases]).prepare()
+    b.render_iteration()
+    b.render_iteration()
+    for k, (args, opts) in enumerate(cases):
+        img_g = b.image(k, scaled=True)
+        img_o = oracle.OracleScene(args, **opts).render(2)
+        bad = np.abs(img_g - img_o) > 2e-4 * np.abs(img_o) + 1e-7
+        print(f"scene {k}: sum {img_g.sum():.6g} vs oracle {img_o.sum():.6g}, outside {bad.mean():.4f}", flush=True)
+        assert img_o.sum() > 0 and bad.mean() <= 0.002, (k, bad.mean())
+    b.release()

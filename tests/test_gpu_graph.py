@@ -55,3 +55,19 @@ def test_graph_with_learned_proposal_runs():
     films, caps = _run(1, 5, photons=65536, guide=1, guide_warmup=1)
     assert caps == 1
     assert np.all(np.isfinite(films[-1])) and films[-1].sum() > 0
+
+
+@pytest.mark.parametrize("overlap", [0, 1])
+def test_graph_render_matches_the_oracle(oracle, overlap):
+    # the replayed iteration against the oracle's render (whole-iteration graph without overlap)
+    args = scenes.c0_scene(32, 32, subdiv=3, samples=2)
+    eng = Engine(args, photons=16384, seed=11, write_exr=0, graph=1, overlap=overlap)
+    eng.prepare()
+    eng.render_iteration()
+    eng.render_iteration()
+    img_g = eng.image(scaled=True)
+    assert eng.get("graph_captures") >= 1
+    eng.release()
+    img_o = oracle.OracleScene(args, photons=16384, seed=11).render(2)
+    bad = np.abs(img_g - img_o) > 2e-4 * np.abs(img_o) + 1e-7
+    assert img_o.sum() > 0 and bad.mean() <= 0.002, bad.mean()
