@@ -426,6 +426,31 @@ def time_to_rmse(a, cpu_s_per_iteration, cpu_all_s_per_iteration=None):
         if mfma:
             legs["guide_training_step"] = mfma
         print(f"[bench]   {legs[f'{name}_tuned']}", file=sys.stderr, flush=True)
+    # batched frames: the tuned plain estimator with stratified camera samples, K consecutive iterations of
+    # the scene per batched launch (convergence.FrameBatch)
+    for guided in (False,):
+        name = ("guided" if guided else "plain") + "_qmc_batched"
+        print(f"[bench] time-to-RMSE: {name}, {a.ttr_frames} frames per launch, tuned light-path count",
+              file=sys.stderr, flush=True)
+        args = scenes.water_scene(a.grid, 1024, 1024, alpha=1.0, samples=1, base_radius=0.01)
+        opts = dict(photons=1 << 20, seed=0, write_exr=0, compat_rows=0, camera_qmc=1)
+        if guided:
+            opts.update(guide=1, guide_warmup=2, guide_train_every=a.guide_train_every)
+        fb = convergence.FrameBatch(args, a.ttr_frames, opts)
+        tuner = convergence.PhotonBudgetTuner(1 << 20, 1 << 18, pilots=6)
+        run = convergence.run_tuned(fb, ref, target, a.ttr_max_iterations, tuner, label=f"{name}",
+                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune)
+        fb.release()
+        f = tuner.fit
+        leg = run.summary(target)
+        leg["frames_per_launch"] = a.ttr_frames
+        if leg["iterations"]:
+            leg["frame_iterations"] = leg["iterations"] * a.ttr_frames
+        if f:
+            leg["fit"] = {"C_per_launch": round(f.C, 6), "P_per_path": f.P, "T0_ms": round(f.T0, 3),
+                          "ms_per_Mpath": round(f.t_per_path * 1e6, 4), "chosen_per_frame": f.best}
+        legs[name] = leg
+        print(f"[bench]   {leg}", file=sys.stderr, flush=True)
     plain = legs["plain_16M"]
     it_plain = plain["iterations"] or plain.get("extrapolated_iterations")
     out = {"target": target, "definition": "RMSE over reference pixels and channels of (img - ref) / mean(ref)",
@@ -520,6 +545,8 @@ def main() -> int:
     ap.add_argument("--ttr-max-iterations", type=int, default=3000)
     ap.add_argument("--ttr-max-seconds", type=float, default=75.0)
     ap.add_argument("--guide-train-every", type=int, default=8)
+    ap.add_argument("--ttr-frames", type=int, default=8,
+                    help="independent frames of the scene per batched launch in the batched time-to-RMSE legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-scene batch leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
     ap.add_argument("--launch", default="", help="fixed launch shapes instead of autotuning, "

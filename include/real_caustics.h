@@ -97,6 +97,11 @@ int main(void);
  * bvh_build 1, init copies the raw Blender arrays to the GPU and converts
  * them there -- the same triangles as the host ingest; init then needs the
  * GPU; 0 = convert on the host).
+ * Iteration indices: "iteration_offset", "iteration_stride" (the k-th
+ * iteration rendered uses iteration index offset + k * stride for its random
+ * numbers, radius and stratified samples; defaults 0 and 1.  K copies of a
+ * scene in a batch with offsets 0 .. K-1 and stride K render consecutive
+ * iterations of one estimator, K per batched launch).
  * Camera sampling: "camera_qmc" (1 = pixel jitter and the first bounce's
  * reflect / refract choice follow a randomly shifted low-discrepancy sequence
  * over iterations -- iteration i takes point i -- instead of independent

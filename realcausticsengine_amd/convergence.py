@@ -187,6 +187,69 @@ class PhotonBudgetTuner:
             self._deltas = {}
 
 
+class FrameBatch:
+    """K frames of one scene rendered by the same launches.
+
+    The frames are copies of the scene in one batch (rc_batch_*) with the
+    same Philox key; frame k renders the iteration indices k, k + K, k + 2K,
+    ... (options iteration_offset = k, iteration_stride = K), so the first B
+    batched iterations render iterations 0 .. K B - 1 of one estimator -- the
+    same samples, and with camera_qmc the same stratified sequence, as K B
+    iterations of the scene alone -- with one light-pass, photon-map,
+    camera-trace and merge launch per pass for all K.  With a fixed merge
+    radius every iteration has the same expectation, so the running image is
+    the mean of the K accumulated films.  It offers the calls run_tuned and
+    PhotonBudgetTuner make of an Engine: `photons` is the light-path count of
+    each frame, `ms_total` the wall time of the last batched iteration, and
+    image() the mean of the frames' accumulated films (one batched iteration
+    then counts as one iteration of that mean)."""
+
+    def __init__(self, args, frames: int, options: dict):
+        from .engine import Batch
+
+        self.frames = frames
+        opts = [dict(options, iteration_offset=k, iteration_stride=frames) for k in range(frames)]
+        self.batch = Batch([args] * frames, opts)
+        self.batch.prepare()
+        self.photons = int(options.get("photons", 0))
+        self.last_ms = 0.0
+        self.W, self.H = args.camera.width, args.camera.height
+
+    def set(self, key: str, value) -> None:
+        for k in range(self.frames):
+            self.batch.set(k, key, value)
+        if key == "photons":
+            self.photons = int(value)
+
+    def get(self, key: str) -> float:
+        if key == "photons":
+            return float(self.photons)
+        return self.batch.lib.rc_get_option(key.encode())
+
+    def render_iteration(self) -> int:
+        t0 = time.perf_counter()
+        rc = self.batch.render_iteration()
+        self.last_ms = (time.perf_counter() - t0) * 1e3
+        return rc
+
+    def image(self, scaled: bool = False) -> np.ndarray:
+        acc = np.zeros((self.H, self.W, 3), dtype=np.float64)
+        for k in range(self.frames):
+            acc += self.batch.image(k, scaled=scaled)
+        return (acc / self.frames).astype(np.float32)
+
+    def stats(self) -> dict:
+        st = self.batch.stats()
+        st["ms_total"] = self.last_ms
+        return st
+
+    def autotune(self, repeats: int = 2) -> dict:
+        return self.batch.autotune(repeats=repeats)
+
+    def release(self) -> None:
+        self.batch.release()
+
+
 def run_tuned(eng: Engine, ref: SubsetReference, target: float, max_iterations: int, tuner: PhotonBudgetTuner,
               label: str = "", max_seconds: float = 1e9, autotune: bool = False) -> ConvergenceRun:
     """run_to_target with the tuner choosing the light-path count.  With

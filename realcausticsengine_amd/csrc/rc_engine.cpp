@@ -128,6 +128,10 @@ struct Engine {
     int query_order = 1;    // 1: the gather takes the merge queries in spatial (Morton) order
     int light_order = 0;    // 1: the light pass traces its paths sorted by emission position and direction
     int camera_qmc = 0;     // 1: camera jitter and first-bounce choice stratified across iterations
+    // the k-th iteration rendered samples iteration index iteration_offset + k * iteration_stride (Philox
+    // counter, radius schedule, stratified sample index): K frames of one scene with offsets 0..K-1 and
+    // stride K together render iterations 0, 1, 2, ... of one estimator
+    int iteration_offset = 0, iteration_stride = 1;
     DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
     DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
     int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
@@ -849,6 +853,9 @@ bool prepare(Engine& e) {
 }
 
 long long light_paths(const Engine& e) { return e.photons > 0 ? e.photons : (long long)e.W * e.H; }
+
+// Iteration index of the k-th iteration a scene renders (options iteration_offset / iteration_stride).
+int sample_iteration(const Engine& e, int k) { return e.iteration_offset + k * e.iteration_stride; }
 
 // Visibility normalisation of the adaptive MLT light pass (Vertex_Merging.cpp:278-282):
 // every photon's throughput times (visible uniform paths) / N, applied to the merge.
@@ -1801,7 +1808,7 @@ bool autotune(Engine& e, int repeats, double* report) {
     if (!prepare(e)) return false;
     DevBuf<float> film;
     if (!film.alloc((size_t)e.W * e.H * 3)) return false;
-    const int it = e.iterations_done, counting = e.count_events, overlap = e.overlap;
+    const int it = sample_iteration(e, e.iterations_done), counting = e.count_events, overlap = e.overlap;
     e.count_events = 0;
     e.overlap = 0;  // the other knobs are timed with the camera trace serial
     Engine* pe = &e;
@@ -2005,6 +2012,8 @@ void copy_options(const Engine& from, Engine& to) {
     to.query_order = from.query_order;
     to.light_order = from.light_order;
     to.camera_qmc = from.camera_qmc;
+    to.iteration_offset = from.iteration_offset;
+    to.iteration_stride = from.iteration_stride;
     to.group_queries = from.group_queries;
     to.gather_tile = from.gather_tile;
     to.gather_block = from.gather_block;
@@ -2300,7 +2309,8 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     for (size_t i = 0; i < n; ++i) {
         Engine& e = *b.scenes[i];
         float radius, r2, norm;
-        iteration_params(e, e.iterations_done, radius, r2, norm);
+        const int it = sample_iteration(e, e.iterations_done);
+        iteration_params(e, it, radius, r2, norm);
         long long N = light_paths(e);
         BatchScene& s = bs[i];
         s.light_base = (uint32_t)light;
@@ -2311,7 +2321,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         s.k1 = (uint32_t)(e.seed >> 32);
         s.vm_norm = norm;
         s.bucket_base = (uint32_t)buckets;
-        s.iteration = (uint32_t)e.iterations_done;
+        s.iteration = (uint32_t)it;
         s.image = e.image.p;
         starts[i] = (uint32_t)light;
         starts[n + 1 + i] = (uint32_t)camera;
@@ -2474,6 +2484,18 @@ bool batch_iteration(Batch& b, bool scratch = false) {
             set_error("photon buffer overflow");
             return false;
         }
+    }
+    if (pm[0].pos != w.s_pos.p || pm[0].dir_tr != w.s_dir.p || pm[0].tp_gb != w.s_tp.p) {
+        // the overflow above reallocated the sorted photon arrays: the merge must read the new ones, not
+        // the freed ones the scenes' photon-map views were uploaded with (the overlapped trace and query
+        // sort read the same table on stream2, so they finish first)
+        if (overlapped) HIP_OK(hipStreamSynchronize(w.stream2));
+        for (size_t i = 0; i < n; ++i) {
+            pm[i].pos = w.s_pos.p;
+            pm[i].dir_tr = w.s_dir.p;
+            pm[i].tp_gb = w.s_tp.p;
+        }
+        HIP_OK(hipMemcpyAsync(b.d_pm.p, pm.data(), n * sizeof(PhotonMapView), hipMemcpyHostToDevice, st));
     }
     uint32_t nq = 0;
     if (graphed) {
@@ -2965,7 +2987,7 @@ __attribute__((visibility("default"))) int main(void) {
     if (e.debug_test == 1) {
         for (int it = e.iterations_done; it < e.samples; ++it) {
             if (g_cancel.load()) break;
-            if (!render_iteration(e, it)) return 3;
+            if (!render_iteration(e, sample_iteration(e, it))) return 3;
             e.iterations_done = it + 1;
         }
     } else if (e.debug_test >= 2 && e.debug_test <= 4) {
@@ -3008,6 +3030,8 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "query_order") g->query_order = (int)v ? 1 : 0;
     else if (k == "light_order") g->light_order = (int)v ? 1 : 0;
     else if (k == "camera_qmc") g->camera_qmc = (int)v ? 1 : 0;
+    else if (k == "iteration_offset") g->iteration_offset = std::max(0, std::min((int)v, 1 << 30));
+    else if (k == "iteration_stride") g->iteration_stride = std::max(1, std::min((int)v, 1 << 16));
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
@@ -3068,6 +3092,8 @@ __attribute__((visibility("default"))) double rc_get_option(const char* key) {
     if (k == "query_order") return g->query_order;
     if (k == "light_order") return g->light_order;
     if (k == "camera_qmc") return g->camera_qmc;
+    if (k == "iteration_offset") return g->iteration_offset;
+    if (k == "iteration_stride") return g->iteration_stride;
     if (k == "group_queries") return g->group_queries;
     if (k == "gather_tile") return g->gather_tile;
     if (k == "gather_block") return g->gather_block;
@@ -3120,7 +3146,7 @@ __attribute__((visibility("default"))) int rc_render_iteration(void) {
         return -2;
     }
     int it = g->iterations_done;
-    if (!render_iteration(*g, it)) return -1;
+    if (!render_iteration(*g, sample_iteration(*g, it))) return -1;
     g->iterations_done = it + 1;
     return it;
 }
@@ -3546,7 +3572,7 @@ __attribute__((visibility("default"))) int rc_batch_render_scene(int scene) {
     }
     Engine& e = *g_batch->scenes[scene];
     int it = e.iterations_done;
-    if (!render_iteration(e, it)) return -1;
+    if (!render_iteration(e, sample_iteration(e, it))) return -1;
     e.iterations_done = it + 1;
     return it;
 }

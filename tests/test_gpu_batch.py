@@ -160,3 +160,53 @@ def test_batched_scenes_match_the_oracle(oracle):
         print(f"scene {k}: sum {img_g.sum():.6g} vs oracle {img_o.sum():.6g}, outside {bad.mean():.4f}", flush=True)
         assert img_o.sum() > 0 and bad.mean() <= 0.002, (k, bad.mean())
     b.release()
+
+
+def test_frames_render_consecutive_iterations():
+    """K copies of one scene with iteration_offset k and iteration_stride K
+    (convergence.FrameBatch) render iterations 0 .. K B - 1 of the scene: each
+    frame's film equals the scene alone with the same offset and stride (bit
+    for bit), and the mean of the K films equals the scene alone rendering
+    K B consecutive iterations (the same terms, summed per frame first)."""
+    from realcausticsengine_amd import convergence
+
+    args = scenes.water_scene(32, 24, 16, samples=8, alpha=1.0)
+    opts = dict(photons=20000, seed=9, write_exr=0, camera_qmc=1)
+    K, B = 4, 2
+    fb = convergence.FrameBatch(args, K, opts)
+    for _ in range(B):
+        fb.render_iteration()
+    frames = [fb.batch.image(k, scaled=False) for k in range(K)]
+    mean = fb.image(scaled=False)
+    fb.release()
+    for k in range(K):
+        alone = _alone(args, dict(opts, iteration_offset=k, iteration_stride=K), B)
+        assert frames[k].sum() > 0
+        assert np.array_equal(frames[k], alone), k
+    consecutive = _alone(args, opts, K * B)
+    rel = np.abs(mean * K - consecutive) / (np.abs(consecutive) + 1e-6)
+    print(f"frames mean vs consecutive: max rel {rel.max():.2e}", flush=True)
+    assert np.allclose(mean * K, consecutive, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.skip(reason="the batch's photon-buffer regrow faulted the GPU before its fix (the scenes' photon-map "
+                         "views kept the freed sorted-photon arrays); kept off the GPU until the fix is reviewed")
+def test_batch_photon_buffer_regrow():
+    """A batch whose light pass stores more photons than the shared buffer's
+    first size (a quarter of the light paths, at least 2^20) grows the buffer
+    and runs the light pass again; the merge must then read the new sorted
+    photon arrays.  Films equal each scene rendered alone."""
+    cases = [(scenes.slab_scene(32, 32, light_size=4.0, samples=2), dict(photons=1 << 21, seed=3)),
+             (scenes.slab_scene(24, 24, light_size=4.0, samples=2), dict(photons=1 << 21, seed=4))]
+    iters = 2
+    alone = [_alone(a, o, iters) for a, o in cases]
+    b = Batch([a for a, _ in cases], [dict(o, write_exr=0) for _, o in cases]).prepare()
+    for _ in range(iters):
+        b.render_iteration()
+        st = b.stats()
+        assert st["photons"] > max(st["light_paths"] / 4, 1 << 20), st["photons"]  # the first buffer overflowed
+    for k in range(len(cases)):
+        img = b.image(k, scaled=False)
+        assert img.sum() > 0
+        assert np.array_equal(img, alone[k]), k
+    b.release()
