@@ -975,6 +975,24 @@ __device__ __forceinline__ float4 uniform_f4(float4 v) {
     return make_float4(uniform_f(v.x), uniform_f(v.y), uniform_f(v.z), uniform_f(v.w));
 }
 // Wave-uniform copy of a DiffuseQuery read from LDS: scalar registers.
+// The diffuse-only merge term of one photon at a query (kFast): the two
+// same-hemisphere tests of bsdf_eval_pdf's diffuse branch in the unfused
+// arithmetic of the reference (they decide whether the term counts), then the
+// MIS weight and the sum with fused multiply-adds (the query's sum is
+// compared with the reference's to a relative tolerance).  pw_inv_pi is the
+// photon's dVM over pi.
+__device__ __forceinline__ void add_diffuse_term(F3& c, const DiffuseQuery& d, bool acc, float4 dt, float2 gb,
+                                                 float pw_inv_pi) {
+    const F3 wi = f3(dt.x, dt.y, dt.z);
+    const float wz = dot(wi, f3(d.ns.x, d.ns.y, d.ns.z));
+    const bool keep = acc & !(dot(wi, f3(d.gw.x, d.gw.y, d.gw.z)) * d.ns.w <= 0.0f) & !(wz * d.gw.w <= 0.0f);
+    // without a branch: a term that does not count adds 0 (the photon's throughput is finite)
+    const float w = keep ? __builtin_amdgcn_rcpf(fmaf(pw_inv_pi, fabsf(wz), d.qk.w)) : 0.0f;
+    c.x = fmaf(dt.w, w, c.x);
+    c.y = fmaf(gb.x, w, c.y);
+    c.z = fmaf(gb.y, w, c.z);
+}
+
 __device__ __forceinline__ DiffuseQuery uniform_dq(const DiffuseQuery& d) {
     return DiffuseQuery{uniform_f4(d.qk), uniform_f4(d.ns), uniform_f4(d.gw)};
 }
@@ -1142,6 +1160,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                 for (uint32_t sb0 = 0; sb0 < total; sb0 += 64u * kB) {
                     float4 P[kB], DT[kB];
                     float2 GB[kB];
+                    float PW[kB];  // the photons' dVM / pi
                     bool V[kB];
                     uint32_t I[kB];
 #pragma unroll
@@ -1173,6 +1192,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                                 GB[bb] = pm.tp_gb[i];
                             }
                         }
+                        PW[bb] = P[bb].w * kInvPI;
                     }
                     for (uint32_t jj = 0; jj < nrun; ++jj) {
                         const uint32_t qj = s0 + jj;
@@ -1181,7 +1201,6 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                         const DiffuseQuery d = dq_lds[wv][qj];
                         const F3 dq = f3(d.qk.x, d.qk.y, d.qk.z);
                         if (kFast && ((fastmask >> qj) & 1u)) {
-                            const F3 dn = f3(d.ns.x, d.ns.y, d.ns.z), dg = f3(d.gw.x, d.gw.y, d.gw.z);
 #pragma unroll
                             for (int bb = 0; bb < kB; ++bb) {
                                 const bool acc = V[bb] && len2(dq - f3(P[bb].x, P[bb].y, P[bb].z)) < pm.r2;
@@ -1190,14 +1209,8 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                                     if (lane == 0) cnt[C_ACCEPTED] += (uint32_t)__builtin_popcountll(am);
                                     if (a.merge_sets && acc) record_merge(a, qslot, I[bb]);
                                 }
-                                // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
-                                // the wrong side adds nothing (its term is 0 there)
-                                const F3 wi = f3(DT[bb].x, DT[bb].y, DT[bb].z);
-                                const float wz = dot(wi, dn);
-                                if (acc && !(dot(wi, dg) * d.ns.w <= 0.0f) && !(wz * d.gw.w <= 0.0f)) {
-                                    const float w = __builtin_amdgcn_rcpf(d.qk.w + P[bb].w * (fabsf(wz) * kInvPI));
-                                    c = c + f3(DT[bb].w, GB[bb].x, GB[bb].y) * w;
-                                }
+                                // a photon on the wrong side adds nothing (its term is 0 there)
+                                add_diffuse_term(c, d, acc, DT[bb], GB[bb], PW[bb]);
                             }
                         } else {
                             const GroupQuery& g = gq_lds[wv][qj];
@@ -1269,6 +1282,7 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                         dt = pm.dir_tr[i];
                         gb = pm.tp_gb[i];
                     }
+                    const float pw = p.w * kInvPI;
 #pragma unroll
                     for (int jj = 0; jj < kQ; ++jj) {
                         if ((uint32_t)jj < nrun) {
@@ -1281,15 +1295,8 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
                                 if (a.merge_sets && acc) record_merge(a, qslot, i);
                             }
                             if (kFast && ((fastmask >> (s0 + jj)) & 1u)) {
-                                // bsdf_eval_pdf's diffuse-only branch and the MIS weight; a photon on
-                                // the wrong side adds nothing (its term is 0 there)
-                                const F3 wi = f3(dt.x, dt.y, dt.z);
-                                const float wz = dot(wi, f3(d.ns.x, d.ns.y, d.ns.z));
-                                if (acc && !(dot(wi, f3(d.gw.x, d.gw.y, d.gw.z)) * d.ns.w <= 0.0f) &&
-                                    !(wz * d.gw.w <= 0.0f)) {
-                                    const float w = __builtin_amdgcn_rcpf(d.qk.w + p.w * (fabsf(wz) * kInvPI));
-                                    c[jj] = c[jj] + f3(dt.w, gb.x, gb.y) * w;
-                                }
+                                // a photon on the wrong side adds nothing (its term is 0 there)
+                                add_diffuse_term(c[jj], d, acc, dt, gb, pw);
                             } else if (acc) {
                                 const GroupQuery& g = gq_lds[wv][s0 + jj];
                                 Bsdf b = g.b;
