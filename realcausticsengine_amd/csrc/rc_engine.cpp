@@ -1310,7 +1310,8 @@ bool guide_iteration(Engine& e, uint32_t photon_count, const MapInfo& mi) {
     launch_guide_targets(e.ph_rec.p, photon_count, pm.origin[0], pm.origin[1],
                          pm.origin[2], pm.inv_cell, pm.mask, gd.vis.p, (uint32_t)n, gd.train_w.p, s);
     HIP_OK(hipEventRecord(e.ev_guide[2], s));
-    bool ok = gd.forward(gd.train_x.p, n, s) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) && gd.adam(s);
+    bool ok = gd.forward(gd.train_x.p, n, s, false) && gd.backward(gd.train_bin.p, gd.train_w.p, n, s) &&
+              gd.adam(s);
     HIP_OK(hipEventRecord(e.ev_guide[3], s));
     return ok && gd.build_table(s);
 }
@@ -2204,7 +2205,8 @@ bool batch_guide_training(Batch& b, const BatchView& view, uint32_t count, uint3
         if (!targets[i].rows) continue;
         GuideModel& gd = b.scenes[i]->guide;
         const int n = (int)targets[i].rows;
-        if (!gd.forward(gd.train_x.p, n, st) || !gd.backward(gd.train_bin.p, gd.train_w.p, n, st) || !gd.adam(st) ||
+        if (!gd.forward(gd.train_x.p, n, st, false) || !gd.backward(gd.train_bin.p, gd.train_w.p, n, st) ||
+            !gd.adam(st) ||
             !gd.build_table(st))
             return false;
         HIP_OK(hipMemcpyAsync(gd.last, gd.scal.p, sizeof gd.last, hipMemcpyDeviceToHost, st));
@@ -3683,7 +3685,7 @@ __attribute__((visibility("default"))) int rc_guide_train_step(const float* x, c
     bool ok = upload_rows(*gd, x, n, padded) && gd->train_bin.alloc(padded) && gd->train_w.alloc(padded) &&
               hipMemcpy(gd->train_bin.p, b.data(), padded * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(gd->train_w.p, ww.data(), padded * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
-              gd->forward(gd->train_x.p, padded, g->stream) &&
+              gd->forward(gd->train_x.p, padded, g->stream, false) &&
               gd->backward(gd->train_bin.p, gd->train_w.p, padded, g->stream) && (!apply || gd->adam(g->stream));
     float sc[4] = {0, 0, 0, 0};
     ok = ok && hipMemcpyAsync(sc, gd->scal.p, sizeof sc, hipMemcpyDeviceToHost, g->stream) == hipSuccess;

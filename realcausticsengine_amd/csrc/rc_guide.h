@@ -98,9 +98,13 @@ struct GemmArgs {
 bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStream_t s);
 
 void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s);
-// dZ and the loss (added into *loss through the per-row losses in row_loss[n]).
-void launch_softmax_ce(const float* Z, const float* w, const int* bins, int n, const float* wsum, uint16_t* dZ,
-                       float* row_loss, float* loss, hipStream_t s);
+// Output layer + softmax cross-entropy of a training step (n % 32 == 0): dZ, the per-row losses (summed
+// into *loss) and the back-propagated dA3 = (dZ W4^T) masked by H3 > 0, from the last hidden layer H3,
+// W4 (BF16, [64][1024]) and b4, without writing the logits.  W4f: 64 x 1024 BF16 of work space (W4 in the
+// kernel's fragment order).
+bool launch_out_softmax(const uint16_t* H3, const uint16_t* W4, uint16_t* W4f, const float* b4, const float* w,
+                        const int* bins, int n, const float* wsum, uint16_t* dZ, float* row_loss, float* loss,
+                        uint16_t* dA3, hipStream_t s);
 // out[0] += sum of v, out[1] += number of entries > 0.
 void launch_sum_count(const float* v, int n, float* out, hipStream_t s);
 struct AdamArgs {

@@ -1,8 +1,9 @@
 // rc_guide.hip -- gfx950 kernels of the learned emission proposal: Fourier
 // encoding, an LDS-tiled BF16 matrix-core product with fused epilogues (the
 // forward layers, the backward data products and the split-K weight
-// gradients), softmax cross-entropy, Adam, the proposal table (softmax ->
-// CDF per position cell) and the training targets.
+// gradients), the training step's output layer fused with softmax
+// cross-entropy, Adam, the proposal table (softmax -> CDF per position cell)
+// and the training targets.
 #include "rc_engine.h"
 #include "rc_guide.h"
 #include "rc_vec.h"
@@ -187,58 +188,166 @@ bool launch_gemm(const GemmArgs& g, int epi, bool a_trans, bool b_trans, hipStre
 }
 
 // ---------------------------------------------------------------------
-// Softmax cross-entropy, one wave per row of 1024 logits (16 per lane):
-//   loss += w_r / W * (log sum exp(z) - z_b),
-//   dZ_r  = bf16(w_r / W * (softmax(z) - onehot(b)))   (W = sum of w)
+// Output layer and softmax cross-entropy of a training step in one kernel:
+// the 1024 logits of a row are made, used and dropped in registers, so they
+// never reach memory (only dZ does).  A workgroup of 8 waves takes 256 rows,
+// 32 per wave.  W4 (64 x 1024) is first copied into LDS in the order of the
+// wave's A fragments (k_w4_frag): entry (t * 4 + kk) * 64 + l holds bin
+// 32 t + (l % 32) and k = 16 kk + 8 (l / 32) .. +7, one 16-byte read per lane
+// and no bank conflicts.  A wave then forms Z^T tiles (32 bins x its 32 rows) with
+// v_mfma_f32_32x32x16_bf16, A = W4^T fragments, B = the rows' H3: lane l
+// holds bins 32 t + 8 (i / 4) + 4 (l / 32) + (i % 4), i = 0..15, of row l % 32,
+// so a row's statistics combine two lanes (l and l ^ 32).  Pass 1 keeps a
+// running maximum and sum of exp(z - max) per lane (rescaled when the
+// maximum grows); pass 2 makes the tiles again and writes
+//   dZ_r = bf16(w_r / W (softmax(z) - onehot(b)))   (4 bins per 8-byte store)
+// and the row's loss w_r / W (log sum exp(z) - z_b), W = sum of w, and feeds
+// each tile's dZ straight into the back-propagated product
+//   dA3_r = bf16((dZ_r W4^T) masked by H3_r > 0)
+// (the bins in the same ascending order as a separate product would take).
 // ---------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z, const float* __restrict__ w,
-                                                    const int* __restrict__ bins, int n,
-                                                    const float* __restrict__ wsum, uint16_t* __restrict__ dZ,
-                                                    float* __restrict__ row_loss) {
-    const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= n) return;
-    const float4* zr = reinterpret_cast<const float4*>(Z + (size_t)r * kGuideBins) + lane * 4;
-    float z[16];
+constexpr int kOutRowsPerWave = 32, kOutWaves = 8;
+constexpr int kOutTiles = kGuideBins / 32;  // 32 tiles of 32 bins
+
+// W4 (BF16, [64][1024]) in the A-fragment order of k_out_softmax: chunk
+// (t * 4 + kk) * 64 + l holds bin 32 t + (l % 32), k = 16 kk + 8 (l / 32) .. +7.
+__global__ __launch_bounds__(256) void k_w4_frag(const uint16_t* __restrict__ W4, uint16_t* __restrict__ out) {
+    const int c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= kOutTiles * 4 * 64) return;
+    const int t = c >> 8, kk = (c >> 6) & 3, l = c & 63;
+    const int bin = 32 * t + (l & 31), k0 = 16 * kk + 8 * (l >> 5);
+    uint16_t e[8];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        float4 v = zr[q];
-        z[4 * q] = v.x;
-        z[4 * q + 1] = v.y;
-        z[4 * q + 2] = v.z;
-        z[4 * q + 3] = v.w;
+    for (int j = 0; j < 8; ++j) e[j] = W4[(size_t)(k0 + j) * kGuideBins + bin];
+    reinterpret_cast<uint4*>(out)[c] = make_uint4(e[0] | ((uint32_t)e[1] << 16), e[2] | ((uint32_t)e[3] << 16),
+                                                  e[4] | ((uint32_t)e[5] << 16), e[6] | ((uint32_t)e[7] << 16));
+}
+
+__global__ __launch_bounds__(kOutWaves * 64) void k_out_softmax(const uint16_t* __restrict__ H3,
+                                                                 const uint16_t* __restrict__ W4,
+                                                                 const uint16_t* __restrict__ W4f,
+                                                                 const float* __restrict__ b4,
+                                                                 const float* __restrict__ w,
+                                                                 const int* __restrict__ bins, int n,
+                                                                 const float* __restrict__ wsum,
+                                                                 uint16_t* __restrict__ dZ,
+                                                                 float* __restrict__ row_loss,
+                                                                 uint16_t* __restrict__ dA3) {
+    __shared__ uint16_t frag[kOutTiles * 4 * 64 * 8];  // 128 KB
+    const int tid = threadIdx.x, lane = tid & 63;
+    // the fragment-ordered copy of W4 (k_w4_frag) into LDS, 16 bytes per thread and step
+    for (int c = tid; c < kOutTiles * 4 * 64; c += kOutWaves * 64)
+        reinterpret_cast<uint4*>(frag)[c] = reinterpret_cast<const uint4*>(W4f)[c];
+    __syncthreads();
+    const int r0 = (blockIdx.x * kOutWaves + (tid >> 6)) * kOutRowsPerWave;
+    if (r0 >= n) return;
+    const int row = r0 + (lane & 31);
+    bf16x8_t hb[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+        hb[kk] = *reinterpret_cast<const bf16x8_t*>(H3 + (size_t)row * kGuideHidden + kk * 16 + (lane >> 5) * 8);
+    const bf16x8_t* fa = reinterpret_cast<const bf16x8_t*>(frag) + lane;
+    auto tile = [&](int t, float (&z)[16]) {
+        f32x16_t acc;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[(t * 4 + kk) * 64], hb[kk], acc, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 b = *reinterpret_cast<const float4*>(b4 + 32 * t + 8 * q + 4 * (lane >> 5));
+            z[4 * q] = acc[4 * q] + b.x;
+            z[4 * q + 1] = acc[4 * q + 1] + b.y;
+            z[4 * q + 2] = acc[4 * q + 2] + b.z;
+            z[4 * q + 3] = acc[4 * q + 3] + b.w;
+        }
+    };
+    // pass 1: maximum and sum of exp(z - maximum) over the lane's 512 bins of the row
+    float m = -INFINITY, s = 0.0f;
+    for (int t = 0; t < kOutTiles; ++t) {
+        float z[16];
+        tile(t, z);
+        float tm = z[0];
+#pragma unroll
+        for (int i = 1; i < 16; ++i) tm = fmaxf(tm, z[i]);
+        const float mn = fmaxf(m, tm);
+        s = s * __expf(m - mn);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) s += __expf(z[i] - mn);
+        m = mn;
     }
-    float m = z[0];
-#pragma unroll
-    for (int j = 1; j < 16; ++j) m = fmaxf(m, z[j]);
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-    float e[16], s = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        e[j] = __expf(z[j] - m);
-        s += e[j];
+    {  // the row's other half is in lane l ^ 32
+        const float mo = __shfl_xor(m, 32), so = __shfl_xor(s, 32);
+        const float mt = fmaxf(m, mo);
+        s = s * __expf(m - mt) + so * __expf(mo - mt);
+        m = mt;
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    // pass 2: dZ, the loss, and the back-propagated dA3 = (dZ W4^T) masked by H3 > 0.  dA3^T (64 x the wave's
+    // 32 rows) is two accumulators, A = W4 rows (k; 8 consecutive bins per lane, 16-byte loads), B = dZ^T:
+    // the 16-bin K-step s of tile t needs bins 32 t + 16 s + 8 (l / 32) .. +7 of the lane's row, four of
+    // which the partner lane l ^ 32 holds
     const float W = *wsum;
-    const int b = bins[r];
-    const float wr = (W > 0.0f && b >= 0) ? w[r] / W : 0.0f;
+    const int b = bins[row];
+    const float wr = (W > 0.0f && b >= 0) ? w[row] / W : 0.0f;
     const float inv_s = 1.0f / s;
-    uint16_t d[16];
+    const int h = lane >> 5;
     float zb = 0.0f;
+    uint16_t* drow = dZ + (size_t)row * kGuideBins;
+    f32x16_t da[2];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        int col = lane * 16 + j;
-        float p = e[j] * inv_s - (col == b ? 1.0f : 0.0f);
-        if (col == b) zb = z[j];
-        d[j] = f2bf(wr * p);
+    for (int i = 0; i < 16; ++i) {
+        da[0][i] = 0.0f;
+        da[1][i] = 0.0f;
     }
-    uint4* out = reinterpret_cast<uint4*>(dZ + (size_t)r * kGuideBins + lane * 16);
-    out[0] = make_uint4(d[0] | ((uint32_t)d[1] << 16), d[2] | ((uint32_t)d[3] << 16), d[4] | ((uint32_t)d[5] << 16),
-                        d[6] | ((uint32_t)d[7] << 16));
-    out[1] = make_uint4(d[8] | ((uint32_t)d[9] << 16), d[10] | ((uint32_t)d[11] << 16),
-                        d[12] | ((uint32_t)d[13] << 16), d[14] | ((uint32_t)d[15] << 16));
-    for (int off = 32; off > 0; off >>= 1) zb += __shfl_xor(zb, off);
-    if (lane == 0) row_loss[r] = wr != 0.0f ? wr * (__logf(s) + m - zb) : 0.0f;
+    for (int t = 0; t < kOutTiles; ++t) {
+        float z[16];
+        tile(t, z);
+        uint2 dq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int bin0 = 32 * t + 8 * q + 4 * h;
+            uint16_t d[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float p = __expf(z[4 * q + j] - m) * inv_s - (bin0 + j == b ? 1.0f : 0.0f);
+                if (bin0 + j == b) zb = z[4 * q + j];
+                d[j] = f2bf(wr * p);
+            }
+            dq[q] = make_uint2(d[0] | ((uint32_t)d[1] << 16), d[2] | ((uint32_t)d[3] << 16));
+            *reinterpret_cast<uint2*>(drow + bin0) = dq[q];
+        }
+#pragma unroll
+        for (int st = 0; st < 2; ++st) {
+            const uint2 send = h ? dq[2 * st] : dq[2 * st + 1];
+            const uint2 recv = make_uint2((uint32_t)__shfl_xor((int)send.x, 32), (uint32_t)__shfl_xor((int)send.y, 32));
+            const uint4 bv = h ? make_uint4(recv.x, recv.y, dq[2 * st + 1].x, dq[2 * st + 1].y)
+                               : make_uint4(dq[2 * st].x, dq[2 * st].y, recv.x, recv.y);
+            const bf16x8_t bf = __builtin_bit_cast(bf16x8_t, bv);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+                const bf16x8_t wf = *reinterpret_cast<const bf16x8_t*>(
+                    W4 + (size_t)(32 * a + (lane & 31)) * kGuideBins + 32 * t + 16 * st + 8 * h);
+                da[a] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf, bf, da[a], 0, 0, 0);
+            }
+        }
+    }
+    zb += __shfl_xor(zb, 32);
+    if (lane < 32) row_loss[row] = wr != 0.0f ? wr * (__logf(s) + m - zb) : 0.0f;
+    // element i of da[a]: k = 32 a + 8 (i / 4) + 4 h + (i % 4) of the lane's row
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k0 = 32 * a + 8 * q + 4 * h;
+            const uint2 hm = *reinterpret_cast<const uint2*>(H3 + (size_t)row * kGuideHidden + k0);
+            const uint16_t hv[4] = {(uint16_t)hm.x, (uint16_t)(hm.x >> 16), (uint16_t)hm.y, (uint16_t)(hm.y >> 16)};
+            uint16_t o[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) o[j] = f2bf(bf2f(hv[j]) > 0.0f ? da[a][4 * q + j] : 0.0f);
+            *reinterpret_cast<uint2*>(dA3 + (size_t)row * kGuideHidden + k0) =
+                make_uint2(o[0] | ((uint32_t)o[1] << 16), o[2] | ((uint32_t)o[3] << 16));
+        }
+    }
 }
 
 // out[0] += sum of v; with `count`, out[1] += entries > 0.  One atomic per
@@ -391,12 +500,17 @@ __global__ void k_guide_targets_batch(const float4* __restrict__ rec, uint32_t n
 void launch_guide_encode(const float* x, int n, uint16_t* X, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_guide_encode, dim3((n + 127) / 128), dim3(128), 0, s, x, n, X);
 }
-void launch_softmax_ce(const float* Z, const float* w, const int* bins, int n, const float* wsum, uint16_t* dZ,
-                       float* row_loss, float* loss, hipStream_t s) {
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_softmax_ce, dim3((n + 3) / 4), dim3(256), 0, s, Z, w, bins, n, wsum, dZ, row_loss);
+bool launch_out_softmax(const uint16_t* H3, const uint16_t* W4, uint16_t* W4f, const float* b4, const float* w,
+                        const int* bins, int n, const float* wsum, uint16_t* dZ, float* row_loss, float* loss,
+                        uint16_t* dA3, hipStream_t s) {
+    if (n <= 0 || n % kOutRowsPerWave) return false;
+    const int rows = kOutWaves * kOutRowsPerWave;
+    hipLaunchKernelGGL(k_w4_frag, dim3(kOutTiles * 4 * 64 / 256), dim3(256), 0, s, W4, W4f);
+    hipLaunchKernelGGL(k_out_softmax, dim3((n + rows - 1) / rows), dim3(kOutWaves * 64), 0, s, H3, W4, W4f, b4, w,
+                       bins, n, wsum, dZ, row_loss, dA3);
     hipLaunchKernelGGL(k_sum_count, dim3(std::min(kReduceBlocks, (n + 255) / 256)), dim3(256), 0, s, row_loss, n,
                        loss, 0);
+    return true;
 }
 void launch_sum_count(const float* v, int n, float* out, hipStream_t s) {
     if (n > 0)

@@ -113,13 +113,13 @@ bool GuideModel::ensure_rows(int n) {
     size_t r = (size_t)n;
     if (!X.alloc(r * kGuideFeat) || !H1.alloc(r * kGuideHidden) || !H2.alloc(r * kGuideHidden) ||
         !H3.alloc(r * kGuideHidden) || !dA1.alloc(r * kGuideHidden) || !dA2.alloc(r * kGuideHidden) ||
-        !dA3.alloc(r * kGuideHidden) || !dZ.alloc(r * kGuideBins) || !Z.alloc(r * kGuideBins) || !row_loss.alloc(r))
+        !dA3.alloc(r * kGuideHidden) || !dZ.alloc(r * kGuideBins) || !row_loss.alloc(r))
         return false;
     rows_cap = n;
     return true;
 }
 
-bool GuideModel::forward(const float* d_x, int n, hipStream_t s) {
+bool GuideModel::forward(const float* d_x, int n, hipStream_t s, bool logits) {
     if (n <= 0 || n % 64) {
         set_error("guide: row count must be a positive multiple of 64");
         return false;
@@ -128,6 +128,7 @@ bool GuideModel::forward(const float* d_x, int n, hipStream_t s) {
     if (!ensure_rows(n)) return false;
     const uint16_t* w = wbf.p;
     const float* b = master.p;
+    if (logits && !Z.alloc((size_t)n * kGuideBins)) return false;
     launch_guide_encode(d_x, n, X.p, s);
     return gemm(X.p, kGuideFeat, false, w + kOffW1, kGuideHidden, false, n, kGuideHidden, kGuideFeat,
                 EPI_BIAS_RELU_BF16, b + kOffB1, nullptr, H1.p, kGuideHidden, s) &&
@@ -135,8 +136,8 @@ bool GuideModel::forward(const float* d_x, int n, hipStream_t s) {
                 EPI_BIAS_RELU_BF16, b + kOffB2, nullptr, H2.p, kGuideHidden, s) &&
            gemm(H2.p, kGuideHidden, false, w + kOffW3, kGuideHidden, false, n, kGuideHidden, kGuideHidden,
                 EPI_BIAS_RELU_BF16, b + kOffB3, nullptr, H3.p, kGuideHidden, s) &&
-           gemm(H3.p, kGuideHidden, false, w + kOffW4, kGuideBins, false, n, kGuideBins, kGuideHidden, EPI_BIAS_F32,
-                b + kOffB4, nullptr, Z.p, kGuideBins, s);
+           (!logits || gemm(H3.p, kGuideHidden, false, w + kOffW4, kGuideBins, false, n, kGuideBins, kGuideHidden,
+                            EPI_BIAS_F32, b + kOffB4, nullptr, Z.p, kGuideBins, s));
 }
 
 bool GuideModel::backward(const int* d_bin, const float* d_w, int n, hipStream_t s) {
@@ -145,13 +146,17 @@ bool GuideModel::backward(const int* d_bin, const float* d_w, int n, hipStream_t
     zero_f32(scal.p, 4, s);
     zero_f32(g, kGuideParams, s);
     launch_sum_count(d_w, n, scal.p, s);  // [0] sum of w, [1] rows with w > 0
-    launch_softmax_ce(Z.p, d_w, d_bin, n, scal.p, dZ.p, row_loss.p, scal.p + 2, s);
+    // output layer, softmax cross-entropy and the output layer's back-propagated product in one kernel:
+    // dZ, the loss and dA3, no logits in memory
+    if (!w4frag.alloc((size_t)kGuideHidden * kGuideBins)) return false;
+    if (!launch_out_softmax(H3.p, w + kOffW4, w4frag.p, master.p + kOffB4, d_w, d_bin, n, scal.p, dZ.p, row_loss.p,
+                            scal.p + 2, dA3.p, s)) {
+        set_error("guide: training rows must be a multiple of 32");
+        return false;
+    }
     // output layer
     if (!gemm(H3.p, kGuideHidden, true, dZ.p, kGuideBins, false, kGuideHidden, kGuideBins, n, EPI_ATOMIC_F32, nullptr,
               nullptr, g + kOffW4, kGuideBins, s, wgrad_split(kGuideHidden, kGuideBins, n), g + kOffB4))
-        return false;
-    if (!gemm(dZ.p, kGuideBins, false, w + kOffW4, kGuideBins, true, n, kGuideHidden, kGuideBins, EPI_MASK_BF16,
-              nullptr, H3.p, dA3.p, kGuideHidden, s))
         return false;
     // hidden layer 3
     if (!gemm(H2.p, kGuideHidden, true, dA3.p, kGuideHidden, false, kGuideHidden, kGuideHidden, n, EPI_ATOMIC_F32,
@@ -277,7 +282,7 @@ GuideView GuideModel::view(bool records, bool sample) const {
 void GuideModel::release() {
     for (DevBuf<float>* b : {&master, &m, &v, &grad, &scal, &train_x, &train_w, &Z, &row_loss, &table_x, &cdf})
         b->release();
-    for (DevBuf<uint16_t>* b : {&wbf, &X, &H1, &H2, &H3, &dA1, &dA2, &dA3, &dZ}) b->release();
+    for (DevBuf<uint16_t>* b : {&wbf, &X, &H1, &H2, &H3, &dA1, &dA2, &dA3, &dZ, &w4frag}) b->release();
     train_bin.release();
     vis.release();
     params_ready = table_ready = false;

@@ -38,6 +38,7 @@ struct GuideModel {
     DevBuf<float> train_x, train_w;
     DevBuf<int> train_bin;
     DevBuf<uint16_t> X, H1, H2, H3, dA1, dA2, dA3, dZ;
+    DevBuf<uint16_t> w4frag;  // the output layer's weights in the fused softmax kernel's fragment order
     DevBuf<float> Z, row_loss;
     DevBuf<float> table_x, cdf;
     DevBuf<uint8_t> vis;
@@ -48,9 +49,10 @@ struct GuideModel {
     bool set_params(const float* host, hipStream_t s);
     bool get_params(float* host, hipStream_t s);
     bool ensure_rows(int n);
-    // Logits of n records (n % 64 == 0) into Z.
-    bool forward(const float* d_x, int n, hipStream_t s);
-    // Loss and gradients of the last forward; weights w, bins b (-1: no target).
+    // Hidden layers of n records (n % 64 == 0) and, with `logits`, the logits into Z (the training step
+    // leaves them out: backward makes the output layer again inside its fused softmax kernel).
+    bool forward(const float* d_x, int n, hipStream_t s, bool logits = true);
+    // Loss and gradients of the last forward (H1..H3 of its rows); weights w, bins b (-1: no target).
     bool backward(const int* d_bin, const float* d_w, int n, hipStream_t s);
     bool adam(hipStream_t s);
     // Slots and table inputs of the scene's lights; center / half extent of the box.
