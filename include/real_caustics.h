@@ -206,7 +206,7 @@ enum {
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the
  * kernels it affects, "overlap" last, by the iteration's wall time (kept only
- * when it saves at least 5 %; the other knobs are timed with the camera trace
+ * when it saves at least 2 %; the other knobs are timed with the camera trace
  * serial).  report (up to RC_TUNE_REPORT
  * doubles): wall ms of one iteration before (camera trace serial) and after,
  * then path_chunk, light_blocks_per_cu, camera_blocks_per_cu, cell_scale,

@@ -1667,8 +1667,9 @@ bool render_iteration(Engine& e, int iteration) {
 // affects (best of `repeats`), one knob after the other.  The last knob,
 // overlap, is judged by the iteration's wall time: the camera trace beside
 // the light pass shortens the iteration only while the light pass leaves
-// compute units idle, and it is kept only when it saves at least 5 % (beside
-// the trace the light pass's own time no longer measures that kernel alone).
+// compute units idle, and it is kept only when it saves at least 2 %, more
+// than the run-to-run spread of the iteration's wall time (beside the trace
+// the light pass's own time no longer measures that kernel alone).
 // ---------------------------------------------------------------------
 enum { T_LIGHT = 0, T_MAP, T_TRACE, T_GATHER, T_WALL, T_PARTS };
 
@@ -1728,7 +1729,7 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"overlap", overlap_candidates,
          [=](double v) { launch_owner().overlap = (int)v; },
          [=]() { return (double)launch_owner().overlap; },
-         [](const double* t) { return t[T_WALL]; }, 0.05},
+         [](const double* t) { return t[T_WALL]; }, 0.02},
     };
 }
 
