@@ -618,8 +618,10 @@ def main() -> int:
     kern_timed = dict(kern)
     if overlap:
         # beside the light pass the camera trace shares the GPU: the per-kernel times of the roofline
-        # come from serial iterations (untimed for the step rate)
+        # come from serial iterations (untimed for the step rate) of the same iteration indices, as the
+        # merge radius shrinks from one iteration to the next
         eng.set("overlap", 0)
+        eng.set("next_iteration", a.warmup)
         kern = {k: 0.0 for k in keys}
         for _ in range(a.steps):
             eng.render_iteration()
