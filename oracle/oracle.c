@@ -22,11 +22,15 @@
  *   lights, camera    closed forms (corner interpolation, cosine lobe moments).
  *   dVM / MIS         hand-derived one- and two-bounce values.
  *   whole image       closed-form mirror/diffuse scene and f32 vs f64 agreement.
- * Parity unpinned: the serial xorshift + adaptive MLT light pass
- * (rng_serial = 1) restates Vertex_Merging.cpp:220-282 but no run of the
- * reference can be made here to compare against (the reference source does not
- * compile with this image's libstdc++), so its image is only checked
- * statistically against the plain estimator.
+ *   serial MLT loop   the serial xorshift + adaptive MLT light pass
+ *                     (rng_serial = 1, Vertex_Merging.cpp:220-282) has no run
+ *                     of the reference to compare against (the reference
+ *                     source does not compile with this image's libstdc++);
+ *                     it is pinned by what the Metropolis construction fixes
+ *                     (tests/test_oracle_mlt.py): the chain's states, rejected
+ *                     mutations repeating the state, are distributed as the
+ *                     visible uniform paths, and with every path visible no
+ *                     mutation happens and the photon list is the plain one.
  */
 #include "oracle.h"
 

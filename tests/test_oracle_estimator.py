@@ -90,8 +90,9 @@ def test_float_and_double_oracles_agree(oracle):
 
 def test_serial_mlt_mode_runs(oracle):
     # Vertex_Merging.cpp:220-282 (adaptive MLT + visibility normalisation): the
-    # restatement has no reference run to compare against (parity unpinned).
-    # Its expectation is the plain estimate times M/N, M = visible uniform paths
+    # restatement has no reference run to compare against; tests/test_oracle_mlt.py
+    # pins its chains by the Metropolis construction.  Here: its expectation is
+    # the plain estimate times M/N, M = visible uniform paths
     # plus accepted mutations <= N: a rejected mutation re-counts nothing, so the
     # reference's default estimator is darker than the plain one.
     args = scenes.c0_scene(32, 32, subdiv=3, samples=1)
