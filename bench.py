@@ -640,7 +640,7 @@ def main() -> int:
     eng.render_iteration()
     cst = eng.stats()
     eng.set("count_events", 0)
-    by = algorithmic_bytes(cst, int(eng.get("node_bytes")), eng.get("group_queries") > 0)
+    by = algorithmic_bytes(cst, int(eng.get("node_bytes")), eng.get("group_queries") > 1)
     dom = max(kern, key=kern.get)
     achieved = by[dom] / (kern[dom] * 1e-3) / 1e9
     traffic = profiled_traffic(dom)

@@ -112,7 +112,9 @@ int main(void);
  * order); the film is the same either way.  "group_queries" (8 = the default):
  * queries per wave of the grouped merge, in which queries with the same cell
  * window share the photon loads (same terms as one query per wave, summed in
- * another order).  "gather_tile" (1024 = the default): the grouped merge deals
+ * another order); 1 = one query per lane, each lane walking its own window
+ * (for sparse photon maps; the same terms again in another order).
+ * "gather_tile" (1024 = the default): the grouped merge deals
  * tiles of this many consecutive queries of the order to the eight XCDs in
  * turn, and each XCD's waves take its groups one at a time as they free up;
  * 0 = each XCD takes one contiguous eighth, dealt to its waves in a fixed
@@ -177,7 +179,7 @@ enum {
     /* sort of the merge queries into spatial order (option "query_order") */
     RC_STAT_MS_QUERY_SORT,
     /* photon records the merge streamed (count_events): once per query with
-     * group_queries = 0, once per group of queries sharing a window otherwise */
+     * group_queries = 0 or 1, once per group of queries sharing a window otherwise */
     RC_STAT_PHOTONS_STREAMED,
     /* adaptive MLT light pass (option "mlt"): visible uniform paths, mutations, accepted mutations */
     RC_STAT_MLT_VISIBLE, RC_STAT_MLT_MUTATIONS, RC_STAT_MLT_ACCEPTED,
@@ -194,7 +196,8 @@ enum {
  * once at least this many of its 64 lanes are idle, or none is live; 1 = at
  * once), "light_order" (1 = trace the light paths sorted by their emission
  * samples, for ray coherence; the film is the same), "query_order" and
- * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave),
+ * "group_queries" (merge queries per wave: 4, 8 or 16; 0 = one per wave;
+ * 1 = one per lane, for sparse photon maps),
  * "gather_tile" (0, 256 or 1024 queries per XCD tile of the merge),
  * "gather_block" (0, 1, 2 or 4 photon steps per block of the merge),
  * "overlap" (1, the default: the camera trace runs on a second stream beside
@@ -210,8 +213,9 @@ enum {
  * serial).  report (up to RC_TUNE_REPORT
  * doubles): wall ms of one iteration before (camera trace serial) and after,
  * then path_chunk, light_blocks_per_cu, camera_blocks_per_cu, cell_scale,
- * gather_blocks_per_cu, light_refill, light_order, camera_refill,
- * group_queries, gather_tile, gather_block, query_order, overlap. */
+ * light_refill, light_order, camera_refill, group_queries, gather_tile,
+ * gather_block, gather_blocks_per_cu (after the merge's shape, which it
+ * depends on), query_order, overlap. */
 enum { RC_TUNE_REPORT = 15 };
 /* Option "graph" = 1: with the plain light pass and the camera trace serial
  * the whole iteration (light pass, photon map, camera pass, resolve) replays

@@ -134,7 +134,7 @@ struct Engine {
     int iteration_offset = 0, iteration_stride = 1;
     DevBuf<unsigned long long> lk_a, lk_b;  // light-path sort buffers (one entry per path)
     DevBuf<uint32_t> lv_a, lv_b, l_hist, l_hist_sums;
-    int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave
+    int group_queries = 8;  // queries per wave of the grouped merge (4, 8, 16), 0: one query per wave, 1: per lane
     uint32_t gather_tile = 1024;  // grouped merge: queries per XCD tile of the order (0: one segment per XCD)
     int gather_block = 2;  // grouped merge: 1 / 2 / 4 = rolled query loop over 64 x this many photons, 0 = unrolled
     // adaptive MLT light pass (option "mlt"): chains, their explicit states and counts
@@ -1701,9 +1701,6 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
          [=]() { return (double)launch_owner().camera_blocks_per_cu; }, T},
         {"cell_scale", {0.67, 0.8, 1.0, 1.25}, set_cell_scale, get_cell_scale,
          [](const double* t) { return t[T_MAP] + t[T_GATHER]; }},
-        {"gather_blocks_per_cu", {5, 8, 12, 16, 24},
-         [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
-         [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
         {"light_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().light_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().light_refill; }, L},
@@ -1713,7 +1710,7 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"camera_refill", {1, 8, 16, 32},
          [=](double v) { launch_owner().camera_refill = (uint32_t)v; },
          [=]() { return (double)launch_owner().camera_refill; }, T},
-        {"group_queries", {4, 8, 16},
+        {"group_queries", {4, 8, 16, 1},
          [=](double v) { launch_owner().group_queries = (int)v; },
          [=]() { return (double)launch_owner().group_queries; }, G},
         {"gather_tile", {1024, 256, 0},
@@ -1722,6 +1719,9 @@ std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::fun
         {"gather_block", {2, 4, 1, 0},
          [=](double v) { launch_owner().gather_block = (int)v; },
          [=]() { return (double)launch_owner().gather_block; }, G},
+        {"gather_blocks_per_cu", {5, 8, 12, 16, 24},
+         [=](double v) { launch_owner().gather_blocks_per_cu = (int)v; },
+         [=]() { return (double)launch_owner().gather_blocks_per_cu; }, G},
         {"query_order", {1, 0},
          [=](double v) { launch_owner().query_order = (int)v; },
          [=]() { return (double)launch_owner().query_order; },
@@ -3037,7 +3037,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "iteration_stride") g->iteration_stride = std::max(1, std::min((int)v, 1 << 16));
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
-    else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 8 : 0;
+    else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 1 : 0;
     else if (k == "overlap") g->overlap = (int)v ? 1 : 0;
     else if (k == "gather_block") g->gather_block = (int)v >= 4 ? 4 : (int)v >= 2 ? 2 : (int)v >= 1 ? 1 : 0;
     else if (k == "gather_tile") g->gather_tile = (uint32_t)std::max(0.0, std::min(v, 1048576.0));

@@ -224,7 +224,7 @@ struct CameraPassArgs {
     const uint32_t* q_order;  // non-null: the gather takes the queries in this order (spatially sorted)
     int camera_qmc;           // 1: jitter and first-bounce choice stratified across iterations
     int group_queries;        // 4, 8 or 16: grouped merge of that many queries per wave (queries with the
-                              // same cell window share photon loads); 0: one query per wave
+                              // same cell window share photon loads); 0: one query per wave; 1: one per lane
     uint32_t gather_tile;     // grouped merge: XCD x takes the tiles t = x mod 8 of this many queries of the
                               // order; 0: XCD x takes the x-th contiguous eighth
     int gather_block;         // grouped merge: 2 or 4 = photons taken 64 x this many at a time and the query

@@ -1340,6 +1340,97 @@ __global__ __launch_bounds__(kGatherBlock) void k_gather_grouped(CameraPassArgs 
     if (kCount) flush_counters(a.counters, cnt);
 }
 
+// One lane per query (group_queries = 1), for sparse photon maps: at a few
+// tens of photons per query window the wave-per-group merge spends most of
+// its instructions on per-query setup, per-sub-run cell lookups and the
+// lane reductions of every query's sum, whereas here a lane sets up its own
+// query, walks its cell window (z, then y, then x, each cell's photons in
+// map order) and adds the accepted photons' terms in that order -- no
+// reduction, no LDS.  Lanes take consecutive queries of the spatial order,
+// so a wave's windows overlap and its photon loads hit the same lines.
+// The radius test is the one of the other merges (same sets); the sum of a
+// query is the same terms in another order.
+template <bool kCount, bool kFast, bool kBatch>
+__global__ __launch_bounds__(kGatherBlock) void k_gather_lane(CameraPassArgs a) {
+    unsigned long long cnt[kCounterFields] = {0};
+    PhotonMapView pm0 = a.pm;
+    if (a.dyn) {
+        pm0.r2 = a.dyn->r2;
+        pm0.r_query = a.dyn->r_query;
+        pm0.inv_cell = a.dyn->inv_cell;
+        pm0.origin[0] = a.dyn->origin[0];
+        pm0.origin[1] = a.dyn->origin[1];
+        pm0.origin[2] = a.dyn->origin[2];
+    }
+    const uint32_t nq = min(*(volatile uint32_t*)(a.path_counter + 1), a.q.capacity);
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < nq; j += gridDim.x * blockDim.x) {
+        const uint32_t qi = a.q_order ? a.q_order[j] : j;
+        const float4 qp = a.q.pos[qi], qd = a.q.dir[qi], qs = a.q.sn[qi], qg = a.q.gn[qi], t = a.q.tpn[qi];
+        const uint32_t slot = __float_as_uint(qs.w);
+        const int scene = kBatch ? __float_as_int(qg.w) : 0;
+        const PhotonMapView& pm = kBatch ? a.batch.pm[scene] : pm0;
+        const DevMaterial* mats = kBatch ? a.batch.sc[scene].mats : a.sc.mats;
+        GroupQuery g;
+        g.q = f3(qp.x, qp.y, qp.z);
+        g.cdvm = qp.w;
+        bsdf_setup(g.b, mats[__float_as_int(qd.w)], f3(qs.x, qs.y, qs.z), f3(qd.x, qd.y, qd.z), true);
+        g.mc = merge_const(g.b, f3(qg.x, qg.y, qg.z));
+        const bool fast = kFast && g.b.kind == BK_FMG && g.mc.f == 0.0f;
+        DiffuseQuery d;
+        d.qk = make_float4(g.q.x, g.q.y, g.q.z, 1.0f + g.cdvm * g.mc.rd);
+        d.ns = make_float4(g.b.fr.z.x, g.b.fr.z.y, g.b.fr.z.z, g.mc.side_wo);
+        d.gw = make_float4(g.mc.gn.x, g.mc.gn.y, g.mc.gn.z, g.b.wo.z);
+        int lo[3], hi[3];
+        query_window(pm, g.q, lo, hi);
+        F3 c = f3s(0.0f);
+        // at most 4 cells per axis (cell >= 2/3 of the query radius): their buckets differ (cell_bucket)
+        for (int cz = lo[2]; cz <= hi[2]; ++cz)
+            for (int cy = lo[1]; cy <= hi[1]; ++cy)
+                for (int cx = lo[0]; cx <= hi[0]; ++cx) {
+                    const uint2 range = pm.cells[cell_bucket(cx, cy, cz, pm.mask)];
+                    if (kCount) {
+                        cnt[C_CELLS]++;
+                        cnt[C_TESTED] += range.y - range.x;
+                        cnt[C_STREAMED] += range.y - range.x;
+                    }
+                    auto photon = [&](uint32_t i, const float4& p) {
+                        if (!(len2(g.q - f3(p.x, p.y, p.z)) < pm.r2)) return;
+                        if (kCount) {
+                            cnt[C_ACCEPTED]++;
+                            if (a.merge_sets) record_merge(a, slot, i);
+                        }
+                        const float4 dt = pm.dir_tr[i];
+                        const float2 gb = pm.tp_gb[i];
+                        if (fast) {
+                            add_diffuse_term(c, d, true, dt, gb, p.w * kInvPI);
+                        } else {
+                            float fwd, rev;
+                            const F3 e = bsdf_eval_pdf<kFast>(g.b, g.mc, f3(dt.x, dt.y, dt.z), fwd, rev);
+                            const float w = mdiv<kFast>(1.0f, 1.0f + g.cdvm * rev + p.w * fwd);
+                            c = c + (e * f3(dt.w, gb.x, gb.y)) * w;
+                        }
+                    };
+                    // four positions in flight at a time; the terms are still added in map order
+                    uint32_t i = range.x;
+                    for (; i + 4u <= range.y; i += 4u) {
+                        const float4 p0 = pm.pos[i], p1 = pm.pos[i + 1], p2 = pm.pos[i + 2], p3 = pm.pos[i + 3];
+                        photon(i, p0);
+                        photon(i + 1, p1);
+                        photon(i + 2, p2);
+                        photon(i + 3, p3);
+                    }
+                    for (; i < range.y; ++i) photon(i, pm.pos[i]);
+                }
+        const F3 r = fast ? g.mc.diffuse * c : c;
+        const F3 o = f3(t.x, t.y, t.z) * r;
+        a.contrib[3 * (size_t)slot] = o.x;
+        a.contrib[3 * (size_t)slot + 1] = o.y;
+        a.contrib[3 * (size_t)slot + 2] = o.z;
+        if (kCount) cnt[C_GATHERS]++;
+    }
+    if (kCount) flush_counters(a.counters, cnt);
+}
+
 // Camera pass, part 3: color = sum of the path's vertex contributions in
 // vertex order, added to the film (Vertex_Merging.cpp:347, :361).
 __global__ void k_resolve(CameraPassArgs a) {
@@ -1393,6 +1484,16 @@ static void blocked_variant(const CameraPassArgs& a, int blocks, hipStream_t s) 
 template <bool kBatch>
 static void gather_variant(const CameraPassArgs& a, int blocks, hipStream_t s) {
     dim3 g(blocks), b(kGatherBlock);
+    if (a.group_queries == 1) {
+        if (a.exact_merge) {
+            if (a.counters) hipLaunchKernelGGL((k_gather_lane<true, false, kBatch>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_gather_lane<false, false, kBatch>), g, b, 0, s, a);
+        } else {
+            if (a.counters) hipLaunchKernelGGL((k_gather_lane<true, true, kBatch>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_gather_lane<false, true, kBatch>), g, b, 0, s, a);
+        }
+        return;
+    }
     if (a.group_queries == 4) blocked_variant<kBatch, 4>(a, blocks, s);
     else if (a.group_queries == 16) blocked_variant<kBatch, 16>(a, blocks, s);
     else if (a.group_queries) blocked_variant<kBatch, 8>(a, blocks, s);

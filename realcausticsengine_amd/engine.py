@@ -29,8 +29,8 @@ EXPORTS = [
     "rc_gather_sets_noalloc", "rc_merge_sets", "rc_write_exr_image",
 ]
 TUNE_KEYS = ["ms_before", "ms_after", "path_chunk", "light_blocks_per_cu", "camera_blocks_per_cu", "cell_scale",
-             "gather_blocks_per_cu", "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile",
-             "gather_block", "query_order", "overlap"]
+             "light_refill", "light_order", "camera_refill", "group_queries", "gather_tile", "gather_block",
+             "gather_blocks_per_cu", "query_order", "overlap"]
 GUIDE_PARAMS = 87232
 GUIDE_BINS = 1024
 STAT_NAMES = ["light_paths", "photons", "camera_paths", "gathers", "photons_tested", "photons_accepted",

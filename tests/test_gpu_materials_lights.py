@@ -83,10 +83,11 @@ def test_every_bsdf_class_photon_list_bit_exact(oracle):
     eng.release()
 
 
-@pytest.mark.parametrize("exact", [1, 0])
-def test_every_bsdf_class_camera_pass(oracle, exact):
+@pytest.mark.parametrize("exact,group", [(1, 8), (0, 8), (1, 1), (0, 1)])
+def test_every_bsdf_class_camera_pass(oracle, exact, group):
+    # group 8: the grouped merge (the default), 1: one query per lane -- both through the general BSDF evaluation
     args = scenes.materials_scene()
-    eng = Engine(args, photons=131072, seed=3, exact_merge=exact)
+    eng = Engine(args, photons=131072, seed=3, exact_merge=exact, group_queries=group)
     orc = oracle.OracleScene(args, photons=131072, seed=3)
     ph = orc.trace_light(0)
     img_o = orc.camera_pass(0, ph)

@@ -32,7 +32,8 @@ def test_mix64_known_answer():
     assert int(mix64(0)) == 0xE220A8397B1DCDAF
 
 
-@pytest.mark.parametrize("group_queries,gather_block,exact", [(8, 2, 0), (16, 0, 1), (4, 4, 0), (0, 2, 0)])
+@pytest.mark.parametrize("group_queries,gather_block,exact", [(8, 2, 0), (16, 0, 1), (4, 4, 0), (0, 2, 0), (1, 2, 0),
+                                                            (1, 2, 1)])
 def test_merge_sets_equal_brute_force(oracle, group_queries, gather_block, exact):
     args = scenes.materials_scene(96, 64)
     args.base_radius = 0.05  # sets of tens of photons

@@ -86,11 +86,12 @@ def test_gather_sets_bit_exact(oracle):
     eng.release()
 
 
-@pytest.mark.parametrize("exact", [0, 1])
-def test_camera_pass_given_photons(oracle, exact):
-    # exact = 1: correctly rounded merge arithmetic; 0: hardware rcp/sqrt (<= 2 ulp)
+@pytest.mark.parametrize("exact,group", [(0, 8), (1, 8), (0, 1), (1, 1)])
+def test_camera_pass_given_photons(oracle, exact, group):
+    # exact = 1: correctly rounded merge arithmetic; 0: hardware rcp/sqrt (<= 2 ulp);
+    # group 8: the grouped merge (the default), 1: one query per lane
     args = scenes.c0_scene(48, 48, subdiv=3)
-    eng = Engine(args, photons=20000, seed=3, exact_merge=exact)
+    eng = Engine(args, photons=20000, seed=3, exact_merge=exact, group_queries=group)
     orc = oracle.OracleScene(args, photons=20000, seed=3)
     ph = orc.trace_light(0)
     img_o = orc.camera_pass(0, ph)

@@ -1,6 +1,7 @@
 """The merge in spatial query order (option query_order = 1, the default)
 against the merge in trace order, and the grouped merge (group_queries = 4,
-8 -- the default -- or 16 queries per wave) against the per-query merge.
+8 -- the default -- or 16 queries per wave) and the merge with one query per
+lane (group_queries = 1) against the per-query merge.
 
 In either merge a query's sum is formed by one wave in an order fixed by the
 query's cell window alone (whichever queries share its wave), and the film
@@ -17,7 +18,7 @@ from realcausticsengine_amd.engine import Batch, Engine
 pytestmark = pytest.mark.gpu
 
 
-def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int = 1024, block: int = 0,
+def _films(order: int, graph: int = 0, iters: int = 2, group: int = 8, tile: int = 1024, block: int = 0,
            exact: int = 0, overlap: int = 1):
     args = scenes.water_scene(128, 96, 80, samples=iters)
     eng = Engine(args, photons=200000, seed=4, write_exr=0, query_order=order, graph=graph, group_queries=group,
@@ -32,7 +33,7 @@ def _films(order: int, graph: int = 0, iters: int = 2, group: int = 1, tile: int
     return films, st
 
 
-@pytest.mark.parametrize("group", [0, 8])
+@pytest.mark.parametrize("group", [0, 8, 1])
 def test_sorted_merge_matches_trace_order(group):
     plain, st0 = _films(0, group=group)
     ordered, st1 = _films(1, group=group)
@@ -49,7 +50,7 @@ def test_sorted_merge_in_graph_matches_trace_order():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("group", [4, 8, 16])
+@pytest.mark.parametrize("group", [4, 8, 16, 1])
 def test_grouped_merge_matches_per_query_merge(group):
     per_query, s0 = _films(1, group=0)
     grouped, s1 = _films(1, group=group)
@@ -109,7 +110,7 @@ def test_overlapped_camera_trace_keeps_the_film(order, graph):
 def test_grouped_merge_counts_match_per_query_merge():
     args = scenes.water_scene(128, 96, 80, samples=1)
     st = []
-    for group, block in ((0, 0), (8, 0), (8, 4)):
+    for group, block in ((0, 0), (8, 0), (8, 4), (1, 0)):
         eng = Engine(args, photons=200000, seed=4, write_exr=0, count_events=1, group_queries=group,
                      gather_block=block)
         eng.prepare()
@@ -117,7 +118,8 @@ def test_grouped_merge_counts_match_per_query_merge():
         st.append(eng.stats())
         eng.release()
     for k in ("gathers", "photons_tested", "photons_accepted"):
-        assert st[0][k] == st[1][k] == st[2][k], k
+        assert st[0][k] == st[1][k] == st[2][k] == st[3][k], k
+    assert st[3]["cell_visits"] == st[0]["cell_visits"]  # one lookup per cell of every query's window
     assert st[1]["cell_visits"] <= st[0]["cell_visits"]  # one window lookup per sub-run of a group
     assert st[2]["cell_visits"] == st[1]["cell_visits"] and st[2]["photons_streamed"] == st[1]["photons_streamed"]
 
