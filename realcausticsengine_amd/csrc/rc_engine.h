@@ -314,6 +314,20 @@ void launch_guide_targets_batch(const float4* rec, uint32_t count, const BatchVi
 void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
                       uint32_t* hist, uint32_t* hist_sums, const uint32_t* count_dev, uint32_t n, int bits,
                       unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s);
+// The same sort, dropping from the keys it writes up to `droppable` low bits
+// once they are sorted (the caller reads only the bits above them); keys
+// travel as 32-bit words once the rest fits.  The sorted keys are `keys`
+// (32-bit words when keys32), shifted right by `dropped`; with droppable >=
+// bits no keys are returned.
+struct SortedPairs {
+    void* keys;
+    uint32_t* vals;
+    bool keys32;
+    int dropped;
+};
+SortedPairs radix_sort_pairs_drop(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a,
+                                  uint32_t* vals_b, uint32_t* hist, const uint32_t* count_dev, uint32_t n, int bits,
+                                  int droppable, hipStream_t s);
 // GPU BVH build (rc_bvh.hip): linear BVH over Morton codes (one triangle per
 // leaf) in the binary node layout; tri / shade / tri_id are reordered into
 // leaf order.  Returns the tree depth (< 0 on a launch error).

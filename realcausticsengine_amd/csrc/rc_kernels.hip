@@ -1694,25 +1694,37 @@ void launch_exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint32_t n, ui
 // ---------------------------------------------------------------------
 // The sort (radix_sort_pairs): one-sweep LSD radix sort of (key, value)
 // pairs on 8-bit digits (Adinets and Merrill, "Onesweep: A Faster Least
-// Significant Digit Radix Sort for GPUs", 2022).  One kernel counts the
-// digits of every pass, one scans them, then each pass is one kernel: a
-// workgroup takes the next tile by ticket, ranks its keys by digit (stable:
-// wave-ordered chunks, ballot matches within a wave), publishes its digit
-// counts and finds the counts of the tiles before it by decoupled look-back,
-// reorders the tile by digit in LDS and writes each digit's run contiguously.
+// Significant Digit Radix Sort for GPUs", 2022).  The digit counts of the
+// first pass come from a counting kernel (or the kernel that makes the keys);
+// then each pass is one kernel: a workgroup takes the next tile by ticket,
+// ranks its keys by digit (stable: wave-ordered chunks, ballot matches within
+// a wave), counts the next pass's digits, publishes its digit counts and finds
+// the counts of the tiles before it by decoupled look-back, reorders the tile
+// by digit in LDS and writes each digit's run contiguously.  A tile is 16
+// keys per thread of a workgroup of 256, 512 or 1024 threads (by key count):
+// a 4 M-key pass is 256 tiles of 16,384 keys (one 1024-thread workgroup per
+// CU, 150 KB of LDS) in one round, with short look-back chains.
 // Scratch (u32 units): two look-back status arrays of tiles x 256 64-bit
 // words (a pass reads one and clears the other for the next pass), the digit
-// offsets of 8 passes, and the tickets.
-constexpr int kOsThreads = 256;
+// counts of 8 passes (each in kOsCopies copies, added to by workgroup index
+// modulo kOsCopies, so that no count takes more than a few hundred atomics),
+// and the tickets.
 constexpr int kOsItems = 16;
-constexpr uint32_t kOsTile = kOsThreads * kOsItems;
-constexpr int kOsWaves = kOsThreads / 64;
+constexpr uint32_t kOsChunk = 64 * kOsItems;  // keys per wave, in index order
+constexpr uint32_t kOsMinTile = 256 * kOsItems;
 constexpr int kOsMaxPasses = 8;
-constexpr uint32_t kOsChunk = kOsTile / kOsWaves;  // keys per wave, in index order
+constexpr int kOsCopies = 8;
 
-__host__ __device__ constexpr size_t onesweep_tiles(size_t n) { return n == 0 ? 1 : (n + kOsTile - 1) / kOsTile; }
+// Tiles of the smallest tile size (the scratch holds that many status rows).
+__host__ __device__ constexpr size_t onesweep_tiles(size_t n) { return n == 0 ? 1 : (n + kOsMinTile - 1) / kOsMinTile; }
+// Threads per workgroup of the passes over n keys (the fastest of 256, 512 and
+// 1024 measured at 0.5, 1, 2, 4 and 8 M keys: one round of about 256 tiles of
+// 16,384 keys is best near 4 M keys, 8,192-key tiles above and below).
+static int onesweep_threads(uint32_t n) {
+    return n < (3u << 18) ? 256 : n < (3u << 20) ? 512 : n < (6u << 20) ? 1024 : 512;
+}
 __host__ __device__ constexpr size_t onesweep_scratch_u32(size_t n) {
-    return 2 * (2 * 256 * onesweep_tiles(n)) + kOsMaxPasses * 256 + 16;
+    return 2 * (2 * 256 * onesweep_tiles(n)) + kOsMaxPasses * kOsCopies * 256 + 16;
 }
 
 size_t photon_map_hist_entries(uint32_t max_count) { return onesweep_scratch_u32(max_count); }
@@ -1721,83 +1733,114 @@ size_t photon_map_sum_entries(uint32_t) { return 1; }  // the one-sweep sort nee
 // Element count of the map kernels: n, or the count in device memory (at most n) when n_dev is given.
 __device__ __forceinline__ uint32_t live_count(const uint32_t* n_dev, uint32_t n) { return n_dev ? min(*n_dev, n) : n; }
 
-__global__ void k_map_keys(const float4* rec, const uint32_t* n_dev, uint32_t n,
-                           const IterParams* dyn, float ox, float oy, float oz, float inv_cell, uint32_t mask,
-                           int uid_bits, unsigned long long* keys, uint32_t* vals) {
+// Pointers into the sort's scratch for up to n keys.
+struct OnesweepScratch {
+    unsigned long long* status[2];
+    uint32_t* dig;      // kOsMaxPasses x kOsCopies x 256 digit counts
+    uint32_t* tickets;  // one per pass, then the failure flag
+    size_t tiles;
+};
+static OnesweepScratch onesweep_scratch(uint32_t* hist, uint32_t n) {
+    OnesweepScratch o;
+    o.tiles = onesweep_tiles(n);
+    o.status[0] = reinterpret_cast<unsigned long long*>(hist);
+    o.status[1] = o.status[0] + 256 * o.tiles;
+    o.dig = hist + 2 * (2 * 256 * o.tiles);
+    o.tickets = o.dig + kOsMaxPasses * kOsCopies * 256;
+    return o;
+}
+// Zeroes the digit counts, the tickets, the flag and the first pass's status rows.
+static void onesweep_clear(const OnesweepScratch& o, hipStream_t s) {
+    launch_fill_u32(o.dig, 0u, kOsMaxPasses * kOsCopies * 256 + 16, s);
+    launch_fill_u32(reinterpret_cast<uint32_t*>(o.status[0]), 0u, 2 * 256 * o.tiles, s);
+}
+
+// Keys of the photon map, (bucket, light path), and the counts of their low
+// digits (the sort's first pass): 16 photons per thread, all record loads
+// issued first, one LDS count per photon and one global add per digit and
+// workgroup.
+constexpr int kKeyThreads = 256, kKeyItems = 16;
+__device__ __forceinline__ void flush_digit_counts(const uint32_t* h, uint32_t* dig) {
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&dig[(blockIdx.x % kOsCopies) * 256 + threadIdx.x], h[threadIdx.x]);
+}
+__global__ __launch_bounds__(kKeyThreads) void k_map_keys(const float4* rec, const uint32_t* n_dev, uint32_t n,
+                                                          const IterParams* dyn, float ox, float oy, float oz,
+                                                          float inv_cell, uint32_t mask, int uid_bits,
+                                                          unsigned long long* keys, uint32_t* vals, uint32_t* dig) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
     n = live_count(n_dev, n);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
     if (dyn) {
         ox = dyn->origin[0];
         oy = dyn->origin[1];
         oz = dyn->origin[2];
         inv_cell = dyn->inv_cell;
     }
-    const float4 p = rec[kPhotonRecord * (size_t)i];
-    const uint32_t path = __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z);
-    int cx = (int)floorf((p.x - ox) * inv_cell);
-    int cy = (int)floorf((p.y - oy) * inv_cell);
-    int cz = (int)floorf((p.z - oz) * inv_cell);
-    uint32_t bkt = cell_bucket(cx, cy, cz, mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)path;
-    vals[i] = i;
+    const uint32_t i0 = blockIdx.x * (kKeyThreads * kKeyItems) + threadIdx.x;
+    float4 p[kKeyItems];
+    uint32_t path[kKeyItems];
+#pragma unroll
+    for (int j = 0; j < kKeyItems; ++j) {
+        const uint32_t i = i0 + j * kKeyThreads;
+        p[j] = i < n ? rec[kPhotonRecord * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        path[j] = i < n ? __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z) : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kKeyItems; ++j) {
+        const uint32_t i = i0 + j * kKeyThreads;
+        if (i < n) {
+            int cx = (int)floorf((p[j].x - ox) * inv_cell);
+            int cy = (int)floorf((p[j].y - oy) * inv_cell);
+            int cz = (int)floorf((p[j].z - oz) * inv_cell);
+            uint32_t bkt = cell_bucket(cx, cy, cz, mask);
+            const unsigned long long k = ((unsigned long long)bkt << uid_bits) | (unsigned long long)path[j];
+            keys[i] = k;
+            vals[i] = i;
+            atomicAdd(&h[(uint32_t)k & 255u], 1u);
+        }
+    }
+    flush_digit_counts(h, dig);
 }
 
-// Digit counts of every pass: dig[p * 256 + d] += keys whose digit p is d
-// (per-wave LDS counts, then one global atomic per digit and workgroup).
-// Also clears the first pass's look-back status rows of the tiles that hold keys.
-__global__ __launch_bounds__(kOsThreads) void k_os_hist(const unsigned long long* keys, const uint32_t* n_dev,
-                                                        uint32_t n, int passes, uint32_t* dig,
-                                                        unsigned long long* status0) {
-    __shared__ uint32_t h[kOsWaves][kOsMaxPasses][256];
-    const uint32_t wave = threadIdx.x >> 6;
-    for (int w = 0; w < kOsWaves; ++w)
-        for (int p = 0; p < passes; ++p) h[w][p][threadIdx.x] = 0;
+// Counts of the low digits of the keys (the sort's first pass), for keys
+// made by a kernel that does not count them.
+constexpr int kOsHistItems = 8;  // keys per lane loaded before they are counted
+__global__ __launch_bounds__(256) void k_os_hist(const unsigned long long* keys, const uint32_t* n_dev, uint32_t n,
+                                                 uint32_t* dig) {
+    __shared__ uint32_t h[4][256];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    for (int w = 0; w < 4; ++w) h[w][threadIdx.x] = 0;
     __syncthreads();
     n = live_count(n_dev, n);
-    const size_t rows = 256 * ((size_t)(n + kOsTile - 1) / kOsTile);
-    for (size_t i = (size_t)blockIdx.x * kOsThreads + threadIdx.x; i < rows; i += (size_t)gridDim.x * kOsThreads)
-        status0[i] = 0ull;
-    // a digit shared by the whole wave (common in the high digits) is added once by the first lane
-    for (uint32_t i0 = blockIdx.x * kOsThreads + (threadIdx.x & ~63u); i0 < n; i0 += gridDim.x * kOsThreads) {
-        const uint32_t i = i0 + (threadIdx.x & 63u);
-        const bool valid = i < n;
-        const unsigned long long k = valid ? keys[i] : 0ull;
-        const uint64_t live = __ballot(valid);
-        for (int p = 0; p < passes; ++p) {
-            const uint32_t d = (uint32_t)(k >> (8 * p)) & 255u;
+    // each wave takes chunks of 64 x kOsHistItems keys, all loads issued before the counting;
+    // a digit shared by the whole wave is added once by the first lane
+    constexpr uint32_t chunk = 64u * kOsHistItems;
+    for (uint32_t c0 = (blockIdx.x * 4 + wave) * chunk; c0 < n; c0 += gridDim.x * 4 * chunk) {
+        unsigned long long k[kOsHistItems];
+#pragma unroll
+        for (int j = 0; j < kOsHistItems; ++j) {
+            const uint32_t i = c0 + j * 64 + lane;
+            k[j] = i < n ? keys[i] : 0ull;
+        }
+#pragma unroll
+        for (int j = 0; j < kOsHistItems; ++j) {
+            const bool valid = c0 + j * 64 + lane < n;
+            const uint64_t live = __ballot(valid);
+            if (live == 0) break;
+            const uint32_t d = (uint32_t)k[j] & 255u;
             const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
             if (__ballot(valid && d == d0) == live) {
-                if ((threadIdx.x & 63u) == 0) h[wave][p][d0] += (uint32_t)__builtin_popcountll(live);
+                if (lane == 0) h[wave][d0] += (uint32_t)__builtin_popcountll(live);
             } else if (valid) {
-                atomicAdd(&h[wave][p][d], 1u);
+                atomicAdd(&h[wave][d], 1u);
             }
         }
     }
     __syncthreads();
-    for (int p = 0; p < passes; ++p) {
-        uint32_t c = 0;
-        for (int w = 0; w < kOsWaves; ++w) c += h[w][p][threadIdx.x];
-        if (c) atomicAdd(&dig[p * 256 + threadIdx.x], c);
-    }
-}
-
-// Exclusive scan of each pass's 256 digit counts, in place (one workgroup).
-__global__ __launch_bounds__(256) void k_os_scan(uint32_t* dig, int passes) {
-    __shared__ uint32_t t[256];
-    for (int p = 0; p < passes; ++p) {
-        const uint32_t v = dig[p * 256 + threadIdx.x];
-        t[threadIdx.x] = v;
-        __syncthreads();
-        for (uint32_t off = 1; off < 256; off <<= 1) {
-            const uint32_t a = threadIdx.x >= off ? t[threadIdx.x - off] : 0u;
-            __syncthreads();
-            t[threadIdx.x] += a;
-            __syncthreads();
-        }
-        dig[p * 256 + threadIdx.x] = t[threadIdx.x] - v;
-        __syncthreads();
-    }
+    const uint32_t c = h[0][threadIdx.x] + h[1][threadIdx.x] + h[2][threadIdx.x] + h[3][threadIdx.x];
+    if (c) atomicAdd(&dig[(blockIdx.x % kOsCopies) * 256 + threadIdx.x], c);
 }
 
 // Look-back status word of (tile, digit): flag in the high half (1: the
@@ -1808,37 +1851,49 @@ constexpr unsigned long long kOsAggregate = 1ull << 32, kOsInclusive = 2ull << 3
 // sleeps): a bound on the wait.  A tile that gives up flags the failure, and
 // every write of the pass stays below n.
 constexpr uint32_t kOsSpinLimit = 1u << 24;
-constexpr int kOsLookback = 4;  // predecessors read per look-back round trip (measured best of 1, 4, 16, 32)
+constexpr int kOsLookback = 4;  // predecessors read per look-back round trip
 
-__global__ __launch_bounds__(kOsThreads) void k_os_pass(const unsigned long long* __restrict__ kin,
-                                                        const uint32_t* __restrict__ vin, const uint32_t* n_dev,
-                                                        uint32_t n, int shift, const uint32_t* __restrict__ dig,
+// One pass: digit (key >> shift) & 255 of keys of type KIn, whose counts over
+// all keys are the sums of the kOsCopies copies of dig[0..255]; the keys are written as KOut(key >> drop) (drop:
+// low bits no later pass or caller reads; 32-bit keys halve the pass's
+// bytes), and the counts of the next pass's digit, (key >> (shift + 8)) & 255,
+// are added to dig_next (when given).  The tile is reordered in LDS through
+// two 32-bit words per key: first the output index and the value, then the
+// output key.
+template <int kOsThreads, typename KIn, typename KOut>
+__global__ __launch_bounds__(kOsThreads) void k_os_pass(const KIn* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                        const uint32_t* n_dev, uint32_t n, int shift, int drop,
+                                                        const uint32_t* __restrict__ dig, uint32_t* dig_next,
                                                         unsigned long long* status, unsigned long long* status_next,
-                                                        uint32_t* ticket, uint32_t* fail,
-                                                        unsigned long long* __restrict__ kout,
+                                                        uint32_t* ticket, uint32_t* fail, KOut* __restrict__ kout,
                                                         uint32_t* __restrict__ vout) {
-    __shared__ unsigned long long s_key[kOsTile];
-    __shared__ uint32_t s_val[kOsTile];
+    constexpr uint32_t kOsTile = kOsThreads * kOsItems;
+    constexpr int kOsWaves = kOsThreads / 64;
+    __shared__ uint32_t s_a[kOsTile];
+    __shared__ uint32_t s_b[kOsTile];
     __shared__ uint32_t s_cnt[kOsWaves][256];  // per wave: digit counts, then the waves-before prefix
     __shared__ uint32_t s_excl[256];           // tile-exclusive digit offsets
-    __shared__ uint32_t s_glob[256];           // output index of this tile's first key of each digit
+    __shared__ uint32_t s_glob[256];           // scan of the global counts, then the output index of position 0
+    __shared__ uint32_t s_next[256];           // the tile's counts of the next pass's digits
     __shared__ uint32_t s_tile;
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    const bool digit_thread = tid < 256;  // thread tid < 256 handles digit tid
     if (tid == 0) s_tile = atomicAdd(ticket, 1u);
-    for (int w = 0; w < kOsWaves; ++w) s_cnt[w][tid] = 0;
+    for (uint32_t i = tid; i < kOsWaves * 256; i += kOsThreads) s_cnt[i >> 8][i & 255u] = 0;
+    if (digit_thread) s_next[tid] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
     n = live_count(n_dev, n);
     const uint32_t base = tile * kOsTile;
     if (base >= n) return;  // no keys: no tile holding keys reads this one's status
-    status_next[(size_t)tile * 256 + tid] = 0ull;  // the next pass's status row (read after this kernel ends)
+    if (digit_thread) status_next[(size_t)tile * 256 + tid] = 0ull;  // read after this kernel ends
     // 1. ranks within the wave's chunk: kOsItems rounds of 64 keys in index order
-    unsigned long long key[kOsItems];
-    uint32_t val[kOsItems], rank[kOsItems];
+    KIn key[kOsItems];
+    uint32_t val[kOsItems], at[kOsItems];
 #pragma unroll
     for (int j = 0; j < kOsItems; ++j) {  // every load issued before the ranking's wave barriers
         const uint32_t i = base + wave * kOsChunk + j * 64 + lane;
-        key[j] = i < n ? kin[i] : 0ull;
+        key[j] = i < n ? kin[i] : (KIn)0;
         val[j] = i < n ? vin[i] : 0u;
     }
 #pragma unroll
@@ -1857,94 +1912,129 @@ __global__ __launch_bounds__(kOsThreads) void k_os_pass(const unsigned long long
         __builtin_amdgcn_wave_barrier();
         if (valid && r == 0) s_cnt[wave][d] = before + (uint32_t)__builtin_popcountll(peers);
         __builtin_amdgcn_wave_barrier();
-        rank[j] = before + r;
+        at[j] = before + r;  // rank among the wave's keys of digit d
+        if (dig_next && valid) atomicAdd(&s_next[(uint32_t)(key[j] >> (shift + 8)) & 255u], 1u);
     }
     __syncthreads();
-    // 2. thread tid = digit tid: the tile's count, the waves-before prefixes, the tile-exclusive offsets
-    uint32_t h = 0;
+    // 2. digit tid: the tile's count, the waves-before prefixes; scans of the tile's counts and of the global counts
+    uint32_t h = 0, g = 0;
+    if (digit_thread) {
 #pragma unroll
-    for (int w = 0; w < kOsWaves; ++w) {
-        const uint32_t c = s_cnt[w][tid];
-        s_cnt[w][tid] = h;
-        h += c;
+        for (int w = 0; w < kOsWaves; ++w) {
+            const uint32_t c = s_cnt[w][tid];
+            s_cnt[w][tid] = h;
+            h += c;
+        }
+#pragma unroll
+        for (int c = 0; c < kOsCopies; ++c) g += dig[c * 256 + tid];
+        s_excl[tid] = h;
+        s_glob[tid] = g;
     }
-    s_excl[tid] = h;
     __syncthreads();
     for (uint32_t off = 1; off < 256; off <<= 1) {
-        const uint32_t a = tid >= off ? s_excl[tid - off] : 0u;
+        uint32_t a = 0, b = 0;
+        if (digit_thread && tid >= off) {
+            a = s_excl[tid - off];
+            b = s_glob[tid - off];
+        }
         __syncthreads();
-        s_excl[tid] += a;
+        if (digit_thread) {
+            s_excl[tid] += a;
+            s_glob[tid] += b;
+        }
         __syncthreads();
     }
-    const uint32_t excl = s_excl[tid] - h;
+    uint32_t excl = 0, gexcl = 0;
+    if (digit_thread) {
+        excl = s_excl[tid] - h;
+        gexcl = s_glob[tid] - g;
+    }
     __syncthreads();
-    s_excl[tid] = excl;
-    // 3. decoupled look-back for digit tid
-    unsigned long long* st = status + (size_t)tile * 256 + tid;
-    __hip_atomic_store(st, (tile == 0 ? kOsInclusive : kOsAggregate) | h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint32_t prefix = 0;
-    bool gave_up = false;
-    if (tile > 0) {
-        // a window of kOsLookback predecessors per round trip (all tiles of the first wave publish at about the
-        // same time, so the nearest inclusive count is often several tiles back)
-        int j = (int)tile - 1;
-        uint32_t spins = 0;
-        bool done = false;
-        while (!done) {
-            unsigned long long w[kOsLookback];
+    if (digit_thread) {
+        s_excl[tid] = excl;
+        if (dig_next && s_next[tid]) atomicAdd(&dig_next[(tile % kOsCopies) * 256 + tid], s_next[tid]);
+        // 3. decoupled look-back for digit tid
+        unsigned long long* st = status + (size_t)tile * 256 + tid;
+        __hip_atomic_store(st, (tile == 0 ? kOsInclusive : kOsAggregate) | h, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t prefix = 0;
+        bool gave_up = false;
+        if (tile > 0) {
+            // a window of kOsLookback predecessors per round trip
+            int j = (int)tile - 1;
+            uint32_t spins = 0;
+            bool done = false;
+            while (!done) {
+                unsigned long long w[kOsLookback];
 #pragma unroll
-            for (int k = 0; k < kOsLookback; ++k)
-                w[k] = j - k >= 0 ? __hip_atomic_load(status + (size_t)(j - k) * 256 + tid, __ATOMIC_RELAXED,
-                                                      __HIP_MEMORY_SCOPE_AGENT)
-                                  : kOsInclusive;  // below tile 0: nothing more to add
-            int k = 0;
-            for (; k < kOsLookback; ++k) {
-                if ((w[k] >> 32) == 0ull) break;  // not yet published: read again from here
-                prefix += (uint32_t)w[k];
-                if ((w[k] >> 32) == 2ull) {
-                    done = true;
-                    break;
+                for (int k = 0; k < kOsLookback; ++k)
+                    w[k] = j - k >= 0 ? __hip_atomic_load(status + (size_t)(j - k) * 256 + tid, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kOsInclusive;  // below tile 0: nothing more to add
+                int k = 0;
+                for (; k < kOsLookback; ++k) {
+                    if ((w[k] >> 32) == 0ull) break;  // not yet published: read again from here
+                    prefix += (uint32_t)w[k];
+                    if ((w[k] >> 32) == 2ull) {
+                        done = true;
+                        break;
+                    }
+                }
+                if (done) break;
+                j -= k;
+                if (k < kOsLookback) {
+                    if (++spins >= kOsSpinLimit) {
+                        gave_up = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
                 }
             }
-            if (done) break;
-            j -= k;
-            if (k < kOsLookback) {
-                if (++spins >= kOsSpinLimit) {
-                    gave_up = true;
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
-            }
+            __hip_atomic_store(st, kOsInclusive | (prefix + h), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        __hip_atomic_store(st, kOsInclusive | (prefix + h), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    if (gave_up) atomicOr(fail, 1u);
-    s_glob[tid] = dig[tid] + prefix;
-    __syncthreads();
-    // 4. the tile in digit order in LDS, then each digit's run written contiguously
-#pragma unroll
-    for (int j = 0; j < kOsItems; ++j) {
-        const uint32_t i = base + wave * kOsChunk + j * 64 + lane;
-        if (i < n) {
-            const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
-            const uint32_t at = s_excl[d] + s_cnt[wave][d] + rank[j];
-            s_key[at] = key[j];
-            s_val[at] = val[j];
-        }
+        if (gave_up) atomicOr(fail, 1u);
+        s_glob[tid] = gexcl + prefix - excl;
     }
     __syncthreads();
+    // 4. the tile in digit order in LDS (position at = digit offset + waves before + rank): first each key's
+    // output index and value, then its output key; each digit's run is written contiguously
     const uint32_t cnt = min(kOsTile, n - base);
 #pragma unroll
     for (int j = 0; j < kOsItems; ++j) {
+        if (base + wave * kOsChunk + j * 64 + lane < n) {
+            const uint32_t d = (uint32_t)(key[j] >> shift) & 255u;
+            at[j] += s_excl[d] + s_cnt[wave][d];
+            s_a[at[j]] = s_glob[d] + at[j];
+            s_b[at[j]] = val[j];
+        }
+    }
+    __syncthreads();
+    uint32_t out[kOsItems];
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
         const uint32_t e = j * kOsThreads + tid;
-        if (e < cnt) {
-            const unsigned long long k = s_key[e];
-            const uint32_t d = (uint32_t)(k >> shift) & 255u;
-            const uint32_t out = s_glob[d] + (e - s_excl[d]);
-            if (out < n) {
-                kout[out] = k;
-                vout[out] = s_val[e];
-            }
+        out[j] = e < cnt ? s_a[e] : n;
+        if (out[j] < n) vout[out[j]] = s_b[e];
+    }
+    if (kout == nullptr) return;  // the caller reads no keys after this pass (uniform over the workgroup)
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        if (base + wave * kOsChunk + j * 64 + lane < n) {
+            const KOut k = (KOut)(key[j] >> drop);
+            s_a[at[j]] = (uint32_t)k;
+            if (sizeof(KOut) == 8) s_b[at[j]] = (uint32_t)((unsigned long long)k >> 32);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kOsItems; ++j) {
+        if (out[j] < n) {
+            const uint32_t e = j * kOsThreads + tid;
+            if (sizeof(KOut) == 8)
+                kout[out[j]] = (KOut)((unsigned long long)s_a[e] | ((unsigned long long)s_b[e] << 32));
+            else
+                kout[out[j]] = (KOut)s_a[e];
         }
     }
 }
@@ -1963,43 +2053,69 @@ __global__ void k_map_permute(const float4* rec, const uint32_t* vals, const uin
     oidx[i] = j;
 }
 
-__global__ void k_map_cells(const unsigned long long* keys, const uint32_t* n_dev, uint32_t n, int uid_bits,
-                            uint2* cells) {
+// Bucket ranges from the sorted keys; the bucket is key >> shift.
+template <typename K>
+__global__ void k_map_cells(const K* keys, const uint32_t* n_dev, uint32_t n, int shift, uint2* cells) {
     n = live_count(n_dev, n);
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t b = (uint32_t)(keys[i] >> uid_bits);
-    if (i == 0 || (uint32_t)(keys[i - 1] >> uid_bits) != b) cells[b].x = i;
-    if (i == n - 1 || (uint32_t)(keys[i + 1] >> uid_bits) != b) cells[b].y = i + 1;
+    uint32_t b = (uint32_t)(keys[i] >> shift);
+    if (i == 0 || (uint32_t)(keys[i - 1] >> shift) != b) cells[b].x = i;
+    if (i == n - 1 || (uint32_t)(keys[i + 1] >> shift) != b) cells[b].y = i + 1;
 }
 
-__global__ void k_map_keys_batch(const float4* rec, const uint32_t* n_dev, uint32_t n,
-                                 BatchView b, int uid_bits, unsigned long long* keys, uint32_t* vals) {
+__global__ __launch_bounds__(kKeyThreads) void k_map_keys_batch(const float4* rec, const uint32_t* n_dev, uint32_t n,
+                                                                BatchView b, int uid_bits, unsigned long long* keys,
+                                                                uint32_t* vals, uint32_t* dig) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
     n = live_count(n_dev, n);
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 p = rec[kPhotonRecord * (size_t)i];
-    const uint32_t u = __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z);
-    int sc = batch_scene_of(b.light_starts, b.n, u);
-    const PhotonMapView& pm = b.pm[sc];
-    int cx = (int)floorf((p.x - pm.origin[0]) * pm.inv_cell);
-    int cy = (int)floorf((p.y - pm.origin[1]) * pm.inv_cell);
-    int cz = (int)floorf((p.z - pm.origin[2]) * pm.inv_cell);
-    uint32_t bkt = b.bs[sc].bucket_base + cell_bucket(cx, cy, cz, pm.mask);
-    keys[i] = ((unsigned long long)bkt << uid_bits) | (unsigned long long)u;
-    vals[i] = i;
+    const uint32_t i0 = blockIdx.x * (kKeyThreads * kKeyItems) + threadIdx.x;
+    float4 p[kKeyItems];
+    uint32_t path[kKeyItems];
+#pragma unroll
+    for (int j = 0; j < kKeyItems; ++j) {
+        const uint32_t i = i0 + j * kKeyThreads;
+        p[j] = i < n ? rec[kPhotonRecord * (size_t)i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        path[j] = i < n ? __float_as_uint(rec[kPhotonRecord * (size_t)i + 2].z) : 0u;
+    }
+    __syncthreads();
+    for (int j = 0; j < kKeyItems; ++j) {
+        const uint32_t i = i0 + j * kKeyThreads;
+        if (i < n) {
+            const uint32_t u = path[j];
+            int sc = batch_scene_of(b.light_starts, b.n, u);
+            const PhotonMapView& pm = b.pm[sc];
+            int cx = (int)floorf((p[j].x - pm.origin[0]) * pm.inv_cell);
+            int cy = (int)floorf((p[j].y - pm.origin[1]) * pm.inv_cell);
+            int cz = (int)floorf((p[j].z - pm.origin[2]) * pm.inv_cell);
+            uint32_t bkt = b.bs[sc].bucket_base + cell_bucket(cx, cy, cz, pm.mask);
+            const unsigned long long k = ((unsigned long long)bkt << uid_bits) | (unsigned long long)u;
+            keys[i] = k;
+            vals[i] = i;
+            atomicAdd(&h[(uint32_t)k & 255u], 1u);
+        }
+    }
+    flush_digit_counts(h, dig);
 }
 
-static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s);
+static void sort_permute_cells(const MapBuildArgs& a, const OnesweepScratch& o, hipStream_t s);
+static SortedPairs onesweep_passes(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a,
+                                   uint32_t* vals_b, const OnesweepScratch& o, const uint32_t* count_dev, uint32_t n,
+                                   int bits, int droppable, hipStream_t s);
+
+static unsigned map_key_blocks(uint32_t n) { return (n + kKeyThreads * kKeyItems - 1) / (kKeyThreads * kKeyItems); }
 
 void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
     uint32_t n = a.count;
     if (n == 0) return;
-    unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys, dim3(g), dim3(256), 0, s, a.in_rec, a.count_dev, n, a.dyn, a.origin[0],
-                       a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a);
-    sort_permute_cells(a, s);
+    const OnesweepScratch o = onesweep_scratch(a.hist, n);
+    onesweep_clear(o, s);
+    hipLaunchKernelGGL(k_map_keys, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev, n, a.dyn,
+                       a.origin[0], a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a,
+                       o.dig);
+    sort_permute_cells(a, o, s);
 }
 
 // a.mask + 1 = the total bucket count of all scenes.
@@ -2007,45 +2123,98 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
     launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
     uint32_t n = a.count;
     if (n == 0) return;
-    unsigned g = (n + 255) / 256;
-    hipLaunchKernelGGL(k_map_keys_batch, dim3(g), dim3(256), 0, s, a.in_rec, a.count_dev, n, b,
-                       a.uid_bits, a.keys_a, a.vals_a);
-    sort_permute_cells(a, s);
+    const OnesweepScratch o = onesweep_scratch(a.hist, n);
+    onesweep_clear(o, s);
+    hipLaunchKernelGGL(k_map_keys_batch, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev, n,
+                       b, a.uid_bits, a.keys_a, a.vals_a, o.dig);
+    sort_permute_cells(a, o, s);
 }
 
-// LSD radix sort of the keys (8-bit digits, stable), permutation of the
-// photons into key order, per-bucket [start, end).
+template <int T>
+static void launch_os_pass(bool in32, bool out32, dim3 g, dim3 b, hipStream_t s, const void* kin, const uint32_t* vin,
+                           const uint32_t* count_dev, uint32_t n, int shift, int drop, const uint32_t* dp,
+                           uint32_t* dnext, unsigned long long* st, unsigned long long* sn, uint32_t* tk, uint32_t* fl,
+                           void* ko, uint32_t* vout) {
+    using u64 = unsigned long long;
+    if (!in32 && !out32)
+        hipLaunchKernelGGL((k_os_pass<T, u64, u64>), g, b, 0, s, (const u64*)kin, vin, count_dev, n, shift, drop, dp,
+                           dnext, st, sn, tk, fl, (u64*)ko, vout);
+    else if (!in32)
+        hipLaunchKernelGGL((k_os_pass<T, u64, uint32_t>), g, b, 0, s, (const u64*)kin, vin, count_dev, n, shift, drop,
+                           dp, dnext, st, sn, tk, fl, (uint32_t*)ko, vout);
+    else
+        hipLaunchKernelGGL((k_os_pass<T, uint32_t, uint32_t>), g, b, 0, s, (const uint32_t*)kin, vin, count_dev, n,
+                           shift, drop, dp, dnext, st, sn, tk, fl, (uint32_t*)ko, vout);
+}
+
+// LSD radix sort of (key, value) pairs on the low `bits` key bits (8-bit
+// digits, stable), the first pass's digit counts already in o.dig.  After
+// each pass the low key bits already sorted are dropped from the keys
+// written, up to `droppable` of them (bits the caller does not read), and
+// once the rest fits 32 bits the keys travel as 32-bit words; with droppable
+// >= bits the last pass writes no keys.
+static SortedPairs onesweep_passes(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a,
+                                   uint32_t* vals_b, const OnesweepScratch& o, const uint32_t* count_dev, uint32_t n,
+                                   int bits, int droppable, hipStream_t s) {
+    const int passes = std::min(kOsMaxPasses, (bits + 7) / 8);
+    void* kin = keys_a;
+    void* kout = keys_b;
+    uint32_t* vin = vals_a;
+    uint32_t* vout = vals_b;
+    int dropped = 0;
+    bool in32 = false;
+    using u64 = unsigned long long;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p - dropped;
+        const int nd = std::min(8 * (p + 1), std::max(droppable, dropped));
+        const bool out32 = bits - nd <= 32;
+        const int drop = nd - dropped;
+        void* ko = (p == passes - 1 && droppable >= bits) ? nullptr : kout;
+        uint32_t* dnext = p + 1 < passes ? o.dig + kOsCopies * 256 * (p + 1) : nullptr;
+        const uint32_t* dp = o.dig + kOsCopies * 256 * p;
+        u64* st = o.status[p & 1];
+        u64* sn = o.status[(p + 1) & 1];
+        uint32_t* tk = o.tickets + p;
+        uint32_t* fl = o.tickets + kOsMaxPasses;
+        const int threads = onesweep_threads(n);
+        const dim3 g((unsigned)((n + threads * kOsItems - 1) / (threads * kOsItems))), b(threads);
+        if (threads == 1024)
+            launch_os_pass<1024>(in32, out32, g, b, s, kin, vin, count_dev, n, shift, drop, dp, dnext, st, sn, tk, fl,
+                                 ko, vout);
+        else if (threads == 512)
+            launch_os_pass<512>(in32, out32, g, b, s, kin, vin, count_dev, n, shift, drop, dp, dnext, st, sn, tk, fl,
+                                ko, vout);
+        else
+            launch_os_pass<256>(in32, out32, g, b, s, kin, vin, count_dev, n, shift, drop, dp, dnext, st, sn, tk, fl,
+                                ko, vout);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        dropped = nd;
+        in32 = out32;
+    }
+    return SortedPairs{kin, vin, in32, dropped};
+}
+
+SortedPairs radix_sort_pairs_drop(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a,
+                                  uint32_t* vals_b, uint32_t* hist, const uint32_t* count_dev, uint32_t n, int bits,
+                                  int droppable, hipStream_t s) {
+    if (n == 0 || bits <= 0) return SortedPairs{keys_a, vals_a, false, 0};
+    const OnesweepScratch o = onesweep_scratch(hist, n);
+    onesweep_clear(o, s);
+    const unsigned hg = (unsigned)std::min<size_t>(512, ((size_t)n + 8191) / 8192);
+    hipLaunchKernelGGL(k_os_hist, dim3(hg), dim3(256), 0, s, keys_a, count_dev, n, o.dig);
+    return onesweep_passes(keys_a, keys_b, vals_a, vals_b, o, count_dev, n, bits, droppable, s);
+}
+
 void radix_sort_pairs(unsigned long long* keys_a, unsigned long long* keys_b, uint32_t* vals_a, uint32_t* vals_b,
                       uint32_t* hist, uint32_t*, const uint32_t* count_dev, uint32_t n, int bits,
                       unsigned long long** keys_out, uint32_t** vals_out, hipStream_t s) {
-    const int passes = std::min(kOsMaxPasses, (bits + 7) / 8);
-    *keys_out = keys_a;
-    *vals_out = vals_a;
-    if (n == 0 || passes == 0) return;
-    const size_t tiles = onesweep_tiles(n);
-    unsigned long long* status[2] = {reinterpret_cast<unsigned long long*>(hist),
-                                     reinterpret_cast<unsigned long long*>(hist) + 256 * tiles};
-    uint32_t* dig = hist + 2 * (2 * 256 * tiles);
-    uint32_t* tickets = dig + kOsMaxPasses * 256;  // one per pass, then the failure flag
-    // the digit counts, the tickets and the flag start at zero; k_os_hist clears the first status rows in use
-    launch_fill_u32(dig, 0u, kOsMaxPasses * 256 + 16, s);
-    hipLaunchKernelGGL(k_os_hist, dim3((unsigned)std::min<size_t>(tiles, 512)), dim3(kOsThreads), 0, s, keys_a,
-                       count_dev, n, passes, dig, status[0]);
-    hipLaunchKernelGGL(k_os_scan, dim3(1), dim3(256), 0, s, dig, passes);
-    unsigned long long* kin = keys_a;
-    unsigned long long* kout = keys_b;
-    uint32_t* vin = vals_a;
-    uint32_t* vout = vals_b;
-    for (int p = 0; p < passes; ++p) {
-        hipLaunchKernelGGL(k_os_pass, dim3((unsigned)tiles), dim3(kOsThreads), 0, s, kin, vin, count_dev, n, 8 * p,
-                           dig + 256 * p, status[p & 1], status[(p + 1) & 1], tickets + p, tickets + kOsMaxPasses,
-                           kout, vout);
-        std::swap(kin, kout);
-        std::swap(vin, vout);
-    }
-    *keys_out = kin;
-    *vals_out = vin;
+    const SortedPairs r = radix_sort_pairs_drop(keys_a, keys_b, vals_a, vals_b, hist, count_dev, n, bits, 0, s);
+    *keys_out = static_cast<unsigned long long*>(r.keys);
+    *vals_out = r.vals;
 }
+
+
 
 __global__ void k_query_keys(CameraPassArgs a, uint32_t n, unsigned long long* keys, uint32_t* vals) {
     n = live_count(a.path_counter + 1, min(n, a.q.capacity));
@@ -2116,10 +2285,7 @@ const uint32_t* sort_light_paths(const LightPassArgs& a, const QuerySortBuffers&
         bits = 32 + sb;
     }
     hipLaunchKernelGGL(k_light_keys, dim3((n + 255) / 256), dim3(256), 0, s, a, b.keys_a, b.vals_a);
-    unsigned long long* kout;
-    uint32_t* vout;
-    radix_sort_pairs(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, b.hist_sums, nullptr, n, bits, &kout, &vout, s);
-    return vout;
+    return radix_sort_pairs_drop(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, nullptr, n, bits, bits, s).vals;
 }
 
 const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b, uint32_t n, hipStream_t s) {
@@ -2131,23 +2297,25 @@ const uint32_t* sort_queries(const CameraPassArgs& a, const QuerySortBuffers& b,
         bits = 36 + sb;
     }
     hipLaunchKernelGGL(k_query_keys, dim3((n + 255) / 256), dim3(256), 0, s, a, n, b.keys_a, b.vals_a);
-    unsigned long long* kout;
-    uint32_t* vout;
-    radix_sort_pairs(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, b.hist_sums, a.path_counter + 1, n, bits, &kout,
-                     &vout, s);
-    return vout;
+    return radix_sort_pairs_drop(b.keys_a, b.keys_b, b.vals_a, b.vals_b, b.hist, a.path_counter + 1, n, bits, bits, s)
+        .vals;
 }
 
-static void sort_permute_cells(const MapBuildArgs& a, hipStream_t s) {
+static void sort_permute_cells(const MapBuildArgs& a, const OnesweepScratch& o, hipStream_t s) {
     uint32_t n = a.count;
     unsigned g = (n + 255) / 256;
-    unsigned long long* kin;
-    uint32_t* vin;
-    radix_sort_pairs(a.keys_a, a.keys_b, a.vals_a, a.vals_b, a.hist, a.hist_sums, a.count_dev, n,
-                     a.bucket_bits + a.uid_bits, &kin, &vin, s);
-    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_rec, vin, a.count_dev, n, a.out_pos, a.out_dir,
-                       a.out_tp, a.out_index);
-    hipLaunchKernelGGL(k_map_cells, dim3(g), dim3(256), 0, s, kin, a.count_dev, n, a.uid_bits, a.cells);
+    // the light-path bits order photons within a bucket; the ranges below read only the bucket bits
+    const SortedPairs r = onesweep_passes(a.keys_a, a.keys_b, a.vals_a, a.vals_b, o, a.count_dev, n,
+                                          a.bucket_bits + a.uid_bits, a.uid_bits, s);
+    hipLaunchKernelGGL(k_map_permute, dim3(g), dim3(256), 0, s, a.in_rec, r.vals, a.count_dev, n, a.out_pos,
+                       a.out_dir, a.out_tp, a.out_index);
+    const int shift = a.uid_bits - r.dropped;
+    if (r.keys32)
+        hipLaunchKernelGGL(k_map_cells<uint32_t>, dim3(g), dim3(256), 0, s, static_cast<const uint32_t*>(r.keys),
+                           a.count_dev, n, shift, a.cells);
+    else
+        hipLaunchKernelGGL(k_map_cells<unsigned long long>, dim3(g), dim3(256), 0, s,
+                           static_cast<const unsigned long long*>(r.keys), a.count_dev, n, shift, a.cells);
 }
 
 // ---------------------------------------------------------------------
