@@ -406,7 +406,8 @@ def time_to_rmse(a, cpu_s_per_iteration, cpu_all_s_per_iteration=None):
         eng = engine(guided, a.photons, qmc)
         tuner = convergence.PhotonBudgetTuner(4 << 20, 1 << 20, pilots=6)
         run = convergence.run_tuned(eng, ref, target, a.ttr_max_iterations, tuner, label=f"{name} tuned",
-                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune)
+                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune,
+                                    online=not a.offline_tune)
         mfma = None
         if guided and not qmc:
             eng.set("guide_train_every", 1)
@@ -439,7 +440,8 @@ def time_to_rmse(a, cpu_s_per_iteration, cpu_all_s_per_iteration=None):
         fb = convergence.FrameBatch(args, a.ttr_frames, opts)
         tuner = convergence.PhotonBudgetTuner(1 << 20, 1 << 18, pilots=6)
         run = convergence.run_tuned(fb, ref, target, a.ttr_max_iterations, tuner, label=f"{name}",
-                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune)
+                                    max_seconds=a.ttr_max_seconds, autotune=not a.no_autotune,
+                                    online=not a.offline_tune)
         fb.release()
         f = tuner.fit
         leg = run.summary(target)
@@ -549,6 +551,9 @@ def main() -> int:
                     help="independent frames of the scene per batched launch in the batched time-to-RMSE legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven-scene batch leg")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes")
+    ap.add_argument("--offline-tune", action="store_true",
+                    help="time-to-RMSE legs: tune launch shapes with rc_autotune (scratch renders, timed) instead of "
+                         "on the run's own iterations")
     ap.add_argument("--launch", default="", help="fixed launch shapes instead of autotuning, "
                     "e.g. path_chunk=512,light_blocks_per_cu=8,gather_blocks_per_cu=32")
     ap.add_argument("--no-graph", action="store_true", help="skip the iteration-graph leg")

@@ -187,6 +187,96 @@ class PhotonBudgetTuner:
             self._deltas = {}
 
 
+# The launch knobs rc_autotune tries, in its order (rc_engine.cpp, tune_knobs), each with its candidates and the
+# per-iteration times (stats, ms) that judge it.  The merge's time is scaled by (r_first / r)^2: with a shrinking merge
+# radius the merge's work shrinks from one iteration to the next.
+ONLINE_KNOBS = (
+    ("path_chunk", (64, 128, 256, 512, 1024), ("ms_light", "ms_camera_trace")),
+    ("light_blocks_per_cu", (4, 6, 8, 12, 16), ("ms_light",)),
+    ("camera_blocks_per_cu", (4, 6, 8, 12, 16), ("ms_camera_trace",)),
+    ("cell_scale", (0.67, 0.8, 1.0, 1.25), ("ms_map", "ms_gather")),
+    ("light_refill", (1, 8, 16, 32), ("ms_light",)),
+    ("light_order", (0, 1), ("ms_light",)),
+    ("camera_refill", (1, 8, 16, 32), ("ms_camera_trace",)),
+    ("group_queries", (4, 8, 16, 1), ("ms_gather",)),
+    ("gather_tile", (1024, 256, 0), ("ms_gather",)),
+    ("gather_block", (2, 4, 1, 0), ("ms_gather",)),
+    ("gather_blocks_per_cu", (5, 8, 12, 16, 24), ("ms_gather",)),
+    ("query_order", (1, 0), ("ms_camera_trace", "ms_query_sort", "ms_gather")),
+    ("overlap", (0, 1), ("ms_total",)),
+)
+OVERLAP_MARGIN = 0.02  # overlap is kept only when it saves this fraction of the iteration's wall time
+
+
+class OnlineTuner:
+    """Launch-shape tuning on the run's own iterations.
+
+    rc_autotune renders one iteration again and again into a scratch film,
+    once per candidate; here each candidate value of each knob renders the
+    next iteration of the run instead, which counts in the image and in the
+    wall time like any other (launch shapes change how the work is spread
+    over the GPU and the order of floating-point sums, not what is
+    estimated).  Each knob keeps its cheapest candidate, one knob after the
+    other in rc_autotune's order, judged by the same kernel times; the camera
+    trace stays serial until the last knob, overlap, which is judged by the
+    iteration's wall time and kept only when it saves OVERLAP_MARGIN.  One
+    iteration per candidate."""
+
+    def __init__(self, eng, knobs=ONLINE_KNOBS):
+        self.eng = eng
+        self.knobs = list(knobs)
+        self.k = 0      # knob being tuned
+        self.c = 0      # its candidate rendering next
+        self.best = None
+        self.r0 = None
+        self.chosen: dict = {}
+        self.iterations = 0
+        self.walls: list = []
+
+    @property
+    def done(self) -> bool:
+        return self.k >= len(self.knobs)
+
+    def before_iteration(self) -> None:
+        if self.done:
+            return
+        if self.k == 0 and self.c == 0:
+            self.eng.set("overlap", 0)  # the kernel knobs are judged with the camera trace serial
+        name, cands, _ = self.knobs[self.k]
+        self.eng.set(name, cands[self.c])
+
+    def after_iteration(self, st: dict) -> None:
+        if self.done:
+            return
+        name, cands, parts = self.knobs[self.k]
+        r = float(st.get("radius", 0.0))
+        if self.r0 is None:
+            self.r0 = r
+        scale = (self.r0 / r) ** 2 if r > 0 and self.r0 > 0 else 1.0
+        cost = sum(float(st[p]) * (scale if p == "ms_gather" else 1.0) for p in parts)
+        margin = OVERLAP_MARGIN if name == "overlap" else 0.0
+        if self.best is None or cost < self.best[0] * (1.0 - margin):
+            self.best = (cost, cands[self.c])
+        self.iterations += 1
+        self.walls.append(float(st["ms_total"]))
+        self.c += 1
+        if self.c == len(cands):
+            self.eng.set(name, self.best[1])
+            self.chosen[name] = self.best[1]
+            self.k += 1
+            self.c = 0
+            self.best = None
+
+    def report(self) -> dict:
+        """The chosen values, the tuning iterations and the wall time of the first and last of them (ms)."""
+        out = dict(self.chosen)
+        out["tuning_iterations"] = self.iterations
+        if self.walls:
+            out["ms_first"] = self.walls[0]
+            out["ms_last"] = self.walls[-1]
+        return out
+
+
 class FrameBatch:
     """K frames of one scene rendered by the same launches.
 
@@ -251,26 +341,38 @@ class FrameBatch:
 
 
 def run_tuned(eng: Engine, ref: SubsetReference, target: float, max_iterations: int, tuner: PhotonBudgetTuner,
-              label: str = "", max_seconds: float = 1e9, autotune: bool = False) -> ConvergenceRun:
+              label: str = "", max_seconds: float = 1e9, autotune: bool = False,
+              online: bool = False) -> ConvergenceRun:
     """run_to_target with the tuner choosing the light-path count.  With
-    `autotune`, rc_autotune picks the launch shapes for the chosen count once
-    the pilots are done; its wall time counts in the run's time (its renders
-    go to a scratch film and add nothing to the image)."""
+    `autotune`, the launch shapes are tuned for the chosen count once the
+    pilots are done: by rc_autotune, whose wall time counts in the run's time
+    (its renders go to a scratch film and add nothing to the image), or, with
+    `online`, by an OnlineTuner on the run's next iterations."""
     run = ConvergenceRun(label)
     spent = 0.0
     tuned = False
+    live = None
     for i in range(max_iterations):
         n = tuner.schedule(i)
         if n is not None:
             eng.set("photons", int(n))
         if autotune and not tuned and tuner.fit is not None:
-            t0 = time.perf_counter()
-            run.autotune = eng.autotune(repeats=1)
-            spent += time.perf_counter() - t0
             tuned = True
+            if online:
+                live = OnlineTuner(eng)
+            else:
+                t0 = time.perf_counter()
+                run.autotune = eng.autotune(repeats=1)
+                spent += time.perf_counter() - t0
+        if live is not None:
+            live.before_iteration()
         t0 = time.perf_counter()
         eng.render_iteration()
         spent += time.perf_counter() - t0
+        if live is not None and not live.done:
+            live.after_iteration(eng.stats())
+            if live.done:
+                run.autotune = live.report()
         img = eng.image(scaled=False)
         tuner.observe(eng, i, img)
         e = ref.rmse(img, i + 1)

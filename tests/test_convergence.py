@@ -52,3 +52,79 @@ def test_reference_fixture_is_consistent():
     est = np.sqrt(np.mean(d["half_diff"].astype(np.float64) ** 2)) / 2.0 / ref.mean.mean()
     assert ref.noise == pytest.approx(est, rel=1e-3) and ref.noise < 0.004
     assert ref.mean.min() >= 0.0
+
+
+class _FakeEngine:
+    """Options in a dict; each render's stats come from a cost model of the options it ran with."""
+
+    def __init__(self, cost, radius=lambda i: 1.0):
+        self.opts = {"overlap": 1, "photons": 1 << 20}
+        self.cost, self.radius, self.i, self.log = cost, radius, 0, []
+
+    def set(self, k, v):
+        self.opts[k] = v
+
+    def get(self, k):
+        return self.opts[k]
+
+    def render(self):
+        self.log.append(dict(self.opts))
+        st = self.cost(self.opts, self.i)
+        st["radius"] = self.radius(self.i)
+        self.i += 1
+        return st
+
+
+def _run(eng, tuner, n):
+    for _ in range(n):
+        tuner.before_iteration()
+        st = eng.render()
+        tuner.after_iteration(st)
+
+
+def test_online_tuner_takes_each_knobs_cheapest_candidate_in_order():
+    best = {"path_chunk": 512, "light_blocks_per_cu": 6, "camera_blocks_per_cu": 12, "cell_scale": 1.0,
+            "light_refill": 16, "light_order": 1, "camera_refill": 8, "group_queries": 1, "gather_tile": 256,
+            "gather_block": 0, "gather_blocks_per_cu": 16, "query_order": 0}
+
+    def cost(o, i):  # each stat is smallest at its knobs' chosen values
+        def d(*ks):
+            return 1.0 + sum(abs(float(o.get(k, best[k])) - best[k]) for k in ks)
+        return {"ms_light": d("path_chunk", "light_blocks_per_cu", "light_refill", "light_order"),
+                "ms_camera_trace": d("path_chunk", "camera_blocks_per_cu", "camera_refill", "query_order"),
+                "ms_map": d("cell_scale"), "ms_query_sort": 0.1,
+                "ms_gather": d("cell_scale", "group_queries", "gather_tile", "gather_block", "gather_blocks_per_cu"),
+                "ms_total": 10.0 - 0.15 * o["overlap"]}  # overlap saves 1.5 %: below the margin
+    eng = _FakeEngine(cost)
+    t = convergence.OnlineTuner(eng)
+    n = sum(len(c) for _, c, _ in convergence.ONLINE_KNOBS)
+    _run(eng, t, n)
+    assert t.done and t.iterations == n
+    r = t.report()
+    for k, v in best.items():
+        assert r[k] == v and eng.get(k) == v, k
+    assert r["overlap"] == 0 and eng.get("overlap") == 0
+    # every kernel knob was timed with the camera trace serial; the iterations tried the candidates in order
+    assert all(o["overlap"] == 0 for o in eng.log[:n - 2])
+    assert [o["path_chunk"] for o in eng.log[:5]] == [64, 128, 256, 512, 1024]
+    # a later knob's iterations run with the earlier knobs' choices
+    assert all(o["path_chunk"] == 512 for o in eng.log[5:])
+    # once done, the tuner sets nothing
+    eng.set("gather_block", 4)
+    t.before_iteration()
+    assert eng.get("gather_block") == 4
+
+
+def test_online_tuner_keeps_overlap_past_the_margin_and_scales_merge_time_by_radius():
+    def cost(o, i):
+        # the merge's time follows r^2 (the radius shrinks every iteration); per unit of work gather_blocks 5 is best
+        r2 = (1.0 - 0.02 * i) ** 2
+        g = (1.0 if o.get("gather_blocks_per_cu", 5) == 5 else 1.01) * r2
+        return {"ms_light": 1.0, "ms_camera_trace": 1.0, "ms_map": 1.0, "ms_query_sort": 0.1, "ms_gather": g,
+                "ms_total": 10.0 - 0.3 * o["overlap"]}  # overlap saves 3 %
+    eng = _FakeEngine(cost, radius=lambda i: 1.0 - 0.02 * i)
+    t = convergence.OnlineTuner(eng)
+    _run(eng, t, sum(len(c) for _, c, _ in convergence.ONLINE_KNOBS))
+    assert t.done
+    assert t.report()["gather_blocks_per_cu"] == 5   # unscaled, the later candidates would look cheaper
+    assert t.report()["overlap"] == 1 and eng.get("overlap") == 1

@@ -8,7 +8,7 @@ order inside a merge changes with the cell size and the merge group size
 import numpy as np
 import pytest
 
-from realcausticsengine_amd import scenes
+from realcausticsengine_amd import convergence, scenes
 from realcausticsengine_amd.engine import Batch, Engine
 
 pytestmark = pytest.mark.gpu
@@ -49,6 +49,38 @@ def test_autotune_keeps_film_and_image():
     for k in CANDIDATES:
         assert eng.get(k) == r[k]
     eng.render_iteration()
+    got = eng.image(scaled=False)
+    eng.release()
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7 * want.max())
+
+
+def test_online_tuner_renders_the_same_iterations():
+    """convergence.OnlineTuner: every candidate renders a real iteration of the run, so after the tuning
+    iterations the image is the image of as many iterations with fixed launch shapes (to the merge's
+    summation order)."""
+    args = scenes.water_scene(64, 96, 64, samples=3)
+    n = sum(len(c) for _, c, _ in convergence.ONLINE_KNOBS) + 2
+    ref = Engine(args, photons=65536, seed=2, write_exr=0)
+    ref.prepare()
+    for _ in range(n):
+        ref.render_iteration()
+    want = ref.image(scaled=False)
+    ref.release()
+
+    eng = Engine(args, photons=65536, seed=2, write_exr=0)
+    eng.prepare()
+    tuner = convergence.OnlineTuner(eng)
+    for _ in range(n):
+        tuner.before_iteration()
+        eng.render_iteration()
+        if not tuner.done:
+            tuner.after_iteration(eng.stats())
+    r = tuner.report()
+    print("online", r, flush=True)
+    assert tuner.done and r["tuning_iterations"] == n - 2
+    assert eng.get("iterations") == n
+    for k, c in CANDIDATES.items():
+        assert int(r[k]) in c and eng.get(k) == r[k], k
     got = eng.image(scaled=False)
     eng.release()
     np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-7 * want.max())
