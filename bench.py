@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -32,17 +33,34 @@ PHOTON_BYTES = 40  # Path_Vertex: position, direction, throughput, dVM (Path_Ver
 PHOTON_RECORD_BYTES = 48  # the light pass's photon record: the 40 B + light path index and length
 
 
+def sort_pass_bytes(light_paths: float) -> int:
+    """Bytes per photon of the photon-map sort's passes: the key is (bucket, light path), bucket bits for
+    max(4096, min(2^24, N / 2)) buckets above ceil(log2 N) light-path bits; after each 8-bit pass the sorted
+    light-path bits are dropped, and once the rest fits 32 bits the keys are 4-B words."""
+    n = max(1, int(light_paths))
+    uid = max(0, math.ceil(math.log2(n)))
+    bucket = math.ceil(math.log2(max(4096, min(1 << 24, n // 2))))
+    bits = uid + bucket
+    total, dropped, in32 = 0, 0, False
+    for p in range(min(8, (bits + 7) // 8)):
+        nd = min(8 * (p + 1), max(uid, dropped))
+        out32 = bits - nd <= 32
+        total += (4 if in32 else 8) + 4 + (4 if out32 else 8) + 4
+        dropped, in32 = nd, out32
+    return total
+
+
 def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> dict:
     """Bytes per launch from the event counts of one counting iteration.
 
     traversal: one node per visit (128 B: the four child boxes and refs of the
     4-wide BVH; 64 B for the binary one); 36 B per triangle
     test (three points); 40 B per closest hit (three normals + material).
-    light pass: + 48 B per stored photon (its record).  photon map, per photon:
-    44 B key generation (position and path read, key and value written), 8 B
-    digit counting, 24 B per radix pass (the pair read and written), 96 B
-    permutation (record and index read, sorted photon and index written),
-    8 B cell ranges.  camera trace:
+    light pass: + 60 B per stored photon (its 48-B record, and its photon-map
+    key and value).  photon map, per photon: the radix passes (the pair read and
+    written: 8-B keys while the key's unsorted bits exceed 32, then 4-B keys;
+    sort_pass_bytes), 96 B permutation (record and index read, sorted photon and
+    index written), 8 B cell ranges.  camera trace:
     + 80 B per merge query written + 8 B per path record.  gather: 80 B per
     query read, 12 B per result, 8 B per cell lookup, and per photon: with one
     query per wave (grouped = False) 12 B per photon tested and 28 B per photon
@@ -50,7 +68,7 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
     queries sharing a window reads it once).  resolve: 8 B per path, 12 B per
     query, 24 B per film update."""
     light = (node_bytes * st["light_node_visits"] + 36 * st["light_tri_tests"] + 40 * st["light_hits"]
-             + PHOTON_RECORD_BYTES * st["photons"])
+             + (PHOTON_RECORD_BYTES + 12) * st["photons"])
     trace = (node_bytes * st["camera_node_visits"] + 36 * st["camera_tri_tests"] + 40 * st["camera_hits"]
              + 80 * st["gathers"] + 8 * st["camera_paths"])
     if grouped:
@@ -59,7 +77,7 @@ def algorithmic_bytes(st: dict, node_bytes: int = 128, grouped: bool = True) -> 
         gather = (92 * st["gathers"] + 8 * st["cell_visits"] + 12 * st["photons_tested"]
                   + 28 * st["photons_accepted"])
     resolve = 32 * st["camera_paths"] + 12 * st["gathers"]
-    photon_map = st["photons"] * (44 + 8 + 24 * st["sort_passes"] + 96 + 8)
+    photon_map = st["photons"] * (sort_pass_bytes(st["light_paths"]) + 96 + 8)
     return {"light_pass": light, "photon_map": photon_map, "camera_trace": trace, "gather": gather,
             "resolve": resolve}
 

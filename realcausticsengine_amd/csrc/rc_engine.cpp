@@ -967,63 +967,6 @@ const uint32_t* light_path_order(Engine& e, const LightPassArgs& a, hipStream_t 
     return sort_light_paths(a, b, s);
 }
 
-// Light pass into the unsorted photon buffers; returns the photon count.
-// after_launch (if set) queues other work once the first light pass is queued
-// and before the host waits for its photon count.
-bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::function<bool()>& after_launch = nullptr) {
-    long long N = light_paths(e);
-    if (N <= 0 || N > (1ll << 31)) {
-        set_error("light path count out of range (1 .. 2^31)");
-        return false;
-    }
-    uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
-    if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
-    if (!ensure_guide_buffers(e)) return false;
-    if (e.mlt) {
-        if (guide_active(e)) {
-            set_error("options mlt and guide are exclusive");
-            return false;
-        }
-        if (!e.mlt_state.alloc((size_t)kMltBlocks * e.mlt_chains) || !e.mlt_counts.alloc(4)) return false;
-    }
-    if (!e.h_count) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_count), sizeof(uint32_t), hipHostMallocDefault));
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        launch_fill_u32(e.counters.p, 0u, 4, e.stream);
-        if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
-        if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
-        LightPassArgs a = light_pass_args(e, iteration);
-        a.counters = e.count_events ? e.stats_dev.p : nullptr;
-        if (e.mlt) {
-            a.mlt_state = e.mlt_state.p;
-            // at most mlt_chains chains of at least 64 indices each (the mutation size adapts along a chain)
-            a.mlt_chains = (uint32_t)std::max<long long>(1, std::min<long long>(e.mlt_chains, N / 64));
-            a.mlt_counts = e.mlt_counts.p;
-        }
-        if (e.light_emit && !e.mlt) {
-            if (!e.emitted.alloc(3 * (size_t)N)) return false;
-            a.emitted = e.emitted.p;
-        }
-        HIP_OK(hipEventRecord(e.ev[0], e.stream));
-        a.path_order = light_path_order(e, a, e.stream);  // inside the light pass's time
-        if (e.light_order && !a.path_order && !e.mlt) return false;
-        launch_light_pass(a, light_blocks(e), e.stream);
-        HIP_OK(hipEventRecord(e.ev[1], e.stream));
-        HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(e.h_count, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
-        if (attempt == 0 && after_launch && !after_launch()) return false;
-        if (e.mlt) HIP_OK(hipMemcpyAsync(e.mlt_last, e.mlt_counts.p, sizeof e.mlt_last, hipMemcpyDeviceToHost, e.stream));
-        HIP_OK(hipStreamSynchronize(e.stream));
-        const uint32_t c = *e.h_count;
-        if (c <= e.capacity) {
-            count = c;
-            return true;
-        }
-        if (!ensure_photon_buffers(e, (uint32_t)std::min<double>((double)c * 1.25 + 1024, 4.0e9))) return false;
-    }
-    set_error("photon buffer overflow");
-    return false;
-}
-
 struct MapInfo {
     PhotonMapView pm;
 };
@@ -1077,7 +1020,87 @@ bool map_grid(Engine& e, float radius, float r2, MapGrid& g) {
     return true;
 }
 
-bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
+// Light pass into the unsorted photon buffers; returns the photon count.
+// after_launch (if set) queues other work once the first light pass is queued
+// and before the host waits for its photon count.
+// The light pass makes the photon map's keys as it writes the photons (the
+// iteration's grid is known before any photon exists): MapKeys of grid g into
+// the engine's sort buffers, laid out for the photon buffer's capacity.
+MapKeys map_keys_of(Engine& e, const MapGrid& g) {
+    MapKeys k{};
+    k.keys = e.keys_a.p;
+    k.vals = e.vals_a.p;
+    k.dig = map_sort_digits(e.hist.p, e.capacity);
+    for (int c = 0; c < 3; ++c) k.origin[c] = g.pm.origin[c];
+    k.inv_cell = g.pm.inv_cell;
+    k.mask = g.pm.mask;
+    k.uid_bits = g.uid_bits;
+    return k;
+}
+
+bool run_light_pass(Engine& e, int iteration, uint32_t& count, const std::function<bool()>& after_launch = nullptr) {
+    long long N = light_paths(e);
+    if (N <= 0 || N > (1ll << 31)) {
+        set_error("light path count out of range (1 .. 2^31)");
+        return false;
+    }
+    float radius, r2, norm;
+    iteration_params(e, iteration, radius, r2, norm);
+    MapGrid grid;
+    if (!map_grid(e, radius, r2, grid)) return false;
+    uint32_t cap = e.photon_capacity > 0 ? (uint32_t)e.photon_capacity : (uint32_t)std::max<long long>(N, 1 << 16);
+    if (!ensure_photon_buffers(e, std::max(cap, e.capacity))) return false;
+    if (!ensure_guide_buffers(e)) return false;
+    if (e.mlt) {
+        if (guide_active(e)) {
+            set_error("options mlt and guide are exclusive");
+            return false;
+        }
+        if (!e.mlt_state.alloc((size_t)kMltBlocks * e.mlt_chains) || !e.mlt_counts.alloc(4)) return false;
+    }
+    if (!e.h_count) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_count), sizeof(uint32_t), hipHostMallocDefault));
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        launch_fill_u32(e.counters.p, 0u, 4, e.stream);
+        if (e.mlt) launch_fill_u32(e.mlt_counts.p, 0u, 4, e.stream);
+        if (e.count_events) launch_fill_u32(reinterpret_cast<uint32_t*>(e.stats_dev.p), 0u, 2 * kCounterFields * 2, e.stream);
+        LightPassArgs a = light_pass_args(e, iteration);
+        a.counters = e.count_events ? e.stats_dev.p : nullptr;
+        clear_map_sort(e.hist.p, e.capacity, e.stream);
+        a.map = map_keys_of(e, grid);
+        if (e.mlt) {
+            a.mlt_state = e.mlt_state.p;
+            // at most mlt_chains chains of at least 64 indices each (the mutation size adapts along a chain)
+            a.mlt_chains = (uint32_t)std::max<long long>(1, std::min<long long>(e.mlt_chains, N / 64));
+            a.mlt_counts = e.mlt_counts.p;
+        }
+        if (e.light_emit && !e.mlt) {
+            if (!e.emitted.alloc(3 * (size_t)N)) return false;
+            a.emitted = e.emitted.p;
+        }
+        HIP_OK(hipEventRecord(e.ev[0], e.stream));
+        a.path_order = light_path_order(e, a, e.stream);  // inside the light pass's time
+        if (e.light_order && !a.path_order && !e.mlt) return false;
+        launch_light_pass(a, light_blocks(e), e.stream);
+        HIP_OK(hipEventRecord(e.ev[1], e.stream));
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(e.h_count, e.counters.p + 1, 4, hipMemcpyDeviceToHost, e.stream));
+        if (attempt == 0 && after_launch && !after_launch()) return false;
+        if (e.mlt) HIP_OK(hipMemcpyAsync(e.mlt_last, e.mlt_counts.p, sizeof e.mlt_last, hipMemcpyDeviceToHost, e.stream));
+        HIP_OK(hipStreamSynchronize(e.stream));
+        const uint32_t c = *e.h_count;
+        if (c <= e.capacity) {
+            count = c;
+            return true;
+        }
+        if (!ensure_photon_buffers(e, (uint32_t)std::min<double>((double)c * 1.25 + 1024, 4.0e9))) return false;
+    }
+    set_error("photon buffer overflow");
+    return false;
+}
+
+// keys_ready: the light pass just made the keys (run_light_pass), else the
+// photons are a caller's and the map build makes them.
+bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi, bool keys_ready = false) {
     MapGrid g;
     if (!map_grid(e, radius, r2, g)) return false;
     const int uid_bits = g.uid_bits, bucket_bits = g.bucket_bits;
@@ -1100,6 +1123,8 @@ bool build_map(Engine& e, uint32_t count, float radius, float r2, MapInfo& mi) {
     a.out_tp = e.s_tp.p;
     a.out_index = e.s_index.p;
     a.cells = e.cells.p;
+    a.keys_ready = keys_ready;
+    a.scratch_n = keys_ready ? e.capacity : 0u;
     e.sort_passes = (bucket_bits + uid_bits + 7) / 8;
     HIP_OK(hipEventRecord(e.ev[2], e.stream));
     build_photon_map(a, e.stream);
@@ -1454,6 +1479,8 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         m.out_tp = e.s_tp.p;
         m.out_index = e.s_index.p;
         m.cells = e.cells.p;
+        m.keys_ready = true;  // by the light pass (in the graph, or run_light_pass before it)
+        m.scratch_n = e.capacity;
         CameraPassArgs a{};
         a.sc = scene_view(e);
         a.pm = pm;
@@ -1493,8 +1520,13 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                    launch_copy_u32(reinterpret_cast<uint32_t*>(e.film_prev.p),
                                                    reinterpret_cast<const uint32_t*>(e.image.p), film, st);
                                    launch_fill_u32(e.counters.p, 0u, 4, st);
+                                   clear_map_sort(e.hist.p, e.capacity, st);
                                    LightPassArgs la = light_pass_args(e, iteration);
                                    la.dyn = e.d_iter.p;
+                                   MapGrid grid;
+                                   grid.pm.mask = buckets - 1;
+                                   grid.uid_bits = uid_bits;
+                                   la.map = map_keys_of(e, grid);  // origin and inv_cell from dyn
                                    if (e.light_emit) la.emitted = e.emitted.p;
                                    la.path_order = light_path_order(e, la, st);
                                    launch_light_pass(la, light_blocks(e), st);
@@ -1578,7 +1610,7 @@ bool render_iteration(Engine& e, int iteration) {
         auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
         if (!run_light_pass(e, iteration, count, overlapped ? sort_queries_next : std::function<bool()>()))
             return false;
-        if (!build_map(e, count, radius, r2, mi)) return false;
+        if (!build_map(e, count, radius, r2, mi, true)) return false;
         if (overlapped) {
             if (!camera_finish(e, iteration, mi, norm, e.image.p, ca)) return false;
         } else if (!run_camera_pass(e, iteration, mi, norm * mlt_factor(e), e.image.p)) {
@@ -1783,7 +1815,7 @@ bool timed_iteration(Engine& e, int it, float* film, double* t) {
     auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
     if (!run_light_pass(e, it, count, overlapped ? sort_queries_next : std::function<bool()>())) return false;
     MapInfo mi;
-    if (!build_map(e, count, radius, r2, mi)) return false;
+    if (!build_map(e, count, radius, r2, mi, true)) return false;
     if (overlapped) {
         if (!camera_finish(e, it, mi, norm, film, ca)) return false;
     } else if (!run_camera_pass(e, it, mi, norm * mlt_factor(e), film)) {
@@ -2263,6 +2295,8 @@ bool batch_graph_pass(Batch& b, const BatchView& view, CameraPassArgs c, uint32_
         m.vals_b = w.vals_b.p;
         m.hist = w.hist.p;
         m.hist_sums = w.hist_sums.p;
+        m.keys_ready = true;  // by the batch's light pass
+        m.scratch_n = w.capacity;
         m.out_pos = w.s_pos.p;
         m.out_dir = w.s_dir.p;
         m.out_tp = w.s_tp.p;
@@ -2474,6 +2508,12 @@ bool batch_iteration(Batch& b, bool scratch = false) {
             if (!w.emitted.alloc(3 * (size_t)light)) return false;
             a.emitted = w.emitted.p;
         }
+        // the photon map's keys with the photons (grid per scene from the views uploaded above)
+        clear_map_sort(w.hist.p, w.capacity, st);
+        a.map.keys = w.keys_a.p;
+        a.map.vals = w.vals_a.p;
+        a.map.dig = map_sort_digits(w.hist.p, w.capacity);
+        a.map.uid_bits = uid_bits;
         HIP_OK(hipEventRecord(w.ev[0], st));
         a.path_order = light_path_order(w, a, st);
         if (w.light_order && !a.path_order) return false;
@@ -2522,6 +2562,8 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         m.out_tp = w.s_tp.p;
         m.out_index = w.s_index.p;
         m.cells = w.cells.p;
+        m.keys_ready = true;  // by the light pass
+        m.scratch_n = w.capacity;
         HIP_OK(hipEventRecord(w.ev[2], st));
         build_photon_map_batch(m, view, st);
         HIP_OK(hipEventRecord(w.ev[3], st));

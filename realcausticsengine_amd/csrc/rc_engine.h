@@ -149,6 +149,22 @@ constexpr int kPhotonRecord = 3;
 
 struct IterParams;
 
+// Photon-map keys made by the light pass as it writes its photons: the key
+// (bucket, light path) of the photon in record slot i goes to keys[i], i to
+// vals[i], and the count of the key's low 8-bit digit (the sort's first pass)
+// into dig (8 copies of 256 counts, by workgroup index modulo 8).  The grid
+// is origin / inv_cell / mask, or the light pass's dyn values, or per scene
+// the batch's photon-map views (bucket = the scene's bucket_base + its own).
+struct MapKeys {
+    unsigned long long* keys;  // null: the light pass makes no keys
+    uint32_t* vals;
+    uint32_t* dig;
+    float origin[3];
+    float inv_cell;
+    uint32_t mask;
+    int uid_bits;
+};
+
 struct LightPassArgs {
     SceneView sc;
     uint32_t k0, k1, iteration, stream;
@@ -172,6 +188,7 @@ struct LightPassArgs {
     uint4* mlt_state;     // kMltBlocks * mlt_chains explicit chain states, block-major
     uint32_t mlt_chains;  // Markov chains; chain c takes the path indices c, c + chains, ...
     uint32_t* mlt_counts; // [0] visible uniform paths, [1] mutations, [2] accepted mutations
+    MapKeys map;          // photon-map keys written with the photons (map.keys null: none)
 };
 // Philox blocks of a light path's primary sample (light sample 0-1, bounces 2-9).
 constexpr int kMltBlocks = 10;
@@ -294,8 +311,16 @@ struct MapBuildArgs {
     float2* out_tp;
     uint32_t* out_index;  // original index of each sorted photon
     uint2* cells;         // mask + 1 entries
+    // true: the light pass made the keys (keys_a, vals_a) and their first digit counts, into the
+    // sort scratch laid out for scratch_n keys and cleared before it (clear_map_sort)
+    bool keys_ready;
+    uint32_t scratch_n;  // keys the sort scratch is laid out for (0: count)
 };
 void build_photon_map(const MapBuildArgs& a, hipStream_t s);
+// The sort scratch (hist) laid out for n keys: zeroes its digit counts, tickets and first status
+// rows before a light pass makes keys into it; map_sort_digits: where those counts go.
+void clear_map_sort(uint32_t* hist, uint32_t n, hipStream_t s);
+uint32_t* map_sort_digits(uint32_t* hist, uint32_t n);
 // Batch: the keys of every scene's photons in one sort (bucket = the scene's
 // bucket_base + its own cell bucket, above the global light path index).
 void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream_t s);

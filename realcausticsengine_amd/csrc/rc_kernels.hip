@@ -66,6 +66,36 @@ struct WavePool {
     }
 };
 
+// Copies of the sort's digit counts: a count is added to copy (workgroup index
+// modulo kDigitCopies), so that no address takes more than a few hundred atomics.
+constexpr int kDigitCopies = 8;
+
+// The photon-map key (bucket, light path) of a photon at `p` (as k_map_keys).
+__device__ __forceinline__ unsigned long long light_photon_key(const LightPassArgs& a, float4 p, uint32_t path) {
+    float ox = a.map.origin[0], oy = a.map.origin[1], oz = a.map.origin[2], inv_cell = a.map.inv_cell;
+    uint32_t mask = a.map.mask, base = 0;
+    if (a.batch.n > 0) {
+        const int sc = batch_scene_of(a.batch.light_starts, a.batch.n, path);
+        const PhotonMapView& pm = a.batch.pm[sc];
+        ox = pm.origin[0];
+        oy = pm.origin[1];
+        oz = pm.origin[2];
+        inv_cell = pm.inv_cell;
+        mask = pm.mask;
+        base = a.batch.bs[sc].bucket_base;
+    } else if (a.dyn) {
+        ox = a.dyn->origin[0];
+        oy = a.dyn->origin[1];
+        oz = a.dyn->origin[2];
+        inv_cell = a.dyn->inv_cell;
+    }
+    int cx = (int)floorf((p.x - ox) * inv_cell);
+    int cy = (int)floorf((p.y - oy) * inv_cell);
+    int cz = (int)floorf((p.z - oz) * inv_cell);
+    const uint32_t bkt = base + cell_bucket(cx, cy, cz, mask);
+    return ((unsigned long long)bkt << a.map.uid_bits) | (unsigned long long)path;
+}
+
 // Photons of a wave are gathered in an LDS ring and written out 64 at a time:
 // one atomic on the shared photon counter per 64 photons instead of one per
 // bounce step (the counter is the light pass's most contended address), and
@@ -93,9 +123,16 @@ struct PhotonQueue {
             if (at < a.capacity) {
                 float4* o = a.out_rec + kPhotonRecord * (size_t)at;
                 const float2 tp = r->tp[e];
-                o[0] = r->pos[e];
+                const float4 pos = r->pos[e];
+                o[0] = pos;
                 o[1] = r->dir[e];
                 o[2] = make_float4(tp.x, tp.y, __uint_as_float(r->path[e]), __uint_as_float((uint32_t)r->len[e]));
+                if (a.map.keys) {  // the photon map's key and its first digit's count
+                    const unsigned long long k = light_photon_key(a, pos, r->path[e]);
+                    a.map.keys[at] = k;
+                    a.map.vals[at] = at;
+                    atomicAdd(&a.map.dig[(blockIdx.x % kDigitCopies) * 256 + ((uint32_t)k & 255u)], 1u);
+                }
             }
         }
         head = (head + n) & (kPhotonRing - 1u);
@@ -1713,7 +1750,7 @@ constexpr int kOsItems = 16;
 constexpr uint32_t kOsChunk = 64 * kOsItems;  // keys per wave, in index order
 constexpr uint32_t kOsMinTile = 256 * kOsItems;
 constexpr int kOsMaxPasses = 8;
-constexpr int kOsCopies = 8;
+constexpr int kOsCopies = kDigitCopies;
 
 // Tiles of the smallest tile size (the scratch holds that many status rows).
 __host__ __device__ constexpr size_t onesweep_tiles(size_t n) { return n == 0 ? 1 : (n + kOsMinTile - 1) / kOsMinTile; }
@@ -1754,6 +1791,8 @@ static void onesweep_clear(const OnesweepScratch& o, hipStream_t s) {
     launch_fill_u32(o.dig, 0u, kOsMaxPasses * kOsCopies * 256 + 16, s);
     launch_fill_u32(reinterpret_cast<uint32_t*>(o.status[0]), 0u, 2 * 256 * o.tiles, s);
 }
+void clear_map_sort(uint32_t* hist, uint32_t n, hipStream_t s) { onesweep_clear(onesweep_scratch(hist, n), s); }
+uint32_t* map_sort_digits(uint32_t* hist, uint32_t n) { return onesweep_scratch(hist, n).dig; }
 
 // Keys of the photon map, (bucket, light path), and the counts of their low
 // digits (the sort's first pass): 16 photons per thread, all record loads
@@ -2110,11 +2149,13 @@ void build_photon_map(const MapBuildArgs& a, hipStream_t s) {
     launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
     uint32_t n = a.count;
     if (n == 0) return;
-    const OnesweepScratch o = onesweep_scratch(a.hist, n);
-    onesweep_clear(o, s);
-    hipLaunchKernelGGL(k_map_keys, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev, n, a.dyn,
-                       a.origin[0], a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a, a.vals_a,
-                       o.dig);
+    const OnesweepScratch o = onesweep_scratch(a.hist, a.scratch_n ? a.scratch_n : n);
+    if (!a.keys_ready) {
+        onesweep_clear(o, s);
+        hipLaunchKernelGGL(k_map_keys, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev, n,
+                           a.dyn, a.origin[0], a.origin[1], a.origin[2], a.inv_cell, a.mask, a.uid_bits, a.keys_a,
+                           a.vals_a, o.dig);
+    }
     sort_permute_cells(a, o, s);
 }
 
@@ -2123,10 +2164,12 @@ void build_photon_map_batch(const MapBuildArgs& a, const BatchView& b, hipStream
     launch_fill_u32(reinterpret_cast<uint32_t*>(a.cells), 0u, 2 * ((size_t)a.mask + 1), s);
     uint32_t n = a.count;
     if (n == 0) return;
-    const OnesweepScratch o = onesweep_scratch(a.hist, n);
-    onesweep_clear(o, s);
-    hipLaunchKernelGGL(k_map_keys_batch, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev, n,
-                       b, a.uid_bits, a.keys_a, a.vals_a, o.dig);
+    const OnesweepScratch o = onesweep_scratch(a.hist, a.scratch_n ? a.scratch_n : n);
+    if (!a.keys_ready) {
+        onesweep_clear(o, s);
+        hipLaunchKernelGGL(k_map_keys_batch, dim3(map_key_blocks(n)), dim3(kKeyThreads), 0, s, a.in_rec, a.count_dev,
+                           n, b, a.uid_bits, a.keys_a, a.vals_a, o.dig);
+    }
     sort_permute_cells(a, o, s);
 }
 
