@@ -273,7 +273,7 @@ long long rc_merge_sets(int iteration, const float *photons, long long count, fl
  * HIP graph capture set-up, tight serving loops, harnesses that forbid
  * hipMalloc).  positions, queries, offsets (nq + 1 int64) and indices (cap)
  * are DEVICE pointers; the photon map and the per-query counts use a fixed
- * scratch array of about 32 MB that the library holds on each GPU until the
+ * scratch array of about 36 MB that the library holds on each GPU until the
  * process ends, so count and nq are at most 262144 each (larger sizes are an
  * error).  The scene must be prepared (rc_prepare) first; the grid is the one
  * rc_gather_sets builds, so the sets are the same.  Returns the total set

@@ -142,10 +142,12 @@ __device__ __forceinline__ int batch_scene_of(const uint32_t* starts, int n, uin
 // An unsorted photon as the light pass writes it: kPhotonRecord float4,
 //   {position, dVM}  {direction towards the light, throughput.r}
 //   {throughput.g, throughput.b, light path index (bits), path length (bits)}
-// one record per photon, so that the map's permutation gathers one record
-// (one or two cache lines) per photon; the light path index is the map's sort
+//   (a fourth, unused: records start on 64-B boundaries)
+// one record per photon, so that the map's permutation gathers one record per
+// photon, and an aligned 64-B record lies in one 128-B cache line (a 48-B one
+// spans two lines for 3 records in 8); the light path index is the map's sort
 // key below the bucket.
-constexpr int kPhotonRecord = 3;
+constexpr int kPhotonRecord = 4;
 
 struct IterParams;
 
