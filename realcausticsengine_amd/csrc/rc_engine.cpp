@@ -159,6 +159,7 @@ struct Engine {
     hipEvent_t ev_join = nullptr;   // the query sort's start
     hipEvent_t ev_qdone = nullptr;  // the query sort's end (stream waits on it before the merge)
     hipEvent_t ev_fork = nullptr;  // stream2 starts after the work already queued on stream
+    hipEvent_t ev_gjoin = nullptr;  // inside a captured iteration: stream's merge waits for stream2's query sort
     DevBuf<uint32_t> cam_counters;
     uint32_t* h_nq = nullptr;  // pinned: the overlapped trace's query count
     uint32_t* h_count = nullptr;  // pinned: the light pass's photon count
@@ -636,6 +637,7 @@ bool ensure_stream2(Engine& e) {
     if (!e.ev_join) HIP_OK(hipEventCreate(&e.ev_join));
     if (!e.ev_qdone) HIP_OK(hipEventCreate(&e.ev_qdone));
     if (!e.ev_fork) HIP_OK(hipEventCreateWithFlags(&e.ev_fork, hipEventDisableTiming));
+    if (!e.ev_gjoin) HIP_OK(hipEventCreateWithFlags(&e.ev_gjoin, hipEventDisableTiming));
     if (!e.h_nq) HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&e.h_nq), sizeof(uint32_t), hipHostMallocDefault));
     if (!e.cam_counters.alloc(kPathCounters)) return false;
     HIP_OK(hipStreamCreate(&e.stream2));
@@ -1414,9 +1416,10 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         return true;
     };
     const bool overlapped = e.overlap && !e.mlt;
-    const bool whole = !overlapped && !e.mlt && !guide_active(e);
+    const bool whole = !e.mlt && !guide_active(e);
     CameraPassArgs ca{};
-    if (overlapped && !(copy_iter() && camera_trace_async(e, iteration, norm, e.image.p, ca))) return false;
+    if (overlapped && !whole && !(copy_iter() && camera_trace_async(e, iteration, norm, e.image.p, ca))) return false;
+    if (overlapped && whole && !ensure_stream2(e)) return false;
     const size_t film = (size_t)e.W * e.H * 3;
     if (whole) {  // the buffers the captured light pass writes, sized as run_light_pass sizes them
         if (N <= 0 || N > (1ll << 31)) {
@@ -1511,7 +1514,8 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
             merge();
         };
         bool ok;
-        if (overlapped) {
+        bool forked = true;  // the stream2 branch of a captured overlapped iteration was joined
+        if (overlapped && !whole) {
             ok = capture_graph(st, [&]() { build_photon_map(m, st); }, e.graph_exec) &&
                  capture_graph(st, sort, e.graph_exec3) && capture_graph(st, merge, e.graph_exec2);
         } else if (whole) {
@@ -1519,6 +1523,18 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                [&]() {
                                    launch_copy_u32(reinterpret_cast<uint32_t*>(e.film_prev.p),
                                                    reinterpret_cast<const uint32_t*>(e.image.p), film, st);
+                                   if (overlapped) {
+                                       // the camera trace and the query sort on stream2, beside the light pass
+                                       // and the photon map (a fork and a join inside the one graph)
+                                       forked = hipEventRecord(e.ev_fork, st) == hipSuccess &&
+                                                hipStreamWaitEvent(e.stream2, e.ev_fork, 0) == hipSuccess;
+                                       if (forked) {
+                                           launch_fill_u32(e.cam_counters.p, 0u, kPathCounters, e.stream2);
+                                           launch_camera_trace(a, camera_blocks(e), e.stream2);
+                                           a.q_order = query_order(e, a, e.q_capacity, e.stream2);
+                                           forked = hipEventRecord(e.ev_gjoin, e.stream2) == hipSuccess;
+                                       }
+                                   }
                                    launch_fill_u32(e.counters.p, 0u, 4, st);
                                    clear_map_sort(e.hist.p, e.capacity, st);
                                    LightPassArgs la = light_pass_args(e, iteration);
@@ -1532,11 +1548,17 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                    launch_light_pass(la, light_blocks(e), st);
                                    launch_copy_u32(e.graph_count.p, e.counters.p + 1, 1, st);
                                    build_photon_map(m, st);
-                                   launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
-                                   launch_camera_trace(a, camera_blocks(e), st);
-                                   camera();
+                                   if (overlapped) {
+                                       if (forked) forked = hipStreamWaitEvent(st, e.ev_gjoin, 0) == hipSuccess;
+                                       merge();
+                                   } else {
+                                       launch_fill_u32(e.counters.p, 0u, kPathCounters, st);
+                                       launch_camera_trace(a, camera_blocks(e), st);
+                                       camera();
+                                   }
                                },
-                               e.graph_exec);
+                               e.graph_exec) &&
+                 forked;
         } else {
             ok = capture_graph(st,
                                [&]() {
@@ -1556,7 +1578,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
         e.graph_key = key;
         e.graph_captures++;
     }
-    if (overlapped) {  // the query sort behind the trace
+    if (overlapped && !whole) {  // the query sort behind the trace
         HIP_OK(hipGraphLaunch(e.graph_exec3, e.stream2));
         HIP_OK(hipEventRecord(e.ev_qdone, e.stream2));
     }
@@ -1566,7 +1588,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
     }
     HIP_OK(hipEventRecord(e.ev[2], st));
     HIP_OK(hipGraphLaunch(e.graph_exec, st));
-    if (overlapped) {
+    if (overlapped && !whole) {
         HIP_OK(hipStreamWaitEvent(st, e.ev_qdone, 0));
         HIP_OK(hipGraphLaunch(e.graph_exec2, st));
     }
@@ -2012,6 +2034,7 @@ void release(Engine*& e) {
     if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     if (e->ev_qdone) (void)hipEventDestroy(e->ev_qdone);
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_gjoin) (void)hipEventDestroy(e->ev_gjoin);
     if (e->stream2) (void)hipStreamDestroy(e->stream2);
     delete e;
     e = nullptr;
