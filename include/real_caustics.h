@@ -217,13 +217,14 @@ enum {
  * gather_block, gather_blocks_per_cu (after the merge's shape, which it
  * depends on), query_order, overlap. */
 enum { RC_TUNE_REPORT = 15 };
-/* Option "graph" = 1: with the plain light pass and the camera trace serial
- * the whole iteration (light pass, photon map, camera pass, resolve) replays
- * one captured hipGraph and the host waits once, after it: photons beyond
- * the buffer restore the film from a copy taken in the graph, grow the
- * buffers and run the iteration again.  Otherwise (MLT, a learned proposal,
- * option overlap) the light pass is a direct launch whose photon count is
- * checked on the host, then the photon-map build and the camera pass replay.
+/* Option "graph" = 1: with the plain light pass the whole iteration (light
+ * pass, photon map, camera pass, resolve; with option overlap the camera
+ * trace and the query sort as a branch on a second stream inside the graph)
+ * replays one captured hipGraph and the host waits once, after it: photons
+ * beyond the buffer restore the film from a copy taken in the graph, grow
+ * the buffers and run the iteration again.  Otherwise (MLT, a learned
+ * proposal) the light pass is a direct launch whose photon count is checked
+ * on the host, then the photon-map build and the camera pass replay.
  * Graphs read counts and per-iteration values from device memory and are
  * captured again when buffers grow or the light-path count changes
  * (rc_get_option("graph_captures")).  The film is identical to the one
