@@ -202,9 +202,12 @@ enum {
  * "gather_block" (0, 1, 2 or 4 photon steps per block of the merge),
  * "overlap" (1, the default: the camera trace runs on a second stream beside
  * the light pass and the photon-map build, which it does not read; the merge
- * waits for both.  Direct launches and graphs of one scene without "mlt",
- * and batches (scene 0's option); not with "count_events"; the film is the
- * same).
+ * waits for both.  2: the same, with the trace queued after the light pass,
+ * so that it fills what the light pass's tail and the photon-map build leave
+ * free instead of taking compute units from the light pass first (a single
+ * scene's direct launches; graphs and batches take it as 1).  Direct launches
+ * and graphs of one scene without "mlt", and batches (scene 0's option); not
+ * with "count_events"; the film is the same).
  * rc_autotune chooses them for the current scene: every candidate renders the
  * scene's next iteration into a scratch film (the film and the iteration count
  * are unchanged); each knob keeps its fastest value (best of `repeats`) for the

@@ -1183,14 +1183,19 @@ CameraPassArgs camera_args(Engine& e, int iteration, float norm, float* image) {
 }
 
 // Overlapped camera pass, part 1: the camera trace on stream2 with its own
-// path counters, its query count copied to pinned memory.
-bool camera_trace_async(Engine& e, int iteration, float norm, float* image, CameraPassArgs& a) {
+// path counters, its query count copied to pinned memory.  It starts after
+// the work queued on stream when ev_fork was recorded: here, or (record_fork
+// false) by the caller before it queued the light pass (overlap = 2: the
+// light pass reaches the GPU first and the trace fills the compute units its
+// tail and the photon-map build leave free).
+bool camera_trace_async(Engine& e, int iteration, float norm, float* image, CameraPassArgs& a,
+                        bool record_fork = true) {
     uint32_t n_paths = (uint32_t)((long long)e.W * e.H);
     uint32_t cap = std::max(e.q_capacity, std::max<uint32_t>(2 * n_paths, 1024));
     if (!ensure_query_buffers(e, cap, n_paths) || !ensure_stream2(e)) return false;
     a = camera_args(e, iteration, norm, image);
     a.path_counter = e.cam_counters.p;
-    HIP_OK(hipEventRecord(e.ev_fork, e.stream));
+    if (record_fork) HIP_OK(hipEventRecord(e.ev_fork, e.stream));
     HIP_OK(hipStreamWaitEvent(e.stream2, e.ev_fork, 0));
     launch_fill_u32(e.cam_counters.p, 0u, kPathCounters, e.stream2);
     HIP_OK(hipEventRecord(e.ev[4], e.stream2));
@@ -1628,8 +1633,16 @@ bool render_iteration(Engine& e, int iteration) {
         // (plain estimator), so it can run beside them
         overlapped = e.overlap && !e.count_events && !e.mlt;
         CameraPassArgs ca{};
-        if (overlapped && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
-        auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
+        const bool light_first = overlapped && e.overlap == 2;
+        if (light_first) {  // the trace's fork point, before the light pass is queued
+            if (!ensure_stream2(e)) return false;
+            HIP_OK(hipEventRecord(e.ev_fork, e.stream));
+        }
+        if (overlapped && !light_first && !camera_trace_async(e, iteration, norm, e.image.p, ca)) return false;
+        auto sort_queries_next = [&]() {
+            if (light_first && !camera_trace_async(e, iteration, norm, e.image.p, ca, false)) return false;
+            return queue_query_sort(e, ca, radius, r2);
+        };
         if (!run_light_pass(e, iteration, count, overlapped ? sort_queries_next : std::function<bool()>()))
             return false;
         if (!build_map(e, count, radius, r2, mi, true)) return false;
@@ -1736,7 +1749,7 @@ struct TuneKnob {
     double margin = 0.0;  // a later candidate must beat the best so far by this fraction
 };
 
-// overlap_candidates: {0, 1}, tried after the other knobs (timed with the camera trace serial).
+// overlap_candidates: {0, 1} or {0, 1, 2}, tried after the other knobs (timed with the camera trace serial).
 std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::function<void(double)> set_cell_scale,
                                  std::function<double()> get_cell_scale, std::vector<double> overlap_candidates) {
     auto L = [](const double* t) { return t[T_LIGHT]; };
@@ -1832,9 +1845,17 @@ bool timed_iteration(Engine& e, int it, float* film, double* t) {
     auto w0 = std::chrono::steady_clock::now();
     const bool overlapped = e.overlap && !e.mlt;
     CameraPassArgs ca{};
-    if (overlapped && !camera_trace_async(e, it, norm, film, ca)) return false;
+    const bool light_first = overlapped && e.overlap == 2;
+    if (light_first) {
+        if (!ensure_stream2(e)) return false;
+        HIP_OK(hipEventRecord(e.ev_fork, e.stream));
+    }
+    if (overlapped && !light_first && !camera_trace_async(e, it, norm, film, ca)) return false;
     uint32_t count = 0;
-    auto sort_queries_next = [&]() { return queue_query_sort(e, ca, radius, r2); };
+    auto sort_queries_next = [&]() {
+        if (light_first && !camera_trace_async(e, it, norm, film, ca, false)) return false;
+        return queue_query_sort(e, ca, radius, r2);
+    };
     if (!run_light_pass(e, it, count, overlapped ? sort_queries_next : std::function<bool()>())) return false;
     MapInfo mi;
     if (!build_map(e, count, radius, r2, mi, true)) return false;
@@ -1869,7 +1890,7 @@ bool autotune(Engine& e, int repeats, double* report) {
     e.overlap = 0;  // the other knobs are timed with the camera trace serial
     Engine* pe = &e;
     auto knobs = tune_knobs([pe]() -> Engine& { return *pe; }, [pe](double v) { pe->cell_scale = v; },
-                            [pe]() { return pe->cell_scale; }, {0, 1});
+                            [pe]() { return pe->cell_scale; }, {0, 1, 2});
     bool ok = run_autotune(knobs, [&](double* t) { return timed_iteration(e, it, film.p, t); }, repeats, report);
     e.count_events = counting;
     if (!ok) e.overlap = overlap;
@@ -3103,7 +3124,7 @@ __attribute__((visibility("default"))) int rc_set_option(const char* key, double
     else if (k == "mlt") g->mlt = (int)v ? 1 : 0;
     else if (k == "mlt_chains") g->mlt_chains = (uint32_t)std::max(1.0, std::min(v, 16777216.0));
     else if (k == "group_queries") g->group_queries = (int)v >= 12 ? 16 : (int)v >= 6 ? 8 : (int)v >= 3 ? 4 : (int)v > 0 ? 1 : 0;
-    else if (k == "overlap") g->overlap = (int)v ? 1 : 0;
+    else if (k == "overlap") g->overlap = (int)v >= 2 ? 2 : (int)v ? 1 : 0;
     else if (k == "gather_block") g->gather_block = (int)v >= 4 ? 4 : (int)v >= 2 ? 2 : (int)v >= 1 ? 1 : 0;
     else if (k == "gather_tile") g->gather_tile = (uint32_t)std::max(0.0, std::min(v, 1048576.0));
     else if (k == "light_blocks_per_cu") g->light_blocks_per_cu = std::max(1, std::min(64, (int)v));

@@ -18,7 +18,7 @@ CANDIDATES = {"path_chunk": {64, 128, 256, 512, 1024}, "light_blocks_per_cu": {4
               "light_refill": {1, 8, 16, 32}, "light_order": {0, 1}, "camera_refill": {1, 8, 16, 32},
               "query_order": {0, 1},
               "group_queries": {1, 4, 8, 16}, "gather_tile": {0, 256, 1024},
-              "gather_block": {0, 1, 2, 4}, "overlap": {0, 1}}
+              "gather_block": {0, 1, 2, 4}, "overlap": {0, 1, 2}}
 
 
 def _check_report(r):

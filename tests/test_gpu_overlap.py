@@ -49,9 +49,10 @@ def _render(overlap, iters=3):
     return films, gathers, paths
 
 
-def test_overlapped_trace_retraces_on_query_overflow():
+@pytest.mark.parametrize("overlap", [1, 2])
+def test_overlapped_trace_retraces_on_query_overflow(overlap):
     serial, g0, paths = _render(0)
-    overlapped, g1, _ = _render(1)
+    overlapped, g1, _ = _render(overlap)
     assert g0[0] > 2 * paths, (g0[0], paths)  # the first trace overflowed the first buffer
     assert g0 == g1
     for a, b in zip(serial, overlapped):

@@ -93,14 +93,15 @@ def test_rolled_query_loop_keeps_the_film(exact):
                 assert np.array_equal(a, b), (group, block)
 
 
-@pytest.mark.parametrize("order,graph", [(0, 0), (1, 0), (1, 1)])
-def test_overlapped_camera_trace_keeps_the_film(order, graph):
+@pytest.mark.parametrize("order,graph,overlap", [(0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 0, 2), (1, 0, 2)])
+def test_overlapped_camera_trace_keeps_the_film(order, graph, overlap):
     """The camera trace on a second stream, beside the light pass and the
-    photon map (option overlap = 1, the default), traces the same queries from
-    the same keys as the serial trace: the same film and query count, with
-    direct launches and with the graph replay after the trace."""
+    photon map (option overlap = 1, the default; 2: queued after the light
+    pass), traces the same queries from the same keys as the serial trace:
+    the same film and query count, with direct launches and with the graph
+    replay."""
     serial, s0 = _films(order, group=8, overlap=0, iters=3)
-    overlapped, s1 = _films(order, group=8, overlap=1, iters=3, graph=graph)
+    overlapped, s1 = _films(order, group=8, overlap=overlap, iters=3, graph=graph)
     for a, b in zip(serial, overlapped):
         assert a.sum() > 0
         assert np.array_equal(a, b)
