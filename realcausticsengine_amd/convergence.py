@@ -203,7 +203,7 @@ ONLINE_KNOBS = (
     ("gather_block", (2, 4, 1, 0), ("ms_gather",)),
     ("gather_blocks_per_cu", (5, 8, 12, 16, 24), ("ms_gather",)),
     ("query_order", (1, 0), ("ms_camera_trace", "ms_query_sort", "ms_gather")),
-    ("overlap", (0, 1, 2), ("ms_total",)),
+    ("overlap", (0, 2, 1), ("ms_total",)),
 )
 OVERLAP_MARGIN = 0.02  # overlap is kept only when it saves this fraction of the iteration's wall time
 

@@ -1749,7 +1749,9 @@ struct TuneKnob {
     double margin = 0.0;  // a later candidate must beat the best so far by this fraction
 };
 
-// overlap_candidates: {0, 1} or {0, 1, 2}, tried after the other knobs (timed with the camera trace serial).
+// overlap_candidates: {0, 1} or {0, 2, 1}, tried after the other knobs (timed with the camera trace serial);
+// a later candidate must beat the best so far by 2 %, so the order breaks near-ties (2 before 1: the light
+// pass first wins by more when the light pass is long).
 std::vector<TuneKnob> tune_knobs(std::function<Engine&()> launch_owner, std::function<void(double)> set_cell_scale,
                                  std::function<double()> get_cell_scale, std::vector<double> overlap_candidates) {
     auto L = [](const double* t) { return t[T_LIGHT]; };
@@ -1890,7 +1892,7 @@ bool autotune(Engine& e, int repeats, double* report) {
     e.overlap = 0;  // the other knobs are timed with the camera trace serial
     Engine* pe = &e;
     auto knobs = tune_knobs([pe]() -> Engine& { return *pe; }, [pe](double v) { pe->cell_scale = v; },
-                            [pe]() { return pe->cell_scale; }, {0, 1, 2});
+                            [pe]() { return pe->cell_scale; }, {0, 2, 1});
     bool ok = run_autotune(knobs, [&](double* t) { return timed_iteration(e, it, film.p, t); }, repeats, report);
     e.count_events = counting;
     if (!ok) e.overlap = overlap;

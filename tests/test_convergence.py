@@ -127,4 +127,5 @@ def test_online_tuner_keeps_overlap_past_the_margin_and_scales_merge_time_by_rad
     _run(eng, t, sum(len(c) for _, c, _ in convergence.ONLINE_KNOBS))
     assert t.done
     assert t.report()["gather_blocks_per_cu"] == 5   # unscaled, the later candidates would look cheaper
-    assert t.report()["overlap"] == 1 and eng.get("overlap") == 1
+    # both overlaps save 3 %: 2, tried before 1, is kept (1 does not beat it by the margin)
+    assert t.report()["overlap"] == 2 and eng.get("overlap") == 2
