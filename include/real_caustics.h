@@ -204,8 +204,8 @@ enum {
  * the light pass and the photon-map build, which it does not read; the merge
  * waits for both.  2: the same, with the trace queued after the light pass,
  * so that it fills what the light pass's tail and the photon-map build leave
- * free instead of taking compute units from the light pass first (a single
- * scene's direct launches; graphs and batches take it as 1).  Direct launches
+ * free instead of taking compute units from the light pass first (direct
+ * launches of a scene or a batch; graphs take it as 1).  Direct launches
  * and graphs of one scene without "mlt", and batches (scene 0's option); not
  * with "count_events"; the film is the same).
  * rc_autotune chooses them for the current scene: every candidate renders the

@@ -2512,12 +2512,10 @@ bool batch_iteration(Batch& b, bool scratch = false) {
     c.camera_qmc = e0.camera_qmc;
     c.batch = view;
     c.counters = counting ? w.stats_dev.p + 1 : nullptr;
-    if (overlapped) {
-        if (!ensure_stream2(w)) return false;
+    // overlap 2: the trace is queued after the light pass (forked before it), as for one scene
+    const bool light_first = overlapped && w.overlap == 2;
+    auto queue_trace = [&]() -> bool {
         hipStream_t s2 = w.stream2;
-        c.path_counter = w.cam_counters.p;
-        c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
-        HIP_OK(hipEventRecord(w.ev_fork, st));
         HIP_OK(hipStreamWaitEvent(s2, w.ev_fork, 0));
         launch_fill_u32(w.cam_counters.p, 0u, kPathCounters, s2);
         HIP_OK(hipEventRecord(w.ev[4], s2));
@@ -2529,6 +2527,14 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         HIP_OK(hipEventRecord(w.ev_join, s2));
         c.q_order = query_order(w, c, w.q_capacity, s2);  // count read from the device
         HIP_OK(hipEventRecord(w.ev_qdone, s2));
+        return true;
+    };
+    if (overlapped) {
+        if (!ensure_stream2(w)) return false;
+        c.path_counter = w.cam_counters.p;
+        c.q = QueryBuffers{w.q_pos.p, w.q_dir.p, w.q_sn.p, w.q_gn.p, w.q_tpn.p, w.q_capacity};
+        HIP_OK(hipEventRecord(w.ev_fork, st));
+        if (!light_first && !queue_trace()) return false;
     }
 
     // light pass (grow and repeat on overflow)
@@ -2564,6 +2570,7 @@ bool batch_iteration(Batch& b, bool scratch = false) {
         a.path_order = light_path_order(w, a, st);
         if (w.light_order && !a.path_order) return false;
         launch_light_pass(a, light_blocks(w), st);
+        if (light_first && attempt == 0 && !queue_trace()) return false;
         HIP_OK(hipEventRecord(w.ev[1], st));
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(&count, w.counters.p + 1, 4, hipMemcpyDeviceToHost, st));
@@ -2724,7 +2731,7 @@ bool batch_autotune(Batch& b, int repeats, double* report) {
                             [pb](double v) {
                                 for (Engine* e : pb->scenes) e->cell_scale = v;
                             },
-                            [pb]() { return pb->scenes[0]->cell_scale; }, {0, 1});
+                            [pb]() { return pb->scenes[0]->cell_scale; }, {0, 2, 1});
     const int overlap = b.work.overlap;
     b.work.overlap = 0;  // the other knobs are timed with the camera trace serial
     const int graph = b.scenes[0]->graph;

@@ -83,9 +83,10 @@ def _render_batch(overlap, iters=3):
     return films, gathers, paths
 
 
-def test_overlapped_batch_trace_retraces_on_query_overflow():
+@pytest.mark.parametrize("overlap", [1, 2])
+def test_overlapped_batch_trace_retraces_on_query_overflow(overlap):
     serial, g0, paths = _render_batch(0)
-    overlapped, g1, _ = _render_batch(1)
+    overlapped, g1, _ = _render_batch(overlap)
     assert g0[0] > 2 * paths, (g0[0], paths)  # the batch's first trace overflowed its first buffer
     assert g0 == g1
     for fs, fo in zip(serial, overlapped):
