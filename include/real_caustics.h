@@ -205,7 +205,8 @@ enum {
  * waits for both.  2: the same, with the trace queued after the light pass,
  * so that it fills what the light pass's tail and the photon-map build leave
  * free instead of taking compute units from the light pass first (direct
- * launches of a scene or a batch; graphs take it as 1).  Direct launches
+ * launches of a scene or a batch, and the whole-iteration graph, whose
+ * trace branch is then captured after the light pass).  Direct launches
  * and graphs of one scene without "mlt", and batches (scene 0's option); not
  * with "count_events"; the film is the same).
  * rc_autotune chooses them for the current scene: every candidate renders the

@@ -1459,7 +1459,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                   (uintptr_t)gather_blocks(e), (uintptr_t)camera_blocks(e),
                                   (uintptr_t)e.exact_merge, (uintptr_t)e.query_order, (uintptr_t)e.group_queries,
                                   (uintptr_t)e.camera_refill, (uintptr_t)e.camera_qmc, (uintptr_t)e.gather_tile,
-                                  (uintptr_t)e.gather_block, (uintptr_t)overlapped, (uintptr_t)whole,
+                                  (uintptr_t)e.gather_block, (uintptr_t)(overlapped ? e.overlap : 0), (uintptr_t)whole,
                                   (uintptr_t)light_blocks(e), (uintptr_t)e.light_refill, (uintptr_t)e.light_order,
                                   (uintptr_t)e.light_emit, (uintptr_t)e.emitted.p, (uintptr_t)e.film_prev.p,
                                   (uintptr_t)e.lk_a.p};
@@ -1528,17 +1528,22 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                [&]() {
                                    launch_copy_u32(reinterpret_cast<uint32_t*>(e.film_prev.p),
                                                    reinterpret_cast<const uint32_t*>(e.image.p), film, st);
-                                   if (overlapped) {
-                                       // the camera trace and the query sort on stream2, beside the light pass
-                                       // and the photon map (a fork and a join inside the one graph)
-                                       forked = hipEventRecord(e.ev_fork, st) == hipSuccess &&
-                                                hipStreamWaitEvent(e.stream2, e.ev_fork, 0) == hipSuccess;
+                                   // the camera trace and the query sort on stream2, beside the light pass
+                                   // and the photon map (a fork and a join inside the one graph); with
+                                   // overlap 2 the branch is captured after the light pass, so the graph
+                                   // holds its nodes after the light pass's
+                                   auto branch = [&]() {
+                                       forked = hipStreamWaitEvent(e.stream2, e.ev_fork, 0) == hipSuccess;
                                        if (forked) {
                                            launch_fill_u32(e.cam_counters.p, 0u, kPathCounters, e.stream2);
                                            launch_camera_trace(a, camera_blocks(e), e.stream2);
                                            a.q_order = query_order(e, a, e.q_capacity, e.stream2);
                                            forked = hipEventRecord(e.ev_gjoin, e.stream2) == hipSuccess;
                                        }
+                                   };
+                                   if (overlapped) {
+                                       forked = hipEventRecord(e.ev_fork, st) == hipSuccess;
+                                       if (forked && e.overlap != 2) branch();
                                    }
                                    launch_fill_u32(e.counters.p, 0u, 4, st);
                                    clear_map_sort(e.hist.p, e.capacity, st);
@@ -1551,6 +1556,7 @@ bool graph_iteration(Engine& e, int iteration, uint32_t& count, MapInfo& mi) {
                                    if (e.light_emit) la.emitted = e.emitted.p;
                                    la.path_order = light_path_order(e, la, st);
                                    launch_light_pass(la, light_blocks(e), st);
+                                   if (overlapped && forked && e.overlap == 2) branch();
                                    launch_copy_u32(e.graph_count.p, e.counters.p + 1, 1, st);
                                    build_photon_map(m, st);
                                    if (overlapped) {

@@ -93,7 +93,7 @@ def test_rolled_query_loop_keeps_the_film(exact):
                 assert np.array_equal(a, b), (group, block)
 
 
-@pytest.mark.parametrize("order,graph,overlap", [(0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 0, 2), (1, 0, 2)])
+@pytest.mark.parametrize("order,graph,overlap", [(0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 0, 2), (1, 0, 2), (1, 1, 2)])
 def test_overlapped_camera_trace_keeps_the_film(order, graph, overlap):
     """The camera trace on a second stream, beside the light pass and the
     photon map (option overlap = 1, the default; 2: queued after the light
