@@ -357,16 +357,17 @@ def guide_steps(a) -> int:
 def graph_leg(a):
     """Iteration time of a small scene (64 x 64 pixels, 65,536 light paths,
     5,120-triangle glass sphere) with direct launches and with the captured
-    iteration graph, without and with the overlapped camera trace.  Without
-    overlap the whole iteration is one graph; with it the light pass and the
-    trace stay direct launches beside each other.  At this size the iteration
+    iteration graph, without and with the overlapped camera trace (overlap 1:
+    trace queued first; 2: after the light pass).  The whole iteration is one
+    graph either way; with overlap the trace and query sort are a second-stream
+    branch of it, joined before the merge.  At this size the iteration
     is bounded by the latency of the longest light and camera paths (paths
     caught in the glass), not by launches."""
     from realcausticsengine_amd import scenes
     from realcausticsengine_amd.engine import Engine
 
     out = {"workload": "glass icosphere (5,120 tris) over a plane, 64x64, 65,536 light paths/iteration"}
-    for overlap in (0, 1):
+    for overlap in (0, 1, 2):
         for graph in (0, 1):
             eng = Engine(scenes.c0_scene(64, 64, subdiv=4, samples=1), photons=65536, seed=0, write_exr=0,
                          graph=graph, overlap=overlap)
